@@ -1,0 +1,113 @@
+"""ctypes declarations of libglia_rd.so (the C ABI in include/glia_rd.h).
+
+The library is built in-tree (glia_amd/libglia_rd.so) by glia_amd/build.py.  There is no fallback:
+importing this module without the library raises ImportError.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libglia_rd.so")
+
+HOST, DEVICE = 0, 1
+OK, ERR_ARG, ERR_STATE, ERR_UNSUPPORTED, ERR_POOL, ERR_NOHISTORY = 0, -2, -3, -4, -5, -6
+FIELD = dict(k=0, rho=1, wm=2, gm=3, csf=4, vt=5, filter=6, obs=7, d1=8, c0=9, ct=10, alpha0=11, work=12)
+MODE_P_KAPPA, MODE_KAPPA_RHO, MODE_ALL = 0, 1, 2
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int * 3),
+        ("nmembers", ctypes.c_int),
+        ("dt", ctypes.c_double),
+        ("nt", ctypes.c_int),
+        ("order", ctypes.c_int),
+        ("pcg_rtol", ctypes.c_double),
+        ("pcg_maxit", ctypes.c_int),
+        ("checkpoint_slots", ctypes.c_int),
+        ("adjoint_store", ctypes.c_int),
+        ("phi_store", ctypes.c_int),
+        ("k_gm_wm", ctypes.c_double),
+        ("k_glm_wm", ctypes.c_double),
+        ("r_gm_wm", ctypes.c_double),
+        ("r_glm_wm", ctypes.c_double),
+        ("beta", ctypes.c_double),
+        ("k_scale", ctypes.c_double),
+        ("rho_scale", ctypes.c_double),
+        ("smoothing_factor", ctypes.c_double),
+        ("stream", ctypes.c_void_p),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("pcg_iterations", ctypes.c_int),
+        ("cn_solves", ctypes.c_int),
+        ("worst_reason", ctypes.c_int),
+        ("last_iterations", ctypes.c_int),
+        ("last_reason", ctypes.c_int),
+        ("kbar", ctypes.c_double),
+        ("prec_coef", ctypes.c_double),
+        ("filter_sum", ctypes.c_double),
+        ("misfit", ctypes.c_double),
+        ("c0_norm2", ctypes.c_double),
+    ]
+
+
+# every entry point declared in include/glia_rd.h, with its ctypes signature
+_vp, _i, _d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+_pd = ctypes.POINTER(ctypes.c_double)
+SIGNATURES = {
+    "glia_rd_config_init": (None, [ctypes.POINTER(Config)]),
+    "glia_rd_create": (_i, [ctypes.POINTER(Config), ctypes.POINTER(_vp)]),
+    "glia_rd_destroy": (_i, [_vp]),
+    "glia_rd_last_error": (ctypes.c_char_p, [_vp]),
+    "glia_rd_sync": (_i, [_vp]),
+    "glia_rd_num_members": (_i, [_vp]),
+    "glia_rd_set_tissue": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i]),
+    "glia_rd_set_tissue_sinusoidal": (_i, [_vp, _i]),
+    "glia_rd_set_coefficients": (_i, [_vp, _i, _d, _d]),
+    "glia_rd_set_k_sinusoidal": (_i, [_vp, _i, _d]),
+    "glia_rd_prec_factor": (_i, [_vp]),
+    "glia_rd_set_ctx_dt": (_i, [_vp, _d]),
+    "glia_rd_get_ctx_dt": (_i, [_vp, _pd]),
+    "glia_rd_set_basis": (_i, [_vp, _i, _i, _vp, _d]),
+    "glia_rd_basis_size": (_i, [_vp, _i]),
+    "glia_rd_set_data": (_i, [_vp, _i, _vp, _vp, _d, _i]),
+    "glia_rd_set_field": (_i, [_vp, _i, _i, _vp, _i]),
+    "glia_rd_get_field": (_i, [_vp, _i, _i, _vp, _i]),
+    "glia_rd_phi_apply": (_i, [_vp, _vp]),
+    "glia_rd_phi_apply_transpose": (_i, [_vp, _i, _vp]),
+    "glia_rd_diffusion_solve": (_i, [_vp, _i, _d]),
+    "glia_rd_apply_d": (_i, [_vp, _i, _i]),
+    "glia_rd_fft_r2c": (_i, [_vp, _i, _vp, _vp]),
+    "glia_rd_fft_c2r": (_i, [_vp, _i, _vp, _vp]),
+    "glia_rd_solve_state": (_i, [_vp, _i]),
+    "glia_rd_solve_adjoint": (_i, [_vp, _i]),
+    "glia_rd_objective": (_i, [_vp, _i, _vp, _vp]),
+    "glia_rd_gradient": (_i, [_vp, _i, _vp, _vp]),
+    "glia_rd_objective_and_gradient": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "glia_rd_stats": (_i, [_vp, _i, ctypes.POINTER(Stats)]),
+    "glia_rd_reset_stats": (_i, [_vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libglia_rd.so (raises ImportError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "libglia_rd.so not found at %s: build it with `python -m glia_amd.build` "
+            "(hipcc --offload-arch=gfx950); there is no CPU fallback" % LIB_PATH
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib

@@ -1,0 +1,380 @@
+// rd_capi.cpp -- extern "C" boundary (include/glia_rd.h) over glia::RDEnsemble.
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/glia_rd.h"
+#include "rd_solver.h"
+
+struct glia_rd_s {
+  std::unique_ptr<glia::RDEnsemble> e;
+  std::string err;
+};
+
+using glia::RDEnsemble;
+
+static int field_slot(int which) {
+  switch (which) {
+    case GLIA_RD_FIELD_K: return glia::RS_K;
+    case GLIA_RD_FIELD_RHO: return glia::RS_RHO;
+    case GLIA_RD_FIELD_WM: return glia::RS_WM;
+    case GLIA_RD_FIELD_GM: return glia::RS_GM;
+    case GLIA_RD_FIELD_CSF: return glia::RS_CSF;
+    case GLIA_RD_FIELD_VT: return glia::RS_VT;
+    case GLIA_RD_FIELD_FILTER: return glia::RS_FILTER;
+    case GLIA_RD_FIELD_OBS: return glia::RS_OBS;
+    case GLIA_RD_FIELD_D1: return glia::RS_D1;
+    case GLIA_RD_FIELD_C0: return glia::RS_C0;
+    case GLIA_RD_FIELD_CT: return glia::RS_CT;
+    case GLIA_RD_FIELD_ALPHA0: return glia::RS_A0;
+    case GLIA_RD_FIELD_WORK: return glia::RS_CH;
+    default: return -1;
+  }
+}
+
+static int ret(glia_rd_t h, int rc) {
+  if (rc != 0 && h && h->e) h->err = h->e->error();
+  return rc;
+}
+
+#define GLIA_H(h)                                      \
+  do {                                                 \
+    if (!(h) || !(h)->e) return GLIA_RD_ERR_ARG;       \
+  } while (0)
+
+extern "C" {
+
+void glia_rd_config_init(glia_rd_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->n[0] = c->n[1] = c->n[2] = 64;
+  c->nmembers = 1;
+  c->dt = 0.5;
+  c->nt = 1;
+  c->order = 2;
+  c->pcg_rtol = 1e-6;
+  c->pcg_maxit = 5000;
+  c->checkpoint_slots = 0;
+  c->adjoint_store = 0;
+  c->phi_store = 0;
+  c->k_gm_wm = 0.0;
+  c->k_glm_wm = 0.0;
+  c->r_gm_wm = 0.0;
+  c->r_glm_wm = 0.0;
+  c->beta = 0.0;
+  c->k_scale = 1.0;
+  c->rho_scale = 1.0;
+  c->smoothing_factor = 1.0;
+  c->stream = nullptr;
+}
+
+int glia_rd_create(const glia_rd_config* cfg, glia_rd_t* out) {
+  if (!cfg || !out) return GLIA_RD_ERR_ARG;
+  *out = nullptr;
+  if (cfg->order != 2) return GLIA_RD_ERR_UNSUPPORTED;
+  glia::SolverConfig sc;
+  for (int i = 0; i < 3; i++) sc.n[i] = cfg->n[i];
+  sc.nmembers = cfg->nmembers;
+  sc.dt = cfg->dt;
+  sc.nt = cfg->nt;
+  sc.rtol = cfg->pcg_rtol;
+  sc.maxit = cfg->pcg_maxit;
+  sc.checkpoint_slots = cfg->checkpoint_slots;
+  sc.adjoint_store = cfg->adjoint_store;
+  sc.phi_store = cfg->phi_store;
+  sc.k_gm_wm = cfg->k_gm_wm;
+  sc.k_glm_wm = cfg->k_glm_wm;
+  sc.r_gm_wm = cfg->r_gm_wm;
+  sc.r_glm_wm = cfg->r_glm_wm;
+  sc.beta = cfg->beta;
+  sc.k_scale = cfg->k_scale;
+  sc.rho_scale = cfg->rho_scale;
+  sc.smoothing_factor = cfg->smoothing_factor;
+  sc.stream = (hipStream_t)cfg->stream;
+  glia_rd_s* h = new (std::nothrow) glia_rd_s();
+  if (!h) return GLIA_RD_ERR_ARG;
+  h->e.reset(new (std::nothrow) RDEnsemble(sc));
+  if (!h->e) {
+    delete h;
+    return GLIA_RD_ERR_ARG;
+  }
+  const int rc = h->e->init();
+  if (rc != 0) {
+    // keep the handle so the caller can read the message, then destroy it
+    h->err = h->e->error();
+    *out = h;
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+int glia_rd_destroy(glia_rd_t h) {
+  if (!h) return GLIA_RD_ERR_ARG;
+  delete h;
+  return 0;
+}
+
+const char* glia_rd_last_error(glia_rd_t h) {
+  if (!h) return "null handle";
+  return h->err.c_str();
+}
+
+int glia_rd_sync(glia_rd_t h) {
+  GLIA_H(h);
+  return ret(h, h->e->sync());
+}
+
+int glia_rd_num_members(glia_rd_t h) {
+  GLIA_H(h);
+  return h->e->nmembers();
+}
+
+int glia_rd_set_tissue(glia_rd_t h, int member, const float* wm, const float* gm, const float* csf, const float* vt,
+                       int mem_kind) {
+  GLIA_H(h);
+  return ret(h, h->e->set_tissue(member, wm, gm, csf, vt, mem_kind == GLIA_RD_DEVICE));
+}
+
+int glia_rd_set_tissue_sinusoidal(glia_rd_t h, int member) {
+  GLIA_H(h);
+  return ret(h, h->e->set_tissue_sinusoidal(member));
+}
+
+int glia_rd_set_coefficients(glia_rd_t h, int member, double kappa, double rho) {
+  GLIA_H(h);
+  if (member < 0 || member >= h->e->nmembers()) return GLIA_RD_ERR_ARG;
+  return ret(h, h->e->set_coefficients(member, kappa, rho));
+}
+
+int glia_rd_set_k_sinusoidal(glia_rd_t h, int member, double scale) {
+  GLIA_H(h);
+  if (member < 0 || member >= h->e->nmembers()) return GLIA_RD_ERR_ARG;
+  return ret(h, h->e->set_k_sinusoidal(member, scale));
+}
+
+int glia_rd_prec_factor(glia_rd_t h) {
+  GLIA_H(h);
+  return ret(h, h->e->prec_factor());
+}
+
+int glia_rd_set_ctx_dt(glia_rd_t h, double dt) {
+  GLIA_H(h);
+  h->e->set_ctx_dt(dt);
+  return 0;
+}
+
+int glia_rd_get_ctx_dt(glia_rd_t h, double* dt) {
+  GLIA_H(h);
+  if (!dt) return GLIA_RD_ERR_ARG;
+  *dt = h->e->ctx_dt();
+  return 0;
+}
+
+int glia_rd_set_basis(glia_rd_t h, int member, int np, const double* centers, double sigma) {
+  GLIA_H(h);
+  return ret(h, h->e->set_basis(member, np, centers, sigma));
+}
+
+int glia_rd_basis_size(glia_rd_t h, int member) {
+  GLIA_H(h);
+  return h->e->basis_size(member);
+}
+
+int glia_rd_set_data(glia_rd_t h, int member, const float* d1, const float* obs_mask, double obs_thr, int mem_kind) {
+  GLIA_H(h);
+  if (!d1) return GLIA_RD_ERR_ARG;
+  return ret(h, h->e->set_data(member, d1, obs_mask, obs_thr, mem_kind == GLIA_RD_DEVICE));
+}
+
+int glia_rd_set_field(glia_rd_t h, int member, int which, const float* src, int mem_kind) {
+  GLIA_H(h);
+  const int s = field_slot(which);
+  if (s < 0) return GLIA_RD_ERR_ARG;
+  return ret(h, h->e->set_field(member, s, src, mem_kind == GLIA_RD_DEVICE));
+}
+
+int glia_rd_get_field(glia_rd_t h, int member, int which, float* dst, int mem_kind) {
+  GLIA_H(h);
+  const int s = field_slot(which);
+  if (s < 0 || !dst) return GLIA_RD_ERR_ARG;
+  return ret(h, h->e->get_field(member, s, dst, mem_kind == GLIA_RD_DEVICE));
+}
+
+static bool split_x(RDEnsemble* e, const double* x, int extra, std::vector<std::vector<double>>& out) {
+  if (!x) return false;
+  const int nm = e->nmembers();
+  out.assign(nm, {});
+  int64_t off = 0;
+  for (int m = 0; m < nm; m++) {
+    const int n = e->basis_size(m) + extra;
+    out[m].assign(x + off, x + off + n);
+    off += n;
+  }
+  return true;
+}
+
+int glia_rd_phi_apply(glia_rd_t h, const double* p) {
+  GLIA_H(h);
+  std::vector<std::vector<double>> pv;
+  if (!split_x(h->e.get(), p, 0, pv)) return GLIA_RD_ERR_ARG;
+  int rc = h->e->phi_apply(pv);
+  if (rc == 0) rc = h->e->sync();
+  return ret(h, rc);
+}
+
+int glia_rd_phi_apply_transpose(glia_rd_t h, int which, double* out) {
+  GLIA_H(h);
+  const int s = field_slot(which);
+  if (s < 0 || !out) return GLIA_RD_ERR_ARG;
+  std::vector<std::vector<double>> o;
+  const int rc = h->e->phi_apply_transpose(s, o);
+  if (rc) return ret(h, rc);
+  int64_t off = 0;
+  for (auto& v : o) {
+    std::memcpy(out + off, v.data(), sizeof(double) * v.size());
+    off += (int64_t)v.size();
+  }
+  return 0;
+}
+
+int glia_rd_diffusion_solve(glia_rd_t h, int which, double tau) {
+  GLIA_H(h);
+  const int s = field_slot(which);
+  if (s < 0) return GLIA_RD_ERR_ARG;
+  return ret(h, h->e->diffusion_solve_field(s, tau));
+}
+
+int glia_rd_apply_d(glia_rd_t h, int which_in, int which_out) {
+  GLIA_H(h);
+  const int a = field_slot(which_in), b = field_slot(which_out);
+  if (a < 0 || b < 0) return GLIA_RD_ERR_ARG;
+  return ret(h, h->e->apply_d_field(a, b));
+}
+
+// packed half spectrum <-> reference layout (kz = 0 and kz = n2/2 planes unpacked by symmetry)
+int glia_rd_fft_r2c(glia_rd_t h, int member, const float* f, float* fhat) {
+  GLIA_H(h);
+  if (!f || !fhat || member < 0 || member >= h->e->nmembers()) return GLIA_RD_ERR_ARG;
+  const glia::Dims& d = h->e->dims();
+  int rc = h->e->set_field(member, glia::RS_CH, f, false);
+  if (!rc) rc = h->e->fft_r2c(glia::RS_CH, glia::CS_T7);
+  std::vector<float2> P(d.nspec);
+  if (!rc) rc = h->e->get_spectrum(member, glia::CS_T7, P.data());
+  if (rc) return ret(h, rc);
+  const int n0 = d.n0, n1 = d.n1, M = d.m2, n2h = M + 1;
+  for (int kx = 0; kx < n0; kx++)
+    for (int ky = 0; ky < n1; ky++) {
+      const int64_t b = ((int64_t)kx * n1 + ky);
+      float* o = fhat + 2 * b * n2h;
+      const float2* p = P.data() + b * M;
+      for (int kz = 1; kz < M; kz++) {
+        o[2 * kz] = p[kz].x;
+        o[2 * kz + 1] = p[kz].y;
+      }
+      const int64_t bm = ((int64_t)((n0 - kx) % n0) * n1 + (n1 - ky) % n1);
+      const float2 a = p[0], c = P[bm * M];  // P[k], P[-k]
+      // A = (P[k] + conj P[-k]) / 2, B = (P[k] - conj P[-k]) / (2i)
+      o[0] = 0.5f * (a.x + c.x);
+      o[1] = 0.5f * (a.y - c.y);
+      const float dx = a.x - c.x, dy = a.y + c.y;
+      o[2 * M] = 0.5f * dy;
+      o[2 * M + 1] = -0.5f * dx;
+    }
+  return 0;
+}
+
+int glia_rd_fft_c2r(glia_rd_t h, int member, const float* fhat, float* f) {
+  GLIA_H(h);
+  if (!f || !fhat || member < 0 || member >= h->e->nmembers()) return GLIA_RD_ERR_ARG;
+  const glia::Dims& d = h->e->dims();
+  const int n0 = d.n0, n1 = d.n1, M = d.m2, n2h = M + 1;
+  std::vector<float2> P(d.nspec);
+  for (int64_t b = 0; b < (int64_t)n0 * n1; b++) {
+    const float* in = fhat + 2 * b * n2h;
+    float2* p = P.data() + b * M;
+    p[0] = make_float2(in[0] - in[2 * M + 1], in[1] + in[2 * M]);  // X0 + i X_{n2/2}
+    for (int kz = 1; kz < M; kz++) p[kz] = make_float2(in[2 * kz], in[2 * kz + 1]);
+  }
+  int rc = h->e->set_spectrum(member, glia::CS_T6, P.data());
+  if (!rc) rc = h->e->fft_c2r(glia::CS_T6, glia::RS_CH);
+  if (!rc) rc = h->e->get_field(member, glia::RS_CH, f, false);
+  if (rc) return ret(h, rc);
+  // the solver's inverse includes 1/N^3; the reference's C2R is unnormalized
+  const double s = (double)d.ng;
+  for (int64_t i = 0; i < d.ng; i++) f[i] = (float)(f[i] * s);
+  return 0;
+}
+
+int glia_rd_solve_state(glia_rd_t h, int linearized) {
+  GLIA_H(h);
+  int rc = h->e->solve_state(linearized, true);
+  if (!rc) rc = h->e->sync();
+  return ret(h, rc);
+}
+
+int glia_rd_solve_adjoint(glia_rd_t h, int linearized) {
+  GLIA_H(h);
+  int rc = h->e->solve_adjoint(linearized);
+  if (!rc) rc = h->e->sync();
+  return ret(h, rc);
+}
+
+static int eval(glia_rd_t h, int mode, const double* x, double* J, double* dJ, bool grad) {
+  GLIA_H(h);
+  std::vector<std::vector<double>> xv, g;
+  std::vector<double> Jv;
+  if (!split_x(h->e.get(), x, 2, xv)) return GLIA_RD_ERR_ARG;
+  const int rc = h->e->objective_and_gradient(mode, xv, grad, Jv, g);
+  if (rc) return ret(h, rc);
+  int64_t off = 0;
+  for (int m = 0; m < h->e->nmembers(); m++) {
+    if (J) J[m] = Jv[m];
+    if (dJ) std::memcpy(dJ + off, g[m].data(), sizeof(double) * g[m].size());
+    off += (int64_t)g[m].size();
+  }
+  return 0;
+}
+
+int glia_rd_objective(glia_rd_t h, int mode, const double* x, double* J) {
+  if (!J) return GLIA_RD_ERR_ARG;
+  return eval(h, mode, x, J, nullptr, false);
+}
+
+int glia_rd_gradient(glia_rd_t h, int mode, const double* x, double* dJ) {
+  if (!dJ) return GLIA_RD_ERR_ARG;
+  return eval(h, mode, x, nullptr, dJ, true);
+}
+
+int glia_rd_objective_and_gradient(glia_rd_t h, int mode, const double* x, double* J, double* dJ) {
+  if (!J || !dJ) return GLIA_RD_ERR_ARG;
+  return eval(h, mode, x, J, dJ, true);
+}
+
+int glia_rd_stats(glia_rd_t h, int member, glia_rd_stats_t* out) {
+  GLIA_H(h);
+  if (!out) return GLIA_RD_ERR_ARG;
+  glia::MemberStats s;
+  const int rc = h->e->stats(member, &s);
+  if (rc) return ret(h, rc);
+  out->pcg_iterations = s.its_total;
+  out->cn_solves = s.solves;
+  out->worst_reason = s.worst_reason;
+  out->last_iterations = s.last_its;
+  out->last_reason = s.last_reason;
+  out->kbar = s.kavg;
+  out->prec_coef = s.pc;
+  out->filter_sum = s.filter_sum;
+  out->misfit = s.m1;
+  out->c0_norm2 = s.c0sq;
+  return 0;
+}
+
+int glia_rd_reset_stats(glia_rd_t h) {
+  GLIA_H(h);
+  return ret(h, h->e->reset_counters());
+}
+
+}  // extern "C"

@@ -1,0 +1,1448 @@
+// rd_kernels.hip -- CDNA4 kernels of the spectral reaction-diffusion solver.
+//
+// Spectral layout ("packed half spectrum"): per member, n0*n1*(n2/2) complex float2, index
+// (kx*n1 + ky)*m2 + kz' with m2 = n2/2.  For kz' >= 1 the entry is the R2C coefficient
+// X[kx,ky,kz']; the kz' = 0 entry packs the two real-along-z planes kz = 0 and kz = n2/2 as
+// X0 + i*X_{n2/2} (both planes have Hermitian symmetry in (kx, ky)).  Every multiplier used here
+// (i*w_d, the preconditioner) has w_z = 0 on both planes (the Nyquist mode is zeroed), so it
+// acts identically on both halves of the packed entry, and Parseval sums over the packed column
+// give the sum over both planes.  The array has exactly n0*n1*n2 floats, no padding.
+//
+// Stages of a 3D transform (inverse order shown; forward is the reverse):
+//   X pass  (column kernel, lines along axis 0, stride n1*m2)   k_x... kernels
+//   Y pass  (column kernel, lines along axis 1, stride m2)      k_y... kernels
+//   Z pass  (line kernel, contiguous n2 reals <-> m2 packed complex) k_z... kernels
+// Column kernels stage a tile of 16 adjacent columns (128-byte rows) in LDS; line kernels stage
+// NL whole lines.  Pointwise work (wave-number multipliers, k(x) products, reaction, PCG vector
+// updates, FP64 reductions) is fused into these passes so no kernel exists only for it.
+#include "rd_launch.h"
+
+#include <hip/hip_runtime.h>
+
+#include "fft_engine.h"
+
+namespace glia {
+using namespace dev;
+
+// ---------------------------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------------------------
+// complex slot s of a member: work slots below CS_NUM, state-pool entries (history/checkpoints) above
+__device__ __forceinline__ float2* cptr(const MemberDev& M, int s) { return (s < CS_NUM) ? M.c[s] : M.hist[s - CS_NUM]; }
+
+// wave number with the Nyquist mode zeroed (reference src/cuda/SpectralOperators.cu:63-72)
+__device__ __forceinline__ float wnum(int k, int n) {
+  const int h = n >> 1;
+  return (k < h) ? (float)k : ((k == h) ? 0.f : (float)(k - n));
+}
+
+// spectral preconditioner factor times N^3 (reference src/cuda/DiffCoef.cu:11-63):
+// FFT(B r) = r_hat / (1 + 0.25 dt kbar |w|^2), or r_hat when the denominator is 0
+__device__ __forceinline__ float precf(float pc, float wx, float wy, float wz) {
+  const float v = 1.0f + pc * (wx * wx + wy * wy + wz * wz);
+  return (v == 0.0f) ? 1.0f : 1.0f / v;
+}
+
+// block-wide FP64 reduction of NQ values; thread 0 writes out[0..NQ)
+template <int NQ>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NQ], double* out) {
+  __shared__ double red[NQ][16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    double x = v[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) red[q][wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      double s = 0;
+      for (int w = 0; w < nw; w++) s += red[q][w];
+      out[q] = s;
+    }
+  }
+}
+
+// block-wide sum of one value, result broadcast to every thread
+__device__ __forceinline__ double block_sum_all(double x) {
+  __shared__ double red[16];
+  __shared__ double res;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+  if (lane == 0) red[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int w = 0; w < nw; w++) s += red[w];
+    res = s;
+  }
+  __syncthreads();
+  const double r = res;
+  __syncthreads();
+  return r;
+}
+
+// sum of column q of part[nb][nq], identical on every thread (fixed order: deterministic)
+__device__ __forceinline__ double sum_partials(const double* part, int nb, int nq, int q) {
+  double s = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(int64_t)b * nq + q];
+  return block_sum_all(s);
+}
+
+// ---------------------------------------------------------------------------------------------
+// column kernels (X and Y passes)
+// ---------------------------------------------------------------------------------------------
+template <int L>
+struct ColK {
+  static constexpr int NT = LineCfg<L>::NT;
+  static constexpr int TPB = 16 * NT;
+  static constexpr int LP = L + 1;
+};
+
+struct Tile {
+  int64_t base, rs;
+  int o, c0;
+};
+
+__device__ __forceinline__ Tile col_tile(int dir, const Dims& d) {
+  const int ntc = d.m2 >> 4;
+  Tile t;
+  t.o = blockIdx.x / ntc;
+  t.c0 = (blockIdx.x % ntc) << 4;
+  if (dir == 0) {
+    t.base = (int64_t)t.o * d.m2 + t.c0;
+    t.rs = (int64_t)d.n1 * d.m2;
+  } else {
+    t.base = (int64_t)t.o * d.n1 * d.m2 + t.c0;
+    t.rs = d.m2;
+  }
+  return t;
+}
+
+// X pass, inverse: U = IFFT_x(p), V = IFFT_x(i w_x p).  With PUPD the search direction is
+// first updated in place: p = B r (i == 0) or p = B r + (beta/betaold) p.
+template <int L, bool PUPD>
+__global__ void __launch_bounds__(ColK<L>::TPB)
+k_xi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_in, int s_r, int s_u, int s_v) {
+  const MemberDev& M = mem[blockIdx.y];
+  const PcgState* st = M.st;
+  if (st->done) return;
+  constexpr int LP = ColK<L>::LP, TPB = ColK<L>::TPB, NT = ColK<L>::NT;
+  __shared__ float2 A[16 * LP];
+  __shared__ float2 B[16 * LP];
+  const Tile tl = col_tile(0, d);
+  const float wy = wnum(tl.o, d.n1);
+  float2* pin = cptr(M, s_in);
+  const float2* rr = cptr(M, s_r);
+  float2* U = cptr(M, s_u);
+  float2* V = cptr(M, s_v);
+  bool first = false;
+  float b = 0.f, pc = 0.f;
+  if (PUPD) {
+    first = (st->i == 0);
+    b = (float)st->bcoef;
+    pc = (float)st->pc;
+  }
+  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
+    const int row = e >> 4, col = e & 15;
+    const int64_t g = tl.base + (int64_t)row * tl.rs + col;
+    const float wx = wnum(row, d.n0);
+    float2 p;
+    if (PUPD) {
+      const float f = precf(pc, wx, wy, (float)(tl.c0 + col));
+      const float2 z = cscale(rr[g], f);
+      if (first) {
+        p = z;
+      } else {
+        const float2 po = pin[g];
+        p = make_float2(z.x + b * po.x, z.y + b * po.y);
+      }
+      pin[g] = p;
+    } else {
+      p = pin[g];
+    }
+    A[col * LP + row] = p;
+    B[col * LP + row] = mul_i(cscale(p, wx));
+  }
+  __syncthreads();
+  const int col = threadIdx.x / NT, t = threadIdx.x % NT;
+  fft_line<L, +1>(A + col * LP, 1, t, tw, true);
+  fft_line<L, +1>(B + col * LP, 1, t, tw, true);
+  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
+    const int row = e >> 4, cc = e & 15;
+    const int64_t g = tl.base + (int64_t)row * tl.rs + cc;
+    U[g] = A[cc * LP + row];
+    V[g] = B[cc * LP + row];
+  }
+}
+
+// Y pass, inverse, gradient: Gx = IFFT_y(V), Gy = IFFT_y(i w_y U), Gz = i w_z IFFT_y(U),
+// and optionally G0 = IFFT_y(U).  Gx must not alias U.
+template <int L, bool WITH0>
+__global__ void __launch_bounds__(ColK<L>::TPB)
+k_yi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_u, int s_v, int s_gx, int s_gy,
+          int s_gz, int s_g0) {
+  const MemberDev& M = mem[blockIdx.y];
+  if (M.st->done) return;
+  constexpr int LP = ColK<L>::LP, TPB = ColK<L>::TPB, NT = ColK<L>::NT;
+  __shared__ float2 A[16 * LP];
+  __shared__ float2 B[16 * LP];
+  const Tile tl = col_tile(1, d);
+  const float2* U = cptr(M, s_u);
+  const float2* V = cptr(M, s_v);
+  float2* Gx = cptr(M, s_gx);
+  float2* Gy = cptr(M, s_gy);
+  float2* Gz = cptr(M, s_gz);
+  float2* G0 = WITH0 ? cptr(M, s_g0) : nullptr;
+  const int col = threadIdx.x / NT, t = threadIdx.x % NT;
+  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
+    const int row = e >> 4, cc = e & 15;
+    A[cc * LP + row] = V[tl.base + (int64_t)row * tl.rs + cc];
+  }
+  __syncthreads();
+  fft_line<L, +1>(A + col * LP, 1, t, tw, true);
+  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
+    const int row = e >> 4, cc = e & 15;
+    Gx[tl.base + (int64_t)row * tl.rs + cc] = A[cc * LP + row];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
+    const int row = e >> 4, cc = e & 15;
+    const float2 u = U[tl.base + (int64_t)row * tl.rs + cc];
+    A[cc * LP + row] = u;
+    B[cc * LP + row] = mul_i(cscale(u, wnum(row, d.n1)));
+  }
+  __syncthreads();
+  fft_line<L, +1>(A + col * LP, 1, t, tw, true);
+  fft_line<L, +1>(B + col * LP, 1, t, tw, true);
+  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
+    const int row = e >> 4, cc = e & 15;
+    const int64_t g = tl.base + (int64_t)row * tl.rs + cc;
+    const float2 a = A[cc * LP + row];
+    const float wz = (float)(tl.c0 + cc);
+    Gz[g] = mul_i(cscale(a, wz));
+    if (WITH0) G0[g] = a;
+    Gy[g] = B[cc * LP + row];
+  }
+}
+
+// Y pass, forward, divergence: Yx = FFT_y(Hx), Rr = FFT_y(Q) + i w_y FFT_y(Hy).
+// Yx must not alias Q or Hy.
+template <int L>
+__global__ void __launch_bounds__(ColK<L>::TPB)
+k_yf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_hx, int s_hy, int s_q, int s_yx,
+         int s_rr) {
+  const MemberDev& M = mem[blockIdx.y];
+  if (M.st->done) return;
+  constexpr int LP = ColK<L>::LP, TPB = ColK<L>::TPB, NT = ColK<L>::NT;
+  __shared__ float2 A[16 * LP];
+  __shared__ float2 B[16 * LP];
+  const Tile tl = col_tile(1, d);
+  const float2* Hx = cptr(M, s_hx);
+  const float2* Hy = cptr(M, s_hy);
+  const float2* Q = cptr(M, s_q);
+  float2* Yx = cptr(M, s_yx);
+  float2* Rr = cptr(M, s_rr);
+  const int col = threadIdx.x / NT, t = threadIdx.x % NT;
+  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
+    const int row = e >> 4, cc = e & 15;
+    A[cc * LP + row] = Hx[tl.base + (int64_t)row * tl.rs + cc];
+  }
+  __syncthreads();
+  fft_line<L, -1>(A + col * LP, 1, t, tw, true);
+  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
+    const int row = e >> 4, cc = e & 15;
+    Yx[tl.base + (int64_t)row * tl.rs + cc] = A[cc * LP + row];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
+    const int row = e >> 4, cc = e & 15;
+    const int64_t g = tl.base + (int64_t)row * tl.rs + cc;
+    A[cc * LP + row] = Q[g];
+    B[cc * LP + row] = Hy[g];
+  }
+  __syncthreads();
+  fft_line<L, -1>(A + col * LP, 1, t, tw, true);
+  fft_line<L, -1>(B + col * LP, 1, t, tw, true);
+  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
+    const int row = e >> 4, cc = e & 15;
+    const int64_t g = tl.base + (int64_t)row * tl.rs + cc;
+    const float wy = wnum(row, d.n1);
+    const float2 a = A[cc * LP + row], bq = B[cc * LP + row];
+    Rr[g] = cadd(a, mul_i(cscale(bq, wy)));
+  }
+}
+
+// X pass, forward, closing the divergence: D = i w_x FFT_x(Yx) + FFT_x(Rr) = FFT(div(k grad p)).
+//   XF_MATVEC: w = p - (tau/2) D; partial <p, w>
+//   XF_SETUP : r = tau D (= b - A x for x0 = c); partials ||B b||^2, ||B r||^2, <B r, r>
+//              with b = x + (tau/2) D
+//   XF_D     : out = D
+enum { XF_MATVEC = 0, XF_SETUP = 1, XF_D = 2 };
+
+template <int L, int MODE>
+__global__ void __launch_bounds__(ColK<L>::TPB)
+k_xf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_yx, int s_rr, int s_p, int s_out) {
+  const MemberDev& M = mem[blockIdx.y];
+  const PcgState* st = M.st;
+  if (st->done) return;
+  constexpr int LP = ColK<L>::LP, TPB = ColK<L>::TPB, NT = ColK<L>::NT;
+  constexpr int NQ = (MODE == XF_SETUP) ? 3 : 1;
+  __shared__ float2 A[16 * LP];
+  __shared__ float2 B[16 * LP];
+  const Tile tl = col_tile(0, d);
+  const float2* Yx = cptr(M, s_yx);
+  const float2* Rr = cptr(M, s_rr);
+  const float2* P = cptr(M, s_p);
+  float2* O = cptr(M, s_out);
+  const float tau = (float)st->tau;
+  const float pc = (float)st->pc;
+  const float wy = wnum(tl.o, d.n1);
+  const int col = threadIdx.x / NT, t = threadIdx.x % NT;
+  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
+    const int row = e >> 4, cc = e & 15;
+    const int64_t g = tl.base + (int64_t)row * tl.rs + cc;
+    A[cc * LP + row] = Yx[g];
+    B[cc * LP + row] = Rr[g];
+  }
+  __syncthreads();
+  fft_line<L, -1>(A + col * LP, 1, t, tw, true);
+  fft_line<L, -1>(B + col * LP, 1, t, tw, true);
+  double acc[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) acc[q] = 0.0;
+  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
+    const int row = e >> 4, cc = e & 15;
+    const int64_t g = tl.base + (int64_t)row * tl.rs + cc;
+    const float wx = wnum(row, d.n0);
+    const float2 D = cadd(mul_i(cscale(A[cc * LP + row], wx)), B[cc * LP + row]);
+    const int kz = tl.c0 + cc;
+    const double wgt = (kz == 0) ? 1.0 : 2.0;
+    if (MODE == XF_MATVEC) {
+      const float2 p = P[g];
+      const float2 w = make_float2(p.x - 0.5f * tau * D.x, p.y - 0.5f * tau * D.y);
+      O[g] = w;
+      acc[0] += wgt * ((double)p.x * w.x + (double)p.y * w.y);
+    } else if (MODE == XF_SETUP) {
+      const float2 x = P[g];
+      const float2 bb = make_float2(x.x + 0.5f * tau * D.x, x.y + 0.5f * tau * D.y);
+      const float2 r = cscale(D, tau);
+      O[g] = r;
+      const float f = precf(pc, wx, wy, (float)kz);
+      const double pb2 = (double)f * f * ((double)bb.x * bb.x + (double)bb.y * bb.y);
+      const double r2 = (double)r.x * r.x + (double)r.y * r.y;
+      acc[0] += wgt * pb2;
+      acc[1] += wgt * (double)f * f * r2;
+      acc[2] += wgt * (double)f * r2;
+    } else {
+      O[g] = D;
+    }
+  }
+  if (MODE != XF_D) block_reduce_store<NQ>(acc, M.part + (int64_t)blockIdx.x * NQ);
+}
+
+// plain column transform: out = FFT_dir(in) along axis DIR (S = -1 forward, +1 inverse)
+template <int L, int DIR, int S>
+__global__ void __launch_bounds__(ColK<L>::TPB)
+k_col_plain(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_in, int s_out, int honor_done) {
+  const MemberDev& M = mem[blockIdx.y];
+  if (honor_done && M.st->done) return;
+  constexpr int LP = ColK<L>::LP, TPB = ColK<L>::TPB, NT = ColK<L>::NT;
+  __shared__ float2 A[16 * LP];
+  const Tile tl = col_tile(DIR, d);
+  const float2* I = cptr(M, s_in);
+  float2* O = cptr(M, s_out);
+  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
+    const int row = e >> 4, cc = e & 15;
+    A[cc * LP + row] = I[tl.base + (int64_t)row * tl.rs + cc];
+  }
+  __syncthreads();
+  const int col = threadIdx.x / NT, t = threadIdx.x % NT;
+  fft_line<L, S>(A + col * LP, 1, t, tw, true);
+  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
+    const int row = e >> 4, cc = e & 15;
+    O[tl.base + (int64_t)row * tl.rs + cc] = A[cc * LP + row];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// line kernels (Z pass): real n2 <-> packed m2 = n2/2 complex via one m2-point complex FFT
+// ---------------------------------------------------------------------------------------------
+template <int M>
+struct ZK {
+  static constexpr int NT = LineCfg<M>::NT;
+  static constexpr int NL = (M == 128) ? 32 : (M == 96) ? 32 : (M == 64) ? 64 : (M == 48) ? 64 : (M == 32) ? 128 : 256;
+  static constexpr int TPB = NL * NT;
+  static constexpr int MP = M + 1;
+  static constexpr int KPT = (M + NT - 1) / NT;  // entries per thread
+};
+
+// coalesced load of the block's NL packed lines into LDS
+template <int M>
+__device__ __forceinline__ void zload(float2* buf, const float2* __restrict__ src, int64_t base) {
+  for (int e = threadIdx.x; e < ZK<M>::NL * M; e += ZK<M>::TPB) {
+    const int li = e / M, k = e - li * M;
+    buf[li * ZK<M>::MP + k] = src[base + e];
+  }
+}
+
+// packed spectrum X -> Z'[k] = (X[k] + conj X[M-k]) + i e^{+2 pi i k/n2} (X[k] - conj X[M-k]),
+// with X[0] = Re P[0], X[M] = Im P[0]; afterwards an inverse M-point FFT gives
+// z[n] = x[2n] + i x[2n+1] (unnormalized C2R of length n2).
+template <int M>
+__device__ __forceinline__ void zprep_c2r(float2* line, int t, const float2* __restrict__ tw2) {
+  constexpr int NT = ZK<M>::NT, KPT = ZK<M>::KPT;
+  float2 v[KPT];
+#pragma unroll
+  for (int j = 0; j < KPT; j++) {
+    const int k = t + j * NT;
+    if (k < M) {
+      if (k == 0) {
+        const float2 p0 = line[0];
+        v[j] = make_float2(p0.x + p0.y, p0.x - p0.y);
+      } else {
+        const float2 a = line[k];
+        const float2 bc = cconj(line[M - k]);
+        const float2 s = cadd(a, bc), df = csub(a, bc);
+        v[j] = cadd(s, mul_i(cmulc(df, tw2[k])));
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < KPT; j++) {
+    const int k = t + j * NT;
+    if (k < M) line[k] = v[j];
+  }
+  __syncthreads();
+}
+
+// after a forward M-point FFT of z[n] = x[2n] + i x[2n+1]: packed spectrum entry k
+template <int M>
+__device__ __forceinline__ float2 zpost_r2c(const float2* line, int k, const float2* __restrict__ tw2) {
+  if (k == 0) {
+    const float2 z0 = line[0];
+    return make_float2(z0.x + z0.y, z0.x - z0.y);
+  }
+  const float2 a = line[k];
+  const float2 bc = cconj(line[M - k]);
+  const float2 e = cscale(cadd(a, bc), 0.5f);
+  const float2 o = cscale(mul_mi(csub(a, bc)), 0.5f);
+  return cadd(e, cmul(o, tw2[k]));
+}
+
+// Z pass of the matvec: for each component d, h_d = k * C2R(G_d)/N^3, out_d = R2C(h_d),
+// the z component multiplied by i w_z.  In place on the three slots.
+template <int M>
+__global__ void __launch_bounds__(ZK<M>::TPB)
+k_z_diff(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ twm, const float2* __restrict__ tw2,
+         int s_gx, int s_gy, int s_gz) {
+  const MemberDev& Mb = mem[blockIdx.y];
+  if (Mb.st->done) return;
+  constexpr int NT = ZK<M>::NT, MP = ZK<M>::MP, KPT = ZK<M>::KPT, NL = ZK<M>::NL;
+  __shared__ float2 buf[NL * MP];
+  const int li = threadIdx.x / NT, t = threadIdx.x % NT;
+  const int64_t base = (int64_t)blockIdx.x * NL * M;
+  float2* line = buf + li * MP;
+  const float2* kf = reinterpret_cast<const float2*>(Mb.r[RS_K]);
+  float2 kv[KPT];
+#pragma unroll
+  for (int j = 0; j < KPT; j++) {
+    const int n = t + j * NT;
+    kv[j] = (n < M) ? kf[base + li * M + n] : make_float2(0.f, 0.f);
+  }
+  const float s = (float)d.inv_ng;
+  const int slots[3] = {s_gx, s_gy, s_gz};
+  for (int comp = 0; comp < 3; comp++) {
+    float2* G = cptr(Mb, slots[comp]);
+    zload<M>(buf, G, base);
+    __syncthreads();
+    zprep_c2r<M>(line, t, tw2);
+    fft_line<M, +1>(line, 1, t, twm, true);
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+      const int n = t + j * NT;
+      if (n < M) {
+        const float2 z = line[n];
+        line[n] = make_float2(kv[j].x * z.x * s, kv[j].y * z.y * s);
+      }
+    }
+    __syncthreads();
+    fft_line<M, -1>(line, 1, t, twm, true);
+    float2 outv[KPT];
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+      const int k = t + j * NT;
+      if (k < M) {
+        float2 x = zpost_r2c<M>(line, k, tw2);
+        if (comp == 2) x = mul_i(cscale(x, (float)k));  // w_z = k' (0 at the packed column)
+        outv[j] = x;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+      const int k = t + j * NT;
+      if (k < M) G[base + li * M + k] = outv[j];
+    }
+    __syncthreads();
+  }
+}
+
+// Z pass for a real-space pointwise stage between an inverse and a forward transform.
+//   ZR_REACT    : c = C2R/N^3; [store c -> chalf]; c = c e^{rho dt} / (c e^{rho dt} + 1 - c)
+//                 (the exact logistic step of src/pde/PdeOperators.cpp:140-190)
+//   ZR_REACT_ADJ: a = C2R/N^3; a = a e^{rho dt} / (ch e^{rho dt} + 1 - ch)^2 with ch from chalf
+//                 (logisticReactionLinearized, src/cuda/PdeOperators.cu:4-13)
+//   ZR_C2R      : real out = C2R/N^3 (no forward transform)
+//   ZR_R2C      : packed out = R2C(real in)
+//   ZR_MISFIT   : c = C2R/N^3 -> CT; t = O c - d1; partial sum t^2; a = -O t; out = R2C(a)
+enum { ZR_REACT = 0, ZR_REACT_ADJ = 1, ZR_C2R = 2, ZR_R2C = 3, ZR_MISFIT = 4 };
+
+template <int M, int MODE>
+__global__ void __launch_bounds__(ZK<M>::TPB)
+k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ twm, const float2* __restrict__ tw2,
+          int s_in, int s_out, int r_slot, int ch_idx, float dt, int honor_done) {
+  const MemberDev& Mb = mem[blockIdx.y];
+  if (honor_done && Mb.st->done) return;
+  constexpr int NT = ZK<M>::NT, MP = ZK<M>::MP, KPT = ZK<M>::KPT, NL = ZK<M>::NL;
+  __shared__ float2 buf[NL * MP];
+  const int li = threadIdx.x / NT, t = threadIdx.x % NT;
+  const int64_t base = (int64_t)blockIdx.x * NL * M;
+  float2* line = buf + li * MP;
+  // real field: a half-step history entry, a real slot, or none
+  float2* rf = reinterpret_cast<float2*>(ch_idx >= 0 ? Mb.chalf[ch_idx] : (r_slot >= 0 ? Mb.r[r_slot] : nullptr));
+  const float s = (float)d.inv_ng;
+  double acc[1] = {0.0};
+  if (MODE == ZR_R2C) {
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+      const int n = t + j * NT;
+      if (n < M) line[n] = rf[base + li * M + n];
+    }
+    __syncthreads();
+  } else {
+    const float2* I = cptr(Mb, s_in);
+    zload<M>(buf, I, base);
+    __syncthreads();
+    zprep_c2r<M>(line, t, tw2);
+    fft_line<M, +1>(line, 1, t, twm, true);
+    const float2* rho = reinterpret_cast<const float2*>(Mb.r[RS_RHO]);
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+      const int n = t + j * NT;
+      if (n < M) {
+        const int64_t gi = base + li * M + n;
+        float2 c = cscale(line[n], s);
+        if (MODE == ZR_C2R) {
+          rf[gi] = c;
+        } else if (MODE == ZR_REACT) {
+          if (rf) rf[gi] = c;
+          const float2 r = rho[gi];
+          const float fx = expf(r.x * dt), fy = expf(r.y * dt);
+          c.x = c.x * fx / (c.x * fx + 1.0f - c.x);
+          c.y = c.y * fy / (c.y * fy + 1.0f - c.y);
+          line[n] = c;
+        } else if (MODE == ZR_REACT_ADJ) {
+          const float2 r = rho[gi];
+          const float2 ch = rf[gi];
+          const float fx = expf(r.x * dt), fy = expf(r.y * dt);
+          const float ax = ch.x * fx + 1.0f - ch.x, ay = ch.y * fy + 1.0f - ch.y;
+          c.x = c.x * fx / (ax * ax);
+          c.y = c.y * fy / (ay * ay);
+          line[n] = c;
+        } else if (MODE == ZR_MISFIT) {
+          reinterpret_cast<float2*>(Mb.r[RS_CT])[gi] = c;
+          const float2 ob = reinterpret_cast<const float2*>(Mb.r[RS_OBS])[gi];
+          const float2 dd = reinterpret_cast<const float2*>(Mb.r[RS_D1])[gi];
+          const float tx = ob.x * c.x - dd.x, ty = ob.y * c.y - dd.y;
+          acc[0] += (double)tx * tx + (double)ty * ty;
+          line[n] = make_float2(-(ob.x * tx), -(ob.y * ty));
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (MODE != ZR_C2R) {
+    fft_line<M, -1>(line, 1, t, twm, true);
+    float2* O = cptr(Mb, s_out);
+    float2 outv[KPT];
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+      const int k = t + j * NT;
+      if (k < M) outv[j] = zpost_r2c<M>(line, k, tw2);
+    }
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+      const int k = t + j * NT;
+      if (k < M) O[base + li * M + k] = outv[j];
+    }
+  }
+  if (MODE == ZR_MISFIT) block_reduce_store<1>(acc, Mb.part + blockIdx.x);
+}
+
+// Z pass of the gradient quadrature at one time point: from the y-transformed fields of c
+// (G0, Gx, Gy, Gz in slots s_c[0..4)) and of alpha (s_a[0..4)):
+//   q_k   = sum (wm + r_k gm) (grad c . grad alpha)
+//   q_rho = sum (wm + r_rho gm) (c^2 - c) alpha
+// (the integrands of gradDiffusion / gradReaction, src/grad/DerivativeOperators.cpp:189-321)
+template <int M>
+__global__ void __launch_bounds__(ZK<M>::TPB)
+k_z_gradq(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ twm, const float2* __restrict__ tw2,
+          int4 s_c, int4 s_a) {
+  const MemberDev& Mb = mem[blockIdx.y];
+  constexpr int NT = ZK<M>::NT, MP = ZK<M>::MP, KPT = ZK<M>::KPT, NL = ZK<M>::NL;
+  __shared__ float2 buf[NL * MP];
+  const int li = threadIdx.x / NT, t = threadIdx.x % NT;
+  const int64_t base = (int64_t)blockIdx.x * NL * M;
+  float2* line = buf + li * MP;
+  const float s = (float)d.inv_ng;
+  float2 dot[KPT], cv[KPT];
+#pragma unroll
+  for (int j = 0; j < KPT; j++) dot[j] = make_float2(0.f, 0.f);
+  const int sc[4] = {s_c.x, s_c.y, s_c.z, s_c.w};
+  const int sa[4] = {s_a.x, s_a.y, s_a.z, s_a.w};
+  double acc[2] = {0.0, 0.0};
+  for (int comp = 0; comp < 4; comp++) {
+    for (int which = 0; which < 2; which++) {
+      const float2* I = cptr(Mb, which == 0 ? sc[comp] : sa[comp]);
+      zload<M>(buf, I, base);
+      __syncthreads();
+      zprep_c2r<M>(line, t, tw2);
+      fft_line<M, +1>(line, 1, t, twm, true);
+#pragma unroll
+      for (int j = 0; j < KPT; j++) {
+        const int n = t + j * NT;
+        if (n < M) {
+          const float2 v = cscale(line[n], s);
+          if (which == 0) {
+            cv[j] = v;
+          } else if (comp > 0) {
+            dot[j].x += cv[j].x * v.x;
+            dot[j].y += cv[j].y * v.y;
+          } else {
+            // comp 0: c and alpha themselves
+            const int64_t gi = base + li * M + n;
+            const float2 wm = reinterpret_cast<const float2*>(Mb.r[RS_WM])[gi];
+            const float2 gm = reinterpret_cast<const float2*>(Mb.r[RS_GM])[gi];
+            const float rx = cv[j].x * cv[j].x - cv[j].x, ry = cv[j].y * cv[j].y - cv[j].y;
+            acc[1] += (double)(wm.x + Mb.rr_gm * gm.x) * (double)(rx * v.x) +
+                      (double)(wm.y + Mb.rr_gm * gm.y) * (double)(ry * v.y);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < KPT; j++) {
+    const int n = t + j * NT;
+    if (n < M) {
+      const int64_t gi = base + li * M + n;
+      const float2 wm = reinterpret_cast<const float2*>(Mb.r[RS_WM])[gi];
+      const float2 gm = reinterpret_cast<const float2*>(Mb.r[RS_GM])[gi];
+      acc[0] += (double)(wm.x + Mb.kr_gm * gm.x) * (double)dot[j].x + (double)(wm.y + Mb.kr_gm * gm.y) * (double)dot[j].y;
+    }
+  }
+  block_reduce_store<2>(acc, Mb.part + (int64_t)blockIdx.x * 2);
+}
+
+// ---------------------------------------------------------------------------------------------
+// PCG vector update and scalar state machine (PETSc 3.11 KSPSolve_CG + KSPConvergedDefault)
+// ---------------------------------------------------------------------------------------------
+constexpr int UPD_BLOCKS = 1024;
+
+// x += alpha p; r -= alpha w; partials ||B r||^2 and <B r, r> (Parseval weights)
+__global__ void __launch_bounds__(256)
+k_pcg_update(const MemberDev* __restrict__ mem, Dims d, int s_x) {
+  const MemberDev& M = mem[blockIdx.y];
+  const PcgState* st = M.st;
+  if (st->done) return;
+  float2* X = cptr(M, s_x);
+  float2* R = M.c[CS_R];
+  const float2* P = M.c[CS_P];
+  const float2* W = M.c[CS_W];
+  const float a = (float)st->alpha;
+  const float pc = (float)st->pc;
+  double acc[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.nspec; i += stride) {
+    const int kz = (int)(i % d.m2);
+    const int64_t q = i / d.m2;
+    const int ky = (int)(q % d.n1);
+    const int kx = (int)(q / d.n1);
+    const float2 p = P[i], w = W[i];
+    float2 x = X[i], r = R[i];
+    x.x += a * p.x;
+    x.y += a * p.y;
+    r.x -= a * w.x;
+    r.y -= a * w.y;
+    X[i] = x;
+    R[i] = r;
+    const float f = precf(pc, wnum(kx, d.n0), wnum(ky, d.n1), (float)kz);
+    const double wgt = (kz == 0) ? 1.0 : 2.0;
+    const double r2 = (double)r.x * r.x + (double)r.y * r.y;
+    acc[0] += wgt * (double)f * f * r2;
+    acc[1] += wgt * (double)f * r2;
+  }
+  block_reduce_store<2>(acc, M.part + (int64_t)blockIdx.x * 2);
+}
+
+__device__ __forceinline__ int ksp_conv_test(const PcgState* s, double rnorm) {
+  if (isnan(rnorm) || isinf(rnorm)) return KSPR_DIVERGED_NANORINF;
+  if (rnorm <= s->ttol) return (rnorm < s->atol) ? KSPR_CONVERGED_ATOL : KSPR_CONVERGED_RTOL;
+  if (rnorm >= s->dtol * s->rnorm0) return KSPR_DIVERGED_DTOL;
+  return 0;
+}
+
+__device__ __forceinline__ void ksp_finish(PcgState* s, int reason, int its) {
+  s->reason = reason;
+  s->its = its;
+  s->done = 1;
+  s->its_total += its;
+  s->solves += 1;
+  if (reason < s->worst_reason) s->worst_reason = reason;
+}
+
+// start of every CN solve: parameters and a clean iteration state
+__global__ void k_pcg_begin(const MemberDev* __restrict__ mem, double tau) {
+  if (threadIdx.x != 0) return;
+  PcgState* s = mem[blockIdx.x].st;
+  s->tau = tau;
+  s->beta = s->betaold = s->dpi = s->dpiold = s->alpha = s->bcoef = 0;
+  s->dp = s->rnorm0 = s->ttol = 0;
+  s->i = 0;
+  s->its = 0;
+  s->reason = 0;
+  s->done = 0;
+  if (!s->active) {  // DiffusionSolver::solve returns at once when k_scale == 0
+    s->done = 1;
+    s->solves += 1;
+  }
+}
+
+// after the setup matvec: the n = 0 convergence test and the first beta
+__global__ void k_pcg_fin_setup(const MemberDev* __restrict__ mem, Dims d, int nb, int* hflag, int hslot, int nm) {
+  const MemberDev& M = mem[blockIdx.x];
+  PcgState* s = M.st;
+  const bool was_done = s->done;
+  const double bb = sum_partials(M.part, nb, 3, 0) * d.inv_ng;
+  const double rr = sum_partials(M.part, nb, 3, 1) * d.inv_ng;
+  const double zr = sum_partials(M.part, nb, 3, 2) * d.inv_ng;
+  if (threadIdx.x == 0) {
+    if (!was_done) {
+      s->dp = sqrt(rr);
+      double snorm = sqrt(bb);
+      if (snorm == 0) snorm = s->dp;
+      s->rnorm0 = snorm;
+      s->ttol = fmax(s->rtol * snorm, s->atol);
+      const int reason = ksp_conv_test(s, s->dp);
+      if (reason) {
+        ksp_finish(s, reason, 0);
+      } else {
+        s->beta = zr;
+        s->i = 0;
+        if (isnan(zr) || isinf(zr)) {
+          ksp_finish(s, KSPR_DIVERGED_NANORINF, 1);
+        } else if (zr == 0.0) {
+          ksp_finish(s, KSPR_CONVERGED_ATOL, 1);
+        }
+      }
+    }
+    if (hflag) {
+      hflag[hslot * nm + blockIdx.x] = s->done;
+      __threadfence_system();
+    }
+  }
+}
+
+// after w = A p: dpi = <p, w>, alpha = beta / dpi
+__global__ void k_pcg_fin_dpi(const MemberDev* __restrict__ mem, Dims d, int nb) {
+  const MemberDev& M = mem[blockIdx.x];
+  PcgState* s = M.st;
+  if (s->done) return;
+  const double dpi = sum_partials(M.part, nb, 1, 0) * d.inv_ng;
+  if (threadIdx.x == 0) {
+    s->dpiold = s->dpi;
+    s->dpi = dpi;
+    s->betaold = s->beta;
+    if (isnan(dpi) || isinf(dpi)) {
+      ksp_finish(s, KSPR_DIVERGED_NANORINF, s->i + 1);
+    } else if (dpi == 0.0 || (s->i > 0 && dpi * s->dpiold <= 0.0)) {
+      ksp_finish(s, KSPR_DIVERGED_INDEFINITE_MAT, s->i + 1);
+    } else {
+      s->alpha = s->beta / dpi;
+    }
+  }
+}
+
+// after the x/r update: convergence test, next beta and the loop-head checks of KSPSolve_CG
+__global__ void k_pcg_fin_update(const MemberDev* __restrict__ mem, Dims d, int nb, int* hflag, int hslot, int nm) {
+  const MemberDev& M = mem[blockIdx.x];
+  PcgState* s = M.st;
+  const bool was_done = s->done;
+  double rr = 0, zr = 0;
+  if (!was_done) {
+    rr = sum_partials(M.part, nb, 2, 0) * d.inv_ng;
+    zr = sum_partials(M.part, nb, 2, 1) * d.inv_ng;
+  }
+  if (threadIdx.x == 0) {
+    if (!was_done) {
+      s->dp = sqrt(rr);
+      const int reason = ksp_conv_test(s, s->dp);
+      if (reason) {
+        ksp_finish(s, reason, s->i + 1);
+      } else if (isnan(zr) || isinf(zr)) {
+        ksp_finish(s, KSPR_DIVERGED_NANORINF, s->i + 1);
+      } else {
+        s->beta = zr;
+        s->i += 1;
+        if (s->i >= s->maxit) {
+          ksp_finish(s, KSPR_DIVERGED_ITS, s->i);
+        } else if (s->beta == 0.0) {
+          ksp_finish(s, KSPR_CONVERGED_ATOL, s->i + 1);
+        } else if (s->beta * s->betaold < 0.0) {
+          ksp_finish(s, KSPR_DIVERGED_INDEFINITE_PC, s->i + 1);
+        } else {
+          s->bcoef = s->beta / s->betaold;
+        }
+      }
+    }
+    if (hflag) {
+      hflag[hslot * nm + blockIdx.x] = s->done;
+      __threadfence_system();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// elementwise / setup kernels
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_copy(const MemberDev* __restrict__ mem, Dims d, int s_in, int s_out) {
+  const MemberDev& M = mem[blockIdx.y];
+  const float4* I = reinterpret_cast<const float4*>(cptr(M, s_in));
+  float4* O = reinterpret_cast<float4*>(cptr(M, s_out));
+  const int64_t n4 = d.nspec / 2;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) O[i] = I[i];
+}
+
+__global__ void __launch_bounds__(256) k_copy_real(const MemberDev* __restrict__ mem, Dims d, int r_in, int r_out, int ch_in,
+                                                    int ch_out) {
+  const MemberDev& M = mem[blockIdx.y];
+  const float4* I = reinterpret_cast<const float4*>(ch_in >= 0 ? M.chalf[ch_in] : M.r[r_in]);
+  float4* O = reinterpret_cast<float4*>(ch_out >= 0 ? M.chalf[ch_out] : M.r[r_out]);
+  const int64_t n4 = d.ng / 4;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) O[i] = I[i];
+}
+
+// MatProp::setValuesCustom (src/mat/MatProp.cpp:134-203): clip, mask rule; partial sum(filter)
+__global__ void __launch_bounds__(256) k_tissue(const MemberDev* __restrict__ mem, Dims d) {
+  const MemberDev& M = mem[blockIdx.y];
+  float* wm = M.r[RS_WM];
+  float* gm = M.r[RS_GM];
+  float* csf = M.r[RS_CSF];
+  float* vt = M.r[RS_VT];
+  float* fl = M.r[RS_FILTER];
+  double acc[1] = {0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.ng; i += stride) {
+    float w = wm[i], g = gm[i], c = csf[i], v = vt[i];
+    w = (w <= 0.f) ? 0.f : w;
+    g = (g <= 0.f) ? 0.f : g;
+    c = (c <= 0.f) ? 0.f : c;
+    v = (v <= 0.f) ? 0.f : v;
+    wm[i] = w;
+    gm[i] = g;
+    csf[i] = c;
+    vt[i] = v;
+    const float f = ((w > 0.1f || g > 0.1f) && v < 0.8f) ? 1.f : 0.f;
+    fl[i] = f;
+    acc[0] += f;
+  }
+  block_reduce_store<1>(acc, M.part + blockIdx.x);
+}
+
+// DiffCoef::setValues / ReacCoef::setValues: field = s_gm gm + s_wm wm + s_csf csf;
+// partial sum of the field (for the mask average kbar)
+__global__ void __launch_bounds__(256) k_coef(const MemberDev* __restrict__ mem, Dims d, int r_out, const double* coefs) {
+  const MemberDev& M = mem[blockIdx.y];
+  const float* wm = M.r[RS_WM];
+  const float* gm = M.r[RS_GM];
+  const float* csf = M.r[RS_CSF];
+  float* o = M.r[r_out];
+  const float a = (float)coefs[3 * blockIdx.y + 0], b = (float)coefs[3 * blockIdx.y + 1], c = (float)coefs[3 * blockIdx.y + 2];
+  double acc[1] = {0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.ng; i += stride) {
+    float v = 0.f;
+    v += a * gm[i];
+    v += b * wm[i];
+    v += c * csf[i];
+    o[i] = v;
+    acc[0] += v;
+  }
+  block_reduce_store<1>(acc, M.part + blockIdx.x);
+}
+
+// Obs::setDefaultFilter (src/mat/Obs.cpp:26-55): O = 1[d1 > thr]
+__global__ void __launch_bounds__(256) k_obs_default(const MemberDev* __restrict__ mem, Dims d, float thr) {
+  const MemberDev& M = mem[blockIdx.y];
+  const float* d1 = M.r[RS_D1];
+  float* ob = M.r[RS_OBS];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.ng; i += stride) ob[i] = (d1[i] > thr) ? 1.f : 0.f;
+}
+
+// generic finalize: red[q] = sum of partial column q (one block per member)
+__global__ void k_fin_sum(const MemberDev* __restrict__ mem, int nb, int nq) {
+  const MemberDev& M = mem[blockIdx.x];
+  for (int q = 0; q < nq; q++) {
+    const double v = sum_partials(M.part, nb, nq, q);
+    if (threadIdx.x == 0) M.st->red[q] = v;
+  }
+}
+
+// finalize kinds that fold a reduction into the state
+enum { FIN_FILTER = 0, FIN_KSUM = 1, FIN_MISFIT = 2, FIN_C0SQ = 3, FIN_QUAD = 4 };
+__global__ void k_fin_state(const MemberDev* __restrict__ mem, int nb, int kind, double w) {
+  const MemberDev& M = mem[blockIdx.x];
+  PcgState* s = M.st;
+  const int nq = (kind == FIN_QUAD) ? 2 : 1;
+  const double v0 = sum_partials(M.part, nb, nq, 0);
+  const double v1 = (nq > 1) ? sum_partials(M.part, nb, nq, 1) : 0.0;
+  if (threadIdx.x == 0) {
+    if (kind == FIN_FILTER) s->filter_sum = v0;
+    else if (kind == FIN_KSUM) { s->ksum = v0; s->kavg = v0 * (1.0 / s->filter_sum); }
+    else if (kind == FIN_MISFIT) s->m1 = v0;
+    else if (kind == FIN_C0SQ) s->c0sq = v0;
+    else { s->gk += w * v0; s->gr += w * v1; }
+  }
+}
+
+// DiffusionSolver::precFactor: the preconditioner is fixed by (prec_dt, kbar) at call time
+__global__ void k_prec_factor(const MemberDev* __restrict__ mem, double prec_dt) {
+  if (threadIdx.x != 0) return;
+  PcgState* s = mem[blockIdx.x].st;
+  s->pc = 0.25 * prec_dt * s->kavg;
+}
+
+__global__ void k_state_reset(const MemberDev* __restrict__ mem, int what) {
+  if (threadIdx.x != 0) return;
+  PcgState* s = mem[blockIdx.x].st;
+  if (what & 1) { s->its_total = 0; s->solves = 0; s->worst_reason = 0; }
+  if (what & 2) { s->gk = 0; s->gr = 0; s->m1 = 0; }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Gaussian basis Phi as compact bricks
+// ---------------------------------------------------------------------------------------------
+// source of the smoothing convolution on the extended brick: mask * exp(-r^2 / (2 sigma^2)) with
+// the non-periodic distance of Phi::initialize (src/mat/Phi.cpp:264-311); indices wrap (mod n)
+// for the periodic convolution.
+
+
+__global__ void k_brick_src(const BrickJob* __restrict__ jobs, Dims d, const MemberDev* __restrict__ mem, double h0,
+                            double h1, double h2, double sigma, double* __restrict__ ext) {
+  // no FMA contraction: the Gaussian argument is computed with the same IEEE operations as the
+  // FP64 restatement of Phi::initialize
+#pragma clang fp contract(off)
+  const BrickJob J = jobs[blockIdx.y];
+  const int e0 = J.b[0] + 2 * J.kr, e1 = J.b[1] + 2 * J.kr, e2 = J.b[2] + 2 * J.kr;
+  const int64_t ne = (int64_t)e0 * e1 * e2;
+  const float* fl = mem[J.member].r[RS_FILTER];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += (int64_t)gridDim.x * blockDim.x) {
+    const int z = (int)(i % e2);
+    const int y = (int)((i / e2) % e1);
+    const int x = (int)(i / ((int64_t)e1 * e2));
+    int X = J.o[0] - J.kr + x, Y = J.o[1] - J.kr + y, Z = J.o[2] - J.kr + z;
+    X = ((X % d.n0) + d.n0) % d.n0;
+    Y = ((Y % d.n1) + d.n1) % d.n1;
+    Z = ((Z % d.n2) + d.n2) % d.n2;
+    const double dx = h0 * X - J.ctr[0], dy = h1 * Y - J.ctr[1], dz = h2 * Z - J.ctr[2];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double R = sqrt(2.0) * sigma;
+    const double m = fl[((int64_t)X * d.n1 + Y) * d.n2 + Z];
+    ext[J.ext_off + i] = exp(-r2 / (R * R)) * m;
+  }
+}
+
+// one axis of the separable smoothing: the kept extent shrinks by 2*kr along 'axis'
+__global__ void k_brick_conv(const BrickJob* __restrict__ jobs, int axis, const double* __restrict__ taps, int tap_stride,
+                             const double* __restrict__ src, double* __restrict__ dst) {
+  const BrickJob J = jobs[blockIdx.y];
+  const int kr = J.kr;
+  int ein[3], eout[3];
+  for (int a = 0; a < 3; a++) {
+    // axes < axis already reduced to b, axes >= axis still extended
+    ein[a] = (a < axis) ? J.b[a] : J.b[a] + 2 * kr;
+    eout[a] = (a <= axis) ? J.b[a] : J.b[a] + 2 * kr;
+  }
+  const int64_t no = (int64_t)eout[0] * eout[1] * eout[2];
+  const double* tp = taps + (int64_t)blockIdx.y * tap_stride * 3 + axis * tap_stride;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < no; i += (int64_t)gridDim.x * blockDim.x) {
+    int c[3];
+    c[2] = (int)(i % eout[2]);
+    c[1] = (int)((i / eout[2]) % eout[1]);
+    c[0] = (int)(i / ((int64_t)eout[1] * eout[2]));
+    double acc = 0;
+    for (int s = -kr; s <= kr; s++) {
+      int ci[3] = {c[0], c[1], c[2]};
+      ci[axis] = c[axis] + kr - s;  // input index of (out - s) in the extended frame
+      acc += tp[s + kr] * src[J.ext_off + ((int64_t)ci[0] * ein[1] + ci[1]) * ein[2] + ci[2]];
+    }
+    dst[J.ext_off + i] = acc;
+  }
+}
+
+// truncation at 5 sigma (Phi::truncate, src/mat/Phi.cpp:238-260) and FP32 store
+__global__ void k_brick_store(const BrickJob* __restrict__ jobs, Dims d, double h0, double h1, double h2, double sigma,
+                              const double* __restrict__ src, float* __restrict__ store) {
+  // Centers on grid points put many voxels at exactly r = 5 sigma; computing r without FMA
+  // contraction makes the r/sigma <= 5 decision identical to the FP64 restatement.
+#pragma clang fp contract(off)
+  const BrickJob J = jobs[blockIdx.y];
+  const int64_t nb = (int64_t)J.b[0] * J.b[1] * J.b[2];
+  float* out = store + J.out_off;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += (int64_t)gridDim.x * blockDim.x) {
+    const int z = (int)(i % J.b[2]);
+    const int y = (int)((i / J.b[2]) % J.b[1]);
+    const int x = (int)(i / ((int64_t)J.b[1] * J.b[2]));
+    const int X = J.o[0] + x, Y = J.o[1] + y, Z = J.o[2] + z;
+    const double dx = h0 * X - J.ctr[0], dy = h1 * Y - J.ctr[1], dz = h2 * Z - J.ctr[2];
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    out[i] = (r / sigma <= 5) ? (float)src[J.ext_off + i] : 0.0f;
+  }
+}
+
+// per-brick maximum (one block per basis) into out[job]
+__global__ void k_brick_max(const BrickJob* __restrict__ jobs, const float* __restrict__ store, double* __restrict__ out) {
+  const BrickJob J = jobs[blockIdx.x];
+  const int64_t nb = (int64_t)J.b[0] * J.b[1] * J.b[2];
+  const float* v = store + J.out_off;
+  float m = -INFINITY;
+  for (int64_t i = threadIdx.x; i < nb; i += blockDim.x) m = fmaxf(m, v[i]);
+  __shared__ float red[256];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// c0 = sum_i scale_i * brick_i (gathered per voxel: deterministic, no atomics); partial ||c0||^2
+__global__ void __launch_bounds__(256) k_phi_apply(const MemberDev* __restrict__ mem, Dims d) {
+  const MemberDev& M = mem[blockIdx.y];
+  float* c0 = M.r[RS_C0];
+  double acc[1] = {0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.ng; i += stride) {
+    const int z = (int)(i % d.n2);
+    const int y = (int)((i / d.n2) % d.n1);
+    const int x = (int)(i / ((int64_t)d.n1 * d.n2));
+    double v = 0.0;
+    for (int b = 0; b < M.np; b++) {
+      const double sc = M.phi_scale[b];
+      if (sc == 0.0) continue;
+      const Brick bk = M.bricks[b];
+      const int lx = x - bk.o[0], ly = y - bk.o[1], lz = z - bk.o[2];
+      if (lx >= 0 && lx < bk.b[0] && ly >= 0 && ly < bk.b[1] && lz >= 0 && lz < bk.b[2])
+        v += sc * (double)M.brick_data[bk.off + ((int64_t)lx * bk.b[1] + ly) * bk.b[2] + lz];
+    }
+    const float vf = (float)v;
+    c0[i] = vf;
+    acc[0] += (double)vf * vf;
+  }
+  block_reduce_store<1>(acc, M.part + blockIdx.x);
+}
+
+// Phi^T v: out_i = scale * <brick_i, v> (one block per basis, FP64)
+__global__ void __launch_bounds__(256) k_phi_applyT(const MemberDev* __restrict__ mem, Dims d, int r_in, double scale) {
+  const MemberDev& M = mem[blockIdx.y];
+  const int b = blockIdx.x;
+  if (b >= M.np) return;
+  const Brick bk = M.bricks[b];
+  const float* v = M.r[r_in];
+  const int64_t nb = (int64_t)bk.b[0] * bk.b[1] * bk.b[2];
+  double acc = 0;
+  for (int64_t i = threadIdx.x; i < nb; i += blockDim.x) {
+    const int lz = (int)(i % bk.b[2]);
+    const int ly = (int)((i / bk.b[2]) % bk.b[1]);
+    const int lx = (int)(i / ((int64_t)bk.b[1] * bk.b[2]));
+    const int64_t g = ((int64_t)(bk.o[0] + lx) * d.n1 + (bk.o[1] + ly)) * d.n2 + (bk.o[2] + lz);
+    acc += (double)M.brick_data[bk.off + i] * (double)v[g];
+  }
+  const double s = block_sum_all(acc);
+  if (threadIdx.x == 0) M.phit_out[b] = s * scale;
+}
+
+// partial <x, y> of two real fields
+__global__ void __launch_bounds__(256) k_dot_real(const MemberDev* __restrict__ mem, Dims d, int r_x, int r_y) {
+  const MemberDev& M = mem[blockIdx.y];
+  const float* x = M.r[r_x];
+  const float* y = M.r[r_y];
+  double acc[1] = {0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.ng; i += stride) acc[0] += (double)x[i] * y[i];
+  block_reduce_store<1>(acc, M.part + blockIdx.x);
+}
+
+__global__ void k_set_done(const MemberDev* __restrict__ mem, int v) {
+  if (threadIdx.x == 0) mem[blockIdx.x].st->done = v;
+}
+
+// real-space helper: out = a*x + b*y (used for alpha0 - beta c0)
+__global__ void __launch_bounds__(256) k_axpby_real(const MemberDev* __restrict__ mem, Dims d, int r_x, double a, int r_y,
+                                                     double b, int r_out) {
+  const MemberDev& M = mem[blockIdx.y];
+  const float* x = M.r[r_x];
+  const float* y = M.r[r_y];
+  float* o = M.r[r_out];
+  const float fa = (float)a, fb = (float)b;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.ng; i += stride) o[i] = fa * x[i] + fb * y[i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// launchers (size dispatch)
+// ---------------------------------------------------------------------------------------------
+#define GLIA_CHECK_LAUNCH() \
+  do {                      \
+    hipError_t e_ = hipGetLastError(); \
+    if (e_ != hipSuccess) return (int)e_; \
+  } while (0)
+
+template <template <int> class F, typename... Args>
+static int dispatch_L(int L, Args&&... args) {
+  switch (L) {
+    case 32: return F<32>::run(args...);
+    case 48: return F<48>::run(args...);
+    case 64: return F<64>::run(args...);
+    case 96: return F<96>::run(args...);
+    case 128: return F<128>::run(args...);
+    case 192: return F<192>::run(args...);
+    case 256: return F<256>::run(args...);
+    default: return -1;
+  }
+}
+
+template <template <int> class F, typename... Args>
+static int dispatch_M(int M, Args&&... args) {
+  switch (M) {
+    case 16: return F<16>::run(args...);
+    case 24: return F<24>::run(args...);
+    case 32: return F<32>::run(args...);
+    case 48: return F<48>::run(args...);
+    case 64: return F<64>::run(args...);
+    case 96: return F<96>::run(args...);
+    case 128: return F<128>::run(args...);
+    default: return -1;
+  }
+}
+
+bool supported_line(int L) { return L == 32 || L == 48 || L == 64 || L == 96 || L == 128 || L == 192 || L == 256; }
+
+template <int L>
+struct RunXiGrad {
+  static int run(const LaunchCtx& c, int s_in, int s_r, int s_u, int s_v, bool pupd) {
+    dim3 g(c.d.n1 * (c.d.m2 / 16), c.nm);
+    if (pupd)
+      k_xi_grad<L, true><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_in, s_r, s_u, s_v);
+    else
+      k_xi_grad<L, false><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_in, s_r, s_u, s_v);
+    GLIA_CHECK_LAUNCH();
+    return 0;
+  }
+};
+int launch_xi_grad(const LaunchCtx& c, int s_in, int s_r, int s_u, int s_v, bool pupd) {
+  return dispatch_L<RunXiGrad>(c.d.n0, c, s_in, s_r, s_u, s_v, pupd);
+}
+
+template <int L>
+struct RunYiGrad {
+  static int run(const LaunchCtx& c, int s_u, int s_v, int s_gx, int s_gy, int s_gz, int s_g0) {
+    dim3 g(c.d.n0 * (c.d.m2 / 16), c.nm);
+    if (s_g0 >= 0)
+      k_yi_grad<L, true><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw1, s_u, s_v, s_gx, s_gy, s_gz, s_g0);
+    else
+      k_yi_grad<L, false><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw1, s_u, s_v, s_gx, s_gy, s_gz, 0);
+    GLIA_CHECK_LAUNCH();
+    return 0;
+  }
+};
+int launch_yi_grad(const LaunchCtx& c, int s_u, int s_v, int s_gx, int s_gy, int s_gz, int s_g0) {
+  return dispatch_L<RunYiGrad>(c.d.n1, c, s_u, s_v, s_gx, s_gy, s_gz, s_g0);
+}
+
+template <int L>
+struct RunYfDiv {
+  static int run(const LaunchCtx& c, int s_hx, int s_hy, int s_q, int s_yx, int s_rr) {
+    dim3 g(c.d.n0 * (c.d.m2 / 16), c.nm);
+    k_yf_div<L><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw1, s_hx, s_hy, s_q, s_yx, s_rr);
+    GLIA_CHECK_LAUNCH();
+    return 0;
+  }
+};
+int launch_yf_div(const LaunchCtx& c, int s_hx, int s_hy, int s_q, int s_yx, int s_rr) {
+  return dispatch_L<RunYfDiv>(c.d.n1, c, s_hx, s_hy, s_q, s_yx, s_rr);
+}
+
+template <int L>
+struct RunXfDiv {
+  static int run(const LaunchCtx& c, int mode, int s_yx, int s_rr, int s_p, int s_out) {
+    dim3 g(c.d.n1 * (c.d.m2 / 16), c.nm);
+    if (mode == XF_MATVEC)
+      k_xf_div<L, XF_MATVEC><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
+    else if (mode == XF_SETUP)
+      k_xf_div<L, XF_SETUP><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
+    else
+      k_xf_div<L, XF_D><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
+    GLIA_CHECK_LAUNCH();
+    return 0;
+  }
+};
+int launch_xf_div(const LaunchCtx& c, int mode, int s_yx, int s_rr, int s_p, int s_out) {
+  return dispatch_L<RunXfDiv>(c.d.n0, c, mode, s_yx, s_rr, s_p, s_out);
+}
+int xf_div_blocks(const Dims& d) { return d.n1 * (d.m2 / 16); }
+
+template <int L>
+struct RunColPlain {
+  static int run(const LaunchCtx& c, int dir, int sign, int s_in, int s_out, int honor) {
+    const int other = (dir == 0) ? c.d.n1 : c.d.n0;
+    const float2* tw = (dir == 0) ? c.tw0 : c.tw1;
+    dim3 g(other * (c.d.m2 / 16), c.nm);
+    if (dir == 0 && sign < 0) k_col_plain<L, 0, -1><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, tw, s_in, s_out, honor);
+    else if (dir == 0) k_col_plain<L, 0, 1><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, tw, s_in, s_out, honor);
+    else if (sign < 0) k_col_plain<L, 1, -1><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, tw, s_in, s_out, honor);
+    else k_col_plain<L, 1, 1><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, tw, s_in, s_out, honor);
+    GLIA_CHECK_LAUNCH();
+    return 0;
+  }
+};
+int launch_col_plain(const LaunchCtx& c, int dir, int sign, int s_in, int s_out, bool honor_done) {
+  return dispatch_L<RunColPlain>(dir == 0 ? c.d.n0 : c.d.n1, c, dir, sign, s_in, s_out, honor_done ? 1 : 0);
+}
+
+template <int M>
+struct RunZDiff {
+  static int run(const LaunchCtx& c, int s_gx, int s_gy, int s_gz) {
+    dim3 g((unsigned)((int64_t)c.d.n0 * c.d.n1 / ZK<M>::NL), c.nm);
+    k_z_diff<M><<<g, ZK<M>::TPB, 0, c.stream>>>(c.mem, c.d, c.twm, c.tw2, s_gx, s_gy, s_gz);
+    GLIA_CHECK_LAUNCH();
+    return 0;
+  }
+};
+int launch_z_diff(const LaunchCtx& c, int s_gx, int s_gy, int s_gz) {
+  return dispatch_M<RunZDiff>(c.d.m2, c, s_gx, s_gy, s_gz);
+}
+
+template <int M>
+struct RunZPoint {
+  static int run(const LaunchCtx& c, int mode, int s_in, int s_out, int r_slot, int ch_idx, float dt, int honor) {
+    dim3 g((unsigned)((int64_t)c.d.n0 * c.d.n1 / ZK<M>::NL), c.nm);
+#define GLIA_ZP(MD) k_z_point<M, MD><<<g, ZK<M>::TPB, 0, c.stream>>>(c.mem, c.d, c.twm, c.tw2, s_in, s_out, r_slot, ch_idx, dt, honor)
+    switch (mode) {
+      case ZR_REACT: GLIA_ZP(ZR_REACT); break;
+      case ZR_REACT_ADJ: GLIA_ZP(ZR_REACT_ADJ); break;
+      case ZR_C2R: GLIA_ZP(ZR_C2R); break;
+      case ZR_R2C: GLIA_ZP(ZR_R2C); break;
+      case ZR_MISFIT: GLIA_ZP(ZR_MISFIT); break;
+      default: return -1;
+    }
+#undef GLIA_ZP
+    GLIA_CHECK_LAUNCH();
+    return 0;
+  }
+};
+int launch_z_point(const LaunchCtx& c, int mode, int s_in, int s_out, int r_slot, int ch_idx, float dt, bool honor_done) {
+  return dispatch_M<RunZPoint>(c.d.m2, c, mode, s_in, s_out, r_slot, ch_idx, dt, honor_done ? 1 : 0);
+}
+int z_blocks(const Dims& d) {
+  switch (d.m2) {
+    case 16: return (int)((int64_t)d.n0 * d.n1 / ZK<16>::NL);
+    case 24: return (int)((int64_t)d.n0 * d.n1 / ZK<24>::NL);
+    case 32: return (int)((int64_t)d.n0 * d.n1 / ZK<32>::NL);
+    case 48: return (int)((int64_t)d.n0 * d.n1 / ZK<48>::NL);
+    case 64: return (int)((int64_t)d.n0 * d.n1 / ZK<64>::NL);
+    case 96: return (int)((int64_t)d.n0 * d.n1 / ZK<96>::NL);
+    case 128: return (int)((int64_t)d.n0 * d.n1 / ZK<128>::NL);
+    default: return -1;
+  }
+}
+int z_lines_per_block(int m2) {
+  switch (m2) {
+    case 16: return ZK<16>::NL;
+    case 24: return ZK<24>::NL;
+    case 32: return ZK<32>::NL;
+    case 48: return ZK<48>::NL;
+    case 64: return ZK<64>::NL;
+    case 96: return ZK<96>::NL;
+    case 128: return ZK<128>::NL;
+    default: return -1;
+  }
+}
+
+template <int M>
+struct RunZGradq {
+  static int run(const LaunchCtx& c, int4 sc, int4 sa) {
+    dim3 g((unsigned)((int64_t)c.d.n0 * c.d.n1 / ZK<M>::NL), c.nm);
+    k_z_gradq<M><<<g, ZK<M>::TPB, 0, c.stream>>>(c.mem, c.d, c.twm, c.tw2, sc, sa);
+    GLIA_CHECK_LAUNCH();
+    return 0;
+  }
+};
+int launch_z_gradq(const LaunchCtx& c, int4 sc, int4 sa) { return dispatch_M<RunZGradq>(c.d.m2, c, sc, sa); }
+
+int launch_pcg_update(const LaunchCtx& c, int s_x) {
+  dim3 g(UPD_BLOCKS, c.nm);
+  k_pcg_update<<<g, 256, 0, c.stream>>>(c.mem, c.d, s_x);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int pcg_update_blocks() { return UPD_BLOCKS; }
+
+int launch_pcg_begin(const LaunchCtx& c, double tau) {
+  k_pcg_begin<<<c.nm, 64, 0, c.stream>>>(c.mem, tau);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int launch_pcg_fin_setup(const LaunchCtx& c, int nb, int* hflag, int hslot) {
+  k_pcg_fin_setup<<<c.nm, 256, 0, c.stream>>>(c.mem, c.d, nb, hflag, hslot, c.nm);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int launch_pcg_fin_dpi(const LaunchCtx& c, int nb) {
+  k_pcg_fin_dpi<<<c.nm, 256, 0, c.stream>>>(c.mem, c.d, nb);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int launch_pcg_fin_update(const LaunchCtx& c, int nb, int* hflag, int hslot) {
+  k_pcg_fin_update<<<c.nm, 256, 0, c.stream>>>(c.mem, c.d, nb, hflag, hslot, c.nm);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+
+constexpr int EW_BLOCKS = 1024;
+int ew_blocks() { return EW_BLOCKS; }
+
+int launch_copy(const LaunchCtx& c, int s_in, int s_out) {
+  k_copy<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d, s_in, s_out);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int launch_copy_real(const LaunchCtx& c, int r_in, int r_out, int ch_in, int ch_out) {
+  k_copy_real<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d, r_in, r_out, ch_in, ch_out);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int launch_tissue(const LaunchCtx& c) {
+  k_tissue<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int launch_coef(const LaunchCtx& c, int r_out, const double* coefs_dev) {
+  k_coef<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d, r_out, coefs_dev);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int launch_obs_default(const LaunchCtx& c, float thr) {
+  k_obs_default<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d, thr);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int launch_fin_sum(const LaunchCtx& c, int nb, int nq) {
+  k_fin_sum<<<c.nm, 256, 0, c.stream>>>(c.mem, nb, nq);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int launch_fin_state(const LaunchCtx& c, int nb, int kind, double w) {
+  k_fin_state<<<c.nm, 256, 0, c.stream>>>(c.mem, nb, kind, w);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int launch_prec_factor(const LaunchCtx& c, double prec_dt) {
+  k_prec_factor<<<c.nm, 64, 0, c.stream>>>(c.mem, prec_dt);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int launch_state_reset(const LaunchCtx& c, int what) {
+  k_state_reset<<<c.nm, 64, 0, c.stream>>>(c.mem, what);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int launch_phi_apply(const LaunchCtx& c) {
+  k_phi_apply<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int launch_phi_applyT(const LaunchCtx& c, int np_max, int r_in, double scale) {
+  if (np_max <= 0) return 0;
+  k_phi_applyT<<<dim3(np_max, c.nm), 256, 0, c.stream>>>(c.mem, c.d, r_in, scale);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int launch_dot_real(const LaunchCtx& c, int r_x, int r_y) {
+  k_dot_real<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d, r_x, r_y);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int launch_set_done(const LaunchCtx& c, int v) {
+  k_set_done<<<c.nm, 64, 0, c.stream>>>(c.mem, v);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+int launch_axpby_real(const LaunchCtx& c, int r_x, double a, int r_y, double b, int r_out) {
+  k_axpby_real<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d, r_x, a, r_y, b, r_out);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+
+int launch_bricks(const LaunchCtx& c, const BrickJob* jobs_host, int njobs, const double* taps_host, int tap_stride,
+                  double h0, double h1, double h2, double sigma, float* store_dev, double* max_out_host) {
+  if (njobs <= 0) return 0;
+  int64_t ext_total = 0;
+  for (int i = 0; i < njobs; i++) {
+    const int64_t ne = (int64_t)(jobs_host[i].b[0] + 2 * jobs_host[i].kr) * (jobs_host[i].b[1] + 2 * jobs_host[i].kr) *
+                       (jobs_host[i].b[2] + 2 * jobs_host[i].kr);
+    if (jobs_host[i].ext_off + ne > ext_total) ext_total = jobs_host[i].ext_off + ne;
+  }
+  BrickJob* jd = nullptr;
+  double *e0 = nullptr, *e1 = nullptr, *taps = nullptr, *mx = nullptr;
+  hipError_t err = hipSuccess;
+#define GLIA_TRY(x) do { err = (x); if (err != hipSuccess) goto fail; } while (0)
+  GLIA_TRY(hipMalloc(&jd, sizeof(BrickJob) * njobs));
+  GLIA_TRY(hipMalloc(&e0, sizeof(double) * ext_total));
+  GLIA_TRY(hipMalloc(&e1, sizeof(double) * ext_total));
+  GLIA_TRY(hipMalloc(&taps, sizeof(double) * (size_t)njobs * 3 * tap_stride));
+  GLIA_TRY(hipMalloc(&mx, sizeof(double) * njobs));
+  GLIA_TRY(hipMemcpyAsync(jd, jobs_host, sizeof(BrickJob) * njobs, hipMemcpyHostToDevice, c.stream));
+  GLIA_TRY(hipMemcpyAsync(taps, taps_host, sizeof(double) * (size_t)njobs * 3 * tap_stride, hipMemcpyHostToDevice, c.stream));
+  k_brick_src<<<dim3(256, njobs), 256, 0, c.stream>>>(jd, c.d, c.mem, h0, h1, h2, sigma, e0);
+  k_brick_conv<<<dim3(256, njobs), 256, 0, c.stream>>>(jd, 0, taps, tap_stride, e0, e1);
+  k_brick_conv<<<dim3(256, njobs), 256, 0, c.stream>>>(jd, 1, taps, tap_stride, e1, e0);
+  k_brick_conv<<<dim3(256, njobs), 256, 0, c.stream>>>(jd, 2, taps, tap_stride, e0, e1);
+  k_brick_store<<<dim3(256, njobs), 256, 0, c.stream>>>(jd, c.d, h0, h1, h2, sigma, e1, store_dev);
+  k_brick_max<<<njobs, 256, 0, c.stream>>>(jd, store_dev, mx);
+  GLIA_TRY(hipGetLastError());
+  GLIA_TRY(hipMemcpyAsync(max_out_host, mx, sizeof(double) * njobs, hipMemcpyDeviceToHost, c.stream));
+  GLIA_TRY(hipStreamSynchronize(c.stream));
+fail:
+#undef GLIA_TRY
+  (void)hipFree(jd);
+  (void)hipFree(e0);
+  (void)hipFree(e1);
+  (void)hipFree(taps);
+  (void)hipFree(mx);
+  return (int)err;
+}
+
+}  // namespace glia

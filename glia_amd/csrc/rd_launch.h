@@ -1,0 +1,65 @@
+// rd_launch.h -- host-side launchers of the kernels in rd_kernels.hip (size dispatch included).
+// All launchers enqueue on c.stream and return 0 or a hipError_t / -1 (unsupported size).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "rd_device.h"
+
+namespace glia {
+
+struct LaunchCtx {
+  Dims d;
+  const MemberDev* mem;  // device array of nm member descriptors
+  int nm;
+  hipStream_t stream;
+  const float2* tw0;  // exp(-2 pi i m / n0), m < n0
+  const float2* tw1;  // exp(-2 pi i m / n1)
+  const float2* twm;  // exp(-2 pi i m / (n2/2))
+  const float2* tw2;  // exp(-2 pi i m / n2), m < n2/2
+};
+
+// modes of launch_xf_div and launch_z_point (values match the kernel enums)
+enum { XFM_MATVEC = 0, XFM_SETUP = 1, XFM_D = 2 };
+enum { ZPM_REACT = 0, ZPM_REACT_ADJ = 1, ZPM_C2R = 2, ZPM_R2C = 3, ZPM_MISFIT = 4 };
+enum { FINK_FILTER = 0, FINK_KSUM = 1, FINK_MISFIT = 2, FINK_C0SQ = 3, FINK_QUAD = 4 };
+
+bool supported_line(int L);
+int xf_div_blocks(const Dims& d);
+int z_blocks(const Dims& d);
+int z_lines_per_block(int m2);
+int pcg_update_blocks();
+int ew_blocks();
+
+int launch_xi_grad(const LaunchCtx& c, int s_in, int s_r, int s_u, int s_v, bool pupd);
+int launch_yi_grad(const LaunchCtx& c, int s_u, int s_v, int s_gx, int s_gy, int s_gz, int s_g0);
+int launch_yf_div(const LaunchCtx& c, int s_hx, int s_hy, int s_q, int s_yx, int s_rr);
+int launch_xf_div(const LaunchCtx& c, int mode, int s_yx, int s_rr, int s_p, int s_out);
+int launch_col_plain(const LaunchCtx& c, int dir, int sign, int s_in, int s_out, bool honor_done);
+int launch_z_diff(const LaunchCtx& c, int s_gx, int s_gy, int s_gz);
+int launch_z_point(const LaunchCtx& c, int mode, int s_in, int s_out, int r_slot, int ch_idx, float dt, bool honor_done);
+int launch_z_gradq(const LaunchCtx& c, int4 sc, int4 sa);
+
+int launch_pcg_begin(const LaunchCtx& c, double tau);
+int launch_pcg_update(const LaunchCtx& c, int s_x);
+int launch_pcg_fin_setup(const LaunchCtx& c, int nb, int* hflag, int hslot);
+int launch_pcg_fin_dpi(const LaunchCtx& c, int nb);
+int launch_pcg_fin_update(const LaunchCtx& c, int nb, int* hflag, int hslot);
+
+int launch_copy(const LaunchCtx& c, int s_in, int s_out);  // slots >= CS_NUM address the state pool
+int launch_copy_real(const LaunchCtx& c, int r_in, int r_out, int ch_in, int ch_out);
+int launch_tissue(const LaunchCtx& c);
+int launch_coef(const LaunchCtx& c, int r_out, const double* coefs_dev);
+int launch_obs_default(const LaunchCtx& c, float thr);
+int launch_fin_sum(const LaunchCtx& c, int nb, int nq);
+int launch_fin_state(const LaunchCtx& c, int nb, int kind, double w);
+int launch_prec_factor(const LaunchCtx& c, double prec_dt);
+int launch_state_reset(const LaunchCtx& c, int what);
+int launch_phi_apply(const LaunchCtx& c);
+int launch_phi_applyT(const LaunchCtx& c, int np_max, int r_in, double scale);
+int launch_dot_real(const LaunchCtx& c, int r_x, int r_y);
+int launch_set_done(const LaunchCtx& c, int v);
+int launch_axpby_real(const LaunchCtx& c, int r_x, double a, int r_y, double b, int r_out);
+int launch_bricks(const LaunchCtx& c, const BrickJob* jobs_host, int njobs, const double* taps_host, int tap_stride,
+                  double h0, double h1, double h2, double sigma, float* store_dev, double* max_out_host);
+
+}  // namespace glia

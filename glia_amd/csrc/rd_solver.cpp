@@ -1,0 +1,935 @@
+// rd_solver.cpp -- host side of the spectral reaction-diffusion solver (see rd_solver.h).
+//
+// The path mirrors the reference call sequence:
+//   DerivativeOperatorsRD::evaluateObjectiveAndGradient (src/grad/DerivativeOperatorsRD.cpp:130-226)
+//     updateReactionAndDiffusion -> Phi::apply -> PdeOperatorsRD::solveState -> misfit ->
+//     PdeOperatorsRD::solveAdjoint -> Phi::applyTranspose, gradDiffusion, gradReaction
+// with every CN half step (DiffusionSolver::solve, src/pde/DiffusionSolver.cpp:217-250) solved by
+// preconditioned CG in spectral space.  The CG iterates are the Fourier transforms of the
+// reference's real-space iterates (the preconditioner is diagonal there and inner products follow
+// from Parseval), and the stopping logic is PETSc 3.11 KSPCG with KSPConvergedDefault, evaluated
+// on the device.  The host only enqueues work and watches a mapped completion flag a few
+// iterations behind the device.
+#include "rd_solver.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+
+namespace glia {
+
+#define GLIA_TRYC(x)            \
+  do {                          \
+    int rc_ = (x);              \
+    if (rc_ != 0) return fail(rc_, std::string("launch failed: ") + #x); \
+  } while (0)
+#define GLIA_HIP(x)                                      \
+  do {                                                   \
+    hipError_t e_ = (x);                                 \
+    if (e_ != hipSuccess) return check(e_, #x);          \
+  } while (0)
+
+static bool supported_z(int n2) {
+  return n2 == 32 || n2 == 64 || n2 == 96 || n2 == 128 || n2 == 192 || n2 == 256;
+}
+
+RDEnsemble::RDEnsemble(const SolverConfig& cfg) : cfg_(cfg) {
+  nm_ = cfg.nmembers;
+  d_.n0 = cfg.n[0];
+  d_.n1 = cfg.n[1];
+  d_.n2 = cfg.n[2];
+  d_.m2 = cfg.n[2] / 2;
+  d_.nspec = (int64_t)d_.n0 * d_.n1 * d_.m2;
+  d_.ng = (int64_t)d_.n0 * d_.n1 * d_.n2;
+  d_.inv_ng = 1.0 / (double)d_.ng;
+  for (int i = 0; i < 3; i++) h_[i] = 2.0 * M_PI / cfg.n[i];
+  ctx_dt_ = cfg.dt;  // Ctx::dt_ = params dt at construction (src/pde/DiffusionSolver.cpp:11)
+}
+
+// blocking copy ordered on the solver stream (the stream is non-blocking w.r.t. the null stream)
+int RDEnsemble::copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, stream_);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  return check(e, "hipMemcpyAsync");
+}
+
+int RDEnsemble::fail(int code, const std::string& msg) {
+  err_ = msg;
+  return code ? code : -1;
+}
+
+int RDEnsemble::check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return 0;
+  std::ostringstream s;
+  s << what << ": " << hipGetErrorString(e);
+  err_ = s.str();
+  return (int)e;
+}
+
+LaunchCtx RDEnsemble::ctx() const {
+  LaunchCtx c;
+  c.d = d_;
+  c.mem = md_;
+  c.nm = nm_;
+  c.stream = stream_;
+  c.tw0 = tw_[0];
+  c.tw1 = tw_[1];
+  c.twm = tw_[2];
+  c.tw2 = tw_[3];
+  return c;
+}
+
+int RDEnsemble::init() {
+  if (nm_ < 1) return fail(-2, "nmembers must be >= 1");
+  if (!supported_line(d_.n0) || !supported_line(d_.n1) || !supported_z(d_.n2))
+    return fail(-2, "unsupported grid: n0, n1 in {32,48,64,96,128,192,256}, n2 in {32,64,96,128,192,256}");
+  if (z_blocks(d_) <= 0 || ((int64_t)d_.n0 * d_.n1) % z_lines_per_block(d_.m2) != 0)
+    return fail(-2, "unsupported grid: n0*n1 must be a multiple of the z-pass line group");
+  if (cfg_.nt < 1 || !(cfg_.dt > 0)) return fail(-2, "nt >= 1 and dt > 0 required");
+  if (cfg_.maxit < 1) return fail(-2, "pcg maxit >= 1 required");
+  if (cfg_.checkpoint_slots < 0) return fail(-2, "checkpoint_slots >= 0 required");
+  if (cfg_.stream) {
+    stream_ = cfg_.stream;
+  } else {
+    GLIA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    own_stream_ = true;
+  }
+  // twiddle tables in FP64, stored as FP32: n0, n1, n2/2, n2
+  {
+    const int lens[4] = {d_.n0, d_.n1, d_.m2, d_.n2};
+    int64_t tot = 0;
+    for (int i = 0; i < 4; i++) tot += lens[i];
+    std::vector<float2> h(tot);
+    int64_t off = 0;
+    int64_t offs[4];
+    for (int i = 0; i < 4; i++) {
+      offs[i] = off;
+      for (int m = 0; m < lens[i]; m++) {
+        const long double a = -2.0L * 3.14159265358979323846264338327950288L * m / lens[i];
+        h[off + m] = make_float2((float)cosl(a), (float)sinl(a));
+      }
+      off += lens[i];
+    }
+    GLIA_HIP(hipMalloc(&twbuf_, sizeof(float2) * tot));
+    GLIA_TRYC(copy_sync(twbuf_, h.data(), sizeof(float2) * tot, hipMemcpyHostToDevice));
+    for (int i = 0; i < 4; i++) tw_[i] = twbuf_ + offs[i];
+  }
+  GLIA_HIP(hipHostMalloc((void**)&hflag_host_, sizeof(int) * kFlagSlots * nm_, hipHostMallocMapped | hipHostMallocCoherent));
+  GLIA_HIP(hipHostGetDevicePointer((void**)&hflag_dev_, hflag_host_, 0));
+  for (int i = 0; i < kFlagSlots; i++) GLIA_HIP(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming));
+  ev_ok_ = true;
+  GLIA_HIP(hipMalloc(&coefs_dev_, sizeof(double) * 3 * nm_));
+
+  hist_all_ = (cfg_.checkpoint_slots == 0);
+  npool_ = hist_all_ ? cfg_.nt + 1 : std::max(1, cfg_.checkpoint_slots);
+  mh_.resize(nm_);
+  pool_.resize(nm_);
+  chalf_.resize(nm_);
+  hist_dev_.assign(nm_, nullptr);
+  chalf_dev_.assign(nm_, nullptr);
+  st_.assign(nm_, nullptr);
+  part_.assign(nm_, nullptr);
+  np_.assign(nm_, 0);
+  phimax_.assign(nm_, {});
+  centers_.assign(nm_, {});
+  sigma_.assign(nm_, 0.0);
+  bricks_dev_.assign(nm_, nullptr);
+  brick_data_.assign(nm_, nullptr);
+  phi_scale_dev_.assign(nm_, nullptr);
+  phit_dev_.assign(nm_, nullptr);
+  kappa_.assign(nm_, 0.0);
+  rho_.assign(nm_, 0.0);
+  active_.assign(nm_, 0);
+  for (int m = 0; m < nm_; m++) {
+    int rc = alloc_member(m);
+    if (rc) return rc;
+  }
+  GLIA_HIP(hipMalloc(&md_, sizeof(MemberDev) * nm_));
+  return upload_members();
+}
+
+int RDEnsemble::alloc_member(int m) {
+  MemberDev& M = mh_[m];
+  std::memset(&M, 0, sizeof(M));
+  const size_t cb = sizeof(float2) * d_.nspec, rb = sizeof(float) * d_.ng;
+  for (int s = 0; s < CS_NUM; s++) {
+    GLIA_HIP(hipMalloc(&M.c[s], cb));
+    GLIA_HIP(hipMemsetAsync(M.c[s], 0, cb, stream_));
+  }
+  for (int s = 0; s < RS_NUM; s++) {
+    GLIA_HIP(hipMalloc(&M.r[s], rb));
+    GLIA_HIP(hipMemsetAsync(M.r[s], 0, rb, stream_));
+  }
+  GLIA_HIP(hipMalloc(&st_[m], sizeof(PcgState)));
+  PcgState s0;
+  std::memset(&s0, 0, sizeof(s0));
+  s0.rtol = cfg_.rtol;
+  s0.atol = 1e-50;  // PETSc defaults
+  s0.dtol = 1e5;
+  s0.maxit = cfg_.maxit;
+  s0.active = 0;
+  s0.filter_sum = (double)d_.ng;
+  s0.done = 1;
+  GLIA_TRYC(copy_sync(st_[m], &s0, sizeof(s0), hipMemcpyHostToDevice));
+  M.st = st_[m];
+  // partial sums: the largest reduction grid times its width
+  int64_t nb = std::max<int64_t>((int64_t)xf_div_blocks(d_) * 3, (int64_t)z_blocks(d_) * 2);
+  nb = std::max<int64_t>(nb, (int64_t)pcg_update_blocks() * 2);
+  nb = std::max<int64_t>(nb, (int64_t)ew_blocks() * 2);
+  GLIA_HIP(hipMalloc(&part_[m], sizeof(double) * nb));
+  M.part = part_[m];
+  pool_[m].assign(npool_, nullptr);
+  for (int i = 0; i < npool_; i++) GLIA_HIP(hipMalloc(&pool_[m][i], cb));
+  GLIA_HIP(hipMalloc(&hist_dev_[m], sizeof(float2*) * npool_));
+  GLIA_TRYC(copy_sync(hist_dev_[m], pool_[m].data(), sizeof(float2*) * npool_, hipMemcpyHostToDevice));
+  M.hist = hist_dev_[m];
+  if (cfg_.adjoint_store && hist_all_) {
+    chalf_[m].assign(cfg_.nt, nullptr);
+    for (int i = 0; i < cfg_.nt; i++) GLIA_HIP(hipMalloc(&chalf_[m][i], rb));
+    GLIA_HIP(hipMalloc(&chalf_dev_[m], sizeof(float*) * cfg_.nt));
+    GLIA_TRYC(copy_sync(chalf_dev_[m], chalf_[m].data(), sizeof(float*) * cfg_.nt, hipMemcpyHostToDevice));
+  }
+  M.chalf = chalf_dev_[m];
+  M.kr_gm = (float)cfg_.k_gm_wm;
+  M.rr_gm = (float)cfg_.r_gm_wm;
+  // MatProp::setValues defaults (src/mat/MatProp.cpp:20-33): wm = 1, filter = 1; Obs filter = 1
+  std::vector<float> ones(d_.ng, 1.0f);
+  GLIA_TRYC(copy_sync(M.r[RS_WM], ones.data(), rb, hipMemcpyHostToDevice));
+  GLIA_TRYC(copy_sync(M.r[RS_FILTER], ones.data(), rb, hipMemcpyHostToDevice));
+  GLIA_TRYC(copy_sync(M.r[RS_OBS], ones.data(), rb, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int RDEnsemble::upload_members() {
+  GLIA_TRYC(copy_sync(md_, mh_.data(), sizeof(MemberDev) * nm_, hipMemcpyHostToDevice));
+  return 0;
+}
+
+RDEnsemble::~RDEnsemble() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (int m = 0; m < (int)mh_.size(); m++) {
+    for (int s = 0; s < CS_NUM; s++) (void)hipFree(mh_[m].c[s]);
+    for (int s = 0; s < RS_NUM; s++) (void)hipFree(mh_[m].r[s]);
+    for (auto p : pool_[m]) (void)hipFree(p);
+    for (auto p : chalf_[m]) (void)hipFree(p);
+    (void)hipFree(hist_dev_[m]);
+    (void)hipFree(chalf_dev_[m]);
+    (void)hipFree(st_[m]);
+    (void)hipFree(part_[m]);
+    (void)hipFree(bricks_dev_[m]);
+    (void)hipFree(brick_data_[m]);
+    (void)hipFree(phi_scale_dev_[m]);
+    (void)hipFree(phit_dev_[m]);
+  }
+  (void)hipFree(md_);
+  (void)hipFree(twbuf_);
+  (void)hipFree(coefs_dev_);
+  if (hflag_host_) (void)hipHostFree(hflag_host_);
+  if (ev_ok_)
+    for (int i = 0; i < kFlagSlots; i++) (void)hipEventDestroy(ev_[i]);
+  if (own_stream_) (void)hipStreamDestroy(stream_);
+}
+
+int RDEnsemble::sync() {
+  GLIA_HIP(hipStreamSynchronize(stream_));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// setup
+// ---------------------------------------------------------------------------------------------
+static LaunchCtx member_ctx(LaunchCtx c, const MemberDev* md, int m) {
+  c.mem = md + m;
+  c.nm = 1;
+  return c;
+}
+
+int RDEnsemble::set_field(int m, int rslot, const float* src, bool device) {
+  if (m < 0 || m >= nm_ || rslot < 0 || rslot >= RS_NUM) return fail(-2, "bad member or field");
+  const size_t rb = sizeof(float) * d_.ng;
+  if (!src) {
+    GLIA_HIP(hipMemsetAsync(mh_[m].r[rslot], 0, rb, stream_));
+  } else {
+    GLIA_HIP(hipMemcpyAsync(mh_[m].r[rslot], src, rb, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
+  }
+  if (!device) GLIA_HIP(hipStreamSynchronize(stream_));
+  return 0;
+}
+
+int RDEnsemble::get_field(int m, int rslot, float* dst, bool device) {
+  if (m < 0 || m >= nm_ || rslot < 0 || rslot >= RS_NUM) return fail(-2, "bad member or field");
+  GLIA_HIP(hipMemcpyAsync(dst, mh_[m].r[rslot], sizeof(float) * d_.ng, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                          stream_));
+  GLIA_HIP(hipStreamSynchronize(stream_));
+  return 0;
+}
+
+// MatProp::setValuesCustom + the recomputation SolverInterface::updateTumorCoefficients
+// (src/SolverInterface.cpp:986-1014) performs: k, rho, Phi and the preconditioner
+int RDEnsemble::set_tissue(int m, const float* wm, const float* gm, const float* csf, const float* vt, bool device) {
+  int rc;
+  if ((rc = set_field(m, RS_WM, wm, device))) return rc;
+  if ((rc = set_field(m, RS_GM, gm, device))) return rc;
+  if ((rc = set_field(m, RS_CSF, csf, device))) return rc;
+  if ((rc = set_field(m, RS_VT, vt, device))) return rc;
+  LaunchCtx c = member_ctx(ctx(), md_, m);
+  GLIA_TRYC(launch_tissue(c));
+  GLIA_TRYC(launch_fin_state(c, ew_blocks(), FINK_FILTER, 0.0));
+  if ((rc = set_kappa(m, kappa_[m]))) return rc;
+  if ((rc = set_rho(m, rho_[m]))) return rc;
+  GLIA_TRYC(launch_prec_factor(c, ctx_dt_));
+  if (np_[m] > 0) {
+    // Phi::setValues on the new mask
+    std::vector<double> ctr = centers_[m];
+    return set_basis(m, np_[m], ctr.data(), sigma_[m]);
+  }
+  return sync();
+}
+
+// MatProp::setValuesSinusoidal (src/mat/MatProp.cpp:34-68)
+int RDEnsemble::set_tissue_sinusoidal(int m) {
+  std::vector<float> wm(d_.ng), gm(d_.ng);
+  const double f = 4.0;
+  for (int x = 0; x < d_.n0; x++)
+    for (int y = 0; y < d_.n1; y++)
+      for (int z = 0; z < d_.n2; z++) {
+        const int64_t i = ((int64_t)x * d_.n1 + y) * d_.n2 + z;
+        const double w = 0.5 + 0.5 * std::sin(f * 2.0 * M_PI / d_.n0 * x) * std::sin(f * 2.0 * M_PI / d_.n1 * y) *
+                                   std::sin(f * 2.0 * M_PI / d_.n2 * z);
+        wm[i] = (float)w;
+        gm[i] = (float)(1 - w);
+      }
+  int rc;
+  if ((rc = set_field(m, RS_WM, wm.data(), false))) return rc;
+  if ((rc = set_field(m, RS_GM, gm.data(), false))) return rc;
+  if ((rc = set_field(m, RS_CSF, nullptr, false))) return rc;
+  if ((rc = set_field(m, RS_VT, nullptr, false))) return rc;
+  std::vector<float> ones(d_.ng, 1.0f);
+  if ((rc = set_field(m, RS_FILTER, ones.data(), false))) return rc;
+  const double fs = (double)d_.ng;
+  GLIA_TRYC(copy_sync(&st_[m]->filter_sum, &fs, sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// DiffCoef::setValues (src/mat/DiffCoef.cpp:77-132): k = dk_gm gm + dk_wm wm + dk_glm csf,
+// kbar = sum(k)/sum(filter)
+int RDEnsemble::set_kappa(int m, double kappa) {
+  kappa_[m] = kappa;
+  double dk_gm = kappa * cfg_.k_gm_wm, dk_wm = kappa, dk_glm = kappa * cfg_.k_glm_wm;
+  dk_gm = (dk_gm <= 0) ? 0.0 : dk_gm;
+  dk_glm = (dk_glm <= 0) ? 0.0 : dk_glm;
+  const double co[3] = {dk_gm, dk_wm, dk_glm};
+  GLIA_TRYC(copy_sync(coefs_dev_ + 3 * m, co, sizeof(co), hipMemcpyHostToDevice));
+  LaunchCtx c = member_ctx(ctx(), md_, m);
+  GLIA_TRYC(launch_coef(c, RS_K, coefs_dev_ + 3 * m));
+  GLIA_TRYC(launch_fin_state(c, ew_blocks(), FINK_KSUM, 0.0));
+  active_[m] = (kappa != 0.0);
+  const int a = active_[m];
+  GLIA_HIP(hipMemcpyAsync(&st_[m]->active, &a, sizeof(int), hipMemcpyHostToDevice, stream_));
+  return sync();
+}
+
+// ReacCoef::setValues (src/mat/ReacCoef.cpp:13-38)
+int RDEnsemble::set_rho(int m, double rho) {
+  rho_[m] = rho;
+  double dr_gm = rho * cfg_.r_gm_wm, dr_wm = rho, dr_glm = rho * cfg_.r_glm_wm;
+  dr_gm = (dr_gm <= 0) ? 0.0 : dr_gm;
+  dr_glm = (dr_glm <= 0) ? 0.0 : dr_glm;
+  const double co[3] = {dr_gm, dr_wm, dr_glm};
+  GLIA_TRYC(copy_sync(coefs_dev_ + 3 * m, co, sizeof(co), hipMemcpyHostToDevice));
+  LaunchCtx c = member_ctx(ctx(), md_, m);
+  GLIA_TRYC(launch_coef(c, RS_RHO, coefs_dev_ + 3 * m));
+  return sync();
+}
+
+int RDEnsemble::set_coefficients(int m, double kappa, double rho) {
+  int rc = set_kappa(m, kappa);
+  if (rc) return rc;
+  return set_rho(m, rho);
+}
+
+// DiffCoef::setValuesSinusoidal (src/mat/DiffCoef.cpp:134-189): kbar = sum(k)/ng
+int RDEnsemble::set_k_sinusoidal(int m, double scale) {
+  std::vector<float> k(d_.ng);
+  const double f = 4.0;
+  double s = 0;
+  for (int x = 0; x < d_.n0; x++)
+    for (int y = 0; y < d_.n1; y++)
+      for (int z = 0; z < d_.n2; z++) {
+        const int64_t i = ((int64_t)x * d_.n1 + y) * d_.n2 + z;
+        const double v = scale * (0.5 + 0.5 * std::sin(f * 2.0 * M_PI / d_.n0 * x) * std::sin(f * 2.0 * M_PI / d_.n1 * y) *
+                                            std::sin(f * 2.0 * M_PI / d_.n2 * z));
+        k[i] = (float)v;
+        s += (double)k[i];
+      }
+  int rc = set_field(m, RS_K, k.data(), false);
+  if (rc) return rc;
+  const double kavg = s / (double)d_.ng;
+  kappa_[m] = scale;
+  active_[m] = (scale != 0.0);
+  GLIA_TRYC(copy_sync(&st_[m]->kavg, &kavg, sizeof(double), hipMemcpyHostToDevice));
+  GLIA_TRYC(copy_sync(&st_[m]->active, &active_[m], sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// DiffusionSolver::precFactor (src/pde/DiffusionSolver.cpp:119-180): fixes (ctx dt, kbar)
+int RDEnsemble::prec_factor() {
+  GLIA_TRYC(launch_prec_factor(ctx(), ctx_dt_));
+  return 0;
+}
+
+// Phi::setGaussians + Phi::setValues (src/mat/Phi.cpp:76-144): bricks of the masked, smoothed,
+// truncated Gaussians; the smoothing is the separable 8-image periodic Gaussian of
+// computeWeierstrassFilter (src/cuda/SpectralOperators.cu:12-51), normalized to unit sum.
+int RDEnsemble::set_basis(int m, int np, const double* centers, double sigma) {
+  if (m < 0 || m >= nm_) return fail(-2, "bad member");
+  if (np < 0 || (np > 0 && !centers) || !(sigma > 0)) return fail(-2, "bad basis");
+  GLIA_HIP(hipStreamSynchronize(stream_));
+  (void)hipFree(bricks_dev_[m]);
+  (void)hipFree(brick_data_[m]);
+  (void)hipFree(phi_scale_dev_[m]);
+  (void)hipFree(phit_dev_[m]);
+  bricks_dev_[m] = nullptr;
+  brick_data_[m] = nullptr;
+  phi_scale_dev_[m] = nullptr;
+  phit_dev_[m] = nullptr;
+  np_[m] = np;
+  phimax_[m].assign(np, 0.0);
+  centers_[m].assign(centers, centers + 3 * np);
+  sigma_[m] = sigma;
+  const int n[3] = {d_.n0, d_.n1, d_.n2};
+  const double sig_s = cfg_.smoothing_factor * 2.0 * M_PI / d_.n0;
+  int kr = 0;
+  for (int a = 0; a < 3; a++)
+    if (sig_s > 0) kr = std::max(kr, std::min(n[a] / 2 - 1, (int)std::ceil(12.0 * sig_s / h_[a])));
+  const int tap_stride = 2 * kr + 1;
+  std::vector<double> taps1(3 * tap_stride);
+  for (int a = 0; a < 3; a++) {
+    if (sig_s <= 0) {
+      taps1[a * tap_stride + kr] = 1.0;
+      continue;
+    }
+    std::vector<double> f1(n[a]);
+    double sum = 0;
+    for (int i = 0; i < n[a]; i++) {
+      const double X = i * h_[a], Xp = X - 2.0 * M_PI;
+      double v = std::exp(-X * X / (2 * sig_s * sig_s)) + std::exp(-Xp * Xp / (2 * sig_s * sig_s));
+      if (v != v) v = 0;
+      f1[i] = v;
+      sum += v;
+    }
+    for (int s = -kr; s <= kr; s++) taps1[a * tap_stride + s + kr] = f1[((s % n[a]) + n[a]) % n[a]] / sum;
+  }
+  std::vector<Brick> bk(np);
+  std::vector<BrickJob> jobs(np);
+  int64_t off = 0;
+  for (int i = 0; i < np; i++) {
+    for (int a = 0; a < 3; a++) {
+      const double c = centers[3 * i + a];
+      int lo = (int)std::floor((c - 5.0 * sigma) / h_[a]);
+      int hi = (int)std::ceil((c + 5.0 * sigma) / h_[a]);
+      lo = std::max(lo, 0);
+      hi = std::min(hi, n[a] - 1);
+      if (hi < lo) hi = lo;
+      bk[i].o[a] = lo;
+      bk[i].b[a] = hi - lo + 1;
+      jobs[i].o[a] = lo;
+      jobs[i].b[a] = hi - lo + 1;
+      jobs[i].ctr[a] = c;
+    }
+    bk[i].off = off;
+    jobs[i].out_off = off;
+    jobs[i].kr = kr;
+    jobs[i].member = 0;
+    off += (int64_t)bk[i].b[0] * bk[i].b[1] * bk[i].b[2];
+  }
+  const int64_t total = std::max<int64_t>(off, 1);
+  GLIA_HIP(hipMalloc(&brick_data_[m], sizeof(float) * total));
+  GLIA_HIP(hipMalloc(&bricks_dev_[m], sizeof(Brick) * std::max(np, 1)));
+  GLIA_HIP(hipMalloc(&phi_scale_dev_[m], sizeof(double) * std::max(np, 1)));
+  GLIA_HIP(hipMalloc(&phit_dev_[m], sizeof(double) * std::max(np, 1)));
+  GLIA_HIP(hipMemsetAsync(phi_scale_dev_[m], 0, sizeof(double) * std::max(np, 1), stream_));
+  if (np > 0) GLIA_TRYC(copy_sync(bricks_dev_[m], bk.data(), sizeof(Brick) * np, hipMemcpyHostToDevice));
+  // build in chunks bounded by ~512 MB of FP64 scratch
+  LaunchCtx c = member_ctx(ctx(), md_, m);
+  int j0 = 0;
+  while (j0 < np) {
+    int64_t ext = 0;
+    int j1 = j0;
+    while (j1 < np) {
+      const int64_t ne = (int64_t)(jobs[j1].b[0] + 2 * kr) * (jobs[j1].b[1] + 2 * kr) * (jobs[j1].b[2] + 2 * kr);
+      if (j1 > j0 && (ext + ne) * 8 > (int64_t)512 << 20) break;
+      jobs[j1].ext_off = ext;
+      ext += ne;
+      j1++;
+    }
+    std::vector<double> taps(3 * tap_stride * (j1 - j0));
+    for (int j = j0; j < j1; j++) std::copy(taps1.begin(), taps1.end(), taps.begin() + (j - j0) * 3 * tap_stride);
+    GLIA_TRYC(launch_bricks(c, jobs.data() + j0, j1 - j0, taps.data(), tap_stride, h_[0], h_[1], h_[2], sigma, brick_data_[m],
+                            phimax_[m].data() + j0));
+    j0 = j1;
+  }
+  mh_[m].bricks = bricks_dev_[m];
+  mh_[m].brick_data = brick_data_[m];
+  mh_[m].phi_scale = phi_scale_dev_[m];
+  mh_[m].phit_out = phit_dev_[m];
+  mh_[m].np = np;
+  return upload_members();
+}
+
+// Obs::setDefaultFilter / setCustomFilter (src/mat/Obs.cpp:26-69)
+int RDEnsemble::set_data(int m, const float* d1, const float* obs_mask, double thr, bool device) {
+  int rc = set_field(m, RS_D1, d1, device);
+  if (rc) return rc;
+  if (obs_mask) return set_field(m, RS_OBS, obs_mask, device);
+  GLIA_TRYC(launch_obs_default(member_ctx(ctx(), md_, m), (float)thr));
+  return sync();
+}
+
+// ---------------------------------------------------------------------------------------------
+// transforms and the CN / PCG solve
+// ---------------------------------------------------------------------------------------------
+int RDEnsemble::clear_done() {
+  GLIA_TRYC(launch_set_done(ctx(), 0));
+  return 0;
+}
+
+int RDEnsemble::forward_transform_real(int rslot, int cslot) {
+  LaunchCtx c = ctx();
+  GLIA_TRYC(launch_z_point(c, ZPM_R2C, 0, cslot, rslot, -1, 0.f, false));
+  GLIA_TRYC(launch_col_plain(c, 1, -1, cslot, cslot, false));
+  GLIA_TRYC(launch_col_plain(c, 0, -1, cslot, cslot, false));
+  return 0;
+}
+
+int RDEnsemble::inverse_to_real(int cslot, int work, int rslot) {
+  LaunchCtx c = ctx();
+  GLIA_TRYC(launch_col_plain(c, 0, +1, cslot, work, false));
+  GLIA_TRYC(launch_col_plain(c, 1, +1, work, work, false));
+  GLIA_TRYC(launch_z_point(c, ZPM_C2R, work, 0, rslot, -1, 0.f, false));
+  return 0;
+}
+
+int RDEnsemble::fft_r2c(int rslot, int cslot) { return forward_transform_real(rslot, cslot); }
+int RDEnsemble::fft_c2r(int cslot, int rslot) { return inverse_to_real(cslot, CS_T7, rslot); }
+
+int RDEnsemble::get_spectrum(int m, int cslot, float2* host) {
+  GLIA_HIP(hipMemcpyAsync(host, mh_[m].c[cslot], sizeof(float2) * d_.nspec, hipMemcpyDeviceToHost, stream_));
+  GLIA_HIP(hipStreamSynchronize(stream_));
+  return 0;
+}
+
+int RDEnsemble::set_spectrum(int m, int cslot, const float2* host) {
+  if (m < 0 || m >= nm_ || cslot < 0 || cslot >= CS_NUM) return fail(-2, "bad member or slot");
+  return copy_sync(mh_[m].c[cslot], host, sizeof(float2) * d_.nspec, hipMemcpyHostToDevice);
+}
+
+// b - A x0 for x0 = c: r = tau D c, with the partial norms of B b and B r (KSPConvergedDefault)
+int RDEnsemble::matvec_setup(int sx) {
+  LaunchCtx c = ctx();
+  GLIA_TRYC(launch_xi_grad(c, sx, 0, CS_T0, CS_T1, false));
+  GLIA_TRYC(launch_yi_grad(c, CS_T0, CS_T1, CS_T2, CS_T3, CS_T4, -1));
+  GLIA_TRYC(launch_z_diff(c, CS_T2, CS_T3, CS_T4));
+  GLIA_TRYC(launch_yf_div(c, CS_T2, CS_T3, CS_T4, CS_T0, CS_T1));
+  GLIA_TRYC(launch_xf_div(c, XFM_SETUP, CS_T0, CS_T1, sx, CS_R));
+  return 0;
+}
+
+// DiffusionSolver::solve(c, tau) on the spectral state in slot sx, in place
+int RDEnsemble::cn_solve(int sx, double tau) {
+  ctx_dt_ = tau;  // ctx->dt_ = dt (src/pde/DiffusionSolver.cpp:226)
+  LaunchCtx c = ctx();
+  const int nbx = xf_div_blocks(d_);
+  GLIA_TRYC(launch_pcg_begin(c, tau));
+  int rc = matvec_setup(sx);
+  if (rc) return rc;
+  int stage = 0;
+  GLIA_TRYC(launch_pcg_fin_setup(c, nbx, hflag_dev_, stage % kFlagSlots));
+  GLIA_HIP(hipEventRecord(ev_[stage % kFlagSlots], stream_));
+  int checked = -1;
+  bool all_done = false;
+  auto check_stage = [&](int s) -> int {
+    GLIA_HIP(hipEventSynchronize(ev_[s % kFlagSlots]));
+    volatile int* f = hflag_host_ + (s % kFlagSlots) * nm_;
+    bool all = true;
+    for (int m = 0; m < nm_; m++) all = all && (f[m] != 0);
+    if (all) all_done = true;
+    return 0;
+  };
+  for (int it = 0; it < cfg_.maxit && !all_done; it++) {
+    stage = it + 1;
+    GLIA_TRYC(launch_xi_grad(c, CS_P, CS_R, CS_T0, CS_T1, true));
+    GLIA_TRYC(launch_yi_grad(c, CS_T0, CS_T1, CS_T2, CS_T3, CS_T4, -1));
+    GLIA_TRYC(launch_z_diff(c, CS_T2, CS_T3, CS_T4));
+    GLIA_TRYC(launch_yf_div(c, CS_T2, CS_T3, CS_T4, CS_T0, CS_T1));
+    GLIA_TRYC(launch_xf_div(c, XFM_MATVEC, CS_T0, CS_T1, CS_P, CS_W));
+    GLIA_TRYC(launch_pcg_fin_dpi(c, nbx));
+    GLIA_TRYC(launch_pcg_update(c, sx));
+    GLIA_TRYC(launch_pcg_fin_update(c, pcg_update_blocks(), hflag_dev_, stage % kFlagSlots));
+    GLIA_HIP(hipEventRecord(ev_[stage % kFlagSlots], stream_));
+    while (!all_done && checked < stage - kLookahead) {
+      checked++;
+      rc = check_stage(checked);
+      if (rc) return rc;
+    }
+  }
+  return 0;
+}
+
+// one Strang step (src/pde/PdeOperators.cpp:264-300): D_{dt/2}, R_dt, D_{dt/2}
+int RDEnsemble::forward_step(int sx, int step, bool store_half) {
+  const double dt = cfg_.dt;
+  int rc = cn_solve(sx, dt / 2.0);
+  if (rc) return rc;
+  LaunchCtx c = ctx();
+  GLIA_TRYC(launch_col_plain(c, 0, +1, sx, sx, false));
+  GLIA_TRYC(launch_col_plain(c, 1, +1, sx, sx, false));
+  GLIA_TRYC(launch_z_point(c, ZPM_REACT, sx, sx, -1, store_half ? step : -1, (float)dt, false));
+  GLIA_TRYC(launch_col_plain(c, 1, -1, sx, sx, false));
+  GLIA_TRYC(launch_col_plain(c, 0, -1, sx, sx, false));
+  return cn_solve(sx, dt / 2.0);
+}
+
+// PdeOperatorsRD::solveState (src/pde/PdeOperators.cpp:235-316) from c0 in RS_C0
+int RDEnsemble::solve_state(int linearized, bool keep_history) {
+  if (linearized != 0) return fail(-4, "solveState: only linearized = 0 (state equation) is supported");
+  int rc = forward_transform_real(RS_C0, CS_X);
+  if (rc) return rc;
+  stored_.clear();
+  free_pool_.clear();
+  std::vector<int> spine;  // time indices stored during the first sweep (checkpoint mode)
+  if (keep_history) {
+    GLIA_TRYC(launch_copy(ctx(), CS_X, state_slot(0)));
+    stored_[0] = 0;
+    if (!hist_all_) {
+      for (int i = npool_ - 1; i >= 1; i--) free_pool_.push_back(i);
+      // the right spine of the recursive schedule: positions the reversal will checkpoint first
+      int a = 0, s = npool_ - 1;
+      while (cfg_.nt - a > 1 && s > 0) {
+        // same rule as reverse(): split point from the cost recursion
+        int l = cfg_.nt - a;
+        // cost table is small: recompute the best first step k for (l, s)
+        std::vector<std::vector<double>> T(l + 1, std::vector<double>(s + 1, 0.0));
+        std::vector<std::vector<int>> K(l + 1, std::vector<int>(s + 1, 1));
+        for (int ll = 1; ll <= l; ll++) T[ll][0] = 0.5 * ll * (ll - 1);
+        for (int ss = 1; ss <= s; ss++)
+          for (int ll = 2; ll <= l; ll++) {
+            double best = 1e300;
+            int bk = 1;
+            for (int k = 1; k < ll; k++) {
+              const double v = k + T[ll - k][ss - 1] + T[k][ss];
+              if (v < best) { best = v; bk = k; }
+            }
+            T[ll][ss] = best;
+            K[ll][ss] = bk;
+          }
+        const int mpos = a + K[l][s];
+        spine.push_back(mpos);
+        a = mpos;
+        s--;
+      }
+    }
+  }
+  size_t sp = 0;
+  for (int i = 0; i < cfg_.nt; i++) {
+    const bool store_half = keep_history && hist_all_ && cfg_.adjoint_store;
+    rc = forward_step(CS_X, i, store_half);
+    if (rc) return rc;
+    if (keep_history) {
+      if (hist_all_) {
+        GLIA_TRYC(launch_copy(ctx(), CS_X, state_slot(i + 1)));
+        stored_[i + 1] = i + 1;
+      } else if (sp < spine.size() && spine[sp] == i + 1) {
+        const int pidx = free_pool_.back();
+        free_pool_.pop_back();
+        GLIA_TRYC(launch_copy(ctx(), CS_X, state_slot(pidx)));
+        stored_[i + 1] = pidx;
+        sp++;
+      }
+    }
+  }
+  return inverse_to_real(CS_X, CS_T5, RS_CT);
+}
+
+// alpha(T) = -O^T (O c(T) - d1) and m1 = ||O c(T) - d1||^2 (src/grad/DerivativeOperatorsRD.cpp:154-162)
+int RDEnsemble::misfit_and_final_adjoint(int cslot_c) {
+  LaunchCtx c = ctx();
+  GLIA_TRYC(launch_col_plain(c, 0, +1, cslot_c, CS_T5, false));
+  GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T5, CS_T5, false));
+  GLIA_TRYC(launch_z_point(c, ZPM_MISFIT, CS_T5, CS_T5, -1, -1, 0.f, false));
+  GLIA_TRYC(launch_col_plain(c, 1, -1, CS_T5, CS_T5, false));
+  GLIA_TRYC(launch_col_plain(c, 0, -1, CS_T5, CS_A, false));
+  GLIA_TRYC(launch_fin_state(c, z_blocks(d_), FINK_MISFIT, 0.0));
+  return 0;
+}
+
+// trapezoid term w * (<m_k, grad c . grad alpha>, <m_rho, (c^2 - c) alpha>) at one time point
+int RDEnsemble::quad(int cslot_c, int cslot_a, double w) {
+  LaunchCtx c = ctx();
+  int rc = clear_done();
+  if (rc) return rc;
+  GLIA_TRYC(launch_xi_grad(c, cslot_c, 0, CS_T0, CS_T1, false));
+  GLIA_TRYC(launch_yi_grad(c, CS_T0, CS_T1, CS_T2, CS_T3, CS_T4, CS_T5));
+  GLIA_TRYC(launch_xi_grad(c, cslot_a, 0, CS_T0, CS_T1, false));
+  GLIA_TRYC(launch_yi_grad(c, CS_T0, CS_T1, CS_T6, CS_T7, CS_R, CS_P));
+  GLIA_TRYC(launch_z_gradq(c, make_int4(CS_T5, CS_T2, CS_T3, CS_T4), make_int4(CS_P, CS_T6, CS_T7, CS_R)));
+  GLIA_TRYC(launch_fin_state(c, z_blocks(d_), FINK_QUAD, w));
+  return 0;
+}
+
+// one backward step of PdeOperatorsRD::solveAdjoint (src/pde/PdeOperators.cpp:372-420) with
+// reactionAdjoint (:318-370), then the trapezoid term at time i
+int RDEnsemble::adjoint_step(int i, int cslot_c) {
+  const double dt = cfg_.dt;
+  int rc = cn_solve(CS_A, dt / 2.0);
+  if (rc) return rc;
+  LaunchCtx c = ctx();
+  int ch = -1;
+  if (hist_all_ && cfg_.adjoint_store) {
+    ch = i;
+  } else {
+    GLIA_TRYC(launch_copy(c, cslot_c, CS_H));
+    rc = cn_solve(CS_H, dt / 2.0);
+    if (rc) return rc;
+    rc = inverse_to_real(CS_H, CS_H, RS_CH);
+    if (rc) return rc;
+  }
+  GLIA_TRYC(launch_col_plain(c, 0, +1, CS_A, CS_A, false));
+  GLIA_TRYC(launch_col_plain(c, 1, +1, CS_A, CS_A, false));
+  GLIA_TRYC(launch_z_point(c, ZPM_REACT_ADJ, CS_A, CS_A, ch >= 0 ? -1 : RS_CH, ch, (float)dt, false));
+  GLIA_TRYC(launch_col_plain(c, 1, -1, CS_A, CS_A, false));
+  GLIA_TRYC(launch_col_plain(c, 0, -1, CS_A, CS_A, false));
+  rc = cn_solve(CS_A, dt / 2.0);
+  if (rc) return rc;
+  return quad(cslot_c, CS_A, (i == 0) ? 0.5 : 1.0);
+}
+
+// reverse the steps b-1 ... a with c_a in pool entry pa and free_slots further entries free
+int RDEnsemble::reverse(int a, int b, int pa, int free_slots) {
+  if (b - a <= 0) return 0;
+  if (b - a == 1) return adjoint_step(a, state_slot(pa));
+  if (free_slots <= 0) {
+    for (int j = b - 1; j >= a; j--) {
+      GLIA_TRYC(launch_copy(ctx(), state_slot(pa), CS_X));
+      for (int t = a; t < j; t++) {
+        int rc = forward_step(CS_X, t, false);
+        if (rc) return rc;
+      }
+      int rc = adjoint_step(j, CS_X);
+      if (rc) return rc;
+    }
+    return 0;
+  }
+  // split point from the recomputation-cost recursion T(l,s) = min_k k + T(l-k,s-1) + T(k,s)
+  const int l = b - a, s = free_slots;
+  std::vector<std::vector<double>> T(l + 1, std::vector<double>(s + 1, 0.0));
+  std::vector<std::vector<int>> K(l + 1, std::vector<int>(s + 1, 1));
+  for (int ll = 1; ll <= l; ll++) T[ll][0] = 0.5 * ll * (ll - 1);
+  for (int ss = 1; ss <= s; ss++)
+    for (int ll = 2; ll <= l; ll++) {
+      double best = 1e300;
+      int bk = 1;
+      for (int k = 1; k < ll; k++) {
+        const double v = k + T[ll - k][ss - 1] + T[k][ss];
+        if (v < best) { best = v; bk = k; }
+      }
+      T[ll][ss] = best;
+      K[ll][ss] = bk;
+    }
+  const int mpos = a + K[l][s];
+  int pm;
+  auto it = stored_.find(mpos);
+  if (it != stored_.end()) {
+    pm = it->second;
+  } else {
+    if (free_pool_.empty()) return fail(-5, "checkpoint pool exhausted");
+    pm = free_pool_.back();
+    free_pool_.pop_back();
+    GLIA_TRYC(launch_copy(ctx(), state_slot(pa), CS_X));
+    for (int t = a; t < mpos; t++) {
+      int rc = forward_step(CS_X, t, false);
+      if (rc) return rc;
+    }
+    GLIA_TRYC(launch_copy(ctx(), CS_X, state_slot(pm)));
+    stored_[mpos] = pm;
+  }
+  int rc = reverse(mpos, b, pm, free_slots - 1);
+  if (rc) return rc;
+  stored_.erase(mpos);
+  free_pool_.push_back(pm);
+  return reverse(a, mpos, pa, free_slots);
+}
+
+// PdeOperatorsRD::solveAdjoint from alpha(T) of the current data, with the gradient quadrature
+int RDEnsemble::solve_adjoint(int linearized) {
+  if (linearized != 0 && linearized != 1) return fail(-4, "solveAdjoint: linearized must be 0 or 1");
+  if (stored_.find(0) == stored_.end()) return fail(-6, "solveAdjoint needs solveState with history first");
+  GLIA_TRYC(launch_state_reset(ctx(), 2));
+  int rc = misfit_and_final_adjoint(CS_X);
+  if (rc) return rc;
+  rc = quad(CS_X, CS_A, 0.5);
+  if (rc) return rc;
+  if (hist_all_) {
+    for (int i = cfg_.nt - 1; i >= 0; i--) {
+      rc = adjoint_step(i, state_slot(i));
+      if (rc) return rc;
+    }
+  } else {
+    // the spine checkpoints of the first sweep are the allocations the recursion makes first
+    rc = reverse(0, cfg_.nt, stored_[0], npool_ - 1);
+    if (rc) return rc;
+  }
+  return inverse_to_real(CS_A, CS_T5, RS_A0);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Phi apply / transpose, derivative operators
+// ---------------------------------------------------------------------------------------------
+int RDEnsemble::phi_apply(const std::vector<std::vector<double>>& p) {
+  for (int m = 0; m < nm_; m++) {
+    const int np = np_[m];
+    std::vector<double> sc(std::max(np, 1), 0.0);
+    double pmax = 0;
+    bool nnz = false;
+    double all = 0;
+    for (int i = 0; i < np; i++) all = std::max(all, phimax_[m][i]);
+    for (int i = 0; i < np; i++)
+      if (p[m][i] != 0) {
+        nnz = true;
+        pmax = std::max(pmax, phimax_[m][i]);
+      }
+    // phi_store_: all bases normalized by the global max; on the fly: by the max over active bases
+    double norm = cfg_.phi_store ? all : (nnz ? pmax : 1.0);
+    if (norm == 0) norm = 1.0;
+    for (int i = 0; i < np; i++) sc[i] = p[m][i] / norm;
+    if (np > 0) GLIA_TRYC(copy_sync(phi_scale_dev_[m], sc.data(), sizeof(double) * np, hipMemcpyHostToDevice));
+  }
+  LaunchCtx c = ctx();
+  GLIA_TRYC(launch_phi_apply(c));
+  GLIA_TRYC(launch_fin_state(c, ew_blocks(), FINK_C0SQ, 0.0));
+  return 0;
+}
+
+int RDEnsemble::phi_apply_transpose(int rslot, std::vector<std::vector<double>>& out) {
+  out.assign(nm_, {});
+  for (int m = 0; m < nm_; m++) {
+    const int np = np_[m];
+    double all = 0;
+    for (int i = 0; i < np; i++) all = std::max(all, phimax_[m][i]);
+    if (all == 0) all = 1.0;
+    GLIA_TRYC(launch_phi_applyT(member_ctx(ctx(), md_, m), np, rslot, 1.0 / all));
+  }
+  GLIA_HIP(hipStreamSynchronize(stream_));
+  for (int m = 0; m < nm_; m++) {
+    out[m].assign(np_[m], 0.0);
+    if (np_[m] > 0) GLIA_TRYC(copy_sync(out[m].data(), phit_dev_[m], sizeof(double) * np_[m], hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int RDEnsemble::diffusion_solve_field(int rslot, double tau) {
+  int rc = forward_transform_real(rslot, CS_X);
+  if (rc) return rc;
+  rc = cn_solve(CS_X, tau);
+  if (rc) return rc;
+  rc = inverse_to_real(CS_X, CS_T5, rslot);
+  if (rc) return rc;
+  return sync();
+}
+
+int RDEnsemble::apply_d_field(int rin, int rout) {
+  int rc = forward_transform_real(rin, CS_X);
+  if (rc) return rc;
+  rc = clear_done();
+  if (rc) return rc;
+  LaunchCtx c = ctx();
+  GLIA_TRYC(launch_xi_grad(c, CS_X, 0, CS_T0, CS_T1, false));
+  GLIA_TRYC(launch_yi_grad(c, CS_T0, CS_T1, CS_T2, CS_T3, CS_T4, -1));
+  GLIA_TRYC(launch_z_diff(c, CS_T2, CS_T3, CS_T4));
+  GLIA_TRYC(launch_yf_div(c, CS_T2, CS_T3, CS_T4, CS_T0, CS_T1));
+  GLIA_TRYC(launch_xf_div(c, XFM_D, CS_T0, CS_T1, CS_X, CS_T5));
+  rc = inverse_to_real(CS_T5, CS_T6, rout);
+  if (rc) return rc;
+  return sync();
+}
+
+int RDEnsemble::reset_counters() {
+  GLIA_TRYC(launch_state_reset(ctx(), 1));
+  return 0;
+}
+
+// DerivativeOperatorsRD::evaluateObjective / evaluateObjectiveAndGradient
+//   mode 0: p and kappa (flag_reaction_inv_ = false); mode 1: kappa and rho (c0 fixed);
+//   mode 2: p, kappa and rho together.  x[m] = [p (np) | kappa | rho].
+int RDEnsemble::objective_and_gradient(int mode, const std::vector<std::vector<double>>& x, bool want_grad,
+                                       std::vector<double>& J, std::vector<std::vector<double>>& dJ) {
+  if (mode < 0 || mode > 2) return fail(-2, "mode must be 0, 1 or 2");
+  if ((int)x.size() != nm_) return fail(-2, "x must hold one vector per member");
+  for (int m = 0; m < nm_; m++)
+    if ((int)x[m].size() != np_[m] + 2) return fail(-2, "x[m] must have np + 2 entries");
+  int rc;
+  // updateReactionAndDiffusion (src/grad/DerivativeOperators.cpp:323-367), nk = nr = 1
+  for (int m = 0; m < nm_; m++) {
+    if ((rc = set_kappa(m, x[m][np_[m]] * cfg_.k_scale))) return rc;
+    if (mode != 0 && (rc = set_rho(m, x[m][np_[m] + 1] * cfg_.rho_scale))) return rc;
+  }
+  if ((rc = prec_factor())) return rc;
+  LaunchCtx c = ctx();
+  if (mode != 1) {
+    if ((rc = phi_apply(x))) return rc;
+  } else {
+    GLIA_TRYC(launch_dot_real(c, RS_C0, RS_C0));
+    GLIA_TRYC(launch_fin_state(c, ew_blocks(), FINK_C0SQ, 0.0));
+  }
+  GLIA_TRYC(launch_state_reset(c, 3));
+  if ((rc = solve_state(0, want_grad))) return rc;
+  std::vector<std::vector<double>> phit;
+  if (want_grad) {
+    if ((rc = solve_adjoint(1))) return rc;
+    if (mode != 1) {
+      GLIA_TRYC(launch_axpby_real(c, RS_A0, 1.0, RS_C0, -cfg_.beta, RS_CH));
+      if ((rc = phi_apply_transpose(RS_CH, phit))) return rc;
+    }
+  } else {
+    if ((rc = misfit_and_final_adjoint(CS_X))) return rc;
+  }
+  if ((rc = sync())) return rc;
+  const double h3 = lebesgue();
+  J.assign(nm_, 0.0);
+  dJ.assign(nm_, {});
+  for (int m = 0; m < nm_; m++) {
+    PcgState s;
+    GLIA_TRYC(copy_sync(&s, st_[m], sizeof(s), hipMemcpyDeviceToHost));
+    const double reg = s.c0sq * 0.5 * cfg_.beta * h3;
+    J[m] = h3 * 0.5 * s.m1 + reg;
+    dJ[m].assign(np_[m] + 2, 0.0);
+    if (want_grad) {
+      if (mode != 1)
+        for (int i = 0; i < np_[m]; i++) dJ[m][i] = -h3 * phit[m][i];
+      dJ[m][np_[m]] = h3 * cfg_.dt * s.gk;
+      dJ[m][np_[m] + 1] = (mode == 0) ? 0.0 : h3 * cfg_.dt * s.gr;
+    }
+  }
+  return 0;
+}
+
+int RDEnsemble::stats(int m, MemberStats* out) {
+  if (m < 0 || m >= nm_) return fail(-2, "bad member");
+  GLIA_HIP(hipStreamSynchronize(stream_));
+  PcgState s;
+  GLIA_TRYC(copy_sync(&s, st_[m], sizeof(s), hipMemcpyDeviceToHost));
+  out->its_total = s.its_total;
+  out->solves = s.solves;
+  out->worst_reason = s.worst_reason;
+  out->last_its = s.its;
+  out->last_reason = s.reason;
+  out->kavg = s.kavg;
+  out->pc = s.pc;
+  out->filter_sum = s.filter_sum;
+  out->m1 = s.m1;
+  out->c0sq = s.c0sq;
+  return 0;
+}
+
+}  // namespace glia

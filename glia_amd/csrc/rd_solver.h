@@ -1,0 +1,151 @@
+// rd_solver.h -- host-side solver of the reaction-diffusion forward / adjoint / gradient path for
+// an ensemble of members that share one grid and one time discretization.  A plain solve is an
+// ensemble of one.  All device work is enqueued on one HIP stream; kernels of a launch cover
+// every member (grid.y = member), and converged members' workgroups exit at once.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "rd_device.h"
+#include "rd_launch.h"
+
+namespace glia {
+
+struct SolverConfig {
+  int n[3] = {64, 64, 64};
+  int nmembers = 1;
+  double dt = 0.5;          // TumorParameters::dt_ default
+  int nt = 1;               // TumorParameters::nt_ default
+  double rtol = 1e-6;       // DiffusionSolver KSP tolerances
+  int maxit = 5000;
+  int checkpoint_slots = 0; // 0: store every time step; s > 0: binomial checkpointing with s states
+  int adjoint_store = 0;    // keep half-step states (store-every-step mode only)
+  int phi_store = 0;        // Phi normalization of phi_store_ (global max) vs on-the-fly
+  double k_gm_wm = 0.0, k_glm_wm = 0.0, r_gm_wm = 0.0, r_glm_wm = 0.0;
+  double beta = 0.0;        // L2 regularization weight
+  double k_scale = 1.0, rho_scale = 1.0;  // OptimizerSettings k_scale_, rho_scale_
+  double smoothing_factor = 1.0;
+  hipStream_t stream = nullptr;
+};
+
+struct MemberStats {
+  int its_total, solves, worst_reason, last_its, last_reason;
+  double kavg, pc, filter_sum, m1, c0sq;
+};
+
+class RDEnsemble {
+ public:
+  explicit RDEnsemble(const SolverConfig& cfg);
+  ~RDEnsemble();
+  int init();  // allocation; returns 0 or an error code (message in error())
+
+  const std::string& error() const { return err_; }
+  const Dims& dims() const { return d_; }
+  int nmembers() const { return nm_; }
+  const SolverConfig& config() const { return cfg_; }
+
+  // --- setup (per member; host or device source pointers, FP32 fields of n0*n1*n2) ---
+  int set_tissue(int m, const float* wm, const float* gm, const float* csf, const float* vt, bool device);
+  int set_tissue_sinusoidal(int m);
+  int set_field(int m, int rslot, const float* src, bool device);
+  int get_field(int m, int rslot, float* dst, bool device);
+  int set_coefficients(int m, double kappa, double rho);     // DiffCoef/ReacCoef::setValues
+  int set_kappa(int m, double kappa);
+  int set_rho(int m, double rho);
+  int set_k_sinusoidal(int m, double scale);                 // DiffCoef::setValuesSinusoidal
+  int prec_factor();                                         // DiffusionSolver::precFactor
+  void set_ctx_dt(double dt) { ctx_dt_ = dt; }
+  double ctx_dt() const { return ctx_dt_; }
+  int set_basis(int m, int np, const double* centers, double sigma);
+  int set_data(int m, const float* d1, const float* obs_mask, double thr, bool device);
+
+  // --- operations on every member ---
+  int phi_apply(const std::vector<std::vector<double>>& p);  // c0 = Phi p (RS_C0)
+  int phi_apply_transpose(int rslot, std::vector<std::vector<double>>& out);  // Phi^T v
+  int diffusion_solve_field(int rslot, double tau);          // real field in place (test/API)
+  int apply_d_field(int rslot_in, int rslot_out);            // real D c (test/API)
+  int fft_r2c(int rslot, int cslot);                         // packed spectrum of a real field
+  int fft_c2r(int cslot, int rslot);                         // real field of a packed spectrum (1/N^3 included)
+  int get_spectrum(int m, int cslot, float2* host);
+  int set_spectrum(int m, int cslot, const float2* host);
+  int solve_state(int linearized, bool keep_history);
+  int solve_adjoint(int linearized);
+  int objective_and_gradient(int mode, const std::vector<std::vector<double>>& x, bool want_grad,
+                             std::vector<double>& J, std::vector<std::vector<double>>& dJ);
+  int stats(int m, MemberStats* s);
+  int basis_size(int m) const { return (m >= 0 && m < nm_) ? np_[m] : -1; }
+  int reset_counters();
+  int sync();
+
+  double lebesgue() const { return h_[0] * h_[1] * h_[2]; }
+
+ private:
+  int alloc_member(int m);
+  int upload_members();
+  int check(hipError_t e, const char* what);
+  int copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind);
+  int fail(int code, const std::string& msg);
+  LaunchCtx ctx() const;
+
+  int cn_solve(int sx, double tau);
+  int matvec_setup(int sx);
+  int forward_step(int sx, int step, bool store_half);
+  int forward_transform_real(int rslot, int cslot);  // R2C of a real slot into a complex slot
+  int inverse_to_real(int cslot, int work, int rslot);
+  int adjoint_step(int i, int cslot_c);
+  int quad(int cslot_c, int cslot_a, double w);
+  int misfit_and_final_adjoint(int cslot_c);
+  int clear_done();
+  int reverse(int a, int b, int pa, int free_slots);
+  int state_slot(int pool_index) const { return CS_NUM + pool_index; }
+
+  SolverConfig cfg_;
+  Dims d_;
+  int nm_ = 0;
+  double h_[3];
+  std::string err_;
+  hipStream_t stream_ = nullptr;
+  bool own_stream_ = false;
+  double ctx_dt_ = 0.5;
+
+  // per-member device memory
+  std::vector<MemberDev> mh_;           // host mirror of the member descriptors
+  MemberDev* md_ = nullptr;             // device array
+  std::vector<std::vector<float2*>> pool_;   // spectral state pool (history / checkpoints)
+  std::vector<std::vector<float*>> chalf_;   // real half-step states (adjoint_store)
+  std::vector<float2**> hist_dev_;
+  std::vector<float**> chalf_dev_;
+  std::vector<PcgState*> st_;
+  std::vector<double*> part_;
+  std::vector<int> np_;
+  std::vector<std::vector<double>> phimax_;  // per-basis maxima of the raw bricks
+  std::vector<std::vector<double>> centers_;
+  std::vector<double> sigma_;
+  std::vector<Brick*> bricks_dev_;
+  std::vector<float*> brick_data_;
+  std::vector<double*> phi_scale_dev_;
+  std::vector<double*> phit_dev_;
+  std::vector<double> kappa_, rho_;
+  std::vector<int> active_;
+  double* coefs_dev_ = nullptr;
+
+  const float2* tw_[4] = {nullptr, nullptr, nullptr, nullptr};
+  float2* twbuf_ = nullptr;
+
+  int* hflag_host_ = nullptr;
+  int* hflag_dev_ = nullptr;
+  static constexpr int kFlagSlots = 4;
+  static constexpr int kLookahead = 2;
+  hipEvent_t ev_[kFlagSlots];
+  bool ev_ok_ = false;
+
+  int npool_ = 0;       // number of pool buffers per member
+  bool hist_all_ = true;
+  std::map<int, int> stored_;  // time index -> pool index (checkpoint mode)
+  std::vector<int> free_pool_;
+};
+
+}  // namespace glia

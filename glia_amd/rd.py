@@ -1,0 +1,299 @@
+"""Python mirror of the reference's reaction-diffusion operator interface over libglia_rd.so.
+
+Class and method names follow the reference (GLIA): ``DiffusionSolver.solve / precFactor``
+(src/pde/DiffusionSolver.cpp), ``PdeOperatorsRD.solveState / solveAdjoint``
+(src/pde/PdeOperators.cpp), ``DerivativeOperatorsRD.evaluateObjective / evaluateGradient /
+evaluateObjectiveAndGradient`` (src/grad/DerivativeOperatorsRD.cpp).  They share one
+``RDSolver`` handle, the counterpart of the reference's Tumor + SpectralOperators + operators
+object graph.  Every call goes to the HIP library; fields may be numpy arrays (host) or torch
+tensors on the GPU (device pointers are passed through without copies).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+_F = _lib.FIELD
+
+
+class GliaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("glia_rd error %d: %s" % (code, msg))
+        self.code = code
+
+
+def _is_device(a):
+    return hasattr(a, "data_ptr") and getattr(a, "is_cuda", False)
+
+
+class RDSolver:
+    """Handle over an ensemble of members sharing one grid (nmembers = 1: the plain solve)."""
+
+    def __init__(self, n, nmembers=1, dt=0.5, nt=1, rtol=1e-6, maxit=5000, checkpoint_slots=0,
+                 adjoint_store=False, phi_store=False, k_gm_wm=0.0, k_glm_wm=0.0, r_gm_wm=0.0,
+                 r_glm_wm=0.0, beta=0.0, k_scale=1.0, rho_scale=1.0, smoothing_factor=1.0, stream=None):
+        self.L = _lib.load()
+        if np.isscalar(n):
+            n = (int(n),) * 3
+        self.n = tuple(int(v) for v in n)
+        self.shape = self.n
+        self.ng = self.n[0] * self.n[1] * self.n[2]
+        self.nmembers = int(nmembers)
+        self.dt, self.nt = float(dt), int(nt)
+        self.h3 = (2 * np.pi) ** 3 / self.ng
+        c = _lib.Config()
+        self.L.glia_rd_config_init(ctypes.byref(c))
+        c.n[0], c.n[1], c.n[2] = self.n
+        c.nmembers = self.nmembers
+        c.dt, c.nt, c.order = self.dt, self.nt, 2
+        c.pcg_rtol, c.pcg_maxit = float(rtol), int(maxit)
+        c.checkpoint_slots = int(checkpoint_slots)
+        c.adjoint_store, c.phi_store = int(bool(adjoint_store)), int(bool(phi_store))
+        c.k_gm_wm, c.k_glm_wm, c.r_gm_wm, c.r_glm_wm = map(float, (k_gm_wm, k_glm_wm, r_gm_wm, r_glm_wm))
+        c.beta, c.k_scale, c.rho_scale = float(beta), float(k_scale), float(rho_scale)
+        c.smoothing_factor = float(smoothing_factor)
+        c.stream = stream
+        self.cfg = c
+        h = ctypes.c_void_p()
+        rc = self.L.glia_rd_create(ctypes.byref(c), ctypes.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.L.glia_rd_last_error(h).decode() if h.value else "create failed"
+            if h.value:
+                self.L.glia_rd_destroy(h)
+            self.h = None
+            raise GliaError(rc, msg)
+        self.np = [0] * self.nmembers
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.glia_rd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise GliaError(rc, self.L.glia_rd_last_error(self.h).decode())
+
+    # ---- field marshalling ---------------------------------------------------
+    def _src(self, a):
+        """(pointer, mem_kind, keepalive) of a field given as numpy or a torch GPU tensor."""
+        if a is None:
+            return None, _lib.HOST, None
+        if _is_device(a):
+            import torch
+            t = a.detach().to(torch.float32).contiguous()
+            if t.numel() != self.ng:
+                raise ValueError("field has %d entries, grid has %d" % (t.numel(), self.ng))
+            return ctypes.c_void_p(t.data_ptr()), _lib.DEVICE, t
+        arr = np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+        if arr.size != self.ng:
+            raise ValueError("field has %d entries, grid has %d" % (arr.size, self.ng))
+        return arr.ctypes.data_as(ctypes.c_void_p), _lib.HOST, arr
+
+    def set_field(self, member, which, a):
+        p, kind, keep = self._src(a)
+        self._chk(self.L.glia_rd_set_field(self.h, member, _F[which], p, kind))
+
+    def get_field(self, member, which, out=None):
+        if out is not None and _is_device(out):
+            self._chk(self.L.glia_rd_get_field(self.h, member, _F[which], ctypes.c_void_p(out.data_ptr()), _lib.DEVICE))
+            return out
+        o = np.empty(self.ng, dtype=np.float32)
+        self._chk(self.L.glia_rd_get_field(self.h, member, _F[which], o.ctypes.data_as(ctypes.c_void_p), _lib.HOST))
+        return o.reshape(self.shape)
+
+    # ---- setup (MatProp, DiffCoef, ReacCoef, Phi, Obs) -------------------------
+    def set_tissue(self, member, wm=None, gm=None, csf=None, vt=None):
+        """MatProp::setValuesCustom + updateTumorCoefficients (src/mat/MatProp.cpp:134-203)."""
+        ps = [self._src(a) for a in (wm, gm, csf, vt)]
+        kinds = {k for (_, k, keep) in ps if keep is not None}
+        if len(kinds) > 1:
+            raise ValueError("tissue fields must all be host or all device arrays")
+        kind = kinds.pop() if kinds else _lib.HOST
+        self._chk(self.L.glia_rd_set_tissue(self.h, member, *[p for (p, _, _) in ps], kind))
+
+    def set_tissue_sinusoidal(self, member=0):
+        self._chk(self.L.glia_rd_set_tissue_sinusoidal(self.h, member))
+
+    def set_coefficients(self, member, kappa, rho):
+        self._chk(self.L.glia_rd_set_coefficients(self.h, member, float(kappa), float(rho)))
+
+    def set_k_sinusoidal(self, member, scale):
+        self._chk(self.L.glia_rd_set_k_sinusoidal(self.h, member, float(scale)))
+
+    def precFactor(self):
+        self._chk(self.L.glia_rd_prec_factor(self.h))
+
+    @property
+    def ctx_dt(self):
+        v = ctypes.c_double()
+        self._chk(self.L.glia_rd_get_ctx_dt(self.h, ctypes.byref(v)))
+        return v.value
+
+    @ctx_dt.setter
+    def ctx_dt(self, dt):
+        self._chk(self.L.glia_rd_set_ctx_dt(self.h, float(dt)))
+
+    def set_basis(self, member, centers, sigma):
+        c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1)
+        self._chk(self.L.glia_rd_set_basis(self.h, member, c.size // 3, c.ctypes.data_as(ctypes.c_void_p), float(sigma)))
+        self.np[member] = c.size // 3
+
+    def set_data(self, member, d1, obs_mask=None, obs_thr=0.0):
+        p, kind, k1 = self._src(d1)
+        q, kind2, k2 = self._src(obs_mask)
+        if obs_mask is not None and kind2 != kind:
+            raise ValueError("d1 and obs_mask must both be host or both device arrays")
+        self._chk(self.L.glia_rd_set_data(self.h, member, p, q, float(obs_thr), kind))
+
+    # ---- operators ---------------------------------------------------------------
+    def phi_apply(self, p):
+        """Phi::apply for every member; p: list of per-member coefficient vectors."""
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.float64).reshape(-1) for v in p]))
+        self._chk(self.L.glia_rd_phi_apply(self.h, flat.ctypes.data_as(ctypes.c_void_p)))
+
+    def phi_apply_transpose(self, which="work"):
+        out = np.zeros(max(sum(self.np), 1))
+        self._chk(self.L.glia_rd_phi_apply_transpose(self.h, _F[which], out.ctypes.data_as(ctypes.c_void_p)))
+        res, off = [], 0
+        for k in self.np:
+            res.append(out[off:off + k].copy())
+            off += k
+        return res
+
+    def diffusion_solve(self, which, tau):
+        self._chk(self.L.glia_rd_diffusion_solve(self.h, _F[which], float(tau)))
+
+    def apply_d(self, which_in, which_out):
+        self._chk(self.L.glia_rd_apply_d(self.h, _F[which_in], _F[which_out]))
+
+    def fft_r2c(self, member, f):
+        f = np.ascontiguousarray(f, dtype=np.float32).reshape(-1)
+        out = np.zeros(self.n[0] * self.n[1] * (self.n[2] // 2 + 1) * 2, dtype=np.float32)
+        self._chk(self.L.glia_rd_fft_r2c(self.h, member, f.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p)))
+        out = out.reshape(self.n[0], self.n[1], self.n[2] // 2 + 1, 2)
+        return out[..., 0] + 1j * out[..., 1]
+
+    def fft_c2r(self, member, F):
+        F = np.ascontiguousarray(F, dtype=np.complex64)
+        buf = np.ascontiguousarray(np.stack([F.real, F.imag], axis=-1), dtype=np.float32).reshape(-1)
+        out = np.zeros(self.ng, dtype=np.float32)
+        self._chk(self.L.glia_rd_fft_c2r(self.h, member, buf.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p)))
+        return out.reshape(self.shape)
+
+    def solve_state(self, linearized=0):
+        self._chk(self.L.glia_rd_solve_state(self.h, int(linearized)))
+
+    def solve_adjoint(self, linearized=1):
+        self._chk(self.L.glia_rd_solve_adjoint(self.h, int(linearized)))
+
+    def _flat_x(self, x):
+        xs = [np.asarray(v, dtype=np.float64).reshape(-1) for v in x]
+        for m, v in enumerate(xs):
+            if v.size != self.np[m] + 2:
+                raise ValueError("member %d: x needs np + 2 = %d entries" % (m, self.np[m] + 2))
+        return np.ascontiguousarray(np.concatenate(xs))
+
+    def _split(self, flat):
+        res, off = [], 0
+        for k in self.np:
+            res.append(flat[off:off + k + 2].copy())
+            off += k + 2
+        return res
+
+    def objective(self, x, mode=_lib.MODE_ALL):
+        flat = self._flat_x(x)
+        J = np.zeros(self.nmembers)
+        self._chk(self.L.glia_rd_objective(self.h, mode, flat.ctypes.data_as(ctypes.c_void_p), J.ctypes.data_as(ctypes.c_void_p)))
+        return J
+
+    def objective_and_gradient(self, x, mode=_lib.MODE_ALL):
+        flat = self._flat_x(x)
+        J = np.zeros(self.nmembers)
+        dJ = np.zeros_like(flat)
+        self._chk(self.L.glia_rd_objective_and_gradient(self.h, mode, flat.ctypes.data_as(ctypes.c_void_p),
+                                                        J.ctypes.data_as(ctypes.c_void_p), dJ.ctypes.data_as(ctypes.c_void_p)))
+        return J, self._split(dJ)
+
+    def stats(self, member=0):
+        s = _lib.Stats()
+        self._chk(self.L.glia_rd_stats(self.h, member, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in _lib.Stats._fields_}
+
+    def reset_stats(self):
+        self._chk(self.L.glia_rd_reset_stats(self.h))
+
+    def sync(self):
+        self._chk(self.L.glia_rd_sync(self.h))
+
+
+class DiffusionSolver:
+    """DiffusionSolver (src/pde/DiffusionSolver.cpp): CN half step by PCG on one field."""
+
+    def __init__(self, solver, field="work"):
+        self.s = solver
+        self.field = field
+        self.ksp_itr_ = 0
+
+    def precFactor(self):
+        self.s.precFactor()
+
+    def solve(self, c, dt):
+        """Solve in place on a host array (numpy) or GPU tensor c; returns c."""
+        for m in range(self.s.nmembers):
+            self.s.set_field(m, self.field, c if self.s.nmembers == 1 else c[m])
+        self.s.diffusion_solve(self.field, dt)
+        self.ksp_itr_ = self.s.stats(0)["last_iterations"]
+        if _is_device(c):
+            self.s.get_field(0, self.field, out=c)
+            return c
+        return self.s.get_field(0, self.field)
+
+
+class PdeOperatorsRD:
+    """PdeOperatorsRD (src/pde/PdeOperators.cpp): solveState / solveAdjoint on the handle's c0."""
+
+    def __init__(self, solver):
+        self.s = solver
+        self.diff_ksp_itr_state_ = 0
+        self.diff_ksp_itr_adj_ = 0
+
+    def solveState(self, linearized=0):
+        self.s.reset_stats()
+        self.s.solve_state(linearized)
+        self.diff_ksp_itr_state_ = self.s.stats(0)["pcg_iterations"]
+
+    def solveAdjoint(self, linearized=1):
+        self.s.reset_stats()
+        self.s.solve_adjoint(linearized)
+        self.diff_ksp_itr_adj_ = self.s.stats(0)["pcg_iterations"]
+
+
+class DerivativeOperatorsRD:
+    """DerivativeOperatorsRD (src/grad/DerivativeOperatorsRD.cpp:7-226); x = [p | kappa | rho]."""
+
+    def __init__(self, solver, mode=_lib.MODE_ALL):
+        self.s = solver
+        self.mode = mode
+
+    def _x(self, x):
+        return [x] if self.s.nmembers == 1 and np.ndim(x[0]) == 0 else x
+
+    def evaluateObjective(self, x):
+        J = self.s.objective(self._x(x), self.mode)
+        return J[0] if self.s.nmembers == 1 else J
+
+    def evaluateGradient(self, x):
+        _, dJ = self.s.objective_and_gradient(self._x(x), self.mode)
+        return dJ[0] if self.s.nmembers == 1 else dJ
+
+    def evaluateObjectiveAndGradient(self, x):
+        J, dJ = self.s.objective_and_gradient(self._x(x), self.mode)
+        return (J[0], dJ[0]) if self.s.nmembers == 1 else (J, dJ)

@@ -1,0 +1,165 @@
+/*
+ * glia_rd.h -- C ABI of the MI355X-native reaction-diffusion forward / adjoint / gradient path.
+ *
+ * This is the drop-in boundary for the reference's RD hot path.  Each entry point names the
+ * reference interface it replaces (file:line in the GLIA sources).  PETSc Vec/Mat/KSP objects
+ * become plain pointers and sizes; a handle owns every device buffer it computes with and copies
+ * its inputs.  Return value: 0 on success, < 0 for argument / state errors, > 0 for a HIP runtime
+ * error code; glia_rd_last_error() gives the message.  All device work of a handle is ordered on
+ * the handle's stream; use one handle per host thread.
+ *
+ * Fields are FP32, n0*n1*n2 values, row-major with z fastest: index x*n1*n2 + y*n2 + z
+ * (reference src/grad/SpectralOperators.cpp:3-17).  mem_kind says whether a pointer is host or
+ * device memory.  A handle holds an ensemble of 'nmembers' members that share the grid and the
+ * time discretization (nmembers = 1 is the plain solve); per-member calls take a member index.
+ */
+#ifndef GLIA_RD_H_
+#define GLIA_RD_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct glia_rd_s* glia_rd_t;
+
+enum { GLIA_RD_HOST = 0, GLIA_RD_DEVICE = 1 };
+
+/* error codes (negative); positive values are hipError_t */
+enum {
+  GLIA_RD_OK = 0,
+  GLIA_RD_ERR_ARG = -2,
+  GLIA_RD_ERR_STATE = -3,
+  GLIA_RD_ERR_UNSUPPORTED = -4,
+  GLIA_RD_ERR_POOL = -5,
+  GLIA_RD_ERR_NOHISTORY = -6
+};
+
+/* fields addressable by set_field / get_field */
+enum {
+  GLIA_RD_FIELD_K = 0,      /* DiffCoef::kxx_ (isotropic k) */
+  GLIA_RD_FIELD_RHO = 1,    /* ReacCoef::rho_vec_ */
+  GLIA_RD_FIELD_WM = 2,     /* MatProp::wm_ */
+  GLIA_RD_FIELD_GM = 3,
+  GLIA_RD_FIELD_CSF = 4,
+  GLIA_RD_FIELD_VT = 5,
+  GLIA_RD_FIELD_FILTER = 6, /* MatProp::filter_ (brain mask) */
+  GLIA_RD_FIELD_OBS = 7,    /* Obs::filter_1_ */
+  GLIA_RD_FIELD_D1 = 8,     /* Data::dt1() */
+  GLIA_RD_FIELD_C0 = 9,     /* Tumor::c_0_ */
+  GLIA_RD_FIELD_CT = 10,    /* Tumor::c_t_ after solveState */
+  GLIA_RD_FIELD_ALPHA0 = 11,/* Tumor::p_0_ after solveAdjoint */
+  GLIA_RD_FIELD_WORK = 12   /* scratch field (half-step state) */
+};
+
+/* evaluation modes of the derivative operators: which parameters the inversion updates */
+enum {
+  GLIA_RD_MODE_P_KAPPA = 0,   /* flag_reaction_inv_ = false: c0 = Phi p; dJ = [g_p | g_kappa | 0] */
+  GLIA_RD_MODE_KAPPA_RHO = 1, /* flag_reaction_inv_ = true: c0 as set; dJ = [0 | g_kappa | g_rho] */
+  GLIA_RD_MODE_ALL = 2        /* p, kappa and rho together: dJ = [g_p | g_kappa | g_rho] */
+};
+
+/* Parameters, TumorParameters, Grid and the DiffusionSolver KSP settings
+   (include/Parameters.h:155-451; src/pde/DiffusionSolver.cpp:25-39) */
+typedef struct glia_rd_config {
+  int n[3];              /* grid; each of n0, n1 in {32,48,64,96,128,192,256}, n2 in {32,64,96,128,192,256} */
+  int nmembers;          /* ensemble members sharing this grid (1 = plain solve) */
+  double dt;             /* TumorParameters::dt_ */
+  int nt;                /* TumorParameters::nt_ */
+  int order;             /* TumorParameters::order_; only 2 (Strang) */
+  double pcg_rtol;       /* KSPSetTolerances rtol (1e-6) */
+  int pcg_maxit;         /* KSPSetTolerances maxits (5000) */
+  int checkpoint_slots;  /* 0: keep every time step (reference); s > 0: binomial checkpointing, s states */
+  int adjoint_store;     /* TumorParameters::adjoint_store_ (keep half-step states; store-all only) */
+  int phi_store;         /* TumorParameters::phi_store_ (Phi normalization semantics) */
+  double k_gm_wm, k_glm_wm, r_gm_wm, r_glm_wm; /* tissue ratios of k and rho */
+  double beta;           /* OptimizerSettings::beta_ (L2 regularization) */
+  double k_scale, rho_scale; /* OptimizerSettings::k_scale_, rho_scale_ */
+  double smoothing_factor;   /* TumorParameters::smoothing_factor_ (Phi smoothing) */
+  void* stream;          /* hipStream_t to use, or NULL for a private stream */
+} glia_rd_config;
+
+typedef struct glia_rd_stats_s {
+  int pcg_iterations;    /* CG iterations since the last reset (diff_ksp_itr_state_ + _adj_) */
+  int cn_solves;         /* DiffusionSolver::solve calls since the last reset */
+  int worst_reason;      /* most negative KSPConvergedReason since the last reset (0 if none) */
+  int last_iterations;   /* DiffusionSolver::ksp_itr_ of the last solve */
+  int last_reason;       /* KSPConvergedReason of the last solve */
+  double kbar;           /* DiffCoef::kxx_avg_ */
+  double prec_coef;      /* 0.25 * dt * kbar baked into the preconditioner */
+  double filter_sum;     /* sum of MatProp::filter_ */
+  double misfit;         /* ||O c(T) - d1||^2 of the last evaluation */
+  double c0_norm2;       /* ||c(0)||^2 */
+} glia_rd_stats_t;
+
+/* defaults of TumorParameters / DiffusionSolver (include/Parameters.h:157-258) */
+void glia_rd_config_init(glia_rd_config* cfg);
+
+/* construction of Tumor, SpectralOperators, PdeOperatorsRD, DerivativeOperatorsRD
+   (SolverInterface::initializeOperators, src/SolverInterface.cpp:929-965; PdeOperatorsRD ctor,
+   src/pde/PdeOperators.cpp:17-54; DiffusionSolver ctor, src/pde/DiffusionSolver.cpp:5-45) */
+int glia_rd_create(const glia_rd_config* cfg, glia_rd_t* out);
+int glia_rd_destroy(glia_rd_t h);
+const char* glia_rd_last_error(glia_rd_t h);
+int glia_rd_sync(glia_rd_t h);
+int glia_rd_num_members(glia_rd_t h);
+
+/* MatProp::setValuesCustom + SolverInterface::updateTumorCoefficients
+   (src/mat/MatProp.cpp:134-203; src/SolverInterface.cpp:986-1014); null fields are zero */
+int glia_rd_set_tissue(glia_rd_t h, int member, const float* wm, const float* gm, const float* csf, const float* vt,
+                       int mem_kind);
+/* MatProp::setValuesSinusoidal (src/mat/MatProp.cpp:34-68) */
+int glia_rd_set_tissue_sinusoidal(glia_rd_t h, int member);
+/* DiffCoef::setValues + ReacCoef::setValues (src/mat/DiffCoef.cpp:77-132; src/mat/ReacCoef.cpp:13-38) */
+int glia_rd_set_coefficients(glia_rd_t h, int member, double kappa, double rho);
+/* DiffCoef::setValuesSinusoidal (src/mat/DiffCoef.cpp:134-189) */
+int glia_rd_set_k_sinusoidal(glia_rd_t h, int member, double scale);
+/* DiffusionSolver::precFactor (src/pde/DiffusionSolver.cpp:119-180), all members */
+int glia_rd_prec_factor(glia_rd_t h);
+/* DiffusionSolver Ctx::dt_ (src/pde/DiffusionSolver.cpp:11,226) */
+int glia_rd_set_ctx_dt(glia_rd_t h, double dt);
+int glia_rd_get_ctx_dt(glia_rd_t h, double* dt);
+/* Phi::setGaussians(centers) + Phi::setValues (src/mat/Phi.cpp:32-144): np centers (x,y,z in [0,2pi)) */
+int glia_rd_set_basis(glia_rd_t h, int member, int np, const double* centers, double sigma);
+int glia_rd_basis_size(glia_rd_t h, int member);
+/* Data::set + Obs::setDefaultFilter / setCustomFilter (src/mat/Obs.cpp:26-69):
+   obs_mask NULL -> O = 1[d1 > obs_thr] */
+int glia_rd_set_data(glia_rd_t h, int member, const float* d1, const float* obs_mask, double obs_thr, int mem_kind);
+int glia_rd_set_field(glia_rd_t h, int member, int which, const float* src, int mem_kind);
+int glia_rd_get_field(glia_rd_t h, int member, int which, float* dst, int mem_kind);
+
+/* Phi::apply (src/mat/Phi.cpp:313-377): c0 = Phi p for every member; p holds np(m) values per member */
+int glia_rd_phi_apply(glia_rd_t h, const double* p);
+/* Phi::applyTranspose (src/mat/Phi.cpp:379-439) of field 'which'; out holds np(m) values per member */
+int glia_rd_phi_apply_transpose(glia_rd_t h, int which, double* out);
+/* DiffusionSolver::solve (src/pde/DiffusionSolver.cpp:217-250) on field 'which', in place, all members */
+int glia_rd_diffusion_solve(glia_rd_t h, int which, double tau);
+/* DiffCoef::applyD (src/mat/DiffCoef.cpp:249-271): out = div(k grad in), all members */
+int glia_rd_apply_d(glia_rd_t h, int which_in, int which_out);
+/* SpectralOperators::executeFFTR2C / executeFFTC2R (src/grad/SpectralOperators.cpp:68-98) for one
+   member, host arrays; spectrum in the reference layout n0*n1*(n2/2+1) interleaved complex.
+   c2r returns the unnormalized inverse (N^3 times the field for a forward spectrum). */
+int glia_rd_fft_r2c(glia_rd_t h, int member, const float* f, float* fhat);
+int glia_rd_fft_c2r(glia_rd_t h, int member, const float* fhat, float* f);
+
+/* PdeOperatorsRD::solveState (src/pde/PdeOperators.cpp:235-316) from c0, keeping the history the
+   checkpoint policy asks for; linearized must be 0 */
+int glia_rd_solve_state(glia_rd_t h, int linearized);
+/* PdeOperatorsRD::solveAdjoint (src/pde/PdeOperators.cpp:372-420) with the final condition
+   -O^T(O c(T) - d1) of DerivativeOperatorsRD (src/grad/DerivativeOperatorsRD.cpp:154-162) */
+int glia_rd_solve_adjoint(glia_rd_t h, int linearized);
+
+/* DerivativeOperatorsRD::evaluateObjective / evaluateGradient / evaluateObjectiveAndGradient
+   (src/grad/DerivativeOperatorsRD.cpp:7-226).  x holds [p (np) | kappa | rho] per member,
+   concatenated over members; J one value per member; dJ like x. */
+int glia_rd_objective(glia_rd_t h, int mode, const double* x, double* J);
+int glia_rd_gradient(glia_rd_t h, int mode, const double* x, double* dJ);
+int glia_rd_objective_and_gradient(glia_rd_t h, int mode, const double* x, double* J, double* dJ);
+
+int glia_rd_stats(glia_rd_t h, int member, glia_rd_stats_t* out);
+int glia_rd_reset_stats(glia_rd_t h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLIA_RD_H_ */

@@ -1,0 +1,1017 @@
+/*
+ * glia_oracle.c -- FP64 CPU restatement of GLIA's reaction-diffusion forward,
+ * adjoint and gradient path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: it is loaded by tests/,
+ * by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py, and by
+ * nothing in the product.  The product (glia_amd/, libglia_rd.so) never links,
+ * imports or calls it.
+ *
+ * It restates the reference operation by operation, in the reference's own
+ * real-space formulation (12 FFTs per CG iteration, PETSc KSPCG stopping
+ * rules), so that its results are what the reference CPU path computes, in
+ * FP64.  Every function cites the reference file:line it follows.
+ *
+ * Layout (reference src/grad/SpectralOperators.cpp:3-17): a real field is
+ * n0 x n1 x n2, row-major with index x*n1*n2 + y*n2 + z; a spectrum is
+ * n0 x n1 x (n2/2+1) interleaved complex, unnormalized forward transform.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+/* ------------------------------------------------------------------------ */
+/* 1D mixed-radix Stockham FFT (radix 4, 2, 3, 5) in FP64.                   */
+/* The reference uses cuFFT / AccFFT+FFTW for this (src/grad/                */
+/* SpectralOperators.cpp:43-50,68-98); any exact DFT is a valid restatement. */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int L;
+  int nr;
+  int radix[40];
+  cplx *tw; /* tw[m] = exp(-2 pi i m / L) */
+} fplan;
+
+static int plan_init(fplan *p, int L) {
+  p->L = L;
+  p->nr = 0;
+  int m = L;
+  while (m % 4 == 0) { p->radix[p->nr++] = 4; m /= 4; }
+  while (m % 2 == 0) { p->radix[p->nr++] = 2; m /= 2; }
+  while (m % 3 == 0) { p->radix[p->nr++] = 3; m /= 3; }
+  while (m % 5 == 0) { p->radix[p->nr++] = 5; m /= 5; }
+  if (m != 1) return -1;
+  p->tw = (cplx *)malloc(sizeof(cplx) * (size_t)L);
+  for (int i = 0; i < L; i++) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)i / (long double)L;
+    p->tw[i] = (double)cosl(a) + I * (double)sinl(a);
+  }
+  return 0;
+}
+
+static void plan_free(fplan *p) { free(p->tw); p->tw = NULL; }
+
+/* small DFT of length R in natural order, sign = -1 forward, +1 inverse */
+static void dft_small(cplx *v, int R, int sign) {
+  cplx out[5];
+  static const double s3 = 0.86602540378443864676372317075294;
+  if (R == 2) {
+    cplx a = v[0], b = v[1];
+    v[0] = a + b; v[1] = a - b;
+    return;
+  }
+  if (R == 4) {
+    cplx a = v[0] + v[2], b = v[0] - v[2], c = v[1] + v[3], d = (v[1] - v[3]) * (sign < 0 ? -I : I);
+    v[0] = a + c; v[2] = a - c; v[1] = b + d; v[3] = b - d;
+    return;
+  }
+  if (R == 3) {
+    cplx w1 = -0.5 + (sign < 0 ? -s3 : s3) * I;
+    cplx w2 = conj(w1);
+    out[0] = v[0] + v[1] + v[2];
+    out[1] = v[0] + w1 * v[1] + w2 * v[2];
+    out[2] = v[0] + w2 * v[1] + w1 * v[2];
+    v[0] = out[0]; v[1] = out[1]; v[2] = out[2];
+    return;
+  }
+  /* R == 5: direct */
+  for (int q = 0; q < R; q++) {
+    cplx s = 0;
+    for (int r = 0; r < R; r++) {
+      double a = sign * 2.0 * M_PI * (double)((r * q) % R) / (double)R;
+      s += v[r] * (cos(a) + I * sin(a));
+    }
+    out[q] = s;
+  }
+  for (int q = 0; q < R; q++) v[q] = out[q];
+}
+
+/* in-place transform of a[0..L) using tmp[0..L) as ping-pong storage */
+static void fft_exec(const fplan *p, cplx *a, cplx *tmp, int sign) {
+  const int L = p->L;
+  cplx *src = a, *dst = tmp;
+  int Ns = 1;
+  for (int s = 0; s < p->nr; s++) {
+    const int R = p->radix[s];
+    const int LR = L / R;
+    const int stride = L / (Ns * R);
+    for (int j = 0; j < LR; j++) {
+      cplx v[5];
+      const int k = j % Ns;
+      for (int r = 0; r < R; r++) {
+        cplx x = src[j + r * LR];
+        if (k) {
+          cplx w = p->tw[(k * r * stride) % L];
+          x *= (sign < 0) ? w : conj(w);
+        }
+        v[r] = x;
+      }
+      dft_small(v, R, sign);
+      const int idxD = (j / Ns) * Ns * R + k;
+      for (int r = 0; r < R; r++) dst[idxD + r * Ns] = v[r];
+    }
+    cplx *t = src; src = dst; dst = t;
+    Ns *= R;
+  }
+  if (src != a) memcpy(a, src, sizeof(cplx) * (size_t)L);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Oracle context: grid, coefficients, solver state, history.               */
+/* Mirrors Parameters/Grid (include/Parameters.h:374-451), Tumor            */
+/* (src/mat/Tumor.cpp:3-57), DiffCoef/ReacCoef/Obs/Phi/MatProp state.       */
+/* ------------------------------------------------------------------------ */
+typedef struct orc {
+  int n[3];
+  int64_t ng;   /* n0*n1*n2 */
+  int n2h;      /* n2/2+1 */
+  int64_t nh;   /* n0*n1*n2h complex entries */
+  double h[3];
+  double lebesgue;
+  fplan pl[3];
+  int nthreads;
+
+  /* MatProp (src/mat/MatProp.cpp:134-203) */
+  double *wm, *gm, *csf, *vt, *bg, *filter;
+  double filter_sum;     /* VecSum(filter) used by DiffCoef::setValues */
+  int k_filter_is_ng;    /* setValuesSinusoidal divides by ng (DiffCoef.cpp:170) */
+
+  /* DiffCoef (isotropic: kxx=kyy=kzz, off-diagonals zero; DiffCoef.cpp:77-132) */
+  double *k;
+  double kavg;
+  double k_scale, k_gm_wm, k_glm_wm;
+  /* ReacCoef (src/mat/ReacCoef.cpp:13-38) */
+  double *rho;
+  double rho_scale, r_gm_wm, r_glm_wm;
+
+  /* DiffusionSolver (src/pde/DiffusionSolver.cpp:5-45) */
+  double ctx_dt;    /* Ctx::dt_, params dt at construction, last tau after solve */
+  double prec_dt;   /* dt value baked into the current precfactor_ */
+  double *prec;     /* precfactor_ (nh real factors) */
+  double rtol, atol, dtol;
+  int maxit;
+  int ksp_itr;      /* ksp_itr_ of last solve */
+  int ksp_reason;   /* >0 converged, <0 diverged, 0 none */
+  int64_t ksp_total_calls;
+
+  /* time stepping (TumorParameters::dt_, nt_) */
+  double dt;
+  int nt;
+  int adjoint_store;
+
+  /* fields of Tumor */
+  double *c_0, *c_t, *p_0, *p_t;
+  /* history (PdeOperatorsRD ctor, src/pde/PdeOperators.cpp:17-54) */
+  double **c_hist, **p_hist, **c_half;
+  int hist_nt;
+  int diff_ksp_itr_state, diff_ksp_itr_adj;
+
+  /* Phi (src/mat/Phi.cpp) */
+  int np;
+  double *centers;
+  double phi_sigma;
+  double smoothing_factor;
+  int phi_store;
+  double **phi;      /* stored bases (normalized if phi_store, raw otherwise) */
+  double phi_max_all;
+
+  /* Obs (src/mat/Obs.cpp) and data */
+  double *obs;
+  double *d1;
+
+  /* optimizer settings (OptimizerSettings) */
+  double beta;
+  double opt_k_scale, opt_rho_scale;
+
+  /* work: w[0] operatorA/solve temp, w[1..6] applyD gradient/flux fields, w[7..9] CG r, z and the
+     KSPConvergedDefault B*b vector, w[10] smoother filter, w[11] divergence accumulator */
+  double *w[12];
+  double *cg_p, *cg_w, *rhs, *adj_temp;
+  cplx *xhat, *what;
+} orc_t;
+
+static double *dalloc(int64_t n) {
+  double *p = (double *)calloc((size_t)n, sizeof(double));
+  return p;
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+orc_t *orc_create(int n0, int n1, int n2) {
+  orc_t *o = (orc_t *)calloc(1, sizeof(orc_t));
+  o->n[0] = n0; o->n[1] = n1; o->n[2] = n2;
+  o->ng = (int64_t)n0 * n1 * n2;
+  o->n2h = n2 / 2 + 1;
+  o->nh = (int64_t)n0 * n1 * o->n2h;
+  for (int d = 0; d < 3; d++) {
+    o->h[d] = 2.0 * M_PI / o->n[d];
+    if (plan_init(&o->pl[d], o->n[d]) != 0) { free(o); return NULL; }
+  }
+  o->lebesgue = o->h[0] * o->h[1] * o->h[2];
+  o->wm = dalloc(o->ng); o->gm = dalloc(o->ng); o->csf = dalloc(o->ng);
+  o->vt = dalloc(o->ng); o->bg = dalloc(o->ng); o->filter = dalloc(o->ng);
+  o->k = dalloc(o->ng); o->rho = dalloc(o->ng);
+  o->prec = dalloc(o->nh);
+  o->c_0 = dalloc(o->ng); o->c_t = dalloc(o->ng); o->p_0 = dalloc(o->ng); o->p_t = dalloc(o->ng);
+  o->obs = dalloc(o->ng); o->d1 = dalloc(o->ng);
+  for (int i = 0; i < 12; i++) o->w[i] = dalloc(o->ng);
+  o->cg_p = dalloc(o->ng); o->cg_w = dalloc(o->ng); o->rhs = dalloc(o->ng); o->adj_temp = dalloc(o->ng);
+  o->xhat = (cplx *)calloc((size_t)o->nh, sizeof(cplx));
+  o->what = (cplx *)calloc((size_t)o->nh, sizeof(cplx));
+  /* MatProp::setValues defaults (src/mat/MatProp.cpp:20-33): wm = 1, filter = 1 */
+  for (int64_t i = 0; i < o->ng; i++) { o->wm[i] = 1.0; o->filter[i] = 1.0; o->obs[i] = 1.0; }
+  o->filter_sum = (double)o->ng;
+  /* defaults: TumorParameters (include/Parameters.h:157-258), DiffusionSolver ctor tolerances
+     (src/pde/DiffusionSolver.cpp:26), PETSc KSP defaults atol = 1e-50, dtol = 1e5 */
+  o->dt = 0.5; o->nt = 1;
+  o->ctx_dt = o->dt; o->prec_dt = o->dt;
+  o->rtol = 1e-6; o->atol = 1e-50; o->dtol = 1e5; o->maxit = 5000;
+  o->k_scale = 1e-2; o->k_gm_wm = 1.0 / 5.0; o->k_glm_wm = 3.0 / 5.0;
+  o->phi_sigma = 2 * M_PI / 64; o->smoothing_factor = 1.0;
+  o->opt_k_scale = 1.0; o->opt_rho_scale = 1.0;
+  return o;
+}
+
+static void free_hist(orc_t *o) {
+  if (o->c_hist) {
+    for (int i = 0; i <= o->hist_nt; i++) { free(o->c_hist[i]); free(o->p_hist[i]); }
+    free(o->c_hist); free(o->p_hist);
+  }
+  if (o->c_half) {
+    for (int i = 0; i < o->hist_nt; i++) free(o->c_half[i]);
+    free(o->c_half);
+  }
+  o->c_hist = o->p_hist = o->c_half = NULL;
+}
+
+static void free_phi(orc_t *o) {
+  if (o->phi) {
+    for (int i = 0; i < o->np; i++) free(o->phi[i]);
+    free(o->phi);
+  }
+  o->phi = NULL;
+  free(o->centers); o->centers = NULL;
+  o->np = 0;
+}
+
+void orc_destroy(orc_t *o) {
+  if (!o) return;
+  free_hist(o);
+  free_phi(o);
+  for (int d = 0; d < 3; d++) plan_free(&o->pl[d]);
+  free(o->wm); free(o->gm); free(o->csf); free(o->vt); free(o->bg); free(o->filter);
+  free(o->k); free(o->rho); free(o->prec);
+  free(o->c_0); free(o->c_t); free(o->p_0); free(o->p_t); free(o->obs); free(o->d1);
+  for (int i = 0; i < 12; i++) free(o->w[i]);
+  free(o->cg_p); free(o->cg_w); free(o->rhs); free(o->adj_temp);
+  free(o->xhat); free(o->what);
+  free(o);
+}
+
+/* ------------------------------------------------------------------------ */
+/* 3D R2C / C2R (cufftExecR2C / cufftExecC2R, unnormalized;                 */
+/* src/grad/SpectralOperators.cpp:68-98).                                   */
+/* ------------------------------------------------------------------------ */
+void orc_r2c(orc_t *o, const double *f, cplx *F) {
+  const int n0 = o->n[0], n1 = o->n[1], n2 = o->n[2], n2h = o->n2h;
+#pragma omp parallel
+  {
+    int Lm = n0 > n1 ? n0 : n1; if (n2 > Lm) Lm = n2;
+    cplx *a = (cplx *)malloc(sizeof(cplx) * Lm), *t = (cplx *)malloc(sizeof(cplx) * Lm);
+#pragma omp for schedule(static)
+    for (int64_t l = 0; l < (int64_t)n0 * n1; l++) { /* z lines */
+      const double *src = f + l * n2;
+      for (int z = 0; z < n2; z++) a[z] = src[z];
+      fft_exec(&o->pl[2], a, t, -1);
+      for (int kz = 0; kz < n2h; kz++) F[l * n2h + kz] = a[kz];
+    }
+#pragma omp for schedule(static)
+    for (int64_t l = 0; l < (int64_t)n0 * n2h; l++) { /* y lines */
+      int x = (int)(l / n2h), kz = (int)(l % n2h);
+      cplx *base = F + (int64_t)x * n1 * n2h + kz;
+      for (int y = 0; y < n1; y++) a[y] = base[(int64_t)y * n2h];
+      fft_exec(&o->pl[1], a, t, -1);
+      for (int y = 0; y < n1; y++) base[(int64_t)y * n2h] = a[y];
+    }
+#pragma omp for schedule(static)
+    for (int64_t l = 0; l < (int64_t)n1 * n2h; l++) { /* x lines */
+      cplx *base = F + l;
+      for (int x = 0; x < n0; x++) a[x] = base[(int64_t)x * n1 * n2h];
+      fft_exec(&o->pl[0], a, t, -1);
+      for (int x = 0; x < n0; x++) base[(int64_t)x * n1 * n2h] = a[x];
+    }
+    free(a); free(t);
+  }
+}
+
+/* F is destroyed (used as scratch), as cuFFT C2R is allowed to do. */
+void orc_c2r(orc_t *o, cplx *F, double *f) {
+  const int n0 = o->n[0], n1 = o->n[1], n2 = o->n[2], n2h = o->n2h;
+#pragma omp parallel
+  {
+    int Lm = n0 > n1 ? n0 : n1; if (n2 > Lm) Lm = n2;
+    cplx *a = (cplx *)malloc(sizeof(cplx) * Lm), *t = (cplx *)malloc(sizeof(cplx) * Lm);
+#pragma omp for schedule(static)
+    for (int64_t l = 0; l < (int64_t)n1 * n2h; l++) {
+      cplx *base = F + l;
+      for (int x = 0; x < n0; x++) a[x] = base[(int64_t)x * n1 * n2h];
+      fft_exec(&o->pl[0], a, t, +1);
+      for (int x = 0; x < n0; x++) base[(int64_t)x * n1 * n2h] = a[x];
+    }
+#pragma omp for schedule(static)
+    for (int64_t l = 0; l < (int64_t)n0 * n2h; l++) {
+      int x = (int)(l / n2h), kz = (int)(l % n2h);
+      cplx *base = F + (int64_t)x * n1 * n2h + kz;
+      for (int y = 0; y < n1; y++) a[y] = base[(int64_t)y * n2h];
+      fft_exec(&o->pl[1], a, t, +1);
+      for (int y = 0; y < n1; y++) base[(int64_t)y * n2h] = a[y];
+    }
+#pragma omp for schedule(static)
+    for (int64_t l = 0; l < (int64_t)n0 * n1; l++) {
+      const cplx *src = F + l * n2h;
+      for (int kz = 0; kz < n2h; kz++) a[kz] = src[kz];
+      for (int kz = n2h; kz < n2; kz++) a[kz] = conj(src[n2 - kz]);
+      fft_exec(&o->pl[2], a, t, +1);
+      double *dst = f + l * n2;
+      for (int z = 0; z < n2; z++) dst[z] = creal(a[z]);
+    }
+    free(a); free(t);
+  }
+}
+
+/* wave number with the Nyquist mode zeroed (src/cuda/SpectralOperators.cu:63-72) */
+static inline double wnum(int64_t K, int n) {
+  double w = (double)K;
+  if ((double)K > n / 2.0) w -= n;
+  if ((double)K == n / 2.0) w = 0;
+  return w;
+}
+
+/* w_f = (i * w_d / N^3) * f  (multiply{X,Y,Z}WaveNumber, src/cuda/SpectralOperators.cu:54-127) */
+static void mult_wave(orc_t *o, int d, const cplx *f, cplx *wf) {
+  const int n0 = o->n[0], n1 = o->n[1], n2h = o->n2h;
+  const double factor = 1.0 / ((double)o->n[0] * o->n[1] * o->n[2]);
+#pragma omp parallel for schedule(static)
+  for (int64_t idx = 0; idx < o->nh; idx++) {
+    int64_t x = idx / ((int64_t)n1 * n2h), y = (idx / n2h) % n1, z = idx % n2h;
+    int64_t K = (d == 0) ? x : (d == 1) ? y : z;
+    double w = wnum(K, o->n[d]);
+    double re = creal(f[idx]), im = cimag(f[idx]);
+    wf[idx] = (-factor * w * im) + I * (factor * w * re);
+  }
+  (void)n0;
+}
+
+/* SpectralOperators::computeGradient (src/grad/SpectralOperators.cpp:100-177) */
+void orc_grad(orc_t *o, const double *x, double *gx, double *gy, double *gz) {
+  orc_r2c(o, x, o->xhat);
+  double *out[3] = {gx, gy, gz};
+  for (int d = 0; d < 3; d++) {
+    mult_wave(o, d, o->xhat, o->what);
+    orc_c2r(o, o->what, out[d]);
+  }
+}
+
+/* SpectralOperators::computeDivergence (src/grad/SpectralOperators.cpp:179-261):
+   div = dx-term, then += dz-term, then += dy-term (cublas axpy order). */
+void orc_div(orc_t *o, const double *dx, const double *dy, const double *dz, double *div) {
+  double *d1 = o->w[11];
+  orc_r2c(o, dx, o->xhat);
+  mult_wave(o, 0, o->xhat, o->what);
+  orc_c2r(o, o->what, div);
+  orc_r2c(o, dz, o->xhat);
+  mult_wave(o, 2, o->xhat, o->what);
+  orc_c2r(o, o->what, d1);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < o->ng; i++) div[i] += d1[i];
+  orc_r2c(o, dy, o->xhat);
+  mult_wave(o, 1, o->xhat, o->what);
+  orc_c2r(o, o->what, d1);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < o->ng; i++) div[i] += d1[i];
+}
+
+/* SpectralOperators::weierstrassSmoother (src/grad/SpectralOperators.cpp:295-381) with the
+   8-image periodic Gaussian filter of computeWeierstrassFilter (src/cuda/SpectralOperators.cu:12-51). */
+void orc_smooth(orc_t *o, const double *c, double *wc, double sigma) {
+  if (sigma == 0) { if (wc != c) memcpy(wc, c, sizeof(double) * o->ng); return; }
+  const int n0 = o->n[0], n1 = o->n[1], n2 = o->n[2];
+  const double twopi = 2.0 * M_PI, hx = o->h[0], hy = o->h[1], hz = o->h[2];
+  const double factor = 1.0 / ((double)n0 * n1 * n2);
+  double *f = o->w[10];
+  double sum = 0;
+#pragma omp parallel for schedule(static) reduction(+ : sum)
+  for (int64_t idx = 0; idx < o->ng; idx++) {
+    int64_t i = idx / ((int64_t)n1 * n2), j = (idx / n2) % n1, k = idx % n2;
+    double X = i * hx, Xp = X - twopi, Y = j * hy, Yp = Y - twopi, Z = k * hz, Zp = Z - twopi;
+    double s2 = sigma * sigma * 2.0;
+    double v = exp((-X * X - Y * Y - Z * Z) / s2) + exp((-Xp * Xp - Yp * Yp - Zp * Zp) / s2);
+    v += exp((-Xp * Xp - Y * Y - Z * Z) / s2) + exp((-X * X - Yp * Yp - Z * Z) / s2);
+    v += exp((-X * X - Y * Y - Zp * Zp) / s2) + exp((-Xp * Xp - Yp * Yp - Z * Z) / s2);
+    v += exp((-Xp * Xp - Y * Y - Zp * Zp) / s2) + exp((-X * X - Yp * Yp - Zp * Zp) / s2);
+    if (v != v) v = 0.;
+    f[idx] = v;
+    sum += v;
+  }
+  double nf = 1. / (sum * hx * hy * hz);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < o->ng; i++) f[i] *= nf;
+  orc_r2c(o, f, o->xhat);
+  orc_r2c(o, c, o->what);
+  const double alp = factor * hx * hy * hz;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < o->nh; i++) o->xhat[i] *= o->what[i] * alp;
+  orc_c2r(o, o->xhat, wc);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Material properties and coefficients                                     */
+/* ------------------------------------------------------------------------ */
+
+/* MatProp::setValuesCustom (src/mat/MatProp.cpp:134-203); null inputs are zero fields. */
+void orc_set_tissue(orc_t *o, const double *wm, const double *gm, const double *csf, const double *vt, const double *bg) {
+  double fs = 0;
+#pragma omp parallel for schedule(static) reduction(+ : fs)
+  for (int64_t i = 0; i < o->ng; i++) {
+    double w = wm ? wm[i] : 0, g = gm ? gm[i] : 0, c = csf ? csf[i] : 0, v = vt ? vt[i] : 0;
+    (void)bg;
+    w = (w <= 0.) ? 0. : w; g = (g <= 0.) ? 0. : g; v = (v <= 0.) ? 0. : v; c = (c <= 0.) ? 0. : c;
+    o->wm[i] = w; o->gm[i] = g; o->csf[i] = c; o->vt[i] = v;
+    o->bg[i] = -((g + w + v + c) - 1.0);
+    o->filter[i] = ((w > 0.1 || g > 0.1) && v < 0.8) ? 1.0 : 0.0;
+    fs += o->filter[i];
+  }
+  o->filter_sum = fs;
+  o->k_filter_is_ng = 0;
+}
+
+/* MatProp::setValuesSinusoidal (src/mat/MatProp.cpp:34-68) */
+void orc_set_tissue_sinusoidal(orc_t *o) {
+  const int n0 = o->n[0], n1 = o->n[1], n2 = o->n[2];
+  const double freq = 4;
+  for (int64_t idx = 0; idx < o->ng; idx++) {
+    int64_t X = idx / ((int64_t)n1 * n2), Y = (idx / n2) % n1, Z = idx % n2;
+    o->wm[idx] = 0.5 + 0.5 * sin(freq * 2.0 * M_PI / n0 * X) * sin(freq * 2.0 * M_PI / n1 * Y) * sin(freq * 2.0 * M_PI / n2 * Z);
+    o->gm[idx] = 1 - o->wm[idx];
+    o->vt[idx] = 0; o->csf[idx] = 0; o->bg[idx] = 0; o->filter[idx] = 1;
+  }
+  o->filter_sum = (double)o->ng;
+  o->k_filter_is_ng = 0;
+}
+
+/* DiffCoef::setValues (src/mat/DiffCoef.cpp:77-132): isotropic k, mask-averaged kbar */
+void orc_set_k(orc_t *o, double k_scale, double k_gm_wm, double k_glm_wm) {
+  o->k_scale = k_scale; o->k_gm_wm = k_gm_wm; o->k_glm_wm = k_glm_wm;
+  double dk_gm = k_scale * k_gm_wm, dk_wm = k_scale, dk_glm = k_scale * k_glm_wm;
+  dk_gm = (dk_gm <= 0) ? 0.0 : dk_gm;
+  dk_glm = (dk_glm <= 0) ? 0.0 : dk_glm;
+  double s = 0;
+#pragma omp parallel for schedule(static) reduction(+ : s)
+  for (int64_t i = 0; i < o->ng; i++) {
+    double v = 0.0;
+    v += dk_gm * o->gm[i];
+    v += dk_wm * o->wm[i];
+    v += dk_glm * o->csf[i];
+    o->k[i] = v;
+    s += v;
+  }
+  o->kavg = s * (1.0 / o->filter_sum);
+}
+
+/* DiffCoef::setValuesSinusoidal (src/mat/DiffCoef.cpp:134-189): kbar = sum(k)/ng */
+void orc_set_k_sinusoidal(orc_t *o, double scale) {
+  const int n0 = o->n[0], n1 = o->n[1], n2 = o->n[2];
+  const double freq = 4;
+  o->k_scale = scale;
+  double s = 0;
+  for (int64_t idx = 0; idx < o->ng; idx++) {
+    int64_t X = idx / ((int64_t)n1 * n2), Y = (idx / n2) % n1, Z = idx % n2;
+    o->k[idx] = scale * (0.5 + 0.5 * sin(freq * 2.0 * M_PI / n0 * X) * sin(freq * 2.0 * M_PI / n1 * Y) * sin(freq * 2.0 * M_PI / n2 * Z));
+    s += o->k[idx];
+  }
+  o->kavg = s * (1.0 / (double)o->ng);
+}
+
+/* ReacCoef::setValues (src/mat/ReacCoef.cpp:13-38) */
+void orc_set_rho(orc_t *o, double rho_scale, double r_gm_wm, double r_glm_wm) {
+  o->rho_scale = rho_scale; o->r_gm_wm = r_gm_wm; o->r_glm_wm = r_glm_wm;
+  double dr_gm = rho_scale * r_gm_wm, dr_wm = rho_scale, dr_glm = rho_scale * r_glm_wm;
+  dr_gm = (dr_gm <= 0) ? 0.0 : dr_gm;
+  dr_glm = (dr_glm <= 0) ? 0.0 : dr_glm;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < o->ng; i++) {
+    double v = 0.0;
+    v += dr_gm * o->gm[i];
+    v += dr_wm * o->wm[i];
+    v += dr_glm * o->csf[i];
+    o->rho[i] = v;
+  }
+}
+
+void orc_get_fields(orc_t *o, double *k, double *rho, double *filter, double *kavg) {
+  if (k) memcpy(k, o->k, sizeof(double) * o->ng);
+  if (rho) memcpy(rho, o->rho, sizeof(double) * o->ng);
+  if (filter) memcpy(filter, o->filter, sizeof(double) * o->ng);
+  if (kavg) *kavg = o->kavg;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Diffusion operator, preconditioner, CN + PETSc-KSPCG                      */
+/* ------------------------------------------------------------------------ */
+
+/* DiffCoef::applyD (src/mat/DiffCoef.cpp:249-271) with applyK (:206-247) for isotropic k */
+void orc_apply_d(orc_t *o, const double *c, double *dc) {
+  double *g0 = o->w[4], *g1 = o->w[5], *g2 = o->w[6];
+  double *t1 = o->w[1], *t2 = o->w[2], *t3 = o->w[3];
+  orc_grad(o, c, g0, g1, g2);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < o->ng; i++) {
+    t1[i] = o->k[i] * g0[i];
+    t2[i] = o->k[i] * g1[i];
+    t3[i] = o->k[i] * g2[i];
+  }
+  orc_div(o, t1, t2, t3, dc);
+}
+
+/* DiffusionSolver::precFactor (src/pde/DiffusionSolver.cpp:119-180) */
+void orc_prec_factor(orc_t *o) {
+  const int n1 = o->n[1], n2h = o->n2h;
+  const double factor = 1.0 / ((double)o->n[0] * o->n[1] * o->n[2]);
+  const double dt = o->ctx_dt, kk = o->kavg;
+  o->prec_dt = dt;
+#pragma omp parallel for schedule(static)
+  for (int64_t idx = 0; idx < o->nh; idx++) {
+    int64_t x = idx / ((int64_t)n1 * n2h), y = (idx / n2h) % n1, z = idx % n2h;
+    double wx = wnum(x, o->n[0]), wy = wnum(y, o->n[1]), wz = wnum(z, o->n[2]);
+    double v = 1 + 0.25 * dt * (kk * wx * wx + kk * wy * wy + kk * wz * wz);
+    o->prec[idx] = (v == 0) ? factor : factor / v;
+  }
+}
+
+/* applyPC (src/pde/DiffusionSolver.cpp:182-215): y = C2R(prec .* R2C(x)) */
+static void apply_pc(orc_t *o, const double *x, double *y) {
+  orc_r2c(o, x, o->xhat);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < o->nh; i++) o->xhat[i] *= o->prec[i];
+  orc_c2r(o, o->xhat, y);
+}
+
+/* operatorA (src/pde/DiffusionSolver.cpp:97-117): y = x - (ctx_dt/2) * D x */
+static void operator_a(orc_t *o, const double *x, double *y) {
+  double *t = o->w[0];
+  orc_apply_d(o, x, t);
+  const double alph = -1.0 / 2.0 * o->ctx_dt;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < o->ng; i++) y[i] = x[i] + alph * t[i];
+}
+
+static double vdot(orc_t *o, const double *a, const double *b) {
+  double s = 0;
+#pragma omp parallel for schedule(static) reduction(+ : s)
+  for (int64_t i = 0; i < o->ng; i++) s += a[i] * b[i];
+  return s;
+}
+
+enum { KSP_CONVERGED_RTOL = 2, KSP_CONVERGED_ATOL = 3, KSP_DIVERGED_ITS = -3, KSP_DIVERGED_DTOL = -4,
+       KSP_DIVERGED_INDEFINITE_PC = -8, KSP_DIVERGED_NANORINF = -9, KSP_DIVERGED_INDEFINITE_MAT = -10 };
+
+/* KSPConvergedDefault (PETSc 3.11, src/ksp/ksp/interface/iterativ.c), default preconditioned norm,
+   nonzero initial guess: rnorm0 = ||B b||, ttol = max(rtol*rnorm0, atol). */
+static int ksp_converged(orc_t *o, int n, double rnorm, double *rnorm0, double *ttol, const double *b) {
+  if (n == 0) {
+    double *z = o->w[9];
+    apply_pc(o, b, z);
+    double snorm = sqrt(vdot(o, z, z));
+    if (snorm == 0) snorm = rnorm;
+    *rnorm0 = snorm;
+    *ttol = fmax(o->rtol * snorm, o->atol);
+  }
+  if (isnan(rnorm) || isinf(rnorm)) return KSP_DIVERGED_NANORINF;
+  if (rnorm <= *ttol) return (rnorm < o->atol) ? KSP_CONVERGED_ATOL : KSP_CONVERGED_RTOL;
+  if (rnorm >= o->dtol * (*rnorm0)) return KSP_DIVERGED_DTOL;
+  return 0;
+}
+
+/* KSPSolve_CG (PETSc 3.11, src/ksp/ksp/impls/cg/cg.c) for the default KSP_NORM_PRECONDITIONED,
+   nonzero initial guess (KSPSetInitialGuessNonzero, src/pde/DiffusionSolver.cpp:28). */
+static int ksp_cg(orc_t *o, const double *b, double *x, int *reason_out) {
+  double *r = o->w[7], *z = o->w[8], *p = o->cg_p, *wv = o->cg_w;
+  double dp, beta = 0, betaold = 0, dpi = 0, dpiold, a, bb, rnorm0 = 0, ttol = 0;
+  int its = 0, reason = 0, i;
+  operator_a(o, x, r);
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < o->ng; k++) r[k] = b[k] - r[k];
+  apply_pc(o, r, z);
+  dp = sqrt(vdot(o, z, z));
+  reason = ksp_converged(o, 0, dp, &rnorm0, &ttol, b);
+  if (reason) { *reason_out = reason; return 0; }
+  beta = vdot(o, z, r);
+  i = 0;
+  do {
+    its = i + 1;
+    if (beta == 0.0) { reason = KSP_CONVERGED_ATOL; break; }
+    else if (i > 0 && beta * betaold < 0.0) { reason = KSP_DIVERGED_INDEFINITE_PC; break; }
+    if (!i) {
+      memcpy(p, z, sizeof(double) * o->ng);
+    } else {
+      bb = beta / betaold;
+#pragma omp parallel for schedule(static)
+      for (int64_t k = 0; k < o->ng; k++) p[k] = z[k] + bb * p[k];
+    }
+    dpiold = dpi;
+    operator_a(o, p, wv);
+    dpi = vdot(o, p, wv);
+    betaold = beta;
+    if (isnan(dpi) || isinf(dpi)) { reason = KSP_DIVERGED_NANORINF; break; }
+    if ((dpi == 0.0) || ((i > 0) && (dpi * dpiold <= 0.0))) { reason = KSP_DIVERGED_INDEFINITE_MAT; break; }
+    a = beta / dpi;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < o->ng; k++) { x[k] += a * p[k]; r[k] -= a * wv[k]; }
+    apply_pc(o, r, z);
+    dp = sqrt(vdot(o, z, z));
+    reason = ksp_converged(o, i + 1, dp, &rnorm0, &ttol, b);
+    if (reason) break;
+    beta = vdot(o, z, r);
+    if (isnan(beta) || isinf(beta)) { reason = KSP_DIVERGED_NANORINF; break; }
+    i++;
+  } while (i < o->maxit);
+  if (i >= o->maxit) reason = KSP_DIVERGED_ITS;
+  *reason_out = reason;
+  return its;
+}
+
+/* DiffusionSolver::solve (src/pde/DiffusionSolver.cpp:217-250): CN half step, in place */
+int orc_diffusion_solve(orc_t *o, double *c, double tau) {
+  o->ctx_dt = tau;
+  o->ksp_total_calls++;
+  if (o->k_scale == 0) { o->ksp_itr = 0; o->ksp_reason = 0; return 0; }
+  double *rhs = o->rhs;
+  double *t = o->w[0];
+  const double alph = 1.0 / 2.0 * o->ctx_dt;
+  orc_apply_d(o, c, t);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < o->ng; i++) rhs[i] = c[i] + alph * t[i];
+  int reason = 0;
+  o->ksp_itr = ksp_cg(o, rhs, c, &reason);
+  o->ksp_reason = reason;
+  return o->ksp_itr;
+}
+
+int orc_last_reason(orc_t *o) { return o->ksp_reason; }
+
+void orc_set_solver(orc_t *o, double rtol, int maxit) { o->rtol = rtol; o->maxit = maxit; }
+
+void orc_set_ctx_dt(orc_t *o, double dt) { o->ctx_dt = dt; }
+double orc_get_ctx_dt(orc_t *o) { return o->ctx_dt; }
+double orc_get_prec_dt(orc_t *o) { return o->prec_dt; }
+
+/* ------------------------------------------------------------------------ */
+/* Reaction (src/pde/PdeOperators.cpp:140-190; src/cuda/PdeOperators.cu:4-25) */
+/* ------------------------------------------------------------------------ */
+void orc_reaction(orc_t *o, double *ct, const double *clin, double dt) {
+  if (!clin) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < o->ng; i++) {
+      double factor = exp(o->rho[i] * dt);
+      double alph = ct[i] / (1.0 - ct[i]);
+      if (isinf(alph)) ct[i] = 1.0;
+      else ct[i] = alph * factor / (alph * factor + 1.0);
+    }
+  } else {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < o->ng; i++) {
+      double factor = exp(o->rho[i] * dt);
+      double alph = (clin[i] * factor + 1.0 - clin[i]);
+      ct[i] = ct[i] * factor / (alph * alph);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Time history and the forward / adjoint solves                            */
+/* ------------------------------------------------------------------------ */
+
+/* TumorParameters dt_/nt_ and PdeOperatorsRD::resizeTimeHistory (src/pde/PdeOperators.cpp:56-103).
+   Setting dt also resets the diffusion solver's context dt, as constructing DiffusionSolver does
+   (src/pde/DiffusionSolver.cpp:11). */
+void orc_set_time(orc_t *o, double dt, int nt, int adjoint_store) {
+  free_hist(o);
+  o->dt = dt; o->nt = nt; o->adjoint_store = adjoint_store;
+  o->ctx_dt = dt;
+  o->hist_nt = nt;
+  o->c_hist = (double **)calloc(nt + 1, sizeof(double *));
+  o->p_hist = (double **)calloc(nt + 1, sizeof(double *));
+  for (int i = 0; i <= nt; i++) { o->c_hist[i] = dalloc(o->ng); o->p_hist[i] = dalloc(o->ng); }
+  if (adjoint_store) {
+    o->c_half = (double **)calloc(nt, sizeof(double *));
+    for (int i = 0; i < nt; i++) o->c_half[i] = dalloc(o->ng);
+  }
+}
+
+void orc_set_c0(orc_t *o, const double *c0) { memcpy(o->c_0, c0, sizeof(double) * o->ng); }
+void orc_set_pt(orc_t *o, const double *pt) { memcpy(o->p_t, pt, sizeof(double) * o->ng); }
+void orc_get_c0(orc_t *o, double *c) { memcpy(c, o->c_0, sizeof(double) * o->ng); }
+void orc_get_ct(orc_t *o, double *c) { memcpy(c, o->c_t, sizeof(double) * o->ng); }
+void orc_get_p0(orc_t *o, double *p) { memcpy(p, o->p_0, sizeof(double) * o->ng); }
+void orc_get_c_hist(orc_t *o, int i, double *c) { memcpy(c, o->c_hist[i], sizeof(double) * o->ng); }
+void orc_get_p_hist(orc_t *o, int i, double *p) { memcpy(p, o->p_hist[i], sizeof(double) * o->ng); }
+void orc_get_ksp(orc_t *o, int *state, int *adj, int *last) {
+  if (state) *state = o->diff_ksp_itr_state;
+  if (adj) *adj = o->diff_ksp_itr_adj;
+  if (last) *last = o->ksp_itr;
+}
+
+/* PdeOperatorsRD::solveState (src/pde/PdeOperators.cpp:235-316), order 2 (Strang),
+   linearized 0 (state) or 1 (tangent linear around c_[i]). */
+int orc_solve_state(orc_t *o, int linearized) {
+  const double dt = o->dt;
+  memcpy(o->c_t, o->c_0, sizeof(double) * o->ng);
+  if (linearized == 0) memcpy(o->c_hist[0], o->c_t, sizeof(double) * o->ng);
+  o->diff_ksp_itr_state = 0;
+  for (int i = 0; i < o->nt; i++) {
+    orc_diffusion_solve(o, o->c_t, dt / 2.0);
+    o->diff_ksp_itr_state += o->ksp_itr;
+    if (linearized == 0 && o->adjoint_store) memcpy(o->c_half[i], o->c_t, sizeof(double) * o->ng);
+    orc_reaction(o, o->c_t, linearized ? o->c_hist[i] : NULL, dt);
+    orc_diffusion_solve(o, o->c_t, dt / 2.0);
+    o->diff_ksp_itr_state += o->ksp_itr;
+    if (linearized == 0) memcpy(o->c_hist[i + 1], o->c_t, sizeof(double) * o->ng);
+  }
+  return o->diff_ksp_itr_state;
+}
+
+/* PdeOperatorsRD::reactionAdjoint (src/pde/PdeOperators.cpp:318-370) */
+static void reaction_adjoint(orc_t *o, int iter) {
+  double *temp = o->adj_temp; /* the role of Tumor::work_[11] */
+  memcpy(temp, o->c_hist[iter], sizeof(double) * o->ng);
+  if (o->adjoint_store) {
+    memcpy(temp, o->c_half[iter], sizeof(double) * o->ng);
+  } else {
+    orc_diffusion_solve(o, temp, o->dt / 2.0);
+    o->diff_ksp_itr_adj += o->ksp_itr;
+  }
+  orc_reaction(o, o->p_0, temp, o->dt);
+}
+
+/* PdeOperatorsRD::solveAdjoint (src/pde/PdeOperators.cpp:372-420), order 2 */
+int orc_solve_adjoint(orc_t *o, int linearized) {
+  const double dt = o->dt;
+  const int nt = o->nt;
+  memcpy(o->p_0, o->p_t, sizeof(double) * o->ng);
+  if (linearized == 1) memcpy(o->p_hist[nt], o->p_0, sizeof(double) * o->ng);
+  o->diff_ksp_itr_adj = 0;
+  for (int i = 0; i < nt; i++) {
+    orc_diffusion_solve(o, o->p_0, dt / 2.0);
+    o->diff_ksp_itr_adj += o->ksp_itr;
+    reaction_adjoint(o, nt - i - 1);
+    orc_diffusion_solve(o, o->p_0, dt / 2.0);
+    o->diff_ksp_itr_adj += o->ksp_itr;
+    memcpy(o->p_hist[nt - i - 1], o->p_0, sizeof(double) * o->ng);
+  }
+  return o->diff_ksp_itr_adj;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Phi: Gaussian basis (src/mat/Phi.cpp:76-439, src/cuda/Phi.cu:11-83)       */
+/* ------------------------------------------------------------------------ */
+
+/* Phi::phiMesh (src/mat/Phi.cpp:148-234); returns the number of centers written */
+int orc_phi_mesh(int np, const double *cm, double sigma, double spacing_factor, double *center) {
+  int h = (int)round(pow(np, 1.0 / 3.0));
+  double space = spacing_factor * sigma;
+  if (np == 1) { center[0] = cm[0]; center[1] = cm[1]; center[2] = cm[2]; return 1; }
+  int ptr = 0;
+  if (np % 2 == 1) {
+    for (int k = -(h - 1) / 2; k <= (h - 1) / 2; k++)
+      for (int j = -(h - 1) / 2; j <= (h - 1) / 2; j++)
+        for (int i = -(h - 1) / 2; i <= (h - 1) / 2; i++) {
+          center[ptr + 0] = i * space + cm[0];
+          center[ptr + 1] = j * space + cm[1];
+          center[ptr + 2] = k * space + cm[2];
+          ptr += 3;
+        }
+  } else {
+    for (int k = -(h) / 2; k <= (h) / 2; k++)
+      for (int j = -(h) / 2; j <= (h) / 2; j++)
+        for (int i = -(h) / 2; i <= (h) / 2; i++) {
+          double m0 = ((i < 0) ? -1 : ((i > 0) ? 1 : 0)) * 0.5;
+          double m1 = ((j < 0) ? -1 : ((j > 0) ? 1 : 0)) * 0.5;
+          double m2 = ((k < 0) ? -1 : ((k > 0) ? 1 : 0)) * 0.5;
+          if ((i != 0) && (j != 0) && (k != 0)) {
+            center[ptr + 0] = (i - m0) * space + cm[0];
+            center[ptr + 1] = (j - m1) * space + cm[1];
+            center[ptr + 2] = (k - m2) * space + cm[2];
+            ptr += 3;
+          }
+        }
+  }
+  return ptr / 3;
+}
+
+/* Phi::initialize (src/mat/Phi.cpp:264-311) + mask + smoother + Phi::truncate (:238-260) */
+static void phi_build_one(orc_t *o, const double *ctr, double *out) {
+  const int n1 = o->n[1], n2 = o->n[2];
+  const double hx = o->h[0], hy = o->h[1], hz = o->h[2];
+  const double R = sqrt(2.) * o->phi_sigma;
+  const double xc = ctr[0], yc = ctr[1], zc = ctr[2];
+#pragma omp parallel for schedule(static)
+  for (int64_t idx = 0; idx < o->ng; idx++) {
+    int64_t X = idx / ((int64_t)n1 * n2), Y = (idx / n2) % n1, Z = idx % n2;
+    double dx = hx * X - xc, dy = hy * Y - yc, dz = hz * Z - zc;
+    double r = sqrt(dx * dx + dy * dy + dz * dz);
+    double ratio = r / R;
+    out[idx] = exp(-ratio * ratio) * o->filter[idx];
+  }
+  double sigma_smooth = o->smoothing_factor * 2.0 * M_PI / o->n[0];
+  orc_smooth(o, out, out, sigma_smooth);
+#pragma omp parallel for schedule(static)
+  for (int64_t idx = 0; idx < o->ng; idx++) {
+    int64_t X = idx / ((int64_t)n1 * n2), Y = (idx / n2) % n1, Z = idx % n2;
+    double dx = hx * X - xc, dy = hy * Y - yc, dz = hz * Z - zc;
+    double r = sqrt(dx * dx + dy * dy + dz * dz);
+    out[idx] = (r / o->phi_sigma <= 5) ? out[idx] : 0.0;
+  }
+}
+
+static double vmax(orc_t *o, const double *v) {
+  double m = -INFINITY;
+  for (int64_t i = 0; i < o->ng; i++) if (v[i] > m) m = v[i];
+  return m;
+}
+
+/* Phi::setGaussians + Phi::setValues (src/mat/Phi.cpp:76-144).  With phi_store the bases are
+   stored normalized by the global max; without it the raw bases are cached here only to save
+   oracle time -- apply/applyTranspose below normalize exactly as the on-the-fly path does. */
+void orc_set_phi(orc_t *o, int np, const double *centers, double sigma, double smoothing_factor, int phi_store) {
+  free_phi(o);
+  o->np = np;
+  o->phi_sigma = sigma;
+  o->smoothing_factor = smoothing_factor;
+  o->phi_store = phi_store;
+  o->centers = (double *)malloc(sizeof(double) * 3 * (np > 0 ? np : 1));
+  memcpy(o->centers, centers, sizeof(double) * 3 * np);
+  o->phi = (double **)calloc(np > 0 ? np : 1, sizeof(double *));
+  double phi_max = 0;
+  for (int i = 0; i < np; i++) {
+    o->phi[i] = dalloc(o->ng);
+    phi_build_one(o, &centers[3 * i], o->phi[i]);
+    double m = vmax(o, o->phi[i]);
+    if (m > phi_max) phi_max = m;
+  }
+  o->phi_max_all = phi_max;
+  if (phi_store) {
+    for (int i = 0; i < np; i++)
+      for (int64_t k = 0; k < o->ng; k++) o->phi[i][k] *= (1.0 / phi_max);
+  }
+}
+
+void orc_get_phi(orc_t *o, int i, double *out) { memcpy(out, o->phi[i], sizeof(double) * o->ng); }
+
+/* Phi::apply (src/mat/Phi.cpp:313-377) */
+void orc_phi_apply(orc_t *o, const double *p, double *out) {
+  memset(out, 0, sizeof(double) * o->ng);
+  if (o->phi_store) {
+    for (int i = 0; i < o->np; i++)
+      for (int64_t k = 0; k < o->ng; k++) out[k] += p[i] * o->phi[i][k];
+    return;
+  }
+  double phi_max = 0;
+  int nnz = 0;
+  for (int i = 0; i < o->np; i++) {
+    if (p[i] != 0) {
+      nnz = 1;
+      double m = vmax(o, o->phi[i]);
+      if (m > phi_max) phi_max = m;
+      for (int64_t k = 0; k < o->ng; k++) out[k] += p[i] * o->phi[i][k];
+    }
+  }
+  if (!nnz) phi_max = 1;
+  for (int64_t k = 0; k < o->ng; k++) out[k] *= (1.0 / phi_max);
+}
+
+/* Phi::applyTranspose (src/mat/Phi.cpp:379-439) */
+void orc_phi_applyT(orc_t *o, const double *in, double *pout) {
+  for (int i = 0; i < o->np; i++) pout[i] = vdot(o, o->phi[i], in);
+  if (!o->phi_store) {
+    for (int i = 0; i < o->np; i++) pout[i] *= (1.0 / o->phi_max_all);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Observation operator and data (src/mat/Obs.cpp:26-132)                    */
+/* ------------------------------------------------------------------------ */
+void orc_set_data(orc_t *o, const double *d1, const double *obs_mask, double thr) {
+  memcpy(o->d1, d1, sizeof(double) * o->ng);
+  for (int64_t i = 0; i < o->ng; i++) o->obs[i] = obs_mask ? obs_mask[i] : (double)(d1[i] > thr);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Derivative operators (src/grad/DerivativeOperatorsRD.cpp:7-226,           */
+/* src/grad/DerivativeOperators.cpp:189-367)                                  */
+/* mode 0: invert p and kappa (flag_reaction_inv_ = false): c0 = Phi p, g_p, g_kappa
+   mode 1: invert kappa and rho (flag_reaction_inv_ = true): c0 fixed, g_kappa, g_rho
+   mode 2: all of p, kappa, rho in one pass (the same formulas; the build's own API)
+   x = [p (np) | kappa | rho]                                                    */
+/* ------------------------------------------------------------------------ */
+void orc_set_opt(orc_t *o, double beta, double k_scale, double rho_scale) {
+  o->beta = beta; o->opt_k_scale = k_scale; o->opt_rho_scale = rho_scale;
+}
+
+/* updateReactionAndDiffusion (src/grad/DerivativeOperators.cpp:323-367) with nk = nr = 1 */
+static void update_reaction_and_diffusion(orc_t *o, const double *x, int mode) {
+  double k1 = x[o->np] * o->opt_k_scale;
+  orc_set_k(o, k1, o->k_gm_wm, o->k_glm_wm);
+  orc_prec_factor(o);
+  if (mode != 0) {
+    double r1 = x[o->np + 1] * o->opt_rho_scale;
+    orc_set_rho(o, r1, o->r_gm_wm, o->r_glm_wm);
+  }
+}
+
+static double misfit(orc_t *o) {
+  double *t = o->w[0];
+  for (int64_t i = 0; i < o->ng; i++) t[i] = o->obs[i] * o->c_t[i] - o->d1[i];
+  return vdot(o, t, t);
+}
+
+double orc_objective(orc_t *o, const double *x, int mode) {
+  update_reaction_and_diffusion(o, x, mode);
+  if (mode != 1) orc_phi_apply(o, x, o->c_0);
+  orc_solve_state(o, 0);
+  double m1 = misfit(o);
+  double reg = vdot(o, o->c_0, o->c_0) * 0.5 * o->beta * o->lebesgue;
+  return o->lebesgue * 0.5 * m1 + reg;
+}
+
+/* gradDiffusion (:189-256) and gradReaction (:258-321) for nk = nr = 1 */
+static void grad_kappa_rho(orc_t *o, double *gk, double *gr) {
+  double *g1 = o->w[1], *g2 = o->w[2], *g3 = o->w[3], *g4 = o->w[4], *g5 = o->w[5], *g6 = o->w[6];
+  double *Q = o->w[7], *P = o->w[8];
+  memset(Q, 0, sizeof(double) * o->ng);
+  memset(P, 0, sizeof(double) * o->ng);
+  for (int i = 0; i < o->nt + 1; i++) {
+    double iw = (i == 0 || i == o->nt) ? 0.5 : 1.0;
+    orc_grad(o, o->c_hist[i], g1, g2, g3);
+    orc_grad(o, o->p_hist[i], g4, g5, g6);
+    const double *c = o->c_hist[i], *a = o->p_hist[i];
+    const double s = o->dt * iw;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < o->ng; k++) {
+      double v = g1[k] * g4[k];
+      v += g2[k] * g5[k];
+      v += g3[k] * g6[k];
+      Q[k] += s * v;
+      double r = c[k] * c[k] - c[k];
+      P[k] += s * (a[k] * r);
+    }
+  }
+  *gk = vdot(o, o->wm, Q) * o->lebesgue + vdot(o, o->gm, Q) * o->lebesgue * o->k_gm_wm;
+  *gr = vdot(o, o->wm, P) * o->lebesgue + vdot(o, o->gm, P) * o->lebesgue * o->r_gm_wm;
+}
+
+/* evaluateObjectiveAndGradient (src/grad/DerivativeOperatorsRD.cpp:130-226) */
+void orc_objective_and_gradient(orc_t *o, const double *x, int mode, double *J, double *dJ) {
+  const int np = o->np;
+  for (int i = 0; i < np + 2; i++) dJ[i] = 0;
+  update_reaction_and_diffusion(o, x, mode);
+  if (mode != 1) orc_phi_apply(o, x, o->c_0);
+  orc_solve_state(o, 0);
+  double *t = o->w[0];
+  for (int64_t i = 0; i < o->ng; i++) t[i] = o->obs[i] * o->c_t[i] - o->d1[i];
+  double m1 = vdot(o, t, t);
+  for (int64_t i = 0; i < o->ng; i++) o->p_t[i] = -(o->obs[i] * t[i]);
+  orc_solve_adjoint(o, 1);
+  if (mode != 1) {
+    for (int64_t i = 0; i < o->ng; i++) o->p_0[i] += -o->beta * o->c_0[i];
+    orc_phi_applyT(o, o->p_0, dJ);
+    for (int i = 0; i < np; i++) dJ[i] *= -o->lebesgue;
+  }
+  double reg = vdot(o, o->c_0, o->c_0) * 0.5 * o->beta * o->lebesgue;
+  *J = o->lebesgue * 0.5 * m1 + reg;
+  double gk, gr;
+  grad_kappa_rho(o, &gk, &gr);
+  dJ[np] = gk;
+  dJ[np + 1] = (mode == 0) ? 0.0 : gr;
+}

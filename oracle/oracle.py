@@ -1,0 +1,316 @@
+"""ctypes wrapper around the FP64 oracle (oracle/glia_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product never imports it.
+
+Each method names the reference function it restates; see glia_oracle.c for
+the file:line citations.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liborc.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_cp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
+
+
+def build():
+    """Compile liborc.so from glia_oracle.c with the Makefile beside it."""
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB) or os.path.getmtime(_LIB) < os.path.getmtime(os.path.join(_HERE, "glia_oracle.c")):
+        build()
+    L = ctypes.CDLL(_LIB)
+    vp = ctypes.c_void_p
+    i, d = ctypes.c_int, ctypes.c_double
+    sig = {
+        "orc_create": (vp, [i, i, i]),
+        "orc_destroy": (None, [vp]),
+        "orc_max_threads": (i, []),
+        "orc_set_threads": (None, [i]),
+        "orc_r2c": (None, [vp, _dp, _cp]),
+        "orc_c2r": (None, [vp, _cp, _dp]),
+        "orc_grad": (None, [vp, _dp, _dp, _dp, _dp]),
+        "orc_div": (None, [vp, _dp, _dp, _dp, _dp]),
+        "orc_smooth": (None, [vp, _dp, _dp, d]),
+        "orc_set_tissue": (None, [vp, vp, vp, vp, vp, vp]),
+        "orc_set_tissue_sinusoidal": (None, [vp]),
+        "orc_set_k": (None, [vp, d, d, d]),
+        "orc_set_k_sinusoidal": (None, [vp, d]),
+        "orc_set_rho": (None, [vp, d, d, d]),
+        "orc_get_fields": (None, [vp, vp, vp, vp, vp]),
+        "orc_apply_d": (None, [vp, _dp, _dp]),
+        "orc_prec_factor": (None, [vp]),
+        "orc_diffusion_solve": (i, [vp, _dp, d]),
+        "orc_last_reason": (i, [vp]),
+        "orc_set_solver": (None, [vp, d, i]),
+        "orc_set_ctx_dt": (None, [vp, d]),
+        "orc_get_ctx_dt": (d, [vp]),
+        "orc_get_prec_dt": (d, [vp]),
+        "orc_reaction": (None, [vp, _dp, vp, d]),
+        "orc_set_time": (None, [vp, d, i, i]),
+        "orc_set_c0": (None, [vp, _dp]),
+        "orc_set_pt": (None, [vp, _dp]),
+        "orc_get_c0": (None, [vp, _dp]),
+        "orc_get_ct": (None, [vp, _dp]),
+        "orc_get_p0": (None, [vp, _dp]),
+        "orc_get_c_hist": (None, [vp, i, _dp]),
+        "orc_get_p_hist": (None, [vp, i, _dp]),
+        "orc_get_ksp": (None, [vp, vp, vp, vp]),
+        "orc_solve_state": (i, [vp, i]),
+        "orc_solve_adjoint": (i, [vp, i]),
+        "orc_phi_mesh": (i, [i, _dp, d, d, _dp]),
+        "orc_set_phi": (None, [vp, i, _dp, d, d, i]),
+        "orc_get_phi": (None, [vp, i, _dp]),
+        "orc_phi_apply": (None, [vp, _dp, _dp]),
+        "orc_phi_applyT": (None, [vp, _dp, _dp]),
+        "orc_set_data": (None, [vp, _dp, vp, d]),
+        "orc_set_opt": (None, [vp, d, d, d]),
+        "orc_objective": (d, [vp, _dp, i]),
+        "orc_objective_and_gradient": (None, [vp, _dp, i, _dp, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def max_threads():
+    return _load().orc_max_threads()
+
+
+def set_threads(n):
+    _load().orc_set_threads(int(n))
+
+
+def phi_mesh(np_, cm, sigma, spacing_factor):
+    """Phi::phiMesh (src/mat/Phi.cpp:148-234)."""
+    out = np.zeros(3 * max(np_, 1) * 2, dtype=np.float64)
+    k = _load().orc_phi_mesh(int(np_), _c(cm), float(sigma), float(spacing_factor), out)
+    return out[: 3 * k].reshape(k, 3)
+
+
+class Oracle:
+    """FP64 restatement of the reference RD path on an n0 x n1 x n2 periodic grid."""
+
+    # evaluation modes of evaluateObjective/Gradient: which parameters the reference inverts
+    MODE_P_KAPPA = 0    # flag_reaction_inv_ = false: c0 = Phi p; g_p, g_kappa
+    MODE_KAPPA_RHO = 1  # flag_reaction_inv_ = true: c0 fixed; g_kappa, g_rho
+    MODE_ALL = 2        # p, kappa, rho in one pass
+
+    def __init__(self, n):
+        if np.isscalar(n):
+            n = (int(n),) * 3
+        self.n = tuple(int(v) for v in n)
+        self.L = _load()
+        self.h = self.L.orc_create(*self.n)
+        if not self.h:
+            raise ValueError("unsupported grid %r (lengths must be 2^a 3^b 5^c)" % (self.n,))
+        self.ng = self.n[0] * self.n[1] * self.n[2]
+        self.shape = self.n
+        self.sshape = (self.n[0], self.n[1], self.n[2] // 2 + 1)
+        self.np = 0
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.L.orc_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def _f(self, a):
+        a = _c(a).reshape(-1)
+        if a.size != self.ng:
+            raise ValueError("field has %d entries, grid has %d" % (a.size, self.ng))
+        return a
+
+    def _new(self):
+        return np.zeros(self.ng, dtype=np.float64)
+
+    # ---- spectral operators -------------------------------------------------
+    def r2c(self, f):
+        out = np.zeros(self.sshape, dtype=np.complex128).reshape(-1)
+        self.L.orc_r2c(self.h, self._f(f), out)
+        return out.reshape(self.sshape)
+
+    def c2r(self, F):
+        F = np.ascontiguousarray(F, dtype=np.complex128).reshape(-1).copy()
+        out = self._new()
+        self.L.orc_c2r(self.h, F, out)
+        return out.reshape(self.shape)
+
+    def grad(self, x):
+        g = [self._new() for _ in range(3)]
+        self.L.orc_grad(self.h, self._f(x), *g)
+        return [v.reshape(self.shape) for v in g]
+
+    def div(self, dx, dy, dz):
+        out = self._new()
+        self.L.orc_div(self.h, self._f(dx), self._f(dy), self._f(dz), out)
+        return out.reshape(self.shape)
+
+    def smooth(self, c, sigma):
+        out = self._new()
+        self.L.orc_smooth(self.h, self._f(c), out, float(sigma))
+        return out.reshape(self.shape)
+
+    # ---- material / coefficients -------------------------------------------
+    def set_tissue(self, wm=None, gm=None, csf=None, vt=None, bg=None):
+        arrs = [None if a is None else self._f(a) for a in (wm, gm, csf, vt, bg)]
+        self._keep = arrs
+        self.L.orc_set_tissue(self.h, *[_ptr(a) for a in arrs])
+
+    def set_tissue_sinusoidal(self):
+        self.L.orc_set_tissue_sinusoidal(self.h)
+
+    def set_k(self, k_scale, k_gm_wm=0.0, k_glm_wm=0.0):
+        self.L.orc_set_k(self.h, float(k_scale), float(k_gm_wm), float(k_glm_wm))
+
+    def set_k_sinusoidal(self, scale):
+        self.L.orc_set_k_sinusoidal(self.h, float(scale))
+
+    def set_rho(self, rho, r_gm_wm=0.0, r_glm_wm=0.0):
+        self.L.orc_set_rho(self.h, float(rho), float(r_gm_wm), float(r_glm_wm))
+
+    def fields(self):
+        k, rho, filt = self._new(), self._new(), self._new()
+        kavg = np.zeros(1)
+        self.L.orc_get_fields(self.h, _ptr(k), _ptr(rho), _ptr(filt), _ptr(kavg))
+        return dict(k=k.reshape(self.shape), rho=rho.reshape(self.shape), filter=filt.reshape(self.shape), kavg=float(kavg[0]))
+
+    # ---- diffusion solver ---------------------------------------------------
+    def set_solver(self, rtol=1e-6, maxit=5000):
+        self.L.orc_set_solver(self.h, float(rtol), int(maxit))
+
+    def set_time(self, dt, nt, adjoint_store=False):
+        self.L.orc_set_time(self.h, float(dt), int(nt), int(bool(adjoint_store)))
+        self.dt, self.nt = float(dt), int(nt)
+
+    def set_ctx_dt(self, dt):
+        self.L.orc_set_ctx_dt(self.h, float(dt))
+
+    @property
+    def ctx_dt(self):
+        return self.L.orc_get_ctx_dt(self.h)
+
+    @property
+    def prec_dt(self):
+        return self.L.orc_get_prec_dt(self.h)
+
+    def prec_factor(self):
+        self.L.orc_prec_factor(self.h)
+
+    def apply_d(self, c):
+        out = self._new()
+        self.L.orc_apply_d(self.h, self._f(c), out)
+        return out.reshape(self.shape)
+
+    def diffusion_solve(self, c, tau):
+        c = self._f(c).copy()
+        its = self.L.orc_diffusion_solve(self.h, c, float(tau))
+        return c.reshape(self.shape), its
+
+    @property
+    def last_reason(self):
+        return self.L.orc_last_reason(self.h)
+
+    def reaction(self, c, dt, clin=None):
+        c = self._f(c).copy()
+        cl = None if clin is None else self._f(clin)
+        self.L.orc_reaction(self.h, c, _ptr(cl), float(dt))
+        return c.reshape(self.shape)
+
+    # ---- forward / adjoint --------------------------------------------------
+    def set_c0(self, c0):
+        self.L.orc_set_c0(self.h, self._f(c0))
+
+    def set_pt(self, pt):
+        self.L.orc_set_pt(self.h, self._f(pt))
+
+    def solve_state(self, linearized=0):
+        return self.L.orc_solve_state(self.h, int(linearized))
+
+    def solve_adjoint(self, linearized=1):
+        return self.L.orc_solve_adjoint(self.h, int(linearized))
+
+    def _get(self, fn, *a):
+        out = self._new()
+        fn(self.h, *a, out)
+        return out.reshape(self.shape)
+
+    def c0(self):
+        return self._get(self.L.orc_get_c0)
+
+    def ct(self):
+        return self._get(self.L.orc_get_ct)
+
+    def p0(self):
+        return self._get(self.L.orc_get_p0)
+
+    def c_hist(self, i):
+        return self._get(self.L.orc_get_c_hist, int(i))
+
+    def p_hist(self, i):
+        return self._get(self.L.orc_get_p_hist, int(i))
+
+    def ksp(self):
+        a, b, c = (ctypes.c_int(), ctypes.c_int(), ctypes.c_int())
+        self.L.orc_get_ksp(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return dict(state=a.value, adjoint=b.value, last=c.value)
+
+    # ---- Phi / Obs / data ---------------------------------------------------
+    def set_phi(self, centers, sigma, smoothing_factor=1.0, phi_store=False):
+        centers = _c(centers).reshape(-1)
+        self.np = centers.size // 3
+        self.L.orc_set_phi(self.h, self.np, centers, float(sigma), float(smoothing_factor), int(bool(phi_store)))
+
+    def phi(self, i):
+        return self._get(self.L.orc_get_phi, int(i))
+
+    def phi_apply(self, p):
+        out = self._new()
+        self.L.orc_phi_apply(self.h, _c(p), out)
+        return out.reshape(self.shape)
+
+    def phi_applyT(self, v):
+        out = np.zeros(max(self.np, 1))
+        self.L.orc_phi_applyT(self.h, self._f(v), out)
+        return out[: self.np]
+
+    def set_data(self, d1, obs_mask=None, thr=0.0):
+        m = None if obs_mask is None else self._f(obs_mask)
+        self.L.orc_set_data(self.h, self._f(d1), _ptr(m), float(thr))
+
+    def set_opt(self, beta=0.0, k_scale=1.0, rho_scale=1.0):
+        self.L.orc_set_opt(self.h, float(beta), float(k_scale), float(rho_scale))
+
+    def objective(self, x, mode=MODE_ALL):
+        return self.L.orc_objective(self.h, _c(x), int(mode))
+
+    def objective_and_gradient(self, x, mode=MODE_ALL):
+        J = np.zeros(1)
+        dJ = np.zeros(self.np + 2)
+        self.L.orc_objective_and_gradient(self.h, _c(x), int(mode), J, dJ)
+        return float(J[0]), dJ
