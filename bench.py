@@ -1,0 +1,264 @@
+"""Benchmark: forward + adjoint gradient evaluations of the reaction-diffusion model on one MI355X.
+
+Workload (default): the 256^3 members of the batching configuration -- an ensemble of 8 members at
+256^3, each with its own seeded tissue maps, kappa ~ U(0.01, 0.1), rho ~ U(4, 15), 8-sparse p over
+64 Gaussian bases, nt = 50 steps of dt = 0.02, observed data thresholded from a seeded
+ground-truth forward run.  One step = one batched evaluation of J and dJ = [g_p | g_kappa | g_rho]
+for every member (FP32 bulk, FP64 reductions).  value = gradient evaluations per second of the
+whole ensemble.  Also reported: the same evaluation run one member at a time ("plain"), the
+relative errors against the FP64 oracle at 64^3 (where the oracle finishes in seconds), the
+achieved bandwidth of the dominant kernel against the MI355X HBM peak, and the oracle timed on
+the host cores on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "grad evals/s at 256^3 (batched FP32 vs plain), rel err vs FP64; HBM GB/s"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_members(n, count, nbases, nnz, seed0):
+    from glia_amd import synth
+    kap, rho = synth.ensemble_params(count, seed0)
+    out = []
+    for m in range(count):
+        pr = synth.problem(n, seed0 + m, nbases=nbases, nnz=nnz, kappa=float(kap[m]), rho=float(rho[m]),
+                           sigma=2 * np.pi / 64)
+        pr["pstar"] = synth.sparse_p(nbases, nnz, seed0 + 1000 + m)
+        out.append(pr)
+    return out
+
+
+def build_solver(members, n, nt, dt, checkpoint_slots=0):
+    from glia_amd import rd
+    s = rd.RDSolver(n, nmembers=len(members), dt=dt, nt=nt, k_gm_wm=0.2, r_gm_wm=0.2,
+                    checkpoint_slots=checkpoint_slots)
+    for m, pr in enumerate(members):
+        s.set_tissue(m, pr["wm"], pr["gm"], pr["csf"], pr["vt"])
+        s.set_coefficients(m, pr["kappa"], pr["rho"])
+        s.set_basis(m, pr["centers"], pr["sigma"])
+    return s
+
+
+def make_data(s, members):
+    """d1 = 1[c*(T) > 0.5] from the ground-truth coefficients (setup, not timed)."""
+    s.phi_apply([pr["pstar"] for pr in members])
+    s.precFactor()
+    s.solve_state(0)
+    for m, pr in enumerate(members):
+        ct = s.get_field(m, "ct")
+        pr["d1"] = (ct > 0.5).astype(np.float32)
+        s.set_data(m, pr["d1"], None, -0.99)
+
+
+def xvec(pr):
+    return np.concatenate([pr["p"], [pr["kappa"], pr["rho"]]])
+
+
+def accuracy_64():
+    """Relative errors of c(T), J and dJ against the FP64 oracle at 64^3 (nt = 10, dt = 0.02)."""
+    from glia_amd import rd, synth
+    from oracle import oracle
+    n, nt, dt = 64, 10, 0.02
+    pr = synth.problem(n, 2, nbases=8, kappa=0.05, rho=8.0)
+    o = oracle.Oracle(n)
+    o.set_tissue(pr["wm"], pr["gm"], pr["csf"], pr["vt"])
+    o.set_time(dt, nt)
+    o.set_k(pr["kappa"], 0.2, 0.0)
+    o.set_rho(pr["rho"], 0.2, 0.0)
+    o.prec_factor()
+    o.set_phi(pr["centers"], pr["sigma"], 1.0, False)
+    o.set_c0(o.phi_apply(pr["p"] * 1.3))
+    o.solve_state(0)
+    d1 = (o.ct() > 0.5).astype(np.float64)
+    o.set_data(d1, None, -0.99)
+    o.set_time(dt, nt)
+    x = xvec(pr)
+    Jo, go = o.objective_and_gradient(x, mode=2)
+    cto = o.ct()
+    s = rd.RDSolver(n, dt=dt, nt=nt, k_gm_wm=0.2, r_gm_wm=0.2)
+    s.set_tissue(0, pr["wm"], pr["gm"], pr["csf"], pr["vt"])
+    s.set_coefficients(0, pr["kappa"], pr["rho"])
+    s.set_basis(0, pr["centers"], pr["sigma"])
+    s.set_data(0, d1, None, -0.99)
+    J, g = s.objective_and_gradient([x])
+    ct = s.get_field(0, "ct")
+    s.close()
+    rl = lambda a, b: float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+    return {"grid": "64^3, nt=10", "cT": rl(ct, cto), "J": abs(J[0] - Jo) / abs(Jo), "dJ": rl(g[0], go)}
+
+
+def cpu_baseline(n, nt, dt, pr, c0):
+    """Oracle (FP64 CPU restatement of the reference path) on one forward Strang step of the
+    same 256^3 member; a gradient evaluation costs nt forward steps plus nt adjoint steps of three
+    CN solves each, so evals/s = 1 / (nt * t_step * (1 + 1.5))."""
+    from oracle import oracle
+    threads = oracle.max_threads()
+    o = oracle.Oracle(n)
+    o.set_tissue(pr["wm"], pr["gm"], pr["csf"], pr["vt"])
+    o.set_time(dt, 1)
+    o.set_k(pr["kappa"], 0.2, 0.0)
+    o.set_rho(pr["rho"], 0.2, 0.0)
+    o.prec_factor()
+    o.set_c0(c0)  # c(0) = Phi p of the GPU run (the sample times the time stepping only)
+    t0 = time.perf_counter()
+    o.solve_state(0)
+    t_step = time.perf_counter() - t0
+    its = o.ksp()["state"]
+    del o
+    evals = 1.0 / (nt * t_step * 2.5)
+    return {"value": evals, "unit": "grad evals/s", "cores": threads, "kind": "port",
+            "sample": "oracle FP64, one forward Strang step (2 CN solves, %d CG iterations) of one %d^3 "
+                      "member in %.1f s on %d threads; gradient eval = %d forward + %d adjoint steps "
+                      "(adjoint step = 1.5 forward steps)" % (its, n, t_step, threads, nt, nt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--members", type=int, default=8)
+    ap.add_argument("--nt", type=int, default=50)
+    ap.add_argument("--dt", type=float, default=0.02)
+    ap.add_argument("--nbases", type=int, default=64)
+    ap.add_argument("--nnz", type=int, default=8)
+    ap.add_argument("--checkpoint-slots", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (ensemble members "
+              "would shard across GPUs with no communication in the hot loop); run with --gpus 1",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    from glia_amd import _lib
+    _lib.load()
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        return 3
+    n, M, nt, dt = args.n, args.members, args.nt, args.dt
+
+    t0 = time.perf_counter()
+    members = make_members(n, M, args.nbases, args.nnz, 100)
+    log("synthetic inputs: %.1f s" % (time.perf_counter() - t0))
+    s = build_solver(members, n, nt, dt, args.checkpoint_slots)
+    make_data(s, members)
+    log("setup + data: %.1f s" % (time.perf_counter() - t0))
+    xs = [xvec(pr) for pr in members]
+
+    for _ in range(args.warmup):
+        s.objective_and_gradient(xs)
+    s.reset_stats()
+    s.profile(True)
+    torch.cuda.synchronize()
+    s.sync()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        J, g = s.objective_and_gradient(xs)
+    s.sync()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    prof = s.profile_get()
+    s.profile(False)
+    st = [s.stats(m) for m in range(M)]
+    ms_step = (t2 - t1) * 1000.0 / args.steps
+    value = M * args.steps / (t2 - t1)
+    its = sum(x["pcg_iterations"] for x in st)
+    solves = sum(x["cn_solves"] for x in st)
+    log("batched: %.1f ms/step, %.3f grad evals/s, %.2f CG its per CN solve" % (ms_step, value, its / max(solves, 1)))
+
+    # dominant kernel and its achieved bandwidth: algorithmic bytes / event-timed duration.  A
+    # launch covers every member, but converged members exit at once, so the bytes of one kernel
+    # over the timed region are (fields per member-iteration) * (CG iterations summed over members).
+    field_bytes = 4.0 * n ** 3
+    dom = max(prof, key=lambda k: prof[k][1])
+    cnt, ms = prof[dom]
+    achieved = _lib.KERNEL_FIELDS[dom] * field_bytes * its / (ms / 1000.0) / 1e9
+    kernels = {k: {"launches": v[0], "ms": round(v[1], 3), "member_iterations": its,
+                   "GBs": round(_lib.KERNEL_FIELDS[k] * field_bytes * its / (v[1] / 1000.0) / 1e9, 1)}
+               for k, v in prof.items() if v[0] > 0}
+    # whole-evaluation model of algorithmic bytes: B = R [nt (230 + 144 m) + 20 (nt + 1)] per member
+    mbar = its / max(solves, 1)
+    b_eval = field_bytes * (nt * (230 + 144 * mbar) + 20 * (nt + 1))
+    model_gbs = b_eval * M * args.steps / (t2 - t1) / 1e9
+    c0_member0 = s.get_field(0, "c0").astype(np.float64)
+    del s
+
+    plain = None
+    if not args.no_plain:
+        # the same evaluations, one member at a time (a one-member solver per member)
+        tsum = 0.0
+        for m in range(M):
+            sp = build_solver(members[m:m + 1], n, nt, dt, args.checkpoint_slots)
+            sp.set_data(0, members[m]["d1"], None, -0.99)
+            for _ in range(args.warmup):
+                sp.objective_and_gradient(xs[m:m + 1])
+            sp.sync()
+            tp0 = time.perf_counter()
+            sp.objective_and_gradient(xs[m:m + 1])
+            sp.sync()
+            tsum += time.perf_counter() - tp0
+            sp.close()
+        plain = M / tsum
+        log("plain (one member at a time): %.3f grad evals/s" % plain)
+
+    acc = None if args.no_accuracy else accuracy_64()
+    if acc:
+        log("accuracy vs FP64 oracle:", acc)
+    cpu = None if args.no_cpu else cpu_baseline(n, nt, dt, members[0], c0_member0)
+    if cpu:
+        log("cpu baseline:", cpu)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "grad evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched forward+adjoint gradient, %d members at %d^3, nt=%d, dt=%g, %d bases "
+                               "(%d nonzero), FP32 bulk + FP64 reductions" % (M, n, nt, dt, args.nbases, args.nnz),
+                   "n": n, "members": M, "nt": nt, "dt": dt, "nbases": args.nbases,
+                   "checkpoint_slots": args.checkpoint_slots},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "plain_value": None if plain is None else round(plain, 4),
+        "batched_vs_plain": None if plain is None else round(value / plain, 3),
+        "cg_iterations_per_solve": round(mbar, 3),
+        "model_hbm_GBs": round(model_gbs, 1),
+        "kernels": kernels,
+        "rel_err_fp64": acc,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -89,7 +89,14 @@ SIGNATURES = {
     "glia_rd_objective_and_gradient": (_i, [_vp, _i, _vp, _vp, _vp]),
     "glia_rd_stats": (_i, [_vp, _i, ctypes.POINTER(Stats)]),
     "glia_rd_reset_stats": (_i, [_vp]),
+    "glia_rd_profile": (_i, [_vp, _i]),
+    "glia_rd_profile_get": (_i, [_vp, _i, _pd, ctypes.POINTER(ctypes.c_longlong)]),
 }
+
+KERNELS = ["xi_pupd", "yi_grad", "z_diff", "yf_div", "xf_matvec", "pcg_update"]
+# algorithmic bytes of one launch per member, in units of one field (4 n0 n1 n2 bytes): every
+# input read once and every output written once
+KERNEL_FIELDS = {"xi_pupd": 5, "yi_grad": 5, "z_diff": 7, "yf_div": 5, "xf_matvec": 4, "pcg_update": 6}
 
 _lib = None
 

@@ -372,6 +372,17 @@ int glia_rd_stats(glia_rd_t h, int member, glia_rd_stats_t* out) {
   return 0;
 }
 
+int glia_rd_profile(glia_rd_t h, int enable) {
+  GLIA_H(h);
+  return ret(h, h->e->profile_enable(enable != 0));
+}
+
+int glia_rd_profile_get(glia_rd_t h, int kind, double* ms, long long* launches) {
+  GLIA_H(h);
+  if (!ms || !launches) return GLIA_RD_ERR_ARG;
+  return ret(h, h->e->profile_get(kind, ms, launches));
+}
+
 int glia_rd_reset_stats(glia_rd_t h) {
   GLIA_H(h);
   return ret(h, h->e->reset_counters());

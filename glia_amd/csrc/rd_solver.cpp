@@ -208,6 +208,11 @@ int RDEnsemble::upload_members() {
 
 RDEnsemble::~RDEnsemble() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& r : prof_recs_) {
+    (void)hipEventDestroy(r.a);
+    if (r.b) (void)hipEventDestroy(r.b);
+  }
+  for (auto e : prof_pool_) (void)hipEventDestroy(e);
   for (int m = 0; m < (int)mh_.size(); m++) {
     for (int s = 0; s < CS_NUM; s++) (void)hipFree(mh_[m].c[s]);
     for (int s = 0; s < RS_NUM; s++) (void)hipFree(mh_[m].r[s]);
@@ -559,13 +564,25 @@ int RDEnsemble::cn_solve(int sx, double tau) {
   };
   for (int it = 0; it < cfg_.maxit && !all_done; it++) {
     stage = it + 1;
+    prof_mark(PK_XI_PUPD, true);
     GLIA_TRYC(launch_xi_grad(c, CS_P, CS_R, CS_T0, CS_T1, true));
+    prof_mark(PK_XI_PUPD, false);
+    prof_mark(PK_YI_GRAD, true);
     GLIA_TRYC(launch_yi_grad(c, CS_T0, CS_T1, CS_T2, CS_T3, CS_T4, -1));
+    prof_mark(PK_YI_GRAD, false);
+    prof_mark(PK_Z_DIFF, true);
     GLIA_TRYC(launch_z_diff(c, CS_T2, CS_T3, CS_T4));
+    prof_mark(PK_Z_DIFF, false);
+    prof_mark(PK_YF_DIV, true);
     GLIA_TRYC(launch_yf_div(c, CS_T2, CS_T3, CS_T4, CS_T0, CS_T1));
+    prof_mark(PK_YF_DIV, false);
+    prof_mark(PK_XF_MATVEC, true);
     GLIA_TRYC(launch_xf_div(c, XFM_MATVEC, CS_T0, CS_T1, CS_P, CS_W));
+    prof_mark(PK_XF_MATVEC, false);
     GLIA_TRYC(launch_pcg_fin_dpi(c, nbx));
+    prof_mark(PK_PCG_UPDATE, true);
     GLIA_TRYC(launch_pcg_update(c, sx));
+    prof_mark(PK_PCG_UPDATE, false);
     GLIA_TRYC(launch_pcg_fin_update(c, pcg_update_blocks(), hflag_dev_, stage % kFlagSlots));
     GLIA_HIP(hipEventRecord(ev_[stage % kFlagSlots], stream_));
     while (!all_done && checked < stage - kLookahead) {
@@ -882,7 +899,7 @@ int RDEnsemble::objective_and_gradient(int mode, const std::vector<std::vector<d
     GLIA_TRYC(launch_dot_real(c, RS_C0, RS_C0));
     GLIA_TRYC(launch_fin_state(c, ew_blocks(), FINK_C0SQ, 0.0));
   }
-  GLIA_TRYC(launch_state_reset(c, 3));
+  GLIA_TRYC(launch_state_reset(c, 2));  // gradient accumulators; CG counters reset by the caller
   if ((rc = solve_state(0, want_grad))) return rc;
   std::vector<std::vector<double>> phit;
   if (want_grad) {
@@ -911,6 +928,73 @@ int RDEnsemble::objective_and_gradient(int mode, const std::vector<std::vector<d
       dJ[m][np_[m] + 1] = (mode == 0) ? 0.0 : h3 * cfg_.dt * s.gr;
     }
   }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// profiling: event pairs around the per-iteration kernels, summed when queried
+// ---------------------------------------------------------------------------------------------
+hipEvent_t RDEnsemble::prof_event() {
+  if (!prof_pool_.empty()) {
+    hipEvent_t e = prof_pool_.back();
+    prof_pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void RDEnsemble::prof_mark(int kind, bool begin) {
+  if (!prof_on_) return;
+  hipEvent_t e = prof_event();
+  if (!e) return;
+  (void)hipEventRecord(e, stream_);
+  if (begin) {
+    prof_recs_.push_back({kind, e, nullptr});
+  } else if (!prof_recs_.empty() && prof_recs_.back().b == nullptr && prof_recs_.back().kind == kind) {
+    prof_recs_.back().b = e;
+  } else {
+    prof_pool_.push_back(e);
+  }
+  if (prof_recs_.size() > 4096) (void)prof_collect();
+}
+
+int RDEnsemble::prof_collect() {
+  if (prof_recs_.empty()) return 0;
+  GLIA_HIP(hipStreamSynchronize(stream_));
+  for (auto& r : prof_recs_) {
+    if (r.b) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+        prof_ms_[r.kind] += ms;
+        prof_n_[r.kind] += 1;
+      }
+      prof_pool_.push_back(r.b);
+    }
+    prof_pool_.push_back(r.a);
+  }
+  prof_recs_.clear();
+  return 0;
+}
+
+int RDEnsemble::profile_enable(bool on) {
+  int rc = prof_collect();
+  if (rc) return rc;
+  prof_on_ = on;
+  for (int k = 0; k < PK_NUM; k++) {
+    prof_ms_[k] = 0;
+    prof_n_[k] = 0;
+  }
+  return 0;
+}
+
+int RDEnsemble::profile_get(int kind, double* ms, long long* count) {
+  if (kind < 0 || kind >= PK_NUM) return fail(-2, "bad profile kind");
+  int rc = prof_collect();
+  if (rc) return rc;
+  *ms = prof_ms_[kind];
+  *count = prof_n_[kind];
   return 0;
 }
 

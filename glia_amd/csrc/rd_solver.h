@@ -31,6 +31,9 @@ struct SolverConfig {
   hipStream_t stream = nullptr;
 };
 
+// kernel kinds timed by the profiling mode (HIP events around each launch on the solver stream)
+enum ProfKind { PK_XI_PUPD = 0, PK_YI_GRAD, PK_Z_DIFF, PK_YF_DIV, PK_XF_MATVEC, PK_PCG_UPDATE, PK_NUM };
+
 struct MemberStats {
   int its_total, solves, worst_reason, last_its, last_reason;
   double kavg, pc, filter_sum, m1, c0sq;
@@ -81,6 +84,10 @@ class RDEnsemble {
   int sync();
 
   double lebesgue() const { return h_[0] * h_[1] * h_[2]; }
+
+  // profiling mode: per-kind launch count and summed event time (ms) since enabled
+  int profile_enable(bool on);
+  int profile_get(int kind, double* ms, long long* count);
 
  private:
   int alloc_member(int m);
@@ -141,6 +148,19 @@ class RDEnsemble {
   static constexpr int kLookahead = 2;
   hipEvent_t ev_[kFlagSlots];
   bool ev_ok_ = false;
+
+  struct ProfRec {
+    int kind;
+    hipEvent_t a, b;
+  };
+  bool prof_on_ = false;
+  std::vector<ProfRec> prof_recs_;
+  std::vector<hipEvent_t> prof_pool_;
+  double prof_ms_[PK_NUM] = {0};
+  long long prof_n_[PK_NUM] = {0};
+  hipEvent_t prof_event();
+  void prof_mark(int kind, bool begin);
+  int prof_collect();
 
   int npool_ = 0;       // number of pool buffers per member
   bool hist_all_ = true;

@@ -227,6 +227,19 @@ class RDSolver:
         self._chk(self.L.glia_rd_stats(self.h, member, ctypes.byref(s)))
         return {f: getattr(s, f) for f, _ in _lib.Stats._fields_}
 
+    def profile(self, enable=True):
+        self._chk(self.L.glia_rd_profile(self.h, int(bool(enable))))
+
+    def profile_get(self):
+        """{kernel: (launches, total ms)} of the timed CG kernels since profile(True)."""
+        out = {}
+        for k, name in enumerate(_lib.KERNELS):
+            ms = ctypes.c_double()
+            cnt = ctypes.c_longlong()
+            self._chk(self.L.glia_rd_profile_get(self.h, k, ctypes.byref(ms), ctypes.byref(cnt)))
+            out[name] = (cnt.value, ms.value)
+        return out
+
     def reset_stats(self):
         self._chk(self.L.glia_rd_reset_stats(self.h))
 

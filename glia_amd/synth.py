@@ -16,15 +16,31 @@ def _ellipsoid(n, center, radii):
     return (X + Y + Z) <= 1.0
 
 
+try:
+    import scipy.fft as _fft
+
+    def _rfftn(a):
+        return _fft.rfftn(a, workers=-1)
+
+    def _irfftn(a, n):
+        return _fft.irfftn(a, s=n, axes=(0, 1, 2), workers=-1)
+except ImportError:  # numpy only
+    def _rfftn(a):
+        return np.fft.rfftn(a)
+
+    def _irfftn(a, n):
+        return np.fft.irfftn(a, s=n, axes=(0, 1, 2))
+
+
 def gaussian_smooth(f, sigma_vox):
     """Periodic Gaussian smoothing (sigma in voxels) through the FFT."""
     n = f.shape
-    F = np.fft.rfftn(f)
+    F = _rfftn(f)
     k = [np.fft.fftfreq(n[0]) * n[0], np.fft.fftfreq(n[1]) * n[1], np.fft.rfftfreq(n[2]) * n[2]]
     w2 = ((2 * np.pi * k[0] / n[0]) ** 2)[:, None, None] + ((2 * np.pi * k[1] / n[1]) ** 2)[None, :, None] \
         + ((2 * np.pi * k[2] / n[2]) ** 2)[None, None, :]
     F *= np.exp(-0.5 * sigma_vox ** 2 * w2)
-    return np.fft.irfftn(F, s=n, axes=(0, 1, 2))
+    return _irfftn(F, n)
 
 
 def brain(n, seed):

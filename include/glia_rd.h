@@ -158,6 +158,20 @@ int glia_rd_objective_and_gradient(glia_rd_t h, int mode, const double* x, doubl
 int glia_rd_stats(glia_rd_t h, int member, glia_rd_stats_t* out);
 int glia_rd_reset_stats(glia_rd_t h);
 
+/* kernel timing (the reference's Event timers, 3rdparty/timings/EventTimings.hpp:31-140, for the
+   CG kernels): with profiling on, HIP events on the handle's stream bracket every launch of the
+   kinds below; glia_rd_profile_get returns launches and summed milliseconds since enabling. */
+enum {
+  GLIA_RD_K_XI_PUPD = 0,   /* X pass: p update + inverse transforms of p and i w_x p */
+  GLIA_RD_K_YI_GRAD = 1,   /* Y pass: inverse, three gradient components */
+  GLIA_RD_K_Z_DIFF = 2,    /* Z pass: C2R, k * grad, R2C for three components */
+  GLIA_RD_K_YF_DIV = 3,    /* Y pass: forward, partial divergence */
+  GLIA_RD_K_XF_MATVEC = 4, /* X pass: forward, w = p - tau/2 D p, <p, w> */
+  GLIA_RD_K_PCG_UPDATE = 5 /* x, r update with ||B r||^2 and <B r, r> */
+};
+int glia_rd_profile(glia_rd_t h, int enable);
+int glia_rd_profile_get(glia_rd_t h, int kind, double* ms, long long* launches);
+
 #ifdef __cplusplus
 }
 #endif
