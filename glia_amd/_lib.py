@@ -7,7 +7,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libglia_rd.so")
+# GLIA_RD_LIBRARY may point at another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("GLIA_RD_LIBRARY") or os.path.join(_HERE, "libglia_rd.so")
 
 HOST, DEVICE = 0, 1
 OK, ERR_ARG, ERR_STATE, ERR_UNSUPPORTED, ERR_POOL, ERR_NOHISTORY = 0, -2, -3, -4, -5, -6

@@ -19,7 +19,19 @@
 
 #include <hip/hip_runtime.h>
 
-#include "fft_engine.h"
+#include "fft2s.h"
+
+// minimum waves per SIMD requested from the compiler for the transform kernels (caps VGPRs)
+#ifndef GLIA_COL_MINW
+#define GLIA_COL_MINW 2
+#endif
+#ifndef GLIA_Z_MINW
+#define GLIA_Z_MINW 2
+#endif
+// threads per block of the line (Z pass) kernels
+#ifndef GLIA_Z_TPB
+#define GLIA_Z_TPB 256
+#endif
 
 namespace glia {
 using namespace dev;
@@ -27,8 +39,38 @@ using namespace dev;
 // ---------------------------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------------------------
+// pointers into device global memory (address space 1): loads and stores through them compile to
+// global_* instructions with a uniform base and 32-bit lane offsets instead of flat accesses
+#define GLIA_GLOBAL __attribute__((address_space(1)))
+typedef float v2f __attribute__((ext_vector_type(2)));
+struct GRef {
+  GLIA_GLOBAL v2f* p;
+  __device__ __forceinline__ operator float2() const {
+    const v2f a = *p;
+    return make_float2(a.x, a.y);
+  }
+  __device__ __forceinline__ GRef& operator=(float2 v) {
+    v2f a;
+    a.x = v.x;
+    a.y = v.y;
+    *p = a;
+    return *this;
+  }
+};
+struct GPtr {
+  GLIA_GLOBAL v2f* p;
+  __device__ __forceinline__ GRef operator[](unsigned i) const { return GRef{p + i}; }
+  __device__ __forceinline__ GPtr operator+(int64_t o) const { return GPtr{p + o}; }
+  __device__ __forceinline__ explicit operator bool() const { return p != nullptr; }
+};
+__device__ __forceinline__ GPtr gptr(const void* q) { return GPtr{(GLIA_GLOBAL v2f*)q}; }
+
 // complex slot s of a member: work slots below CS_NUM, state-pool entries (history/checkpoints) above
-__device__ __forceinline__ float2* cptr(const MemberDev& M, int s) { return (s < CS_NUM) ? M.c[s] : M.hist[s - CS_NUM]; }
+__device__ __forceinline__ GPtr cptr(const MemberDev& M, int s) { return gptr((s < CS_NUM) ? M.c[s] : M.hist[s - CS_NUM]); }
+// the same slot as a plain pointer (elementwise kernels)
+__device__ __forceinline__ float2* craw(const MemberDev& M, int s) { return (s < CS_NUM) ? M.c[s] : M.hist[s - CS_NUM]; }
+// a real field viewed as float2 pairs (x[2n], x[2n+1])
+__device__ __forceinline__ GPtr rptr2(const MemberDev& M, int r) { return gptr(M.r[r]); }
 
 // wave number with the Nyquist mode zeroed (reference src/cuda/SpectralOperators.cu:63-72)
 __device__ __forceinline__ float wnum(int k, int n) {
@@ -96,17 +138,32 @@ __device__ __forceinline__ double sum_partials(const double* part, int nb, int n
 }
 
 // ---------------------------------------------------------------------------------------------
-// column kernels (X and Y passes)
+// column kernels (X and Y passes): a tile of 16 adjacent columns (128-byte rows), NT threads per
+// column; thread (col = tid & 15, t = tid >> 4) loads its stage-1 rows straight from HBM, the
+// two-stage transform exchanges once through LDS, results are stored from registers.
 // ---------------------------------------------------------------------------------------------
 template <int L>
-struct ColK {
-  static constexpr int NT = LineCfg<L>::NT;
+struct CK {
+  static constexpr int R1 = Plan2<L>::R1, R2 = Plan2<L>::R2, NT = Plan2<L>::NT;
   static constexpr int TPB = 16 * NT;
-  static constexpr int LP = L + 1;
+  static constexpr int VPT = L / NT;
+  static constexpr int CP = XPitch<L>::value;
 };
 
+// row of stage-1 input slot j of thread t: j = b*R1 + n1 -> R2*n1 + t + b*NT
+template <int R1, int R2, int NT>
+__device__ __forceinline__ int in_row(int t, int j) {
+  return R2 * (j % R1) + t + (j / R1) * NT;
+}
+// row of output slot j of thread t: j = b*R2 + k2 -> t + b*NT + R1*k2
+template <int R1, int R2, int NT>
+__device__ __forceinline__ int out_row(int t, int j) {
+  return t + (j / R2) * NT + R1 * (j % R2);
+}
+
 struct Tile {
-  int64_t base, rs;
+  int64_t base;   // element offset of the tile's first entry (uniform)
+  unsigned rs;    // row stride in elements
   int o, c0;
 };
 
@@ -117,165 +174,153 @@ __device__ __forceinline__ Tile col_tile(int dir, const Dims& d) {
   t.c0 = (blockIdx.x % ntc) << 4;
   if (dir == 0) {
     t.base = (int64_t)t.o * d.m2 + t.c0;
-    t.rs = (int64_t)d.n1 * d.m2;
+    t.rs = (unsigned)(d.n1 * d.m2);
   } else {
     t.base = (int64_t)t.o * d.n1 * d.m2 + t.c0;
-    t.rs = d.m2;
+    t.rs = (unsigned)d.m2;
   }
   return t;
 }
 
+#define GLIA_COL_PRELUDE(L)                                  \
+  using K = CK<L>;                                           \
+  constexpr int R1 = K::R1, R2 = K::R2, NT = K::NT;          \
+  __shared__ float2 ex[16 * K::CP];                          \
+  const int col = threadIdx.x & 15, t = threadIdx.x >> 4;    \
+  float2* exc = ex + col * K::CP;
+
 // X pass, inverse: U = IFFT_x(p), V = IFFT_x(i w_x p).  With PUPD the search direction is
 // first updated in place: p = B r (i == 0) or p = B r + (beta/betaold) p.
 template <int L, bool PUPD>
-__global__ void __launch_bounds__(ColK<L>::TPB)
+__global__ void __launch_bounds__(CK<L>::TPB, GLIA_COL_MINW)
 k_xi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_in, int s_r, int s_u, int s_v) {
   const MemberDev& M = mem[blockIdx.y];
   const PcgState* st = M.st;
   if (st->done) return;
-  constexpr int LP = ColK<L>::LP, TPB = ColK<L>::TPB, NT = ColK<L>::NT;
-  __shared__ float2 A[16 * LP];
-  __shared__ float2 B[16 * LP];
+  GLIA_COL_PRELUDE(L)
   const Tile tl = col_tile(0, d);
-  const float wy = wnum(tl.o, d.n1);
-  float2* pin = cptr(M, s_in);
-  const float2* rr = cptr(M, s_r);
-  float2* U = cptr(M, s_u);
-  float2* V = cptr(M, s_v);
+  const float wy = wnum(tl.o, d.n1), wz = (float)(tl.c0 + col);
+  GPtr pin = cptr(M, s_in) + tl.base;
+  GPtr rr = cptr(M, s_r) + tl.base;
+  GPtr U = cptr(M, s_u) + tl.base;
+  GPtr V = cptr(M, s_v) + tl.base;
   bool first = false;
-  float b = 0.f, pc = 0.f;
+  float bc = 0.f, pc = 0.f;
   if (PUPD) {
     first = (st->i == 0);
-    b = (float)st->bcoef;
+    bc = (float)st->bcoef;
     pc = (float)st->pc;
   }
-  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
-    const int row = e >> 4, col = e & 15;
-    const int64_t g = tl.base + (int64_t)row * tl.rs + col;
+  float2 a[K::VPT], b[K::VPT];
+#pragma unroll
+  for (int j = 0; j < K::VPT; j++) {
+    const int row = in_row<R1, R2, NT>(t, j);
+    const unsigned g = (unsigned)row * tl.rs + col;
     const float wx = wnum(row, d.n0);
     float2 p;
     if (PUPD) {
-      const float f = precf(pc, wx, wy, (float)(tl.c0 + col));
-      const float2 z = cscale(rr[g], f);
+      const float2 z = cscale(rr[g], precf(pc, wx, wy, wz));
       if (first) {
         p = z;
       } else {
         const float2 po = pin[g];
-        p = make_float2(z.x + b * po.x, z.y + b * po.y);
+        p = make_float2(z.x + bc * po.x, z.y + bc * po.y);
       }
       pin[g] = p;
     } else {
       p = pin[g];
     }
-    A[col * LP + row] = p;
-    B[col * LP + row] = mul_i(cscale(p, wx));
+    a[j] = p;
+    b[j] = mul_i(cscale(p, wx));
   }
+  fft2s<R1, R2, NT, +1>(a, exc, t, tw);
   __syncthreads();
-  const int col = threadIdx.x / NT, t = threadIdx.x % NT;
-  fft_line<L, +1>(A + col * LP, 1, t, tw, true);
-  fft_line<L, +1>(B + col * LP, 1, t, tw, true);
-  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
-    const int row = e >> 4, cc = e & 15;
-    const int64_t g = tl.base + (int64_t)row * tl.rs + cc;
-    U[g] = A[cc * LP + row];
-    V[g] = B[cc * LP + row];
+  fft2s<R1, R2, NT, +1>(b, exc, t, tw);
+#pragma unroll
+  for (int j = 0; j < K::VPT; j++) {
+    const unsigned g = (unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col;
+    U[g] = a[j];
+    V[g] = b[j];
   }
 }
 
 // Y pass, inverse, gradient: Gx = IFFT_y(V), Gy = IFFT_y(i w_y U), Gz = i w_z IFFT_y(U),
 // and optionally G0 = IFFT_y(U).  Gx must not alias U.
 template <int L, bool WITH0>
-__global__ void __launch_bounds__(ColK<L>::TPB)
+__global__ void __launch_bounds__(CK<L>::TPB, GLIA_COL_MINW)
 k_yi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_u, int s_v, int s_gx, int s_gy,
           int s_gz, int s_g0) {
   const MemberDev& M = mem[blockIdx.y];
   if (M.st->done) return;
-  constexpr int LP = ColK<L>::LP, TPB = ColK<L>::TPB, NT = ColK<L>::NT;
-  __shared__ float2 A[16 * LP];
-  __shared__ float2 B[16 * LP];
+  GLIA_COL_PRELUDE(L)
   const Tile tl = col_tile(1, d);
-  const float2* U = cptr(M, s_u);
-  const float2* V = cptr(M, s_v);
-  float2* Gx = cptr(M, s_gx);
-  float2* Gy = cptr(M, s_gy);
-  float2* Gz = cptr(M, s_gz);
-  float2* G0 = WITH0 ? cptr(M, s_g0) : nullptr;
-  const int col = threadIdx.x / NT, t = threadIdx.x % NT;
-  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
-    const int row = e >> 4, cc = e & 15;
-    A[cc * LP + row] = V[tl.base + (int64_t)row * tl.rs + cc];
+  const float wz = (float)(tl.c0 + col);
+  GPtr U = cptr(M, s_u) + tl.base;
+  GPtr V = cptr(M, s_v) + tl.base;
+  GPtr Gx = cptr(M, s_gx) + tl.base;
+  GPtr Gy = cptr(M, s_gy) + tl.base;
+  GPtr Gz = cptr(M, s_gz) + tl.base;
+  GPtr G0 = WITH0 ? cptr(M, s_g0) + tl.base : gptr(nullptr);
+  float2 a[K::VPT], b[K::VPT];
+#pragma unroll
+  for (int j = 0; j < K::VPT; j++) a[j] = V[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
+  fft2s<R1, R2, NT, +1>(a, exc, t, tw);
+#pragma unroll
+  for (int j = 0; j < K::VPT; j++) Gx[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
+#pragma unroll
+  for (int j = 0; j < K::VPT; j++) {
+    const int row = in_row<R1, R2, NT>(t, j);
+    a[j] = U[(unsigned)row * tl.rs + col];
+    b[j] = mul_i(cscale(a[j], wnum(row, d.n1)));
   }
   __syncthreads();
-  fft_line<L, +1>(A + col * LP, 1, t, tw, true);
-  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
-    const int row = e >> 4, cc = e & 15;
-    Gx[tl.base + (int64_t)row * tl.rs + cc] = A[cc * LP + row];
-  }
+  fft2s<R1, R2, NT, +1>(a, exc, t, tw);
   __syncthreads();
-  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
-    const int row = e >> 4, cc = e & 15;
-    const float2 u = U[tl.base + (int64_t)row * tl.rs + cc];
-    A[cc * LP + row] = u;
-    B[cc * LP + row] = mul_i(cscale(u, wnum(row, d.n1)));
-  }
-  __syncthreads();
-  fft_line<L, +1>(A + col * LP, 1, t, tw, true);
-  fft_line<L, +1>(B + col * LP, 1, t, tw, true);
-  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
-    const int row = e >> 4, cc = e & 15;
-    const int64_t g = tl.base + (int64_t)row * tl.rs + cc;
-    const float2 a = A[cc * LP + row];
-    const float wz = (float)(tl.c0 + cc);
-    Gz[g] = mul_i(cscale(a, wz));
-    if (WITH0) G0[g] = a;
-    Gy[g] = B[cc * LP + row];
+  fft2s<R1, R2, NT, +1>(b, exc, t, tw);
+#pragma unroll
+  for (int j = 0; j < K::VPT; j++) {
+    const unsigned g = (unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col;
+    Gz[g] = mul_i(cscale(a[j], wz));
+    if (WITH0) G0[g] = a[j];
+    Gy[g] = b[j];
   }
 }
 
 // Y pass, forward, divergence: Yx = FFT_y(Hx), Rr = FFT_y(Q) + i w_y FFT_y(Hy).
 // Yx must not alias Q or Hy.
 template <int L>
-__global__ void __launch_bounds__(ColK<L>::TPB)
+__global__ void __launch_bounds__(CK<L>::TPB, GLIA_COL_MINW)
 k_yf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_hx, int s_hy, int s_q, int s_yx,
          int s_rr) {
   const MemberDev& M = mem[blockIdx.y];
   if (M.st->done) return;
-  constexpr int LP = ColK<L>::LP, TPB = ColK<L>::TPB, NT = ColK<L>::NT;
-  __shared__ float2 A[16 * LP];
-  __shared__ float2 B[16 * LP];
+  GLIA_COL_PRELUDE(L)
   const Tile tl = col_tile(1, d);
-  const float2* Hx = cptr(M, s_hx);
-  const float2* Hy = cptr(M, s_hy);
-  const float2* Q = cptr(M, s_q);
-  float2* Yx = cptr(M, s_yx);
-  float2* Rr = cptr(M, s_rr);
-  const int col = threadIdx.x / NT, t = threadIdx.x % NT;
-  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
-    const int row = e >> 4, cc = e & 15;
-    A[cc * LP + row] = Hx[tl.base + (int64_t)row * tl.rs + cc];
+  GPtr Hx = cptr(M, s_hx) + tl.base;
+  GPtr Hy = cptr(M, s_hy) + tl.base;
+  GPtr Q = cptr(M, s_q) + tl.base;
+  GPtr Yx = cptr(M, s_yx) + tl.base;
+  GPtr Rr = cptr(M, s_rr) + tl.base;
+  float2 a[K::VPT], b[K::VPT];
+#pragma unroll
+  for (int j = 0; j < K::VPT; j++) a[j] = Hx[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
+  fft2s<R1, R2, NT, -1>(a, exc, t, tw);
+#pragma unroll
+  for (int j = 0; j < K::VPT; j++) Yx[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
+#pragma unroll
+  for (int j = 0; j < K::VPT; j++) {
+    const unsigned g = (unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col;
+    a[j] = Q[g];
+    b[j] = Hy[g];
   }
   __syncthreads();
-  fft_line<L, -1>(A + col * LP, 1, t, tw, true);
-  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
-    const int row = e >> 4, cc = e & 15;
-    Yx[tl.base + (int64_t)row * tl.rs + cc] = A[cc * LP + row];
-  }
+  fft2s<R1, R2, NT, -1>(a, exc, t, tw);
   __syncthreads();
-  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
-    const int row = e >> 4, cc = e & 15;
-    const int64_t g = tl.base + (int64_t)row * tl.rs + cc;
-    A[cc * LP + row] = Q[g];
-    B[cc * LP + row] = Hy[g];
-  }
-  __syncthreads();
-  fft_line<L, -1>(A + col * LP, 1, t, tw, true);
-  fft_line<L, -1>(B + col * LP, 1, t, tw, true);
-  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
-    const int row = e >> 4, cc = e & 15;
-    const int64_t g = tl.base + (int64_t)row * tl.rs + cc;
-    const float wy = wnum(row, d.n1);
-    const float2 a = A[cc * LP + row], bq = B[cc * LP + row];
-    Rr[g] = cadd(a, mul_i(cscale(bq, wy)));
+  fft2s<R1, R2, NT, -1>(b, exc, t, tw);
+#pragma unroll
+  for (int j = 0; j < K::VPT; j++) {
+    const int row = out_row<R1, R2, NT>(t, j);
+    Rr[(unsigned)row * tl.rs + col] = cadd(a[j], mul_i(cscale(b[j], wnum(row, d.n1))));
   }
 }
 
@@ -287,43 +332,38 @@ k_yf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
 enum { XF_MATVEC = 0, XF_SETUP = 1, XF_D = 2 };
 
 template <int L, int MODE>
-__global__ void __launch_bounds__(ColK<L>::TPB)
+__global__ void __launch_bounds__(CK<L>::TPB, GLIA_COL_MINW)
 k_xf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_yx, int s_rr, int s_p, int s_out) {
   const MemberDev& M = mem[blockIdx.y];
   const PcgState* st = M.st;
   if (st->done) return;
-  constexpr int LP = ColK<L>::LP, TPB = ColK<L>::TPB, NT = ColK<L>::NT;
-  constexpr int NQ = (MODE == XF_SETUP) ? 3 : 1;
-  __shared__ float2 A[16 * LP];
-  __shared__ float2 B[16 * LP];
+  GLIA_COL_PRELUDE(L)
   const Tile tl = col_tile(0, d);
-  const float2* Yx = cptr(M, s_yx);
-  const float2* Rr = cptr(M, s_rr);
-  const float2* P = cptr(M, s_p);
-  float2* O = cptr(M, s_out);
+  GPtr Yx = cptr(M, s_yx) + tl.base;
+  GPtr Rr = cptr(M, s_rr) + tl.base;
+  GPtr P = cptr(M, s_p) + tl.base;
+  GPtr O = cptr(M, s_out) + tl.base;
   const float tau = (float)st->tau;
   const float pc = (float)st->pc;
-  const float wy = wnum(tl.o, d.n1);
-  const int col = threadIdx.x / NT, t = threadIdx.x % NT;
-  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
-    const int row = e >> 4, cc = e & 15;
-    const int64_t g = tl.base + (int64_t)row * tl.rs + cc;
-    A[cc * LP + row] = Yx[g];
-    B[cc * LP + row] = Rr[g];
-  }
-  __syncthreads();
-  fft_line<L, -1>(A + col * LP, 1, t, tw, true);
-  fft_line<L, -1>(B + col * LP, 1, t, tw, true);
-  double acc[NQ];
+  const float wy = wnum(tl.o, d.n1), wz = (float)(tl.c0 + col);
+  const double wgt = (tl.c0 + col == 0) ? 1.0 : 2.0;
+  float2 a[K::VPT], b[K::VPT];
 #pragma unroll
-  for (int q = 0; q < NQ; q++) acc[q] = 0.0;
-  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
-    const int row = e >> 4, cc = e & 15;
-    const int64_t g = tl.base + (int64_t)row * tl.rs + cc;
+  for (int j = 0; j < K::VPT; j++) {
+    const unsigned g = (unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col;
+    a[j] = Yx[g];
+    b[j] = Rr[g];
+  }
+  fft2s<R1, R2, NT, -1>(a, exc, t, tw);
+  __syncthreads();
+  fft2s<R1, R2, NT, -1>(b, exc, t, tw);
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < K::VPT; j++) {
+    const int row = out_row<R1, R2, NT>(t, j);
+    const unsigned g = (unsigned)row * tl.rs + col;
     const float wx = wnum(row, d.n0);
-    const float2 D = cadd(mul_i(cscale(A[cc * LP + row], wx)), B[cc * LP + row]);
-    const int kz = tl.c0 + cc;
-    const double wgt = (kz == 0) ? 1.0 : 2.0;
+    const float2 D = cadd(mul_i(cscale(a[j], wx)), b[j]);
     if (MODE == XF_MATVEC) {
       const float2 p = P[g];
       const float2 w = make_float2(p.x - 0.5f * tau * D.x, p.y - 0.5f * tau * D.y);
@@ -334,162 +374,158 @@ k_xf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
       const float2 bb = make_float2(x.x + 0.5f * tau * D.x, x.y + 0.5f * tau * D.y);
       const float2 r = cscale(D, tau);
       O[g] = r;
-      const float f = precf(pc, wx, wy, (float)kz);
-      const double pb2 = (double)f * f * ((double)bb.x * bb.x + (double)bb.y * bb.y);
+      const float f = precf(pc, wx, wy, wz);
       const double r2 = (double)r.x * r.x + (double)r.y * r.y;
-      acc[0] += wgt * pb2;
+      acc[0] += wgt * (double)f * f * ((double)bb.x * bb.x + (double)bb.y * bb.y);
       acc[1] += wgt * (double)f * f * r2;
       acc[2] += wgt * (double)f * r2;
     } else {
       O[g] = D;
     }
   }
-  if (MODE != XF_D) block_reduce_store<NQ>(acc, M.part + (int64_t)blockIdx.x * NQ);
+  if (MODE == XF_SETUP) block_reduce_store<3>(acc, M.part + (int64_t)blockIdx.x * 3);
+  if (MODE == XF_MATVEC) {
+    double a1[1] = {acc[0]};
+    block_reduce_store<1>(a1, M.part + blockIdx.x);
+  }
 }
 
 // plain column transform: out = FFT_dir(in) along axis DIR (S = -1 forward, +1 inverse)
 template <int L, int DIR, int S>
-__global__ void __launch_bounds__(ColK<L>::TPB)
+__global__ void __launch_bounds__(CK<L>::TPB, GLIA_COL_MINW)
 k_col_plain(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_in, int s_out, int honor_done) {
   const MemberDev& M = mem[blockIdx.y];
   if (honor_done && M.st->done) return;
-  constexpr int LP = ColK<L>::LP, TPB = ColK<L>::TPB, NT = ColK<L>::NT;
-  __shared__ float2 A[16 * LP];
+  GLIA_COL_PRELUDE(L)
   const Tile tl = col_tile(DIR, d);
-  const float2* I = cptr(M, s_in);
-  float2* O = cptr(M, s_out);
-  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
-    const int row = e >> 4, cc = e & 15;
-    A[cc * LP + row] = I[tl.base + (int64_t)row * tl.rs + cc];
-  }
-  __syncthreads();
-  const int col = threadIdx.x / NT, t = threadIdx.x % NT;
-  fft_line<L, S>(A + col * LP, 1, t, tw, true);
-  for (int e = threadIdx.x; e < 16 * L; e += TPB) {
-    const int row = e >> 4, cc = e & 15;
-    O[tl.base + (int64_t)row * tl.rs + cc] = A[cc * LP + row];
-  }
+  GPtr I = cptr(M, s_in) + tl.base;
+  GPtr O = cptr(M, s_out) + tl.base;
+  float2 a[K::VPT];
+#pragma unroll
+  for (int j = 0; j < K::VPT; j++) a[j] = I[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
+  fft2s<R1, R2, NT, S>(a, exc, t, tw);
+#pragma unroll
+  for (int j = 0; j < K::VPT; j++) O[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
 }
 
 // ---------------------------------------------------------------------------------------------
-// line kernels (Z pass): real n2 <-> packed m2 = n2/2 complex via one m2-point complex FFT
+// line kernels (Z pass): real n2 <-> packed m2 = n2/2 complex through one m2-point complex FFT.
+// NL lines per block, NT threads per line.  The inverse uses plan (R1, R2); its register output
+// is the input layout of the forward plan (R2, R1), so pointwise work between the two needs no
+// exchange.  Thread t of a line holds z[n] for n = out_row(t, j).
 // ---------------------------------------------------------------------------------------------
 template <int M>
 struct ZK {
-  static constexpr int NT = LineCfg<M>::NT;
-  static constexpr int NL = (M == 128) ? 32 : (M == 96) ? 32 : (M == 64) ? 64 : (M == 48) ? 64 : (M == 32) ? 128 : 256;
+  static constexpr int R1 = Plan2<M>::R1, R2 = Plan2<M>::R2, NT = Plan2<M>::NT;
+  static constexpr int NL = GLIA_Z_TPB / NT;
   static constexpr int TPB = NL * NT;
-  static constexpr int MP = M + 1;
-  static constexpr int KPT = (M + NT - 1) / NT;  // entries per thread
+  static constexpr int LP = XPitch<M>::value;
+  static constexpr int VPT = M / NT;
 };
 
-// coalesced load of the block's NL packed lines into LDS
+// coalesced load of the block's NL packed lines into the padded line buffers
 template <int M>
-__device__ __forceinline__ void zload(float2* buf, const float2* __restrict__ src, int64_t base) {
+__device__ __forceinline__ void zl_load(float2* lb, GPtr src, int64_t base) {
+  GPtr sb = src + base;
   for (int e = threadIdx.x; e < ZK<M>::NL * M; e += ZK<M>::TPB) {
     const int li = e / M, k = e - li * M;
-    buf[li * ZK<M>::MP + k] = src[base + e];
+    lb[li * ZK<M>::LP + xpad(k)] = sb[e];
   }
 }
 
-// packed spectrum X -> Z'[k] = (X[k] + conj X[M-k]) + i e^{+2 pi i k/n2} (X[k] - conj X[M-k]),
-// with X[0] = Re P[0], X[M] = Im P[0]; afterwards an inverse M-point FFT gives
-// z[n] = x[2n] + i x[2n+1] (unnormalized C2R of length n2).
+// packed spectrum entry k -> Z'[k] = (X[k] + conj X[M-k]) + i e^{+2 pi i k/n2} (X[k] - conj X[M-k]),
+// with X[0] = Re P[0], X[M] = Im P[0]; an inverse M-point FFT of Z' gives z[n] = x[2n] + i x[2n+1]
+// (unnormalized C2R of length n2).
 template <int M>
-__device__ __forceinline__ void zprep_c2r(float2* line, int t, const float2* __restrict__ tw2) {
-  constexpr int NT = ZK<M>::NT, KPT = ZK<M>::KPT;
-  float2 v[KPT];
-#pragma unroll
-  for (int j = 0; j < KPT; j++) {
-    const int k = t + j * NT;
-    if (k < M) {
-      if (k == 0) {
-        const float2 p0 = line[0];
-        v[j] = make_float2(p0.x + p0.y, p0.x - p0.y);
-      } else {
-        const float2 a = line[k];
-        const float2 bc = cconj(line[M - k]);
-        const float2 s = cadd(a, bc), df = csub(a, bc);
-        v[j] = cadd(s, mul_i(cmulc(df, tw2[k])));
-      }
-    }
+__device__ __forceinline__ float2 zprep(const float2* line, int k, const float2* __restrict__ tw2) {
+  if (k == 0) {
+    const float2 p0 = line[0];
+    return make_float2(p0.x + p0.y, p0.x - p0.y);
   }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < KPT; j++) {
-    const int k = t + j * NT;
-    if (k < M) line[k] = v[j];
-  }
-  __syncthreads();
+  const float2 a = line[xpad(k)];
+  const float2 bc = cconj(line[xpad(M - k)]);
+  return cadd(cadd(a, bc), mul_i(cmulc(csub(a, bc), tw2[k])));
 }
 
 // after a forward M-point FFT of z[n] = x[2n] + i x[2n+1]: packed spectrum entry k
 template <int M>
-__device__ __forceinline__ float2 zpost_r2c(const float2* line, int k, const float2* __restrict__ tw2) {
+__device__ __forceinline__ float2 zpost(const float2* line, int k, const float2* __restrict__ tw2) {
   if (k == 0) {
     const float2 z0 = line[0];
     return make_float2(z0.x + z0.y, z0.x - z0.y);
   }
-  const float2 a = line[k];
-  const float2 bc = cconj(line[M - k]);
+  const float2 a = line[xpad(k)];
+  const float2 bc = cconj(line[xpad(M - k)]);
   const float2 e = cscale(cadd(a, bc), 0.5f);
   const float2 o = cscale(mul_mi(csub(a, bc)), 0.5f);
   return cadd(e, cmul(o, tw2[k]));
 }
 
+// inverse: the block's packed lines (already in lb) -> v[j] = z[out_row(t, j)] of line li
+template <int M>
+__device__ __forceinline__ void zl_inverse(float2* v, float2* line, int t, const float2* __restrict__ twm,
+                                           const float2* __restrict__ tw2) {
+  constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT;
+#pragma unroll
+  for (int j = 0; j < ZK<M>::VPT; j++) v[j] = zprep<M>(line, in_row<R1, R2, NT>(t, j), tw2);
+  __syncthreads();
+  fft2s<R1, R2, NT, +1>(v, line, t, twm);
+}
+
+// forward: v[j] = z[out_row(t, j)] -> packed spectrum of every line, stored block-cooperatively
+// to dst[base ...] (multiplied by i w_z = i k when mulz)
+template <int M>
+__device__ __forceinline__ void zl_forward_store(float2* v, float2* lb, float2* line, int t, const float2* __restrict__ twm,
+                                                 const float2* __restrict__ tw2, GPtr dst, int64_t base,
+                                                 bool mulz) {
+  GPtr db = dst + base;
+  constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT;
+  __syncthreads();
+  fft2s<R2, R1, NT, -1>(v, line, t, twm);
+  __syncthreads();
+  // output slot j = b*R1 + k2 of the (R2, R1) plan holds Z[t + b*NT + R2*k2]
+#pragma unroll
+  for (int j = 0; j < ZK<M>::VPT; j++) line[xpad(out_row<R2, R1, NT>(t, j))] = v[j];
+  __syncthreads();
+  for (int e = threadIdx.x; e < ZK<M>::NL * M; e += ZK<M>::TPB) {
+    const int li = e / M, k = e - li * M;
+    float2 x = zpost<M>(lb + li * ZK<M>::LP, k, tw2);
+    if (mulz) x = mul_i(cscale(x, (float)k));  // w_z = k' (0 at the packed column)
+    db[e] = x;
+  }
+}
+
 // Z pass of the matvec: for each component d, h_d = k * C2R(G_d)/N^3, out_d = R2C(h_d),
 // the z component multiplied by i w_z.  In place on the three slots.
 template <int M>
-__global__ void __launch_bounds__(ZK<M>::TPB)
+__global__ void __launch_bounds__(ZK<M>::TPB, GLIA_Z_MINW)
 k_z_diff(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ twm, const float2* __restrict__ tw2,
          int s_gx, int s_gy, int s_gz) {
   const MemberDev& Mb = mem[blockIdx.y];
   if (Mb.st->done) return;
-  constexpr int NT = ZK<M>::NT, MP = ZK<M>::MP, KPT = ZK<M>::KPT, NL = ZK<M>::NL;
-  __shared__ float2 buf[NL * MP];
+  constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT, VPT = ZK<M>::VPT;
+  __shared__ float2 lb[ZK<M>::NL * ZK<M>::LP];
   const int li = threadIdx.x / NT, t = threadIdx.x % NT;
-  const int64_t base = (int64_t)blockIdx.x * NL * M;
-  float2* line = buf + li * MP;
-  const float2* kf = reinterpret_cast<const float2*>(Mb.r[RS_K]);
-  float2 kv[KPT];
-#pragma unroll
-  for (int j = 0; j < KPT; j++) {
-    const int n = t + j * NT;
-    kv[j] = (n < M) ? kf[base + li * M + n] : make_float2(0.f, 0.f);
-  }
+  const int64_t base = (int64_t)blockIdx.x * ZK<M>::NL * M;
+  float2* line = lb + li * ZK<M>::LP;
+  GPtr kf = rptr2(Mb, RS_K) + base;
+  const int lo = li * M;
   const float s = (float)d.inv_ng;
+  float2 kv[VPT], v[VPT];
+#pragma unroll
+  for (int j = 0; j < VPT; j++) {
+    const float2 k2 = kf[lo + out_row<R1, R2, NT>(t, j)];
+    kv[j] = make_float2(k2.x * s, k2.y * s);
+  }
   const int slots[3] = {s_gx, s_gy, s_gz};
   for (int comp = 0; comp < 3; comp++) {
-    float2* G = cptr(Mb, slots[comp]);
-    zload<M>(buf, G, base);
+    GPtr G = cptr(Mb, slots[comp]);
+    zl_load<M>(lb, G, base);
     __syncthreads();
-    zprep_c2r<M>(line, t, tw2);
-    fft_line<M, +1>(line, 1, t, twm, true);
+    zl_inverse<M>(v, line, t, twm, tw2);
 #pragma unroll
-    for (int j = 0; j < KPT; j++) {
-      const int n = t + j * NT;
-      if (n < M) {
-        const float2 z = line[n];
-        line[n] = make_float2(kv[j].x * z.x * s, kv[j].y * z.y * s);
-      }
-    }
-    __syncthreads();
-    fft_line<M, -1>(line, 1, t, twm, true);
-    float2 outv[KPT];
-#pragma unroll
-    for (int j = 0; j < KPT; j++) {
-      const int k = t + j * NT;
-      if (k < M) {
-        float2 x = zpost_r2c<M>(line, k, tw2);
-        if (comp == 2) x = mul_i(cscale(x, (float)k));  // w_z = k' (0 at the packed column)
-        outv[j] = x;
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < KPT; j++) {
-      const int k = t + j * NT;
-      if (k < M) G[base + li * M + k] = outv[j];
-    }
+    for (int j = 0; j < VPT; j++) v[j] = make_float2(kv[j].x * v[j].x, kv[j].y * v[j].y);
+    zl_forward_store<M>(v, lb, line, t, twm, tw2, G, base, comp == 2);
     __syncthreads();
   }
 }
@@ -505,149 +541,125 @@ k_z_diff(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
 enum { ZR_REACT = 0, ZR_REACT_ADJ = 1, ZR_C2R = 2, ZR_R2C = 3, ZR_MISFIT = 4 };
 
 template <int M, int MODE>
-__global__ void __launch_bounds__(ZK<M>::TPB)
+__global__ void __launch_bounds__(ZK<M>::TPB, GLIA_Z_MINW)
 k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ twm, const float2* __restrict__ tw2,
           int s_in, int s_out, int r_slot, int ch_idx, float dt, int honor_done) {
   const MemberDev& Mb = mem[blockIdx.y];
   if (honor_done && Mb.st->done) return;
-  constexpr int NT = ZK<M>::NT, MP = ZK<M>::MP, KPT = ZK<M>::KPT, NL = ZK<M>::NL;
-  __shared__ float2 buf[NL * MP];
+  constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT, VPT = ZK<M>::VPT;
+  __shared__ float2 lb[ZK<M>::NL * ZK<M>::LP];
   const int li = threadIdx.x / NT, t = threadIdx.x % NT;
-  const int64_t base = (int64_t)blockIdx.x * NL * M;
-  float2* line = buf + li * MP;
-  // real field: a half-step history entry, a real slot, or none
-  float2* rf = reinterpret_cast<float2*>(ch_idx >= 0 ? Mb.chalf[ch_idx] : (r_slot >= 0 ? Mb.r[r_slot] : nullptr));
+  const int64_t base = (int64_t)blockIdx.x * ZK<M>::NL * M;
+  const int lo = li * M;
+  float2* line = lb + li * ZK<M>::LP;
+  // real field: a half-step history entry, a real slot, or none (block-base pointers, uniform)
+  GPtr rf = gptr(ch_idx >= 0 ? (const void*)Mb.chalf[ch_idx] : (r_slot >= 0 ? (const void*)Mb.r[r_slot] : nullptr));
+  if (rf) rf = rf + base;
   const float s = (float)d.inv_ng;
   double acc[1] = {0.0};
+  float2 v[VPT];
   if (MODE == ZR_R2C) {
 #pragma unroll
-    for (int j = 0; j < KPT; j++) {
-      const int n = t + j * NT;
-      if (n < M) line[n] = rf[base + li * M + n];
-    }
-    __syncthreads();
+    for (int j = 0; j < VPT; j++) v[j] = rf[lo + out_row<R1, R2, NT>(t, j)];
   } else {
-    const float2* I = cptr(Mb, s_in);
-    zload<M>(buf, I, base);
+    zl_load<M>(lb, cptr(Mb, s_in), base);
     __syncthreads();
-    zprep_c2r<M>(line, t, tw2);
-    fft_line<M, +1>(line, 1, t, twm, true);
-    const float2* rho = reinterpret_cast<const float2*>(Mb.r[RS_RHO]);
+    zl_inverse<M>(v, line, t, twm, tw2);
+    GPtr rho = rptr2(Mb, RS_RHO) + base;
+    GPtr ctp = rptr2(Mb, RS_CT) + base;
+    GPtr obp = rptr2(Mb, RS_OBS) + base;
+    GPtr d1p = rptr2(Mb, RS_D1) + base;
 #pragma unroll
-    for (int j = 0; j < KPT; j++) {
-      const int n = t + j * NT;
-      if (n < M) {
-        const int64_t gi = base + li * M + n;
-        float2 c = cscale(line[n], s);
-        if (MODE == ZR_C2R) {
-          rf[gi] = c;
-        } else if (MODE == ZR_REACT) {
-          if (rf) rf[gi] = c;
-          const float2 r = rho[gi];
-          const float fx = expf(r.x * dt), fy = expf(r.y * dt);
-          c.x = c.x * fx / (c.x * fx + 1.0f - c.x);
-          c.y = c.y * fy / (c.y * fy + 1.0f - c.y);
-          line[n] = c;
-        } else if (MODE == ZR_REACT_ADJ) {
-          const float2 r = rho[gi];
-          const float2 ch = rf[gi];
-          const float fx = expf(r.x * dt), fy = expf(r.y * dt);
-          const float ax = ch.x * fx + 1.0f - ch.x, ay = ch.y * fy + 1.0f - ch.y;
-          c.x = c.x * fx / (ax * ax);
-          c.y = c.y * fy / (ay * ay);
-          line[n] = c;
-        } else if (MODE == ZR_MISFIT) {
-          reinterpret_cast<float2*>(Mb.r[RS_CT])[gi] = c;
-          const float2 ob = reinterpret_cast<const float2*>(Mb.r[RS_OBS])[gi];
-          const float2 dd = reinterpret_cast<const float2*>(Mb.r[RS_D1])[gi];
-          const float tx = ob.x * c.x - dd.x, ty = ob.y * c.y - dd.y;
-          acc[0] += (double)tx * tx + (double)ty * ty;
-          line[n] = make_float2(-(ob.x * tx), -(ob.y * ty));
-        }
+    for (int j = 0; j < VPT; j++) {
+      const int gi = lo + out_row<R1, R2, NT>(t, j);
+      float2 c = cscale(v[j], s);
+      if (MODE == ZR_C2R) {
+        rf[gi] = c;
+      } else if (MODE == ZR_REACT) {
+        if (rf) rf[gi] = c;
+        const float2 r = rho[gi];
+        const float fx = expf(r.x * dt), fy = expf(r.y * dt);
+        c.x = c.x * fx / (c.x * fx + 1.0f - c.x);
+        c.y = c.y * fy / (c.y * fy + 1.0f - c.y);
+        v[j] = c;
+      } else if (MODE == ZR_REACT_ADJ) {
+        const float2 r = rho[gi];
+        const float2 ch = rf[gi];
+        const float fx = expf(r.x * dt), fy = expf(r.y * dt);
+        const float ax = ch.x * fx + 1.0f - ch.x, ay = ch.y * fy + 1.0f - ch.y;
+        c.x = c.x * fx / (ax * ax);
+        c.y = c.y * fy / (ay * ay);
+        v[j] = c;
+      } else if (MODE == ZR_MISFIT) {
+        ctp[gi] = c;
+        const float2 ob = obp[gi];
+        const float2 dd = d1p[gi];
+        const float tx = ob.x * c.x - dd.x, ty = ob.y * c.y - dd.y;
+        acc[0] += (double)tx * tx + (double)ty * ty;
+        v[j] = make_float2(-(ob.x * tx), -(ob.y * ty));
       }
     }
-    __syncthreads();
   }
-  if (MODE != ZR_C2R) {
-    fft_line<M, -1>(line, 1, t, twm, true);
-    float2* O = cptr(Mb, s_out);
-    float2 outv[KPT];
-#pragma unroll
-    for (int j = 0; j < KPT; j++) {
-      const int k = t + j * NT;
-      if (k < M) outv[j] = zpost_r2c<M>(line, k, tw2);
-    }
-#pragma unroll
-    for (int j = 0; j < KPT; j++) {
-      const int k = t + j * NT;
-      if (k < M) O[base + li * M + k] = outv[j];
-    }
-  }
+  if (MODE != ZR_C2R) zl_forward_store<M>(v, lb, line, t, twm, tw2, cptr(Mb, s_out), base, false);
   if (MODE == ZR_MISFIT) block_reduce_store<1>(acc, Mb.part + blockIdx.x);
 }
 
 // Z pass of the gradient quadrature at one time point: from the y-transformed fields of c
-// (G0, Gx, Gy, Gz in slots s_c[0..4)) and of alpha (s_a[0..4)):
+// (G0, Gx, Gy, Gz in slots s_c) and of alpha (s_a):
 //   q_k   = sum (wm + r_k gm) (grad c . grad alpha)
 //   q_rho = sum (wm + r_rho gm) (c^2 - c) alpha
 // (the integrands of gradDiffusion / gradReaction, src/grad/DerivativeOperators.cpp:189-321)
 template <int M>
-__global__ void __launch_bounds__(ZK<M>::TPB)
+__global__ void __launch_bounds__(ZK<M>::TPB, GLIA_Z_MINW)
 k_z_gradq(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ twm, const float2* __restrict__ tw2,
           int4 s_c, int4 s_a) {
   const MemberDev& Mb = mem[blockIdx.y];
-  constexpr int NT = ZK<M>::NT, MP = ZK<M>::MP, KPT = ZK<M>::KPT, NL = ZK<M>::NL;
-  __shared__ float2 buf[NL * MP];
+  constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT, VPT = ZK<M>::VPT;
+  __shared__ float2 lb[ZK<M>::NL * ZK<M>::LP];
   const int li = threadIdx.x / NT, t = threadIdx.x % NT;
-  const int64_t base = (int64_t)blockIdx.x * NL * M;
-  float2* line = buf + li * MP;
+  const int64_t base = (int64_t)blockIdx.x * ZK<M>::NL * M;
+  const int lo = li * M;
+  float2* line = lb + li * ZK<M>::LP;
   const float s = (float)d.inv_ng;
-  float2 dot[KPT], cv[KPT];
+  GPtr wmp = rptr2(Mb, RS_WM) + base;
+  GPtr gmp = rptr2(Mb, RS_GM) + base;
+  float2 dot[VPT], cv[VPT], v[VPT];
 #pragma unroll
-  for (int j = 0; j < KPT; j++) dot[j] = make_float2(0.f, 0.f);
+  for (int j = 0; j < VPT; j++) dot[j] = make_float2(0.f, 0.f);
   const int sc[4] = {s_c.x, s_c.y, s_c.z, s_c.w};
   const int sa[4] = {s_a.x, s_a.y, s_a.z, s_a.w};
   double acc[2] = {0.0, 0.0};
   for (int comp = 0; comp < 4; comp++) {
     for (int which = 0; which < 2; which++) {
-      const float2* I = cptr(Mb, which == 0 ? sc[comp] : sa[comp]);
-      zload<M>(buf, I, base);
+      zl_load<M>(lb, cptr(Mb, which == 0 ? sc[comp] : sa[comp]), base);
       __syncthreads();
-      zprep_c2r<M>(line, t, tw2);
-      fft_line<M, +1>(line, 1, t, twm, true);
+      zl_inverse<M>(v, line, t, twm, tw2);
 #pragma unroll
-      for (int j = 0; j < KPT; j++) {
-        const int n = t + j * NT;
-        if (n < M) {
-          const float2 v = cscale(line[n], s);
-          if (which == 0) {
-            cv[j] = v;
-          } else if (comp > 0) {
-            dot[j].x += cv[j].x * v.x;
-            dot[j].y += cv[j].y * v.y;
-          } else {
-            // comp 0: c and alpha themselves
-            const int64_t gi = base + li * M + n;
-            const float2 wm = reinterpret_cast<const float2*>(Mb.r[RS_WM])[gi];
-            const float2 gm = reinterpret_cast<const float2*>(Mb.r[RS_GM])[gi];
-            const float rx = cv[j].x * cv[j].x - cv[j].x, ry = cv[j].y * cv[j].y - cv[j].y;
-            acc[1] += (double)(wm.x + Mb.rr_gm * gm.x) * (double)(rx * v.x) +
-                      (double)(wm.y + Mb.rr_gm * gm.y) * (double)(ry * v.y);
-          }
+      for (int j = 0; j < VPT; j++) {
+        const float2 x = cscale(v[j], s);
+        if (which == 0) {
+          cv[j] = x;
+        } else if (comp > 0) {
+          dot[j].x += cv[j].x * x.x;
+          dot[j].y += cv[j].y * x.y;
+        } else {
+          const int gi = lo + out_row<R1, R2, NT>(t, j);
+          const float2 wm = wmp[gi];
+          const float2 gm = gmp[gi];
+          const float rx = cv[j].x * cv[j].x - cv[j].x, ry = cv[j].y * cv[j].y - cv[j].y;
+          acc[1] += (double)(wm.x + Mb.rr_gm * gm.x) * (double)(rx * x.x) +
+                    (double)(wm.y + Mb.rr_gm * gm.y) * (double)(ry * x.y);
         }
       }
       __syncthreads();
     }
   }
 #pragma unroll
-  for (int j = 0; j < KPT; j++) {
-    const int n = t + j * NT;
-    if (n < M) {
-      const int64_t gi = base + li * M + n;
-      const float2 wm = reinterpret_cast<const float2*>(Mb.r[RS_WM])[gi];
-      const float2 gm = reinterpret_cast<const float2*>(Mb.r[RS_GM])[gi];
-      acc[0] += (double)(wm.x + Mb.kr_gm * gm.x) * (double)dot[j].x + (double)(wm.y + Mb.kr_gm * gm.y) * (double)dot[j].y;
-    }
+  for (int j = 0; j < VPT; j++) {
+    const int gi = lo + out_row<R1, R2, NT>(t, j);
+    const float2 wm = wmp[gi];
+    const float2 gm = gmp[gi];
+    acc[0] += (double)(wm.x + Mb.kr_gm * gm.x) * (double)dot[j].x + (double)(wm.y + Mb.kr_gm * gm.y) * (double)dot[j].y;
   }
   block_reduce_store<2>(acc, Mb.part + (int64_t)blockIdx.x * 2);
 }
@@ -663,7 +675,7 @@ k_pcg_update(const MemberDev* __restrict__ mem, Dims d, int s_x) {
   const MemberDev& M = mem[blockIdx.y];
   const PcgState* st = M.st;
   if (st->done) return;
-  float2* X = cptr(M, s_x);
+  float2* X = craw(M, s_x);
   float2* R = M.c[CS_R];
   const float2* P = M.c[CS_P];
   const float2* W = M.c[CS_W];
@@ -825,8 +837,8 @@ __global__ void k_pcg_fin_update(const MemberDev* __restrict__ mem, Dims d, int 
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_copy(const MemberDev* __restrict__ mem, Dims d, int s_in, int s_out) {
   const MemberDev& M = mem[blockIdx.y];
-  const float4* I = reinterpret_cast<const float4*>(cptr(M, s_in));
-  float4* O = reinterpret_cast<float4*>(cptr(M, s_out));
+  const float4* I = reinterpret_cast<const float4*>(craw(M, s_in));
+  float4* O = reinterpret_cast<float4*>(craw(M, s_out));
   const int64_t n4 = d.nspec / 2;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) O[i] = I[i];
@@ -1155,9 +1167,9 @@ struct RunXiGrad {
   static int run(const LaunchCtx& c, int s_in, int s_r, int s_u, int s_v, bool pupd) {
     dim3 g(c.d.n1 * (c.d.m2 / 16), c.nm);
     if (pupd)
-      k_xi_grad<L, true><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_in, s_r, s_u, s_v);
+      k_xi_grad<L, true><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_in, s_r, s_u, s_v);
     else
-      k_xi_grad<L, false><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_in, s_r, s_u, s_v);
+      k_xi_grad<L, false><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_in, s_r, s_u, s_v);
     GLIA_CHECK_LAUNCH();
     return 0;
   }
@@ -1171,9 +1183,9 @@ struct RunYiGrad {
   static int run(const LaunchCtx& c, int s_u, int s_v, int s_gx, int s_gy, int s_gz, int s_g0) {
     dim3 g(c.d.n0 * (c.d.m2 / 16), c.nm);
     if (s_g0 >= 0)
-      k_yi_grad<L, true><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw1, s_u, s_v, s_gx, s_gy, s_gz, s_g0);
+      k_yi_grad<L, true><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw1, s_u, s_v, s_gx, s_gy, s_gz, s_g0);
     else
-      k_yi_grad<L, false><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw1, s_u, s_v, s_gx, s_gy, s_gz, 0);
+      k_yi_grad<L, false><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw1, s_u, s_v, s_gx, s_gy, s_gz, 0);
     GLIA_CHECK_LAUNCH();
     return 0;
   }
@@ -1186,7 +1198,7 @@ template <int L>
 struct RunYfDiv {
   static int run(const LaunchCtx& c, int s_hx, int s_hy, int s_q, int s_yx, int s_rr) {
     dim3 g(c.d.n0 * (c.d.m2 / 16), c.nm);
-    k_yf_div<L><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw1, s_hx, s_hy, s_q, s_yx, s_rr);
+    k_yf_div<L><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw1, s_hx, s_hy, s_q, s_yx, s_rr);
     GLIA_CHECK_LAUNCH();
     return 0;
   }
@@ -1200,11 +1212,11 @@ struct RunXfDiv {
   static int run(const LaunchCtx& c, int mode, int s_yx, int s_rr, int s_p, int s_out) {
     dim3 g(c.d.n1 * (c.d.m2 / 16), c.nm);
     if (mode == XF_MATVEC)
-      k_xf_div<L, XF_MATVEC><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
+      k_xf_div<L, XF_MATVEC><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
     else if (mode == XF_SETUP)
-      k_xf_div<L, XF_SETUP><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
+      k_xf_div<L, XF_SETUP><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
     else
-      k_xf_div<L, XF_D><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
+      k_xf_div<L, XF_D><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
     GLIA_CHECK_LAUNCH();
     return 0;
   }
@@ -1220,10 +1232,10 @@ struct RunColPlain {
     const int other = (dir == 0) ? c.d.n1 : c.d.n0;
     const float2* tw = (dir == 0) ? c.tw0 : c.tw1;
     dim3 g(other * (c.d.m2 / 16), c.nm);
-    if (dir == 0 && sign < 0) k_col_plain<L, 0, -1><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, tw, s_in, s_out, honor);
-    else if (dir == 0) k_col_plain<L, 0, 1><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, tw, s_in, s_out, honor);
-    else if (sign < 0) k_col_plain<L, 1, -1><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, tw, s_in, s_out, honor);
-    else k_col_plain<L, 1, 1><<<g, ColK<L>::TPB, 0, c.stream>>>(c.mem, c.d, tw, s_in, s_out, honor);
+    if (dir == 0 && sign < 0) k_col_plain<L, 0, -1><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, tw, s_in, s_out, honor);
+    else if (dir == 0) k_col_plain<L, 0, 1><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, tw, s_in, s_out, honor);
+    else if (sign < 0) k_col_plain<L, 1, -1><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, tw, s_in, s_out, honor);
+    else k_col_plain<L, 1, 1><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, tw, s_in, s_out, honor);
     GLIA_CHECK_LAUNCH();
     return 0;
   }
