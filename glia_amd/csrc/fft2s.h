@@ -33,7 +33,7 @@ __device__ __forceinline__ float2 mul_i(float2 a) { return make_float2(-a.y, a.x
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }  // -i * a
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 
-// cos(2 pi j / 48); every radix used here (2, 3, 4, 6, 8, 12, 16) divides 48.  Non-recursive so
+// cos(2 pi j / 48); every radix used here (2, 3, 4, 6, 8, 12, 16, 24) divides 48.  Non-recursive so
 // that it folds to a constant wherever j is a compile-time constant.
 __device__ __forceinline__ constexpr float c48q(int q) {  // q in [0, 12]
   return q == 0 ? 1.0f : q == 1 ? 0.99144486137381041f : q == 2 ? 0.96592582628906829f
@@ -143,6 +143,10 @@ template <int S>
 struct Dft<16, S> {
   __device__ __forceinline__ static void run(float2* v) { dft_split<4, 4, S>(v); }
 };
+template <int S>
+struct Dft<24, S> {
+  __device__ __forceinline__ static void run(float2* v) { dft_split<3, 8, S>(v); }
+};
 
 // ---------------------------------------------------------------------------------------------
 // plans: L = R1 * R2 with NT threads per line (NT divides R1 and R2)
@@ -161,7 +165,7 @@ GLIA_PLAN2(48, 12, 4, 4)
 GLIA_PLAN2(64, 8, 8, 8)
 GLIA_PLAN2(96, 12, 8, 4)
 GLIA_PLAN2(128, 8, 16, 8)
-GLIA_PLAN2(192, 12, 16, 4)
+GLIA_PLAN2(192, 24, 8, 8)
 GLIA_PLAN2(256, 16, 16, 16)
 #undef GLIA_PLAN2
 

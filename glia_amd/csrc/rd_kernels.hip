@@ -148,6 +148,8 @@ struct CK {
   static constexpr int TPB = 16 * NT;
   static constexpr int VPT = L / NT;
   static constexpr int CP = XPitch<L>::value;
+  // a kernel's later operands are loaded at kernel start when the extra registers fit
+  static constexpr bool PF = VPT <= 16;
 };
 
 // row of stage-1 input slot j of thread t: j = b*R1 + n1 -> R2*n1 + t + b*NT
@@ -182,6 +184,19 @@ __device__ __forceinline__ Tile col_tile(int dir, const Dims& d) {
   return t;
 }
 
+// Twiddle table of a block, staged in LDS after the block's data loads have been issued.  Read
+// from LDS after a barrier, the twiddles of each transform are loaded where they are used instead
+// of being held in registers across the whole kernel.
+template <int L, int TPB>
+__device__ __forceinline__ void stage_twiddles(float2* twl, const float2* __restrict__ tw) {
+#pragma unroll
+  for (int i = 0; i < (L + TPB - 1) / TPB; i++) {
+    const int e = (int)threadIdx.x + i * TPB;
+    if (e < L) twl[e] = tw[e];
+  }
+  __syncthreads();
+}
+
 #define GLIA_COL_PRELUDE(L)                                  \
   using K = CK<L>;                                           \
   constexpr int R1 = K::R1, R2 = K::R2, NT = K::NT;          \
@@ -212,30 +227,40 @@ k_xi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
     pc = (float)st->pc;
   }
   float2 a[K::VPT], b[K::VPT];
+  // 'first' is uniform: the two load loops are kept separate so that every load of a loop is
+  // issued before the first use
+  if (PUPD && first) {
+#pragma unroll
+    for (int j = 0; j < K::VPT; j++) a[j] = rr[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
+  } else if (PUPD) {
+#pragma unroll
+    for (int j = 0; j < K::VPT; j++) {
+      const unsigned g = (unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col;
+      a[j] = rr[g];
+      b[j] = pin[g];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < K::VPT; j++) a[j] = pin[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
+  }
 #pragma unroll
   for (int j = 0; j < K::VPT; j++) {
     const int row = in_row<R1, R2, NT>(t, j);
-    const unsigned g = (unsigned)row * tl.rs + col;
     const float wx = wnum(row, d.n0);
-    float2 p;
+    float2 p = a[j];
     if (PUPD) {
-      const float2 z = cscale(rr[g], precf(pc, wx, wy, wz));
-      if (first) {
-        p = z;
-      } else {
-        const float2 po = pin[g];
-        p = make_float2(z.x + bc * po.x, z.y + bc * po.y);
-      }
-      pin[g] = p;
-    } else {
-      p = pin[g];
+      p = cscale(p, precf(pc, wx, wy, wz));
+      if (!first) p = make_float2(p.x + bc * b[j].x, p.y + bc * b[j].y);
+      pin[(unsigned)row * tl.rs + col] = p;
     }
     a[j] = p;
     b[j] = mul_i(cscale(p, wx));
   }
-  fft2s<R1, R2, NT, +1>(a, exc, t, tw);
+  __shared__ float2 twl[L];
+  stage_twiddles<L, K::TPB>(twl, tw);
+  fft2s<R1, R2, NT, +1>(a, exc, t, twl);
   __syncthreads();
-  fft2s<R1, R2, NT, +1>(b, exc, t, tw);
+  fft2s<R1, R2, NT, +1>(b, exc, t, twl);
 #pragma unroll
   for (int j = 0; j < K::VPT; j++) {
     const unsigned g = (unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col;
@@ -261,22 +286,29 @@ k_yi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   GPtr Gy = cptr(M, s_gy) + tl.base;
   GPtr Gz = cptr(M, s_gz) + tl.base;
   GPtr G0 = WITH0 ? cptr(M, s_g0) + tl.base : gptr(nullptr);
-  float2 a[K::VPT], b[K::VPT];
+  float2 a[K::VPT], b[K::VPT], u[K::PF ? K::VPT : 1];
+  // both inputs are loaded up front (when registers allow) so that their latencies overlap
 #pragma unroll
-  for (int j = 0; j < K::VPT; j++) a[j] = V[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
-  fft2s<R1, R2, NT, +1>(a, exc, t, tw);
+  for (int j = 0; j < K::VPT; j++) {
+    const unsigned g = (unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col;
+    a[j] = V[g];
+    if (K::PF) u[j] = U[g];
+  }
+  __shared__ float2 twl[L];
+  stage_twiddles<L, K::TPB>(twl, tw);
+  fft2s<R1, R2, NT, +1>(a, exc, t, twl);
 #pragma unroll
   for (int j = 0; j < K::VPT; j++) Gx[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
 #pragma unroll
   for (int j = 0; j < K::VPT; j++) {
     const int row = in_row<R1, R2, NT>(t, j);
-    a[j] = U[(unsigned)row * tl.rs + col];
+    a[j] = K::PF ? u[j] : U[(unsigned)row * tl.rs + col];
     b[j] = mul_i(cscale(a[j], wnum(row, d.n1)));
   }
   __syncthreads();
-  fft2s<R1, R2, NT, +1>(a, exc, t, tw);
+  fft2s<R1, R2, NT, +1>(a, exc, t, twl);
   __syncthreads();
-  fft2s<R1, R2, NT, +1>(b, exc, t, tw);
+  fft2s<R1, R2, NT, +1>(b, exc, t, twl);
 #pragma unroll
   for (int j = 0; j < K::VPT; j++) {
     const unsigned g = (unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col;
@@ -301,22 +333,32 @@ k_yf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
   GPtr Q = cptr(M, s_q) + tl.base;
   GPtr Yx = cptr(M, s_yx) + tl.base;
   GPtr Rr = cptr(M, s_rr) + tl.base;
-  float2 a[K::VPT], b[K::VPT];
+  float2 a[K::VPT], b[K::VPT], q[K::PF ? K::VPT : 1];
+  // all three inputs are loaded up front (when registers allow) so that their latencies overlap
 #pragma unroll
-  for (int j = 0; j < K::VPT; j++) a[j] = Hx[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
-  fft2s<R1, R2, NT, -1>(a, exc, t, tw);
+  for (int j = 0; j < K::VPT; j++) {
+    const unsigned g = (unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col;
+    a[j] = Hx[g];
+    if (K::PF) {
+      q[j] = Q[g];
+      b[j] = Hy[g];
+    }
+  }
+  __shared__ float2 twl[L];
+  stage_twiddles<L, K::TPB>(twl, tw);
+  fft2s<R1, R2, NT, -1>(a, exc, t, twl);
 #pragma unroll
   for (int j = 0; j < K::VPT; j++) Yx[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
 #pragma unroll
   for (int j = 0; j < K::VPT; j++) {
     const unsigned g = (unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col;
-    a[j] = Q[g];
-    b[j] = Hy[g];
+    a[j] = K::PF ? q[j] : Q[g];
+    if (!K::PF) b[j] = Hy[g];
   }
   __syncthreads();
-  fft2s<R1, R2, NT, -1>(a, exc, t, tw);
+  fft2s<R1, R2, NT, -1>(a, exc, t, twl);
   __syncthreads();
-  fft2s<R1, R2, NT, -1>(b, exc, t, tw);
+  fft2s<R1, R2, NT, -1>(b, exc, t, twl);
 #pragma unroll
   for (int j = 0; j < K::VPT; j++) {
     const int row = out_row<R1, R2, NT>(t, j);
@@ -347,16 +389,24 @@ k_xf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
   const float pc = (float)st->pc;
   const float wy = wnum(tl.o, d.n1), wz = (float)(tl.c0 + col);
   const double wgt = (tl.c0 + col == 0) ? 1.0 : 2.0;
-  float2 a[K::VPT], b[K::VPT];
+  float2 a[K::VPT], b[K::VPT], pv[K::PF ? K::VPT : 1];
 #pragma unroll
   for (int j = 0; j < K::VPT; j++) {
     const unsigned g = (unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col;
     a[j] = Yx[g];
     b[j] = Rr[g];
   }
-  fft2s<R1, R2, NT, -1>(a, exc, t, tw);
+  // the operand of the output stage is loaded before the transforms (a barrier would otherwise
+  // keep its loads behind them)
+  if (MODE != XF_D && K::PF) {
+#pragma unroll
+    for (int j = 0; j < K::VPT; j++) pv[j] = P[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col];
+  }
+  __shared__ float2 twl[L];
+  stage_twiddles<L, K::TPB>(twl, tw);
+  fft2s<R1, R2, NT, -1>(a, exc, t, twl);
   __syncthreads();
-  fft2s<R1, R2, NT, -1>(b, exc, t, tw);
+  fft2s<R1, R2, NT, -1>(b, exc, t, twl);
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int j = 0; j < K::VPT; j++) {
@@ -365,12 +415,12 @@ k_xf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
     const float wx = wnum(row, d.n0);
     const float2 D = cadd(mul_i(cscale(a[j], wx)), b[j]);
     if (MODE == XF_MATVEC) {
-      const float2 p = P[g];
+      const float2 p = K::PF ? pv[j] : P[g];
       const float2 w = make_float2(p.x - 0.5f * tau * D.x, p.y - 0.5f * tau * D.y);
       O[g] = w;
       acc[0] += wgt * ((double)p.x * w.x + (double)p.y * w.y);
     } else if (MODE == XF_SETUP) {
-      const float2 x = P[g];
+      const float2 x = K::PF ? pv[j] : P[g];
       const float2 bb = make_float2(x.x + 0.5f * tau * D.x, x.y + 0.5f * tau * D.y);
       const float2 r = cscale(D, tau);
       O[g] = r;
@@ -403,7 +453,9 @@ k_col_plain(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict_
   float2 a[K::VPT];
 #pragma unroll
   for (int j = 0; j < K::VPT; j++) a[j] = I[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
-  fft2s<R1, R2, NT, S>(a, exc, t, tw);
+  __shared__ float2 twl[L];
+  stage_twiddles<L, K::TPB>(twl, tw);
+  fft2s<R1, R2, NT, S>(a, exc, t, twl);
 #pragma unroll
   for (int j = 0; j < K::VPT; j++) O[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
 }
@@ -423,13 +475,48 @@ struct ZK {
   static constexpr int VPT = M / NT;
 };
 
-// coalesced load of the block's NL packed lines into the padded line buffers
+// coalesced load of the block's NL packed lines into the padded line buffers, in two halves:
+// zl_fetch issues every global load of the block into registers (a compile-time trip count keeps
+// the loop unrolled), zl_put writes them to LDS.  Between the two the caller may do other work.
+template <int M>
+struct ZIt {
+  static constexpr int value = ZK<M>::NL * M / ZK<M>::TPB;
+  static_assert(value * ZK<M>::TPB == ZK<M>::NL * M, "block must tile its lines");
+};
+template <int M>
+__device__ __forceinline__ void zl_fetch(float2 (&tmp)[ZIt<M>::value], GPtr src, int64_t base) {
+  GPtr sb = src + base;
+#pragma unroll
+  for (int i = 0; i < ZIt<M>::value; i++) tmp[i] = sb[(int)threadIdx.x + i * ZK<M>::TPB];
+}
+template <int M>
+__device__ __forceinline__ void zl_put(float2* lb, const float2 (&tmp)[ZIt<M>::value]) {
+#pragma unroll
+  for (int i = 0; i < ZIt<M>::value; i++) {
+    const int e = (int)threadIdx.x + i * ZK<M>::TPB;
+    const int li = e / M, k = e - li * M;
+    lb[li * ZK<M>::LP + xpad(k)] = tmp[i];
+  }
+}
 template <int M>
 __device__ __forceinline__ void zl_load(float2* lb, GPtr src, int64_t base) {
-  GPtr sb = src + base;
-  for (int e = threadIdx.x; e < ZK<M>::NL * M; e += ZK<M>::TPB) {
-    const int li = e / M, k = e - li * M;
-    lb[li * ZK<M>::LP + xpad(k)] = sb[e];
+  float2 tmp[ZIt<M>::value];
+  zl_fetch<M>(tmp, src, base);
+  zl_put<M>(lb, tmp);
+}
+
+// both twiddle tables of a line kernel, staged in LDS (see stage_twiddles)
+template <int M>
+__device__ __forceinline__ void stage_line_twiddles(float2* twml, float2* tw2l, const float2* __restrict__ twm,
+                                                    const float2* __restrict__ tw2) {
+  constexpr int TPB = ZK<M>::TPB;
+#pragma unroll
+  for (int i = 0; i < (M + TPB - 1) / TPB; i++) {
+    const int e = (int)threadIdx.x + i * TPB;
+    if (e < M) {
+      twml[e] = twm[e];
+      tw2l[e] = tw2[e];
+    }
   }
 }
 
@@ -438,27 +525,22 @@ __device__ __forceinline__ void zl_load(float2* lb, GPtr src, int64_t base) {
 // (unnormalized C2R of length n2).
 template <int M>
 __device__ __forceinline__ float2 zprep(const float2* line, int k, const float2* __restrict__ tw2) {
-  if (k == 0) {
-    const float2 p0 = line[0];
-    return make_float2(p0.x + p0.y, p0.x - p0.y);
-  }
+  // branch-free: the k = 0 entry reads itself as its mirror and is replaced by a select
   const float2 a = line[xpad(k)];
-  const float2 bc = cconj(line[xpad(M - k)]);
-  return cadd(cadd(a, bc), mul_i(cmulc(csub(a, bc), tw2[k])));
+  const float2 bc = cconj(line[xpad(k == 0 ? 0 : M - k)]);
+  const float2 g = cadd(cadd(a, bc), mul_i(cmulc(csub(a, bc), tw2[k])));
+  return (k == 0) ? make_float2(a.x + a.y, a.x - a.y) : g;
 }
 
 // after a forward M-point FFT of z[n] = x[2n] + i x[2n+1]: packed spectrum entry k
 template <int M>
 __device__ __forceinline__ float2 zpost(const float2* line, int k, const float2* __restrict__ tw2) {
-  if (k == 0) {
-    const float2 z0 = line[0];
-    return make_float2(z0.x + z0.y, z0.x - z0.y);
-  }
   const float2 a = line[xpad(k)];
-  const float2 bc = cconj(line[xpad(M - k)]);
+  const float2 bc = cconj(line[xpad(k == 0 ? 0 : M - k)]);
   const float2 e = cscale(cadd(a, bc), 0.5f);
   const float2 o = cscale(mul_mi(csub(a, bc)), 0.5f);
-  return cadd(e, cmul(o, tw2[k]));
+  const float2 g = cadd(e, cmul(o, tw2[k]));
+  return (k == 0) ? make_float2(a.x + a.y, a.x - a.y) : g;
 }
 
 // inverse: the block's packed lines (already in lb) -> v[j] = z[out_row(t, j)] of line li
@@ -487,7 +569,11 @@ __device__ __forceinline__ void zl_forward_store(float2* v, float2* lb, float2* 
 #pragma unroll
   for (int j = 0; j < ZK<M>::VPT; j++) line[xpad(out_row<R2, R1, NT>(t, j))] = v[j];
   __syncthreads();
-  for (int e = threadIdx.x; e < ZK<M>::NL * M; e += ZK<M>::TPB) {
+  constexpr int TPB = ZK<M>::TPB, IT = ZK<M>::NL * M / TPB;
+  // unrolled by 8 at most: a longer unroll gathers all its LDS loads up front and spills
+#pragma unroll 8
+  for (int i = 0; i < IT; i++) {
+    const int e = (int)threadIdx.x + i * TPB;
     const int li = e / M, k = e - li * M;
     float2 x = zpost<M>(lb + li * ZK<M>::LP, k, tw2);
     if (mulz) x = mul_i(cscale(x, (float)k));  // w_z = k' (0 at the packed column)
@@ -505,27 +591,35 @@ k_z_diff(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
   if (Mb.st->done) return;
   constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT, VPT = ZK<M>::VPT;
   __shared__ float2 lb[ZK<M>::NL * ZK<M>::LP];
+  __shared__ float2 twml[M], tw2l[M];
   const int li = threadIdx.x / NT, t = threadIdx.x % NT;
   const int64_t base = (int64_t)blockIdx.x * ZK<M>::NL * M;
   float2* line = lb + li * ZK<M>::LP;
   GPtr kf = rptr2(Mb, RS_K) + base;
   const int lo = li * M;
   const float s = (float)d.inv_ng;
+  const int slots[3] = {s_gx, s_gy, s_gz};
+  // software pipeline (when registers allow): the loads of component c+1 are in flight while
+  // component c is transformed
+  constexpr bool PIPE = VPT <= 16;
+  float2 nxt[ZIt<M>::value];
+  zl_fetch<M>(nxt, cptr(Mb, slots[0]), base);
   float2 kv[VPT], v[VPT];
 #pragma unroll
   for (int j = 0; j < VPT; j++) {
     const float2 k2 = kf[lo + out_row<R1, R2, NT>(t, j)];
     kv[j] = make_float2(k2.x * s, k2.y * s);
   }
-  const int slots[3] = {s_gx, s_gy, s_gz};
+  stage_line_twiddles<M>(twml, tw2l, twm, tw2);
   for (int comp = 0; comp < 3; comp++) {
-    GPtr G = cptr(Mb, slots[comp]);
-    zl_load<M>(lb, G, base);
+    if (!PIPE && comp > 0) zl_fetch<M>(nxt, cptr(Mb, slots[comp]), base);
+    zl_put<M>(lb, nxt);
     __syncthreads();
-    zl_inverse<M>(v, line, t, twm, tw2);
+    if (PIPE && comp < 2) zl_fetch<M>(nxt, cptr(Mb, slots[comp + 1]), base);
+    zl_inverse<M>(v, line, t, twml, tw2l);
 #pragma unroll
     for (int j = 0; j < VPT; j++) v[j] = make_float2(kv[j].x * v[j].x, kv[j].y * v[j].y);
-    zl_forward_store<M>(v, lb, line, t, twm, tw2, G, base, comp == 2);
+    zl_forward_store<M>(v, lb, line, t, twml, tw2l, cptr(Mb, slots[comp]), base, comp == 2);
     __syncthreads();
   }
 }
@@ -547,7 +641,11 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   const MemberDev& Mb = mem[blockIdx.y];
   if (honor_done && Mb.st->done) return;
   constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT, VPT = ZK<M>::VPT;
+  // pointwise operands are loaded before the transform when the registers fit
+  constexpr bool PF = VPT <= 16;
+  constexpr bool TWO = (MODE == ZR_REACT_ADJ || MODE == ZR_MISFIT);
   __shared__ float2 lb[ZK<M>::NL * ZK<M>::LP];
+  __shared__ float2 twml[M], tw2l[M];
   const int li = threadIdx.x / NT, t = threadIdx.x % NT;
   const int64_t base = (int64_t)blockIdx.x * ZK<M>::NL * M;
   const int lo = li * M;
@@ -558,49 +656,60 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   const float s = (float)d.inv_ng;
   double acc[1] = {0.0};
   float2 v[VPT];
+  GPtr op1 = rptr2(Mb, MODE == ZR_MISFIT ? RS_OBS : RS_RHO) + base;
+  GPtr op2 = MODE == ZR_MISFIT ? rptr2(Mb, RS_D1) + base : rf;
+  GPtr ctp = rptr2(Mb, RS_CT) + base;
+  float2 o1[PF ? VPT : 1], o2[PF && TWO ? VPT : 1];
   if (MODE == ZR_R2C) {
 #pragma unroll
     for (int j = 0; j < VPT; j++) v[j] = rf[lo + out_row<R1, R2, NT>(t, j)];
   } else {
     zl_load<M>(lb, cptr(Mb, s_in), base);
-    __syncthreads();
-    zl_inverse<M>(v, line, t, twm, tw2);
-    GPtr rho = rptr2(Mb, RS_RHO) + base;
-    GPtr ctp = rptr2(Mb, RS_CT) + base;
-    GPtr obp = rptr2(Mb, RS_OBS) + base;
-    GPtr d1p = rptr2(Mb, RS_D1) + base;
+    if (PF && MODE != ZR_C2R) {
+#pragma unroll
+      for (int j = 0; j < VPT; j++) {
+        const int gi = lo + out_row<R1, R2, NT>(t, j);
+        o1[j] = op1[gi];
+        if (TWO) o2[j] = op2[gi];
+      }
+    }
+  }
+  stage_line_twiddles<M>(twml, tw2l, twm, tw2);
+  __syncthreads();
+  if (MODE != ZR_R2C) {
+    zl_inverse<M>(v, line, t, twml, tw2l);
 #pragma unroll
     for (int j = 0; j < VPT; j++) {
       const int gi = lo + out_row<R1, R2, NT>(t, j);
       float2 c = cscale(v[j], s);
       if (MODE == ZR_C2R) {
         rf[gi] = c;
-      } else if (MODE == ZR_REACT) {
-        if (rf) rf[gi] = c;
-        const float2 r = rho[gi];
-        const float fx = expf(r.x * dt), fy = expf(r.y * dt);
-        c.x = c.x * fx / (c.x * fx + 1.0f - c.x);
-        c.y = c.y * fy / (c.y * fy + 1.0f - c.y);
-        v[j] = c;
-      } else if (MODE == ZR_REACT_ADJ) {
-        const float2 r = rho[gi];
-        const float2 ch = rf[gi];
-        const float fx = expf(r.x * dt), fy = expf(r.y * dt);
-        const float ax = ch.x * fx + 1.0f - ch.x, ay = ch.y * fy + 1.0f - ch.y;
-        c.x = c.x * fx / (ax * ax);
-        c.y = c.y * fy / (ay * ay);
-        v[j] = c;
-      } else if (MODE == ZR_MISFIT) {
-        ctp[gi] = c;
-        const float2 ob = obp[gi];
-        const float2 dd = d1p[gi];
-        const float tx = ob.x * c.x - dd.x, ty = ob.y * c.y - dd.y;
-        acc[0] += (double)tx * tx + (double)ty * ty;
-        v[j] = make_float2(-(ob.x * tx), -(ob.y * ty));
+      } else {
+        const float2 a1 = PF ? o1[j] : op1[gi];
+        if (MODE == ZR_REACT) {
+          if (rf) rf[gi] = c;
+          const float fx = expf(a1.x * dt), fy = expf(a1.y * dt);
+          c.x = c.x * fx / (c.x * fx + 1.0f - c.x);
+          c.y = c.y * fy / (c.y * fy + 1.0f - c.y);
+          v[j] = c;
+        } else if (MODE == ZR_REACT_ADJ) {
+          const float2 ch = PF ? o2[j] : op2[gi];
+          const float fx = expf(a1.x * dt), fy = expf(a1.y * dt);
+          const float ax = ch.x * fx + 1.0f - ch.x, ay = ch.y * fy + 1.0f - ch.y;
+          c.x = c.x * fx / (ax * ax);
+          c.y = c.y * fy / (ay * ay);
+          v[j] = c;
+        } else if (MODE == ZR_MISFIT) {
+          ctp[gi] = c;
+          const float2 dd = PF ? o2[j] : op2[gi];
+          const float tx = a1.x * c.x - dd.x, ty = a1.y * c.y - dd.y;
+          acc[0] += (double)tx * tx + (double)ty * ty;
+          v[j] = make_float2(-(a1.x * tx), -(a1.y * ty));
+        }
       }
     }
   }
-  if (MODE != ZR_C2R) zl_forward_store<M>(v, lb, line, t, twm, tw2, cptr(Mb, s_out), base, false);
+  if (MODE != ZR_C2R) zl_forward_store<M>(v, lb, line, t, twml, tw2l, cptr(Mb, s_out), base, false);
   if (MODE == ZR_MISFIT) block_reduce_store<1>(acc, Mb.part + blockIdx.x);
 }
 
@@ -629,11 +738,23 @@ k_z_gradq(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   const int sc[4] = {s_c.x, s_c.y, s_c.z, s_c.w};
   const int sa[4] = {s_a.x, s_a.y, s_a.z, s_a.w};
   double acc[2] = {0.0, 0.0};
+  __shared__ float2 twml[M], tw2l[M];
+  // the eight line loads (c and alpha for each of the 4 fields) are pipelined when registers allow
+  constexpr bool PIPE = VPT <= 8;
+  float2 nxt[ZIt<M>::value];
+  zl_fetch<M>(nxt, cptr(Mb, sc[0]), base);
+  stage_line_twiddles<M>(twml, tw2l, twm, tw2);
   for (int comp = 0; comp < 4; comp++) {
     for (int which = 0; which < 2; which++) {
-      zl_load<M>(lb, cptr(Mb, which == 0 ? sc[comp] : sa[comp]), base);
+      const int step = comp * 2 + which;
+      if (!PIPE && step > 0) zl_fetch<M>(nxt, cptr(Mb, which == 0 ? sc[comp] : sa[comp]), base);
+      zl_put<M>(lb, nxt);
       __syncthreads();
-      zl_inverse<M>(v, line, t, twm, tw2);
+      if (PIPE && step < 7) {
+        const int c2 = (step + 1) >> 1, w2 = (step + 1) & 1;
+        zl_fetch<M>(nxt, cptr(Mb, w2 == 0 ? sc[c2] : sa[c2]), base);
+      }
+      zl_inverse<M>(v, line, t, twml, tw2l);
 #pragma unroll
       for (int j = 0; j < VPT; j++) {
         const float2 x = cscale(v[j], s);

@@ -117,7 +117,11 @@ int RDEnsemble::init() {
   }
   GLIA_HIP(hipHostMalloc((void**)&hflag_host_, sizeof(int) * kFlagSlots * nm_, hipHostMallocMapped | hipHostMallocCoherent));
   GLIA_HIP(hipHostGetDevicePointer((void**)&hflag_dev_, hflag_host_, 0));
-  for (int i = 0; i < kFlagSlots; i++) GLIA_HIP(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming));
+  // The host reads the convergence flags from coherent host memory that the finalize kernels
+  // write with a system-scope fence, so these events only need a device-scope release (a
+  // system-scope release would write back L2 on every CG iteration).
+  for (int i = 0; i < kFlagSlots; i++)
+    GLIA_HIP(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming | hipEventReleaseToDevice));
   ev_ok_ = true;
   GLIA_HIP(hipMalloc(&coefs_dev_, sizeof(double) * 3 * nm_));
 
@@ -941,7 +945,8 @@ hipEvent_t RDEnsemble::prof_event() {
     return e;
   }
   hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  // device-scope release: timing events must not add an L2 write-back between kernels
+  if (hipEventCreateWithFlags(&e, hipEventReleaseToDevice) != hipSuccess) return nullptr;
   return e;
 }
 
