@@ -27,6 +27,21 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "grad evals/s at 256^3 (batched FP32 vs plain), rel err vs FP64; HBM GB/s"
 
 
+def traffic_from_profiles(kernel, n, members):
+    """HBM bytes per launch of `kernel` (all members active), from a committed counter summary.
+
+    The summaries under profiles/ named hbm_traffic_*.txt hold lines
+    `traffic <kernel> <n> <members> <bytes per launch>`, computed from rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes scaled by the same counters read for a device copy of known size."""
+    import glob
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "hbm_traffic_*.txt"))):
+        for line in open(f):
+            p = line.split()
+            if len(p) == 5 and p[0] == "traffic" and p[1] == kernel and int(p[2]) == n and int(p[3]) == members:
+                return float(p[4])
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -142,6 +157,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 256 MiB device-to-device copy first (the reference dispatch for "
+                         "scaling FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -157,6 +175,12 @@ def main():
         print("no GPU visible", file=sys.stderr)
         return 3
     n, M, nt, dt = args.n, args.members, args.nt, args.dt
+    if args.copy_calibration:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     t0 = time.perf_counter()
     members = make_members(n, M, args.nbases, args.nnz, 100)
@@ -247,7 +271,9 @@ def main():
                    "n": n, "members": M, "nt": nt, "dt": dt, "nbases": args.nbases,
                    "checkpoint_slots": args.checkpoint_slots},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "algorithmic_bytes_per_launch": _lib.KERNEL_FIELDS[dom] * field_bytes * M,
+                     "traffic": traffic_from_profiles(dom, n, M)},
         "cpu_baseline": cpu,
         "plain_value": None if plain is None else round(plain, 4),
         "batched_vs_plain": None if plain is None else round(value / plain, 3),

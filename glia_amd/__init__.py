@@ -7,4 +7,4 @@ torch.autograd.Function.
 """
 from . import _lib  # noqa: F401
 
-__all__ = ["rd", "synth"]
+__all__ = ["rd", "synth", "autograd"]

@@ -581,29 +581,42 @@ __device__ __forceinline__ void zl_forward_store(float2* v, float2* lb, float2* 
   }
 }
 
-// Z pass of the matvec: for each component d, h_d = k * C2R(G_d)/N^3, out_d = R2C(h_d),
-// the z component multiplied by i w_z.  In place on the three slots.
+// Z pass of the matvec: for each component d, h_d = k * C2R(G_d)/N^3, out_d = R2C(h_d), the z
+// component multiplied by i w_z; in place on the three slots.  One component per block (3 blocks
+// per line group): the three blocks of a line group get ids b, b+8, b+16 so that they run on the
+// same XCD (blocks are dispatched to the 8 XCDs round-robin) at about the same time and share the
+// k(x) tile through that XCD's L2.
+#ifndef GLIA_ZD_MINW
+#define GLIA_ZD_MINW 3
+#endif
 template <int M>
-__global__ void __launch_bounds__(ZK<M>::TPB, GLIA_Z_MINW)
+__global__ void __launch_bounds__(ZK<M>::TPB, GLIA_ZD_MINW)
 k_z_diff(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ twm, const float2* __restrict__ tw2,
-         int s_gx, int s_gy, int s_gz) {
+          int s_gx, int s_gy, int s_gz) {
   const MemberDev& Mb = mem[blockIdx.y];
   if (Mb.st->done) return;
   constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT, VPT = ZK<M>::VPT;
   __shared__ float2 lb[ZK<M>::NL * ZK<M>::LP];
   __shared__ float2 twml[M], tw2l[M];
+  const int ngroups = (int)gridDim.x / 3;
+  int comp, grp;
+  if ((ngroups & 7) == 0) {
+    const int x = (int)blockIdx.x & 7, q = (int)blockIdx.x >> 3;
+    comp = q % 3;
+    grp = (q / 3) * 8 + x;
+  } else {
+    comp = (int)blockIdx.x % 3;
+    grp = (int)blockIdx.x / 3;
+  }
   const int li = threadIdx.x / NT, t = threadIdx.x % NT;
-  const int64_t base = (int64_t)blockIdx.x * ZK<M>::NL * M;
+  const int64_t base = (int64_t)grp * ZK<M>::NL * M;
   float2* line = lb + li * ZK<M>::LP;
   GPtr kf = rptr2(Mb, RS_K) + base;
   const int lo = li * M;
   const float s = (float)d.inv_ng;
-  const int slots[3] = {s_gx, s_gy, s_gz};
-  // software pipeline (when registers allow): the loads of component c+1 are in flight while
-  // component c is transformed
-  constexpr bool PIPE = VPT <= 16;
+  const int slot = comp == 0 ? s_gx : (comp == 1 ? s_gy : s_gz);
   float2 nxt[ZIt<M>::value];
-  zl_fetch<M>(nxt, cptr(Mb, slots[0]), base);
+  zl_fetch<M>(nxt, cptr(Mb, slot), base);
   float2 kv[VPT], v[VPT];
 #pragma unroll
   for (int j = 0; j < VPT; j++) {
@@ -611,17 +624,12 @@ k_z_diff(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
     kv[j] = make_float2(k2.x * s, k2.y * s);
   }
   stage_line_twiddles<M>(twml, tw2l, twm, tw2);
-  for (int comp = 0; comp < 3; comp++) {
-    if (!PIPE && comp > 0) zl_fetch<M>(nxt, cptr(Mb, slots[comp]), base);
-    zl_put<M>(lb, nxt);
-    __syncthreads();
-    if (PIPE && comp < 2) zl_fetch<M>(nxt, cptr(Mb, slots[comp + 1]), base);
-    zl_inverse<M>(v, line, t, twml, tw2l);
+  zl_put<M>(lb, nxt);
+  __syncthreads();
+  zl_inverse<M>(v, line, t, twml, tw2l);
 #pragma unroll
-    for (int j = 0; j < VPT; j++) v[j] = make_float2(kv[j].x * v[j].x, kv[j].y * v[j].y);
-    zl_forward_store<M>(v, lb, line, t, twml, tw2l, cptr(Mb, slots[comp]), base, comp == 2);
-    __syncthreads();
-  }
+  for (int j = 0; j < VPT; j++) v[j] = make_float2(kv[j].x * v[j].x, kv[j].y * v[j].y);
+  zl_forward_store<M>(v, lb, line, t, twml, tw2l, cptr(Mb, slot), base, comp == 2);
 }
 
 // Z pass for a real-space pointwise stage between an inverse and a forward transform.
@@ -1368,7 +1376,7 @@ int launch_col_plain(const LaunchCtx& c, int dir, int sign, int s_in, int s_out,
 template <int M>
 struct RunZDiff {
   static int run(const LaunchCtx& c, int s_gx, int s_gy, int s_gz) {
-    dim3 g((unsigned)((int64_t)c.d.n0 * c.d.n1 / ZK<M>::NL), c.nm);
+    dim3 g((unsigned)((int64_t)c.d.n0 * c.d.n1 / ZK<M>::NL * 3), c.nm);
     k_z_diff<M><<<g, ZK<M>::TPB, 0, c.stream>>>(c.mem, c.d, c.twm, c.tw2, s_gx, s_gy, s_gz);
     GLIA_CHECK_LAUNCH();
     return 0;
