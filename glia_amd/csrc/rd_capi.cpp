@@ -2,8 +2,12 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <chrono>
 #include <string>
+#include <thread>
 #include <vector>
+
+#include <hip/hip_runtime.h>
 
 #include "../../include/glia_rd.h"
 #include "rd_solver.h"
@@ -351,6 +355,36 @@ int glia_rd_gradient(glia_rd_t h, int mode, const double* x, double* dJ) {
 int glia_rd_objective_and_gradient(glia_rd_t h, int mode, const double* x, double* J, double* dJ) {
   if (!J || !dJ) return GLIA_RD_ERR_ARG;
   return eval(h, mode, x, J, dJ, true);
+}
+
+int glia_rd_objective_and_gradient_group(int nh, const glia_rd_t* handles, const int* modes, const double* const* x,
+                                         double* const* J, double* const* dJ, double* ms) {
+  if (nh <= 0 || !handles || !modes || !x || !J || !dJ) return GLIA_RD_ERR_ARG;
+  for (int i = 0; i < nh; i++) {
+    if (!handles[i] || !handles[i]->e || !x[i] || !J[i] || !dJ[i]) return GLIA_RD_ERR_ARG;
+    for (int k = 0; k < i; k++)
+      if (handles[k] == handles[i]) return GLIA_RD_ERR_ARG;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return GLIA_RD_ERR_STATE;
+  std::vector<int> rc(nh, 0);
+  std::vector<double> t(nh, 0.0);
+  std::vector<std::thread> th;
+  th.reserve(nh);
+  for (int i = 0; i < nh; i++) {
+    th.emplace_back([&, i]() {
+      const auto t0 = std::chrono::steady_clock::now();
+      const hipError_t e = hipSetDevice(dev);
+      rc[i] = (e != hipSuccess) ? (int)e : eval(handles[i], modes[i], x[i], J[i], dJ[i], true);
+      t[i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    });
+  }
+  for (auto& w : th) w.join();
+  if (ms)
+    for (int i = 0; i < nh; i++) ms[i] = t[i];
+  for (int i = 0; i < nh; i++)
+    if (rc[i]) return rc[i];
+  return 0;
 }
 
 int glia_rd_stats(glia_rd_t h, int member, glia_rd_stats_t* out) {

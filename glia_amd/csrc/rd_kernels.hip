@@ -21,9 +21,26 @@
 
 #include "fft2s.h"
 
-// minimum waves per SIMD requested from the compiler for the transform kernels (caps VGPRs)
+// minimum waves per SIMD requested from the compiler for the transform kernels (caps VGPRs); the
+// per-kernel values apply to plans of at most 16 values per thread, longer plans use 2
 #ifndef GLIA_COL_MINW
 #define GLIA_COL_MINW 2
+#endif
+#ifndef GLIA_XI_MINW
+#define GLIA_XI_MINW 3
+#endif
+#ifndef GLIA_YI_MINW
+#define GLIA_YI_MINW 3
+#endif
+#ifndef GLIA_YF_MINW
+#define GLIA_YF_MINW 2
+#endif
+#ifndef GLIA_XF_MINW
+#define GLIA_XF_MINW 2
+#endif
+// xf_div loads p before its transforms (1) or in its output stage (0)
+#ifndef GLIA_XF_PF
+#define GLIA_XF_PF 1
 #endif
 #ifndef GLIA_Z_MINW
 #define GLIA_Z_MINW 2
@@ -43,6 +60,7 @@ using namespace dev;
 // global_* instructions with a uniform base and 32-bit lane offsets instead of flat accesses
 #define GLIA_GLOBAL __attribute__((address_space(1)))
 typedef float v2f __attribute__((ext_vector_type(2)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 struct GRef {
   GLIA_GLOBAL v2f* p;
   __device__ __forceinline__ operator float2() const {
@@ -207,7 +225,7 @@ __device__ __forceinline__ void stage_twiddles(float2* twl, const float2* __rest
 // X pass, inverse: U = IFFT_x(p), V = IFFT_x(i w_x p).  With PUPD the search direction is
 // first updated in place: p = B r (i == 0) or p = B r + (beta/betaold) p.
 template <int L, bool PUPD>
-__global__ void __launch_bounds__(CK<L>::TPB, GLIA_COL_MINW)
+__global__ void __launch_bounds__(CK<L>::TPB, CK<L>::VPT <= 16 ? GLIA_XI_MINW : 2)
 k_xi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_in, int s_r, int s_u, int s_v) {
   const MemberDev& M = mem[blockIdx.y];
   const PcgState* st = M.st;
@@ -272,7 +290,7 @@ k_xi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
 // Y pass, inverse, gradient: Gx = IFFT_y(V), Gy = IFFT_y(i w_y U), Gz = i w_z IFFT_y(U),
 // and optionally G0 = IFFT_y(U).  Gx must not alias U.
 template <int L, bool WITH0>
-__global__ void __launch_bounds__(CK<L>::TPB, GLIA_COL_MINW)
+__global__ void __launch_bounds__(CK<L>::TPB, CK<L>::VPT <= 16 ? GLIA_YI_MINW : 2)
 k_yi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_u, int s_v, int s_gx, int s_gy,
           int s_gz, int s_g0) {
   const MemberDev& M = mem[blockIdx.y];
@@ -321,7 +339,7 @@ k_yi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
 // Y pass, forward, divergence: Yx = FFT_y(Hx), Rr = FFT_y(Q) + i w_y FFT_y(Hy).
 // Yx must not alias Q or Hy.
 template <int L>
-__global__ void __launch_bounds__(CK<L>::TPB, GLIA_COL_MINW)
+__global__ void __launch_bounds__(CK<L>::TPB, CK<L>::VPT <= 16 ? GLIA_YF_MINW : 2)
 k_yf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_hx, int s_hy, int s_q, int s_yx,
          int s_rr) {
   const MemberDev& M = mem[blockIdx.y];
@@ -374,7 +392,7 @@ k_yf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
 enum { XF_MATVEC = 0, XF_SETUP = 1, XF_D = 2 };
 
 template <int L, int MODE>
-__global__ void __launch_bounds__(CK<L>::TPB, GLIA_COL_MINW)
+__global__ void __launch_bounds__(CK<L>::TPB, CK<L>::VPT <= 16 ? GLIA_XF_MINW : 2)
 k_xf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_yx, int s_rr, int s_p, int s_out) {
   const MemberDev& M = mem[blockIdx.y];
   const PcgState* st = M.st;
@@ -389,7 +407,8 @@ k_xf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
   const float pc = (float)st->pc;
   const float wy = wnum(tl.o, d.n1), wz = (float)(tl.c0 + col);
   const double wgt = (tl.c0 + col == 0) ? 1.0 : 2.0;
-  float2 a[K::VPT], b[K::VPT], pv[K::PF ? K::VPT : 1];
+  constexpr bool PF = K::PF && GLIA_XF_PF;
+  float2 a[K::VPT], b[K::VPT], pv[PF ? K::VPT : 1];
 #pragma unroll
   for (int j = 0; j < K::VPT; j++) {
     const unsigned g = (unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col;
@@ -398,7 +417,7 @@ k_xf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
   }
   // the operand of the output stage is loaded before the transforms (a barrier would otherwise
   // keep its loads behind them)
-  if (MODE != XF_D && K::PF) {
+  if (MODE != XF_D && PF) {
 #pragma unroll
     for (int j = 0; j < K::VPT; j++) pv[j] = P[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col];
   }
@@ -415,12 +434,12 @@ k_xf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
     const float wx = wnum(row, d.n0);
     const float2 D = cadd(mul_i(cscale(a[j], wx)), b[j]);
     if (MODE == XF_MATVEC) {
-      const float2 p = K::PF ? pv[j] : P[g];
+      const float2 p = PF ? pv[j] : P[g];
       const float2 w = make_float2(p.x - 0.5f * tau * D.x, p.y - 0.5f * tau * D.y);
       O[g] = w;
       acc[0] += wgt * ((double)p.x * w.x + (double)p.y * w.y);
     } else if (MODE == XF_SETUP) {
-      const float2 x = K::PF ? pv[j] : P[g];
+      const float2 x = PF ? pv[j] : P[g];
       const float2 bb = make_float2(x.x + 0.5f * tau * D.x, x.y + 0.5f * tau * D.y);
       const float2 r = cscale(D, tau);
       O[g] = r;
@@ -804,33 +823,40 @@ k_pcg_update(const MemberDev* __restrict__ mem, Dims d, int s_x) {
   const MemberDev& M = mem[blockIdx.y];
   const PcgState* st = M.st;
   if (st->done) return;
-  float2* X = craw(M, s_x);
-  float2* R = M.c[CS_R];
-  const float2* P = M.c[CS_P];
-  const float2* W = M.c[CS_W];
+  // a lane handles a pair of adjacent entries along z (m2 is even, so both share kx and ky) with
+  // 16-byte accesses, and two pairs per loop trip so that eight loads are in flight per lane
+  GLIA_GLOBAL v4f* X = (GLIA_GLOBAL v4f*)craw(M, s_x);
+  GLIA_GLOBAL v4f* R = (GLIA_GLOBAL v4f*)M.c[CS_R];
+  const GLIA_GLOBAL v4f* P = (const GLIA_GLOBAL v4f*)M.c[CS_P];
+  const GLIA_GLOBAL v4f* W = (const GLIA_GLOBAL v4f*)M.c[CS_W];
   const float a = (float)st->alpha;
   const float pc = (float)st->pc;
+  const unsigned npairs = (unsigned)(d.nspec >> 1), hm = (unsigned)(d.m2 >> 1), n1 = (unsigned)d.n1;
   double acc[2] = {0.0, 0.0};
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.nspec; i += stride) {
-    const int kz = (int)(i % d.m2);
-    const int64_t q = i / d.m2;
-    const int ky = (int)(q % d.n1);
-    const int kx = (int)(q / d.n1);
-    const float2 p = P[i], w = W[i];
-    float2 x = X[i], r = R[i];
-    x.x += a * p.x;
-    x.y += a * p.y;
-    r.x -= a * w.x;
-    r.y -= a * w.y;
+  auto finish = [&](unsigned i, v4f x, v4f r, v4f p, v4f w) {
+    x += a * p;
+    r -= a * w;
     X[i] = x;
     R[i] = r;
-    const float f = precf(pc, wnum(kx, d.n0), wnum(ky, d.n1), (float)kz);
-    const double wgt = (kz == 0) ? 1.0 : 2.0;
-    const double r2 = (double)r.x * r.x + (double)r.y * r.y;
-    acc[0] += wgt * (double)f * f * r2;
-    acc[1] += wgt * (double)f * r2;
+    const unsigned line = i / hm, kz = 2u * (i - line * hm);
+    const unsigned kx = line / n1, ky = line - kx * n1;
+    const float wx = wnum((int)kx, d.n0), wy = wnum((int)ky, d.n1);
+    const float f0 = precf(pc, wx, wy, (float)kz), f1 = precf(pc, wx, wy, (float)(kz + 1));
+    const double w0 = (kz == 0) ? 1.0 : 2.0;
+    const double r0 = (double)r.x * r.x + (double)r.y * r.y, r1 = (double)r.z * r.z + (double)r.w * r.w;
+    acc[0] += w0 * (double)f0 * f0 * r0 + 2.0 * (double)f1 * f1 * r1;
+    acc[1] += w0 * (double)f0 * r0 + 2.0 * (double)f1 * r1;
+  };
+  const unsigned stride = gridDim.x * blockDim.x;
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + stride < npairs; i += 2 * stride) {
+    const unsigned j = i + stride;
+    const v4f p0 = P[i], w0 = W[i], x0 = X[i], r0 = R[i];
+    const v4f p1 = P[j], w1 = W[j], x1 = X[j], r1 = R[j];
+    finish(i, x0, r0, p0, w0);
+    finish(j, x1, r1, p1, w1);
   }
+  if (i < npairs) finish(i, X[i], R[i], P[i], W[i]);
   block_reduce_store<2>(acc, M.part + (int64_t)blockIdx.x * 2);
 }
 

@@ -83,9 +83,9 @@ LaunchCtx RDEnsemble::ctx() const {
 int RDEnsemble::init() {
   if (nm_ < 1) return fail(-2, "nmembers must be >= 1");
   if (!supported_line(d_.n0) || !supported_line(d_.n1) || !supported_z(d_.n2))
-    return fail(-2, "unsupported grid: n0, n1 in {32,48,64,96,128,192,256}, n2 in {32,64,96,128,192,256}");
+    return fail(-4, "unsupported grid: n0, n1 in {32,48,64,96,128,192,256}, n2 in {32,64,96,128,192,256}");
   if (z_blocks(d_) <= 0 || ((int64_t)d_.n0 * d_.n1) % z_lines_per_block(d_.m2) != 0)
-    return fail(-2, "unsupported grid: n0*n1 must be a multiple of the z-pass line group");
+    return fail(-4, "unsupported grid: n0*n1 must be a multiple of the z-pass line group");
   if (cfg_.nt < 1 || !(cfg_.dt > 0)) return fail(-2, "nt >= 1 and dt > 0 required");
   if (cfg_.maxit < 1) return fail(-2, "pcg maxit >= 1 required");
   if (cfg_.checkpoint_slots < 0) return fail(-2, "checkpoint_slots >= 0 required");

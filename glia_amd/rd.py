@@ -247,6 +247,35 @@ class RDSolver:
         self._chk(self.L.glia_rd_sync(self.h))
 
 
+def objective_and_gradient_group(solvers, xs, modes=None):
+    """Evaluate J and dJ of several RDSolver handles (possibly of different grids) concurrently.
+
+    solvers: list of distinct RDSolver; xs[i]: list of per-member parameter vectors of solvers[i];
+    modes[i]: evaluation mode (default MODE_ALL).  Returns (J list, dJ list, ms) with ms[i] the
+    wall time of handle i in milliseconds."""
+    nh = len(solvers)
+    if nh == 0 or len(xs) != nh:
+        raise ValueError("one x list per solver")
+    L = solvers[0].L
+    modes = [_lib.MODE_ALL] * nh if modes is None else list(modes)
+    flats = [s._flat_x(x) for s, x in zip(solvers, xs)]
+    Js = [np.zeros(s.nmembers) for s in solvers]
+    dJs = [np.zeros_like(f) for f in flats]
+    ms = np.zeros(nh)
+    vp = ctypes.c_void_p
+    H = (vp * nh)(*[s.h.value for s in solvers])
+    M = (ctypes.c_int * nh)(*modes)
+    X = (vp * nh)(*[f.ctypes.data for f in flats])
+    JP = (vp * nh)(*[j.ctypes.data for j in Js])
+    DP = (vp * nh)(*[d.ctypes.data for d in dJs])
+    rc = L.glia_rd_objective_and_gradient_group(nh, ctypes.cast(H, vp), ctypes.cast(M, vp), ctypes.cast(X, vp),
+                                                ctypes.cast(JP, vp), ctypes.cast(DP, vp), ms.ctypes.data_as(vp))
+    if rc != 0:
+        msgs = [s.L.glia_rd_last_error(s.h).decode() for s in solvers]
+        raise GliaError(rc, "; ".join(m for m in msgs if m))
+    return Js, [s._split(d) for s, d in zip(solvers, dJs)], ms
+
+
 class DiffusionSolver:
     """DiffusionSolver (src/pde/DiffusionSolver.cpp): CN half step by PCG on one field."""
 

@@ -61,7 +61,7 @@ enum {
 /* Parameters, TumorParameters, Grid and the DiffusionSolver KSP settings
    (include/Parameters.h:155-451; src/pde/DiffusionSolver.cpp:25-39) */
 typedef struct glia_rd_config {
-  int n[3];              /* grid; each of n0, n1, n2 in {32,48,64,96,128,192,256} */
+  int n[3];              /* grid; n0, n1 in {32,48,64,96,128,192,256}, n2 in {32,64,96,128,192,256} */
   int nmembers;          /* ensemble members sharing this grid (1 = plain solve) */
   double dt;             /* TumorParameters::dt_ */
   int nt;                /* TumorParameters::nt_ */
@@ -154,6 +154,16 @@ int glia_rd_solve_adjoint(glia_rd_t h, int linearized);
 int glia_rd_objective(glia_rd_t h, int mode, const double* x, double* J);
 int glia_rd_gradient(glia_rd_t h, int mode, const double* x, double* dJ);
 int glia_rd_objective_and_gradient(glia_rd_t h, int mode, const double* x, double* J, double* dJ);
+/* The same evaluation for several handles at once, which may have different grids (an ensemble
+   of uneven sizes, e.g. 128^3, 192^3 and 256^3 members).  Each handle is evaluated by its own
+   host thread on its own stream, so the GPU overlaps them; handles must be distinct and should
+   not share a stream.  x[i], J[i], dJ[i] are laid out as for glia_rd_objective_and_gradient on
+   handles[i]; ms (optional, nh entries) receives each handle's wall time in milliseconds.  Returns
+   the first nonzero code in handle order.  No single reference counterpart: it batches
+   DerivativeOperatorsRD::evaluateObjectiveAndGradient (src/grad/DerivativeOperatorsRD.cpp:130-226)
+   over independent problems. */
+int glia_rd_objective_and_gradient_group(int nh, const glia_rd_t* handles, const int* modes, const double* const* x,
+                                         double* const* J, double* const* dJ, double* ms);
 
 int glia_rd_stats(glia_rd_t h, int member, glia_rd_stats_t* out);
 int glia_rd_reset_stats(glia_rd_t h);

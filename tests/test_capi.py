@@ -55,5 +55,8 @@ def test_create_rejects_bad_arguments_without_touching_the_gpu():
     c.order = 1
     h = ctypes.c_void_p()
     assert lib.glia_rd_create(ctypes.byref(c), ctypes.byref(h)) == _lib.ERR_UNSUPPORTED
+    c.order = 2
+    c.n[2] = 48  # the packed half-spectrum row (24) is not a multiple of the 16-column tile
+    assert lib.glia_rd_create(ctypes.byref(c), ctypes.byref(h)) == _lib.ERR_UNSUPPORTED
     assert lib.glia_rd_create(None, ctypes.byref(h)) == _lib.ERR_ARG
     assert lib.glia_rd_destroy(None) == _lib.ERR_ARG

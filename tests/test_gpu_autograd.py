@@ -58,14 +58,18 @@ def test_autograd_backward_equals_gradient(problem):
     import torch
     from glia_amd.autograd import rd_objective
     print("case autograd backward", flush=True)
+    # two fresh handles: the preconditioner of an evaluation depends on the solver context's dt,
+    # which (as in the reference) is the last CN step size after the first solve
     s = make_solver(problem)
     x = torch.tensor(problem["x"][None, :], dtype=torch.float64, requires_grad=True)
     J = rd_objective(x, s)
     J.sum().backward()
-    J2, g2 = s.objective_and_gradient([problem["x"]])
-    assert float(J[0]) == J2[0]
+    s2 = make_solver(problem)
+    J2, g2 = s2.objective_and_gradient([problem["x"]])
+    assert float(J.detach()[0]) == J2[0]
     assert np.array_equal(x.grad.numpy()[0], g2[0])
     s.close()
+    s2.close()
 
 
 def test_checkpointing_10_states_bitwise(problem):
@@ -108,6 +112,7 @@ def test_finite_differences_10_directions(problem):
     print("case finite differences", flush=True)
     s = make_solver(problem, checkpoint_slots=10)
     x0 = problem["x"]
+    s.objective([x0])  # every later evaluation then starts from the same solver-context dt
     x = torch.tensor(x0[None, :], dtype=torch.float64, requires_grad=True)
     rd_objective(x, s).sum().backward()
     g = x.grad.numpy()[0]
