@@ -202,6 +202,69 @@ def batch_config(args):
     return 0
 
 
+def diff_config(args):
+    """The differentiation configuration at 256^3: nt = 100 steps of dt = 0.01, 128 Gaussian bases
+    (8 nonzero), gradient w.r.t. p, kappa and rho through the torch.autograd binding; binomial
+    checkpointing with 10 stored states against store-all (time, device memory, bitwise equality);
+    central finite differences (h = 1e-3) along 10 random unit directions."""
+    import torch
+    from glia_amd import rd
+    from glia_amd.autograd import rd_objective
+    n, nt, dt, nb = 256, 100, 0.01, 128
+    members = make_members(n, 1, nb, 8, 700)
+    out = {}
+    grads = {}
+    for slots in (0, 10):
+        torch.cuda.synchronize()
+        free0 = torch.cuda.mem_get_info()[0]
+        s = build_solver(members, n, nt, dt, slots)
+        make_data(s, members)
+        free1 = torch.cuda.mem_get_info()[0]
+        x0 = xvec(members[0])
+        s.objective([x0])  # warm-up; later evaluations share the solver-context dt
+        x = torch.tensor(x0[None, :], dtype=torch.float64, requires_grad=True)
+        s.sync()
+        t0 = time.perf_counter()
+        J = rd_objective(x, s)
+        J.sum().backward()
+        s.sync()
+        t_grad = time.perf_counter() - t0
+        g = x.grad.numpy()[0].copy()
+        grads[slots] = (float(J.detach()[0]), g)
+        rec = {"grad_eval_s": round(t_grad, 3), "device_bytes": int(free0 - free1)}
+        if slots == 10:
+            rng = np.random.default_rng(3)
+            errs = []
+            t0 = time.perf_counter()
+            for _ in range(10):
+                v = rng.standard_normal(x0.size)
+                v /= np.linalg.norm(v)
+                h = 1e-3
+                fd = (s.objective([x0 + h * v])[0] - s.objective([x0 - h * v])[0]) / (2 * h)
+                ad = float(g @ v)
+                errs.append(abs(fd - ad) / max(abs(ad), 1e-300))
+            rec["fd_rel_err"] = [float("%.3g" % e) for e in errs]
+            rec["fd_s"] = round(time.perf_counter() - t0, 2)
+        out["checkpoint_%d" % slots if slots else "store_all"] = rec
+        s.close()
+        del s
+    out["bitwise_equal"] = bool(grads[0][0] == grads[10][0] and np.array_equal(grads[0][1], grads[10][1]))
+    line = {
+        "metric": METRIC,
+        "value": round(1.0 / out["checkpoint_10"]["grad_eval_s"], 4),
+        "unit": "grad evals/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(out["checkpoint_10"]["grad_eval_s"] * 1000, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "differentiation: one 256^3 member, nt=100, dt=0.01, 128 bases (8 nonzero), "
+                               "torch.autograd, 10 checkpoint slots vs store-all, FD h=1e-3 on 10 directions",
+                   "n": n, "nt": nt, "dt": dt, "nbases": nb},
+        "diff": out,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,9 +280,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--config", choices=["ensemble256", "batch"], default="ensemble256",
+    ap.add_argument("--config", choices=["ensemble256", "batch", "diff"], default="ensemble256",
                     help="ensemble256: the 256^3 members (default, the headline metric); batch: the "
-                         "uneven-grid batching configuration")
+                         "uneven-grid batching configuration; diff: the 256^3 differentiation "
+                         "configuration (checkpointing, finite differences)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device-to-device copy first (the reference dispatch for "
                          "scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -239,6 +303,8 @@ def main():
         return 3
     if args.config == "batch":
         return batch_config(args)
+    if args.config == "diff":
+        return diff_config(args)
     n, M, nt, dt = args.n, args.members, args.nt, args.dt
     if args.copy_calibration:
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")

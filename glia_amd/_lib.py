@@ -91,6 +91,7 @@ SIGNATURES = {
     "glia_rd_objective_and_gradient_group": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "glia_rd_stats": (_i, [_vp, _i, ctypes.POINTER(Stats)]),
     "glia_rd_reset_stats": (_i, [_vp]),
+    "glia_rd_use_graphs": (_i, [_vp, _i]),
     "glia_rd_profile": (_i, [_vp, _i]),
     "glia_rd_profile_get": (_i, [_vp, _i, _pd, ctypes.POINTER(ctypes.c_longlong)]),
 }

@@ -406,6 +406,11 @@ int glia_rd_stats(glia_rd_t h, int member, glia_rd_stats_t* out) {
   return 0;
 }
 
+int glia_rd_use_graphs(glia_rd_t h, int enable) {
+  GLIA_H(h);
+  return ret(h, h->e->set_use_graphs(enable != 0));
+}
+
 int glia_rd_profile(glia_rd_t h, int enable) {
   GLIA_H(h);
   return ret(h, h->e->profile_enable(enable != 0));

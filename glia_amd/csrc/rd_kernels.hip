@@ -609,7 +609,7 @@ __device__ __forceinline__ void zl_forward_store(float2* v, float2* lb, float2* 
 #define GLIA_ZD_MINW 3
 #endif
 template <int M>
-__global__ void __launch_bounds__(ZK<M>::TPB, GLIA_ZD_MINW)
+__global__ void __launch_bounds__(ZK<M>::TPB, ZK<M>::VPT <= 16 ? GLIA_ZD_MINW : 2)
 k_z_diff(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ twm, const float2* __restrict__ tw2,
           int s_gx, int s_gy, int s_gz) {
   const MemberDev& Mb = mem[blockIdx.y];

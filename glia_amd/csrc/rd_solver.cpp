@@ -212,6 +212,7 @@ int RDEnsemble::upload_members() {
 
 RDEnsemble::~RDEnsemble() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
   for (auto& r : prof_recs_) {
     (void)hipEventDestroy(r.a);
     if (r.b) (void)hipEventDestroy(r.b);
@@ -568,26 +569,15 @@ int RDEnsemble::cn_solve(int sx, double tau) {
   };
   for (int it = 0; it < cfg_.maxit && !all_done; it++) {
     stage = it + 1;
-    prof_mark(PK_XI_PUPD, true);
-    GLIA_TRYC(launch_xi_grad(c, CS_P, CS_R, CS_T0, CS_T1, true));
-    prof_mark(PK_XI_PUPD, false);
-    prof_mark(PK_YI_GRAD, true);
-    GLIA_TRYC(launch_yi_grad(c, CS_T0, CS_T1, CS_T2, CS_T3, CS_T4, -1));
-    prof_mark(PK_YI_GRAD, false);
-    prof_mark(PK_Z_DIFF, true);
-    GLIA_TRYC(launch_z_diff(c, CS_T2, CS_T3, CS_T4));
-    prof_mark(PK_Z_DIFF, false);
-    prof_mark(PK_YF_DIV, true);
-    GLIA_TRYC(launch_yf_div(c, CS_T2, CS_T3, CS_T4, CS_T0, CS_T1));
-    prof_mark(PK_YF_DIV, false);
-    prof_mark(PK_XF_MATVEC, true);
-    GLIA_TRYC(launch_xf_div(c, XFM_MATVEC, CS_T0, CS_T1, CS_P, CS_W));
-    prof_mark(PK_XF_MATVEC, false);
-    GLIA_TRYC(launch_pcg_fin_dpi(c, nbx));
-    prof_mark(PK_PCG_UPDATE, true);
-    GLIA_TRYC(launch_pcg_update(c, sx));
-    prof_mark(PK_PCG_UPDATE, false);
-    GLIA_TRYC(launch_pcg_fin_update(c, pcg_update_blocks(), hflag_dev_, stage % kFlagSlots));
+    if (use_graphs_ && !prof_on_) {
+      hipGraphExec_t g = nullptr;
+      rc = iteration_graph(sx, stage % kFlagSlots, &g);
+      if (rc) return rc;
+      GLIA_HIP(hipGraphLaunch(g, stream_));
+    } else {
+      rc = launch_iteration(sx, stage % kFlagSlots);
+      if (rc) return rc;
+    }
     GLIA_HIP(hipEventRecord(ev_[stage % kFlagSlots], stream_));
     while (!all_done && checked < stage - kLookahead) {
       checked++;
@@ -595,6 +585,66 @@ int RDEnsemble::cn_solve(int sx, double tau) {
       if (rc) return rc;
     }
   }
+  return 0;
+}
+
+// One CG iteration: matvec w = A p (five transform kernels), alpha, x/r update, the PETSc
+// convergence test with its flag written to host slot 'slot'.  Every argument is fixed for a
+// handle except the solved slot sx and the flag slot, so the sequence can be captured once per
+// (sx, slot) and replayed.
+int RDEnsemble::launch_iteration(int sx, int slot) {
+  LaunchCtx c = ctx();
+  const int nbx = xf_div_blocks(d_);
+  prof_mark(PK_XI_PUPD, true);
+  GLIA_TRYC(launch_xi_grad(c, CS_P, CS_R, CS_T0, CS_T1, true));
+  prof_mark(PK_XI_PUPD, false);
+  prof_mark(PK_YI_GRAD, true);
+  GLIA_TRYC(launch_yi_grad(c, CS_T0, CS_T1, CS_T2, CS_T3, CS_T4, -1));
+  prof_mark(PK_YI_GRAD, false);
+  prof_mark(PK_Z_DIFF, true);
+  GLIA_TRYC(launch_z_diff(c, CS_T2, CS_T3, CS_T4));
+  prof_mark(PK_Z_DIFF, false);
+  prof_mark(PK_YF_DIV, true);
+  GLIA_TRYC(launch_yf_div(c, CS_T2, CS_T3, CS_T4, CS_T0, CS_T1));
+  prof_mark(PK_YF_DIV, false);
+  prof_mark(PK_XF_MATVEC, true);
+  GLIA_TRYC(launch_xf_div(c, XFM_MATVEC, CS_T0, CS_T1, CS_P, CS_W));
+  prof_mark(PK_XF_MATVEC, false);
+  GLIA_TRYC(launch_pcg_fin_dpi(c, nbx));
+  prof_mark(PK_PCG_UPDATE, true);
+  GLIA_TRYC(launch_pcg_update(c, sx));
+  prof_mark(PK_PCG_UPDATE, false);
+  GLIA_TRYC(launch_pcg_fin_update(c, pcg_update_blocks(), hflag_dev_, slot));
+  return 0;
+}
+
+int RDEnsemble::iteration_graph(int sx, int slot, hipGraphExec_t* out) {
+  const auto key = std::make_pair(sx, slot);
+  auto it = graphs_.find(key);
+  if (it != graphs_.end()) {
+    *out = it->second;
+    return 0;
+  }
+  GLIA_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  const int rc = launch_iteration(sx, slot);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(stream_, &g);
+  if (rc) {
+    if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  GLIA_HIP(e);
+  hipGraphExec_t ex = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  GLIA_HIP(ei);
+  graphs_[key] = ex;
+  *out = ex;
+  return 0;
+}
+
+int RDEnsemble::set_use_graphs(bool on) {
+  use_graphs_ = on;
   return 0;
 }
 

@@ -87,6 +87,7 @@ class RDEnsemble {
 
   // profiling mode: per-kind launch count and summed event time (ms) since enabled
   int profile_enable(bool on);
+  int set_use_graphs(bool on);
   int profile_get(int kind, double* ms, long long* count);
 
  private:
@@ -98,6 +99,8 @@ class RDEnsemble {
   LaunchCtx ctx() const;
 
   int cn_solve(int sx, double tau);
+  int launch_iteration(int sx, int slot);                       // the launches of one CG iteration
+  int iteration_graph(int sx, int slot, hipGraphExec_t* out);   // the same, captured once and cached
   int matvec_setup(int sx);
   int forward_step(int sx, int step, bool store_half);
   int forward_transform_real(int rslot, int cslot);  // R2C of a real slot into a complex slot
@@ -153,6 +156,8 @@ class RDEnsemble {
     int kind;
     hipEvent_t a, b;
   };
+  bool use_graphs_ = true;  // replay CG iterations from captured graphs (not while profiling)
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs_;  // (state slot, flag slot) -> graph
   bool prof_on_ = false;
   std::vector<ProfRec> prof_recs_;
   std::vector<hipEvent_t> prof_pool_;

@@ -227,6 +227,10 @@ class RDSolver:
         self._chk(self.L.glia_rd_stats(self.h, member, ctypes.byref(s)))
         return {f: getattr(s, f) for f, _ in _lib.Stats._fields_}
 
+    def use_graphs(self, enable=True):
+        """Replay CG iterations from captured HIP graphs (default) or launch kernel by kernel."""
+        self._chk(self.L.glia_rd_use_graphs(self.h, int(bool(enable))))
+
     def profile(self, enable=True):
         self._chk(self.L.glia_rd_profile(self.h, int(bool(enable))))
 

@@ -179,6 +179,10 @@ enum {
   GLIA_RD_K_XF_MATVEC = 4, /* X pass: forward, w = p - tau/2 D p, <p, w> */
   GLIA_RD_K_PCG_UPDATE = 5 /* x, r update with ||B r||^2 and <B r, r> */
 };
+/* Replay each CG iteration from a captured HIP graph (default on; one graph per solved slot and
+   flag slot, built on first use).  Profiling runs always launch kernel by kernel.  No reference
+   counterpart: the reference launches every kernel from the host. */
+int glia_rd_use_graphs(glia_rd_t h, int enable);
 int glia_rd_profile(glia_rd_t h, int enable);
 int glia_rd_profile_get(glia_rd_t h, int kind, double* ms, long long* launches);
 
