@@ -357,6 +357,22 @@ int glia_rd_objective_and_gradient(glia_rd_t h, int mode, const double* x, doubl
   return eval(h, mode, x, J, dJ, true);
 }
 
+int glia_rd_hessian(glia_rd_t h, const double* x, double* y) {
+  GLIA_H(h);
+  if (!x || !y) return GLIA_RD_ERR_ARG;
+  std::vector<std::vector<double>> xv, yv;
+  if (!split_x(h->e.get(), x, 2, xv)) return GLIA_RD_ERR_ARG;
+  for (auto& v : xv) v.resize(v.size() - 2);  // p~ only: the kappa and rho directions are ignored
+  const int rc = h->e->hessian_p(xv, yv);
+  if (rc) return ret(h, rc);
+  int64_t off = 0;
+  for (int m = 0; m < h->e->nmembers(); m++) {
+    std::memcpy(y + off, yv[m].data(), sizeof(double) * yv[m].size());
+    off += (int64_t)yv[m].size();
+  }
+  return 0;
+}
+
 int glia_rd_objective_and_gradient_group(int nh, const glia_rd_t* handles, const int* modes, const double* const* x,
                                          double* const* J, double* const* dJ, double* ms) {
   if (nh <= 0 || !handles || !modes || !x || !J || !dJ) return GLIA_RD_ERR_ARG;

@@ -659,7 +659,8 @@ k_z_diff(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
 //   ZR_C2R      : real out = C2R/N^3 (no forward transform)
 //   ZR_R2C      : packed out = R2C(real in)
 //   ZR_MISFIT   : c = C2R/N^3 -> CT; t = O c - d1; partial sum t^2; a = -O t; out = R2C(a)
-enum { ZR_REACT = 0, ZR_REACT_ADJ = 1, ZR_C2R = 2, ZR_R2C = 3, ZR_MISFIT = 4 };
+//   ZR_OBS2     : c = C2R/N^3; a = -O (O c); out = R2C(a)  (final condition of the incremental adjoint)
+enum { ZR_REACT = 0, ZR_REACT_ADJ = 1, ZR_C2R = 2, ZR_R2C = 3, ZR_MISFIT = 4, ZR_OBS2 = 5 };
 
 template <int M, int MODE>
 __global__ void __launch_bounds__(ZK<M>::TPB, GLIA_Z_MINW)
@@ -683,7 +684,7 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   const float s = (float)d.inv_ng;
   double acc[1] = {0.0};
   float2 v[VPT];
-  GPtr op1 = rptr2(Mb, MODE == ZR_MISFIT ? RS_OBS : RS_RHO) + base;
+  GPtr op1 = rptr2(Mb, (MODE == ZR_MISFIT || MODE == ZR_OBS2) ? RS_OBS : RS_RHO) + base;
   GPtr op2 = MODE == ZR_MISFIT ? rptr2(Mb, RS_D1) + base : rf;
   GPtr ctp = rptr2(Mb, RS_CT) + base;
   float2 o1[PF ? VPT : 1], o2[PF && TWO ? VPT : 1];
@@ -726,6 +727,8 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
           c.x = c.x * fx / (ax * ax);
           c.y = c.y * fy / (ay * ay);
           v[j] = c;
+        } else if (MODE == ZR_OBS2) {
+          v[j] = make_float2(-(a1.x * (a1.x * c.x)), -(a1.y * (a1.y * c.y)));
         } else if (MODE == ZR_MISFIT) {
           ctp[gi] = c;
           const float2 dd = PF ? o2[j] : op2[gi];
@@ -1423,6 +1426,7 @@ struct RunZPoint {
       case ZR_C2R: GLIA_ZP(ZR_C2R); break;
       case ZR_R2C: GLIA_ZP(ZR_R2C); break;
       case ZR_MISFIT: GLIA_ZP(ZR_MISFIT); break;
+      case ZR_OBS2: GLIA_ZP(ZR_OBS2); break;
       default: return -1;
     }
 #undef GLIA_ZP

@@ -20,7 +20,7 @@ struct LaunchCtx {
 
 // modes of launch_xf_div and launch_z_point (values match the kernel enums)
 enum { XFM_MATVEC = 0, XFM_SETUP = 1, XFM_D = 2 };
-enum { ZPM_REACT = 0, ZPM_REACT_ADJ = 1, ZPM_C2R = 2, ZPM_R2C = 3, ZPM_MISFIT = 4 };
+enum { ZPM_REACT = 0, ZPM_REACT_ADJ = 1, ZPM_C2R = 2, ZPM_R2C = 3, ZPM_MISFIT = 4, ZPM_OBS2 = 5 };
 enum { FINK_FILTER = 0, FINK_KSUM = 1, FINK_MISFIT = 2, FINK_C0SQ = 3, FINK_QUAD = 4 };
 
 bool supported_line(int L);

@@ -773,7 +773,82 @@ int RDEnsemble::adjoint_step(int i, int cslot_c) {
   GLIA_TRYC(launch_col_plain(c, 0, -1, CS_A, CS_A, false));
   rc = cn_solve(CS_A, dt / 2.0);
   if (rc) return rc;
-  return quad(cslot_c, CS_A, (i == 0) ? 0.5 : 1.0);
+  return quad_on_ ? quad(cslot_c, CS_A, (i == 0) ? 0.5 : 1.0) : 0;
+}
+
+// PdeOperatorsRD::solveState(1) step (src/pde/PdeOperators.cpp:264-300 with reaction(1, i),
+// :140-190): D_{dt/2}; c~ *= e^{rho dt} / (c_i e^{rho dt} + 1 - c_i)^2 with c_i the state at the
+// start of the step (not the half-step state the adjoint linearizes around); D_{dt/2}
+int RDEnsemble::tangent_step(int sx, int cslot_c) {
+  const double dt = cfg_.dt;
+  int rc = cn_solve(sx, dt / 2.0);
+  if (rc) return rc;
+  rc = inverse_to_real(cslot_c, CS_H, RS_CH);
+  if (rc) return rc;
+  LaunchCtx c = ctx();
+  GLIA_TRYC(launch_col_plain(c, 0, +1, sx, sx, false));
+  GLIA_TRYC(launch_col_plain(c, 1, +1, sx, sx, false));
+  GLIA_TRYC(launch_z_point(c, ZPM_REACT_ADJ, sx, sx, RS_CH, -1, (float)dt, false));
+  GLIA_TRYC(launch_col_plain(c, 1, -1, sx, sx, false));
+  GLIA_TRYC(launch_col_plain(c, 0, -1, sx, sx, false));
+  return cn_solve(sx, dt / 2.0);
+}
+
+// DerivativeOperatorsRD::evaluateHessian, no diffusivity inversion, L2 regularization
+// (src/grad/DerivativeOperatorsRD.cpp:420-436): c~0 = Phi p~; tangent-linear forward
+// (solveState(1)); a~(T) = -O^T O c~(T); adjoint sweep (solveAdjoint(2)); y = beta h^3 Phi^T Phi p~
+// - h^3 Phi^T a~0.  The linearization point is the state history of the last evaluation; with
+// checkpointing the base states are recomputed in lockstep with the tangent states.
+int RDEnsemble::hessian_p(const std::vector<std::vector<double>>& ptilde, std::vector<std::vector<double>>& y) {
+  if (stored_.find(0) == stored_.end()) return fail(-6, "the Hessian needs the state history of an evaluation first");
+  int rc = phi_apply(ptilde);
+  if (rc) return rc;
+  rc = forward_transform_real(RS_C0, CS_A);
+  if (rc) return rc;
+  if (hist_all_) {
+    for (int i = 0; i < cfg_.nt; i++) {
+      rc = tangent_step(CS_A, state_slot(i));
+      if (rc) return rc;
+    }
+  } else {
+    GLIA_TRYC(launch_copy(ctx(), state_slot(stored_[0]), CS_X));
+    for (int i = 0; i < cfg_.nt; i++) {
+      rc = tangent_step(CS_A, CS_X);
+      if (rc) return rc;
+      rc = forward_step(CS_X, i, false);
+      if (rc) return rc;
+    }
+  }
+  {
+    LaunchCtx c = ctx();
+    GLIA_TRYC(launch_col_plain(c, 0, +1, CS_A, CS_T5, false));
+    GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T5, CS_T5, false));
+    GLIA_TRYC(launch_z_point(c, ZPM_OBS2, CS_T5, CS_T5, -1, -1, 0.f, false));
+    GLIA_TRYC(launch_col_plain(c, 1, -1, CS_T5, CS_T5, false));
+    GLIA_TRYC(launch_col_plain(c, 0, -1, CS_T5, CS_A, false));
+  }
+  quad_on_ = false;
+  if (hist_all_) {
+    for (int i = cfg_.nt - 1; i >= 0 && rc == 0; i--) rc = adjoint_step(i, state_slot(i));
+  } else {
+    rc = reverse(0, cfg_.nt, stored_[0], npool_ - 1);
+  }
+  quad_on_ = true;
+  if (rc) return rc;
+  rc = inverse_to_real(CS_A, CS_T5, RS_A0);
+  if (rc) return rc;
+  std::vector<std::vector<double>> t1, t2;
+  rc = phi_apply_transpose(RS_C0, t1);
+  if (rc) return rc;
+  rc = phi_apply_transpose(RS_A0, t2);
+  if (rc) return rc;
+  const double h3 = lebesgue();
+  y.assign(nm_, {});
+  for (int m = 0; m < nm_; m++) {
+    y[m].assign(np_[m] + 2, 0.0);
+    for (int i = 0; i < np_[m]; i++) y[m][i] = cfg_.beta * h3 * t1[m][i] - h3 * t2[m][i];
+  }
+  return 0;
 }
 
 // reverse the steps b-1 ... a with c_a in pool entry pa and free_slots further entries free

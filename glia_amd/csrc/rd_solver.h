@@ -78,6 +78,9 @@ class RDEnsemble {
   int solve_adjoint(int linearized);
   int objective_and_gradient(int mode, const std::vector<std::vector<double>>& x, bool want_grad,
                              std::vector<double>& J, std::vector<std::vector<double>>& dJ);
+  // Gauss-Newton Hessian matvec, p block (evaluateHessian without diffusivity inversion): needs
+  // the state history of a previous evaluation at the linearization point
+  int hessian_p(const std::vector<std::vector<double>>& ptilde, std::vector<std::vector<double>>& y);
   int stats(int m, MemberStats* s);
   int basis_size(int m) const { return (m >= 0 && m < nm_) ? np_[m] : -1; }
   int reset_counters();
@@ -106,6 +109,7 @@ class RDEnsemble {
   int forward_transform_real(int rslot, int cslot);  // R2C of a real slot into a complex slot
   int inverse_to_real(int cslot, int work, int rslot);
   int adjoint_step(int i, int cslot_c);
+  int tangent_step(int sx, int cslot_c);  // linearized forward step around the state in cslot_c
   int quad(int cslot_c, int cslot_a, double w);
   int misfit_and_final_adjoint(int cslot_c);
   int clear_done();
@@ -156,6 +160,7 @@ class RDEnsemble {
     int kind;
     hipEvent_t a, b;
   };
+  bool quad_on_ = true;     // adjoint steps accumulate the kappa/rho gradient quadratures
   bool use_graphs_ = true;  // replay CG iterations from captured graphs (not while profiling)
   std::map<std::pair<int, int>, hipGraphExec_t> graphs_;  // (state slot, flag slot) -> graph
   bool prof_on_ = false;

@@ -991,6 +991,26 @@ static void grad_kappa_rho(orc_t *o, double *gk, double *gr) {
   *gr = vdot(o, o->wm, P) * o->lebesgue + vdot(o, o->gm, P) * o->lebesgue * o->r_gm_wm;
 }
 
+/* Gauss-Newton Hessian matvec, p block: DerivativeOperatorsRD::evaluateHessian without diffusivity
+   inversion and with L2 regularization (src/grad/DerivativeOperatorsRD.cpp:420-436).  Needs the
+   state history of a previous orc_solve_state(o, 0) at the current x (the linearization point).
+     c~0 = Phi p~;  c~(T) = tangent-linear forward (solveState(1), reaction linearized around c_[i]);
+     a~(T) = -O^T O c~(T);  a~0 = adjoint sweep (solveAdjoint(2));
+     y = beta h^3 Phi^T (Phi p~) - h^3 Phi^T a~0.                                                  */
+void orc_hessian_p(orc_t *o, const double *ptilde, double *y) {
+  const int np = o->np;
+  double *ptemp = (double *)calloc((size_t)(np > 0 ? np : 1), sizeof(double));
+  orc_phi_apply(o, ptilde, o->c_0);
+  orc_solve_state(o, 1);
+  for (int64_t i = 0; i < o->ng; i++) o->p_t[i] = -(o->obs[i] * (o->obs[i] * o->c_t[i]));
+  orc_solve_adjoint(o, 2);
+  orc_phi_applyT(o, o->p_0, ptemp);
+  for (int i = 0; i < np; i++) ptemp[i] *= o->lebesgue;
+  orc_phi_applyT(o, o->c_0, y);
+  for (int i = 0; i < np; i++) y[i] = y[i] * (o->beta * o->lebesgue) - ptemp[i];
+  free(ptemp);
+}
+
 /* evaluateObjectiveAndGradient (src/grad/DerivativeOperatorsRD.cpp:130-226) */
 void orc_objective_and_gradient(orc_t *o, const double *x, int mode, double *J, double *dJ) {
   const int np = o->np;

@@ -79,6 +79,7 @@ def _load():
         "orc_set_opt": (None, [vp, d, d, d]),
         "orc_objective": (d, [vp, _dp, i]),
         "orc_objective_and_gradient": (None, [vp, _dp, i, _dp, _dp]),
+        "orc_hessian_p": (None, [vp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -314,3 +315,9 @@ class Oracle:
         dJ = np.zeros(self.np + 2)
         self.L.orc_objective_and_gradient(self.h, _c(x), int(mode), J, dJ)
         return float(J[0]), dJ
+
+    def hessian_p(self, ptilde):
+        """Gauss-Newton Hessian matvec, p block (needs the state history of the last evaluation)."""
+        y = np.zeros(self.np)
+        self.L.orc_hessian_p(self.h, _c(ptilde), y)
+        return y

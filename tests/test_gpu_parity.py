@@ -303,3 +303,30 @@ def test_graph_replay_equals_stream_launches(rd, oracle_mod):
     for (J0, g0), (J1, g1) in zip(out[0], out[1]):
         assert J0[0] == J1[0]
         assert np.array_equal(g0[0], g1[0])
+
+
+def test_hessian_vs_oracle(rd, oracle_mod):
+    """Gauss-Newton Hessian matvec in p (evaluateHessian without diffusivity inversion) against the
+    FP64 oracle at the same linearization point (1e-4 relative L2, the gradient bar), and
+    binomial checkpointing bit-identical to store-all."""
+    pr, o = grad_problem(oracle_mod, nt=10)
+    print("case hessian", flush=True)
+    x = np.concatenate([pr["p"], [pr["kappa"], pr["rho"]]])
+    v = np.random.default_rng(31).standard_normal(len(pr["p"]))
+    o.set_data(pr["d1"], None, -0.99)
+    o.set_opt(beta=1e-3)
+    o.set_time(pr["dt"], pr["nt"])
+    o.objective_and_gradient(x, mode=2)
+    ho = o.hessian_p(v)
+    out = []
+    for kw in (dict(), dict(checkpoint_slots=3)):
+        s = setup_gpu(rd, pr, beta=1e-3, **kw)
+        s.set_data(0, pr["d1"], None, -0.99)
+        s.objective_and_gradient([x])
+        hv = s.hessian([np.concatenate([v, [0.0, 0.0]])])[0]
+        out.append(hv)
+        s.close()
+    print("hessian rel", rel(out[0][:-2], ho), flush=True)
+    assert rel(out[0][:-2], ho) < 1e-4
+    assert out[0][-2] == 0.0 and out[0][-1] == 0.0
+    assert np.array_equal(out[0], out[1])

@@ -318,3 +318,77 @@ def test_synthetic_tissue_properties():
     inside = s > 0
     assert np.allclose(s[inside], 1.0)
     assert wm.mean() > 0.01 and gm.mean() > 0.01 and csf.mean() > 0.005
+
+
+# ---- Gauss-Newton Hessian matvec, p block (DerivativeOperatorsRD::evaluateHessian, no diffusivity
+# inversion).  The reference's tangent-linear forward linearizes the reaction around the state at
+# the start of each step while its adjoint uses the half-step state, so the operator is the exact
+# Hessian only when one of the two processes is off; otherwise the gap is O(dt).
+
+def grad_p(o, x, mode=0):
+    o.set_time(o.dt, o.nt)
+    return o.objective_and_gradient(x, mode)[1][:o.np]
+
+
+def hess_p(o, x, v, mode=0):
+    grad_p(o, x, mode)  # the linearization point: state history at x
+    return o.hessian_p(v)
+
+
+def test_hessian_linear_problem_is_exact_and_symmetric(oracle_mod):
+    """rho = 0: J is quadratic in p, so H p~ equals central differences of the gradient and the
+    operator is symmetric (to the CG tolerance, rtol 1e-12)."""
+    pr, o = small_problem(oracle_mod, rho=0.0, nt=4, dt=0.05, rtol=1e-12)
+    x = np.concatenate([pr["p"], [0.05, 0.0]])
+    o.set_c0(o.phi_apply(pr["p"] * 1.2))
+    o.solve_state(0)
+    o.set_data((o.ct() > 0.3).astype(float), None, -0.99)
+    o.set_opt(beta=1e-2)
+    rng = np.random.default_rng(21)
+    u, v = rng.standard_normal(o.np), rng.standard_normal(o.np)
+    hv = hess_p(o, x, v)
+    h = 1e-3
+    dv = np.concatenate([v, [0.0, 0.0]])
+    fd = (grad_p(o, x + h * dv) - grad_p(o, x - h * dv)) / (2 * h)
+    assert rel(hv, fd) < 1e-7
+    hu = hess_p(o, x, u)
+    assert abs(u @ hv - v @ hu) / abs(u @ hv) < 1e-9
+    assert v @ hv > 0  # Gauss-Newton with beta > 0 is positive definite on the active bases
+
+
+def test_hessian_pure_reaction_matches_fd_at_zero_residual(oracle_mod):
+    """kappa = 0 (the half-step state equals the step's start state) with data = O c(T): the
+    Gauss-Newton operator is the exact Hessian; central differences of the gradient agree."""
+    pr, o = small_problem(oracle_mod, kappa=0.0, nt=4, dt=0.05, rtol=1e-12)
+    x = np.concatenate([pr["p"], [0.0, 8.0]])
+    o.set_c0(o.phi_apply(pr["p"]))
+    o.solve_state(0)
+    o.set_data(o.ct(), np.ones(o.ng), 0.0)
+    o.set_opt(beta=1e-3)
+    v = np.random.default_rng(22).standard_normal(o.np)
+    hv = hess_p(o, x, v)
+    h = 1e-4
+    dv = np.concatenate([v, [0.0, 0.0]])
+    fd = (grad_p(o, x + h * dv) - grad_p(o, x - h * dv)) / (2 * h)
+    assert rel(hv, fd) < 1e-6
+
+
+def test_hessian_gap_to_exact_hessian_shrinks_with_dt(oracle_mod):
+    """With diffusion and reaction, the reference operator differs from the exact Hessian by its
+    linearization point (start of step vs half step); the gap shrinks as dt is refined."""
+    gaps = []
+    for nt in (2, 4, 8):
+        pr, o = small_problem(oracle_mod, nt=nt, dt=0.2 / nt, rtol=1e-12)
+        x = np.concatenate([pr["p"], [0.05, 8.0]])
+        o.set_c0(o.phi_apply(pr["p"]))
+        o.solve_state(0)
+        o.set_data(o.ct(), np.ones(o.ng), 0.0)
+        o.set_opt(beta=1e-3)
+        v = np.random.default_rng(23).standard_normal(o.np)
+        hv = hess_p(o, x, v)
+        h = 1e-4
+        dv = np.concatenate([v, [0.0, 0.0]])
+        fd = (grad_p(o, x + h * dv) - grad_p(o, x - h * dv)) / (2 * h)
+        gaps.append(rel(hv, fd))
+    assert gaps[1] < gaps[0] and gaps[2] < gaps[1], gaps
+    assert gaps[2] < 0.2, gaps
