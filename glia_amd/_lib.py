@@ -92,10 +92,38 @@ SIGNATURES = {
     "glia_rd_hessian": (_i, [_vp, _vp, _vp]),
     "glia_rd_stats": (_i, [_vp, _i, ctypes.POINTER(Stats)]),
     "glia_rd_reset_stats": (_i, [_vp]),
+    "glia_rd_smooth_field": (_i, [_vp, _i, _i, _d]),
+    "glia_rd_refresh_tissue": (_i, [_vp, _i]),
     "glia_rd_use_graphs": (_i, [_vp, _i]),
     "glia_rd_profile": (_i, [_vp, _i]),
     "glia_rd_profile_get": (_i, [_vp, _i, _pd, ctypes.POINTER(ctypes.c_longlong)]),
 }
+
+_i64 = ctypes.c_int64
+_cp = ctypes.c_char_p
+# include/glia_io.h (host-side input); glia_rd_file_config is read through FileConfig
+IO_SIGNATURES = {
+    "glia_nc_inquire": (_i, [_cp, _cp, ctypes.POINTER(_i), ctypes.POINTER(_i64), ctypes.POINTER(_i)]),
+    "glia_nc_read_f32": (_i, [_cp, _cp, _vp, _i64]),
+    "glia_nc_read_f64": (_i, [_cp, _cp, _vp, _i64]),
+    "glia_nc_write3_f32": (_i, [_cp, _cp, _vp, _vp, _i]),
+    "glia_split_segmentation": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "glia_rd_parse_config": (_i, [_cp, _vp]),
+}
+
+
+class FileConfig(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int), ("model", ctypes.c_int), ("nt", ctypes.c_int), ("dt", ctypes.c_double),
+        ("k", ctypes.c_double), ("rho", ctypes.c_double), ("k_gm_wm", ctypes.c_double), ("r_gm_wm", ctypes.c_double),
+        ("beta", ctypes.c_double), ("smoothing_factor", ctypes.c_double), ("obs_threshold_1", ctypes.c_double),
+        ("np", ctypes.c_int), ("sigma_factor", ctypes.c_double), ("sigma_spacing", ctypes.c_double),
+        ("phi_store", ctypes.c_int), ("adjoint_store", ctypes.c_int), ("atlas_labels", ctypes.c_int * 8),
+        ("a_seg_path", ctypes.c_char * 512), ("a_wm_path", ctypes.c_char * 512), ("a_gm_path", ctypes.c_char * 512),
+        ("a_csf_path", ctypes.c_char * 512), ("a_vt_path", ctypes.c_char * 512), ("d1_path", ctypes.c_char * 512),
+        ("obs_mask_path", ctypes.c_char * 512),
+    ]
+
 
 KERNELS = ["xi_pupd", "yi_grad", "z_diff", "yf_div", "xf_matvec", "pcg_update"]
 # algorithmic bytes of one launch per member, in units of one field (4 n0 n1 n2 bytes): every
@@ -116,7 +144,7 @@ def load():
             "(hipcc --offload-arch=gfx950); there is no CPU fallback" % LIB_PATH
         )
     lib = ctypes.CDLL(LIB_PATH)
-    for name, (res, args) in SIGNATURES.items():
+    for name, (res, args) in list(SIGNATURES.items()) + list(IO_SIGNATURES.items()):
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args

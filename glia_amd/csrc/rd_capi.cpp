@@ -422,6 +422,18 @@ int glia_rd_stats(glia_rd_t h, int member, glia_rd_stats_t* out) {
   return 0;
 }
 
+int glia_rd_smooth_field(glia_rd_t h, int member, int which, double sigma) {
+  GLIA_H(h);
+  const int slot = field_slot(which);
+  if (slot < 0) return GLIA_RD_ERR_ARG;
+  return ret(h, h->e->smooth_field(member, slot, sigma));
+}
+
+int glia_rd_refresh_tissue(glia_rd_t h, int member) {
+  GLIA_H(h);
+  return ret(h, h->e->refresh_tissue(member));
+}
+
 int glia_rd_use_graphs(glia_rd_t h, int enable) {
   GLIA_H(h);
   return ret(h, h->e->set_use_graphs(enable != 0));

@@ -1143,6 +1143,28 @@ __global__ void k_brick_src(const BrickJob* __restrict__ jobs, Dims d, const Mem
   }
 }
 
+// One axis of the separable periodic smoothing of a real field (the reference's
+// weierstrassSmoother, src/grad/SpectralOperators.cpp:264-381, whose 8-image Gaussian kernel of
+// src/cuda/SpectralOperators.cu:12-51 is a product of per-axis kernels): out(x) = sum_s taps[s]
+// in(x - s e_axis), indices periodic.  Taps beyond 12 sigma (< e^-72) are dropped.
+__global__ void k_smooth_axis(const float* __restrict__ in, float* __restrict__ out, Dims d, int axis,
+                              const double* __restrict__ taps, int kr) {
+  const int n[3] = {d.n0, d.n1, d.n2};
+  const int64_t st[3] = {(int64_t)d.n1 * d.n2, (int64_t)d.n2, 1};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.ng; i += (int64_t)gridDim.x * blockDim.x) {
+    const int c[3] = {(int)(i / st[0]), (int)((i / d.n2) % d.n1), (int)(i % d.n2)};
+    const int64_t base = i - (int64_t)c[axis] * st[axis];
+    double acc = 0.0;
+    for (int s = -kr; s <= kr; s++) {
+      int q = c[axis] - s;
+      q += (q < 0) ? n[axis] : 0;
+      q -= (q >= n[axis]) ? n[axis] : 0;
+      acc += taps[s + kr] * (double)in[base + (int64_t)q * st[axis]];
+    }
+    out[i] = (float)acc;
+  }
+}
+
 // one axis of the separable smoothing: the kept extent shrinks by 2*kr along 'axis'
 __global__ void k_brick_conv(const BrickJob* __restrict__ jobs, int axis, const double* __restrict__ taps, int tap_stride,
                              const double* __restrict__ src, double* __restrict__ dst) {
@@ -1472,6 +1494,12 @@ struct RunZGradq {
   }
 };
 int launch_z_gradq(const LaunchCtx& c, int4 sc, int4 sa) { return dispatch_M<RunZGradq>(c.d.m2, c, sc, sa); }
+
+int launch_smooth_axis(const LaunchCtx& c, const float* in, float* out, int axis, const double* taps, int kr) {
+  k_smooth_axis<<<1024, 256, 0, c.stream>>>(in, out, c.d, axis, taps, kr);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
 
 int launch_pcg_update(const LaunchCtx& c, int s_x) {
   dim3 g(UPD_BLOCKS, c.nm);

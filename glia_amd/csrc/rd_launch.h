@@ -27,6 +27,7 @@ bool supported_line(int L);
 int xf_div_blocks(const Dims& d);
 int z_blocks(const Dims& d);
 int z_lines_per_block(int m2);
+int launch_smooth_axis(const LaunchCtx& c, const float* in, float* out, int axis, const double* taps, int kr);
 int pcg_update_blocks();
 int ew_blocks();
 

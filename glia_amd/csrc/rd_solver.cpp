@@ -283,18 +283,7 @@ int RDEnsemble::set_tissue(int m, const float* wm, const float* gm, const float*
   if ((rc = set_field(m, RS_GM, gm, device))) return rc;
   if ((rc = set_field(m, RS_CSF, csf, device))) return rc;
   if ((rc = set_field(m, RS_VT, vt, device))) return rc;
-  LaunchCtx c = member_ctx(ctx(), md_, m);
-  GLIA_TRYC(launch_tissue(c));
-  GLIA_TRYC(launch_fin_state(c, ew_blocks(), FINK_FILTER, 0.0));
-  if ((rc = set_kappa(m, kappa_[m]))) return rc;
-  if ((rc = set_rho(m, rho_[m]))) return rc;
-  GLIA_TRYC(launch_prec_factor(c, ctx_dt_));
-  if (np_[m] > 0) {
-    // Phi::setValues on the new mask
-    std::vector<double> ctr = centers_[m];
-    return set_basis(m, np_[m], ctr.data(), sigma_[m]);
-  }
-  return sync();
+  return refresh_tissue(m);  // Phi::setValues runs again on the new mask
 }
 
 // MatProp::setValuesSinusoidal (src/mat/MatProp.cpp:34-68)
@@ -411,26 +400,9 @@ int RDEnsemble::set_basis(int m, int np, const double* centers, double sigma) {
   const int n[3] = {d_.n0, d_.n1, d_.n2};
   const double sig_s = cfg_.smoothing_factor * 2.0 * M_PI / d_.n0;
   int kr = 0;
-  for (int a = 0; a < 3; a++)
-    if (sig_s > 0) kr = std::max(kr, std::min(n[a] / 2 - 1, (int)std::ceil(12.0 * sig_s / h_[a])));
+  std::vector<double> taps1;
+  smoother_taps(sig_s, taps1, kr);
   const int tap_stride = 2 * kr + 1;
-  std::vector<double> taps1(3 * tap_stride);
-  for (int a = 0; a < 3; a++) {
-    if (sig_s <= 0) {
-      taps1[a * tap_stride + kr] = 1.0;
-      continue;
-    }
-    std::vector<double> f1(n[a]);
-    double sum = 0;
-    for (int i = 0; i < n[a]; i++) {
-      const double X = i * h_[a], Xp = X - 2.0 * M_PI;
-      double v = std::exp(-X * X / (2 * sig_s * sig_s)) + std::exp(-Xp * Xp / (2 * sig_s * sig_s));
-      if (v != v) v = 0;
-      f1[i] = v;
-      sum += v;
-    }
-    for (int s = -kr; s <= kr; s++) taps1[a * tap_stride + s + kr] = f1[((s % n[a]) + n[a]) % n[a]] / sum;
-  }
   std::vector<Brick> bk(np);
   std::vector<BrickJob> jobs(np);
   int64_t off = 0;
@@ -1058,6 +1030,75 @@ int RDEnsemble::objective_and_gradient(int mode, const std::vector<std::vector<d
     }
   }
   return 0;
+}
+
+// Per-axis taps of the reference's periodic smoothing kernel (computeWeierstrassFilter,
+// src/cuda/SpectralOperators.cu:12-51): g(x) + g(x - 2 pi) per axis, normalized by its discrete
+// sum; the 3D kernel is their product.  Taps beyond 12 sigma are dropped (kr = 0: identity).
+void RDEnsemble::smoother_taps(double sigma, std::vector<double>& taps, int& kr) const {
+  const int n[3] = {d_.n0, d_.n1, d_.n2};
+  kr = 0;
+  for (int a = 0; a < 3; a++)
+    if (sigma > 0) kr = std::max(kr, std::min(n[a] / 2 - 1, (int)std::ceil(12.0 * sigma / h_[a])));
+  const int stride = 2 * kr + 1;
+  taps.assign(3 * stride, 0.0);
+  for (int a = 0; a < 3; a++) {
+    if (sigma <= 0) {
+      taps[a * stride + kr] = 1.0;
+      continue;
+    }
+    std::vector<double> f1(n[a]);
+    double sum = 0;
+    for (int i = 0; i < n[a]; i++) {
+      const double X = i * h_[a], Xp = X - 2.0 * M_PI;
+      double v = std::exp(-X * X / (2 * sigma * sigma)) + std::exp(-Xp * Xp / (2 * sigma * sigma));
+      if (v != v) v = 0;
+      f1[i] = v;
+      sum += v;
+    }
+    for (int s = -kr; s <= kr; s++) taps[a * stride + s + kr] = f1[((s % n[a]) + n[a]) % n[a]] / sum;
+  }
+}
+
+// SpectralOperators::weierstrassSmoother (src/grad/SpectralOperators.cpp:264-381) of real field
+// rslot of member m, in place: three periodic axis convolutions through a scratch buffer.
+int RDEnsemble::smooth_field(int m, int rslot, double sigma) {
+  if (m < 0 || m >= nm_ || rslot < 0 || rslot >= RS_NUM) return fail(-2, "bad member or field");
+  if (!(sigma > 0)) return 0;
+  std::vector<double> taps;
+  int kr = 0;
+  smoother_taps(sigma, taps, kr);
+  double* td = nullptr;
+  GLIA_HIP(hipMalloc((void**)&td, sizeof(double) * taps.size()));
+  int rc = copy_sync(td, taps.data(), sizeof(double) * taps.size(), hipMemcpyHostToDevice);
+  float* f = mh_[m].r[rslot];
+  float* scratch = reinterpret_cast<float*>(mh_[m].c[CS_T7]);  // n0*n1*n2 floats
+  LaunchCtx c = ctx();
+  const int stride = 2 * kr + 1;
+  if (!rc) rc = launch_smooth_axis(c, f, scratch, 0, td, kr);
+  if (!rc) rc = launch_smooth_axis(c, scratch, f, 1, td + stride, kr);
+  if (!rc) rc = launch_smooth_axis(c, f, scratch, 2, td + 2 * stride, kr);
+  if (!rc) rc = check(hipMemcpyAsync(f, scratch, sizeof(float) * d_.ng, hipMemcpyDeviceToDevice, stream_), "smooth copy");
+  if (!rc) rc = sync();
+  (void)hipFree(td);
+  return rc;
+}
+
+// the tail of set_tissue: mask, k, rho, preconditioner and Phi from the stored tissue fields
+int RDEnsemble::refresh_tissue(int m) {
+  if (m < 0 || m >= nm_) return fail(-2, "bad member");
+  int rc;
+  LaunchCtx c = member_ctx(ctx(), md_, m);
+  GLIA_TRYC(launch_tissue(c));
+  GLIA_TRYC(launch_fin_state(c, ew_blocks(), FINK_FILTER, 0.0));
+  if ((rc = set_kappa(m, kappa_[m]))) return rc;
+  if ((rc = set_rho(m, rho_[m]))) return rc;
+  GLIA_TRYC(launch_prec_factor(c, ctx_dt_));
+  if (np_[m] > 0) {
+    std::vector<double> ctr = centers_[m];
+    return set_basis(m, np_[m], ctr.data(), sigma_[m]);
+  }
+  return sync();
 }
 
 // ---------------------------------------------------------------------------------------------

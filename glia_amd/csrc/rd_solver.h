@@ -91,6 +91,8 @@ class RDEnsemble {
   // profiling mode: per-kind launch count and summed event time (ms) since enabled
   int profile_enable(bool on);
   int set_use_graphs(bool on);
+  int smooth_field(int m, int rslot, double sigma);  // weierstrassSmoother, in place
+  int refresh_tissue(int m);                         // derived quantities from stored tissue fields
   int profile_get(int kind, double* ms, long long* count);
 
  private:
@@ -101,6 +103,7 @@ class RDEnsemble {
   int fail(int code, const std::string& msg);
   LaunchCtx ctx() const;
 
+  void smoother_taps(double sigma, std::vector<double>& taps, int& kr) const;
   int cn_solve(int sx, double tau);
   int launch_iteration(int sx, int slot);                       // the launches of one CG iteration
   int iteration_graph(int sx, int slot, hipGraphExec_t* out);   // the same, captured once and cached

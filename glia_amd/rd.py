@@ -236,6 +236,14 @@ class RDSolver:
         self._chk(self.L.glia_rd_stats(self.h, member, ctypes.byref(s)))
         return {f: getattr(s, f) for f, _ in _lib.Stats._fields_}
 
+    def smooth_field(self, member, which, sigma):
+        """weierstrassSmoother of a stored real field, in place."""
+        self._chk(self.L.glia_rd_smooth_field(self.h, int(member), _lib.FIELD[which], float(sigma)))
+
+    def refresh_tissue(self, member):
+        """Mask, k, rho, preconditioner and Phi from the stored tissue fields."""
+        self._chk(self.L.glia_rd_refresh_tissue(self.h, int(member)))
+
     def use_graphs(self, enable=True):
         """Replay CG iterations from captured HIP graphs (default) or launch kernel by kernel."""
         self._chk(self.L.glia_rd_use_graphs(self.h, int(bool(enable))))

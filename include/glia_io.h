@@ -1,0 +1,68 @@
+/*
+ * glia_io.h -- host-side input of the reaction-diffusion path: NetCDF classic fields, segmentation
+ * labels, config files.
+ *
+ * These replace the reference's IO layer for the RD path (src/utils/IO.cpp) with plain C entry
+ * points in libglia_rd.so.  They use no GPU and no external library.  Return value: 0 on
+ * success, negative GLIA_RD_ERR_* codes otherwise (glia_rd.h).
+ */
+#ifndef GLIA_IO_H_
+#define GLIA_IO_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* NetCDF classic formats (CDF-1 and CDF-2; big-endian, non-record variables).
+   glia_nc_inquire: dimensions of variable 'var' (the reference reads "data"); ndims <= 8;
+   type is the NetCDF external type (1 byte, 3 short, 4 int, 5 float, 6 double). */
+int glia_nc_inquire(const char* path, const char* var, int* ndims, int64_t* dims, int* type);
+/* dataIn (src/utils/IO.cpp:511-539): the whole variable, row-major (last dimension fastest), as
+   FP32 or FP64; count must equal the product of the dimensions */
+int glia_nc_read_f32(const char* path, const char* var, float* out, int64_t count);
+int glia_nc_read_f64(const char* path, const char* var, double* out, int64_t count);
+/* dataOut (src/utils/IO.cpp:541-601): CDF-2 (64-bit offsets) file with dimensions "x", "y", "z",
+   one variable 'var' (NC_FLOAT, or NC_DOUBLE when as_double != 0) and the global attribute
+   "CDF-5 mode" = 0 the reference writes */
+int glia_nc_write3_f32(const char* path, const char* var, const int* n, const float* data, int as_double);
+
+/* splitSegmentation (src/utils/Utils.cpp:592-660): labels[8] = {wm, gm, vt, csf, tc, nec, en, ed}
+   (values <= 0: absent).  wm, gm, vt are required; csf, tc, ed may be NULL.  With a tumor map
+   (tc non-NULL), the tumor core (tc, or nec together with en) and edema also count as white
+   matter and ed receives the edema label; without one, ed is zero -- as in the reference. */
+int glia_split_segmentation(const float* seg, int64_t count, const int* labels, float* wm, float* gm, float* vt,
+                            float* csf, float* tc, float* ed);
+
+/* The RD-relevant part of a reference config file (parseConfig / setParameter,
+   src/utils/IO.cpp:6-257): "key=value" lines, whitespace removed, '#' comments. */
+typedef struct glia_rd_file_config {
+  int n;                    /* n (cubic grid) */
+  int model;                /* model (1 = reaction-diffusion) */
+  int nt;                   /* nt_inv */
+  double dt;                /* dt_inv */
+  double k;                 /* init_k */
+  double rho;               /* init_rho */
+  double k_gm_wm, r_gm_wm;  /* k_gm_wm, r_gm_wm */
+  double beta;              /* beta_p */
+  double smoothing_factor;  /* smoothing_factor */
+  double obs_threshold_1;   /* obs_threshold_1 */
+  int np;                   /* number_gaussians */
+  double sigma_factor;      /* sigma_factor (sigma = factor * 2 pi / n) */
+  double sigma_spacing;     /* sigma_spacing */
+  int phi_store, adjoint_store; /* store_phi, store_adjoint */
+  int atlas_labels[8];      /* atlas_labels [wm, gm, vt, csf, tc, nec, en, ed]; -1 absent */
+  char a_seg_path[512], a_wm_path[512], a_gm_path[512], a_csf_path[512], a_vt_path[512];
+  char d1_path[512], obs_mask_path[512];
+} glia_rd_file_config;
+
+/* Defaults of the reference parameter structs for the fields above, then the file's values.
+   Unknown keys are ignored; a missing file is GLIA_RD_ERR_ARG. */
+int glia_rd_parse_config(const char* path, glia_rd_file_config* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -118,6 +118,14 @@ int glia_rd_prec_factor(glia_rd_t h);
 /* DiffusionSolver Ctx::dt_ (src/pde/DiffusionSolver.cpp:11,226) */
 int glia_rd_set_ctx_dt(glia_rd_t h, double dt);
 int glia_rd_get_ctx_dt(glia_rd_t h, double* dt);
+/* SpectralOperators::weierstrassSmoother (src/grad/SpectralOperators.cpp:264-381) of a stored real
+   field of one member, in place, with standard deviation sigma (the reference smooths atlas
+   tissue with sigma = smoothing_factor * 2 pi / n0, SolverInterface::readAtlas,
+   src/SolverInterface.cpp:502-575) */
+int glia_rd_smooth_field(glia_rd_t h, int member, int which, double sigma);
+/* MatProp::setValuesCustom on the tissue fields already stored (e.g. after glia_rd_set_field and
+   glia_rd_smooth_field): mask, k, rho, preconditioner and Phi of the member */
+int glia_rd_refresh_tissue(glia_rd_t h, int member);
 /* Phi::setGaussians(centers) + Phi::setValues (src/mat/Phi.cpp:32-144): np centers (x,y,z in [0,2pi)) */
 int glia_rd_set_basis(glia_rd_t h, int member, int np, const double* centers, double sigma);
 int glia_rd_basis_size(glia_rd_t h, int member);

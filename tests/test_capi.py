@@ -1,5 +1,5 @@
 """The C-ABI library loads on a machine without a GPU and exports every entry point that
-include/glia_rd.h declares (no compute calls here)."""
+include/glia_rd.h and include/glia_io.h declare (no compute calls here)."""
 import ctypes
 import os
 import re
@@ -8,10 +8,10 @@ import subprocess
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared():
-    src = open(os.path.join(ROOT, "include", "glia_rd.h")).read()
+def declared(header="glia_rd.h"):
+    src = open(os.path.join(ROOT, "include", header)).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(glia_rd_[a-z0-9_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(glia_[a-z0-9_]+)\s*\(", src)))
 
 
 def test_header_declares_the_boundary():
@@ -29,13 +29,15 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in declared() if not hasattr(lib, n)]
     assert not missing, missing
     out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
-    exported = set(re.findall(r"\b(glia_rd_[a-z0-9_]+)\b", out))
+    exported = set(re.findall(r"\b(glia_[a-z0-9_]+)\b", out))
     assert set(declared()) <= exported
+    assert set(declared("glia_io.h")) <= exported
 
 
 def test_python_binding_covers_every_symbol():
     from glia_amd import _lib
     assert set(declared()) == set(_lib.SIGNATURES)
+    assert set(declared("glia_io.h")) == set(_lib.IO_SIGNATURES)
 
 
 def test_config_defaults_without_gpu():
