@@ -187,11 +187,11 @@ struct Tile {
   int o, c0;
 };
 
-__device__ __forceinline__ Tile col_tile(int dir, const Dims& d) {
+__device__ __forceinline__ Tile col_tile_at(int dir, const Dims& d, int idx) {
   const int ntc = d.m2 >> 4;
   Tile t;
-  t.o = blockIdx.x / ntc;
-  t.c0 = (blockIdx.x % ntc) << 4;
+  t.o = idx / ntc;
+  t.c0 = (idx % ntc) << 4;
   if (dir == 0) {
     t.base = (int64_t)t.o * d.m2 + t.c0;
     t.rs = (unsigned)(d.n1 * d.m2);
@@ -201,6 +201,7 @@ __device__ __forceinline__ Tile col_tile(int dir, const Dims& d) {
   }
   return t;
 }
+__device__ __forceinline__ Tile col_tile(int dir, const Dims& d) { return col_tile_at(dir, d, (int)blockIdx.x); }
 
 // Twiddle table of a block, staged in LDS after the block's data loads have been issued.  Read
 // from LDS after a barrier, the twiddles of each transform are loaded where they are used instead
@@ -336,6 +337,7 @@ k_yi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   }
 }
 
+
 // Y pass, forward, divergence: Yx = FFT_y(Hx), Rr = FFT_y(Q) + i w_y FFT_y(Hy).
 // Yx must not alias Q or Hy.
 template <int L>
@@ -383,6 +385,7 @@ k_yf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
     Rr[(unsigned)row * tl.rs + col] = cadd(a[j], mul_i(cscale(b[j], wnum(row, d.n1))));
   }
 }
+
 
 // X pass, forward, closing the divergence: D = i w_x FFT_x(Yx) + FFT_x(Rr) = FFT(div(k grad p)).
 //   XF_MATVEC: w = p - (tau/2) D; partial <p, w>
