@@ -357,13 +357,12 @@ int glia_rd_objective_and_gradient(glia_rd_t h, int mode, const double* x, doubl
   return eval(h, mode, x, J, dJ, true);
 }
 
-int glia_rd_hessian(glia_rd_t h, const double* x, double* y) {
+int glia_rd_hessian(glia_rd_t h, int diffusivity, const double* x, double* y) {
   GLIA_H(h);
   if (!x || !y) return GLIA_RD_ERR_ARG;
   std::vector<std::vector<double>> xv, yv;
   if (!split_x(h->e.get(), x, 2, xv)) return GLIA_RD_ERR_ARG;
-  for (auto& v : xv) v.resize(v.size() - 2);  // p~ only: the kappa and rho directions are ignored
-  const int rc = h->e->hessian_p(xv, yv);
+  const int rc = h->e->hessian(xv, diffusivity != 0, yv);
   if (rc) return ret(h, rc);
   int64_t off = 0;
   for (int m = 0; m < h->e->nmembers(); m++) {

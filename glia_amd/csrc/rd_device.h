@@ -39,6 +39,7 @@ enum RSlot {
   RS_CT,      // c(T)
   RS_A0,      // alpha(0)
   RS_CH,      // half-step state c_half (real) for the adjoint reaction
+  RS_KS,      // secondary diffusion coefficient k~ (wm + k_gm gm + k_glm csf) of the Hessian
   RS_NUM
 };
 

@@ -614,7 +614,7 @@ __device__ __forceinline__ void zl_forward_store(float2* v, float2* lb, float2* 
 template <int M>
 __global__ void __launch_bounds__(ZK<M>::TPB, ZK<M>::VPT <= 16 ? GLIA_ZD_MINW : 2)
 k_z_diff(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ twm, const float2* __restrict__ tw2,
-          int s_gx, int s_gy, int s_gz) {
+          int s_gx, int s_gy, int s_gz, int k_slot) {
   const MemberDev& Mb = mem[blockIdx.y];
   if (Mb.st->done) return;
   constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT, VPT = ZK<M>::VPT;
@@ -633,7 +633,7 @@ k_z_diff(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
   const int li = threadIdx.x / NT, t = threadIdx.x % NT;
   const int64_t base = (int64_t)grp * ZK<M>::NL * M;
   float2* line = lb + li * ZK<M>::LP;
-  GPtr kf = rptr2(Mb, RS_K) + base;
+  GPtr kf = rptr2(Mb, k_slot) + base;
   const int lo = li * M;
   const float s = (float)d.inv_ng;
   const int slot = comp == 0 ? s_gx : (comp == 1 ? s_gy : s_gz);
@@ -1306,6 +1306,21 @@ __global__ void __launch_bounds__(256) k_axpby_real(const MemberDev* __restrict_
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.ng; i += stride) o[i] = fa * x[i] + fb * y[i];
 }
 
+// out = a x + b y on packed spectra (all members; slots >= CS_NUM address the state pool)
+__global__ void __launch_bounds__(256) k_axpby_c(const MemberDev* __restrict__ mem, Dims d, int s_x, double a, int s_y,
+                                                 double b, int s_out) {
+  const MemberDev& M = mem[blockIdx.y];
+  const float2* x = craw(M, s_x);
+  const float2* y = craw(M, s_y);
+  float2* o = craw(M, s_out);
+  const float fa = (float)a, fb = (float)b;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.nspec; i += stride) {
+    const float2 u = x[i], v = y[i];
+    o[i] = make_float2(fa * u.x + fb * v.x, fa * u.y + fb * v.y);
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // launchers (size dispatch)
 // ---------------------------------------------------------------------------------------------
@@ -1429,15 +1444,15 @@ int launch_col_plain(const LaunchCtx& c, int dir, int sign, int s_in, int s_out,
 
 template <int M>
 struct RunZDiff {
-  static int run(const LaunchCtx& c, int s_gx, int s_gy, int s_gz) {
+  static int run(const LaunchCtx& c, int s_gx, int s_gy, int s_gz, int k_slot) {
     dim3 g((unsigned)((int64_t)c.d.n0 * c.d.n1 / ZK<M>::NL * 3), c.nm);
-    k_z_diff<M><<<g, ZK<M>::TPB, 0, c.stream>>>(c.mem, c.d, c.twm, c.tw2, s_gx, s_gy, s_gz);
+    k_z_diff<M><<<g, ZK<M>::TPB, 0, c.stream>>>(c.mem, c.d, c.twm, c.tw2, s_gx, s_gy, s_gz, k_slot);
     GLIA_CHECK_LAUNCH();
     return 0;
   }
 };
-int launch_z_diff(const LaunchCtx& c, int s_gx, int s_gy, int s_gz) {
-  return dispatch_M<RunZDiff>(c.d.m2, c, s_gx, s_gy, s_gz);
+int launch_z_diff(const LaunchCtx& c, int s_gx, int s_gy, int s_gz, int k_slot) {
+  return dispatch_M<RunZDiff>(c.d.m2, c, s_gx, s_gy, s_gz, k_slot);
 }
 
 template <int M>
@@ -1604,6 +1619,12 @@ int launch_set_done(const LaunchCtx& c, int v) {
 }
 int launch_axpby_real(const LaunchCtx& c, int r_x, double a, int r_y, double b, int r_out) {
   k_axpby_real<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d, r_x, a, r_y, b, r_out);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+
+int launch_axpby_c(const LaunchCtx& c, int s_x, double a, int s_y, double b, int s_out) {
+  k_axpby_c<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d, s_x, a, s_y, b, s_out);
   GLIA_CHECK_LAUNCH();
   return 0;
 }

@@ -36,7 +36,7 @@ int launch_yi_grad(const LaunchCtx& c, int s_u, int s_v, int s_gx, int s_gy, int
 int launch_yf_div(const LaunchCtx& c, int s_hx, int s_hy, int s_q, int s_yx, int s_rr);
 int launch_xf_div(const LaunchCtx& c, int mode, int s_yx, int s_rr, int s_p, int s_out);
 int launch_col_plain(const LaunchCtx& c, int dir, int sign, int s_in, int s_out, bool honor_done);
-int launch_z_diff(const LaunchCtx& c, int s_gx, int s_gy, int s_gz);
+int launch_z_diff(const LaunchCtx& c, int s_gx, int s_gy, int s_gz, int k_slot);
 int launch_z_point(const LaunchCtx& c, int mode, int s_in, int s_out, int r_slot, int ch_idx, float dt, bool honor_done);
 int launch_z_gradq(const LaunchCtx& c, int4 sc, int4 sa);
 
@@ -60,6 +60,7 @@ int launch_phi_applyT(const LaunchCtx& c, int np_max, int r_in, double scale);
 int launch_dot_real(const LaunchCtx& c, int r_x, int r_y);
 int launch_set_done(const LaunchCtx& c, int v);
 int launch_axpby_real(const LaunchCtx& c, int r_x, double a, int r_y, double b, int r_out);
+int launch_axpby_c(const LaunchCtx& c, int s_x, double a, int s_y, double b, int s_out);
 int launch_bricks(const LaunchCtx& c, const BrickJob* jobs_host, int njobs, const double* taps_host, int tap_stride,
                   double h0, double h1, double h2, double sigma, float* store_dev, double* max_out_host);
 

@@ -512,7 +512,7 @@ int RDEnsemble::matvec_setup(int sx) {
   LaunchCtx c = ctx();
   GLIA_TRYC(launch_xi_grad(c, sx, 0, CS_T0, CS_T1, false));
   GLIA_TRYC(launch_yi_grad(c, CS_T0, CS_T1, CS_T2, CS_T3, CS_T4, -1));
-  GLIA_TRYC(launch_z_diff(c, CS_T2, CS_T3, CS_T4));
+  GLIA_TRYC(launch_z_diff(c, CS_T2, CS_T3, CS_T4, RS_K));
   GLIA_TRYC(launch_yf_div(c, CS_T2, CS_T3, CS_T4, CS_T0, CS_T1));
   GLIA_TRYC(launch_xf_div(c, XFM_SETUP, CS_T0, CS_T1, sx, CS_R));
   return 0;
@@ -574,7 +574,7 @@ int RDEnsemble::launch_iteration(int sx, int slot) {
   GLIA_TRYC(launch_yi_grad(c, CS_T0, CS_T1, CS_T2, CS_T3, CS_T4, -1));
   prof_mark(PK_YI_GRAD, false);
   prof_mark(PK_Z_DIFF, true);
-  GLIA_TRYC(launch_z_diff(c, CS_T2, CS_T3, CS_T4));
+  GLIA_TRYC(launch_z_diff(c, CS_T2, CS_T3, CS_T4, RS_K));
   prof_mark(PK_Z_DIFF, false);
   prof_mark(PK_YF_DIV, true);
   GLIA_TRYC(launch_yf_div(c, CS_T2, CS_T3, CS_T4, CS_T0, CS_T1));
@@ -696,13 +696,13 @@ int RDEnsemble::solve_state(int linearized, bool keep_history) {
 }
 
 // alpha(T) = -O^T (O c(T) - d1) and m1 = ||O c(T) - d1||^2 (src/grad/DerivativeOperatorsRD.cpp:154-162)
-int RDEnsemble::misfit_and_final_adjoint(int cslot_c) {
+int RDEnsemble::misfit_and_final_adjoint(int cslot_c, int out) {
   LaunchCtx c = ctx();
   GLIA_TRYC(launch_col_plain(c, 0, +1, cslot_c, CS_T5, false));
   GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T5, CS_T5, false));
   GLIA_TRYC(launch_z_point(c, ZPM_MISFIT, CS_T5, CS_T5, -1, -1, 0.f, false));
   GLIA_TRYC(launch_col_plain(c, 1, -1, CS_T5, CS_T5, false));
-  GLIA_TRYC(launch_col_plain(c, 0, -1, CS_T5, CS_A, false));
+  GLIA_TRYC(launch_col_plain(c, 0, -1, CS_T5, out, false));
   GLIA_TRYC(launch_fin_state(c, z_blocks(d_), FINK_MISFIT, 0.0));
   return 0;
 }
@@ -766,40 +766,75 @@ int RDEnsemble::tangent_step(int sx, int cslot_c) {
   return cn_solve(sx, dt / 2.0);
 }
 
-// DerivativeOperatorsRD::evaluateHessian, no diffusivity inversion, L2 regularization
-// (src/grad/DerivativeOperatorsRD.cpp:420-436): c~0 = Phi p~; tangent-linear forward
-// (solveState(1)); a~(T) = -O^T O c~(T); adjoint sweep (solveAdjoint(2)); y = beta h^3 Phi^T Phi p~
-// - h^3 Phi^T a~0.  The linearization point is the state history of the last evaluation; with
-// checkpointing the base states are recomputed in lockstep with the tangent states.
-int RDEnsemble::hessian_p(const std::vector<std::vector<double>>& ptilde, std::vector<std::vector<double>>& y) {
-  if (stored_.find(0) == stored_.end()) return fail(-6, "the Hessian needs the state history of an evaluation first");
-  int rc = phi_apply(ptilde);
+// out = FFT(div(k_sec grad IFFT(in))) on packed spectra: DiffCoef::applyDWithSecondaryCoeffs
+// (src/mat/DiffCoef.cpp:273-300) with the secondary coefficient field RS_KS
+int RDEnsemble::apply_dsec(int in, int out) {
+  int rc = clear_done();
   if (rc) return rc;
-  rc = forward_transform_real(RS_C0, CS_A);
+  LaunchCtx c = ctx();
+  GLIA_TRYC(launch_xi_grad(c, in, 0, CS_T0, CS_T1, false));
+  GLIA_TRYC(launch_yi_grad(c, CS_T0, CS_T1, CS_T2, CS_T3, CS_T4, -1));
+  GLIA_TRYC(launch_z_diff(c, CS_T2, CS_T3, CS_T4, RS_KS));
+  GLIA_TRYC(launch_yf_div(c, CS_T2, CS_T3, CS_T4, CS_T0, CS_T1));
+  GLIA_TRYC(launch_xf_div(c, XFM_D, CS_T0, CS_T1, in, out));
+  return 0;
+}
+
+// PdeOperatorsRD::solveIncremental (src/pde/PdeOperators.cpp:192-233), called with dt/2:
+// c~ += dt/4 D_sec(wa c_i + wb c_{i+1}), (wa, wb) = (1.5, 0.5) before the step, (0.5, 1.5) after
+int RDEnsemble::incremental_source(int sx, int ci, int ci1, double wa, double wb) {
+  LaunchCtx c = ctx();
+  GLIA_TRYC(launch_axpby_c(c, ci, wa, ci1, wb, CS_W));
+  int rc = apply_dsec(CS_W, CS_P);
   if (rc) return rc;
+  GLIA_TRYC(launch_axpby_c(c, sx, 1.0, CS_P, cfg_.dt / 4.0, sx));
+  return 0;
+}
+
+// tangent-linear forward in slot sx (solveState(1), or solveState(2) with the secondary sources);
+// under checkpointing the base states c_i, c_{i+1} come from a lockstep forward sweep in CS_X and
+// CS_T6.  Leaves c(T) in slot *ct_slot.
+int RDEnsemble::tangent_forward(int sx, bool sources, int* ct_slot) {
+  int rc = 0;
   if (hist_all_) {
-    for (int i = 0; i < cfg_.nt; i++) {
-      rc = tangent_step(CS_A, state_slot(i));
-      if (rc) return rc;
+    for (int i = 0; i < cfg_.nt && rc == 0; i++) {
+      if (sources && (rc = incremental_source(sx, state_slot(i), state_slot(i + 1), 1.5, 0.5))) return rc;
+      if ((rc = tangent_step(sx, state_slot(i)))) return rc;
+      if (sources) rc = incremental_source(sx, state_slot(i), state_slot(i + 1), 0.5, 1.5);
     }
-  } else {
-    GLIA_TRYC(launch_copy(ctx(), state_slot(stored_[0]), CS_X));
-    for (int i = 0; i < cfg_.nt; i++) {
-      rc = tangent_step(CS_A, CS_X);
-      if (rc) return rc;
-      rc = forward_step(CS_X, i, false);
-      if (rc) return rc;
-    }
+    *ct_slot = state_slot(cfg_.nt);
+    return rc;
   }
-  {
-    LaunchCtx c = ctx();
-    GLIA_TRYC(launch_col_plain(c, 0, +1, CS_A, CS_T5, false));
-    GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T5, CS_T5, false));
-    GLIA_TRYC(launch_z_point(c, ZPM_OBS2, CS_T5, CS_T5, -1, -1, 0.f, false));
-    GLIA_TRYC(launch_col_plain(c, 1, -1, CS_T5, CS_T5, false));
-    GLIA_TRYC(launch_col_plain(c, 0, -1, CS_T5, CS_A, false));
+  GLIA_TRYC(launch_copy(ctx(), state_slot(stored_[0]), CS_X));
+  for (int i = 0; i < cfg_.nt; i++) {
+    GLIA_TRYC(launch_copy(ctx(), CS_X, CS_T6));
+    if ((rc = forward_step(CS_X, i, false))) return rc;
+    if (sources && (rc = incremental_source(sx, CS_T6, CS_X, 1.5, 0.5))) return rc;
+    if ((rc = tangent_step(sx, CS_T6))) return rc;
+    if (sources && (rc = incremental_source(sx, CS_T6, CS_X, 0.5, 1.5))) return rc;
   }
-  quad_on_ = false;
+  *ct_slot = CS_X;
+  return 0;
+}
+
+// the incremental adjoint after a tangent forward: a~(T) = -O^T O c~(T), optionally the kappa
+// quadratures, then the reverse sweep (solveAdjoint(2)); leaves a~0 in RS_A0.  With quadratures,
+// the i = nt term uses the adjoint history entry p_[nt] that solveAdjoint(2) leaves at the last
+// gradient's a(T) = -O^T (O c(T) - d1) (src/pde/PdeOperators.cpp:384-386), as the reference does.
+int RDEnsemble::incremental_adjoint(int ct_slot, bool quads) {
+  LaunchCtx c = ctx();
+  int rc;
+  if (quads) {
+    if ((rc = misfit_and_final_adjoint(ct_slot, CS_W))) return rc;
+    if ((rc = quad(ct_slot, CS_W, 0.5))) return rc;
+  }
+  GLIA_TRYC(launch_col_plain(c, 0, +1, CS_A, CS_T5, false));
+  GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T5, CS_T5, false));
+  GLIA_TRYC(launch_z_point(c, ZPM_OBS2, CS_T5, CS_T5, -1, -1, 0.f, false));
+  GLIA_TRYC(launch_col_plain(c, 1, -1, CS_T5, CS_T5, false));
+  GLIA_TRYC(launch_col_plain(c, 0, -1, CS_T5, CS_A, false));
+  quad_on_ = quads;
+  rc = 0;
   if (hist_all_) {
     for (int i = cfg_.nt - 1; i >= 0 && rc == 0; i--) rc = adjoint_step(i, state_slot(i));
   } else {
@@ -807,20 +842,67 @@ int RDEnsemble::hessian_p(const std::vector<std::vector<double>>& ptilde, std::v
   }
   quad_on_ = true;
   if (rc) return rc;
-  rc = inverse_to_real(CS_A, CS_T5, RS_A0);
-  if (rc) return rc;
-  std::vector<std::vector<double>> t1, t2;
-  rc = phi_apply_transpose(RS_C0, t1);
-  if (rc) return rc;
-  rc = phi_apply_transpose(RS_A0, t2);
-  if (rc) return rc;
+  return inverse_to_real(CS_A, CS_T5, RS_A0);
+}
+
+// DerivativeOperatorsRD::evaluateHessian, L2 regularization, nk = 1
+// (src/grad/DerivativeOperatorsRD.cpp:229-438).  xt = [p~ | k~ | rho~] per member; the linearization
+// point is the state history of the last evaluation.
+//   without diffusivity: y_p = beta h^3 Phi^T Phi p~ - h^3 Phi^T a~0 (solveState(1), solveAdjoint(2))
+//   with diffusivity:    y = [Hpp p~ + Hpk k~ ; Hkp p~ + Hkk k~ ; 0]; the kappa row is the
+//   gradDiffusion quadrature after each incremental adjoint, the k~ part is solveState(2) from
+//   c~0 = 0 with the secondary coefficient k~ (wm + k_gm gm + k_glm csf).
+// Under checkpointing the base states are recomputed in lockstep with the tangent states.
+int RDEnsemble::hessian(const std::vector<std::vector<double>>& xt, bool diffusivity,
+                        std::vector<std::vector<double>>& y) {
+  if (stored_.find(0) == stored_.end()) return fail(-6, "the Hessian needs the state history of an evaluation first");
+  if ((int)xt.size() != nm_) return fail(-2, "one direction per member");
+  for (int m = 0; m < nm_; m++)
+    if ((int)xt[m].size() != np_[m] + 2) return fail(-2, "direction must have np + 2 entries");
+  std::vector<std::vector<double>> pt(nm_);
+  for (int m = 0; m < nm_; m++) pt[m].assign(xt[m].begin(), xt[m].begin() + np_[m]);
+  LaunchCtx c = ctx();
   const double h3 = lebesgue();
+  int rc, ct_slot = CS_X;
+  // Hpp p~ (and Hkp p~)
+  if (diffusivity) GLIA_TRYC(launch_state_reset(c, 2));
+  if ((rc = phi_apply(pt))) return rc;
+  if ((rc = forward_transform_real(RS_C0, CS_A))) return rc;
+  if ((rc = tangent_forward(CS_A, false, &ct_slot))) return rc;
+  if ((rc = incremental_adjoint(ct_slot, diffusivity))) return rc;
+  std::vector<std::vector<double>> t1, t2;
+  if ((rc = phi_apply_transpose(RS_C0, t1))) return rc;
+  if ((rc = phi_apply_transpose(RS_A0, t2))) return rc;
   y.assign(nm_, {});
   for (int m = 0; m < nm_; m++) {
     y[m].assign(np_[m] + 2, 0.0);
     for (int i = 0; i < np_[m]; i++) y[m][i] = cfg_.beta * h3 * t1[m][i] - h3 * t2[m][i];
   }
-  return 0;
+  if (!diffusivity) return 0;
+  auto add_kappa_row = [&]() -> int {
+    for (int m = 0; m < nm_; m++) {
+      PcgState s;
+      GLIA_TRYC(copy_sync(&s, st_[m], sizeof(s), hipMemcpyDeviceToHost));
+      y[m][np_[m]] += h3 * cfg_.dt * s.gk;
+    }
+    return 0;
+  };
+  if ((rc = add_kappa_row())) return rc;
+  // Hpk k~ and Hkk k~
+  GLIA_TRYC(launch_state_reset(c, 2));
+  for (int m = 0; m < nm_; m++) {
+    const double kt = xt[m][np_[m]];
+    LaunchCtx cm = member_ctx(c, md_, m);
+    GLIA_TRYC(launch_axpby_real(cm, RS_WM, kt, RS_GM, kt * cfg_.k_gm_wm, RS_KS));
+    GLIA_TRYC(launch_axpby_real(cm, RS_KS, 1.0, RS_CSF, kt * cfg_.k_glm_wm, RS_KS));
+  }
+  GLIA_TRYC(launch_axpby_c(c, CS_A, 0.0, CS_A, 0.0, CS_A));  // c~0 = 0
+  if ((rc = tangent_forward(CS_A, true, &ct_slot))) return rc;
+  if ((rc = incremental_adjoint(ct_slot, true))) return rc;
+  if ((rc = phi_apply_transpose(RS_A0, t2))) return rc;
+  for (int m = 0; m < nm_; m++)
+    for (int i = 0; i < np_[m]; i++) y[m][i] += -h3 * t2[m][i];
+  return add_kappa_row();
 }
 
 // reverse the steps b-1 ... a with c_a in pool entry pa and free_slots further entries free
@@ -964,7 +1046,7 @@ int RDEnsemble::apply_d_field(int rin, int rout) {
   LaunchCtx c = ctx();
   GLIA_TRYC(launch_xi_grad(c, CS_X, 0, CS_T0, CS_T1, false));
   GLIA_TRYC(launch_yi_grad(c, CS_T0, CS_T1, CS_T2, CS_T3, CS_T4, -1));
-  GLIA_TRYC(launch_z_diff(c, CS_T2, CS_T3, CS_T4));
+  GLIA_TRYC(launch_z_diff(c, CS_T2, CS_T3, CS_T4, RS_K));
   GLIA_TRYC(launch_yf_div(c, CS_T2, CS_T3, CS_T4, CS_T0, CS_T1));
   GLIA_TRYC(launch_xf_div(c, XFM_D, CS_T0, CS_T1, CS_X, CS_T5));
   rc = inverse_to_real(CS_T5, CS_T6, rout);

@@ -78,9 +78,9 @@ class RDEnsemble {
   int solve_adjoint(int linearized);
   int objective_and_gradient(int mode, const std::vector<std::vector<double>>& x, bool want_grad,
                              std::vector<double>& J, std::vector<std::vector<double>>& dJ);
-  // Gauss-Newton Hessian matvec, p block (evaluateHessian without diffusivity inversion): needs
-  // the state history of a previous evaluation at the linearization point
-  int hessian_p(const std::vector<std::vector<double>>& ptilde, std::vector<std::vector<double>>& y);
+  // Gauss-Newton Hessian matvec (evaluateHessian; kappa blocks with diffusivity): needs the state
+  // history of a previous evaluation at the linearization point
+  int hessian(const std::vector<std::vector<double>>& xt, bool diffusivity, std::vector<std::vector<double>>& y);
   int stats(int m, MemberStats* s);
   int basis_size(int m) const { return (m >= 0 && m < nm_) ? np_[m] : -1; }
   int reset_counters();
@@ -114,7 +114,11 @@ class RDEnsemble {
   int adjoint_step(int i, int cslot_c);
   int tangent_step(int sx, int cslot_c);  // linearized forward step around the state in cslot_c
   int quad(int cslot_c, int cslot_a, double w);
-  int misfit_and_final_adjoint(int cslot_c);
+  int misfit_and_final_adjoint(int cslot_c, int out = CS_A);
+  int apply_dsec(int in, int out);
+  int incremental_source(int sx, int ci, int ci1, double wa, double wb);
+  int tangent_forward(int sx, bool sources, int* ct_slot);
+  int incremental_adjoint(int ct_slot, bool quads);
   int clear_done();
   int reverse(int a, int b, int pa, int free_slots);
   int state_slot(int pool_index) const { return CS_NUM + pool_index; }

@@ -222,13 +222,14 @@ class RDSolver:
                                                         J.ctypes.data_as(ctypes.c_void_p), dJ.ctypes.data_as(ctypes.c_void_p)))
         return J, self._split(dJ)
 
-    def hessian(self, x):
-        """Gauss-Newton Hessian matvec in p (DerivativeOperatorsRD::evaluateHessian without
-        diffusivity inversion) at the linearization point of the last evaluation; x and the
-        result are per-member [p | kappa | rho] vectors (kappa/rho of the result are 0)."""
+    def hessian(self, x, diffusivity=False):
+        """Gauss-Newton Hessian matvec (DerivativeOperatorsRD::evaluateHessian; kappa blocks with
+        diffusivity) at the linearization point of the last evaluation; x and the result are
+        per-member [p | kappa | rho] vectors."""
         flat = self._flat_x(x)
         y = np.zeros_like(flat)
-        self._chk(self.L.glia_rd_hessian(self.h, flat.ctypes.data_as(ctypes.c_void_p), y.ctypes.data_as(ctypes.c_void_p)))
+        self._chk(self.L.glia_rd_hessian(self.h, int(bool(diffusivity)), flat.ctypes.data_as(ctypes.c_void_p),
+                                         y.ctypes.data_as(ctypes.c_void_p)))
         return self._split(y)
 
     def stats(self, member=0):

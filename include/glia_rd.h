@@ -162,15 +162,19 @@ int glia_rd_solve_adjoint(glia_rd_t h, int linearized);
 int glia_rd_objective(glia_rd_t h, int mode, const double* x, double* J);
 int glia_rd_gradient(glia_rd_t h, int mode, const double* x, double* dJ);
 int glia_rd_objective_and_gradient(glia_rd_t h, int mode, const double* x, double* J, double* dJ);
-/* DerivativeOperatorsRD::evaluateHessian without diffusivity inversion, L2 regularization
-   (src/grad/DerivativeOperatorsRD.cpp:420-436): the Gauss-Newton Hessian applied to a direction in
-   p, y = beta h^3 Phi^T Phi p~ - h^3 Phi^T a~0, with the tangent-linear forward (solveState(1),
-   src/pde/PdeOperators.cpp:235-316) and incremental adjoint (solveAdjoint(2), :372-420).  x and y
-   are laid out like dJ ([p | kappa | rho] per member); the kappa and rho entries of x are ignored
-   and those of y are 0.  The linearization point is the state history of the last evaluation
-   (objective, gradient or both) on this handle, as in the reference; the call overwrites c(0)
-   with Phi p~. */
-int glia_rd_hessian(glia_rd_t h, const double* x, double* y);
+/* DerivativeOperatorsRD::evaluateHessian with L2 regularization, nk = 1
+   (src/grad/DerivativeOperatorsRD.cpp:229-438): the Gauss-Newton Hessian applied to a direction
+   x = [p~ | kappa~ | rho~] per member (laid out like dJ).
+   diffusivity = 0 (:420-436): y_p = beta h^3 Phi^T Phi p~ - h^3 Phi^T a~0 with the tangent-linear
+     forward (solveState(1), src/pde/PdeOperators.cpp:235-316) and incremental adjoint
+     (solveAdjoint(2), :372-420); kappa~ is ignored and y_kappa = 0.
+   diffusivity = 1 (:244-418): y = [Hpp p~ + Hpk kappa~ ; Hkp p~ + Hkk kappa~]; the kappa row is the
+     gradDiffusion quadrature after each incremental adjoint, including -- as in the reference --
+     the i = nt term with the adjoint history entry that solveAdjoint(2) leaves from the last
+     gradient; the kappa~ part is solveState(2) with the secondary sources of solveIncremental.
+   y_rho = 0 and rho~ is ignored.  The linearization point is the state history of the last
+   evaluation on this handle; the call overwrites c(0) with Phi p~. */
+int glia_rd_hessian(glia_rd_t h, int diffusivity, const double* x, double* y);
 /* The same evaluation for several handles at once, which may have different grids (an ensemble
    of uneven sizes, e.g. 128^3, 192^3 and 256^3 members).  Each handle is evaluated by its own
    host thread on its own stream, so the GPU overlaps them; handles must be distinct and should

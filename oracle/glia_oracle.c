@@ -146,6 +146,7 @@ typedef struct orc {
 
   /* DiffCoef (isotropic: kxx=kyy=kzz, off-diagonals zero; DiffCoef.cpp:77-132) */
   double *k;
+  double *ksec;  /* DiffCoef temp_[7]: secondary coefficient sum_i m_i k~_i (Hessian, diffusivity) */
   double kavg;
   double k_scale, k_gm_wm, k_glm_wm;
   /* ReacCoef (src/mat/ReacCoef.cpp:13-38) */
@@ -232,7 +233,7 @@ orc_t *orc_create(int n0, int n1, int n2) {
   o->lebesgue = o->h[0] * o->h[1] * o->h[2];
   o->wm = dalloc(o->ng); o->gm = dalloc(o->ng); o->csf = dalloc(o->ng);
   o->vt = dalloc(o->ng); o->bg = dalloc(o->ng); o->filter = dalloc(o->ng);
-  o->k = dalloc(o->ng); o->rho = dalloc(o->ng);
+  o->k = dalloc(o->ng); o->rho = dalloc(o->ng); o->ksec = dalloc(o->ng);
   o->prec = dalloc(o->nh);
   o->c_0 = dalloc(o->ng); o->c_t = dalloc(o->ng); o->p_0 = dalloc(o->ng); o->p_t = dalloc(o->ng);
   o->obs = dalloc(o->ng); o->d1 = dalloc(o->ng);
@@ -282,7 +283,7 @@ void orc_destroy(orc_t *o) {
   free_phi(o);
   for (int d = 0; d < 3; d++) plan_free(&o->pl[d]);
   free(o->wm); free(o->gm); free(o->csf); free(o->vt); free(o->bg); free(o->filter);
-  free(o->k); free(o->rho); free(o->prec);
+  free(o->k); free(o->rho); free(o->prec); free(o->ksec);
   free(o->c_0); free(o->c_t); free(o->p_0); free(o->p_t); free(o->obs); free(o->d1);
   for (int i = 0; i < 12; i++) free(o->w[i]);
   free(o->cg_p); free(o->cg_w); free(o->rhs); free(o->adj_temp);
@@ -542,17 +543,25 @@ void orc_get_fields(orc_t *o, double *k, double *rho, double *filter, double *ka
 /* ------------------------------------------------------------------------ */
 
 /* DiffCoef::applyD (src/mat/DiffCoef.cpp:249-271) with applyK (:206-247) for isotropic k */
-void orc_apply_d(orc_t *o, const double *c, double *dc) {
+static void apply_d_with(orc_t *o, const double *kf, const double *c, double *dc) {
   double *g0 = o->w[4], *g1 = o->w[5], *g2 = o->w[6];
   double *t1 = o->w[1], *t2 = o->w[2], *t3 = o->w[3];
   orc_grad(o, c, g0, g1, g2);
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < o->ng; i++) {
-    t1[i] = o->k[i] * g0[i];
-    t2[i] = o->k[i] * g1[i];
-    t3[i] = o->k[i] * g2[i];
+    t1[i] = kf[i] * g0[i];
+    t2[i] = kf[i] * g1[i];
+    t3[i] = kf[i] * g2[i];
   }
   orc_div(o, t1, t2, t3, dc);
+}
+
+void orc_apply_d(orc_t *o, const double *c, double *dc) { apply_d_with(o, o->k, c, dc); }
+
+/* DiffCoef::setSecondaryCoefficients (src/mat/DiffCoef.cpp:44-58), nk = 1 */
+void orc_set_secondary(orc_t *o, double k1) {
+  const double k2 = o->k_gm_wm * k1, k3 = o->k_glm_wm * k1;
+  for (int64_t i = 0; i < o->ng; i++) o->ksec[i] = o->wm[i] * k1 + k2 * o->gm[i] + k3 * o->csf[i];
 }
 
 /* DiffusionSolver::precFactor (src/pde/DiffusionSolver.cpp:119-180) */
@@ -743,20 +752,35 @@ void orc_get_ksp(orc_t *o, int *state, int *adj, int *last) {
   if (last) *last = o->ksp_itr;
 }
 
+/* PdeOperatorsRD::solveIncremental (src/pde/PdeOperators.cpp:192-233) with its dt argument equal
+   to dt/2: c~ += dt/4 * D_sec(1.5 c_i + 0.5 c_{i+1}) (mode 1) or D_sec(0.5 c_i + 1.5 c_{i+1})
+   (mode 2), D_sec = div(k_sec grad) (applyDWithSecondaryCoeffs, src/mat/DiffCoef.cpp:273-300) */
+static void solve_incremental(orc_t *o, int iter, int mode) {
+  double *temp = o->w[9];
+  const double *a = o->c_hist[iter], *b = o->c_hist[iter + 1];
+  for (int64_t i = 0; i < o->ng; i++) temp[i] = 0.5 * (a[i] + b[i]) + (mode == 1 ? a[i] : b[i]);
+  apply_d_with(o, o->ksec, temp, temp);
+  const double s = (o->dt / 2.0) / 2.0;
+  for (int64_t i = 0; i < o->ng; i++) o->c_t[i] += s * temp[i];
+}
+
 /* PdeOperatorsRD::solveState (src/pde/PdeOperators.cpp:235-316), order 2 (Strang),
-   linearized 0 (state) or 1 (tangent linear around c_[i]). */
+   linearized 0 (state), 1 (tangent linear around c_[i]) or 2 (tangent linear with the secondary
+   diffusion sources of the Hessian with diffusivity inversion). */
 int orc_solve_state(orc_t *o, int linearized) {
   const double dt = o->dt;
   memcpy(o->c_t, o->c_0, sizeof(double) * o->ng);
   if (linearized == 0) memcpy(o->c_hist[0], o->c_t, sizeof(double) * o->ng);
   o->diff_ksp_itr_state = 0;
   for (int i = 0; i < o->nt; i++) {
+    if (linearized == 2) solve_incremental(o, i, 1);
     orc_diffusion_solve(o, o->c_t, dt / 2.0);
     o->diff_ksp_itr_state += o->ksp_itr;
     if (linearized == 0 && o->adjoint_store) memcpy(o->c_half[i], o->c_t, sizeof(double) * o->ng);
     orc_reaction(o, o->c_t, linearized ? o->c_hist[i] : NULL, dt);
     orc_diffusion_solve(o, o->c_t, dt / 2.0);
     o->diff_ksp_itr_state += o->ksp_itr;
+    if (linearized == 2) solve_incremental(o, i, 2);
     if (linearized == 0) memcpy(o->c_hist[i + 1], o->c_t, sizeof(double) * o->ng);
   }
   return o->diff_ksp_itr_state;
@@ -991,16 +1015,23 @@ static void grad_kappa_rho(orc_t *o, double *gk, double *gr) {
   *gr = vdot(o, o->wm, P) * o->lebesgue + vdot(o, o->gm, P) * o->lebesgue * o->r_gm_wm;
 }
 
-/* Gauss-Newton Hessian matvec, p block: DerivativeOperatorsRD::evaluateHessian without diffusivity
-   inversion and with L2 regularization (src/grad/DerivativeOperatorsRD.cpp:420-436).  Needs the
-   state history of a previous orc_solve_state(o, 0) at the current x (the linearization point).
-     c~0 = Phi p~;  c~(T) = tangent-linear forward (solveState(1), reaction linearized around c_[i]);
-     a~(T) = -O^T O c~(T);  a~0 = adjoint sweep (solveAdjoint(2));
-     y = beta h^3 Phi^T (Phi p~) - h^3 Phi^T a~0.                                                  */
-void orc_hessian_p(orc_t *o, const double *ptilde, double *y) {
+static void grad_kappa_rho(orc_t *o, double *gk, double *gr);
+
+/* Gauss-Newton Hessian matvec: DerivativeOperatorsRD::evaluateHessian with L2 regularization
+   (src/grad/DerivativeOperatorsRD.cpp:229-438), nk = 1.  x~ and y are [p (np) | kappa | rho].
+   Needs the state history of a previous evaluation at the linearization point.
+   Without diffusivity inversion (:420-436): y_p = beta h^3 Phi^T Phi p~ - h^3 Phi^T a~0 with
+     c~0 = Phi p~, solveState(1), a~(T) = -O^T O c~(T), solveAdjoint(2).
+   With diffusivity inversion (:244-418), y = [Hpp p~ + Hpk k~ ; Hkp p~ + Hkk k~]: the kappa row
+   is the gradDiffusion quadrature of grad c_i . grad a~_i after each incremental adjoint, whose
+   i = nt term reads the adjoint history entry p_[nt] that solveAdjoint(2) leaves from the last
+   gradient; the k~ part runs solveState(2) from c~0 = 0 with secondary coefficients k~ m.
+   The rho entry of y is 0. */
+void orc_hessian(orc_t *o, const double *xt, double *y, int diffusivity) {
   const int np = o->np;
   double *ptemp = (double *)calloc((size_t)(np > 0 ? np : 1), sizeof(double));
-  orc_phi_apply(o, ptilde, o->c_0);
+  for (int i = 0; i < np + 2; i++) y[i] = 0;
+  orc_phi_apply(o, xt, o->c_0);
   orc_solve_state(o, 1);
   for (int64_t i = 0; i < o->ng; i++) o->p_t[i] = -(o->obs[i] * (o->obs[i] * o->c_t[i]));
   orc_solve_adjoint(o, 2);
@@ -1008,7 +1039,32 @@ void orc_hessian_p(orc_t *o, const double *ptilde, double *y) {
   for (int i = 0; i < np; i++) ptemp[i] *= o->lebesgue;
   orc_phi_applyT(o, o->c_0, y);
   for (int i = 0; i < np; i++) y[i] = y[i] * (o->beta * o->lebesgue) - ptemp[i];
+  if (diffusivity) {
+    double gk, gr;
+    grad_kappa_rho(o, &gk, &gr);
+    y[np] = gk;
+    orc_set_secondary(o, xt[np]);
+    memset(o->c_0, 0, sizeof(double) * o->ng);
+    orc_solve_state(o, 2);
+    for (int64_t i = 0; i < o->ng; i++) o->p_t[i] = -(o->obs[i] * (o->obs[i] * o->c_t[i]));
+    orc_solve_adjoint(o, 2);
+    orc_phi_applyT(o, o->p_0, ptemp);
+    for (int i = 0; i < np; i++) y[i] += -o->lebesgue * ptemp[i];
+    grad_kappa_rho(o, &gk, &gr);
+    y[np] += gk;
+  }
   free(ptemp);
+}
+
+void orc_hessian_p(orc_t *o, const double *ptilde, double *y) {
+  const int np = o->np;
+  double *xt = (double *)calloc((size_t)np + 2, sizeof(double));
+  double *yy = (double *)calloc((size_t)np + 2, sizeof(double));
+  memcpy(xt, ptilde, sizeof(double) * np);
+  orc_hessian(o, xt, yy, 0);
+  memcpy(y, yy, sizeof(double) * np);
+  free(xt);
+  free(yy);
 }
 
 /* evaluateObjectiveAndGradient (src/grad/DerivativeOperatorsRD.cpp:130-226) */

@@ -80,6 +80,8 @@ def _load():
         "orc_objective": (d, [vp, _dp, i]),
         "orc_objective_and_gradient": (None, [vp, _dp, i, _dp, _dp]),
         "orc_hessian_p": (None, [vp, _dp, _dp]),
+        "orc_hessian": (None, [vp, _dp, _dp, i]),
+        "orc_set_secondary": (None, [vp, d]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -315,6 +317,12 @@ class Oracle:
         dJ = np.zeros(self.np + 2)
         self.L.orc_objective_and_gradient(self.h, _c(x), int(mode), J, dJ)
         return float(J[0]), dJ
+
+    def hessian(self, xt, diffusivity=False):
+        """Gauss-Newton Hessian matvec on [p | kappa | rho] (kappa blocks with diffusivity)."""
+        y = np.zeros(self.np + 2)
+        self.L.orc_hessian(self.h, _c(xt), y, int(bool(diffusivity)))
+        return y
 
     def hessian_p(self, ptilde):
         """Gauss-Newton Hessian matvec, p block (needs the state history of the last evaluation)."""

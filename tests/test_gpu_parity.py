@@ -330,3 +330,30 @@ def test_hessian_vs_oracle(rd, oracle_mod):
     assert rel(out[0][:-2], ho) < 1e-4
     assert out[0][-2] == 0.0 and out[0][-1] == 0.0
     assert np.array_equal(out[0], out[1])
+
+
+def test_hessian_with_diffusivity_vs_oracle(rd, oracle_mod):
+    """Gauss-Newton Hessian with the kappa blocks (evaluateHessian with diffusivity inversion,
+    nk = 1) against the FP64 oracle (1e-4 relative L2), checkpointing bit-identical to store-all."""
+    pr, o = grad_problem(oracle_mod, nt=8)
+    print("case hessian diffusivity", flush=True)
+    x = np.concatenate([pr["p"], [pr["kappa"], pr["rho"]]])
+    rng = np.random.default_rng(32)
+    d = np.concatenate([rng.standard_normal(len(pr["p"])), [0.01, 0.0]])
+    o.set_data(pr["d1"], None, -0.99)
+    o.set_opt(beta=1e-3)
+    o.set_time(pr["dt"], pr["nt"])
+    o.objective_and_gradient(x, mode=2)
+    ho = o.hessian(d, diffusivity=True)
+    out = []
+    for kw in (dict(), dict(checkpoint_slots=3)):
+        s = setup_gpu(rd, pr, beta=1e-3, **kw)
+        s.set_data(0, pr["d1"], None, -0.99)
+        s.objective_and_gradient([x])
+        out.append(s.hessian([d], diffusivity=True)[0])
+        s.close()
+    print("hessian diffusivity rel", rel(out[0], ho), out[0][-2], ho[-2], flush=True)
+    assert rel(out[0][:-2], ho[:-2]) < 1e-4
+    assert abs(out[0][-2] - ho[-2]) <= 1e-4 * abs(ho[-2])
+    assert out[0][-1] == 0.0
+    assert np.array_equal(out[0], out[1])

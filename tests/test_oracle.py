@@ -392,3 +392,67 @@ def test_hessian_gap_to_exact_hessian_shrinks_with_dt(oracle_mod):
         gaps.append(rel(hv, fd))
     assert gaps[1] < gaps[0] and gaps[2] < gaps[1], gaps
     assert gaps[2] < 0.2, gaps
+
+
+def test_hessian_with_diffusivity_is_affine_with_stale_offset(oracle_mod):
+    """The kappa blocks enter through the secondary coefficient k~ m (setSecondaryCoefficients),
+    linearly in the direction.  The kappa row also holds, from each of its two quadratures, the
+    i = nt term with the adjoint history entry p_[nt], which solveAdjoint(2) leaves at the last
+    gradient's a(T): so H(0) = [0 | 2 * (dt/2) h^3 <wm + k_gm gm, grad c(T) . grad a(T)> | 0] and
+    H is affine (to the CG tolerance)."""
+    pr, o = small_problem(oracle_mod, nt=3, dt=0.05, rtol=1e-12)
+    x = np.concatenate([pr["p"], [0.05, 8.0]])
+    o.set_c0(o.phi_apply(pr["p"] * 1.2))
+    o.solve_state(0)
+    o.set_data((o.ct() > 0.3).astype(float), None, -0.99)
+    o.set_opt(beta=1e-3)
+    rng = np.random.default_rng(41)
+    u = np.concatenate([rng.standard_normal(o.np), [0.02, 0.0]])
+    v = np.concatenate([rng.standard_normal(o.np), [-0.01, 0.0]])
+
+    def H(d):
+        o.set_time(o.dt, o.nt)
+        o.objective_and_gradient(x, 0)
+        return o.hessian(d, diffusivity=True)
+
+    h0, hu, hv, huv = H(np.zeros_like(u)), H(u), H(v), H(2 * u - 3 * v)
+    assert rel(huv - h0, 2 * (hu - h0) - 3 * (hv - h0)) < 1e-9
+    assert np.all(h0[:-2] == 0.0) and hu[-1] == 0.0
+    # the offset from the stale adjoint entry, recomputed from the oracle's own history
+    o.set_time(o.dt, o.nt)
+    o.objective_and_gradient(x, 0)
+    gc = o.grad(o.c_hist(o.nt))
+    ga = o.grad(o.p_hist(o.nt))
+    q = sum(a * b for a, b in zip(gc, ga))
+    h3 = (2 * np.pi) ** 3 / o.ng
+    expect = 2 * 0.5 * o.dt * h3 * np.sum((np.clip(pr["wm"], 0, None) + 0.2 * np.clip(pr["gm"], 0, None)) * q)
+    assert abs(h0[-2] - expect) <= 1e-10 * abs(expect)
+
+
+def test_hessian_kappa_blocks_gap_shrinks_with_dt(oracle_mod):
+    """At zero residual, the reference operator (tangent linearized at the step start, pseudo-time
+    averaged secondary sources, the stale adjoint entry p_[nt]) differs from central differences
+    of [g_p | g_kappa] by terms that vanish with dt."""
+    gaps = []
+    for nt in (2, 4, 8):
+        pr, o = small_problem(oracle_mod, nt=nt, dt=0.2 / nt, rtol=1e-12)
+        x = np.concatenate([pr["p"], [0.05, 8.0]])
+        o.set_c0(o.phi_apply(pr["p"]))
+        o.solve_state(0)
+        o.set_data(o.ct(), np.ones(o.ng), 0.0)
+        o.set_opt(beta=1e-3)
+        rng = np.random.default_rng(42)
+        d = np.concatenate([rng.standard_normal(o.np), [0.02, 0.0]])
+        o.set_time(o.dt, o.nt)
+        o.objective_and_gradient(x, 0)
+        hd = o.hessian(d, diffusivity=True)[:-1]
+        h = 1e-4
+
+        def g(xx):
+            o.set_time(o.dt, o.nt)
+            return o.objective_and_gradient(xx, 0)[1][:-1]
+
+        fd = (g(x + h * d) - g(x - h * d)) / (2 * h)
+        gaps.append(rel(hd, fd))
+    assert gaps[1] < gaps[0] and gaps[2] < gaps[1], gaps
+    assert gaps[2] < 0.2, gaps
