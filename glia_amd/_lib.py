@@ -112,6 +112,28 @@ IO_SIGNATURES = {
 }
 
 
+class OptSettings(ctypes.Structure):
+    _fields_ = [("max_it", ctypes.c_int), ("min_it", ctypes.c_int), ("gatol", ctypes.c_double),
+                ("gttol", ctypes.c_double), ("lbfgs_vectors", ctypes.c_int), ("ls_max_funcs", ctypes.c_int),
+                ("ls_sigma", ctypes.c_double), ("ls_minstep", ctypes.c_double)]
+
+
+class OptResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("fevals", ctypes.c_int), ("reason", ctypes.c_int),
+                ("f", ctypes.c_double), ("gnorm", ctypes.c_double), ("gnorm0", ctypes.c_double)]
+
+
+# int (*)(void* ctx, int nprob, const double* x, double* f, double* g)
+FG_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double))
+# include/glia_opt.h
+OPT_SIGNATURES = {
+    "glia_opt_settings_init": (None, [ctypes.POINTER(OptSettings)]),
+    "glia_blmvm": (_i, [_i, _i, _vp, _vp, _vp, FG_FN, _vp, ctypes.POINTER(OptSettings), _vp]),
+    "glia_rd_invert": (_i, [_vp, _i, _vp, _vp, _vp, ctypes.POINTER(OptSettings), _vp]),
+}
+
+
 class FileConfig(ctypes.Structure):
     _fields_ = [
         ("n", ctypes.c_int), ("model", ctypes.c_int), ("nt", ctypes.c_int), ("dt", ctypes.c_double),
@@ -144,7 +166,7 @@ def load():
             "(hipcc --offload-arch=gfx950); there is no CPU fallback" % LIB_PATH
         )
     lib = ctypes.CDLL(LIB_PATH)
-    for name, (res, args) in list(SIGNATURES.items()) + list(IO_SIGNATURES.items()):
+    for name, (res, args) in list(SIGNATURES.items()) + list(IO_SIGNATURES.items()) + list(OPT_SIGNATURES.items()):
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args

@@ -1,5 +1,5 @@
 """The C-ABI library loads on a machine without a GPU and exports every entry point that
-include/glia_rd.h and include/glia_io.h declare (no compute calls here)."""
+include/glia_rd.h, include/glia_io.h and include/glia_opt.h declare (no compute calls here)."""
 import ctypes
 import os
 import re
@@ -32,12 +32,14 @@ def test_library_exports_every_declared_symbol():
     exported = set(re.findall(r"\b(glia_[a-z0-9_]+)\b", out))
     assert set(declared()) <= exported
     assert set(declared("glia_io.h")) <= exported
+    assert set(declared("glia_opt.h")) <= exported
 
 
 def test_python_binding_covers_every_symbol():
     from glia_amd import _lib
     assert set(declared()) == set(_lib.SIGNATURES)
     assert set(declared("glia_io.h")) == set(_lib.IO_SIGNATURES)
+    assert set(declared("glia_opt.h")) == set(_lib.OPT_SIGNATURES)
 
 
 def test_config_defaults_without_gpu():

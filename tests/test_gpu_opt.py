@@ -1,0 +1,54 @@
+"""The inversion loop on the GPU gradient (glia_rd_invert, include/glia_opt.h): a two-member
+kappa/rho inversion (RDOptimizer, src/opt/RDOptimizer.cpp) on exact data reaches the truth from
+two starts, and the FP64 oracle driving the same loop from the first start agrees with it."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_kappa_rho_inversion_on_gpu_recovers_truth(oracle_mod):
+    from glia_amd import opt, rd, synth
+    n, nt, dt = 32, 8, 0.05
+    print("case invert kappa/rho 32^3 nt 8, 2 members", flush=True)
+    pr = synth.problem(n, 5, nbases=6, kappa=0.05, rho=8.0, sigma=2 * np.pi / 32)
+    o = oracle_mod.Oracle(n)
+    o.set_tissue(pr["wm"], pr["gm"], pr["csf"], pr["vt"])
+    o.set_solver(1e-6, 5000)
+    o.set_time(dt, nt)
+    o.set_k(0.05, 0.2, 0.0)
+    o.set_rho(8.0, 0.2, 0.0)
+    o.prec_factor()
+    o.set_phi(pr["centers"], pr["sigma"], 1.0, False)
+    o.set_c0(o.phi_apply(pr["p"]))
+    o.solve_state(0)
+    d = o.ct()
+    o.set_data(d, None, -0.99)
+    o.set_time(dt, nt)
+
+    s = rd.RDSolver(n, nmembers=2, dt=dt, nt=nt, k_gm_wm=0.2, r_gm_wm=0.2)
+    for m in range(2):
+        s.set_tissue(m, pr["wm"], pr["gm"], pr["csf"], pr["vt"])
+        s.set_coefficients(m, 0.05, 8.0)
+        s.set_basis(m, pr["centers"], pr["sigma"])
+        s.set_data(m, d, None, -0.99)
+    s.phi_apply([pr["p"], pr["p"]])
+    starts = [np.concatenate([pr["p"], [0.1, 5.0]]), np.concatenate([pr["p"], [0.02, 12.0]])]
+    lb = np.concatenate([np.full(6, -np.inf), [1e-3, 1.0]])
+    ub = np.concatenate([np.full(6, np.inf), [1.0, 15.0]])
+    settings = dict(gatol=1e-9, gttol=1e-5, max_it=50)
+    xs, res = opt.invert(s, starts, lb, ub, mode=1, **settings)
+    s.close()
+    print("gpu", [x[-2:] for x in xs], res, flush=True)
+    for x, r in zip(xs, res):
+        assert r["reason"] == "gttol", r
+        assert abs(x[-2] - 0.05) < 1e-4 and abs(x[-1] - 8.0) < 2e-3, x[-2:]
+        assert np.array_equal(x[:-2], pr["p"])
+
+    def fg(X):
+        f, g = o.objective_and_gradient(X[0], mode=1)
+        return np.array([f]), g[None]
+    xo, ro = opt.minimize(fg, starts[0], lb, ub, **settings)
+    print("oracle", xo[0][-2:], ro, flush=True)
+    assert abs(xs[0][-2] - xo[0][-2]) < 1e-4 and abs(xs[0][-1] - xo[0][-1]) < 2e-3
+    assert abs(res[0]["iterations"] - ro[0]["iterations"]) <= 3
