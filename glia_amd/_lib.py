@@ -72,6 +72,9 @@ SIGNATURES = {
     "glia_rd_prec_factor": (_i, [_vp]),
     "glia_rd_set_ctx_dt": (_i, [_vp, _d]),
     "glia_rd_get_ctx_dt": (_i, [_vp, _pd]),
+    "glia_rd_set_beta": (_i, [_vp, _d]),
+    "glia_rd_get_grid": (_i, [_vp, _vp]),
+    "glia_rd_get_beta": (_i, [_vp, _pd]),
     "glia_rd_set_basis": (_i, [_vp, _i, _i, _vp, _d]),
     "glia_rd_basis_size": (_i, [_vp, _i]),
     "glia_rd_set_data": (_i, [_vp, _i, _vp, _vp, _d, _i]),
@@ -126,11 +129,39 @@ class OptResult(ctypes.Structure):
 # int (*)(void* ctx, int nprob, const double* x, double* f, double* g)
 FG_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double))
+
+
+class SparseTilSettings(ctypes.Structure):
+    _fields_ = [("sparsity", ctypes.c_int), ("max_it", ctypes.c_int), ("ftol", ctypes.c_double),
+                ("prune_components", ctypes.c_int), ("thresh_component_weight", ctypes.c_double),
+                ("diffusivity_inversion", ctypes.c_int), ("reaction_inversion", ctypes.c_int),
+                ("beta", ctypes.c_double), ("k_lb", ctypes.c_double), ("k_ub", ctypes.c_double),
+                ("rho_lb", ctypes.c_double), ("rho_ub", ctypes.c_double), ("til", OptSettings), ("rd", OptSettings)]
+
+
+class SparseTilResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("stop", ctypes.c_int), ("nnz", ctypes.c_int), ("J", ctypes.c_double),
+                ("til", OptResult), ("rd", OptResult)]
+
+
+# int (*)(void* ctx, int mode, double beta, const double* x, double* f, double* g)
+EVAL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                           ctypes.POINTER(ctypes.c_double))
+# int (*)(void* ctx, const double* p, double* cmax)
+C0_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                         ctypes.POINTER(ctypes.c_double))
 # include/glia_opt.h
 OPT_SIGNATURES = {
     "glia_opt_settings_init": (None, [ctypes.POINTER(OptSettings)]),
     "glia_blmvm": (_i, [_i, _i, _vp, _vp, _vp, FG_FN, _vp, ctypes.POINTER(OptSettings), _vp]),
     "glia_rd_invert": (_i, [_vp, _i, _vp, _vp, _vp, ctypes.POINTER(OptSettings), _vp]),
+    "glia_hard_threshold": (_i, [_vp, _i, _i, _vp]),
+    "glia_hard_threshold_components": (_i, [_vp, _i, _i, _vp, _vp, _i, _d, _i, _vp]),
+    "glia_sparse_til_settings_init": (None, [ctypes.POINTER(SparseTilSettings)]),
+    "glia_sparse_til": (_i, [_i, _i, _vp, _vp, _vp, _vp, _i, EVAL_FN, C0_FN, _vp,
+                             ctypes.POINTER(SparseTilSettings), _vp]),
+    "glia_rd_sparse_til": (_i, [_vp, _vp, _vp, _vp, _vp, _i, ctypes.POINTER(SparseTilSettings), _vp]),
 }
 
 

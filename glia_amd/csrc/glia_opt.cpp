@@ -23,7 +23,10 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <limits>
+#include <queue>
+#include <utility>
 #include <vector>
 
 #include "../../include/glia_opt.h"
@@ -36,7 +39,7 @@ struct Problem {
   const double* lb = nullptr;
   const double* ub = nullptr;
   std::vector<double> g, gp, d, xt;
-  double f = 0, gnorm = 0, gnorm0 = 1, gtd = 0, alpha = 1;
+  double f = 0, gnorm = 0, gnorm0 = 0, gtd = 0, alpha = 1;
   int ls_k = 0, it = 0, fevals = 0, reason = GLIA_OPT_CONTINUE;
   bool started = false, reset_tried = false;
   std::deque<std::vector<double>> S, Y;
@@ -132,7 +135,7 @@ void advance(Problem& p, const glia_opt_settings& s, double f, const double* g) 
     std::copy(g, g + p.n, p.g.begin());
     project_gradient(p, p.x, g, p.gp);
     p.gnorm = std::sqrt(dot(p.gp, p.gp));
-    p.gnorm0 = (p.gnorm > 0) ? p.gnorm : 1.0;
+    if (!(p.gnorm0 > 0)) p.gnorm0 = (p.gnorm > 0) ? p.gnorm : 1.0;
     if (!converged(p, s, 1.0)) new_search(p);
     return;
   }
@@ -183,36 +186,13 @@ void advance(Problem& p, const glia_opt_settings& s, double f, const double* g) 
   p.reason = GLIA_OPT_DIVERGED_LS_FAILURE;
 }
 
-struct RdCtx {
-  glia_rd_t h;
-  int mode;
-};
+using FG = std::function<int(int nprob, const double* x, double* f, double* g)>;
 
-int rd_fg(void* vctx, int nprob, const double* x, double* f, double* g) {
-  RdCtx* c = static_cast<RdCtx*>(vctx);
-  (void)nprob;
-  return glia_rd_objective_and_gradient(c->h, c->mode, x, f, g);
-}
-
-}  // namespace
-
-extern "C" {
-
-void glia_opt_settings_init(glia_opt_settings* s) {
-  if (!s) return;
-  s->max_it = 30;
-  s->min_it = 1;
-  s->gatol = 1e-4;
-  s->gttol = 1e-3;
-  s->lbfgs_vectors = 10;
-  s->ls_max_funcs = 10;
-  s->ls_sigma = 1e-4;
-  s->ls_minstep = std::numeric_limits<double>::epsilon();
-}
-
-int glia_blmvm(int nprob, int n, double* x, const double* lb, const double* ub, glia_fg_fn fg, void* ctx,
-               const glia_opt_settings* sp, glia_opt_result* r) {
-  if (nprob <= 0 || n <= 0 || !x || !fg) return GLIA_RD_ERR_ARG;
+// the loop; gref (nprob, optional): reference gradient norms for the relative test (the
+// reference's gradnorm0_); without them the initial projected gradient norm is used
+int blmvm(int nprob, int n, double* x, const double* lb, const double* ub, const FG& fg, const glia_opt_settings* sp,
+          const double* gref, glia_opt_result* r) {
+  if (nprob <= 0 || n <= 0 || !x) return GLIA_RD_ERR_ARG;
   glia_opt_settings s;
   if (sp) {
     s = *sp;
@@ -229,6 +209,7 @@ int glia_blmvm(int nprob, int n, double* x, const double* lb, const double* ub, 
     p.g.assign(n, 0.0);
     p.gp.assign(n, 0.0);
     p.xt.assign(n, 0.0);
+    if (gref && gref[m] > 0) p.gnorm0 = gref[m];
     for (int i = 0; i < n; i++) p.x[i] = std::min(std::max(p.x[i], lo(p, i)), hi(p, i));
     std::copy(p.x, p.x + n, p.xt.begin());
   }
@@ -248,7 +229,7 @@ int glia_blmvm(int nprob, int n, double* x, const double* lb, const double* ub, 
       }
     }
     if (!active) break;
-    const int rc = fg(ctx, nprob, xe.data(), fe.data(), ge.data());
+    const int rc = fg(nprob, xe.data(), fe.data(), ge.data());
     if (rc) return rc;
     for (int m = 0; m < nprob; m++)
       if (P[m].reason == GLIA_OPT_CONTINUE) advance(P[m], s, fe[m], ge.data() + (size_t)m * n);
@@ -265,15 +246,418 @@ int glia_blmvm(int nprob, int n, double* x, const double* lb, const double* ub, 
   return 0;
 }
 
+// the reference's gradnorm0_ for a p-inversion (src/opt/TaoInterface.cpp:285-300): the gradient
+// at the zero vector of the unknowns the solve may move (lb < ub) -- entries fixed by their
+// bounds keep their values, as rho does outside the TIL vector -- and its norm over those unknowns
+int gradient_norm_at_zero(int nprob, int n, const double* x, const double* lb, const double* ub, const FG& fg,
+                          std::vector<double>& gref) {
+  const size_t N = (size_t)nprob * n;
+  auto fixed = [&](size_t k) { return lb && ub && lb[k] >= ub[k]; };
+  std::vector<double> x0(N), f(nprob), g(N);
+  for (size_t k = 0; k < N; k++) x0[k] = fixed(k) ? x[k] : 0.0;
+  const int rc = fg(nprob, x0.data(), f.data(), g.data());
+  if (rc) return rc;
+  gref.assign(nprob, 0.0);
+  for (int m = 0; m < nprob; m++) {
+    double s = 0;
+    for (int i = 0; i < n; i++) {
+      const size_t k = (size_t)m * n + i;
+      if (!fixed(k)) s += g[k] * g[k];
+    }
+    gref[m] = std::sqrt(s);
+  }
+  return 0;
+}
+
+// plain and component-wise hardThreshold (src/utils/Utils.cpp:339-463): a max-priority queue of
+// (value, index) pairs -- signed values, ties to the larger index -- popped while |value| > 0
+int threshold_simple(const double* x, int n, int s, std::vector<int>& out) {
+  std::priority_queue<std::pair<double, int>> q;
+  for (int i = 0; i < n; i++) q.push({x[i], i});
+  int nnz = 0;
+  for (int k = 0; k < s && !q.empty(); k++) {
+    if (!(std::fabs(q.top().first) > 0.0)) break;
+    out.push_back(q.top().second);
+    nnz++;
+    q.pop();
+  }
+  return nnz;
+}
+
+int threshold_components(const double* x, int n, int s, const int* labels, const double* w, int ncomp, double thresh,
+                         bool double_mode, std::vector<int>& out) {
+  if (double_mode) s /= 2;
+  int ncc = 0, ncc_t = 0;
+  const double reduce = thresh / 10.0;
+  for (int c = 0; c < ncomp; c++) {
+    if (w[c] >= thresh) ncc++;
+    if (w[c] < thresh && w[c] >= reduce) ncc_t++;
+  }
+  std::vector<int> budget;
+  int nnz = 0;
+  for (int c = 0; c < ncomp; c++) {
+    int sp;
+    if (c != ncomp - 1) {
+      if (w[c] >= thresh) {
+        sp = 3 + (int)std::floor(w[c] * (s - 3 * ncc - ncc_t));
+      } else if (w[c] >= reduce) {
+        sp = 1;
+      } else {
+        sp = 0;
+      }
+      if (double_mode) sp *= 2;
+    } else {  // the last component takes the rest of the budget
+      int used = 0;
+      for (int b : budget) used += b;
+      if (double_mode) s *= 2;
+      sp = s - used;
+    }
+    budget.push_back(sp);
+    std::priority_queue<std::pair<double, int>> q;
+    for (int i = 0; i < n; i++)
+      if (labels[i] == c + 1) q.push({x[i], i});
+    for (int k = 0; k < sp && !q.empty(); k++) {
+      if (!(std::fabs(q.top().first) > 0.0)) break;
+      out.push_back(q.top().second);
+      nnz++;
+      q.pop();
+    }
+  }
+  return nnz;
+}
+
+void sort_unique(std::vector<int>& v) {
+  std::sort(v.begin(), v.end());
+  v.erase(std::unique(v.begin(), v.end()), v.end());
+}
+
+struct TilOps {
+  std::function<int(int mode, double beta, const double* x, double* f, double* g)> eval;
+  std::function<int(const double* p, double* cmax)> set_c0;
+};
+
+// SparseTILOptimizer::solve (src/opt/SparseTILOptimizer.cpp:220-595) for nprob problems in lockstep
+int sparse_til(int nprob, int np, double* x, int* support_out, const int* labels, const double* weights, int ncomp,
+               const TilOps& ops, const glia_sparse_til_settings& s, glia_sparse_til_result* r) {
+  const int n = np + 2;
+  const size_t N = (size_t)nprob * n;
+  std::vector<std::vector<int>> support(nprob);
+  std::vector<int> its(nprob, 0), stop(nprob, 0);
+  std::vector<double> J(nprob), Jref(nprob), f(nprob), g(N);
+  const bool comps = labels && weights && ncomp > 0;
+  // the injected support: nonzero entries of the initial p (a warm start sets p on its support)
+  for (int m = 0; m < nprob; m++)
+    for (int i = 0; i < np; i++)
+      if (x[(size_t)m * n + i] != 0.0) support[m].push_back(i);
+  auto til_fg = [&](int nprob_, const double* xe, double* fe, double* ge) {
+    (void)nprob_;
+    return ops.eval(GLIA_RD_MODE_P_KAPPA, s.beta, xe, fe, ge);
+  };
+  // corrective L2 solve (TILOptimizer) on each problem's support; problems in 'frozen' keep x
+  auto til_solve = [&](const std::vector<int>& frozen, glia_opt_result* res) {
+    std::vector<double> lb(N), ub(N);
+    for (int m = 0; m < nprob; m++) {
+      double* xm = x + (size_t)m * n;
+      double* l = lb.data() + (size_t)m * n;
+      double* u = ub.data() + (size_t)m * n;
+      for (int i = 0; i < n; i++) l[i] = u[i] = xm[i];
+      if (frozen[m]) continue;
+      for (int i : support[m]) {
+        l[i] = 0.0;
+        u[i] = std::numeric_limits<double>::infinity();
+      }
+      if (s.diffusivity_inversion) {
+        l[np] = s.k_lb;
+        u[np] = s.k_ub;
+      }
+    }
+    std::vector<double> gref;
+    int rc = gradient_norm_at_zero(nprob, n, x, lb.data(), ub.data(), til_fg, gref);
+    if (rc) return rc;
+    return blmvm(nprob, n, x, lb.data(), ub.data(), til_fg, &s.til, gref.data(), res);
+  };
+  // (1) reference objective and gradient, beta = 0 for thresholding
+  int rc = ops.eval(GLIA_RD_MODE_P_KAPPA, 0.0, x, Jref.data(), g.data());
+  if (rc) return rc;
+  J = Jref;
+  std::vector<int> active(nprob, 1), frozen(nprob, 0);
+  std::vector<double> tmp(np), x_old(N);
+  std::vector<glia_opt_result> res(nprob);
+  for (;;) {
+    bool any = false;
+    for (int m = 0; m < nprob; m++) any = any || active[m];
+    if (!any) break;
+    for (int m = 0; m < nprob; m++) {
+      frozen[m] = !active[m];
+      if (!active[m]) continue;
+      its[m]++;
+      // hard-threshold |g_p| to 2s and merge with the support
+      for (int i = 0; i < np; i++) tmp[i] = std::fabs(g[(size_t)m * n + i]);
+      std::vector<int> idx;
+      if (comps) {
+        threshold_components(tmp.data(), np, 2 * s.sparsity, labels + (size_t)m * np, weights + (size_t)m * ncomp,
+                             ncomp, s.thresh_component_weight, true, idx);
+      } else {
+        threshold_simple(tmp.data(), np, 2 * s.sparsity, idx);
+      }
+      support[m].insert(support[m].end(), idx.begin(), idx.end());
+      sort_unique(support[m]);
+    }
+    std::copy(x, x + N, x_old.begin());
+    if ((rc = til_solve(frozen, res.data()))) return rc;
+    for (int m = 0; m < nprob; m++) {
+      if (!active[m]) continue;
+      double* xm = x + (size_t)m * n;
+      // prune the solution to s entries; everything off the support is zero
+      std::vector<int> idx;
+      if (comps && s.prune_components) {
+        threshold_components(xm, np, s.sparsity, labels + (size_t)m * np, weights + (size_t)m * ncomp, ncomp,
+                             s.thresh_component_weight, false, idx);
+      } else {
+        threshold_simple(xm, np, s.sparsity, idx);
+      }
+      support[m] = idx;
+      sort_unique(support[m]);
+      std::vector<double> keep(np, 0.0);
+      for (int i : support[m]) keep[i] = xm[i];
+      std::copy(keep.begin(), keep.end(), xm);
+    }
+    // mismatch and gradient (beta = 0) of the pruned solution; stopping rules of the reference
+    std::vector<double> Jold = J;
+    if ((rc = ops.eval(GLIA_RD_MODE_P_KAPPA, 0.0, x, f.data(), g.data()))) return rc;
+    for (int m = 0; m < nprob; m++) {
+      if (!active[m]) continue;
+      J[m] = f[m];
+      if (its[m] >= s.max_it) {
+        stop[m] = 1;
+      } else if (std::fabs(J[m]) < 1e-5) {
+        stop[m] = 2;
+      } else if (std::fabs(Jold[m] - J[m]) < s.ftol * std::fabs(1.0 + Jref[m])) {
+        stop[m] = 3;
+      }
+      if (stop[m]) active[m] = 0;
+    }
+  }
+  // (3) final L2 solve on the support
+  std::fill(frozen.begin(), frozen.end(), 0);
+  if ((rc = til_solve(frozen, res.data()))) return rc;
+  std::vector<glia_opt_result> rd_res(nprob);
+  std::memset(rd_res.data(), 0, sizeof(glia_opt_result) * nprob);
+  // (4) reaction/diffusion inversion with c0 = Phi p rescaled to max 1 (RDOptimizer::setInitialGuess,
+  // src/opt/RDOptimizer.cpp:80-135) and rho started from the best of a scan
+  if (s.reaction_inversion) {
+    std::vector<double> p((size_t)nprob * np), cmax(nprob);
+    for (int m = 0; m < nprob; m++) std::copy(x + (size_t)m * n, x + (size_t)m * n + np, p.begin() + (size_t)m * np);
+    if ((rc = ops.set_c0(p.data(), cmax.data()))) return rc;
+    for (int m = 0; m < nprob; m++) {
+      if (!(cmax[m] > 0)) return GLIA_RD_ERR_ARG;  // c0 = 0: nothing to rescale
+      const double scale = 1.0 / cmax[m];
+      for (int i = 0; i < np; i++) {
+        p[(size_t)m * np + i] *= scale;
+        x[(size_t)m * n + i] *= scale;
+      }
+    }
+    if ((rc = ops.set_c0(p.data(), cmax.data()))) return rc;
+    static const double scan[7] = {0, 3, 6, 9, 10, 12, 15};
+    std::vector<double> best(nprob, 1e300), xs(x, x + N);
+    std::vector<double> rho0(nprob, 0.0);
+    for (double rho : scan) {
+      for (int m = 0; m < nprob; m++) xs[(size_t)m * n + np + 1] = rho;
+      if ((rc = ops.eval(GLIA_RD_MODE_KAPPA_RHO, 0.0, xs.data(), f.data(), nullptr))) return rc;
+      for (int m = 0; m < nprob; m++)
+        if (f[m] < best[m]) {
+          best[m] = f[m];
+          rho0[m] = rho;
+        }
+    }
+    std::vector<double> lb(N), ub(N);
+    for (int m = 0; m < nprob; m++) {
+      double* xm = x + (size_t)m * n;
+      xm[np + 1] = rho0[m];
+      double* l = lb.data() + (size_t)m * n;
+      double* u = ub.data() + (size_t)m * n;
+      for (int i = 0; i < n; i++) l[i] = u[i] = xm[i];
+      l[np] = s.k_lb;
+      u[np] = s.k_ub;
+      l[np + 1] = s.rho_lb;
+      u[np + 1] = s.rho_ub;
+    }
+    auto rd_fg = [&](int nprob_, const double* xe, double* fe, double* ge) {
+      (void)nprob_;
+      return ops.eval(GLIA_RD_MODE_KAPPA_RHO, 0.0, xe, fe, ge);
+    };
+    if ((rc = blmvm(nprob, n, x, lb.data(), ub.data(), rd_fg, &s.rd, nullptr, rd_res.data()))) return rc;
+  }
+  for (int m = 0; m < nprob; m++) {
+    if (support_out) {
+      int* so = support_out + (size_t)m * np;
+      for (int i = 0; i < np; i++) so[i] = (i < (int)support[m].size()) ? support[m][i] : -1;
+    }
+    if (r) {
+      r[m].iterations = its[m];
+      r[m].stop = stop[m];
+      r[m].nnz = (int)support[m].size();
+      r[m].J = J[m];
+      r[m].til = res[m];
+      r[m].rd = rd_res[m];
+    }
+  }
+  return 0;
+}
+
+struct RdCtx {
+  glia_rd_t h;
+  int mode;
+};
+
+}  // namespace
+
+extern "C" {
+
+void glia_opt_settings_init(glia_opt_settings* s) {
+  if (!s) return;
+  s->max_it = 30;
+  s->min_it = 1;
+  s->gatol = 1e-4;
+  s->gttol = 1e-3;
+  s->lbfgs_vectors = 10;
+  s->ls_max_funcs = 10;
+  s->ls_sigma = 1e-4;
+  s->ls_minstep = std::numeric_limits<double>::epsilon();
+}
+
+int glia_blmvm(int nprob, int n, double* x, const double* lb, const double* ub, glia_fg_fn fg, void* ctx,
+               const glia_opt_settings* sp, glia_opt_result* r) {
+  if (!fg) return GLIA_RD_ERR_ARG;
+  FG f = [&](int np_, const double* xe, double* fe, double* ge) { return fg(ctx, np_, xe, fe, ge); };
+  return blmvm(nprob, n, x, lb, ub, f, sp, nullptr, r);
+}
+
 int glia_rd_invert(glia_rd_t h, int mode, double* x, const double* lb, const double* ub, const glia_opt_settings* s,
                    glia_opt_result* r) {
+  const int nm = glia_rd_num_members(h);
+  if (nm <= 0 || !x || mode < 0 || mode > 2) return GLIA_RD_ERR_ARG;
+  const int np = glia_rd_basis_size(h, 0);
+  for (int m = 1; m < nm; m++)
+    if (glia_rd_basis_size(h, m) != np) return GLIA_RD_ERR_ARG;
+  FG fg = [&](int, const double* xe, double* fe, double* ge) {
+    return glia_rd_objective_and_gradient(h, mode, xe, fe, ge);
+  };
+  const int n = np + 2;
+  if (mode == GLIA_RD_MODE_KAPPA_RHO) {
+    // RDOptimizer::solve (src/opt/RDOptimizer.cpp:170-250): beta = 0 during the solve, reference
+    // gradient = the first one
+    double beta = 0;
+    int rc = glia_rd_get_beta(h, &beta);
+    if (rc || (rc = glia_rd_set_beta(h, 0.0))) return rc;
+    rc = blmvm(nm, n, x, lb, ub, fg, s, nullptr, r);
+    const int rc2 = glia_rd_set_beta(h, beta);
+    return rc ? rc : rc2;
+  }
+  // TILOptimizer: reference gradient at x = 0
+  std::vector<double> gref;
+  int rc = gradient_norm_at_zero(nm, n, x, lb, ub, fg, gref);
+  if (rc) return rc;
+  return blmvm(nm, n, x, lb, ub, fg, s, gref.data(), r);
+}
+
+void glia_sparse_til_settings_init(glia_sparse_til_settings* s) {
+  if (!s) return;
+  s->sparsity = 5;
+  s->max_it = 5;
+  s->ftol = 1e-5;
+  s->prune_components = 1;
+  s->thresh_component_weight = 1e-3;
+  s->diffusivity_inversion = 1;
+  s->reaction_inversion = 0;
+  s->beta = 0.0;
+  s->k_lb = 1e-3;
+  s->k_ub = 1.0;
+  s->rho_lb = 1.0;
+  s->rho_ub = 15.0;
+  glia_opt_settings_init(&s->til);
+  glia_opt_settings_init(&s->rd);
+}
+
+int glia_hard_threshold(const double* x, int n, int s, int* support) {
+  if (!x || n < 0 || s < 0 || (!support && s > 0)) return GLIA_RD_ERR_ARG;
+  std::vector<int> out;
+  const int nnz = threshold_simple(x, n, s, out);
+  std::copy(out.begin(), out.end(), support);
+  return nnz;
+}
+
+int glia_hard_threshold_components(const double* x, int n, int s, const int* labels, const double* weights,
+                                   int ncomp, double thresh, int double_mode, int* support) {
+  if (!x || !labels || !weights || n < 0 || s < 0 || ncomp <= 0 || !support) return GLIA_RD_ERR_ARG;
+  std::vector<int> out;
+  const int nnz = threshold_components(x, n, s, labels, weights, ncomp, thresh, double_mode != 0, out);
+  std::copy(out.begin(), out.end(), support);
+  return nnz;
+}
+
+int glia_sparse_til(int nprob, int np, double* x, int* support, const int* labels, const double* weights, int ncomp,
+                    glia_eval_fn eval, glia_c0_fn set_c0, void* ctx, const glia_sparse_til_settings* sp,
+                    glia_sparse_til_result* r) {
+  if (nprob <= 0 || np <= 0 || !x || !eval) return GLIA_RD_ERR_ARG;
+  glia_sparse_til_settings s;
+  if (sp) {
+    s = *sp;
+  } else {
+    glia_sparse_til_settings_init(&s);
+  }
+  if (s.reaction_inversion && !set_c0) return GLIA_RD_ERR_ARG;
+  TilOps ops;
+  ops.eval = [&](int mode, double beta, const double* xe, double* f, double* g) {
+    return eval(ctx, mode, beta, xe, f, g);
+  };
+  ops.set_c0 = [&](const double* p, double* cmax) { return set_c0(ctx, p, cmax); };
+  return sparse_til(nprob, np, x, support, labels, weights, ncomp, ops, s, r);
+}
+
+int glia_rd_sparse_til(glia_rd_t h, double* x, int* support, const int* labels, const double* weights, int ncomp,
+                       const glia_sparse_til_settings* sp, glia_sparse_til_result* r) {
   const int nm = glia_rd_num_members(h);
   if (nm <= 0 || !x) return GLIA_RD_ERR_ARG;
   const int np = glia_rd_basis_size(h, 0);
   for (int m = 1; m < nm; m++)
     if (glia_rd_basis_size(h, m) != np) return GLIA_RD_ERR_ARG;
-  RdCtx c{h, mode};
-  return glia_blmvm(nm, np + 2, x, lb, ub, rd_fg, &c, s, r);
+  glia_sparse_til_settings s;
+  if (sp) {
+    s = *sp;
+  } else {
+    glia_sparse_til_settings_init(&s);
+  }
+  int grid[3];
+  int rc = glia_rd_get_grid(h, grid);
+  if (rc) return rc;
+  const size_t ng = (size_t)grid[0] * grid[1] * grid[2];
+  double beta0 = 0;
+  if ((rc = glia_rd_get_beta(h, &beta0))) return rc;
+  // the TIL stages run with the given rho (tu_->rho_)
+  for (int m = 0; m < nm; m++)
+    if ((rc = glia_rd_set_coefficients(h, m, x[(size_t)m * (np + 2) + np], x[(size_t)m * (np + 2) + np + 1])))
+      return rc;
+  std::vector<float> c0;
+  TilOps ops;
+  ops.eval = [&](int mode, double beta, const double* xe, double* f, double* g) {
+    int e = glia_rd_set_beta(h, beta);
+    if (e) return e;
+    return g ? glia_rd_objective_and_gradient(h, mode, xe, f, g) : glia_rd_objective(h, mode, xe, f);
+  };
+  ops.set_c0 = [&](const double* p, double* cmax) {
+    int e = glia_rd_phi_apply(h, p);
+    if (e) return e;
+    c0.resize(ng);
+    for (int m = 0; m < nm; m++) {
+      if ((e = glia_rd_get_field(h, m, GLIA_RD_FIELD_C0, c0.data(), GLIA_RD_HOST))) return e;
+      cmax[m] = *std::max_element(c0.begin(), c0.end());
+    }
+    return 0;
+  };
+  rc = sparse_til(nm, np, x, support, labels, weights, ncomp, ops, s, r);
+  const int rc2 = glia_rd_set_beta(h, beta0);
+  return rc ? rc : rc2;
 }
 
 }  // extern "C"

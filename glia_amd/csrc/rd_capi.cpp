@@ -136,6 +136,13 @@ int glia_rd_num_members(glia_rd_t h) {
   return h->e->nmembers();
 }
 
+int glia_rd_get_grid(glia_rd_t h, int* n) {
+  GLIA_H(h);
+  if (!n) return GLIA_RD_ERR_ARG;
+  for (int d = 0; d < 3; d++) n[d] = h->e->config().n[d];
+  return 0;
+}
+
 int glia_rd_set_tissue(glia_rd_t h, int member, const float* wm, const float* gm, const float* csf, const float* vt,
                        int mem_kind) {
   GLIA_H(h);
@@ -174,6 +181,20 @@ int glia_rd_get_ctx_dt(glia_rd_t h, double* dt) {
   GLIA_H(h);
   if (!dt) return GLIA_RD_ERR_ARG;
   *dt = h->e->ctx_dt();
+  return 0;
+}
+
+int glia_rd_set_beta(glia_rd_t h, double beta) {
+  GLIA_H(h);
+  if (!(beta >= 0)) return GLIA_RD_ERR_ARG;
+  h->e->set_beta(beta);
+  return 0;
+}
+
+int glia_rd_get_beta(glia_rd_t h, double* beta) {
+  GLIA_H(h);
+  if (!beta) return GLIA_RD_ERR_ARG;
+  *beta = h->e->config().beta;
   return 0;
 }
 

@@ -61,6 +61,7 @@ class RDEnsemble {
   int set_k_sinusoidal(int m, double scale);                 // DiffCoef::setValuesSinusoidal
   int prec_factor();                                         // DiffusionSolver::precFactor
   void set_ctx_dt(double dt) { ctx_dt_ = dt; }
+  void set_beta(double beta) { cfg_.beta = beta; }
   double ctx_dt() const { return ctx_dt_; }
   int set_basis(int m, int np, const double* centers, double sigma);
   int set_data(int m, const float* d1, const float* obs_mask, double thr, bool device);

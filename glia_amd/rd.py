@@ -141,6 +141,17 @@ class RDSolver:
     def ctx_dt(self, dt):
         self._chk(self.L.glia_rd_set_ctx_dt(self.h, float(dt)))
 
+    @property
+    def beta(self):
+        """OptimizerSettings::beta_ (L2 regularization of c0)."""
+        v = ctypes.c_double()
+        self._chk(self.L.glia_rd_get_beta(self.h, ctypes.byref(v)))
+        return v.value
+
+    @beta.setter
+    def beta(self, beta):
+        self._chk(self.L.glia_rd_set_beta(self.h, float(beta)))
+
     def set_basis(self, member, centers, sigma):
         c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1)
         self._chk(self.L.glia_rd_set_basis(self.h, member, c.size // 3, c.ctypes.data_as(ctypes.c_void_p), float(sigma)))

@@ -2,7 +2,8 @@
  * glia_opt.h -- the inversion loop around the gradient: a bound-constrained limited-memory
  * quasi-Newton method with an Armijo line search (the reference drives TAO's BLMVM with the
  * Armijo line search, src/opt/Optimizer.cpp:116-260, and stops with its own test,
- * checkConvergenceGrad, src/opt/TaoInterface.cpp:362-520).
+ * checkConvergenceGrad, src/opt/TaoInterface.cpp:362-520), and the CoSaMP sparse-TIL driver
+ * around it (src/opt/SparseTILOptimizer.cpp).
  *
  * PETSc/TAO is not available, so this is a restatement of the published BLMVM algorithm, not a
  * bitwise reproduction of TAO: same inputs, options and stopping rule, own quasi-Newton matrix
@@ -61,9 +62,76 @@ int glia_blmvm(int nprob, int n, double* x, const double* lb, const double* ub, 
 
 /* The inversion of a handle: every member minimizes J over x = [p | kappa | rho] of its own
    (mode as for glia_rd_objective_and_gradient; the entries the mode does not invert keep their
-   initial values since their gradient is 0).  All members need the same number of bases. */
+   initial values since their gradient is 0).  All members need the same number of bases.
+   GLIA_RD_MODE_KAPPA_RHO is RDOptimizer::solve (src/opt/RDOptimizer.cpp:145-250): beta is 0 during
+   the solve and the relative test is against the first gradient.  The p modes are
+   TILOptimizer::solve (src/opt/TILOptimizer.cpp:114-230): the relative test is against the
+   gradient at x = 0 (src/opt/TaoInterface.cpp:285-300), taken over the entries with lb < ub (the
+   others keep their values). */
 int glia_rd_invert(glia_rd_t h, int mode, double* x, const double* lb, const double* ub, const glia_opt_settings* s,
                    glia_opt_result* r);
+
+/* hardThreshold (src/utils/Utils.cpp:339-367): indices of the s largest entries of x[0..n) by
+   signed value (ties: larger index first), stopping at the first entry with |x| = 0.  support
+   receives them in that order; returns their count. */
+int glia_hard_threshold(const double* x, int n, int s, int* support);
+/* hardThreshold by connected component (src/utils/Utils.cpp:370-463): labels[i] in 1..ncomp,
+   weights[ncomp]; components of weight >= thresh get 3 + floor(w (s - 3 ncc - ncc_t)) entries,
+   those of weight in [thresh/10, thresh) get 1, the last component the rest; double_mode halves
+   s for that split and doubles every share (the CoSaMP gradient step).  Returns the count. */
+int glia_hard_threshold_components(const double* x, int n, int s, const int* labels, const double* weights,
+                                   int ncomp, double thresh, int double_mode, int* support);
+
+/* The operations sparse TIL needs from an RD model of nprob problems, x = [p | kappa | rho] per
+   problem (np + 2 entries):
+   eval: J (nprob) and, when g != NULL, dJ (nprob*(np+2)) at x with the mode of
+         glia_rd_objective_and_gradient and regularization beta;
+   set_c0: c0 = Phi p for every problem (p: nprob*np); cmax (nprob) receives max c0 over the grid. */
+typedef int (*glia_eval_fn)(void* ctx, int mode, double beta, const double* x, double* f, double* g);
+typedef int (*glia_c0_fn)(void* ctx, const double* p, double* cmax);
+
+typedef struct glia_sparse_til_settings {
+  int sparsity;                    /* sparsity_level_ (5) */
+  int max_it;                      /* gist_maxit_ (5): CoSaMP iterations */
+  double ftol;                     /* 1e-5: |J_old - J| < ftol |1 + J_ref| stops */
+  int prune_components;            /* prune_components_ (1): component-wise pruning when labels are given */
+  double thresh_component_weight;  /* thresh_component_weight_ (1e-3) */
+  int diffusivity_inversion;       /* diffusivity_inversion_ (1): kappa is an unknown of the TIL solves */
+  int reaction_inversion;          /* reaction_inversion_ (0): final kappa/rho solve */
+  double beta;                     /* beta_p of the TIL solves */
+  double k_lb, k_ub, rho_lb, rho_ub; /* OptimizerSettings bounds (1e-3, 1, 1, 15) */
+  glia_opt_settings til, rd;       /* settings of the TIL and kappa/rho solves */
+} glia_sparse_til_settings;
+
+typedef struct glia_sparse_til_result {
+  int iterations;           /* CoSaMP iterations */
+  int stop;                 /* 1: max_it, 2: |J| < 1e-5, 3: relative objective change */
+  int nnz;                  /* size of the final support */
+  double J;                 /* mismatch (beta = 0) after the last CoSaMP iteration */
+  glia_opt_result til, rd;  /* the final TIL solve and the kappa/rho solve */
+} glia_sparse_til_result;
+
+void glia_sparse_til_settings_init(glia_sparse_til_settings* s);
+
+/* SparseTILOptimizer::solve (src/opt/SparseTILOptimizer.cpp:220-595) for nprob problems in
+   lockstep.  Per problem: the support starts as the nonzero entries of the initial p (the
+   multilevel warm start of src/Solver.cpp:285-305); each CoSaMP iteration thresholds |g_p| of the
+   beta = 0 gradient to 2s entries, merges them into the support, runs the TIL solve restricted to
+   the support (p >= 0 on it, 0 off it; kappa in [k_lb, k_ub] when inverted), prunes p to s entries
+   and stops on max_it, |J| < 1e-5 or the relative change.  A final TIL solve follows; with
+   reaction_inversion, p is rescaled so that max Phi p = 1, rho starts at the best of
+   {0, 3, 6, 9, 10, 12, 15} and kappa/rho are inverted with c0 fixed.  The restricted solves keep
+   the full basis with the off-support coefficients fixed at 0, which is the same problem.
+   x (nprob*(np+2)) holds the initial guess and receives the result; support (nprob*np, optional)
+   receives each final support, padded with -1; labels (nprob*np) and weights (nprob*ncomp) are
+   optional. */
+int glia_sparse_til(int nprob, int np, double* x, int* support, const int* labels, const double* weights, int ncomp,
+                    glia_eval_fn eval, glia_c0_fn set_c0, void* ctx, const glia_sparse_til_settings* s,
+                    glia_sparse_til_result* r);
+/* glia_sparse_til on the members of a handle (all with the same number of bases).  The TIL
+   stages use rho = x[np + 1] of each member; the handle's beta is restored at the end. */
+int glia_rd_sparse_til(glia_rd_t h, double* x, int* support, const int* labels, const double* weights, int ncomp,
+                       const glia_sparse_til_settings* s, glia_sparse_til_result* r);
 
 #ifdef __cplusplus
 }

@@ -102,6 +102,8 @@ int glia_rd_destroy(glia_rd_t h);
 const char* glia_rd_last_error(glia_rd_t h);
 int glia_rd_sync(glia_rd_t h);
 int glia_rd_num_members(glia_rd_t h);
+/* the grid of the handle (Parameters::grid_->n_): n[3] */
+int glia_rd_get_grid(glia_rd_t h, int* n);
 
 /* MatProp::setValuesCustom + SolverInterface::updateTumorCoefficients
    (src/mat/MatProp.cpp:134-203; src/SolverInterface.cpp:986-1014); null fields are zero */
@@ -118,6 +120,11 @@ int glia_rd_prec_factor(glia_rd_t h);
 /* DiffusionSolver Ctx::dt_ (src/pde/DiffusionSolver.cpp:11,226) */
 int glia_rd_set_ctx_dt(glia_rd_t h, double dt);
 int glia_rd_get_ctx_dt(glia_rd_t h, double* dt);
+/* OptimizerSettings::beta_ (L2 regularization of c0), which the reference's optimizers switch
+   to 0 for gradient thresholding and reaction/diffusion solves (src/opt/SparseTILOptimizer.cpp:328,
+   src/opt/RDOptimizer.cpp:170); beta >= 0 */
+int glia_rd_set_beta(glia_rd_t h, double beta);
+int glia_rd_get_beta(glia_rd_t h, double* beta);
 /* SpectralOperators::weierstrassSmoother (src/grad/SpectralOperators.cpp:264-381) of a stored real
    field of one member, in place, with standard deviation sigma (the reference smooths atlas
    tissue with sigma = smoothing_factor * 2 pi / n0, SolverInterface::readAtlas,

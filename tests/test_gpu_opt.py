@@ -52,3 +52,34 @@ def test_kappa_rho_inversion_on_gpu_recovers_truth(oracle_mod):
     print("oracle", xo[0][-2:], ro, flush=True)
     assert abs(xs[0][-2] - xo[0][-2]) < 1e-4 and abs(xs[0][-1] - xo[0][-1]) < 2e-3
     assert abs(res[0]["iterations"] - ro[0]["iterations"]) <= 3
+
+
+def test_sparse_til_on_gpu_matches_oracle_driven_run(oracle_mod):
+    """glia_rd_sparse_til (SparseTILOptimizer with the final kappa/rho solve) on a two-member
+    handle: both members find the truth's support and coefficients, and member 0 agrees with the
+    same driver run on the FP64 oracle."""
+    from glia_amd import opt, rd
+    from opt_cases import SPARSE_RD, SPARSE_STARTS, oracle_ops, rd_sparse_problem
+    print("case sparse TIL 32^3 nt 4, 2 members", flush=True)
+    pr, o, p, d = rd_sparse_problem(oracle_mod)
+    s = rd.RDSolver(32, nmembers=2, dt=0.05, nt=4, k_gm_wm=0.2, r_gm_wm=0.2)
+    for m in range(2):
+        s.set_tissue(m, pr["wm"], pr["gm"], pr["csf"], pr["vt"])
+        s.set_coefficients(m, 0.05, 8.0)
+        s.set_basis(m, pr["centers"], pr["sigma"])
+        s.set_data(m, d, None, -0.99)
+    starts = [np.concatenate([np.zeros(12), st]) for st in SPARSE_STARTS]
+    xs, sups, res = opt.rd_sparse_til(s, starts, **SPARSE_RD)
+    s.close()
+    print("gpu", [np.round(x, 5) for x in xs], sups, res, flush=True)
+    want = sorted(np.flatnonzero(p).tolist())
+    for x, sup in zip(xs, sups):
+        assert sup == want
+        assert np.allclose(x[:12], p, atol=3e-3)
+        assert abs(x[12] - 0.05) < 5e-4 and abs(x[13] - 8.0) < 0.05, x[12:]
+    ev, c0 = oracle_ops(o)
+    xo, supo, reso = opt.sparse_til(ev, c0, starts[0], 12, **SPARSE_RD)
+    print("oracle", np.round(xo[0], 5), supo, reso, flush=True)
+    assert supo[0] == sups[0] and reso[0]["iterations"] == res[0]["iterations"]
+    assert np.allclose(xs[0][:12], xo[0][:12], atol=1e-4)
+    assert abs(xs[0][12] - xo[0][12]) < 1e-4 and abs(xs[0][13] - xo[0][13]) < 5e-3

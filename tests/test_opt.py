@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from glia_amd import opt
+from opt_cases import SPARSE_RD, SPARSE_STARTS, oracle_ops, rd_sparse_problem
 
 A = np.array([1.0, 10.0, 100.0])
 
@@ -129,3 +130,124 @@ def test_oracle_kappa_rho_inversion_recovers_truth(oracle_mod):
     assert abs(x[0][-2] - 0.05) < 1e-4 and abs(x[0][-1] - 8.0) < 1e-3, x[0][-2:]
     assert np.array_equal(x[0][:-2], pr["p"])  # p is not an unknown of this mode
     assert J[-1] < 1e-8 * J[0]
+
+
+# ------------------------------------------------------------------ hard thresholds ----
+def test_hard_threshold_signed_order_ties_and_zero_stop():
+    # src/utils/Utils.cpp:339-367: a max-heap of (value, index): signed values, the larger index
+    # first on ties, and the scan ends at the first zero (so negative entries are never reached)
+    x = [0.1, 0.5, 0.5, -2.0, 0.0, 0.3]
+    assert opt.hard_threshold(x, 3) == [2, 1, 5]
+    assert opt.hard_threshold(x, 10) == [2, 1, 5, 0]
+    assert opt.hard_threshold([0.0, 0.0], 2) == []
+
+
+def test_hard_threshold_matches_brute_force_on_positive_vectors():
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        x = np.round(rng.uniform(0, 1, 30), 1)  # rounding makes ties
+        x[rng.integers(0, 30, 5)] = 0.0
+        s = int(rng.integers(1, 12))
+        order = sorted(range(30), key=lambda i: (x[i], i), reverse=True)
+        want = [i for i in order if x[i] > 0][:s]
+        assert opt.hard_threshold(x, s) == want
+
+
+def test_hard_threshold_components_budgets():
+    # src/utils/Utils.cpp:370-463: weight >= thresh -> 3 + floor(w (s - 3 ncc - ncc_t)),
+    # thresh/10 <= w < thresh -> 1, the last component takes the rest
+    x = [0.9, 0.5, 0.4, 0.2, 0.8, 0.3, 0.7]
+    lab = [1, 1, 1, 2, 2, 2, 2]
+    # s = 5, ncc = 2: component 1 gets 3 + floor(0.6 * (5 - 6)) = 2, component 2 the other 3
+    assert opt.hard_threshold(x, 5, lab, [0.6, 0.4]) == [0, 1, 4, 6, 5]
+    # s = 8, weights (0.5, 5e-4, 0.4995): ncc = 2, ncc_t = 1 -> budgets 3 + floor(0.5 * 1) = 3, 1, 4
+    x = np.arange(1, 13, dtype=float)
+    lab = [1] * 4 + [2] * 4 + [3] * 4
+    got = opt.hard_threshold(x, 8, lab, [0.5, 5e-4, 0.4995])
+    assert got == [3, 2, 1, 7, 11, 10, 9, 8]
+    # double mode (the CoSaMP gradient step): s = 10 is split as s = 5 and every share doubled:
+    # weights (0.7, 0.2, 0.1): 3 + floor(0.7 * (5 - 9)) = 0, 3 + floor(0.2 * (-4)) = 2 -> 4, rest 6
+    got = opt.hard_threshold(x, 10, lab, [0.7, 0.2, 0.1], double_mode=True)
+    assert got == [7, 6, 5, 4, 11, 10, 9, 8]
+
+
+# ------------------------------------------------------------------------ sparse TIL ----
+def linear_model(A, b, kappa_star=0.3, rho_star=7.0):
+    """A stand-in model with the shape of the RD operations: J = 1/2 |A p - b|^2 + 1/2 (kappa -
+    kappa*)^2 (+ 1/2 (rho - rho*)^2 in the kappa/rho mode), c0 = A p."""
+    np_ = A.shape[1]
+
+    def ev(mode, beta, X, want):
+        f, G = np.zeros(len(X)), np.zeros_like(X)
+        for m, x in enumerate(X):
+            p, k, r = x[:np_], x[np_], x[np_ + 1]
+            res = A @ p - b[m]
+            f[m] = 0.5 * res @ res + 0.5 * (k - kappa_star) ** 2 + 0.5 * beta * p @ p
+            G[m, np_] = k - kappa_star
+            if mode != 1:
+                G[m, :np_] = A.T @ res + beta * p
+            if mode != 0:
+                f[m] += 0.5 * (r - rho_star) ** 2
+                G[m, np_ + 1] = r - rho_star
+        return f, (G if want else None)
+
+    def c0(P):
+        return np.array([(A @ p).max() for p in P])
+    return ev, c0
+
+
+def test_cosamp_recovers_sparse_vector_from_compressed_measurements():
+    """CoSaMP's defining property: a 3-sparse nonnegative p* in R^40 is recovered from 20 random
+    Gaussian measurements, with its exact support."""
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((20, 40)) / np.sqrt(20)
+    supports, ps = [[3, 17, 31], [5, 6, 38]], []
+    for s in supports:
+        p = np.zeros(40)
+        p[s] = rng.uniform(0.5, 1.5, 3)
+        ps.append(p)
+    b = np.array([A @ p for p in ps])
+    ev, c0 = linear_model(A, b)
+    x0 = np.tile(np.concatenate([np.zeros(40), [0.5, 7.0]]), (2, 1))
+    kw = dict(sparsity=3, max_it=10, til=dict(max_it=200, gatol=1e-12, gttol=1e-12))
+    x, sup, res = opt.sparse_til(ev, c0, x0, 40, **kw)
+    for m in range(2):
+        assert sup[m] == supports[m], (sup, res)
+        assert np.allclose(x[m][:40], ps[m], atol=1e-6)
+        assert abs(x[m][40] - 0.3) < 1e-6
+    # batched problems in lockstep give what separate runs give
+    for m in range(2):
+        ev1, c01 = linear_model(A, b[m:m + 1])
+        x1, sup1, res1 = opt.sparse_til(ev1, c01, x0[m:m + 1], 40, **kw)
+        assert sup1[0] == sup[m] and np.array_equal(x1[0], x[m]) and res1[0] == res[m]
+
+
+def test_cosamp_final_reaction_diffusion_stage_rescales_and_scans_rho():
+    """With reaction_inversion: p is rescaled so that max c0 = 1, rho starts from the best of
+    the scan {0, 3, 6, 9, 10, 12, 15}, then kappa/rho are solved within their bounds."""
+    rng = np.random.default_rng(12)
+    A = np.abs(rng.standard_normal((20, 40))) / np.sqrt(20)
+    p = np.zeros(40)
+    p[[2, 9]] = [1.0, 0.5]
+    b = (A @ p)[None]
+    ev, c0 = linear_model(A, b, rho_star=11.0)
+    x0 = np.concatenate([np.zeros(40), [0.5, 4.0]])[None]
+    x, sup, res = opt.sparse_til(ev, c0, x0, 40, sparsity=2, reaction_inversion=1,
+                                 til=dict(max_it=200, gatol=1e-12, gttol=1e-12),
+                                 rd=dict(max_it=50, gatol=1e-10, gttol=1e-12))
+    assert sup[0] == [2, 9]
+    assert (A @ x[0][:40]).max() == pytest.approx(1.0, rel=1e-12)
+    assert np.allclose(x[0][:40], p / (A @ p).max(), rtol=1e-6)
+    assert abs(x[0][41] - 11.0) < 1e-6 and abs(x[0][40] - 0.3) < 1e-6
+
+
+def test_oracle_sparse_til_recovers_support_and_coefficients(oracle_mod):
+    """SparseTILOptimizer on exact data: from p = 0, kappa = 0.1 the CoSaMP loop finds the two
+    active bases, and the final kappa/rho solve (rho from the scan) recovers (0.05, 8)."""
+    pr, o, p, _ = rd_sparse_problem(oracle_mod)
+    ev, c0 = oracle_ops(o)
+    x0 = np.concatenate([np.zeros(12), SPARSE_STARTS[0]])
+    x, sup, res = opt.sparse_til(ev, c0, x0, 12, **SPARSE_RD)
+    assert sup[0] == sorted(np.flatnonzero(p).tolist()), (sup, res)
+    assert np.allclose(x[0][:12], p, atol=3e-3)
+    assert abs(x[0][12] - 0.05) < 5e-4 and abs(x[0][13] - 8.0) < 0.05, x[0][12:]
