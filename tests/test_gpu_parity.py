@@ -357,3 +357,55 @@ def test_hessian_with_diffusivity_vs_oracle(rd, oracle_mod):
     assert abs(out[0][-2] - ho[-2]) <= 1e-4 * abs(ho[-2])
     assert out[0][-1] == 0.0
     assert np.array_equal(out[0], out[1])
+
+
+def test_gradient_non_cubic_grid_vs_oracle(rd, oracle_mod):
+    """J and dJ = [g_p | g_kappa | g_rho] on a 48 x 96 x 64 grid (every axis a different length
+    and FFT plan) against the FP64 oracle."""
+    from glia_amd import synth
+    n = (48, 96, 64)
+    print("case gradient 48x96x64", flush=True)
+    pr = synth.problem(n, 3, nbases=6, kappa=0.05, rho=8.0, sigma=2 * np.pi / 64)
+    pr.update(n=n, nt=6, dt=0.02, k_gm_wm=0.2, r_gm_wm=0.2)
+    o = setup_oracle(oracle_mod, pr)
+    o.set_c0(o.phi_apply(pr["p"] * 1.3))
+    o.solve_state(0)
+    d1 = (o.ct() > 0.4).astype(np.float64)
+    x = np.concatenate([pr["p"], [pr["kappa"], pr["rho"]]])
+    o.set_data(d1, None, -0.99)
+    o.set_opt(beta=1e-3)
+    o.set_time(pr["dt"], pr["nt"])
+    Jo, go = o.objective_and_gradient(x, mode=2)
+    s = setup_gpu(rd, pr, beta=1e-3)
+    s.set_data(0, d1, None, -0.99)
+    J, g = s.objective_and_gradient([x])
+    s.close()
+    print("J", J[0], Jo, "rel g", rel(g[0], go), flush=True)
+    assert abs(J[0] - Jo) / abs(Jo) < 1e-5
+    assert rel(g[0], go) < 1e-4
+    # g_rho nearly cancels here (|g_rho| ~ 3e-5 |g|): each coefficient entry is held to 1e-4 of
+    # max(|entry|, 1e-2 |g|)
+    for k in (-2, -1):
+        assert abs(g[0][k] - go[k]) / max(abs(go[k]), 1e-2 * np.linalg.norm(go)) < 1e-4
+
+
+def test_gradient_at_zero_vs_oracle(rd, oracle_mod):
+    """The gradient at x = 0 (p = 0 so c = 0, kappa = 0 so the diffusion solves return at once),
+    which the TIL solves take as their reference norm, in modes 0 and 2."""
+    pr, o = grad_problem(oracle_mod, nt=6)
+    print("case gradient at zero", flush=True)
+    o.set_data(pr["d1"], None, -0.99)
+    o.set_opt(beta=1e-3)
+    s = setup_gpu(rd, pr, beta=1e-3)
+    s.set_data(0, pr["d1"], None, -0.99)
+    x = np.zeros(len(pr["p"]) + 2)
+    for mode in (0, 2):
+        o.set_time(pr["dt"], pr["nt"])
+        o.set_rho(pr["rho"], pr["r_gm_wm"], 0.0)
+        s.set_coefficients(0, pr["kappa"], pr["rho"])
+        Jo, go = o.objective_and_gradient(x, mode=mode)
+        J, g = s.objective_and_gradient([x], mode=mode)
+        print("mode", mode, "J", J[0], Jo, "rel g", rel(g[0], go), flush=True)
+        assert abs(J[0] - Jo) / abs(Jo) < 1e-5
+        assert rel(g[0], go) < 1e-4
+    s.close()
