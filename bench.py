@@ -5,7 +5,8 @@ Workload (default): the 256^3 members of the batching configuration -- an ensemb
 64 Gaussian bases, nt = 50 steps of dt = 0.02, observed data thresholded from a seeded
 ground-truth forward run.  One step = one batched evaluation of J and dJ = [g_p | g_kappa | g_rho]
 for every member (FP32 bulk, FP64 reductions).  value = gradient evaluations per second of the
-whole ensemble.  Also reported: the same evaluation run one member at a time ("plain"), the
+whole ensemble.  Also reported: the same evaluation run one member at a time ("plain"), a plain
+FP32 GPU path built the way the reference's CUDA backend solves (timed sample, "refstyle_plain"), the
 relative errors against the FP64 oracle at 64^3 (where the oracle finishes in seconds), the
 achieved bandwidth of the dominant kernel against the MI355X HBM peak, and the oracle timed on
 the host cores on a bounded sample of the same workload.
@@ -146,6 +147,100 @@ def cpu_baseline(n, nt, dt, pr, c0):
             "sample": "oracle FP64, one forward Strang step (2 CN solves, %d CG iterations in the last) of one "
                       "%d^3 member, median of 3 after a warm-up: %.2f s on %d threads; gradient eval = %d forward "
                       "+ %d adjoint steps (adjoint step = 1.5 forward steps)" % (its, n, t_step, threads, nt, nt)}
+
+
+def refstyle_plain(n, nt, dt, k, rho, mask, c0, rtol=1e-6, dev="cuda"):
+    """The plain FP32 GPU path built the way the reference's CUDA backend runs a solve, timed on the
+    same GPU as a comparison leg (torch FFTs; not the engine): real-space vectors, PETSc-KSPCG
+    preconditioned CG with three host-synchronized reductions per iteration, a matvec of eight full
+    3D transforms (computeGradient: 1 R2C + 3 C2R; computeDivergence: 3 R2C + 1 C2R;
+    src/mat/DiffCoef.cpp:255-263, src/grad/SpectralOperators.cpp) and a preconditioner of two
+    (applyPC, src/pde/DiffusionSolver.cpp:182).  One forward Strang step of one member (CN half
+    step, logistic reaction, CN half step) is timed, median of 3 after a warm-up, and extrapolated
+    like the CPU baseline: evals/s = 1 / (nt * t_step * 2.5)."""
+    import torch
+    f32 = torch.float32
+    sync = torch.cuda.synchronize if dev == "cuda" else (lambda: None)
+    kd = torch.as_tensor(k, dtype=f32, device=dev)
+    rd_ = torch.as_tensor(rho, dtype=f32, device=dev)
+    shape = kd.shape
+
+    def wn(m, half=False):
+        w = torch.fft.rfftfreq(m, 1.0 / m) if half else torch.fft.fftfreq(m, 1.0 / m)
+        w = w.clone()
+        w[w.abs() == m // 2] = 0.0  # multiplyXWaveNumber zeroes the Nyquist mode
+        return w.to(device=dev, dtype=f32)
+    wx = wn(shape[0])[:, None, None]
+    wy = wn(shape[1])[None, :, None]
+    wz = wn(shape[2], True)[None, None, :]
+    w2 = wx * wx + wy * wy + wz * wz
+    # kxx_avg_ = sum(k) / sum(brain mask) (src/mat/DiffCoef.cpp:103-111); factor 1 / (1 + dt/4 kbar |w|^2)
+    kbar = float(np.sum(k, dtype=np.float64) / np.sum(mask, dtype=np.float64))
+    its = [0]
+
+    def grad(x):
+        X = torch.fft.rfftn(x)
+        return [torch.fft.irfftn(1j * w * X, s=shape) for w in (wx, wy, wz)]
+
+    def div(a, b, c):
+        return torch.fft.irfftn(1j * wx * torch.fft.rfftn(a) + 1j * wy * torch.fft.rfftn(b) + 1j * wz * torch.fft.rfftn(c),
+                                s=shape)
+
+    def applyD(x):
+        gx, gy, gz = grad(x)
+        return div(kd * gx, kd * gy, kd * gz)
+
+    def cn(c, tau):
+        A = lambda v: v - 0.5 * tau * applyD(v)
+        P = 1.0 / (1.0 + 0.5 * tau * kbar * w2)
+        B = lambda v: torch.fft.irfftn(torch.fft.rfftn(v) * P, s=shape)
+        b = c + 0.5 * tau * applyD(c)
+        x = c.clone()
+        r = b - A(x)
+        z = B(r)
+        dp = float(torch.linalg.vector_norm(z))
+        # KSPConvergedDefault with a nonzero initial guess: ttol = rtol ||B b||
+        rnorm0 = float(torch.linalg.vector_norm(B(b)))
+        beta = float(torch.sum(z * r))
+        p = z.clone()
+        for _ in range(1000):
+            if dp <= rtol * rnorm0:
+                break
+            w = A(p)
+            alpha = beta / float(torch.sum(p * w))
+            x.add_(p, alpha=alpha)
+            r.sub_(w, alpha=alpha)
+            z = B(r)
+            dp = float(torch.linalg.vector_norm(z))
+            its[0] += 1
+            if dp <= rtol * rnorm0:
+                break
+            betaold, beta = beta, float(torch.sum(z * r))
+            p = z + (beta / betaold) * p
+        return x
+
+    def step(c):
+        c = cn(c, 0.5 * dt)
+        e = torch.exp(rd_ * dt)
+        c = c * e / (c * e + 1.0 - c)
+        return cn(c, 0.5 * dt)
+
+    c0d = torch.as_tensor(c0, dtype=f32, device=dev)
+    times = []
+    for rep in range(4):
+        its[0] = 0
+        sync()
+        t0 = time.perf_counter()
+        step(c0d.clone())
+        sync()
+        if rep > 0:
+            times.append(time.perf_counter() - t0)
+    t_step = float(np.median(times))
+    return {"value": 1.0 / (nt * t_step * 2.5), "unit": "grad evals/s",
+            "sample": "reference-structured plain FP32 GPU path (torch FFTs, real-space PETSc-style CG, 10 full 3D "
+                      "FFTs and 3 host-synchronized reductions per CG iteration), one forward Strang step of one "
+                      "%d^3 member (%d CG iterations), median of 3: %.1f ms; gradient eval = %d forward + %d adjoint "
+                      "steps (adjoint step = 1.5 forward steps)" % (shape[0], its[0], t_step * 1e3, nt, nt)}
 
 
 def batch_config(args):
@@ -364,6 +459,7 @@ def main():
     b_eval = field_bytes * (nt * (230 + 144 * mbar) + 20 * (nt + 1))
     model_gbs = b_eval * M * args.steps / (t2 - t1) / 1e9
     c0_member0 = s.get_field(0, "c0").astype(np.float64)
+    k0, rho0, mask0 = s.get_field(0, "k"), s.get_field(0, "rho"), s.get_field(0, "filter")
     del s
 
     plain = None
@@ -384,6 +480,10 @@ def main():
         plain = M / tsum
         log("plain (one member at a time): %.3f grad evals/s" % plain)
 
+    ref = None
+    if not args.no_plain:
+        ref = refstyle_plain(n, nt, dt, k0, rho0, mask0, c0_member0.astype(np.float32))
+        log("reference-structured plain path:", ref)
     acc = None if args.no_accuracy else accuracy_64()
     if acc:
         log("accuracy vs FP64 oracle:", acc)
@@ -415,6 +515,8 @@ def main():
         "cpu_baseline": cpu,
         "plain_value": None if plain is None else round(plain, 4),
         "batched_vs_plain": None if plain is None else round(value / plain, 3),
+        "refstyle_plain": ref,
+        "batched_vs_refstyle_plain": None if ref is None else round(value / ref["value"], 2),
         "cg_iterations_per_solve": round(mbar, 3),
         "model_hbm_GBs": round(model_gbs, 1),
         "kernels": kernels,
