@@ -68,6 +68,8 @@ SIGNATURES = {
     "glia_rd_set_tissue": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i]),
     "glia_rd_set_tissue_sinusoidal": (_i, [_vp, _i]),
     "glia_rd_set_coefficients": (_i, [_vp, _i, _d, _d]),
+    "glia_rd_set_kappa": (_i, [_vp, _i, _d]),
+    "glia_rd_set_rho": (_i, [_vp, _i, _d]),
     "glia_rd_set_k_sinusoidal": (_i, [_vp, _i, _d]),
     "glia_rd_prec_factor": (_i, [_vp]),
     "glia_rd_set_ctx_dt": (_i, [_vp, _d]),
@@ -75,6 +77,7 @@ SIGNATURES = {
     "glia_rd_set_beta": (_i, [_vp, _d]),
     "glia_rd_get_grid": (_i, [_vp, _vp]),
     "glia_rd_get_beta": (_i, [_vp, _pd]),
+    "glia_rd_set_ratios": (_i, [_vp, _d, _d, _d, _d]),
     "glia_rd_set_basis": (_i, [_vp, _i, _i, _vp, _d]),
     "glia_rd_basis_size": (_i, [_vp, _i]),
     "glia_rd_set_data": (_i, [_vp, _i, _vp, _vp, _d, _i]),

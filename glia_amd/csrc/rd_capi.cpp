@@ -160,6 +160,18 @@ int glia_rd_set_coefficients(glia_rd_t h, int member, double kappa, double rho) 
   return ret(h, h->e->set_coefficients(member, kappa, rho));
 }
 
+int glia_rd_set_kappa(glia_rd_t h, int member, double kappa) {
+  GLIA_H(h);
+  if (member < 0 || member >= h->e->nmembers()) return GLIA_RD_ERR_ARG;
+  return ret(h, h->e->set_kappa(member, kappa));
+}
+
+int glia_rd_set_rho(glia_rd_t h, int member, double rho) {
+  GLIA_H(h);
+  if (member < 0 || member >= h->e->nmembers()) return GLIA_RD_ERR_ARG;
+  return ret(h, h->e->set_rho(member, rho));
+}
+
 int glia_rd_set_k_sinusoidal(glia_rd_t h, int member, double scale) {
   GLIA_H(h);
   if (member < 0 || member >= h->e->nmembers()) return GLIA_RD_ERR_ARG;
@@ -188,6 +200,12 @@ int glia_rd_set_beta(glia_rd_t h, double beta) {
   GLIA_H(h);
   if (!(beta >= 0)) return GLIA_RD_ERR_ARG;
   h->e->set_beta(beta);
+  return 0;
+}
+
+int glia_rd_set_ratios(glia_rd_t h, double k_gm_wm, double k_glm_wm, double r_gm_wm, double r_glm_wm) {
+  GLIA_H(h);
+  h->e->set_ratios(k_gm_wm, k_glm_wm, r_gm_wm, r_glm_wm);
   return 0;
 }
 

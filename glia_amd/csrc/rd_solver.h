@@ -62,6 +62,12 @@ class RDEnsemble {
   int prec_factor();                                         // DiffusionSolver::precFactor
   void set_ctx_dt(double dt) { ctx_dt_ = dt; }
   void set_beta(double beta) { cfg_.beta = beta; }
+  void set_ratios(double k_gm, double k_glm, double r_gm, double r_glm) {
+    cfg_.k_gm_wm = k_gm;
+    cfg_.k_glm_wm = k_glm;
+    cfg_.r_gm_wm = r_gm;
+    cfg_.r_glm_wm = r_glm;
+  }
   double ctx_dt() const { return ctx_dt_; }
   int set_basis(int m, int np, const double* centers, double sigma);
   int set_data(int m, const float* d1, const float* obs_mask, double thr, bool device);

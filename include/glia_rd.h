@@ -113,6 +113,10 @@ int glia_rd_set_tissue(glia_rd_t h, int member, const float* wm, const float* gm
 int glia_rd_set_tissue_sinusoidal(glia_rd_t h, int member);
 /* DiffCoef::setValues + ReacCoef::setValues (src/mat/DiffCoef.cpp:77-132; src/mat/ReacCoef.cpp:13-38) */
 int glia_rd_set_coefficients(glia_rd_t h, int member, double kappa, double rho);
+/* DiffCoef::setValues alone (src/mat/DiffCoef.cpp:77-132) and ReacCoef::setValues alone
+   (src/mat/ReacCoef.cpp:13-38) */
+int glia_rd_set_kappa(glia_rd_t h, int member, double kappa);
+int glia_rd_set_rho(glia_rd_t h, int member, double rho);
 /* DiffCoef::setValuesSinusoidal (src/mat/DiffCoef.cpp:134-189) */
 int glia_rd_set_k_sinusoidal(glia_rd_t h, int member, double scale);
 /* DiffusionSolver::precFactor (src/pde/DiffusionSolver.cpp:119-180), all members */
@@ -125,6 +129,10 @@ int glia_rd_get_ctx_dt(glia_rd_t h, double* dt);
    src/opt/RDOptimizer.cpp:170); beta >= 0 */
 int glia_rd_set_beta(glia_rd_t h, double beta);
 int glia_rd_get_beta(glia_rd_t h, double* beta);
+/* the tissue ratios that DiffCoef::setValues / ReacCoef::setValues take with each call
+   (src/mat/DiffCoef.cpp:77-92, src/mat/ReacCoef.cpp:13-38); they apply from the next
+   glia_rd_set_coefficients or objective evaluation on */
+int glia_rd_set_ratios(glia_rd_t h, double k_gm_wm, double k_glm_wm, double r_gm_wm, double r_glm_wm);
 /* SpectralOperators::weierstrassSmoother (src/grad/SpectralOperators.cpp:264-381) of a stored real
    field of one member, in place, with standard deviation sigma (the reference smooths atlas
    tissue with sigma = smoothing_factor * 2 pi / n0, SolverInterface::readAtlas,

@@ -1,0 +1,365 @@
+// The reference's own unit tests of the reaction-diffusion path, run through the C++ interface in
+// include/glia/ (the reference's class names over libglia_rd.so), plus checks of that interface
+// against the C ABI and against finite differences.
+//
+//   "Running diffusion solver"       src/test/pdesolver.cpp:7-57 (||c||_2 = 2.0487, 5 CG iterations)
+//   "Evaluating objective function"  src/test/grad.cpp:7-80 (J = 0 at the truth, J > 0 otherwise)
+//
+// Exit status: the number of failed checks.  Needs a GPU.
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "glia/glia.h"
+
+using namespace glia;
+
+static int failures = 0, checks = 0;
+#define REQUIRE(cond)                                                        \
+  do {                                                                       \
+    checks++;                                                                \
+    if (!(cond)) {                                                           \
+      std::printf("FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);         \
+      failures++;                                                            \
+    }                                                                        \
+  } while (0)
+
+// Catch2's Approx with its default epsilon (100 FLT_EPSILON, relative to the larger magnitude)
+struct Approx {
+  double v;
+  explicit Approx(double x) : v(x) {}
+};
+static bool operator==(double a, Approx b) {
+  return std::fabs(a - b.v) <= 100 * FLT_EPSILON * std::max(std::fabs(a), std::fabs(b.v));
+}
+
+// helper.cpp: createPVec, createTestFunction (a Gaussian of radius sqrt(2) 2 pi / 64 at the centre)
+static PetscErrorCode createPVec(Vec& x, std::shared_ptr<Parameters> params) {
+  const int n_inv = 1 + params->get_nk() + params->get_nr();
+  PetscErrorCode ierr = VecCreateSeq(PETSC_COMM_SELF, n_inv, &x);
+  if (ierr) return ierr;
+  return VecSet(x, 0);
+}
+static PetscErrorCode createTestFunction(Vec x, std::shared_ptr<Parameters> params) {
+  ScalarType* x_ptr;
+  const ScalarType R = std::sqrt(2.) * (2 * M_PI) / 64;
+  PetscErrorCode ierr = VecGetArray(x, &x_ptr);
+  if (ierr) return ierr;
+  const Grid& g = *params->grid_;
+  for (int X = 0; X < g.isize_[0]; X++)
+    for (int Y = 0; Y < g.isize_[1]; Y++)
+      for (int Z = 0; Z < g.isize_[2]; Z++) {
+        const int64_t ptr = ((int64_t)X * g.isize_[1] + Y) * g.isize_[2] + Z;
+        const double dx = g.h_[0] * X - M_PI, dy = g.h_[1] * Y - M_PI, dz = g.h_[2] * Z - M_PI;
+        const double ratio = std::sqrt(dx * dx + dy * dy + dz * dz) / R;
+        x_ptr[ptr] = std::exp(-ratio * ratio);
+      }
+  return VecRestoreArray(x, &x_ptr);
+}
+
+// ---- src/test/pdesolver.cpp:7-57 -------------------------------------------------------------
+static void test_diffusion_solver() {
+  std::printf("TEST Running diffusion solver\n");
+  auto params = std::make_shared<Parameters>();
+  auto spec_ops = std::make_shared<SpectralOperators>(CUFFT);
+  initializeGrid(64, params);
+  auto k = std::make_shared<DiffCoef>(params, spec_ops);
+  Vec* work = new Vec[8];
+  for (int i = 0; i < 8; i++) {
+    VecDuplicate(k->kxx_, &work[i]);
+    VecSet(work[i], 0);
+  }
+  k->setWorkVecs(work);
+  REQUIRE(k->setValuesSinusoidal(params, 1E-2) == 0);
+  auto diff_solver = std::make_shared<DiffusionSolver>(params, spec_ops, k);
+  Vec c;
+  VecDuplicate(work[0], &c);
+  VecSet(c, 0);
+  createTestFunction(c, params);
+  REQUIRE(diff_solver->precFactor() == 0);
+  for (int i = 0; i < 10; i++) REQUIRE(diff_solver->solve(c, 0.02) == 0);
+  ScalarType nrm;
+  VecNorm(c, NORM_2, &nrm);
+  std::printf("  ||c||_2 = %.6f, last solve %d CG iterations\n", nrm, diff_solver->ksp_itr_);
+  REQUIRE(nrm == Approx(2.0487));
+  REQUIRE(diff_solver->ksp_itr_ == 5);
+  for (int i = 0; i < 8; i++) VecDestroy(&work[i]);
+  delete[] work;
+  VecDestroy(&c);
+}
+
+// ---- src/test/grad.cpp:7-80 ------------------------------------------------------------------
+static void test_objective() {
+  std::printf("TEST Evaluating objective function\n");
+  auto params = std::make_shared<Parameters>();
+  params->tu_->dt_ = 0.01;
+  params->tu_->nt_ = 100;
+  params->tu_->k_ = 0.01;
+  params->tu_->rho_ = 10;
+  params->tu_->model_ = 1;
+  params->opt_->flag_reaction_inv_ = true;  // disables phi apply in obj
+  params->opt_->reaction_inversion_ = true;
+  params->opt_->diffusivity_inversion_ = true;
+  params->opt_->beta_ = 0;
+  auto spec_ops = std::make_shared<SpectralOperators>(CUFFT);
+  initializeGrid(64, params);
+
+  Vec p;
+  createPVec(p, params);
+  auto tumor = std::make_shared<Tumor>(params, spec_ops);
+  REQUIRE(tumor->initialize(p, params, spec_ops, nullptr, nullptr) == 0);
+  REQUIRE(tumor->setSinusoidalCoefficients(params) == 0);
+  std::shared_ptr<PdeOperators> pde_operators = std::make_shared<PdeOperatorsRD>(tumor, params, spec_ops);
+  std::shared_ptr<DerivativeOperators> derivative_operators =
+      std::make_shared<DerivativeOperatorsRD>(pde_operators, params, tumor);
+
+  Vec x;
+  VecDuplicate(tumor->c_t_, &x);
+  VecSet(x, 0);
+  createTestFunction(x, params);
+
+  // test update ~ will change all coefficients to zero
+  ScalarType normk, normr;
+  ScalarType rt, kt;
+  rt = params->tu_->k_;
+  kt = params->tu_->k_;
+  REQUIRE(derivative_operators->updateReactionAndDiffusion(p) == 0);
+  VecNorm(tumor->k_->kxx_, NORM_2, &normk);
+  VecNorm(tumor->rho_->rho_vec_, NORM_2, &normr);
+  REQUIRE(normk == 0);
+  REQUIRE(normr == 0);
+
+  // set correct values to p
+  ScalarType* p_ptr;
+  VecGetArray(p, &p_ptr);
+  p_ptr[1] = kt;
+  p_ptr[params->tu_->nk_ + 1] = rt;
+  VecRestoreArray(p, &p_ptr);
+
+  // generate test data
+  VecCopy(x, tumor->c_0_);
+  REQUIRE(derivative_operators->updateReactionAndDiffusion(p) == 0);
+  REQUIRE(pde_operators->solveState(0) == 0);
+  VecCopy(tumor->c_t_, x);
+  auto data = std::make_shared<Data>();
+  data->set(x, nullptr);
+
+  // compute objective with ground truth
+  ScalarType j;
+  REQUIRE(derivative_operators->evaluateObjective(&j, p, data) == 0);
+  std::printf("  J(truth) = %.3e\n", j);
+  REQUIRE(j < 1E-7);
+  VecSet(x, 0);
+  REQUIRE(derivative_operators->evaluateObjective(&j, p, data) == 0);
+  std::printf("  J(zero data) = %.3e\n", j);
+  REQUIRE(j > 0);
+  VecDestroy(&x);
+  VecDestroy(&p);
+}
+
+// ---- the interface against the C ABI, and g_p against central differences --------------------
+struct TilSetup {
+  std::shared_ptr<Parameters> params;
+  std::shared_ptr<Tumor> tumor;
+  std::shared_ptr<PdeOperators> pde;
+  std::shared_ptr<DerivativeOperators> der;
+  std::shared_ptr<Data> data;
+  Vec d = nullptr;
+  std::vector<double> centers;
+  double sigma = 0;
+};
+
+static const int NP = 4;
+
+static void make_til(TilSetup& s) {
+  s.params = std::make_shared<Parameters>();
+  auto& P = *s.params;
+  P.tu_->dt_ = 0.02;
+  P.tu_->nt_ = 5;
+  P.tu_->k_ = 0.05;
+  P.tu_->rho_ = 8;
+  P.tu_->k_gm_wm_ratio_ = 0.2;
+  P.tu_->r_gm_wm_ratio_ = 0.2;
+  P.opt_->beta_ = 1e-3;
+  auto spec_ops = std::make_shared<SpectralOperators>(CUFFT);
+  initializeGrid(64, s.params);
+  s.tumor = std::make_shared<Tumor>(s.params, spec_ops);
+  s.tumor->initialize(nullptr, s.params, spec_ops);
+  s.tumor->setSinusoidalCoefficients(s.params);
+  const double off[NP][3] = {{0, 0, 0}, {0.3, 0, 0}, {0, 0.3, 0.1}, {-0.2, -0.2, 0.3}};
+  for (int i = 0; i < NP; i++)
+    for (int d = 0; d < 3; d++) s.centers.push_back(M_PI + off[i][d]);
+  s.sigma = 2 * M_PI / 32;
+  s.tumor->phi_->setGaussians(s.centers, s.sigma);
+  s.pde = std::make_shared<PdeOperatorsRD>(s.tumor, s.params, spec_ops);
+  s.der = std::make_shared<DerivativeOperatorsRD>(s.pde, s.params, s.tumor);
+  // data: c(T) of p* = (0.8, 0.5, 0, 0.3), thresholded at 0.3
+  Vec pstar;
+  VecCreateSeq(PETSC_COMM_SELF, NP, &pstar);
+  double* pp;
+  VecGetArray(pstar, &pp);
+  pp[0] = 0.8, pp[1] = 0.5, pp[2] = 0.0, pp[3] = 0.3;
+  VecRestoreArray(pstar, &pp);
+  s.tumor->phi_->apply(s.tumor->c_0_, pstar);
+  s.pde->solveState(0);
+  VecDuplicate(s.tumor->c_t_, &s.d);
+  VecCopy(s.tumor->c_t_, s.d);
+  double* dp;
+  VecGetArray(s.d, &dp);
+  for (int64_t i = 0; i < s.d->n; i++) dp[i] = dp[i] > 0.3 ? 1.0 : 0.0;
+  VecRestoreArray(s.d, &dp);
+  s.data = std::make_shared<Data>();
+  s.data->set(s.d, nullptr);
+  VecDestroy(&pstar);
+}
+
+static void test_interface_matches_abi_and_fd() {
+  std::printf("TEST TIL objective and gradient through the interface\n");
+  TilSetup s;
+  make_til(s);
+  const int n = NP + 2;
+  Vec x, g;
+  VecCreateSeq(PETSC_COMM_SELF, n, &x);
+  VecCreateSeq(PETSC_COMM_SELF, n, &g);
+  const double x0[n] = {0.6, 0.6, 0.2, 0.4, 0.05, 8.0};
+  double* xp;
+  VecGetArray(x, &xp);
+  for (int i = 0; i < n; i++) xp[i] = x0[i];
+  VecRestoreArray(x, &xp);
+  // a fresh solver context for every evaluation compared (the preconditioner uses the context dt)
+  double J = 0;
+  REQUIRE(s.der->evaluateObjectiveAndGradient(&J, g, x, s.data) == 0);
+
+  // the same evaluation through the C ABI on its own handle
+  glia_rd_config c;
+  glia_rd_config_init(&c);
+  c.n[0] = c.n[1] = c.n[2] = 64;
+  c.dt = 0.02;
+  c.nt = 5;
+  c.k_gm_wm = 0.2;
+  c.r_gm_wm = 0.2;
+  c.beta = 1e-3;
+  glia_rd_t h = nullptr;
+  REQUIRE(glia_rd_create(&c, &h) == 0);
+  glia_rd_set_tissue_sinusoidal(h, 0);
+  glia_rd_set_coefficients(h, 0, 0.05, 8.0);
+  glia_rd_set_basis(h, 0, NP, s.centers.data(), s.sigma);
+  std::vector<float> d1(s.d->n);
+  double* dp;
+  VecGetArray(s.d, &dp);
+  for (int64_t i = 0; i < s.d->n; i++) d1[i] = (float)dp[i];
+  VecRestoreArray(s.d, &dp);
+  glia_rd_set_data(h, 0, d1.data(), nullptr, -INFINITY, GLIA_RD_HOST);
+  double Ja = 0, ga[n];
+  REQUIRE(glia_rd_objective_and_gradient(h, GLIA_RD_MODE_P_KAPPA, x0, &Ja, ga) == 0);
+  glia_rd_destroy(h);
+  double* gp;
+  VecGetArray(g, &gp);
+  double err = 0, nrm = 0;
+  for (int i = 0; i < n; i++) {
+    err += (gp[i] - ga[i]) * (gp[i] - ga[i]);
+    nrm += ga[i] * ga[i];
+  }
+  std::printf("  J %.9e (ABI %.9e), |dJ - dJ_ABI| / |dJ_ABI| = %.2e, g_rho = %g\n", J, Ja, std::sqrt(err / nrm), gp[n - 1]);
+  REQUIRE(std::fabs(J - Ja) <= 1e-6 * std::fabs(Ja));
+  REQUIRE(std::sqrt(err / nrm) < 1e-5);
+  REQUIRE(gp[n - 1] == 0.0);  // TIL: [g_p | g_kappa | 0]
+
+  // g_p against central differences of J (the reference's checkGradient procedure)
+  std::mt19937 rng(7);
+  std::normal_distribution<double> nd;
+  double gpd[NP];
+  for (int i = 0; i < NP; i++) gpd[i] = gp[i];
+  VecRestoreArray(g, &gp);
+  for (int t = 0; t < 3; t++) {
+    double dir[NP], gd = 0;
+    for (int i = 0; i < NP; i++) {
+      dir[i] = nd(rng);
+      gd += gpd[i] * dir[i];
+    }
+    const double hstep = 1e-2;
+    double Jp = 0, Jm = 0;
+    for (int sgn = -1; sgn <= 1; sgn += 2) {
+      VecGetArray(x, &xp);
+      for (int i = 0; i < NP; i++) xp[i] = x0[i] + sgn * hstep * dir[i];
+      VecRestoreArray(x, &xp);
+      REQUIRE(s.der->evaluateObjective(sgn < 0 ? &Jm : &Jp, x, s.data) == 0);
+    }
+    const double fd = (Jp - Jm) / (2 * hstep);
+    std::printf("  direction %d: FD %.6e, g.d %.6e, rel %.2e\n", t, fd, gd, std::fabs(fd - gd) / std::fabs(fd));
+    REQUIRE(std::fabs(fd - gd) <= 1e-3 * std::fabs(fd));
+  }
+  VecDestroy(&x);
+  VecDestroy(&g);
+  VecDestroy(&s.d);
+}
+
+// ---- Phi apply / transpose and the diffusion operator through the interface -------------------
+static void test_phi_and_apply_d() {
+  std::printf("TEST Phi and DiffCoef::applyD through the interface\n");
+  TilSetup s;
+  make_til(s);
+  const int64_t ng = s.params->grid_->ng_;
+  Vec p, c0, v, pt;
+  VecCreateSeq(PETSC_COMM_SELF, NP, &p);
+  VecCreateSeq(PETSC_COMM_SELF, NP, &pt);
+  VecCreateSeq(PETSC_COMM_SELF, (PetscInt)ng, &c0);
+  VecCreateSeq(PETSC_COMM_SELF, (PetscInt)ng, &v);
+  double* pp;
+  // one active basis: Phi p is that basis normalized by its own maximum (Phi::apply normalizes by
+  // the maximum over the active bases), so max c0 = 1
+  VecSet(p, 0.0);
+  VecGetArray(p, &pp);
+  pp[1] = 1.0;
+  VecRestoreArray(p, &pp);
+  REQUIRE(s.tumor->phi_->apply(c0, p) == 0);
+  double mx = 0;
+  VecMax(c0, nullptr, &mx);
+  REQUIRE(std::fabs(mx - 1.0) < 1e-6);
+  // every basis active: Phi and Phi^T normalize by the same global maximum, so
+  // <Phi^T v, p> = <v, Phi p> for any field v
+  VecGetArray(p, &pp);
+  pp[0] = 1.0, pp[1] = 0.5, pp[2] = 0.25, pp[3] = 2.0;
+  VecRestoreArray(p, &pp);
+  REQUIRE(s.tumor->phi_->apply(c0, p) == 0);
+  createTestFunction(v, s.params);
+  REQUIRE(s.tumor->phi_->applyTranspose(pt, v) == 0);
+  double lhs = 0, rhs = 0;
+  VecDot(pt, p, &lhs);
+  VecDot(v, c0, &rhs);
+  std::printf("  <Phi^T v, p> = %.8e, <v, Phi p> = %.8e\n", lhs, rhs);
+  REQUIRE(std::fabs(lhs - rhs) <= 1e-5 * std::fabs(rhs));
+  // applyD is linear: D(2 c0) = 2 D(c0)
+  Vec d1, d2;
+  VecCreateSeq(PETSC_COMM_SELF, (PetscInt)ng, &d1);
+  VecCreateSeq(PETSC_COMM_SELF, (PetscInt)ng, &d2);
+  REQUIRE(s.tumor->k_->applyD(d1, c0) == 0);
+  VecScale(c0, 2.0);
+  REQUIRE(s.tumor->k_->applyD(d2, c0) == 0);
+  VecAXPY(d2, -2.0, d1);
+  double e = 0, n1 = 0;
+  VecNorm(d2, NORM_2, &e);
+  VecNorm(d1, NORM_2, &n1);
+  std::printf("  ||D(2c) - 2D(c)|| / ||D(c)|| = %.2e\n", e / n1);
+  REQUIRE(n1 > 0 && e <= 1e-5 * n1);
+  VecDestroy(&p);
+  VecDestroy(&pt);
+  VecDestroy(&c0);
+  VecDestroy(&v);
+  VecDestroy(&d1);
+  VecDestroy(&d2);
+  VecDestroy(&s.d);
+}
+
+int main() {
+  test_diffusion_solver();
+  test_objective();
+  test_interface_matches_abi_and_fd();
+  test_phi_and_apply_d();
+  std::printf("%d checks, %d failed\n", checks, failures);
+  return failures;
+}

@@ -64,3 +64,13 @@ def test_create_rejects_bad_arguments_without_touching_the_gpu():
     assert lib.glia_rd_create(ctypes.byref(c), ctypes.byref(h)) == _lib.ERR_UNSUPPORTED
     assert lib.glia_rd_create(None, ctypes.byref(h)) == _lib.ERR_ARG
     assert lib.glia_rd_destroy(None) == _lib.ERR_ARG
+
+
+def test_cpp_interface_headers_compile():
+    """include/glia/ (the reference's class names over the C ABI) is self-contained C++17."""
+    src = '#include "glia/glia.h"\nint main() { return 0; }\n'
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                        "-I" + os.path.join(ROOT, "include"), "-x", "c++", "-"],
+                       input=src, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
