@@ -39,6 +39,10 @@
 #define GLIA_XF_MINW 2
 #endif
 // xf_div loads p before its transforms (1) or in its output stage (0)
+// xf_div loads p at kernel start up to this many values per thread (the 192-point plan has 24)
+#ifndef GLIA_XF_PF_VPT
+#define GLIA_XF_PF_VPT 24
+#endif
 #ifndef GLIA_XF_PF
 #define GLIA_XF_PF 1
 #endif
@@ -410,7 +414,7 @@ k_xf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
   const float pc = (float)st->pc;
   const float wy = wnum(tl.o, d.n1), wz = (float)(tl.c0 + col);
   const double wgt = (tl.c0 + col == 0) ? 1.0 : 2.0;
-  constexpr bool PF = K::PF && GLIA_XF_PF;
+  constexpr bool PF = K::VPT <= GLIA_XF_PF_VPT && GLIA_XF_PF;
   float2 a[K::VPT], b[K::VPT], pv[PF ? K::VPT : 1];
 #pragma unroll
   for (int j = 0; j < K::VPT; j++) {
@@ -488,10 +492,23 @@ k_col_plain(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict_
 // is the input layout of the forward plan (R2, R1), so pointwise work between the two needs no
 // exchange.  Thread t of a line holds z[n] for n = out_row(t, j).
 // ---------------------------------------------------------------------------------------------
+// threads per Z block; 96-point lines (24 values per thread, 149 VGPRs) take 16 lines per block:
+// with 64 lines only two blocks fit in LDS, with 16 the register file sets occupancy (3 waves/SIMD)
+#ifndef GLIA_Z_TPB96
+#define GLIA_Z_TPB96 64
+#endif
+template <int M>
+struct ZTpb {
+  static constexpr int value = GLIA_Z_TPB;
+};
+template <>
+struct ZTpb<96> {
+  static constexpr int value = GLIA_Z_TPB96;
+};
 template <int M>
 struct ZK {
   static constexpr int R1 = Plan2<M>::R1, R2 = Plan2<M>::R2, NT = Plan2<M>::NT;
-  static constexpr int NL = GLIA_Z_TPB / NT;
+  static constexpr int NL = ZTpb<M>::value / NT;
   static constexpr int TPB = NL * NT;
   static constexpr int LP = XPitch<M>::value;
   static constexpr int VPT = M / NT;
@@ -611,8 +628,12 @@ __device__ __forceinline__ void zl_forward_store(float2* v, float2* lb, float2* 
 #ifndef GLIA_ZD_MINW
 #define GLIA_ZD_MINW 3
 #endif
+// minimum blocks per CU of z_diff for plans with more than 16 values per thread
+#ifndef GLIA_ZD_MINW24
+#define GLIA_ZD_MINW24 2
+#endif
 template <int M>
-__global__ void __launch_bounds__(ZK<M>::TPB, ZK<M>::VPT <= 16 ? GLIA_ZD_MINW : 2)
+__global__ void __launch_bounds__(ZK<M>::TPB, ZK<M>::VPT <= 16 ? GLIA_ZD_MINW : GLIA_ZD_MINW24)
 k_z_diff(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ twm, const float2* __restrict__ tw2,
           int s_gx, int s_gy, int s_gz, int k_slot) {
   const MemberDev& Mb = mem[blockIdx.y];
