@@ -1,6 +1,6 @@
-// glia_opt.cpp -- bound-constrained limited-memory quasi-Newton minimization with an Armijo line
-// search (the BLMVM algorithm the reference runs through TAO) and its use on an RD handle
-// (include/glia_opt.h).
+// glia_opt.cpp -- bound-constrained limited-memory quasi-Newton minimization with a More-Thuente or
+// an Armijo line search (the BLMVM algorithm the reference runs through TAO, src/opt/Optimizer.cpp:
+// 116-260) and its use on an RD handle (include/glia_opt.h).
 //
 // P(g) is the projected gradient: components of g that would push x out through an active bound
 // (x_i at its lower bound with g_i > 0, at its upper bound with g_i < 0) are zero.
@@ -8,8 +8,12 @@
 //   d = -H g (two-loop L-BFGS, initial scaling s'y / y'y of the newest pair; without pairs
 //   H = delta I with delta = 2 |f| / ||g||^2, or 2 / ||g||^2 when f = 0, TAO's first-step scaling);
 //   components that point out of an active bound are zeroed; if g'd >= 0 the memory is dropped
-//   and d = -delta g.  Armijo: x(a) = P(x + a d) for a = 1, 1/2, 1/4, ... up to ls_max_funcs
-//   trials, accepted when f(x(a)) <= f + sigma g'(x(a) - x).  A failed search is retried once along
+//   and d = -delta g.  Trial points are x(a) = P(x + a d).  More-Thuente (the default): the
+//   strong Wolfe conditions phi(a) <= phi(0) + sigma a phi'(0), |phi'(a)| <= gtol |phi'(0)| on
+//   phi(a) = f(x(a)), with phi'(a) = g(x(a))'(x(a) - x)/a along the projected path, a capped where
+//   every moving variable sits on its bound, at most ls_max_funcs evaluations; a search that halts
+//   first takes its best sufficient-decrease point, if any.  Armijo: a = 1, 1/2, 1/4, ... up to
+//   ls_max_funcs trials, accepted when f(x(a)) <= f + sigma g'(x(a) - x).  A failed search is retried once along
 //   -delta g with an empty memory; a direction with g'd >= 0 even then (the projected gradient
 //   vanishes) ends the search as failed, as an Armijo search along an ascent direction does.  The
 //   pair (s, y) with y = P(g_new) - P(g_old) (BLMVM's update with projected gradients, so an active
@@ -33,6 +37,20 @@
 
 namespace {
 
+// More-Thuente search state (the interval [stx, sty] that brackets a step satisfying the strong
+// Wolfe conditions, with function values and derivatives of phi(a) = f(P(x + a d)) at its ends)
+struct MtState {
+  double stx = 0, fx = 0, gx = 0, sty = 0, fy = 0, gy = 0;
+  double stmin = 0, stmax = 0, stpmin = 0, stpmax = 0, width = 0, width1 = 0;
+  double finit = 0, ginit = 0, gtest = 0;
+  int stage = 1;
+  bool brackt = false;
+  // the point at stx (the best sufficient-decrease point so far), kept for a halted search
+  std::vector<double> xb, gb;
+  double fb = 0, ab = 0;
+  bool have_best = false;
+};
+
 struct Problem {
   int n = 0;
   double* x = nullptr;
@@ -43,6 +61,7 @@ struct Problem {
   int ls_k = 0, it = 0, fevals = 0, reason = GLIA_OPT_CONTINUE;
   bool started = false, reset_tried = false;
   std::deque<std::vector<double>> S, Y;
+  MtState mt;
 };
 
 double dot(const std::vector<double>& a, const std::vector<double>& b) {
@@ -93,7 +112,196 @@ void trial(Problem& p) {
   for (int i = 0; i < p.n; i++) p.xt[i] = std::min(std::max(p.x[i] + p.alpha * p.d[i], lo(p, i)), hi(p, i));
 }
 
-void new_search(Problem& p) {
+// ---- More-Thuente line search (J. J. More and D. J. Thuente, "Line search algorithms with
+// guaranteed sufficient decrease", ACM TOMS 20 (1994) 286-307; the scheme of MINPACK-2's
+// dcsrch/dcstep): the step update of the safeguarded interval
+void mt_step(double& stx, double& fx, double& dx, double& sty, double& fy, double& dy, double& stp, double fp, double dp,
+             bool& brackt, double stpmin, double stpmax) {
+  const double sgnd = dp * (dx / std::fabs(dx));
+  double stpf;
+  auto cubic_gamma = [](double theta, double a, double b, double s, bool clamp) {
+    double r = (theta / s) * (theta / s) - (a / s) * (b / s);
+    if (clamp) r = std::max(0.0, r);
+    return s * std::sqrt(r);
+  };
+  if (fp > fx) {
+    // higher function value: the minimizer is bracketed; cubic step, or halfway to the quadratic
+    const double theta = 3.0 * (fx - fp) / (stp - stx) + dx + dp;
+    const double s = std::max({std::fabs(theta), std::fabs(dx), std::fabs(dp)});
+    double gamma = cubic_gamma(theta, dx, dp, s, false);
+    if (stp < stx) gamma = -gamma;
+    const double pp = (gamma - dx) + theta, q = ((gamma - dx) + gamma) + dp;
+    const double stpc = stx + (pp / q) * (stp - stx);
+    const double stpq = stx + ((dx / ((fx - fp) / (stp - stx) + dx)) / 2.0) * (stp - stx);
+    stpf = (std::fabs(stpc - stx) < std::fabs(stpq - stx)) ? stpc : stpc + (stpq - stpc) / 2.0;
+    brackt = true;
+  } else if (sgnd < 0.0) {
+    // lower value, derivatives of opposite sign: bracketed; cubic or secant, the farther
+    const double theta = 3.0 * (fx - fp) / (stp - stx) + dx + dp;
+    const double s = std::max({std::fabs(theta), std::fabs(dx), std::fabs(dp)});
+    double gamma = cubic_gamma(theta, dx, dp, s, false);
+    if (stp > stx) gamma = -gamma;
+    const double pp = (gamma - dp) + theta, q = ((gamma - dp) + gamma) + dx;
+    const double stpc = stp + (pp / q) * (stx - stp);
+    const double stpq = stp + (dp / (dp - dx)) * (stx - stp);
+    stpf = (std::fabs(stpc - stp) > std::fabs(stpq - stp)) ? stpc : stpq;
+    brackt = true;
+  } else if (std::fabs(dp) < std::fabs(dx)) {
+    // lower value, same-sign derivatives, derivative magnitude decreasing
+    const double theta = 3.0 * (fx - fp) / (stp - stx) + dx + dp;
+    const double s = std::max({std::fabs(theta), std::fabs(dx), std::fabs(dp)});
+    double gamma = cubic_gamma(theta, dx, dp, s, true);
+    if (stp > stx) gamma = -gamma;
+    const double pp = (gamma - dp) + theta, q = (gamma + (dx - dp)) + gamma;
+    const double r = pp / q;
+    double stpc;
+    if (r < 0.0 && gamma != 0.0) stpc = stp + r * (stx - stp);
+    else if (stp > stx) stpc = stpmax;
+    else stpc = stpmin;
+    const double stpq = stp + (dp / (dp - dx)) * (stx - stp);
+    if (brackt) {
+      stpf = (std::fabs(stpc - stp) < std::fabs(stpq - stp)) ? stpc : stpq;
+      stpf = (stp > stx) ? std::min(stp + 0.66 * (sty - stp), stpf) : std::max(stp + 0.66 * (sty - stp), stpf);
+    } else {
+      stpf = (std::fabs(stpc - stp) > std::fabs(stpq - stp)) ? stpc : stpq;
+      stpf = std::max(stpmin, std::min(stpmax, stpf));
+    }
+  } else {
+    // lower value, same-sign derivatives, derivative magnitude not decreasing
+    if (brackt) {
+      const double theta = 3.0 * (fp - fy) / (sty - stp) + dy + dp;
+      const double s = std::max({std::fabs(theta), std::fabs(dy), std::fabs(dp)});
+      double gamma = cubic_gamma(theta, dy, dp, s, false);
+      if (stp > sty) gamma = -gamma;
+      const double pp = (gamma - dp) + theta, q = ((gamma - dp) + gamma) + dy;
+      stpf = stp + (pp / q) * (sty - stp);
+    } else {
+      stpf = (stp > stx) ? stpmax : stpmin;
+    }
+  }
+  // the new interval
+  if (fp > fx) {
+    sty = stp;
+    fy = fp;
+    dy = dp;
+  } else {
+    if (sgnd < 0.0) {
+      sty = stx;
+      fy = fx;
+      dy = dx;
+    }
+    stx = stp;
+    fx = fp;
+    dx = dp;
+  }
+  stp = stpf;
+}
+
+// largest step before every moving variable sits on its bound (the projected path is constant
+// beyond it)
+double max_feasible_step(const Problem& p) {
+  double rmax = 0;
+  for (int i = 0; i < p.n; i++) {
+    if (p.d[i] > 0) {
+      const double b = hi(p, i);
+      if (!std::isfinite(b)) return std::numeric_limits<double>::infinity();
+      rmax = std::max(rmax, (b - p.x[i]) / p.d[i]);
+    } else if (p.d[i] < 0) {
+      const double b = lo(p, i);
+      if (!std::isfinite(b)) return std::numeric_limits<double>::infinity();
+      rmax = std::max(rmax, (b - p.x[i]) / p.d[i]);
+    }
+  }
+  return rmax;
+}
+
+void mt_begin(Problem& p, const glia_opt_settings& s) {
+  MtState& m = p.mt;
+  m = MtState();
+  m.finit = p.f;
+  m.ginit = p.gtd;
+  m.gtest = s.ls_sigma * m.ginit;
+  m.stpmin = s.ls_minstep;
+  m.stpmax = std::min(max_feasible_step(p), 1e15);
+  m.width = m.stpmax - m.stpmin;
+  m.width1 = 2.0 * m.width;
+  m.fx = m.fy = m.finit;
+  m.gx = m.gy = m.ginit;
+  p.alpha = std::max(m.stpmin, std::min(1.0, m.stpmax));
+  m.stmin = 0;
+  m.stmax = p.alpha + 4.0 * p.alpha;
+}
+
+enum { LS_NEXT = 0, LS_ACCEPT = 1, LS_ACCEPT_BEST = 2, LS_FAIL = 3 };
+
+// one evaluation of the search at step p.alpha (point p.xt, value f, gradient g)
+int mt_advance(Problem& p, const glia_opt_settings& s, double f, const double* g) {
+  MtState& m = p.mt;
+  double stp = p.alpha;
+  if (!std::isfinite(f)) return m.have_best ? LS_ACCEPT_BEST : LS_FAIL;
+  // derivative of phi along the projected path
+  double dg = 0;
+  for (int i = 0; i < p.n; i++) dg += g[i] * (p.xt[i] - p.x[i]);
+  dg /= stp;
+  const double ftest = m.finit + stp * m.gtest;
+  if (m.stage == 1 && f <= ftest && dg >= 0) m.stage = 2;
+  if (f <= ftest && std::fabs(dg) <= s.ls_gtol * (-m.ginit)) return LS_ACCEPT;
+  // a point with sufficient decrease that lowers phi becomes the fallback of a halted search
+  const bool record = f <= ftest && (!m.have_best || f < m.fb);
+  if (record) {
+    m.xb = p.xt;
+    m.gb.assign(g, g + p.n);
+    m.fb = f;
+    m.ab = stp;
+    m.have_best = true;
+  }
+  const bool halted = (m.brackt && (stp <= m.stmin || stp >= m.stmax)) ||
+                      (m.brackt && m.stmax - m.stmin <= s.ls_xtol * m.stmax) ||
+                      (stp == m.stpmax && f <= ftest && dg <= m.gtest) ||
+                      (stp == m.stpmin && (f > ftest || dg >= m.gtest)) || (++p.ls_k >= s.ls_max_funcs);
+  if (halted) return m.have_best ? LS_ACCEPT_BEST : LS_FAIL;
+  if (m.stage == 1 && f <= m.fx && f > ftest) {
+    // the modified function psi(a) = phi(a) - a gtest until a sufficient-decrease point is found
+    double fm = f - stp * m.gtest, fxm = m.fx - m.stx * m.gtest, fym = m.fy - m.sty * m.gtest;
+    double gm = dg - m.gtest, gxm = m.gx - m.gtest, gym = m.gy - m.gtest;
+    mt_step(m.stx, fxm, gxm, m.sty, fym, gym, stp, fm, gm, m.brackt, m.stmin, m.stmax);
+    m.fx = fxm + m.stx * m.gtest;
+    m.fy = fym + m.sty * m.gtest;
+    m.gx = gxm + m.gtest;
+    m.gy = gym + m.gtest;
+  } else {
+    mt_step(m.stx, m.fx, m.gx, m.sty, m.fy, m.gy, stp, f, dg, m.brackt, m.stmin, m.stmax);
+  }
+  if (m.brackt) {
+    if (std::fabs(m.sty - m.stx) >= 0.66 * m.width1) stp = m.stx + 0.5 * (m.sty - m.stx);
+    m.width1 = m.width;
+    m.width = std::fabs(m.sty - m.stx);
+    m.stmin = std::min(m.stx, m.sty);
+    m.stmax = std::max(m.stx, m.sty);
+  } else {
+    m.stmin = stp + 1.1 * (stp - m.stx);
+    m.stmax = stp + 4.0 * (stp - m.stx);
+  }
+  stp = std::max(m.stpmin, std::min(m.stpmax, stp));
+  if ((m.brackt && (stp <= m.stmin || stp >= m.stmax)) || (m.brackt && m.stmax - m.stmin <= s.ls_xtol * m.stmax))
+    stp = m.stx;
+  if (!(stp > 0)) return m.have_best ? LS_ACCEPT_BEST : LS_FAIL;
+  p.alpha = stp;
+  trial(p);
+  return LS_NEXT;
+}
+
+void start_search(Problem& p, const glia_opt_settings& s) {
+  p.ls_k = 0;
+  if (s.line_search == GLIA_OPT_LS_MORE_THUENTE) {
+    mt_begin(p, s);
+  } else {
+    p.alpha = 1.0;
+  }
+  trial(p);
+}
+
+void new_search(Problem& p, const glia_opt_settings& s) {
   direction(p, false);
   if (!(p.gtd < 0)) {
     p.S.clear();
@@ -104,9 +312,7 @@ void new_search(Problem& p) {
     p.reason = GLIA_OPT_DIVERGED_LS_FAILURE;
     return;
   }
-  p.alpha = 1.0;
-  p.ls_k = 0;
-  trial(p);
+  start_search(p, s);
 }
 
 // the reference's stopping test after an accepted step (or at the start)
@@ -126,6 +332,48 @@ bool converged(Problem& p, const glia_opt_settings& s, double step) {
   return p.reason != GLIA_OPT_CONTINUE;
 }
 
+// take the point xn (value f, gradient g) reached with step length a as the next iterate
+void accept(Problem& p, const glia_opt_settings& s, const std::vector<double>& xn, double f, const double* g,
+            double a) {
+  std::vector<double> sv(p.n), yv(p.n), gp(p.n);
+  project_gradient(p, xn.data(), g, gp);
+  for (int i = 0; i < p.n; i++) {
+    sv[i] = xn[i] - p.x[i];
+    yv[i] = gp[i] - p.gp[i];
+  }
+  if (dot(sv, yv) > 0) {
+    p.S.push_back(sv);
+    p.Y.push_back(yv);
+    if ((int)p.S.size() > std::max(1, s.lbfgs_vectors)) {
+      p.S.pop_front();
+      p.Y.pop_front();
+    }
+  }
+  std::copy(xn.begin(), xn.end(), p.x);
+  p.f = f;
+  std::copy(g, g + p.n, p.g.begin());
+  p.gp.swap(gp);
+  p.gnorm = std::sqrt(dot(p.gp, p.gp));
+  p.it++;
+  p.reset_tried = false;
+  if (!converged(p, s, a)) new_search(p, s);
+}
+
+// a failed search is retried once along the scaled steepest descent with an empty memory
+void fail_search(Problem& p, const glia_opt_settings& s) {
+  if (!p.reset_tried) {
+    p.reset_tried = true;
+    p.S.clear();
+    p.Y.clear();
+    direction(p, true);
+    if (p.gtd < 0) {
+      start_search(p, s);
+      return;
+    }
+  }
+  p.reason = GLIA_OPT_DIVERGED_LS_FAILURE;
+}
+
 // consume one evaluation (f, g at the problem's requested point)
 void advance(Problem& p, const glia_opt_settings& s, double f, const double* g) {
   p.fevals++;
@@ -136,34 +384,26 @@ void advance(Problem& p, const glia_opt_settings& s, double f, const double* g) 
     project_gradient(p, p.x, g, p.gp);
     p.gnorm = std::sqrt(dot(p.gp, p.gp));
     if (!(p.gnorm0 > 0)) p.gnorm0 = (p.gnorm > 0) ? p.gnorm : 1.0;
-    if (!converged(p, s, 1.0)) new_search(p);
+    if (!converged(p, s, 1.0)) new_search(p, s);
     return;
   }
+  if (s.line_search == GLIA_OPT_LS_MORE_THUENTE) {
+    const int st = mt_advance(p, s, f, g);
+    if (st == LS_ACCEPT) {
+      accept(p, s, p.xt, f, g, p.alpha);
+    } else if (st == LS_ACCEPT_BEST) {
+      const std::vector<double> xb = p.mt.xb, gb = p.mt.gb;
+      accept(p, s, xb, p.mt.fb, gb.data(), p.mt.ab);
+    } else if (st == LS_FAIL) {
+      fail_search(p, s);
+    }
+    return;
+  }
+  // Armijo
   double decrease = 0;
   for (int i = 0; i < p.n; i++) decrease += p.g[i] * (p.xt[i] - p.x[i]);
   if (std::isfinite(f) && f <= p.f + s.ls_sigma * decrease) {
-    std::vector<double> sv(p.n), yv(p.n), gp(p.n);
-    project_gradient(p, p.xt.data(), g, gp);
-    for (int i = 0; i < p.n; i++) {
-      sv[i] = p.xt[i] - p.x[i];
-      yv[i] = gp[i] - p.gp[i];
-    }
-    if (dot(sv, yv) > 0) {
-      p.S.push_back(sv);
-      p.Y.push_back(yv);
-      if ((int)p.S.size() > std::max(1, s.lbfgs_vectors)) {
-        p.S.pop_front();
-        p.Y.pop_front();
-      }
-    }
-    std::copy(p.xt.begin(), p.xt.end(), p.x);
-    p.f = f;
-    std::copy(g, g + p.n, p.g.begin());
-    p.gp.swap(gp);
-    p.gnorm = std::sqrt(dot(p.gp, p.gp));
-    p.it++;
-    p.reset_tried = false;
-    if (!converged(p, s, p.alpha)) new_search(p);
+    accept(p, s, p.xt, f, g, p.alpha);
     return;
   }
   if (++p.ls_k < s.ls_max_funcs) {
@@ -171,19 +411,7 @@ void advance(Problem& p, const glia_opt_settings& s, double f, const double* g) 
     trial(p);
     return;
   }
-  if (!p.reset_tried) {
-    p.reset_tried = true;
-    p.S.clear();
-    p.Y.clear();
-    direction(p, true);
-    if (p.gtd < 0) {
-      p.alpha = 1.0;
-      p.ls_k = 0;
-      trial(p);
-      return;
-    }
-  }
-  p.reason = GLIA_OPT_DIVERGED_LS_FAILURE;
+  fail_search(p, s);
 }
 
 using FG = std::function<int(int nprob, const double* x, double* f, double* g)>;
@@ -524,6 +752,9 @@ void glia_opt_settings_init(glia_opt_settings* s) {
   s->ls_max_funcs = 10;
   s->ls_sigma = 1e-4;
   s->ls_minstep = std::numeric_limits<double>::epsilon();
+  s->line_search = GLIA_OPT_LS_MORE_THUENTE;
+  s->ls_gtol = 0.9;
+  s->ls_xtol = 1e-10;
 }
 
 int glia_blmvm(int nprob, int n, double* x, const double* lb, const double* ub, glia_fg_fn fg, void* ctx,

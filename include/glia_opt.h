@@ -5,9 +5,9 @@
  * checkConvergenceGrad, src/opt/TaoInterface.cpp:362-520), and the CoSaMP sparse-TIL driver
  * around it (src/opt/SparseTILOptimizer.cpp).
  *
- * PETSc/TAO is not available, so this is a restatement of the published BLMVM algorithm, not a
- * bitwise reproduction of TAO: same inputs, options and stopping rule, own quasi-Newton matrix
- * (two-loop L-BFGS with the scalar initial scaling s'y / y'y).
+ * PETSc/TAO is not available, so this is a restatement of the published BLMVM algorithm and of the
+ * More-Thuente line search, not a bitwise reproduction of TAO: same inputs, options and stopping
+ * rule, own quasi-Newton matrix (two-loop L-BFGS with the scalar initial scaling s'y / y'y).
  */
 #ifndef GLIA_OPT_H_
 #define GLIA_OPT_H_
@@ -29,6 +29,9 @@ enum {
   GLIA_OPT_DIVERGED_LS_FAILURE = -6
 };
 
+/* line searches (OptimizerSettings::linesearch_) */
+enum { GLIA_OPT_LS_ARMIJO = 0, GLIA_OPT_LS_MORE_THUENTE = 1 };
+
 typedef struct glia_opt_settings {
   int max_it;         /* OptimizerSettings::newton_maxit_ (30) */
   int min_it;         /* OptimizerSettings::newton_minit_ (0) */
@@ -38,6 +41,9 @@ typedef struct glia_opt_settings {
   int ls_max_funcs;   /* ls_max_func_evals (10) */
   double ls_sigma;    /* Armijo sufficient-decrease constant (1e-4) */
   double ls_minstep;  /* ls_minstep_ (machine epsilon): a smaller accepted step stops (STEPTOL) */
+  int line_search;    /* linesearch_: GLIA_OPT_LS_MORE_THUENTE (the reference's default) or _ARMIJO */
+  double ls_gtol;     /* curvature constant of the More-Thuente search (0.9) */
+  double ls_xtol;     /* relative interval tolerance of the More-Thuente search (1e-10) */
 } glia_opt_settings;
 
 typedef struct glia_opt_result {

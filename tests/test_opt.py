@@ -251,3 +251,62 @@ def test_oracle_sparse_til_recovers_support_and_coefficients(oracle_mod):
     assert sup[0] == sorted(np.flatnonzero(p).tolist()), (sup, res)
     assert np.allclose(x[0][:12], p, atol=3e-3)
     assert abs(x[0][12] - 0.05) < 5e-4 and abs(x[0][13] - 8.0) < 0.05, x[0][12:]
+
+
+# ------------------------------------------------------------------ More-Thuente search ----
+def quartic(X):
+    return np.sum(X ** 4 / 4 + X ** 2 / 2, axis=1), X ** 3 + X
+
+
+def offset_quadratic(x0):
+    """f = |x|^2 / 2 - 0.99 |x0|^2 / 2: at x0, |f| is small against |g|^2, so the first step of
+    length delta = 2 |f0| / |g0|^2 is 100 times too short and breaks the curvature condition."""
+    c = 0.99 * 0.5 * float(x0 @ x0)
+
+    def fg(X):
+        return 0.5 * np.sum(X * X, axis=1) - c, X.copy()
+    return fg
+
+
+def first_step(fg, x0, ls):
+    f0, g0 = fg(x0[None])
+    d = -2 * abs(f0[0]) / (g0[0] @ g0[0]) * g0[0]
+    x1, r = opt.minimize(fg, x0[None], max_it=0, min_it=0, gatol=0.0, gttol=0.0, line_search=ls)
+    assert r[0]["iterations"] == 1 and r[0]["reason"] == "maxits"
+    step = x1[0] - x0
+    a = step @ d / (d @ d)
+    assert np.allclose(step, a * d, rtol=0, atol=1e-12)  # along d
+    f1, g1 = fg(x1)
+    return a, f0[0], g0[0] @ d, f1[0], g1[0] @ d
+
+
+def test_more_thuente_first_step_satisfies_strong_wolfe():
+    """With max_it = 0 the loop stops after one accepted step along the first direction
+    d = -delta g0.  More-Thuente's step satisfies phi(a) <= phi(0) + 1e-4 a phi'(0) and
+    |phi'(a)| <= 0.9 |phi'(0)| (More and Thuente, 1994); the Armijo step, accepted at a = 1, does
+    not meet the curvature condition here."""
+    for x0 in ([1.0, 1.0], [3.0, -0.5, 2.0]):
+        x0 = np.array(x0)
+        fg = offset_quadratic(x0)
+        a, f0, dphi0, f1, dphi1 = first_step(fg, x0, 1)
+        assert f1 <= f0 + 1e-4 * a * dphi0
+        assert abs(dphi1) <= 0.9 * abs(dphi0)
+        a, f0, dphi0, f1, dphi1 = first_step(fg, x0, 0)
+        assert abs(a - 1.0) < 1e-9 and abs(dphi1) > 0.9 * abs(dphi0)
+    for x0 in ([2.0, -1.5], [3.0, 0.5, -2.0]):  # a well-scaled first step is kept as it is
+        a, f0, dphi0, f1, dphi1 = first_step(quartic, np.array(x0), 1)
+        assert f1 <= f0 + 1e-4 * a * dphi0 and abs(dphi1) <= 0.9 * abs(dphi0)
+
+
+def test_more_thuente_and_armijo_both_reach_the_minimum():
+    for ls in (0, 1):
+        x, r = opt.minimize(rosen, [[-1.2, 1.0], [2.0, 2.0]], gatol=1e-8, gttol=1e-12, max_it=500, line_search=ls)
+        assert np.allclose(x, 1.0, atol=1e-6), (ls, x)
+        x, r = opt.minimize(rosen, [[-1.2, 1.0]], lb=[-2, -2], ub=[0.5, 2], gatol=1e-8, gttol=1e-12, max_it=500,
+                            line_search=ls)
+        assert np.allclose(x, [[0.5, 0.25]], atol=1e-7), (ls, x)
+
+
+def test_more_thuente_defaults_match_the_reference():
+    s = opt.settings()
+    assert s.line_search == 1 and s.ls_gtol == 0.9 and s.ls_max_funcs == 10 and s.ls_sigma == 1e-4
