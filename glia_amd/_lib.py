@@ -122,7 +122,7 @@ class OptSettings(ctypes.Structure):
     _fields_ = [("max_it", ctypes.c_int), ("min_it", ctypes.c_int), ("gatol", ctypes.c_double),
                 ("gttol", ctypes.c_double), ("lbfgs_vectors", ctypes.c_int), ("ls_max_funcs", ctypes.c_int),
                 ("ls_sigma", ctypes.c_double), ("ls_minstep", ctypes.c_double), ("line_search", ctypes.c_int),
-                ("ls_gtol", ctypes.c_double), ("ls_xtol", ctypes.c_double)]
+                ("ls_gtol", ctypes.c_double), ("ls_xtol", ctypes.c_double), ("gnorm_ref", ctypes.c_double)]
 
 
 class OptResult(ctypes.Structure):

@@ -438,6 +438,7 @@ int blmvm(int nprob, int n, double* x, const double* lb, const double* ub, const
     p.gp.assign(n, 0.0);
     p.xt.assign(n, 0.0);
     if (gref && gref[m] > 0) p.gnorm0 = gref[m];
+    else if (s.gnorm_ref > 0) p.gnorm0 = s.gnorm_ref;
     for (int i = 0; i < n; i++) p.x[i] = std::min(std::max(p.x[i], lo(p, i)), hi(p, i));
     std::copy(p.x, p.x + n, p.xt.begin());
   }
@@ -755,6 +756,7 @@ void glia_opt_settings_init(glia_opt_settings* s) {
   s->line_search = GLIA_OPT_LS_MORE_THUENTE;
   s->ls_gtol = 0.9;
   s->ls_xtol = 1e-10;
+  s->gnorm_ref = 0.0;
 }
 
 int glia_blmvm(int nprob, int n, double* x, const double* lb, const double* ub, glia_fg_fn fg, void* ctx,

@@ -38,6 +38,8 @@ struct TumorParameters {
   ScalarType smoothing_factor_ = 1.0;
   int order_ = 2;
   bool phi_store_ = false, adjoint_store_ = false;
+  int sparsity_level_ = 5;
+  ScalarType thresh_component_weight_ = 1e-3;
 };
 
 /* OptimizerSettings defaults (include/Parameters.h:18-111) */
@@ -47,6 +49,14 @@ struct OptimizerSettings {
   bool flag_reaction_inv_ = false;
   bool diffusivity_inversion_ = true;
   bool reaction_inversion_ = false;
+  ScalarType opttolgrad_ = 1e-3, gatol_ = 1e-4;
+  int newton_maxit_ = 30, newton_minit_ = 1;
+  int linesearch_ = 1;  // 0 Armijo, 1 More-Thuente
+  int ls_max_func_evals = 10, lbfgs_vectors_ = 10;
+  ScalarType k_lb_ = 1e-3, k_ub_ = 1.0, rho_lb_ = 1.0, rho_ub_ = 15.0;
+  bool rescale_init_cond_ = false;
+  int gist_maxit_ = 5;
+  bool prune_components_ = true;
 };
 
 /* the device handle of one Parameters object, and the coefficients last pushed to it */

@@ -44,6 +44,8 @@ typedef struct glia_opt_settings {
   int line_search;    /* linesearch_: GLIA_OPT_LS_MORE_THUENTE (the reference's default) or _ARMIJO */
   double ls_gtol;     /* curvature constant of the More-Thuente search (0.9) */
   double ls_xtol;     /* relative interval tolerance of the More-Thuente search (1e-10) */
+  double gnorm_ref;   /* > 0: the reference gradient norm of the relative test (gradnorm0_) for
+                         every problem; 0 (default): each problem's first projected gradient norm */
 } glia_opt_settings;
 
 typedef struct glia_opt_result {

@@ -5,6 +5,8 @@
 //   "Running diffusion solver"       src/test/pdesolver.cpp:7-57 (||c||_2 = 2.0487, 5 CG iterations)
 //   "Evaluating objective function"  src/test/grad.cpp:7-80 (J = 0 at the truth, J > 0 otherwise)
 //
+//   TILOptimizer, RDOptimizer, SparseTILOptimizer on exact data (src/opt/*.cpp through include/glia/Optimizer.h)
+//
 // Exit status: the number of failed checks.  Needs a GPU.
 #include <cfloat>
 #include <cmath>
@@ -355,11 +357,152 @@ static void test_phi_and_apply_d() {
   VecDestroy(&s.d);
 }
 
+// ---- the inversion classes on exact data ------------------------------------------------------
+// the TIL problem of make_til with exact data d = c(T) of p* and the truth kappa = 0.05, rho = 8
+static void make_exact(TilSetup& s) {
+  make_til(s);
+  Vec pstar;
+  VecCreateSeq(PETSC_COMM_SELF, NP, &pstar);
+  double* pp;
+  VecGetArray(pstar, &pp);
+  pp[0] = 0.8, pp[1] = 0.5, pp[2] = 0.0, pp[3] = 0.3;
+  VecRestoreArray(pstar, &pp);
+  s.tumor->phi_->apply(s.tumor->c_0_, pstar);
+  s.pde->solveState(0);
+  VecCopy(s.tumor->c_t_, s.d);
+  VecDestroy(&pstar);
+}
+
+static void test_til_optimizer() {
+  std::printf("TEST TILOptimizer (p and kappa) against glia_rd_invert\n");
+  TilSetup s;
+  make_exact(s);
+  s.params->opt_->opttolgrad_ = 1e-5;
+  TILOptimizer til;
+  til.initialize(s.der, s.pde, s.params, s.tumor);
+  til.setData(s.data);
+  Vec x0;
+  VecCreateSeq(PETSC_COMM_SELF, NP + 2, &x0);
+  const double start[NP + 2] = {0.5, 0.5, 0.5, 0.5, 0.05, 8.0};
+  double* xp;
+  VecGetArray(x0, &xp);
+  for (int i = 0; i < NP + 2; i++) xp[i] = start[i];
+  VecRestoreArray(x0, &xp);
+  REQUIRE(til.setInitialGuess(x0) == 0);
+  REQUIRE(til.solve() == 0);
+  Vec xo = til.getSolution();
+  REQUIRE(xo != nullptr && xo->n == NP + 1);
+  std::printf("  TIL: p = (%.4f %.4f %.4f %.4f), kappa %.5f, reason %d, %d iterations, J %.3e\n", xo->v[0], xo->v[1],
+              xo->v[2], xo->v[3], xo->v[4], til.result_.reason, til.result_.iterations, til.result_.f);
+  REQUIRE(til.result_.reason > 0);
+  // the same inversion through the C ABI on a handle of its own
+  glia_rd_config c;
+  glia_rd_config_init(&c);
+  c.n[0] = c.n[1] = c.n[2] = 64;
+  c.dt = 0.02;
+  c.nt = 5;
+  c.k_gm_wm = 0.2;
+  c.r_gm_wm = 0.2;
+  c.beta = 1e-3;
+  glia_rd_t h = nullptr;
+  REQUIRE(glia_rd_create(&c, &h) == 0);
+  glia_rd_set_tissue_sinusoidal(h, 0);
+  glia_rd_set_coefficients(h, 0, 0.05, 8.0);
+  glia_rd_set_basis(h, 0, NP, s.centers.data(), s.sigma);
+  std::vector<float> d1(s.d->n);
+  double* dp;
+  VecGetArray(s.d, &dp);
+  for (int64_t i = 0; i < s.d->n; i++) d1[i] = (float)dp[i];
+  VecRestoreArray(s.d, &dp);
+  glia_rd_set_data(h, 0, d1.data(), nullptr, -INFINITY, GLIA_RD_HOST);
+  double xa[NP + 2], lb[NP + 2], ub[NP + 2];
+  for (int i = 0; i < NP + 2; i++) {
+    xa[i] = start[i];
+    lb[i] = 0.0;
+    ub[i] = INFINITY;
+  }
+  lb[NP] = 1e-3, ub[NP] = 1.0, lb[NP + 1] = ub[NP + 1] = 8.0;
+  glia_opt_settings os;
+  glia_opt_settings_init(&os);
+  os.gttol = 1e-5;
+  glia_opt_result r;
+  REQUIRE(glia_rd_invert(h, GLIA_RD_MODE_P_KAPPA, xa, lb, ub, &os, &r) == 0);
+  glia_rd_destroy(h);
+  double dmax = 0;
+  for (int i = 0; i <= NP; i++) dmax = std::max(dmax, std::fabs(xa[i] - xo->v[i]));
+  std::printf("  ABI: p = (%.4f %.4f %.4f %.4f), kappa %.5f, %d iterations; max |difference| %.2e\n", xa[0], xa[1],
+              xa[2], xa[3], xa[4], r.iterations, dmax);
+  REQUIRE(dmax < 1e-3);
+  VecDestroy(&x0);
+  VecDestroy(&s.d);
+}
+
+static void test_rd_optimizer() {
+  std::printf("TEST RDOptimizer (kappa and rho) recovers the truth\n");
+  TilSetup s;
+  make_exact(s);
+  s.params->opt_->opttolgrad_ = 1e-6;
+  s.params->opt_->gatol_ = 1e-12;  // the default absolute tolerance (1e-4) stops 0.3% short here
+  RDOptimizer rdo;
+  rdo.initialize(s.der, s.pde, s.params, s.tumor);
+  rdo.setData(s.data);
+  Vec x0;
+  VecCreateSeq(PETSC_COMM_SELF, NP + 2, &x0);
+  const double start[NP + 2] = {0.8, 0.5, 0.0, 0.3, 0.1, 5.0};
+  double* xp;
+  VecGetArray(x0, &xp);
+  for (int i = 0; i < NP + 2; i++) xp[i] = start[i];
+  VecRestoreArray(x0, &xp);
+  REQUIRE(rdo.setInitialGuess(x0) == 0);
+  REQUIRE(rdo.solve() == 0);
+  Vec xo = rdo.getSolution();
+  std::printf("  kappa %.6f rho %.5f, reason %d, %d iterations\n", xo->v[NP], xo->v[NP + 1], rdo.result_.reason,
+              rdo.result_.iterations);
+  REQUIRE(rdo.result_.reason > 0);
+  REQUIRE(std::fabs(xo->v[NP] - 0.05) < 1e-3 * 0.05);
+  REQUIRE(std::fabs(xo->v[NP + 1] - 8.0) < 1e-3 * 8.0);
+  REQUIRE(s.params->opt_->beta_ == 1e-3 && s.params->tu_->np_ == NP);  // restored after the solve
+  VecDestroy(&x0);
+  VecDestroy(&s.d);
+}
+
+static void test_sparse_til_optimizer() {
+  std::printf("TEST SparseTILOptimizer finds the active bases\n");
+  TilSetup s;
+  make_exact(s);
+  s.params->tu_->sparsity_level_ = 3;
+  s.params->opt_->opttolgrad_ = 1e-5;
+  SparseTILOptimizer sp;
+  sp.initialize(s.der, s.pde, s.params, s.tumor);
+  sp.setData(s.data);
+  Vec x0;
+  VecCreateSeq(PETSC_COMM_SELF, NP + 2, &x0);
+  VecSet(x0, 0.0);
+  double* xp;
+  VecGetArray(x0, &xp);
+  xp[NP] = 0.05, xp[NP + 1] = 8.0;
+  VecRestoreArray(x0, &xp);
+  REQUIRE(sp.setInitialGuess(x0) == 0);
+  REQUIRE(sp.solve() == 0);
+  Vec xo = sp.getSolution();
+  std::printf("  support:");
+  for (int i : sp.support_) std::printf(" %d", i);
+  std::printf("; p = (%.4f %.4f %.4f %.4f), CoSaMP iterations %d\n", xo->v[0], xo->v[1], xo->v[2], xo->v[3],
+              sp.sparse_result_.iterations);
+  REQUIRE((sp.support_ == std::vector<int>{0, 1, 3}));
+  REQUIRE(xo->v[2] == 0.0);
+  VecDestroy(&x0);
+  VecDestroy(&s.d);
+}
+
 int main() {
   test_diffusion_solver();
   test_objective();
   test_interface_matches_abi_and_fd();
   test_phi_and_apply_d();
+  test_til_optimizer();
+  test_rd_optimizer();
+  test_sparse_til_optimizer();
   std::printf("%d checks, %d failed\n", checks, failures);
   return failures;
 }
