@@ -43,6 +43,12 @@ static int ret(glia_rd_t h, int rc) {
   return rc;
 }
 
+// a rejected argument: the message is what glia_rd_last_error reports
+static int arg_err(glia_rd_t h, const char* what) {
+  h->err = what;
+  return GLIA_RD_ERR_ARG;
+}
+
 #define GLIA_H(h)                                      \
   do {                                                 \
     if (!(h) || !(h)->e) return GLIA_RD_ERR_ARG;       \
@@ -138,7 +144,7 @@ int glia_rd_num_members(glia_rd_t h) {
 
 int glia_rd_get_grid(glia_rd_t h, int* n) {
   GLIA_H(h);
-  if (!n) return GLIA_RD_ERR_ARG;
+  if (!n) return arg_err(h, "null output pointer");
   for (int d = 0; d < 3; d++) n[d] = h->e->config().n[d];
   return 0;
 }
@@ -156,25 +162,25 @@ int glia_rd_set_tissue_sinusoidal(glia_rd_t h, int member) {
 
 int glia_rd_set_coefficients(glia_rd_t h, int member, double kappa, double rho) {
   GLIA_H(h);
-  if (member < 0 || member >= h->e->nmembers()) return GLIA_RD_ERR_ARG;
+  if (member < 0 || member >= h->e->nmembers()) return arg_err(h, "member out of range");
   return ret(h, h->e->set_coefficients(member, kappa, rho));
 }
 
 int glia_rd_set_kappa(glia_rd_t h, int member, double kappa) {
   GLIA_H(h);
-  if (member < 0 || member >= h->e->nmembers()) return GLIA_RD_ERR_ARG;
+  if (member < 0 || member >= h->e->nmembers()) return arg_err(h, "member out of range");
   return ret(h, h->e->set_kappa(member, kappa));
 }
 
 int glia_rd_set_rho(glia_rd_t h, int member, double rho) {
   GLIA_H(h);
-  if (member < 0 || member >= h->e->nmembers()) return GLIA_RD_ERR_ARG;
+  if (member < 0 || member >= h->e->nmembers()) return arg_err(h, "member out of range");
   return ret(h, h->e->set_rho(member, rho));
 }
 
 int glia_rd_set_k_sinusoidal(glia_rd_t h, int member, double scale) {
   GLIA_H(h);
-  if (member < 0 || member >= h->e->nmembers()) return GLIA_RD_ERR_ARG;
+  if (member < 0 || member >= h->e->nmembers()) return arg_err(h, "member out of range");
   return ret(h, h->e->set_k_sinusoidal(member, scale));
 }
 
@@ -191,14 +197,14 @@ int glia_rd_set_ctx_dt(glia_rd_t h, double dt) {
 
 int glia_rd_get_ctx_dt(glia_rd_t h, double* dt) {
   GLIA_H(h);
-  if (!dt) return GLIA_RD_ERR_ARG;
+  if (!dt) return arg_err(h, "null output pointer");
   *dt = h->e->ctx_dt();
   return 0;
 }
 
 int glia_rd_set_beta(glia_rd_t h, double beta) {
   GLIA_H(h);
-  if (!(beta >= 0)) return GLIA_RD_ERR_ARG;
+  if (!(beta >= 0)) return arg_err(h, "beta must be >= 0");
   h->e->set_beta(beta);
   return 0;
 }
@@ -211,7 +217,7 @@ int glia_rd_set_ratios(glia_rd_t h, double k_gm_wm, double k_glm_wm, double r_gm
 
 int glia_rd_get_beta(glia_rd_t h, double* beta) {
   GLIA_H(h);
-  if (!beta) return GLIA_RD_ERR_ARG;
+  if (!beta) return arg_err(h, "null output pointer");
   *beta = h->e->config().beta;
   return 0;
 }
@@ -228,21 +234,21 @@ int glia_rd_basis_size(glia_rd_t h, int member) {
 
 int glia_rd_set_data(glia_rd_t h, int member, const float* d1, const float* obs_mask, double obs_thr, int mem_kind) {
   GLIA_H(h);
-  if (!d1) return GLIA_RD_ERR_ARG;
+  if (!d1) return arg_err(h, "null data pointer");
   return ret(h, h->e->set_data(member, d1, obs_mask, obs_thr, mem_kind == GLIA_RD_DEVICE));
 }
 
 int glia_rd_set_field(glia_rd_t h, int member, int which, const float* src, int mem_kind) {
   GLIA_H(h);
   const int s = field_slot(which);
-  if (s < 0) return GLIA_RD_ERR_ARG;
+  if (s < 0) return arg_err(h, "unknown field id");
   return ret(h, h->e->set_field(member, s, src, mem_kind == GLIA_RD_DEVICE));
 }
 
 int glia_rd_get_field(glia_rd_t h, int member, int which, float* dst, int mem_kind) {
   GLIA_H(h);
   const int s = field_slot(which);
-  if (s < 0 || !dst) return GLIA_RD_ERR_ARG;
+  if (s < 0 || !dst) return arg_err(h, "unknown field id or null destination");
   return ret(h, h->e->get_field(member, s, dst, mem_kind == GLIA_RD_DEVICE));
 }
 
@@ -262,7 +268,7 @@ static bool split_x(RDEnsemble* e, const double* x, int extra, std::vector<std::
 int glia_rd_phi_apply(glia_rd_t h, const double* p) {
   GLIA_H(h);
   std::vector<std::vector<double>> pv;
-  if (!split_x(h->e.get(), p, 0, pv)) return GLIA_RD_ERR_ARG;
+  if (!split_x(h->e.get(), p, 0, pv)) return arg_err(h, "p does not match the basis sizes");
   int rc = h->e->phi_apply(pv);
   if (rc == 0) rc = h->e->sync();
   return ret(h, rc);
@@ -271,7 +277,7 @@ int glia_rd_phi_apply(glia_rd_t h, const double* p) {
 int glia_rd_phi_apply_transpose(glia_rd_t h, int which, double* out) {
   GLIA_H(h);
   const int s = field_slot(which);
-  if (s < 0 || !out) return GLIA_RD_ERR_ARG;
+  if (s < 0 || !out) return arg_err(h, "unknown field id or null output");
   std::vector<std::vector<double>> o;
   const int rc = h->e->phi_apply_transpose(s, o);
   if (rc) return ret(h, rc);
@@ -286,21 +292,21 @@ int glia_rd_phi_apply_transpose(glia_rd_t h, int which, double* out) {
 int glia_rd_diffusion_solve(glia_rd_t h, int which, double tau) {
   GLIA_H(h);
   const int s = field_slot(which);
-  if (s < 0) return GLIA_RD_ERR_ARG;
+  if (s < 0) return arg_err(h, "unknown field id");
   return ret(h, h->e->diffusion_solve_field(s, tau));
 }
 
 int glia_rd_apply_d(glia_rd_t h, int which_in, int which_out) {
   GLIA_H(h);
   const int a = field_slot(which_in), b = field_slot(which_out);
-  if (a < 0 || b < 0) return GLIA_RD_ERR_ARG;
+  if (a < 0 || b < 0) return arg_err(h, "unknown field id");
   return ret(h, h->e->apply_d_field(a, b));
 }
 
 // packed half spectrum <-> reference layout (kz = 0 and kz = n2/2 planes unpacked by symmetry)
 int glia_rd_fft_r2c(glia_rd_t h, int member, const float* f, float* fhat) {
   GLIA_H(h);
-  if (!f || !fhat || member < 0 || member >= h->e->nmembers()) return GLIA_RD_ERR_ARG;
+  if (!f || !fhat || member < 0 || member >= h->e->nmembers()) return arg_err(h, "null buffer or member out of range");
   const glia::Dims& d = h->e->dims();
   int rc = h->e->set_field(member, glia::RS_CH, f, false);
   if (!rc) rc = h->e->fft_r2c(glia::RS_CH, glia::CS_T7);
@@ -331,7 +337,7 @@ int glia_rd_fft_r2c(glia_rd_t h, int member, const float* f, float* fhat) {
 
 int glia_rd_fft_c2r(glia_rd_t h, int member, const float* fhat, float* f) {
   GLIA_H(h);
-  if (!f || !fhat || member < 0 || member >= h->e->nmembers()) return GLIA_RD_ERR_ARG;
+  if (!f || !fhat || member < 0 || member >= h->e->nmembers()) return arg_err(h, "null buffer or member out of range");
   const glia::Dims& d = h->e->dims();
   const int n0 = d.n0, n1 = d.n1, M = d.m2, n2h = M + 1;
   std::vector<float2> P(d.nspec);
@@ -369,7 +375,7 @@ static int eval(glia_rd_t h, int mode, const double* x, double* J, double* dJ, b
   GLIA_H(h);
   std::vector<std::vector<double>> xv, g;
   std::vector<double> Jv;
-  if (!split_x(h->e.get(), x, 2, xv)) return GLIA_RD_ERR_ARG;
+  if (!split_x(h->e.get(), x, 2, xv)) return arg_err(h, "x does not match the basis sizes");
   const int rc = h->e->objective_and_gradient(mode, xv, grad, Jv, g);
   if (rc) return ret(h, rc);
   int64_t off = 0;
@@ -382,25 +388,25 @@ static int eval(glia_rd_t h, int mode, const double* x, double* J, double* dJ, b
 }
 
 int glia_rd_objective(glia_rd_t h, int mode, const double* x, double* J) {
-  if (!J) return GLIA_RD_ERR_ARG;
+  if (!J) return arg_err(h, "null output pointer");
   return eval(h, mode, x, J, nullptr, false);
 }
 
 int glia_rd_gradient(glia_rd_t h, int mode, const double* x, double* dJ) {
-  if (!dJ) return GLIA_RD_ERR_ARG;
+  if (!dJ) return arg_err(h, "null output pointer");
   return eval(h, mode, x, nullptr, dJ, true);
 }
 
 int glia_rd_objective_and_gradient(glia_rd_t h, int mode, const double* x, double* J, double* dJ) {
-  if (!J || !dJ) return GLIA_RD_ERR_ARG;
+  if (!J || !dJ) return arg_err(h, "null output pointer");
   return eval(h, mode, x, J, dJ, true);
 }
 
 int glia_rd_hessian(glia_rd_t h, int diffusivity, const double* x, double* y) {
   GLIA_H(h);
-  if (!x || !y) return GLIA_RD_ERR_ARG;
+  if (!x || !y) return arg_err(h, "null x or y");
   std::vector<std::vector<double>> xv, yv;
-  if (!split_x(h->e.get(), x, 2, xv)) return GLIA_RD_ERR_ARG;
+  if (!split_x(h->e.get(), x, 2, xv)) return arg_err(h, "x does not match the basis sizes");
   const int rc = h->e->hessian(xv, diffusivity != 0, yv);
   if (rc) return ret(h, rc);
   int64_t off = 0;
@@ -443,7 +449,7 @@ int glia_rd_objective_and_gradient_group(int nh, const glia_rd_t* handles, const
 
 int glia_rd_stats(glia_rd_t h, int member, glia_rd_stats_t* out) {
   GLIA_H(h);
-  if (!out) return GLIA_RD_ERR_ARG;
+  if (!out) return arg_err(h, "null output pointer");
   glia::MemberStats s;
   const int rc = h->e->stats(member, &s);
   if (rc) return ret(h, rc);
@@ -463,7 +469,7 @@ int glia_rd_stats(glia_rd_t h, int member, glia_rd_stats_t* out) {
 int glia_rd_smooth_field(glia_rd_t h, int member, int which, double sigma) {
   GLIA_H(h);
   const int slot = field_slot(which);
-  if (slot < 0) return GLIA_RD_ERR_ARG;
+  if (slot < 0) return arg_err(h, "unknown field id");
   return ret(h, h->e->smooth_field(member, slot, sigma));
 }
 
@@ -484,7 +490,7 @@ int glia_rd_profile(glia_rd_t h, int enable) {
 
 int glia_rd_profile_get(glia_rd_t h, int kind, double* ms, long long* launches) {
   GLIA_H(h);
-  if (!ms || !launches) return GLIA_RD_ERR_ARG;
+  if (!ms || !launches) return arg_err(h, "null output pointer");
   return ret(h, h->e->profile_get(kind, ms, launches));
 }
 
