@@ -122,7 +122,8 @@ class OptSettings(ctypes.Structure):
     _fields_ = [("max_it", ctypes.c_int), ("min_it", ctypes.c_int), ("gatol", ctypes.c_double),
                 ("gttol", ctypes.c_double), ("lbfgs_vectors", ctypes.c_int), ("ls_max_funcs", ctypes.c_int),
                 ("ls_sigma", ctypes.c_double), ("ls_minstep", ctypes.c_double), ("line_search", ctypes.c_int),
-                ("ls_gtol", ctypes.c_double), ("ls_xtol", ctypes.c_double), ("gnorm_ref", ctypes.c_double)]
+                ("ls_gtol", ctypes.c_double), ("ls_xtol", ctypes.c_double), ("gnorm_ref", ctypes.c_double),
+                ("newton_solver", ctypes.c_int), ("krylov_maxit", ctypes.c_int), ("forcing", ctypes.c_int)]
 
 
 class OptResult(ctypes.Structure):
@@ -155,11 +156,15 @@ EVAL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c
 # int (*)(void* ctx, const double* p, double* cmax)
 C0_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                          ctypes.POINTER(ctypes.c_double))
+# int (*)(void* ctx, int nprob, const double* v, double* y)
+HV_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                         ctypes.POINTER(ctypes.c_double))
 # include/glia_opt.h
 OPT_SIGNATURES = {
     "glia_opt_settings_init": (None, [ctypes.POINTER(OptSettings)]),
     "glia_blmvm": (_i, [_i, _i, _vp, _vp, _vp, FG_FN, _vp, ctypes.POINTER(OptSettings), _vp]),
     "glia_rd_invert": (_i, [_vp, _i, _vp, _vp, _vp, ctypes.POINTER(OptSettings), _vp]),
+    "glia_gauss_newton": (_i, [_i, _i, _vp, _vp, _vp, FG_FN, HV_FN, _vp, ctypes.POINTER(OptSettings), _vp]),
     "glia_hard_threshold": (_i, [_vp, _i, _i, _vp]),
     "glia_hard_threshold_components": (_i, [_vp, _i, _i, _vp, _vp, _i, _d, _i, _vp]),
     "glia_sparse_til_settings_init": (None, [ctypes.POINTER(SparseTilSettings)]),

@@ -475,6 +475,228 @@ int blmvm(int nprob, int n, double* x, const double* lb, const double* ub, const
   return 0;
 }
 
+using HV = std::function<int(int nprob, const double* v, double* y)>;
+
+// take the point reached by the line search as the next Gauss-Newton iterate (no quasi-Newton
+// memory); returns whether the problem continues
+void accept_gn(Problem& p, const glia_opt_settings& s, const std::vector<double>& xn, double f, const double* g,
+               double a) {
+  std::copy(xn.begin(), xn.end(), p.x);
+  p.f = f;
+  std::copy(g, g + p.n, p.g.begin());
+  project_gradient(p, p.x, g, p.gp);
+  p.gnorm = std::sqrt(dot(p.gp, p.gp));
+  p.it++;
+  converged(p, s, a);
+}
+
+// Gauss-Newton-Krylov with bounds (the reference's GAUSSNEWTON solver runs TAO's BNLS,
+// src/opt/Optimizer.cpp:127-131, with a CG Krylov solve of at most krylov_maxit iterations and the
+// Eisenstat-Walker tolerance of preKrylovSolve, src/opt/TaoInterface.cpp:160-193).  Per iteration
+// and problem: the variables on an active bound (P(g)_i = 0 there) are held, H_FF s_F = -g_F is
+// solved by CG to relative residual max(1e-10, min(0.5, sqrt(||g_F|| / ||g_F,0||))) (quadratic
+// forcing: without the square root), a direction that is not a descent direction is replaced by
+// -delta P(g), and the line search of the settings runs along it.  The problems of a batch share
+// every Hessian round (hv) and evaluation round (fg).
+int gauss_newton(int nprob, int n, double* x, const double* lb, const double* ub, const FG& fg, const HV& hv,
+                 const glia_opt_settings* sp, const double* gref, glia_opt_result* r) {
+  if (nprob <= 0 || n <= 0 || !x) return GLIA_RD_ERR_ARG;
+  glia_opt_settings s;
+  if (sp) {
+    s = *sp;
+  } else {
+    glia_opt_settings_init(&s);
+  }
+  const size_t N = (size_t)nprob * n;
+  std::vector<Problem> P(nprob);
+  for (int m = 0; m < nprob; m++) {
+    Problem& p = P[m];
+    p.n = n;
+    p.x = x + (size_t)m * n;
+    p.lb = lb ? lb + (size_t)m * n : nullptr;
+    p.ub = ub ? ub + (size_t)m * n : nullptr;
+    p.g.assign(n, 0.0);
+    p.gp.assign(n, 0.0);
+    p.xt.assign(n, 0.0);
+    p.d.assign(n, 0.0);
+    if (gref && gref[m] > 0) p.gnorm0 = gref[m];
+    else if (s.gnorm_ref > 0) p.gnorm0 = s.gnorm_ref;
+    for (int i = 0; i < n; i++) p.x[i] = std::min(std::max(p.x[i], lo(p, i)), hi(p, i));
+  }
+  std::vector<double> xe(N), fe(nprob), ge(N), v(N), y(N);
+  int rc = fg(nprob, x, fe.data(), ge.data());
+  if (rc) return rc;
+  for (int m = 0; m < nprob; m++) {
+    Problem& p = P[m];
+    p.fevals++;
+    p.started = true;
+    p.f = fe[m];
+    std::copy(ge.begin() + (size_t)m * n, ge.begin() + (size_t)(m + 1) * n, p.g.begin());
+    project_gradient(p, p.x, p.g.data(), p.gp);
+    p.gnorm = std::sqrt(dot(p.gp, p.gp));
+    if (!(p.gnorm0 > 0)) p.gnorm0 = (p.gnorm > 0) ? p.gnorm : 1.0;
+    converged(p, s, 1.0);
+  }
+  std::vector<double> bnorm0(nprob, 0.0);
+  std::vector<std::vector<double>> sv(nprob), rv(nprob), dv(nprob);
+  std::vector<std::vector<char>> held(nprob);
+  std::vector<double> rr(nprob), tol(nprob);
+  std::vector<char> cg_live(nprob), searching(nprob), took_best(nprob);
+  for (;;) {
+    bool any = false;
+    for (int m = 0; m < nprob; m++) any = any || P[m].reason == GLIA_OPT_CONTINUE;
+    if (!any) break;
+    // ---- CG on the free variables, all live problems in lockstep
+    for (int m = 0; m < nprob; m++) {
+      Problem& p = P[m];
+      cg_live[m] = p.reason == GLIA_OPT_CONTINUE;
+      sv[m].assign(n, 0.0);
+      rv[m].assign(n, 0.0);
+      dv[m].assign(n, 0.0);
+      took_best[m] = 0;
+      if (!cg_live[m]) continue;
+      // held: on a bound with a gradient that does not point into the box
+      held[m].assign(n, 0);
+      for (int i = 0; i < n; i++) {
+        held[m][i] = (p.x[i] <= lo(p, i) && p.g[i] >= 0) || (p.x[i] >= hi(p, i) && p.g[i] <= 0);
+        rv[m][i] = held[m][i] ? 0.0 : -p.g[i];
+      }
+      const double bn = std::sqrt(dot(rv[m], rv[m]));
+      if (!(bnorm0[m] > 0)) bnorm0[m] = bn > 0 ? bn : 1.0;
+      const double ratio = bn / bnorm0[m];
+      const double rel = std::max(1e-10, std::min(0.5, s.forcing == GLIA_OPT_FORCING_QUADRATIC ? ratio : std::sqrt(ratio)));
+      tol[m] = rel * bn;
+      dv[m] = rv[m];
+      rr[m] = dot(rv[m], rv[m]);
+      if (!(bn > 0)) cg_live[m] = 0;
+    }
+    for (int k = 0; k < std::max(1, s.krylov_maxit); k++) {
+      bool live = false;
+      for (int m = 0; m < nprob; m++) {
+        std::copy(dv[m].begin(), dv[m].end(), v.begin() + (size_t)m * n);
+        if (!cg_live[m]) std::fill(v.begin() + (size_t)m * n, v.begin() + (size_t)(m + 1) * n, 0.0);
+        live = live || cg_live[m];
+      }
+      if (!live) break;
+      if ((rc = hv(nprob, v.data(), y.data()))) return rc;
+      for (int m = 0; m < nprob; m++) {
+        if (!cg_live[m]) continue;
+        std::vector<double> hd(y.begin() + (size_t)m * n, y.begin() + (size_t)(m + 1) * n);
+        for (int i = 0; i < n; i++)
+          if (held[m][i]) hd[i] = 0.0;
+        const double curv = dot(dv[m], hd);
+        if (!(curv > 0)) {  // no positive curvature along d: keep what CG has, or the steepest step
+          if (k == 0) sv[m] = rv[m];
+          cg_live[m] = 0;
+          continue;
+        }
+        const double alpha = rr[m] / curv;
+        for (int i = 0; i < n; i++) {
+          sv[m][i] += alpha * dv[m][i];
+          rv[m][i] -= alpha * hd[i];
+        }
+        const double rrn = dot(rv[m], rv[m]);
+        if (std::sqrt(rrn) <= tol[m]) {
+          cg_live[m] = 0;
+          continue;
+        }
+        const double beta = rrn / rr[m];
+        rr[m] = rrn;
+        for (int i = 0; i < n; i++) dv[m][i] = rv[m][i] + beta * dv[m][i];
+      }
+    }
+    // ---- directions and line searches
+    for (int m = 0; m < nprob; m++) {
+      Problem& p = P[m];
+      searching[m] = 0;
+      if (p.reason != GLIA_OPT_CONTINUE) continue;
+      p.d = sv[m];
+      for (int i = 0; i < n; i++)
+        if ((p.x[i] <= lo(p, i) && p.d[i] < 0) || (p.x[i] >= hi(p, i) && p.d[i] > 0)) p.d[i] = 0;
+      p.gtd = dot(p.g, p.d);
+      if (!(p.gtd < 0)) {
+        direction(p, true);  // -delta P(g)
+        if (!(p.gtd < 0)) {
+          p.reason = GLIA_OPT_DIVERGED_LS_FAILURE;
+          continue;
+        }
+      }
+      p.reset_tried = false;
+      start_search(p, s);
+      searching[m] = 1;
+    }
+    for (;;) {
+      bool any_search = false;
+      for (int m = 0; m < nprob; m++) {
+        const Problem& p = P[m];
+        any_search = any_search || searching[m];
+        if (searching[m]) std::copy(p.xt.begin(), p.xt.end(), xe.begin() + (size_t)m * n);
+        else std::copy(p.x, p.x + n, xe.begin() + (size_t)m * n);
+      }
+      if (!any_search) break;
+      if ((rc = fg(nprob, xe.data(), fe.data(), ge.data()))) return rc;
+      for (int m = 0; m < nprob; m++) {
+        if (!searching[m]) continue;
+        Problem& p = P[m];
+        p.fevals++;
+        const double* g = ge.data() + (size_t)m * n;
+        int st;
+        if (s.line_search == GLIA_OPT_LS_MORE_THUENTE) {
+          st = mt_advance(p, s, fe[m], g);
+        } else {
+          double dec = 0;
+          for (int i = 0; i < n; i++) dec += p.g[i] * (p.xt[i] - p.x[i]);
+          if (std::isfinite(fe[m]) && fe[m] <= p.f + s.ls_sigma * dec) st = LS_ACCEPT;
+          else if (++p.ls_k < s.ls_max_funcs) {
+            p.alpha *= 0.5;
+            trial(p);
+            st = LS_NEXT;
+          } else {
+            st = LS_FAIL;
+          }
+        }
+        if (st == LS_ACCEPT) {
+          accept_gn(p, s, p.xt, fe[m], g, p.alpha);
+          searching[m] = 0;
+        } else if (st == LS_ACCEPT_BEST) {
+          const std::vector<double> xb = p.mt.xb, gb = p.mt.gb;
+          accept_gn(p, s, xb, p.mt.fb, gb.data(), p.mt.ab);
+          searching[m] = 0;
+          took_best[m] = 1;
+        } else if (st == LS_FAIL) {
+          // one retry along the scaled steepest descent, as the quasi-Newton loop does
+          if (!p.reset_tried) {
+            p.reset_tried = true;
+            direction(p, true);
+            if (p.gtd < 0) {
+              start_search(p, s);
+              continue;
+            }
+          }
+          p.reason = GLIA_OPT_DIVERGED_LS_FAILURE;
+          searching[m] = 0;
+        }
+      }
+    }
+    // every problem was evaluated at its current point in the last round, so the Hessian's
+    // linearization point (the last evaluation) is right for the next CG solve, except after an
+    // accepted best point of a halted search: evaluate again there
+    bool refresh = false;
+    for (int m = 0; m < nprob; m++) refresh = refresh || (P[m].reason == GLIA_OPT_CONTINUE && took_best[m]);
+    if (refresh && (rc = fg(nprob, x, fe.data(), ge.data()))) return rc;
+  }
+  if (r)
+    for (int m = 0; m < nprob; m++) {
+      r[m].iterations = P[m].it;
+      r[m].fevals = P[m].fevals;
+      r[m].reason = P[m].reason;
+      r[m].f = P[m].f;
+      r[m].gnorm = P[m].gnorm;
+      r[m].gnorm0 = P[m].gnorm0;
+    }
+  return 0;
+}
+
 // the reference's gradnorm0_ for a p-inversion (src/opt/TaoInterface.cpp:285-300): the gradient
 // at the zero vector of the unknowns the solve may move (lb < ub) -- entries fixed by their
 // bounds keep their values, as rho does outside the TIL vector -- and its norm over those unknowns
@@ -757,6 +979,9 @@ void glia_opt_settings_init(glia_opt_settings* s) {
   s->ls_gtol = 0.9;
   s->ls_xtol = 1e-10;
   s->gnorm_ref = 0.0;
+  s->newton_solver = GLIA_OPT_QUASI_NEWTON;
+  s->krylov_maxit = 30;
+  s->forcing = GLIA_OPT_FORCING_SUPERLINEAR;
 }
 
 int glia_blmvm(int nprob, int n, double* x, const double* lb, const double* ub, glia_fg_fn fg, void* ctx,
@@ -764,6 +989,14 @@ int glia_blmvm(int nprob, int n, double* x, const double* lb, const double* ub, 
   if (!fg) return GLIA_RD_ERR_ARG;
   FG f = [&](int np_, const double* xe, double* fe, double* ge) { return fg(ctx, np_, xe, fe, ge); };
   return blmvm(nprob, n, x, lb, ub, f, sp, nullptr, r);
+}
+
+int glia_gauss_newton(int nprob, int n, double* x, const double* lb, const double* ub, glia_fg_fn fg, glia_hv_fn hv,
+                      void* ctx, const glia_opt_settings* sp, glia_opt_result* r) {
+  if (!fg || !hv) return GLIA_RD_ERR_ARG;
+  FG f = [&](int np_, const double* xe, double* fe, double* ge) { return fg(ctx, np_, xe, fe, ge); };
+  HV h = [&](int np_, const double* v, double* y) { return hv(ctx, np_, v, y); };
+  return gauss_newton(nprob, n, x, lb, ub, f, h, sp, nullptr, r);
 }
 
 int glia_rd_invert(glia_rd_t h, int mode, double* x, const double* lb, const double* ub, const glia_opt_settings* s,
@@ -778,6 +1011,7 @@ int glia_rd_invert(glia_rd_t h, int mode, double* x, const double* lb, const dou
   };
   const int n = np + 2;
   if (mode == GLIA_RD_MODE_KAPPA_RHO) {
+    if (s && s->newton_solver == GLIA_OPT_GAUSS_NEWTON) return GLIA_RD_ERR_UNSUPPORTED;
     // RDOptimizer::solve (src/opt/RDOptimizer.cpp:170-250): beta = 0 during the solve, reference
     // gradient = the first one
     double beta = 0;
@@ -791,6 +1025,18 @@ int glia_rd_invert(glia_rd_t h, int mode, double* x, const double* lb, const dou
   std::vector<double> gref;
   int rc = gradient_norm_at_zero(nm, n, x, lb, ub, fg, gref);
   if (rc) return rc;
+  if (s && s->newton_solver == GLIA_OPT_GAUSS_NEWTON) {
+    // the Gauss-Newton Hessian with its kappa blocks (DerivativeOperatorsRD::evaluateHessian);
+    // rho is not an unknown of the p modes' Hessian, so mode 2 holds it
+    if (mode == GLIA_RD_MODE_ALL) return GLIA_RD_ERR_UNSUPPORTED;
+    int diffusivity = 0;  // the kappa blocks only where some member's kappa is free
+    for (int m = 0; m < nm; m++) {
+      const size_t k = (size_t)m * n + np;
+      diffusivity |= !(lb && ub && lb[k] >= ub[k]);
+    }
+    HV hv = [&](int, const double* v, double* y) { return glia_rd_hessian(h, diffusivity, v, y); };
+    return gauss_newton(nm, n, x, lb, ub, fg, hv, s, gref.data(), r);
+  }
   return blmvm(nm, n, x, lb, ub, fg, s, gref.data(), r);
 }
 

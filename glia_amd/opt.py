@@ -185,3 +185,45 @@ def rd_sparse_til(solver, x0, labels=None, weights=None, **kw):
                                      ctypes.cast(res, ctypes.c_void_p))
     solver._chk(rc)
     return solver._split(flat), _support(sup, np_), _til_results(res, nm)
+
+
+def gauss_newton(fg, hv, x0, lb=None, ub=None, **kw):
+    """Bound-constrained Gauss-Newton-Krylov over Python callbacks: fg(X) -> (f, G) as for
+    minimize; hv(V) -> H V at the point of the last fg call (V, result: (nprob, n)).  Returns
+    (x, results)."""
+    L = _lib.load()
+    x = np.array(x0, dtype=np.float64, ndmin=2, copy=True)
+    nprob, n = x.shape
+    lo, hi = _bounds(lb, x.shape), _bounds(ub, x.shape)
+    err = []
+
+    def cb(_ctx, npb, xp, fp, gp):
+        try:
+            X = np.ctypeslib.as_array(xp, shape=(npb * n,)).reshape(npb, n).copy()
+            f, g = fg(X)
+            np.ctypeslib.as_array(fp, shape=(npb,))[:] = np.asarray(f, dtype=np.float64).reshape(npb)
+            np.ctypeslib.as_array(gp, shape=(npb * n,))[:] = np.asarray(g, dtype=np.float64).reshape(-1)
+            return 0
+        except Exception as e:
+            err.append(e)
+            return 1
+
+    def hcb(_ctx, npb, vp, yp):
+        try:
+            V = np.ctypeslib.as_array(vp, shape=(npb * n,)).reshape(npb, n).copy()
+            np.ctypeslib.as_array(yp, shape=(npb * n,))[:] = np.asarray(hv(V), dtype=np.float64).reshape(-1)
+            return 0
+        except Exception as e:
+            err.append(e)
+            return 1
+
+    fn, hn = _lib.FG_FN(cb), _lib.HV_FN(hcb)
+    res = (_lib.OptResult * nprob)()
+    ptr = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+    rc = L.glia_gauss_newton(nprob, n, x.ctypes.data_as(ctypes.c_void_p), ptr(lo), ptr(hi), fn, hn, None,
+                             ctypes.byref(settings(**kw)), ctypes.cast(res, ctypes.c_void_p))
+    if err:
+        raise err[0]
+    if rc:
+        raise RuntimeError("glia_gauss_newton failed with code %d" % rc)
+    return x, _results(res, nprob)

@@ -55,6 +55,7 @@ class Optimizer {
   Vec getSolution() { return xout_; }
   PetscErrorCode evalGradient(Vec x, Vec dJ) { return derivative_operators_->evaluateGradient(dJ, x, data_); }
   PetscErrorCode evalObjective(Vec x, PetscReal* J) { return derivative_operators_->evaluateObjective(J, x, data_); }
+  PetscErrorCode evalHessian(Vec y, Vec x) { return derivative_operators_->evaluateHessian(y, x); }
   PetscErrorCode evalObjectiveAndGradient(Vec x, PetscReal* J, Vec dJ) {
     return derivative_operators_->evaluateObjectiveAndGradient(J, dJ, x, data_);
   }
@@ -74,6 +75,9 @@ class Optimizer {
     s.lbfgs_vectors = o.lbfgs_vectors_;
     s.ls_max_funcs = o.ls_max_func_evals;
     s.line_search = o.linesearch_;
+    s.newton_solver = o.newton_solver_;
+    s.krylov_maxit = o.krylov_maxit_;
+    s.forcing = o.fseqtype_;
     s.gnorm_ref = gnorm_ref;
     Vec xv = nullptr, gv = nullptr;
     VecCreateSeq(PETSC_COMM_SELF, (PetscInt)x.size(), &xv);
@@ -92,7 +96,22 @@ class Optimizer {
       std::copy(k->g->v.begin(), k->g->v.end(), g);
       return 0;
     };
-    PetscErrorCode rc = glia_blmvm(1, (int)x.size(), x.data(), lb_.data(), ub_.data(), fg, &ctx, &s, &result_);
+    PetscErrorCode rc;
+    if (o.newton_solver_ == GAUSSNEWTON) {
+      // hessianMatVec of the TAO BNLS solve (src/opt/TaoInterface.cpp:101-121), no preconditioner
+      auto hv = [](void* c, int, const double* va, double* ya) -> int {
+        Ctx* k = static_cast<Ctx*>(c);
+        k->x->ensure();
+        std::copy(va, va + k->x->n, k->x->v.begin());
+        PetscErrorCode e = k->self->evalHessian(k->g, k->x);
+        if (e) return e;
+        std::copy(k->g->v.begin(), k->g->v.end(), ya);
+        return 0;
+      };
+      rc = glia_gauss_newton(1, (int)x.size(), x.data(), lb_.data(), ub_.data(), fg, hv, &ctx, &s, &result_);
+    } else {
+      rc = glia_blmvm(1, (int)x.size(), x.data(), lb_.data(), ub_.data(), fg, &ctx, &s, &result_);
+    }
     VecDestroy(&xv);
     VecDestroy(&gv);
     return rc;

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <memory>
 
+#include "../glia_opt.h"
 #include "Vec.h"
 
 namespace glia {
@@ -42,6 +43,10 @@ struct TumorParameters {
   ScalarType thresh_component_weight_ = 1e-3;
 };
 
+/* the reference's forcing sequences and Newton solvers (include/utils/TypeDefs.h:10-12) */
+enum { QDFS = GLIA_OPT_FORCING_QUADRATIC, SLFS = GLIA_OPT_FORCING_SUPERLINEAR };
+enum { GAUSSNEWTON = GLIA_OPT_GAUSS_NEWTON, QUASINEWTON = GLIA_OPT_QUASI_NEWTON };
+
 /* OptimizerSettings defaults (include/Parameters.h:18-111) */
 struct OptimizerSettings {
   ScalarType beta_ = 1e-4;
@@ -52,6 +57,7 @@ struct OptimizerSettings {
   ScalarType opttolgrad_ = 1e-3, gatol_ = 1e-4;
   int newton_maxit_ = 30, newton_minit_ = 1;
   int linesearch_ = 1;  // 0 Armijo, 1 More-Thuente
+  int newton_solver_ = QUASINEWTON, krylov_maxit_ = 30, fseqtype_ = SLFS;
   int ls_max_func_evals = 10, lbfgs_vectors_ = 10;
   ScalarType k_lb_ = 1e-3, k_ub_ = 1.0, rho_lb_ = 1.0, rho_ub_ = 15.0;
   bool rescale_init_cond_ = false;

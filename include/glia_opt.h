@@ -31,6 +31,11 @@ enum {
 
 /* line searches (OptimizerSettings::linesearch_) */
 enum { GLIA_OPT_LS_ARMIJO = 0, GLIA_OPT_LS_MORE_THUENTE = 1 };
+/* Newton solvers, numbered as the reference's GAUSSNEWTON and QUASINEWTON
+   (include/utils/TypeDefs.h:12, OptimizerSettings::newton_solver_), and Krylov forcing sequences
+   numbered as QDFS and SLFS (:10, fseqtype_) */
+enum { GLIA_OPT_GAUSS_NEWTON = 0, GLIA_OPT_QUASI_NEWTON = 1 };
+enum { GLIA_OPT_FORCING_QUADRATIC = 0, GLIA_OPT_FORCING_SUPERLINEAR = 1 };
 
 typedef struct glia_opt_settings {
   int max_it;         /* OptimizerSettings::newton_maxit_ (30) */
@@ -46,6 +51,10 @@ typedef struct glia_opt_settings {
   double ls_xtol;     /* relative interval tolerance of the More-Thuente search (1e-10) */
   double gnorm_ref;   /* > 0: the reference gradient norm of the relative test (gradnorm0_) for
                          every problem; 0 (default): each problem's first projected gradient norm */
+  int newton_solver;  /* newton_solver_: GLIA_OPT_QUASI_NEWTON (BLMVM, default) or _GAUSS_NEWTON */
+  int krylov_maxit;   /* krylov_maxit_ (30): CG iterations per Gauss-Newton step */
+  int forcing;        /* fseqtype_: CG tolerance max(1e-10, min(0.5, sqrt(||g_F|| / ||g_F,0||)))
+                         (superlinear, default) or without the square root (quadratic) */
 } glia_opt_settings;
 
 typedef struct glia_opt_result {
@@ -68,6 +77,20 @@ typedef int (*glia_fg_fn)(void* ctx, int nprob, const double* x, double* f, doub
 int glia_blmvm(int nprob, int n, double* x, const double* lb, const double* ub, glia_fg_fn fg, void* ctx,
                const glia_opt_settings* s, glia_opt_result* r);
 
+/* Hessian-vector products of nprob problems at the point of the last fg call: y = H v (v, y:
+   nprob*n).  Returns 0, or nonzero to abort with that code. */
+typedef int (*glia_hv_fn)(void* ctx, int nprob, const double* v, double* y);
+
+/* Bound-constrained Gauss-Newton-Krylov (the reference's GAUSSNEWTON solver, TAO's BNLS with a CG
+   Krylov solve and the Eisenstat-Walker tolerance of preKrylovSolve, src/opt/TaoInterface.cpp:
+   160-193): per iteration the variables on an active bound are held, H_FF s_F = -g_F is solved by
+   CG (at most krylov_maxit iterations, the forcing tolerance of the settings), a non-descent
+   direction is replaced by the scaled steepest descent, and the settings' line search runs along
+   it; the stopping test is the one of glia_blmvm.  Batched like glia_blmvm: every CG iteration is
+   one hv call and every line-search round one fg call for all problems. */
+int glia_gauss_newton(int nprob, int n, double* x, const double* lb, const double* ub, glia_fg_fn fg, glia_hv_fn hv,
+                      void* ctx, const glia_opt_settings* s, glia_opt_result* r);
+
 /* The inversion of a handle: every member minimizes J over x = [p | kappa | rho] of its own
    (mode as for glia_rd_objective_and_gradient; the entries the mode does not invert keep their
    initial values since their gradient is 0).  All members need the same number of bases.
@@ -75,7 +98,10 @@ int glia_blmvm(int nprob, int n, double* x, const double* lb, const double* ub, 
    the solve and the relative test is against the first gradient.  The p modes are
    TILOptimizer::solve (src/opt/TILOptimizer.cpp:114-230): the relative test is against the
    gradient at x = 0 (src/opt/TaoInterface.cpp:285-300), taken over the entries with lb < ub (the
-   others keep their values). */
+   others keep their values).  With newton_solver = GLIA_OPT_GAUSS_NEWTON, mode
+   GLIA_RD_MODE_P_KAPPA runs glia_gauss_newton on glia_rd_hessian, with the kappa blocks unless every
+   member's kappa is fixed (lb >= ub); the other modes return GLIA_RD_ERR_UNSUPPORTED (the Hessian
+   has no rho row). */
 int glia_rd_invert(glia_rd_t h, int mode, double* x, const double* lb, const double* ub, const glia_opt_settings* s,
                    glia_opt_result* r);
 

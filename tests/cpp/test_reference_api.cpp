@@ -5,7 +5,7 @@
 //   "Running diffusion solver"       src/test/pdesolver.cpp:7-57 (||c||_2 = 2.0487, 5 CG iterations)
 //   "Evaluating objective function"  src/test/grad.cpp:7-80 (J = 0 at the truth, J > 0 otherwise)
 //
-//   TILOptimizer, RDOptimizer, SparseTILOptimizer on exact data (src/opt/*.cpp through include/glia/Optimizer.h)
+//   TILOptimizer (quasi- and Gauss-Newton), RDOptimizer, SparseTILOptimizer on exact data (src/opt/*.cpp through include/glia/Optimizer.h)
 //
 // Exit status: the number of failed checks.  Needs a GPU.
 #include <cfloat>
@@ -373,11 +373,13 @@ static void make_exact(TilSetup& s) {
   VecDestroy(&pstar);
 }
 
-static void test_til_optimizer() {
-  std::printf("TEST TILOptimizer (p and kappa) against glia_rd_invert\n");
+static void test_til_optimizer(int solver) {
+  std::printf("TEST TILOptimizer (p and kappa, %s) against glia_rd_invert\n",
+              solver == GAUSSNEWTON ? "Gauss-Newton" : "quasi-Newton");
   TilSetup s;
   make_exact(s);
   s.params->opt_->opttolgrad_ = 1e-5;
+  s.params->opt_->newton_solver_ = solver;
   TILOptimizer til;
   til.initialize(s.der, s.pde, s.params, s.tumor);
   til.setData(s.data);
@@ -425,6 +427,7 @@ static void test_til_optimizer() {
   glia_opt_settings os;
   glia_opt_settings_init(&os);
   os.gttol = 1e-5;
+  os.newton_solver = solver;
   glia_opt_result r;
   REQUIRE(glia_rd_invert(h, GLIA_RD_MODE_P_KAPPA, xa, lb, ub, &os, &r) == 0);
   glia_rd_destroy(h);
@@ -500,7 +503,8 @@ int main() {
   test_objective();
   test_interface_matches_abi_and_fd();
   test_phi_and_apply_d();
-  test_til_optimizer();
+  test_til_optimizer(QUASINEWTON);
+  test_til_optimizer(GAUSSNEWTON);
   test_rd_optimizer();
   test_sparse_til_optimizer();
   std::printf("%d checks, %d failed\n", checks, failures);

@@ -310,3 +310,128 @@ def test_more_thuente_and_armijo_both_reach_the_minimum():
 def test_more_thuente_defaults_match_the_reference():
     s = opt.settings()
     assert s.line_search == 1 and s.ls_gtol == 0.9 and s.ls_max_funcs == 10 and s.ls_sigma == 1e-4
+
+
+# ------------------------------------------------------------------ Gauss-Newton-Krylov ----
+def lsq_rosen(X):
+    """Rosenbrock as least squares, r = [10 (y - x^2), 1 - x], f = |r|^2: the Gauss-Newton
+    Hessian 2 J^T J is exact at the zero-residual minimum (1, 1)."""
+    x, y = X[:, 0], X[:, 1]
+    r = np.stack([10 * (y - x * x), 1 - x], axis=1)
+    J = np.zeros((len(x), 2, 2))
+    J[:, 0, 0], J[:, 0, 1], J[:, 1, 0] = -20 * x, 10.0, -1.0
+    return r, J
+
+
+def lsq_fg_hv():
+    state = {}
+
+    def fg(X):
+        r, J = lsq_rosen(X)
+        state["J"] = J
+        return np.sum(r * r, axis=1), 2 * np.einsum("mij,mi->mj", J, r)
+
+    def hv(V):
+        J = state["J"]
+        return 2 * np.einsum("mij,mik,mk->mj", J, J, V)
+    return fg, hv
+
+
+def test_gauss_newton_one_dimensional_problems_stop_after_the_exact_step():
+    """In one unknown the first conjugate-gradient step solves H s = -g exactly, and the full
+    step of the line search is the minimizer: the first iterate is c (clipped to the bounds)."""
+    a = np.array([[0.5], [3.0], [40.0]])
+    c = np.array([[2.0], [-1.5], [0.25]])
+    fg = lambda X: (0.5 * np.sum(a * (X - c) ** 2, axis=1), a * (X - c))
+    hv = lambda V: a * V
+    x, r = opt.gauss_newton(fg, hv, np.zeros((3, 1)), max_it=0, min_it=0, gatol=0.0, gttol=0.0)
+    assert np.allclose(x, c, rtol=1e-14, atol=1e-15)
+    assert all(q["iterations"] == 1 for q in r)
+    x, r = opt.gauss_newton(fg, hv, np.zeros((3, 1)), lb=[-1.0], ub=[1.0], max_it=0, min_it=0, gatol=0.0, gttol=0.0)
+    assert np.allclose(x, np.clip(c, -1.0, 1.0), rtol=1e-14, atol=1e-15)
+
+
+def test_gauss_newton_single_krylov_step_is_the_cauchy_point():
+    """krylov_maxit = 1 leaves one conjugate-gradient step, s = -(g.g / g.A g) g, the exact
+    minimizer of a quadratic along -g, which the line search takes at unit length."""
+    Q = np.array([[4.0, 1.0, 0.0], [1.0, 3.0, 0.5], [0.0, 0.5, 2.0]])
+    b = np.array([1.0, -2.0, 0.5])
+    fg = lambda X: (0.5 * np.einsum("mi,ij,mj->m", X, Q, X) - X @ b, X @ Q - b)
+    hv = lambda V: V @ Q
+    x0 = np.array([1.0, 1.0, -1.0])
+    g = Q @ x0 - b
+    want = x0 - (g @ g) / (g @ Q @ g) * g
+    x, r = opt.gauss_newton(fg, hv, x0, max_it=0, min_it=0, gatol=0.0, gttol=0.0, krylov_maxit=1)
+    assert r[0]["iterations"] == 1
+    assert np.allclose(x[0], want, rtol=1e-13, atol=1e-14)
+
+
+def test_gauss_newton_least_squares_rosenbrock():
+    """Unbounded from two starts (batched) to (1, 1), and with x <= 0.5 to (0.5, 0.25); the
+    inexact Newton steps (forcing term min(0.5, sqrt(|g| / |g0|))) take fewer iterations than
+    the quasi-Newton loop from the same start."""
+    fg, hv = lsq_fg_hv()
+    starts = [[-1.2, 1.0], [2.0, 2.0]]
+    x, r = opt.gauss_newton(fg, hv, starts, gatol=1e-10, gttol=1e-14, max_it=200)
+    assert np.allclose(x, 1.0, atol=1e-7), x
+    assert all(q["reason"] in ("gatol", "gttol") for q in r), r
+    _, rq = opt.minimize(lambda X: fg(X), starts, gatol=1e-10, gttol=1e-14, max_it=500)
+    assert r[1]["iterations"] < rq[1]["iterations"], (r, rq)
+    x, r = opt.gauss_newton(fg, hv, starts[0], lb=[-2, -2], ub=[0.5, 2], gatol=1e-10, gttol=1e-14, max_it=200)
+    assert np.allclose(x, [[0.5, 0.25]], atol=1e-8), x
+
+
+def test_gauss_newton_settings_follow_the_reference_numbering():
+    """GAUSSNEWTON = 0, QUASINEWTON = 1 and QDFS = 0, SLFS = 1 (include/utils/TypeDefs.h:10-12);
+    OptimizerSettings defaults: quasi-Newton, 30 Krylov iterations, superlinear forcing."""
+    s = opt.settings()
+    assert s.newton_solver == 1 and s.krylov_maxit == 30 and s.forcing == 1
+    fg, hv = lsq_fg_hv()
+    for forcing in (0, 1):
+        x, r = opt.gauss_newton(fg, hv, [[-1.2, 1.0]], gatol=1e-10, gttol=1e-14, max_it=200, forcing=forcing)
+        assert np.allclose(x, 1.0, atol=1e-7), (forcing, x)
+
+
+def test_gauss_newton_callback_errors_propagate():
+    fg, hv = lsq_fg_hv()
+
+    def bad(V):
+        raise ValueError("hv")
+    with pytest.raises(ValueError):
+        opt.gauss_newton(fg, bad, [[-1.2, 1.0]])
+
+
+def test_oracle_til_gauss_newton_recovers_the_initial_condition(oracle_mod):
+    """TILOptimizer with newton_solver = GAUSSNEWTON (src/opt/TILOptimizer.cpp) on exact data of a
+    32^3 problem: the Gauss-Newton Hessian matvec of the FP64 oracle (evaluateHessian, kappa
+    fixed) drives the loop from p = 0 to the truth within a few Newton steps."""
+    from glia_amd import synth
+    n, nt, dt = 32, 4, 0.05
+    pr = synth.problem(n, 5, nbases=6, kappa=0.05, rho=8.0, sigma=2 * np.pi / 32)
+    o = oracle_mod.Oracle(n)
+    o.set_tissue(pr["wm"], pr["gm"], pr["csf"], pr["vt"])
+    o.set_solver(1e-6, 5000)
+    o.set_time(dt, nt)
+    o.set_k(0.05, 0.2, 0.0)
+    o.set_rho(8.0, 0.2, 0.0)
+    o.prec_factor()
+    o.set_phi(pr["centers"], pr["sigma"], 1.0, False)
+    o.set_c0(o.phi_apply(pr["p"]))
+    o.solve_state(0)
+    o.set_data(o.ct(), None, -0.99)
+    o.set_time(dt, nt)
+    o.set_opt(beta=1e-6)
+
+    def fg(X):
+        f, g = o.objective_and_gradient(X[0], mode=0)
+        return np.array([f]), g[None]
+
+    def hv(V):
+        return o.hessian(V[0], False)[None]
+    x0 = np.concatenate([np.zeros(6), [0.05, 8.0]])
+    fixed = np.concatenate([np.full(6, -np.inf), [0.05, 8.0]])
+    x, r = opt.gauss_newton(fg, hv, x0, fixed, np.concatenate([np.full(6, np.inf), [0.05, 8.0]]),
+                            gatol=1e-9, gttol=1e-5, max_it=30)
+    assert r[0]["reason"] == "gttol" and r[0]["iterations"] <= 8, r
+    assert np.allclose(x[0][:6], pr["p"], atol=5e-5), x[0][:6] - pr["p"]
+    assert x[0][6] == 0.05 and x[0][7] == 8.0
