@@ -424,6 +424,73 @@ def invert_config(args):
     return 0
 
 
+def advect_config(args):
+    """The semi-Lagrangian transport step of the mass-effect models
+    (src/pde/PdeOperatorsMassEffect.cpp:605-620) at 256^3: a new velocity (the reference's test field,
+    src/test/helper.cpp:47-80), RK2 trajectories, div v, and six fields (gm, wm, vt, csf, c, mri with
+    modes 1, 1, 2, 2, 1, 2) advected in one pass.  The advection kernel is also timed alone (fresh
+    trajectories) for its roofline: per point it reads the offsets (12 B), div v at the grid and
+    departure points (8 B) and each field (4 B), and writes each field (4 B)."""
+    import torch
+    from glia_amd import adv
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
+    from adv_cases import reference_velocity
+    n, dt = 256, 0.02
+    ng = n ** 3
+    torch.manual_seed(11)
+    v = [torch.from_numpy(a.astype(np.float32).reshape(-1)).cuda() for a in reference_velocity(n)]
+    fields = [torch.rand(ng, device="cuda") for _ in range(6)]
+    outs = [torch.empty_like(f) for f in fields]
+    modes = [1, 1, 2, 2, 1, 2]
+    torch.cuda.synchronize()
+    # the solver's work and the timing events share one stream (default priority)
+    stream = torch.cuda.Stream()
+    s = adv.SemiLagrangianSolver(n, stream=stream.cuda_stream)
+
+    def step():
+        s.set_velocity(*v)
+        s.advect(fields, modes, dt, out=outs)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    # the advection kernel alone: trajectories and div v fresh, so each call is one launch
+    reps = 20
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        s.advect(fields, modes, dt, out=outs)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    k_ms = e0.elapsed_time(e1) / reps
+    s.close()
+    bytes_launch = ng * (12 + 8 + 8 * len(fields))
+    achieved = bytes_launch / (k_ms / 1000.0) / 1e9
+    line = {
+        "metric": "advection steps/s at 256^3 (6 fields, RK2 trajectories, cubic Lagrange)",
+        "value": round(1.0 / t_step, 3),
+        "unit": "steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_step * 1000, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "semi-Lagrangian step of the mass-effect model at 256^3: velocity update, RK2 "
+                               "trajectories, spectral div v, 6 fields advected in one pass (dt 0.02)",
+                   "n": n, "fields": len(fields), "modes": modes},
+        "roofline": {"bound": "hbm", "kernel": "sl_apply", "achieved": round(achieved, 1), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(achieved / 8000.0, 4),
+                     "algorithmic_bytes_per_launch": bytes_launch, "traffic": None},
+        "kernel_ms": round(k_ms, 4),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -439,11 +506,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--config", choices=["ensemble256", "batch", "diff", "invert"], default="ensemble256",
+    ap.add_argument("--config", choices=["ensemble256", "batch", "diff", "invert", "advect"], default="ensemble256",
                     help="ensemble256: the 256^3 members (default, the headline metric); batch: the "
                          "uneven-grid batching configuration; diff: the 256^3 differentiation "
                          "configuration (checkpointing, finite differences); invert: a batched kappa/rho "
-                         "inversion of 16 members at 128^3 through the inversion loop")
+                         "inversion of 16 members at 128^3 through the inversion loop; advect: the "
+                         "semi-Lagrangian step of the mass-effect models at 256^3")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device-to-device copy first (the reference dispatch for "
                          "scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -467,6 +535,8 @@ def main():
         return diff_config(args)
     if args.config == "invert":
         return invert_config(args)
+    if args.config == "advect":
+        return advect_config(args)
     n, M, nt, dt = args.n, args.members, args.nt, args.dt
     if args.copy_calibration:
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")

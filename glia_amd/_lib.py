@@ -187,6 +187,22 @@ class FileConfig(ctypes.Structure):
     ]
 
 
+# include/glia_adv.h (semi-Lagrangian advection)
+ADV_SIGNATURES = {
+    "glia_sl_create": (_i, [_vp, _vp, ctypes.POINTER(_vp)]),
+    "glia_sl_destroy": (_i, [_vp]),
+    "glia_sl_last_error": (ctypes.c_char_p, [_vp]),
+    "glia_sl_sync": (_i, [_vp]),
+    "glia_sl_set_velocity": (_i, [_vp, _vp, _vp, _vp, _i]),
+    "glia_sl_invalidate": (_i, [_vp]),
+    "glia_sl_compute_trajectories": (_i, [_vp, _d]),
+    "glia_sl_get_offsets": (_i, [_vp, _vp, _vp, _vp, _i]),
+    "glia_sl_interpolate": (_i, [_vp, _vp, _vp, _i]),
+    "glia_sl_solve": (_i, [_vp, _vp, _d, _i, _i]),
+    "glia_sl_advect": (_i, [_vp, _i, _vp, _vp, _vp, _d]),
+}
+
+
 KERNELS = ["xi_pupd", "yi_grad", "z_diff", "yf_div", "xf_matvec", "pcg_update"]
 # algorithmic bytes of one launch per member, in units of one field (4 n0 n1 n2 bytes): every
 # input read once and every output written once
@@ -200,13 +216,21 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
+    # torch ships its own HIP runtime under the soname of ROCm's (libamdhip64.so.7), and the first
+    # one loaded serves the whole process; loading torch first gives torch and this library one
+    # runtime, so device tensors can be passed to the library in any call order
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             "libglia_rd.so not found at %s: build it with `python -m glia_amd.build` "
             "(hipcc --offload-arch=gfx950); there is no CPU fallback" % LIB_PATH
         )
     lib = ctypes.CDLL(LIB_PATH)
-    for name, (res, args) in list(SIGNATURES.items()) + list(IO_SIGNATURES.items()) + list(OPT_SIGNATURES.items()):
+    tables = [SIGNATURES, IO_SIGNATURES, OPT_SIGNATURES, ADV_SIGNATURES]
+    for name, (res, args) in [kv for t in tables for kv in t.items()]:
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args

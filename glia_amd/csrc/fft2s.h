@@ -33,7 +33,7 @@ __device__ __forceinline__ float2 mul_i(float2 a) { return make_float2(-a.y, a.x
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }  // -i * a
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 
-// cos(2 pi j / 48); every radix used here (2, 3, 4, 6, 8, 12, 16, 24) divides 48.  Non-recursive so
+// cos(2 pi j / 48); the radices 2, 3, 4, 6, 8, 12, 16 and 24 divide 48.  Non-recursive so
 // that it folds to a constant wherever j is a compile-time constant.
 __device__ __forceinline__ constexpr float c48q(int q) {  // q in [0, 12]
   return q == 0 ? 1.0f : q == 1 ? 0.99144486137381041f : q == 2 ? 0.96592582628906829f
@@ -48,10 +48,26 @@ __device__ __forceinline__ constexpr float c48(int j) {
 }
 __device__ __forceinline__ constexpr float s48(int j) { return c48(j - 12); }
 
-// W_N^m with sign S: exp(S * 2 pi i m / N)
+// cos(2 pi j / 20) for the radices 5, 10 and 20
+__device__ __forceinline__ constexpr float c20q(int q) {  // q in [0, 5]
+  return q == 0 ? 1.0f : q == 1 ? 0.95105651629515357f : q == 2 ? 0.80901699437494742f
+       : q == 3 ? 0.58778525229247313f : q == 4 ? 0.30901699437494742f : 0.0f;
+}
+__device__ __forceinline__ constexpr float c20(int j) {
+  const int m = ((j % 20) + 20) % 20;
+  const int q = (m <= 10) ? m : 20 - m;
+  return (q <= 5) ? c20q(q) : -c20q(10 - q);
+}
+__device__ __forceinline__ constexpr float s20(int j) { return c20(j - 5); }
+
+// W_N^m with sign S: exp(S * 2 pi i m / N), for N dividing 48 or 20
 template <int N, int S>
 __device__ __forceinline__ constexpr float2 wn(int m) {
-  return make_float2(c48(m * (48 / N)), (S < 0 ? -1.0f : 1.0f) * s48(m * (48 / N)));
+  static_assert(48 % N == 0 || 20 % N == 0, "no twiddle constants for this radix");
+  if constexpr (48 % N == 0)
+    return make_float2(c48(m * (48 / N)), (S < 0 ? -1.0f : 1.0f) * s48(m * (48 / N)));
+  else
+    return make_float2(c20(m * (20 / N)), (S < 0 ? -1.0f : 1.0f) * s20(m * (20 / N)));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -102,6 +118,30 @@ struct Dft<4, S> {
   }
 };
 
+// radix 5: with t1 = x1 + x4, t2 = x2 + x3, t3 = x1 - x4, t4 = x2 - x3 and c_k, s_k the cosine and
+// sine of 2 pi k / 5, X1,4 = x0 + c1 t1 + c2 t2 -/+ S' i (s1 t3 + s2 t4) and
+// X2,3 = x0 + c2 t1 + c1 t2 -/+ S' i (s2 t3 - s1 t4), S' = 1 forward and -1 inverse
+template <int S>
+struct Dft<5, S> {
+  __device__ __forceinline__ static void run(float2* v) {
+    const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
+    const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;
+    const float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    const float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    const float2 a1 = make_float2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+    const float2 a2 = make_float2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+    float2 b1 = make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+    float2 b2 = make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+    b1 = (S < 0) ? mul_mi(b1) : mul_i(b1);
+    b2 = (S < 0) ? mul_mi(b2) : mul_i(b2);
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = cadd(a1, b1);
+    v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[3] = csub(a2, b2);
+  }
+};
+
 // N = R1 * R2 in registers: n = R2*n1 + n2, k = k1 + R1*k2
 template <int R1, int R2, int S>
 __device__ __forceinline__ void dft_split(float2* v) {
@@ -147,6 +187,14 @@ template <int S>
 struct Dft<24, S> {
   __device__ __forceinline__ static void run(float2* v) { dft_split<3, 8, S>(v); }
 };
+template <int S>
+struct Dft<10, S> {
+  __device__ __forceinline__ static void run(float2* v) { dft_split<5, 2, S>(v); }
+};
+template <int S>
+struct Dft<20, S> {
+  __device__ __forceinline__ static void run(float2* v) { dft_split<5, 4, S>(v); }
+};
 
 // ---------------------------------------------------------------------------------------------
 // plans: L = R1 * R2 with NT threads per line (NT divides R1 and R2)
@@ -163,8 +211,10 @@ GLIA_PLAN2(24, 6, 4, 2)
 GLIA_PLAN2(32, 4, 8, 4)
 GLIA_PLAN2(48, 12, 4, 4)
 GLIA_PLAN2(64, 8, 8, 8)
+GLIA_PLAN2(80, 20, 4, 4)
 GLIA_PLAN2(96, 12, 8, 4)
 GLIA_PLAN2(128, 8, 16, 8)
+GLIA_PLAN2(160, 20, 8, 4)
 GLIA_PLAN2(192, 24, 8, 8)
 GLIA_PLAN2(256, 16, 16, 16)
 #undef GLIA_PLAN2

@@ -1342,6 +1342,30 @@ __global__ void __launch_bounds__(256) k_axpby_c(const MemberDev* __restrict__ m
   }
 }
 
+// out = i (w_x X + w_y Y + w_z Z) on packed spectra: the spectrum of div (x, y, z) up to the 1/N^3
+// of the inverse (computeDivergence, src/grad/SpectralOperators.cpp:179-261).  w = k below n/2, k - n
+// above, 0 at n/2; the packed column kz = 0 holds the kz = 0 and Nyquist planes, both with w_z = 0.
+__global__ void __launch_bounds__(256) k_spec_div(const MemberDev* __restrict__ mem, Dims d, int s_x, int s_y, int s_z,
+                                                  int s_out) {
+  const MemberDev& M = mem[blockIdx.y];
+  const float2* x = craw(M, s_x);
+  const float2* y = craw(M, s_y);
+  const float2* z = craw(M, s_z);
+  float2* o = craw(M, s_out);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.nspec; i += stride) {
+    const int kz = (int)(i % d.m2);
+    const int64_t r = i / d.m2;
+    const int ky = (int)(r % d.n1), kx = (int)(r / d.n1);
+    const float wx = (2 * kx == d.n0) ? 0.0f : (float)(2 * kx < d.n0 ? kx : kx - d.n0);
+    const float wy = (2 * ky == d.n1) ? 0.0f : (float)(2 * ky < d.n1 ? ky : ky - d.n1);
+    const float wz = (float)kz;
+    const float2 a = x[i], b = y[i], c = z[i];
+    const float re = wx * a.x + wy * b.x + wz * c.x, im = wx * a.y + wy * b.y + wz * c.y;
+    o[i] = make_float2(-im, re);
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // launchers (size dispatch)
 // ---------------------------------------------------------------------------------------------
@@ -1357,8 +1381,10 @@ static int dispatch_L(int L, Args&&... args) {
     case 32: return F<32>::run(args...);
     case 48: return F<48>::run(args...);
     case 64: return F<64>::run(args...);
+    case 80: return F<80>::run(args...);
     case 96: return F<96>::run(args...);
     case 128: return F<128>::run(args...);
+    case 160: return F<160>::run(args...);
     case 192: return F<192>::run(args...);
     case 256: return F<256>::run(args...);
     default: return -1;
@@ -1373,13 +1399,16 @@ static int dispatch_M(int M, Args&&... args) {
     case 32: return F<32>::run(args...);
     case 48: return F<48>::run(args...);
     case 64: return F<64>::run(args...);
+    case 80: return F<80>::run(args...);
     case 96: return F<96>::run(args...);
     case 128: return F<128>::run(args...);
     default: return -1;
   }
 }
 
-bool supported_line(int L) { return L == 32 || L == 48 || L == 64 || L == 96 || L == 128 || L == 192 || L == 256; }
+bool supported_line(int L) {
+  return L == 32 || L == 48 || L == 64 || L == 80 || L == 96 || L == 128 || L == 160 || L == 192 || L == 256;
+}
 
 template <int L>
 struct RunXiGrad {
@@ -1505,6 +1534,7 @@ int z_blocks(const Dims& d) {
     case 32: return (int)((int64_t)d.n0 * d.n1 / ZK<32>::NL);
     case 48: return (int)((int64_t)d.n0 * d.n1 / ZK<48>::NL);
     case 64: return (int)((int64_t)d.n0 * d.n1 / ZK<64>::NL);
+    case 80: return (int)((int64_t)d.n0 * d.n1 / ZK<80>::NL);
     case 96: return (int)((int64_t)d.n0 * d.n1 / ZK<96>::NL);
     case 128: return (int)((int64_t)d.n0 * d.n1 / ZK<128>::NL);
     default: return -1;
@@ -1517,6 +1547,7 @@ int z_lines_per_block(int m2) {
     case 32: return ZK<32>::NL;
     case 48: return ZK<48>::NL;
     case 64: return ZK<64>::NL;
+    case 80: return ZK<80>::NL;
     case 96: return ZK<96>::NL;
     case 128: return ZK<128>::NL;
     default: return -1;
@@ -1640,6 +1671,12 @@ int launch_set_done(const LaunchCtx& c, int v) {
 }
 int launch_axpby_real(const LaunchCtx& c, int r_x, double a, int r_y, double b, int r_out) {
   k_axpby_real<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d, r_x, a, r_y, b, r_out);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+
+int launch_spec_div(const LaunchCtx& c, int s_x, int s_y, int s_z, int s_out) {
+  k_spec_div<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d, s_x, s_y, s_z, s_out);
   GLIA_CHECK_LAUNCH();
   return 0;
 }

@@ -31,7 +31,7 @@ namespace glia {
   } while (0)
 
 static bool supported_z(int n2) {
-  return n2 == 32 || n2 == 64 || n2 == 96 || n2 == 128 || n2 == 192 || n2 == 256;
+  return n2 == 32 || n2 == 64 || n2 == 96 || n2 == 128 || n2 == 160 || n2 == 192 || n2 == 256;
 }
 
 RDEnsemble::RDEnsemble(const SolverConfig& cfg) : cfg_(cfg) {
@@ -83,7 +83,7 @@ LaunchCtx RDEnsemble::ctx() const {
 int RDEnsemble::init() {
   if (nm_ < 1) return fail(-2, "nmembers must be >= 1");
   if (!supported_line(d_.n0) || !supported_line(d_.n1) || !supported_z(d_.n2))
-    return fail(-4, "unsupported grid: n0, n1 in {32,48,64,96,128,192,256}, n2 in {32,64,96,128,192,256}");
+    return fail(-4, "unsupported grid: n0, n1 in {32,48,64,80,96,128,160,192,256}, n2 in {32,64,96,128,160,192,256}");
   if (z_blocks(d_) <= 0 || ((int64_t)d_.n0 * d_.n1) % z_lines_per_block(d_.m2) != 0)
     return fail(-4, "unsupported grid: n0*n1 must be a multiple of the z-pass line group");
   if (cfg_.nt < 1 || !(cfg_.dt > 0)) return fail(-2, "nt >= 1 and dt > 0 required");
@@ -495,6 +495,15 @@ int RDEnsemble::inverse_to_real(int cslot, int work, int rslot) {
 
 int RDEnsemble::fft_r2c(int rslot, int cslot) { return forward_transform_real(rslot, cslot); }
 int RDEnsemble::fft_c2r(int cslot, int rslot) { return inverse_to_real(cslot, CS_T7, rslot); }
+
+int RDEnsemble::divergence(int rx, int ry, int rz, int rout) {
+  int rc;
+  if ((rc = forward_transform_real(rx, CS_T0)) || (rc = forward_transform_real(ry, CS_T1)) ||
+      (rc = forward_transform_real(rz, CS_T2)))
+    return rc;
+  GLIA_TRYC(launch_spec_div(ctx(), CS_T0, CS_T1, CS_T2, CS_T3));
+  return inverse_to_real(CS_T3, CS_T7, rout);
+}
 
 int RDEnsemble::get_spectrum(int m, int cslot, float2* host) {
   GLIA_HIP(hipMemcpyAsync(host, mh_[m].c[cslot], sizeof(float2) * d_.nspec, hipMemcpyDeviceToHost, stream_));

@@ -55,6 +55,8 @@ class RDEnsemble {
   int set_tissue_sinusoidal(int m);
   int set_field(int m, int rslot, const float* src, bool device);
   int get_field(int m, int rslot, float* dst, bool device);
+  float* field_ptr(int m, int rslot) const { return mh_[m].r[rslot]; }  // device address of a real slot
+  hipStream_t stream() const { return stream_; }
   int set_coefficients(int m, double kappa, double rho);     // DiffCoef/ReacCoef::setValues
   int set_kappa(int m, double kappa);
   int set_rho(int m, double rho);
@@ -79,6 +81,8 @@ class RDEnsemble {
   int apply_d_field(int rslot_in, int rslot_out);            // real D c (test/API)
   int fft_r2c(int rslot, int cslot);                         // packed spectrum of a real field
   int fft_c2r(int cslot, int rslot);                         // real field of a packed spectrum (1/N^3 included)
+  // out = div (x, y, z) of three real fields of every member, spectrally (uses CS_T0..CS_T3, CS_T7)
+  int divergence(int rx, int ry, int rz, int rout);
   int get_spectrum(int m, int cslot, float2* host);
   int set_spectrum(int m, int cslot, const float2* host);
   int solve_state(int linearized, bool keep_history);

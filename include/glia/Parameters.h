@@ -28,6 +28,7 @@ struct Grid {
 /* TumorParameters defaults (include/Parameters.h:157-258) */
 struct TumorParameters {
   int model_ = 1;
+  bool adv_velocity_set_ = false;
   int np_ = 1, nk_ = 1, nr_ = 1;
   int nt_ = 1;
   ScalarType dt_ = 0.5;

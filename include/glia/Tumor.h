@@ -295,6 +295,12 @@ class Tumor {
       VecCreateField(h_, 0, GLIA_RD_FIELD_CT, n, &c_t_);
       VecCreateField(h_, 0, GLIA_RD_FIELD_ALPHA0, n, &p_0_);
       VecCreateSeq(PETSC_COMM_SELF, (PetscInt)n, &p_t_);
+      // velocity and work field of the advection models (src/mat/Tumor.cpp:35-38)
+      const int model = params->tu_->model_;
+      if (params->tu_->adv_velocity_set_ || model == 4 || model == 5) {
+        velocity_ = std::make_shared<VecField>(n, n);
+        work_field_ = std::make_shared<VecField>(n, n);
+      }
     }
   }
   ~Tumor() {
@@ -334,6 +340,7 @@ class Tumor {
     return phi_ ? phi_->setValues(mat_prop_) : 0;
   }
 
+  std::shared_ptr<VecField> velocity_, work_field_;
   std::shared_ptr<DiffCoef> k_;
   std::shared_ptr<ReacCoef> rho_;
   std::shared_ptr<Phi> phi_;

@@ -170,6 +170,34 @@ inline PetscErrorCode VecRestoreArray(Vec x, ScalarType** p) {
   return x->push();
 }
 
+/* VecField (include/utils/Utils.h:127-148): three component vectors */
+class VecField {
+ public:
+  VecField(int64_t nl, int64_t /*ng*/) {
+    VecCreateSeq(PETSC_COMM_SELF, (PetscInt)nl, &x_);
+    VecCreateSeq(PETSC_COMM_SELF, (PetscInt)nl, &y_);
+    VecCreateSeq(PETSC_COMM_SELF, (PetscInt)nl, &z_);
+  }
+  ~VecField() {
+    VecDestroy(&x_);
+    VecDestroy(&y_);
+    VecDestroy(&z_);
+  }
+  VecField(const VecField&) = delete;
+  VecField& operator=(const VecField&) = delete;
+  PetscErrorCode set(ScalarType a) {
+    PetscErrorCode rc = VecSet(x_, a);
+    if (!rc) rc = VecSet(y_, a);
+    return rc ? rc : VecSet(z_, a);
+  }
+  PetscErrorCode scale(ScalarType a) {
+    PetscErrorCode rc = VecScale(x_, a);
+    if (!rc) rc = VecScale(y_, a);
+    return rc ? rc : VecScale(z_, a);
+  }
+  Vec x_ = nullptr, y_ = nullptr, z_ = nullptr;
+};
+
 }  // namespace glia
 
 #endif

@@ -61,7 +61,8 @@ enum {
 /* Parameters, TumorParameters, Grid and the DiffusionSolver KSP settings
    (include/Parameters.h:155-451; src/pde/DiffusionSolver.cpp:25-39) */
 typedef struct glia_rd_config {
-  int n[3];              /* grid; n0, n1 in {32,48,64,96,128,192,256}, n2 in {32,64,96,128,192,256} */
+  int n[3];              /* grid; n0, n1 in {32,48,64,80,96,128,160,192,256},
+                            n2 in {32,64,96,128,160,192,256} */
   int nmembers;          /* ensemble members sharing this grid (1 = plain solve) */
   double dt;             /* TumorParameters::dt_ */
   int nt;                /* TumorParameters::nt_ */

@@ -1091,3 +1091,106 @@ void orc_objective_and_gradient(orc_t *o, const double *x, int mode, double *J, 
   dJ[np] = gk;
   dJ[np + 1] = (mode == 0) ? 0.0 : gr;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Semi-Lagrangian advection (src/pde/AdvectionSolver.cpp:279-704) with the */
+/* tensor-product cubic Lagrange interpolation of the GPU path              */
+/* (cubicTex3DFastLagrange, src/cuda/InterpolationTexture.cu:270-366, on a  */
+/* periodic texture: cudaAddressModeWrap).  Query points are kept as        */
+/* offsets from their grid point, in grid units.                            */
+/* ------------------------------------------------------------------------ */
+
+/* lagrange_weights (3rdparty/interp_poly_kernel.cu:56-69): the cubic Lagrange basis on the nodes
+   -1, 0, 1, 2 at fraction t in [0, 1) */
+static void lag_weights(double t, double w[4]) {
+  w[0] = -t * (t - 1.0) * (t - 2.0) / 6.0;
+  w[1] = (t + 1.0) * (t - 1.0) * (t - 2.0) / 2.0;
+  w[2] = -(t + 1.0) * t * (t - 2.0) / 2.0;
+  w[3] = (t + 1.0) * t * (t - 1.0) / 6.0;
+}
+
+static int64_t wrap_index(int64_t v, int n) {
+  v %= n;
+  return v < 0 ? v + n : v;
+}
+
+/* f at grid coordinate (i + dx, j + dy, k + dz), periodic */
+static double interp_at(const orc_t *o, const double *f, int64_t i, int64_t j, int64_t k, double dx, double dy,
+                        double dz) {
+  const int n0 = o->n[0], n1 = o->n[1], n2 = o->n[2];
+  const double d[3] = {dx, dy, dz};
+  const int64_t b[3] = {i, j, k};
+  int64_t ix[3][4];
+  double w[3][4];
+  const int nn[3] = {n0, n1, n2};
+  for (int a = 0; a < 3; a++) {
+    const double fl = floor(d[a]);
+    lag_weights(d[a] - fl, w[a]);
+    for (int t = 0; t < 4; t++) ix[a][t] = wrap_index(b[a] + (int64_t)fl - 1 + t, nn[a]);
+  }
+  double s = 0.0;
+  for (int p = 0; p < 4; p++)
+    for (int q = 0; q < 4; q++) {
+      const double *row = f + (ix[0][p] * n1 + ix[1][q]) * n2;
+      double r = 0.0;
+      for (int t = 0; t < 4; t++) r += w[2][t] * row[ix[2][t]];
+      s += w[0][p] * w[1][q] * r;
+    }
+  return s;
+}
+
+/* SemiLagrangianSolver::interpolate (src/pde/AdvectionSolver.cpp:364-417): out(x) = f(x + q(x)) */
+void orc_sl_interp(orc_t *o, const double *f, const double *qx, const double *qy, const double *qz, double *out) {
+  const int n1 = o->n[1], n2 = o->n[2];
+#pragma omp parallel for schedule(static)
+  for (int64_t p = 0; p < o->ng; p++) {
+    const int64_t i = p / ((int64_t)n1 * n2), j = (p / n2) % n1, k = p % n2;
+    out[p] = interp_at(o, f, i, j, k, qx[p], qy[p], qz[p]);
+  }
+}
+
+/* computeTrajectories (src/pde/AdvectionSolver.cpp:470-590, single-GPU branch): Euler points
+   X* = x - dt v / h, v* = v(X*), departure points X = x - dt (v + v*) / (2 h); h = 2 pi / n per axis.
+   Returns X - x in grid units. */
+void orc_sl_trajectories(orc_t *o, const double *vx, const double *vy, const double *vz, double dt, double *qx,
+                         double *qy, double *qz) {
+  const int n1 = o->n[1], n2 = o->n[2];
+  const double s0 = dt * o->n[0] / (2.0 * M_PI), s1 = dt * o->n[1] / (2.0 * M_PI), s2 = dt * o->n[2] / (2.0 * M_PI);
+#pragma omp parallel for schedule(static)
+  for (int64_t p = 0; p < o->ng; p++) {
+    const int64_t i = p / ((int64_t)n1 * n2), j = (p / n2) % n1, k = p % n2;
+    const double ex = -s0 * vx[p], ey = -s1 * vy[p], ez = -s2 * vz[p];
+    const double wx = interp_at(o, vx, i, j, k, ex, ey, ez);
+    const double wy = interp_at(o, vy, i, j, k, ex, ey, ez);
+    const double wz = interp_at(o, vz, i, j, k, ex, ey, ez);
+    qx[p] = -0.5 * s0 * (vx[p] + wx);
+    qy[p] = -0.5 * s1 * (vy[p] + wy);
+    qz[p] = -0.5 * s2 * (vz[p] + wz);
+  }
+}
+
+/* SemiLagrangianSolver::solve (src/pde/AdvectionSolver.cpp:592-660) on the departure offsets q.
+   mode 2 (pure advection): c <- c(X).  mode 1 (mass conservation, d_t c + v.grad c = -c div v):
+   ci = c(X), di = (div v)(X), c* = ci - dt ci di, c <- ci + dt/2 (-ci di - c* div v). */
+void orc_sl_solve(orc_t *o, double *c, const double *vx, const double *vy, const double *vz, const double *qx,
+                  const double *qy, const double *qz, double dt, int mode) {
+  double *ci = (double *)malloc(sizeof(double) * o->ng);
+  orc_sl_interp(o, c, qx, qy, qz, ci);
+  if (mode == 2) {
+    memcpy(c, ci, sizeof(double) * o->ng);
+  } else {
+    double *dv = (double *)malloc(sizeof(double) * o->ng);
+    double *di = (double *)malloc(sizeof(double) * o->ng);
+    orc_div(o, vx, vy, vz, dv);
+    orc_sl_interp(o, dv, qx, qy, qz, di);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < o->ng; p++) {
+      const double src = -ci[p] * di[p];
+      const double cstar = ci[p] + dt * src;
+      c[p] = ci[p] + 0.5 * dt * (src - cstar * dv[p]);
+    }
+    free(dv);
+    free(di);
+  }
+  free(ci);
+}

@@ -43,6 +43,9 @@ def _load():
         "orc_c2r": (None, [vp, _cp, _dp]),
         "orc_grad": (None, [vp, _dp, _dp, _dp, _dp]),
         "orc_div": (None, [vp, _dp, _dp, _dp, _dp]),
+        "orc_sl_interp": (None, [vp, _dp, _dp, _dp, _dp, _dp]),
+        "orc_sl_trajectories": (None, [vp, _dp, _dp, _dp, d, _dp, _dp, _dp]),
+        "orc_sl_solve": (None, [vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, d, i]),
         "orc_smooth": (None, [vp, _dp, _dp, d]),
         "orc_set_tissue": (None, [vp, vp, vp, vp, vp, vp]),
         "orc_set_tissue_sinusoidal": (None, [vp]),
@@ -172,6 +175,25 @@ class Oracle:
     def div(self, dx, dy, dz):
         out = self._new()
         self.L.orc_div(self.h, self._f(dx), self._f(dy), self._f(dz), out)
+        return out.reshape(self.shape)
+
+    def sl_interp(self, f, q):
+        """f at the points x + q (q: three offset fields in grid units), cubic Lagrange, periodic."""
+        out = self._new()
+        self.L.orc_sl_interp(self.h, self._f(f), *[self._f(a) for a in q], out)
+        return out.reshape(self.shape)
+
+    def sl_trajectories(self, v, dt):
+        """Offsets (grid units) of the RK2 departure points of the velocity v over dt."""
+        q = [self._new() for _ in range(3)]
+        self.L.orc_sl_trajectories(self.h, *[self._f(a) for a in v], float(dt), *q)
+        return [a.reshape(self.shape) for a in q]
+
+    def sl_solve(self, c, v, q, dt, mode=2):
+        """One semi-Lagrangian step of c along v with departure offsets q (mode 2: pure advection,
+        1: mass conservation)."""
+        out = np.array(c, dtype=np.float64, order="C").reshape(-1).copy()
+        self.L.orc_sl_solve(self.h, out, *[self._f(a) for a in v], *[self._f(a) for a in q], float(dt), int(mode))
         return out.reshape(self.shape)
 
     def smooth(self, c, sigma):

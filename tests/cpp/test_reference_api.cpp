@@ -3,6 +3,7 @@
 // against the C ABI and against finite differences.
 //
 //   "Running diffusion solver"       src/test/pdesolver.cpp:7-57 (||c||_2 = 2.0487, 5 CG iterations)
+//   "Running advection solver"       src/test/pdesolver.cpp:59-108 (round trip below 1%)
 //   "Evaluating objective function"  src/test/grad.cpp:7-80 (J = 0 at the truth, J > 0 otherwise)
 //
 //   TILOptimizer (quasi- and Gauss-Newton), RDOptimizer, SparseTILOptimizer on exact data (src/opt/*.cpp through include/glia/Optimizer.h)
@@ -92,6 +93,61 @@ static void test_diffusion_solver() {
   for (int i = 0; i < 8; i++) VecDestroy(&work[i]);
   delete[] work;
   VecDestroy(&c);
+}
+
+// helper.cpp createTestField (src/test/helper.cpp:47-80)
+static PetscErrorCode createTestField(std::shared_ptr<VecField> v, std::shared_ptr<Parameters> params) {
+  ScalarType *x_ptr, *y_ptr, *z_ptr;
+  PetscErrorCode ierr;
+  if ((ierr = VecGetArray(v->x_, &x_ptr)) || (ierr = VecGetArray(v->y_, &y_ptr)) || (ierr = VecGetArray(v->z_, &z_ptr)))
+    return ierr;
+  const Grid& g = *params->grid_;
+  const ScalarType freq = 4;
+  for (int X = 0; X < g.isize_[0]; X++)
+    for (int Y = 0; Y < g.isize_[1]; Y++)
+      for (int Z = 0; Z < g.isize_[2]; Z++) {
+        const int64_t i = ((int64_t)X * g.isize_[1] + Y) * g.isize_[2] + Z;
+        const double a = freq * 2.0 * M_PI / g.n_[0] * X, b = freq * 2.0 * M_PI / g.n_[1] * Y,
+                     c = freq * 2.0 * M_PI / g.n_[2] * Z;
+        x_ptr[i] = 1 + 0.25 * std::sin(a) * std::sin(b) * std::sin(c);
+        y_ptr[i] = 1.5 + 0.1 * std::sin(a) * std::cos(b) * std::sin(c);
+        z_ptr[i] = 1 + std::cos(a) * std::cos(b) * std::sin(c);
+      }
+  VecRestoreArray(v->x_, &x_ptr);
+  VecRestoreArray(v->y_, &y_ptr);
+  return VecRestoreArray(v->z_, &z_ptr);
+}
+
+// ---- src/test/pdesolver.cpp:59-108 -----------------------------------------------------------
+static void test_advection_solver() {
+  std::printf("TEST Running advection solver\n");
+  auto params = std::make_shared<Parameters>();
+  auto spec_ops = std::make_shared<SpectralOperators>(CUFFT);
+  ScalarType norm_c, norm;
+  Vec c, c_copy;
+  initializeGrid(256, params);
+  params->tu_->model_ = 4;  // creates velocities
+  auto tumor = std::make_shared<Tumor>(params, spec_ops);
+  REQUIRE(tumor->velocity_ != nullptr);
+  createTestField(tumor->velocity_, params);
+  VecDuplicate(tumor->c_t_, &c);
+  VecDuplicate(tumor->c_t_, &c_copy);
+  VecSet(c, 0);
+  createTestFunction(c, params);
+  VecCopy(c, c_copy);
+  VecNorm(c, NORM_2, &norm_c);
+  auto adv_solver = std::make_shared<SemiLagrangianSolver>(params, tumor, spec_ops);
+  adv_solver->advection_mode_ = 2;  // pure advection
+  for (int i = 0; i < 4; i++) REQUIRE(adv_solver->solve(c, tumor->velocity_, 0.02) == 0);
+  tumor->velocity_->scale(-1);
+  adv_solver->trajectoryIsComputed_ = false;
+  for (int i = 0; i < 4; i++) REQUIRE(adv_solver->solve(c, tumor->velocity_, 0.02) == 0);
+  VecAXPY(c, -1, c_copy);
+  VecNorm(c, NORM_2, &norm);
+  std::printf("  ||c - c0|| / ||c0|| = %.4e\n", norm / norm_c);
+  REQUIRE(norm / norm_c < 1E-2);
+  VecDestroy(&c);
+  VecDestroy(&c_copy);
 }
 
 // ---- src/test/grad.cpp:7-80 ------------------------------------------------------------------
@@ -500,6 +556,7 @@ static void test_sparse_til_optimizer() {
 
 int main() {
   test_diffusion_solver();
+  test_advection_solver();
   test_objective();
   test_interface_matches_abi_and_fd();
   test_phi_and_apply_d();

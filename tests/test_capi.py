@@ -1,5 +1,6 @@
 """The C-ABI library loads on a machine without a GPU and exports every entry point that
-include/glia_rd.h, include/glia_io.h and include/glia_opt.h declare (no compute calls here)."""
+include/glia_rd.h, include/glia_io.h, include/glia_opt.h and include/glia_adv.h declare (no compute
+calls here)."""
 import ctypes
 import os
 import re
@@ -33,6 +34,7 @@ def test_library_exports_every_declared_symbol():
     assert set(declared()) <= exported
     assert set(declared("glia_io.h")) <= exported
     assert set(declared("glia_opt.h")) <= exported
+    assert set(declared("glia_adv.h")) <= exported
 
 
 def test_python_binding_covers_every_symbol():
@@ -40,6 +42,7 @@ def test_python_binding_covers_every_symbol():
     assert set(declared()) == set(_lib.SIGNATURES)
     assert set(declared("glia_io.h")) == set(_lib.IO_SIGNATURES)
     assert set(declared("glia_opt.h")) == set(_lib.OPT_SIGNATURES)
+    assert set(declared("glia_adv.h")) == set(_lib.ADV_SIGNATURES)
 
 
 def test_config_defaults_without_gpu():

@@ -30,7 +30,8 @@ def gauss_test_function(n):
     return np.exp(-((X[:, None, None] ** 2 + X[None, :, None] ** 2 + X[None, None, :] ** 2)) / R ** 2)
 
 
-@pytest.mark.parametrize("n", [(32, 32, 32), (64, 64, 64), (48, 96, 64), (96, 64, 96), (192, 64, 192), (64, 192, 64)])
+@pytest.mark.parametrize("n", [(32, 32, 32), (64, 64, 64), (48, 96, 64), (96, 64, 96), (192, 64, 192), (64, 192, 64),
+                               (160, 80, 160), (80, 160, 32)])
 def test_fft_r2c_c2r_vs_numpy(rd, n):
     print("case fft", n, flush=True)
     s = rd.RDSolver(n)
@@ -459,3 +460,52 @@ def test_forward_on_192_point_columns_vs_oracle(rd, oracle_mod):
     print("c(T) rel err", rel(ct, o.ct()), "its gpu", st["pcg_iterations"], "oracle", o.ksp()["state"], flush=True)
     assert rel(ct, o.ct()) < 1e-5
 
+
+
+def test_gradient_on_160_point_lines_vs_oracle(rd, oracle_mod):
+    """J and dJ on a 32 x 32 x 160 grid: the Z passes on 80-point packed lines (the radix-5 plan
+    20 x 4) against the FP64 oracle's mixed-radix transform."""
+    from glia_amd import synth
+    n = (32, 32, 160)
+    print("case gradient 32x32x160", flush=True)
+    pr = synth.problem(n, 4, nbases=6, kappa=0.05, rho=8.0, sigma=2 * np.pi / 64)
+    pr.update(n=n, nt=4, dt=0.02, k_gm_wm=0.2, r_gm_wm=0.2)
+    o = setup_oracle(oracle_mod, pr)
+    o.set_c0(o.phi_apply(pr["p"] * 1.3))
+    o.solve_state(0)
+    d1 = (o.ct() > 0.4).astype(np.float64)
+    x = np.concatenate([pr["p"], [pr["kappa"], pr["rho"]]])
+    o.set_data(d1, None, -0.99)
+    o.set_opt(beta=1e-3)
+    o.set_time(pr["dt"], pr["nt"])
+    Jo, go = o.objective_and_gradient(x, mode=2)
+    s = setup_gpu(rd, pr, beta=1e-3)
+    s.set_data(0, d1, None, -0.99)
+    J, g = s.objective_and_gradient([x])
+    s.close()
+    print("J", J[0], Jo, "rel g", rel(g[0], go), flush=True)
+    assert abs(J[0] - Jo) / abs(Jo) < 1e-5
+    assert rel(g[0], go) < 1e-4
+    for k in (-2, -1):
+        assert abs(g[0][k] - go[k]) / max(abs(go[k]), 1e-2 * np.linalg.norm(go)) < 1e-4
+
+
+@pytest.mark.parametrize("n", [(160, 32, 32), (32, 160, 32), (80, 80, 32)])
+def test_forward_on_radix5_columns_vs_oracle(rd, oracle_mod, n):
+    """c(T) with the X and Y passes on 160-point (20 x 8) and 80-point (20 x 4) column plans."""
+    from glia_amd import synth
+    print("case forward", n, flush=True)
+    pr = synth.problem(n, 5, nbases=6, kappa=0.05, rho=8.0, sigma=2 * np.pi / 64)
+    pr.update(n=n, nt=6, dt=0.02, k_gm_wm=0.2, r_gm_wm=0.2)
+    s = setup_gpu(rd, pr)
+    o = setup_oracle(oracle_mod, pr)
+    s.phi_apply([pr["p"]])
+    s.precFactor()
+    s.solve_state(0)
+    ct = s.get_field(0, "ct")
+    st = s.stats(0)
+    s.close()
+    o.set_c0(o.phi_apply(pr["p"]))
+    o.solve_state(0)
+    print("c(T) rel err", rel(ct, o.ct()), "its gpu", st["pcg_iterations"], "oracle", o.ksp()["state"], flush=True)
+    assert rel(ct, o.ct()) < 1e-5
