@@ -123,3 +123,27 @@ def test_batched_advection_equals_single_solves():
     with pytest.raises(Exception):
         s.advect([dev[0]], [1], 0.05, out=[dev[0]])
     s.close()
+
+
+def test_large_displacements_wrap_and_match_the_oracle(oracle_mod):
+    """A rough velocity of tens of cells per step sends departure points far from their grid
+    points and across the periodic boundary several times; the values still match the oracle's
+    interpolant at the GPU's own departure offsets."""
+    from glia_amd import adv
+    n = 32
+    print("case advection large displacements 32^3", flush=True)
+    rng = np.random.default_rng(9)
+    v = [rng.uniform(-40, 40, (n, n, n)) for _ in range(3)]
+    f = rng.standard_normal((n, n, n))
+    s = adv.SemiLagrangianSolver(n)
+    s.set_velocity(*v)
+    s.computeTrajectories(2 * np.pi / n)  # dt / h = 1: offsets of tens of cells
+    q = s.offsets()
+    got = s.interpolate(f.astype(np.float32))
+    s.close()
+    o = oracle_mod.Oracle(n)
+    want = o.sl_interp(f.astype(np.float32).astype(np.float64), [a.astype(np.float64) for a in q])
+    span = max(float(a.max() - a.min()) for a in q)
+    print("offset span", span, "rel", rel(got, want), flush=True)
+    assert span > 40
+    assert rel(got, want) < 1e-6
