@@ -165,6 +165,7 @@ OPT_SIGNATURES = {
     "glia_blmvm": (_i, [_i, _i, _vp, _vp, _vp, FG_FN, _vp, ctypes.POINTER(OptSettings), _vp]),
     "glia_rd_invert": (_i, [_vp, _i, _vp, _vp, _vp, ctypes.POINTER(OptSettings), _vp]),
     "glia_gauss_newton": (_i, [_i, _i, _vp, _vp, _vp, FG_FN, HV_FN, _vp, ctypes.POINTER(OptSettings), _vp]),
+    "glia_opt_parse_config": (_i, [ctypes.c_char_p, ctypes.POINTER(OptSettings), ctypes.POINTER(SparseTilSettings)]),
     "glia_hard_threshold": (_i, [_vp, _i, _i, _vp]),
     "glia_hard_threshold_components": (_i, [_vp, _i, _i, _vp, _vp, _i, _d, _i, _vp]),
     "glia_sparse_til_settings_init": (None, [ctypes.POINTER(SparseTilSettings)]),

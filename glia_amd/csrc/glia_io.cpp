@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/glia_io.h"
+#include "../../include/glia_opt.h"
 #include "../../include/glia_rd.h"
 
 namespace {
@@ -373,6 +374,55 @@ int glia_rd_parse_config(const char* path, glia_rd_file_config* o) {
               if (item.substr(0, e) == names[i]) o->atlas_labels[i] = std::stoi(item.substr(e + 1));
           pos = c + 1;
         }
+      }
+    } catch (...) {
+      return GLIA_RD_ERR_ARG;
+    }
+  }
+  return 0;
+}
+
+// the optimizer keys of setParameter (src/utils/IO.cpp:82-100, 171-172) over the values the caller
+// initialized; the same OptimizerSettings drive every solve of the reference, so they also go to
+// the TIL and reaction/diffusion stages of the sparse-TIL settings
+int glia_opt_parse_config(const char* path, glia_opt_settings* settings, glia_sparse_til_settings* sparse) {
+  std::ifstream f(path ? path : "");
+  if (!f) return GLIA_RD_ERR_ARG;
+  std::vector<glia_opt_settings*> opts;
+  if (settings) opts.push_back(settings);
+  if (sparse) {
+    opts.push_back(&sparse->til);
+    opts.push_back(&sparse->rd);
+  }
+  std::string line;
+  while (std::getline(f, line)) {
+    line.erase(std::remove_if(line.begin(), line.end(), [](unsigned char c) { return std::isspace(c); }), line.end());
+    const size_t hash = line.find('#');
+    if (hash != std::string::npos) line = line.substr(0, hash);
+    const size_t eq = line.find('=');
+    if (line.empty() || eq == std::string::npos) continue;
+    const std::string k = line.substr(0, eq), v = line.substr(eq + 1);
+    try {
+      for (glia_opt_settings* o : opts) {
+        if (k == "newton_solver") o->newton_solver = (v == "GN") ? GLIA_OPT_GAUSS_NEWTON : GLIA_OPT_QUASI_NEWTON;
+        else if (k == "line_search") o->line_search = (v == "armijo") ? GLIA_OPT_LS_ARMIJO : GLIA_OPT_LS_MORE_THUENTE;
+        else if (k == "opttol_grad") o->gttol = std::stod(v);
+        else if (k == "newton_maxit") o->max_it = std::stoi(v);
+        else if (k == "krylov_maxit") o->krylov_maxit = std::stoi(v);
+        else if (k == "lbfgs_vectors") o->lbfgs_vectors = std::stoi(v);
+        else if (k == "ls_max_func_evals") o->ls_max_funcs = std::stoi(v);
+      }
+      if (sparse) {
+        if (k == "gist_maxit") sparse->max_it = std::stoi(v);
+        else if (k == "kappa_lb") sparse->k_lb = std::stod(v);
+        else if (k == "kappa_ub") sparse->k_ub = std::stod(v);
+        else if (k == "rho_lb") sparse->rho_lb = std::stod(v);
+        else if (k == "rho_ub") sparse->rho_ub = std::stod(v);
+        else if (k == "beta_p") sparse->beta = std::stod(v);
+        else if (k == "invert_diff") sparse->diffusivity_inversion = std::stoi(v) > 0;
+        else if (k == "invert_reac") sparse->reaction_inversion = std::stoi(v) > 0;
+        else if (k == "sparsity_level") sparse->sparsity = std::stoi(v);
+        else if (k == "thresh_component_weight") sparse->thresh_component_weight = std::stod(v);
       }
     } catch (...) {
       return GLIA_RD_ERR_ARG;

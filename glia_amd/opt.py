@@ -110,6 +110,17 @@ def sparse_til_settings(til=None, rd=None, **kw):
     return s
 
 
+def parse_config(path, settings_in=None, sparse_in=None):
+    """The optimizer keys of a reference config file (setParameter, src/utils/IO.cpp:82-100,
+    171-172) over the defaults (or the given structs): (glia_opt_settings, glia_sparse_til_settings)."""
+    st = settings() if settings_in is None else settings_in
+    sp = sparse_til_settings() if sparse_in is None else sparse_in
+    rc = _lib.load().glia_opt_parse_config(str(path).encode(), ctypes.byref(st), ctypes.byref(sp))
+    if rc:
+        raise OSError("cannot parse config %s (code %d)" % (path, rc))
+    return st, sp
+
+
 def _til_results(res, nprob):
     one = lambda o: dict(iterations=o.iterations, fevals=o.fevals, reason=REASONS.get(o.reason, o.reason), f=o.f,
                          gnorm=o.gnorm, gnorm0=o.gnorm0)

@@ -167,6 +167,15 @@ int glia_sparse_til(int nprob, int np, double* x, int* support, const int* label
 int glia_rd_sparse_til(glia_rd_t h, double* x, int* support, const int* labels, const double* weights, int ncomp,
                        const glia_sparse_til_settings* s, glia_sparse_til_result* r);
 
+/* The optimizer keys of a reference config file (setParameter, src/utils/IO.cpp:82-100, 171-172):
+   newton_solver (GN / QN), line_search (armijo / mt), opttol_grad, newton_maxit, krylov_maxit,
+   lbfgs_vectors, ls_max_func_evals into *settings and into sparse->til and sparse->rd; gist_maxit,
+   kappa_lb/ub, rho_lb/ub, beta_p, invert_diff, invert_reac, sparsity_level and
+   thresh_component_weight into *sparse.  Values the file does not set keep what the structs hold
+   (initialize them with the *_init functions first); either pointer may be NULL.  A missing file
+   or an unreadable value is GLIA_RD_ERR_ARG. */
+int glia_opt_parse_config(const char* path, glia_opt_settings* settings, glia_sparse_til_settings* sparse);
+
 #ifdef __cplusplus
 }
 #endif

@@ -435,3 +435,27 @@ def test_oracle_til_gauss_newton_recovers_the_initial_condition(oracle_mod):
     assert r[0]["reason"] == "gttol" and r[0]["iterations"] <= 8, r
     assert np.allclose(x[0][:6], pr["p"], atol=5e-5), x[0][:6] - pr["p"]
     assert x[0][6] == 0.05 and x[0][7] == 8.0
+
+
+def test_optimizer_keys_of_a_reference_config_file(tmp_path):
+    """setParameter (src/utils/IO.cpp:82-100, 171-172) with the optimizer block of the reference's
+    config/test_sparsetil_config.txt; absent keys keep the OptimizerSettings defaults."""
+    p = tmp_path / "c.txt"
+    p.write_text("# optimizer\nnewton_solver=QN\nline_search=armijo\nopttol_grad=1e-05\nnewton_maxit=10\n"
+                 "krylov_maxit=1\ngist_maxit=2\nkappa_lb=0\nkappa_ub=0.1\nrho_lb=0\nrho_ub=15\n"
+                 "lbfgs_vectors=50\nls_max_func_evals=10\nsparsity_level=5\nbeta_p=1\ninvert_diff=1\n"
+                 "invert_reac=1\n")
+    st, sp = opt.parse_config(p)
+    for o in (st, sp.til, sp.rd):
+        assert (o.newton_solver, o.line_search, o.gttol, o.max_it, o.krylov_maxit) == (1, 0, 1e-5, 10, 1)
+        assert (o.lbfgs_vectors, o.ls_max_funcs, o.gatol) == (50, 10, 1e-4)
+    assert (sp.max_it, sp.k_lb, sp.k_ub, sp.rho_lb, sp.rho_ub, sp.sparsity, sp.beta) == (2, 0.0, 0.1, 0.0, 15.0, 5, 1.0)
+    assert sp.diffusivity_inversion == 1 and sp.reaction_inversion == 1 and sp.thresh_component_weight == 1e-3
+    p.write_text("newton_solver = GN  # Gauss-Newton\nline_search=mt\n")
+    st, sp = opt.parse_config(p)
+    assert st.newton_solver == 0 and st.line_search == 1 and sp.til.newton_solver == 0
+    p.write_text("newton_maxit=ten\n")
+    with pytest.raises(OSError):
+        opt.parse_config(p)
+    with pytest.raises(OSError):
+        opt.parse_config(tmp_path / "missing.txt")
