@@ -197,7 +197,8 @@ struct Dft<20, S> {
 };
 
 // ---------------------------------------------------------------------------------------------
-// plans: L = R1 * R2 with NT threads per line (NT divides R1 and R2)
+// plans: L = R1 * R2 with NT threads per line (NT divides R2; 160 = 20 x 8 splits its 20 stage-2
+// rows over 8 threads as 3, 3, 3, 3, 2, 2, 2, 2)
 // ---------------------------------------------------------------------------------------------
 template <int L>
 struct Plan2;
@@ -214,7 +215,7 @@ GLIA_PLAN2(64, 8, 8, 8)
 GLIA_PLAN2(80, 20, 4, 4)
 GLIA_PLAN2(96, 12, 8, 4)
 GLIA_PLAN2(128, 8, 16, 8)
-GLIA_PLAN2(160, 20, 8, 4)
+GLIA_PLAN2(160, 20, 8, 8)
 GLIA_PLAN2(192, 24, 8, 8)
 GLIA_PLAN2(256, 16, 16, 16)
 #undef GLIA_PLAN2
@@ -228,14 +229,17 @@ struct XPitch {
 };
 
 // One transform of a line of length R1*R2.  Input v[b*R1 + n1] = x[R2*n1 + n2] for the thread's
-// n2 = t + b*NT (b < R2/NT); output v[b*R2 + k2] = X[k1 + R1*k2] for k1 = t + b*NT (b < R1/NT).
+// n2 = t + b*NT (b < R2/NT, NT divides R2); output v[b*R2 + k2] = X[k1 + R1*k2] for k1 = t + b*NT
+// (b < ceil(R1/NT), k1 < R1: when NT does not divide R1 the last row is absent on some threads and
+// its slots are left as they are).
 // 'ex' is the line's exchange area (xpad layout); one barrier inside, and the caller must place a
 // barrier between two uses of the same exchange area.
 template <int R1, int R2, int NT, int S>
 __device__ __forceinline__ void fft2s(float2* v, float2* ex, int t, const float2* __restrict__ tw) {
   constexpr int L = R1 * R2;
-  constexpr int B1 = R2 / NT, B2 = R1 / NT;
-  static_assert(B1 * NT == R2 && B2 * NT == R1, "NT must divide R1 and R2");
+  constexpr int B1 = R2 / NT, B2 = (R1 + NT - 1) / NT;
+  static_assert(B1 * NT == R2, "NT must divide R2");
+  constexpr bool EVEN = B2 * NT == R1;  // otherwise the last stage-2 row exists on t < R1 % NT only
 #pragma unroll
   for (int b = 0; b < B1; b++) {
     Dft<R1, S>::run(v + b * R1);
@@ -254,9 +258,11 @@ __device__ __forceinline__ void fft2s(float2* v, float2* ex, int t, const float2
 #pragma unroll
   for (int b = 0; b < B2; b++) {
     const int k1 = t + b * NT;
+    if (EVEN || k1 < R1) {
 #pragma unroll
-    for (int n2 = 0; n2 < R2; n2++) v[b * R2 + n2] = ex[xpad(k1 * R2 + n2)];
-    Dft<R2, S>::run(v + b * R2);
+      for (int n2 = 0; n2 < R2; n2++) v[b * R2 + n2] = ex[xpad(k1 * R2 + n2)];
+      Dft<R2, S>::run(v + b * R2);
+    }
   }
   (void)L;
 }

@@ -168,7 +168,14 @@ template <int L>
 struct CK {
   static constexpr int R1 = Plan2<L>::R1, R2 = Plan2<L>::R2, NT = Plan2<L>::NT;
   static constexpr int TPB = 16 * NT;
-  static constexpr int VPT = L / NT;
+  // stage 1 gives each thread R2 / NT columns (NT divides R2); stage 2 gives it ceil(R1 / NT) rows,
+  // the last of which is missing on some threads when NT does not divide R1 (uneven plans)
+  static constexpr int B1 = R2 / NT, B2 = (R1 + NT - 1) / NT;
+  static constexpr bool EVEN = B2 * NT == R1;
+  static constexpr int VIN = R1 * B1, VOUT = R2 * B2;  // input and output values per thread
+  static constexpr int VPT = VIN > VOUT ? VIN : VOUT;
+  // output slot j = b * R2 + k2 holds stage-2 row k1 = t + b * NT, which exists when k1 < R1
+  __device__ __forceinline__ static bool out_ok(int t, int j) { return EVEN || t + (j / R2) * NT < R1; }
   static constexpr int CP = XPitch<L>::value;
   // a kernel's later operands are loaded at kernel start when the extra registers fit
   static constexpr bool PF = VPT <= 16;
@@ -254,20 +261,20 @@ k_xi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   // issued before the first use
   if (PUPD && first) {
 #pragma unroll
-    for (int j = 0; j < K::VPT; j++) a[j] = rr[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
+    for (int j = 0; j < K::VIN; j++) a[j] = rr[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
   } else if (PUPD) {
 #pragma unroll
-    for (int j = 0; j < K::VPT; j++) {
+    for (int j = 0; j < K::VIN; j++) {
       const unsigned g = (unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col;
       a[j] = rr[g];
       b[j] = pin[g];
     }
   } else {
 #pragma unroll
-    for (int j = 0; j < K::VPT; j++) a[j] = pin[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
+    for (int j = 0; j < K::VIN; j++) a[j] = pin[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
   }
 #pragma unroll
-  for (int j = 0; j < K::VPT; j++) {
+  for (int j = 0; j < K::VIN; j++) {
     const int row = in_row<R1, R2, NT>(t, j);
     const float wx = wnum(row, d.n0);
     float2 p = a[j];
@@ -285,7 +292,8 @@ k_xi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   __syncthreads();
   fft2s<R1, R2, NT, +1>(b, exc, t, twl);
 #pragma unroll
-  for (int j = 0; j < K::VPT; j++) {
+  for (int j = 0; j < K::VOUT; j++) {
+    if (!K::out_ok(t, j)) continue;
     const unsigned g = (unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col;
     U[g] = a[j];
     V[g] = b[j];
@@ -312,7 +320,7 @@ k_yi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   float2 a[K::VPT], b[K::VPT], u[K::PF ? K::VPT : 1];
   // both inputs are loaded up front (when registers allow) so that their latencies overlap
 #pragma unroll
-  for (int j = 0; j < K::VPT; j++) {
+  for (int j = 0; j < K::VIN; j++) {
     const unsigned g = (unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col;
     a[j] = V[g];
     if (K::PF) u[j] = U[g];
@@ -321,9 +329,9 @@ k_yi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   stage_twiddles<L, K::TPB>(twl, tw);
   fft2s<R1, R2, NT, +1>(a, exc, t, twl);
 #pragma unroll
-  for (int j = 0; j < K::VPT; j++) Gx[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
+  for (int j = 0; j < K::VOUT; j++) if (K::out_ok(t, j)) Gx[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
 #pragma unroll
-  for (int j = 0; j < K::VPT; j++) {
+  for (int j = 0; j < K::VIN; j++) {
     const int row = in_row<R1, R2, NT>(t, j);
     a[j] = K::PF ? u[j] : U[(unsigned)row * tl.rs + col];
     b[j] = mul_i(cscale(a[j], wnum(row, d.n1)));
@@ -333,7 +341,8 @@ k_yi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   __syncthreads();
   fft2s<R1, R2, NT, +1>(b, exc, t, twl);
 #pragma unroll
-  for (int j = 0; j < K::VPT; j++) {
+  for (int j = 0; j < K::VOUT; j++) {
+    if (!K::out_ok(t, j)) continue;
     const unsigned g = (unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col;
     Gz[g] = mul_i(cscale(a[j], wz));
     if (WITH0) G0[g] = a[j];
@@ -360,7 +369,7 @@ k_yf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
   float2 a[K::VPT], b[K::VPT], q[K::PF ? K::VPT : 1];
   // all three inputs are loaded up front (when registers allow) so that their latencies overlap
 #pragma unroll
-  for (int j = 0; j < K::VPT; j++) {
+  for (int j = 0; j < K::VIN; j++) {
     const unsigned g = (unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col;
     a[j] = Hx[g];
     if (K::PF) {
@@ -372,9 +381,9 @@ k_yf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
   stage_twiddles<L, K::TPB>(twl, tw);
   fft2s<R1, R2, NT, -1>(a, exc, t, twl);
 #pragma unroll
-  for (int j = 0; j < K::VPT; j++) Yx[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
+  for (int j = 0; j < K::VOUT; j++) if (K::out_ok(t, j)) Yx[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
 #pragma unroll
-  for (int j = 0; j < K::VPT; j++) {
+  for (int j = 0; j < K::VIN; j++) {
     const unsigned g = (unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col;
     a[j] = K::PF ? q[j] : Q[g];
     if (!K::PF) b[j] = Hy[g];
@@ -384,7 +393,8 @@ k_yf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
   __syncthreads();
   fft2s<R1, R2, NT, -1>(b, exc, t, twl);
 #pragma unroll
-  for (int j = 0; j < K::VPT; j++) {
+  for (int j = 0; j < K::VOUT; j++) {
+    if (!K::out_ok(t, j)) continue;
     const int row = out_row<R1, R2, NT>(t, j);
     Rr[(unsigned)row * tl.rs + col] = cadd(a[j], mul_i(cscale(b[j], wnum(row, d.n1))));
   }
@@ -417,7 +427,7 @@ k_xf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
   constexpr bool PF = K::VPT <= GLIA_XF_PF_VPT && GLIA_XF_PF;
   float2 a[K::VPT], b[K::VPT], pv[PF ? K::VPT : 1];
 #pragma unroll
-  for (int j = 0; j < K::VPT; j++) {
+  for (int j = 0; j < K::VIN; j++) {
     const unsigned g = (unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col;
     a[j] = Yx[g];
     b[j] = Rr[g];
@@ -426,7 +436,7 @@ k_xf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
   // keep its loads behind them)
   if (MODE != XF_D && PF) {
 #pragma unroll
-    for (int j = 0; j < K::VPT; j++) pv[j] = P[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col];
+    for (int j = 0; j < K::VOUT; j++) if (K::out_ok(t, j)) pv[j] = P[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col];
   }
   __shared__ float2 twl[L];
   stage_twiddles<L, K::TPB>(twl, tw);
@@ -435,7 +445,8 @@ k_xf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
   fft2s<R1, R2, NT, -1>(b, exc, t, twl);
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-  for (int j = 0; j < K::VPT; j++) {
+  for (int j = 0; j < K::VOUT; j++) {
+    if (!K::out_ok(t, j)) continue;
     const int row = out_row<R1, R2, NT>(t, j);
     const unsigned g = (unsigned)row * tl.rs + col;
     const float wx = wnum(row, d.n0);
@@ -478,12 +489,12 @@ k_col_plain(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict_
   GPtr O = cptr(M, s_out) + tl.base;
   float2 a[K::VPT];
 #pragma unroll
-  for (int j = 0; j < K::VPT; j++) a[j] = I[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
+  for (int j = 0; j < K::VIN; j++) a[j] = I[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
   __shared__ float2 twl[L];
   stage_twiddles<L, K::TPB>(twl, tw);
   fft2s<R1, R2, NT, S>(a, exc, t, twl);
 #pragma unroll
-  for (int j = 0; j < K::VPT; j++) O[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
+  for (int j = 0; j < K::VOUT; j++) if (K::out_ok(t, j)) O[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
 }
 
 // ---------------------------------------------------------------------------------------------
