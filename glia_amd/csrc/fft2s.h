@@ -197,8 +197,9 @@ struct Dft<20, S> {
 };
 
 // ---------------------------------------------------------------------------------------------
-// plans: L = R1 * R2 with NT threads per line (NT divides R2; 160 = 20 x 8 splits its 20 stage-2
-// rows over 8 threads as 3, 3, 3, 3, 2, 2, 2, 2)
+// plans: L = R1 * R2 with NT threads per line.  NT divides R2; when it does not divide R1 the
+// stage-2 rows are split unevenly: 160 = 20 x 8 gives 8 threads 3, 3, 3, 3, 2, 2, 2, 2 rows, and
+// 192 = 12 x 16 gives 12 of 16 threads one row.
 // ---------------------------------------------------------------------------------------------
 template <int L>
 struct Plan2;
@@ -216,7 +217,7 @@ GLIA_PLAN2(80, 20, 4, 4)
 GLIA_PLAN2(96, 12, 8, 4)
 GLIA_PLAN2(128, 8, 16, 8)
 GLIA_PLAN2(160, 20, 8, 8)
-GLIA_PLAN2(192, 24, 8, 8)
+GLIA_PLAN2(192, 12, 16, 16)
 GLIA_PLAN2(256, 16, 16, 16)
 #undef GLIA_PLAN2
 
