@@ -131,7 +131,7 @@ def test_phi_vs_oracle(rd, oracle_mod):
 
 
 def test_forward_c1_vs_oracle(rd, oracle_mod):
-    """64^3, nt = 50, dt = 0.02, c(T) within 1e-5 of the FP64 oracle."""
+    """The 64^3 forward workload (nt = 50, dt = 0.02, FP64 reference): c(T) within 1e-5 of the oracle."""
     pr = c1_problem()
     print("case forward c1", flush=True)
     s = setup_gpu(rd, pr)
