@@ -491,6 +491,53 @@ def advect_config(args):
     return 0
 
 
+def elastic_config(args):
+    """The linear-elasticity solve of the mass-effect model (VariableLinearElasticitySolver,
+    src/pde/ElasticitySolver.cpp:471-533) at 160^3, the mass-effect grid: synthetic tissue with a
+    background, a Gaussian tumor, material properties from them, and the force along the tumor
+    gradient; one step = one CG solve from u = 0 (rtol 1e-3 in the preconditioned norm).
+    Algorithmic bytes per CG iteration: 188 fields of 4 N^3 bytes (operator 116: 20 transforms,
+    the spectral combinations, 7 products, 3 screened updates and the staging copies;
+    preconditioner 30; three dot products 15; the CG updates 27)."""
+    import torch
+    from glia_amd import elasticity
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
+    from el_cases import tissue_and_tumor, tumor_force
+    n = 160
+    fields = [a.astype(np.float32) for a in tissue_and_tumor(n)]
+    f = tumor_force(fields[0].astype(np.float64)).astype(np.float32)
+    s = elasticity.ElasticitySolver(n)
+    s.set_material(*fields)
+    for _ in range(args.warmup):
+        s.solve(f)
+    t0 = time.perf_counter()
+    its = []
+    for _ in range(args.steps):
+        u, k, reason, rn = s.solve(f)
+        its.append(k)
+    t = (time.perf_counter() - t0) / args.steps
+    s.close()
+    R = 4.0 * n ** 3
+    bytes_it = 188 * R
+    achieved = bytes_it * np.mean(its) / t / 1e9
+    line = {
+        "metric": "elasticity solves/s at 160^3 (CG, spectral preconditioner)",
+        "value": round(1.0 / t, 3), "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t * 1000, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "linear elasticity of the mass-effect model at 160^3: material properties from "
+                               "synthetic tissue and a Gaussian tumor, force along the tumor gradient, KSPCG "
+                               "rtol 1e-3 from u = 0", "n": n},
+        "cg_iterations": its, "reason": reason, "ms_per_iteration": round(t * 1000 / np.mean(its), 3),
+        "roofline": {"bound": "hbm", "kernel": "whole CG iteration", "achieved": round(achieved, 1), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(achieved / 8000.0, 4), "algorithmic_bytes_per_iteration": bytes_it,
+                     "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -506,12 +553,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--config", choices=["ensemble256", "batch", "diff", "invert", "advect"], default="ensemble256",
+    ap.add_argument("--config", choices=["ensemble256", "batch", "diff", "invert", "advect", "elastic"],
+                    default="ensemble256",
                     help="ensemble256: the 256^3 members (default, the headline metric); batch: the "
                          "uneven-grid batching configuration; diff: the 256^3 differentiation "
                          "configuration (checkpointing, finite differences); invert: a batched kappa/rho "
                          "inversion of 16 members at 128^3 through the inversion loop; advect: the "
-                         "semi-Lagrangian step of the mass-effect models at 256^3")
+                         "semi-Lagrangian step of the mass-effect models at 256^3; elastic: the "
+                         "elasticity solve of the mass-effect model at 160^3")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device-to-device copy first (the reference dispatch for "
                          "scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -537,6 +586,8 @@ def main():
         return invert_config(args)
     if args.config == "advect":
         return advect_config(args)
+    if args.config == "elastic":
+        return elastic_config(args)
     n, M, nt, dt = args.n, args.members, args.nt, args.dt
     if args.copy_calibration:
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")

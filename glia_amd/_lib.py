@@ -204,6 +204,25 @@ ADV_SIGNATURES = {
 }
 
 
+# include/glia_el.h (linear elasticity)
+class ElParams(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ("E_healthy", "nu_healthy", "E_tumor", "nu_tumor", "E_bg", "nu_bg",
+                                                 "E_csf", "nu_csf", "screen_low", "screen_high")]
+
+
+EL_SIGNATURES = {
+    "glia_el_params_init": (None, [ctypes.POINTER(ElParams)]),
+    "glia_el_create": (_i, [_vp, _vp, ctypes.POINTER(_vp)]),
+    "glia_el_destroy": (_i, [_vp]),
+    "glia_el_last_error": (ctypes.c_char_p, [_vp]),
+    "glia_el_set_material": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(ElParams), _i]),
+    "glia_el_get_material": (_i, [_vp, _vp, _vp, _vp, _vp, _i]),
+    "glia_el_apply": (_i, [_vp, _vp, _vp, _i]),
+    "glia_el_precondition": (_i, [_vp, _vp, _vp, _i]),
+    "glia_el_solve": (_i, [_vp, _vp, _vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_d)]),
+}
+
+
 KERNELS = ["xi_pupd", "yi_grad", "z_diff", "yf_div", "xf_matvec", "pcg_update"]
 # algorithmic bytes of one launch per member, in units of one field (4 n0 n1 n2 bytes): every
 # input read once and every output written once
@@ -230,7 +249,7 @@ def load():
             "(hipcc --offload-arch=gfx950); there is no CPU fallback" % LIB_PATH
         )
     lib = ctypes.CDLL(LIB_PATH)
-    tables = [SIGNATURES, IO_SIGNATURES, OPT_SIGNATURES, ADV_SIGNATURES]
+    tables = [SIGNATURES, IO_SIGNATURES, OPT_SIGNATURES, ADV_SIGNATURES, EL_SIGNATURES]
     for name, (res, args) in [kv for t in tables for kv in t.items()]:
         f = getattr(lib, name)
         f.restype = res

@@ -1,0 +1,135 @@
+// el_kernels.hip -- pointwise and reduction kernels of the linear-elasticity solver of the
+// mass-effect model (VariableLinearElasticitySolver, src/pde/ElasticitySolver.cpp:6-533).  The
+// spectral parts (transforms, the strain/force combinations, the preconditioner) run on the RD
+// engine's packed spectra; these kernels work on plain FP32 fields of ng values.
+#include <hip/hip_runtime.h>
+
+#include "el_launch.h"
+
+namespace glia {
+namespace el {
+
+constexpr int EB = 1024;  // blocks of the grid-stride kernels (the partial-sum count)
+
+// computeMaterialProperties (src/pde/ElasticitySolver.cpp:376-455) with computeTumorLame and
+// computeScreening (src/cuda/ElasticitySolver.cu:13-34); partial sums of mu, lam and screen per block
+__global__ void __launch_bounds__(256) k_el_material(int64_t ng, const float* __restrict__ c, const float* __restrict__ wm,
+                                                     const float* __restrict__ gm, const float* __restrict__ csf,
+                                                     const float* __restrict__ vt, const float* __restrict__ bg,
+                                                     ElMaterial m, float* __restrict__ mu, float* __restrict__ lam,
+                                                     float* __restrict__ scr, double* __restrict__ part) {
+  double s[3] = {0.0, 0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += stride) {
+    float a = m.mu_bg * bg[i] + m.mu_h * wm[i] + m.mu_h * gm[i] + m.mu_c * vt[i] + m.mu_c * csf[i];
+    float b = m.lam_bg * bg[i] + m.lam_h * wm[i] + m.lam_h * gm[i] + m.lam_c * vt[i] + m.lam_c * csf[i];
+    a = a > 0.0f ? a : 0.0f;
+    b = b > 0.0f ? b : 0.0f;
+    const float ci = c[i];
+    if (ci > 0.0f) {
+      a += m.mu_t * ci;
+      b += m.lam_t * ci;
+    }
+    float sc = (ci >= 0.005f) ? m.screen_low : m.screen_high;
+    if (bg[i] > 0.95f) sc = 1e6f;
+    mu[i] = a;
+    lam[i] = b;
+    scr[i] = sc;
+    s[0] += a;
+    s[1] += b;
+    s[2] += sc;
+  }
+  __shared__ double red[3][256];
+  for (int k = 0; k < 3; k++) red[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 3; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// out = a * b
+__global__ void __launch_bounds__(256) k_mul(int64_t ng, const float* __restrict__ a, const float* __restrict__ b,
+                                             float* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += stride) out[i] = a[i] * b[i];
+}
+
+// out = a - b * c
+__global__ void __launch_bounds__(256) k_fnms(int64_t ng, const float* __restrict__ a, const float* __restrict__ b,
+                                              const float* __restrict__ c, float* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += stride) out[i] = a[i] - b[i] * c[i];
+}
+
+// per-block FP64 partials of <x, y> over the three components
+__global__ void __launch_bounds__(256) k_dot3(int64_t ng, Vec3C x, Vec3C y, double* __restrict__ part) {
+  double s = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int k = 0; k < 3; k++)
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += stride)
+      s += (double)x.p[k][i] * (double)y.p[k][i];
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// x += a p and r -= a w (the CG update)
+__global__ void __launch_bounds__(256) k_cg_update(int64_t ng, Vec3 x, Vec3 r, Vec3C p, Vec3C w, double a) {
+  const float fa = (float)a;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int k = 0; k < 3; k++)
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += stride) {
+      x.p[k][i] += fa * p.p[k][i];
+      r.p[k][i] -= fa * w.p[k][i];
+    }
+}
+
+// p = z + b p (b = 0: p = z); y = x - y when sub (the initial residual b - A x)
+__global__ void __launch_bounds__(256) k_axpby3(int64_t ng, Vec3 p, Vec3C z, double b, int sub) {
+  const float fb = (float)b;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int k = 0; k < 3; k++)
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += stride)
+      p.p[k][i] = sub ? z.p[k][i] - p.p[k][i] : z.p[k][i] + fb * p.p[k][i];
+}
+
+int partial_count() { return EB; }
+
+int launch_material(hipStream_t st, int64_t ng, const float* c, const float* wm, const float* gm, const float* csf,
+                    const float* vt, const float* bg, const ElMaterial& m, float* mu, float* lam, float* scr,
+                    double* part) {
+  k_el_material<<<EB, 256, 0, st>>>(ng, c, wm, gm, csf, vt, bg, m, mu, lam, scr, part);
+  return (int)hipGetLastError();
+}
+int launch_mul(hipStream_t st, int64_t ng, const float* a, const float* b, float* out) {
+  k_mul<<<EB, 256, 0, st>>>(ng, a, b, out);
+  return (int)hipGetLastError();
+}
+int launch_fnms(hipStream_t st, int64_t ng, const float* a, const float* b, const float* c, float* out) {
+  k_fnms<<<EB, 256, 0, st>>>(ng, a, b, c, out);
+  return (int)hipGetLastError();
+}
+int launch_dot3(hipStream_t st, int64_t ng, const Vec3C& x, const Vec3C& y, double* part) {
+  k_dot3<<<EB, 256, 0, st>>>(ng, x, y, part);
+  return (int)hipGetLastError();
+}
+int launch_cg_update(hipStream_t st, int64_t ng, const Vec3& x, const Vec3& r, const Vec3C& p, const Vec3C& w,
+                     double a) {
+  k_cg_update<<<EB, 256, 0, st>>>(ng, x, r, p, w, a);
+  return (int)hipGetLastError();
+}
+int launch_axpby3(hipStream_t st, int64_t ng, const Vec3& p, const Vec3C& z, double b, int sub) {
+  k_axpby3<<<EB, 256, 0, st>>>(ng, p, z, b, sub);
+  return (int)hipGetLastError();
+}
+
+}  // namespace el
+}  // namespace glia

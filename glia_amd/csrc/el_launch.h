@@ -1,0 +1,33 @@
+// el_launch.h -- launchers of the elasticity kernels (el_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace glia {
+namespace el {
+
+// Lame parameters of the tissue classes and the screening coefficients
+struct ElMaterial {
+  float mu_h, mu_t, mu_bg, mu_c, lam_h, lam_t, lam_bg, lam_c, screen_low, screen_high;
+};
+struct Vec3 {
+  float* p[3];
+};
+struct Vec3C {
+  const float* p[3];
+};
+
+int partial_count();
+int launch_material(hipStream_t st, int64_t ng, const float* c, const float* wm, const float* gm, const float* csf,
+                    const float* vt, const float* bg, const ElMaterial& m, float* mu, float* lam, float* scr,
+                    double* part);
+int launch_mul(hipStream_t st, int64_t ng, const float* a, const float* b, float* out);
+int launch_fnms(hipStream_t st, int64_t ng, const float* a, const float* b, const float* c, float* out);
+int launch_dot3(hipStream_t st, int64_t ng, const Vec3C& x, const Vec3C& y, double* part);
+int launch_cg_update(hipStream_t st, int64_t ng, const Vec3& x, const Vec3& r, const Vec3C& p, const Vec3C& w,
+                     double a);
+int launch_axpby3(hipStream_t st, int64_t ng, const Vec3& p, const Vec3C& z, double b, int sub);
+
+}  // namespace el
+}  // namespace glia

@@ -1377,6 +1377,60 @@ __global__ void __launch_bounds__(256) k_spec_div(const MemberDev* __restrict__ 
   }
 }
 
+// out = i sum_k w_{d_k} X_k over up to 4 packed spectra (gradient, divergence and strain terms of the
+// elasticity operator, src/pde/ElasticitySolver.cpp:268-374); wave numbers as in k_spec_div
+struct SpecTerms {
+  int slot[4], dir[4], n;
+};
+__global__ void __launch_bounds__(256) k_spec_lin(const MemberDev* __restrict__ mem, Dims d, SpecTerms T, int s_out) {
+  const MemberDev& M = mem[blockIdx.y];
+  float2* o = craw(M, s_out);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.nspec; i += stride) {
+    const int kz = (int)(i % d.m2);
+    const int64_t r = i / d.m2;
+    const int ky = (int)(r % d.n1), kx = (int)(r / d.n1);
+    const float w[3] = {(2 * kx == d.n0) ? 0.0f : (float)(2 * kx < d.n0 ? kx : kx - d.n0),
+                        (2 * ky == d.n1) ? 0.0f : (float)(2 * ky < d.n1 ? ky : ky - d.n1), (float)kz};
+    float re = 0.0f, im = 0.0f;
+    for (int t = 0; t < T.n; t++) {
+      const float2 a = craw(M, T.slot[t])[i];
+      const float wt = w[T.dir[t]];
+      re += wt * a.x;
+      im += wt * a.y;
+    }
+    o[i] = make_float2(-im, re);
+  }
+}
+
+// precFactorElasticity (src/cuda/ElasticitySolver.cu:36-106) on packed spectra, without the 1/N^3
+// that the inverse transform applies: s1 = -screen - mu |w|^2, s2 = lam + mu,
+// s3 = 1 / (1 - |w|^2 s2 / s1), U_d = F_d / s1 + (s2 s3 / s1^2) w_d (w.F).  Out of place.
+__global__ void __launch_bounds__(256) k_el_prec(const MemberDev* __restrict__ mem, Dims d, int4 sf, int4 su, float mu,
+                                                 float lam, float scr) {
+  const MemberDev& M = mem[blockIdx.y];
+  const float2* F[3] = {craw(M, sf.x), craw(M, sf.y), craw(M, sf.z)};
+  float2* U[3] = {craw(M, su.x), craw(M, su.y), craw(M, su.z)};
+  const float s2 = lam + mu;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.nspec; i += stride) {
+    const int kz = (int)(i % d.m2);
+    const int64_t r = i / d.m2;
+    const int ky = (int)(r % d.n1), kx = (int)(r / d.n1);
+    const float w[3] = {(2 * kx == d.n0) ? 0.0f : (float)(2 * kx < d.n0 ? kx : kx - d.n0),
+                        (2 * ky == d.n1) ? 0.0f : (float)(2 * ky < d.n1 ? ky : ky - d.n1), (float)kz};
+    const float ww = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    const float s1 = -scr - mu * ww;
+    const float s3 = 1.0f / (1.0f - ww * s2 / s1);
+    const float c = s2 * s3 / (s1 * s1);
+    const float2 f0 = F[0][i], f1 = F[1][i], f2 = F[2][i];
+    const float wfx = w[0] * f0.x + w[1] * f1.x + w[2] * f2.x, wfy = w[0] * f0.y + w[1] * f1.y + w[2] * f2.y;
+    const float2 f[3] = {f0, f1, f2};
+#pragma unroll
+    for (int k = 0; k < 3; k++) U[k][i] = make_float2(f[k].x / s1 + c * w[k] * wfx, f[k].y / s1 + c * w[k] * wfy);
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // launchers (size dispatch)
 // ---------------------------------------------------------------------------------------------
@@ -1682,6 +1736,25 @@ int launch_set_done(const LaunchCtx& c, int v) {
 }
 int launch_axpby_real(const LaunchCtx& c, int r_x, double a, int r_y, double b, int r_out) {
   k_axpby_real<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d, r_x, a, r_y, b, r_out);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+
+int launch_spec_lin(const LaunchCtx& c, int nterms, const int* slots, const int* dirs, int s_out) {
+  if (nterms < 1 || nterms > 4) return (int)hipErrorInvalidValue;
+  SpecTerms T{};
+  T.n = nterms;
+  for (int t = 0; t < nterms; t++) {
+    T.slot[t] = slots[t];
+    T.dir[t] = dirs[t];
+  }
+  k_spec_lin<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d, T, s_out);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+
+int launch_el_prec(const LaunchCtx& c, int4 sf, int4 su, double mu, double lam, double scr) {
+  k_el_prec<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d, sf, su, (float)mu, (float)lam, (float)scr);
   GLIA_CHECK_LAUNCH();
   return 0;
 }

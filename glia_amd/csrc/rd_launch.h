@@ -62,6 +62,8 @@ int launch_set_done(const LaunchCtx& c, int v);
 int launch_axpby_real(const LaunchCtx& c, int r_x, double a, int r_y, double b, int r_out);
 int launch_axpby_c(const LaunchCtx& c, int s_x, double a, int s_y, double b, int s_out);
 int launch_spec_div(const LaunchCtx& c, int s_x, int s_y, int s_z, int s_out);
+int launch_spec_lin(const LaunchCtx& c, int nterms, const int* slots, const int* dirs, int s_out);
+int launch_el_prec(const LaunchCtx& c, int4 sf, int4 su, double mu, double lam, double scr);
 int launch_bricks(const LaunchCtx& c, const BrickJob* jobs_host, int njobs, const double* taps_host, int tap_stride,
                   double h0, double h1, double h2, double sigma, float* store_dev, double* max_out_host);
 

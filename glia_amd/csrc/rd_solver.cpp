@@ -496,6 +496,16 @@ int RDEnsemble::inverse_to_real(int cslot, int work, int rslot) {
 int RDEnsemble::fft_r2c(int rslot, int cslot) { return forward_transform_real(rslot, cslot); }
 int RDEnsemble::fft_c2r(int cslot, int rslot) { return inverse_to_real(cslot, CS_T7, rslot); }
 
+int RDEnsemble::spec_lin(int nterms, const int* slots, const int* dirs, int cout) {
+  GLIA_TRYC(launch_spec_lin(ctx(), nterms, slots, dirs, cout));
+  return 0;
+}
+
+int RDEnsemble::el_prec(const int f[3], const int u[3], double mu, double lam, double scr) {
+  GLIA_TRYC(launch_el_prec(ctx(), make_int4(f[0], f[1], f[2], 0), make_int4(u[0], u[1], u[2], 0), mu, lam, scr));
+  return 0;
+}
+
 int RDEnsemble::divergence(int rx, int ry, int rz, int rout) {
   int rc;
   if ((rc = forward_transform_real(rx, CS_T0)) || (rc = forward_transform_real(ry, CS_T1)) ||

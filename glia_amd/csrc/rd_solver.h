@@ -83,6 +83,10 @@ class RDEnsemble {
   int fft_c2r(int cslot, int rslot);                         // real field of a packed spectrum (1/N^3 included)
   // out = div (x, y, z) of three real fields of every member, spectrally (uses CS_T0..CS_T3, CS_T7)
   int divergence(int rx, int ry, int rz, int rout);
+  // out = i sum_k w_{dirs[k]} X_{slots[k]} on packed spectra (1 to 4 terms)
+  int spec_lin(int nterms, const int* slots, const int* dirs, int cout);
+  // the constant-coefficient elasticity preconditioner on packed spectra, (fx, fy, fz) -> (ux, uy, uz)
+  int el_prec(const int f[3], const int u[3], double mu, double lam, double scr);
   int get_spectrum(int m, int cslot, float2* host);
   int set_spectrum(int m, int cslot, const float2* host);
   int solve_state(int linearized, bool keep_history);

@@ -1194,3 +1194,178 @@ void orc_sl_solve(orc_t *o, double *c, const double *vx, const double *vy, const
   }
   free(ci);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Linear elasticity of the mass-effect model (VariableLinearElasticity-    */
+/* Solver, src/pde/ElasticitySolver.cpp:6-533): material properties, the   */
+/* variable-coefficient Navier-Lame operator, the constant-coefficient      */
+/* spectral preconditioner and PETSc's KSPCG (rtol 1e-3, maxit 100).        */
+/* Vectors of 3 ng entries hold the x, y and z components one after the     */
+/* other.                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* CtxElasticity::computeMu / computeLam (include/pde/ElasticitySolver.h:39-41) */
+static double el_mu(double E, double nu) { return E / (2 * (1 + nu)); }
+static double el_lam(double E, double nu) { return nu * E / ((1 + nu) * (1 - 2 * nu)); }
+
+/* computeMaterialProperties (src/pde/ElasticitySolver.cpp:376-455).  prm = E_healthy, nu_healthy,
+   E_tumor, nu_tumor, E_bg, nu_bg, E_csf, nu_csf, screen_low, screen_high; avg = mean mu, lam,
+   screen over the grid. */
+void orc_el_material(orc_t *o, const double *c, const double *wm, const double *gm, const double *csf,
+                     const double *vt, const double *bg, const double *prm, double *mu, double *lam,
+                     double *screen, double *avg) {
+  const double mu_h = el_mu(prm[0], prm[1]), mu_t = el_mu(prm[2], prm[3]), mu_b = el_mu(prm[4], prm[5]),
+               mu_c = el_mu(prm[6], prm[7]);
+  const double la_h = el_lam(prm[0], prm[1]), la_t = el_lam(prm[2], prm[3]), la_b = el_lam(prm[4], prm[5]),
+               la_c = el_lam(prm[6], prm[7]);
+  const double c_threshold = 0.005;
+  double sm = 0, sl = 0, ss = 0;
+  for (int64_t i = 0; i < o->ng; i++) {
+    double m = mu_b * bg[i] + mu_h * wm[i] + mu_h * gm[i] + mu_c * vt[i] + mu_c * csf[i];
+    double l = la_b * bg[i] + la_h * wm[i] + la_h * gm[i] + la_c * vt[i] + la_c * csf[i];
+    m = (m > 0) ? m : 0;
+    l = (l > 0) ? l : 0;
+    m += (c[i] > 0) ? mu_t * c[i] : 0;
+    l += (c[i] > 0) ? la_t * c[i] : 0;
+    double s = (c[i] >= c_threshold) ? prm[8] : prm[9];
+    if (bg[i] > 0.95) s = 1E6;
+    mu[i] = m;
+    lam[i] = l;
+    screen[i] = s;
+    sm += m;
+    sl += l;
+    ss += s;
+  }
+  avg[0] = sm / o->ng;
+  avg[1] = sl / o->ng;
+  avg[2] = ss / o->ng;
+}
+
+/* operatorVariableCoefficients (src/pde/ElasticitySolver.cpp:268-374):
+   f = div(mu (grad u + grad u^T)) + grad(lam div u) - screen u, in the reference's order of
+   gradient and divergence calls */
+void orc_el_matvec(orc_t *o, const double *mu, const double *lam, const double *screen, const double *u,
+                   double *f) {
+  const int64_t ng = o->ng;
+  double *buf = (double *)malloc(sizeof(double) * ng * 16);
+  double *g[3], *du[3][3], *a = buf + 12 * ng, *b = buf + 13 * ng, *cc = buf + 14 * ng, *ld = buf + 15 * ng;
+  for (int k = 0; k < 3; k++) g[k] = buf + k * ng;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) du[i][j] = buf + (3 + 3 * i + j) * ng;  /* du[i][j] = d_j u_i */
+  const double *ux = u, *uy = u + ng, *uz = u + 2 * ng;
+  orc_div(o, ux, uy, uz, ld);
+  for (int64_t k = 0; k < ng; k++) ld[k] *= lam[k];
+  orc_grad(o, ld, g[0], g[1], g[2]);
+  orc_grad(o, ux, du[0][0], du[0][1], du[0][2]);
+  orc_grad(o, uy, du[1][0], du[1][1], du[1][2]);
+  orc_grad(o, uz, du[2][0], du[2][1], du[2][2]);
+  for (int i = 0; i < 3; i++) {
+    double *cols[3] = {a, b, cc};
+    for (int j = 0; j < 3; j++)
+      for (int64_t k = 0; k < ng; k++) cols[j][k] = mu[k] * (du[i][j][k] + du[j][i][k]);
+    double *fi = f + i * ng;
+    orc_div(o, a, b, cc, fi);
+    const double *ui = u + i * ng;
+    for (int64_t k = 0; k < ng; k++) fi[k] = fi[k] + g[i][k] - screen[k] * ui[k];
+  }
+  free(buf);
+}
+
+/* operatorConstantCoefficients with precFactorElasticity (src/pde/ElasticitySolver.cpp:122-266,
+   src/cuda/ElasticitySolver.cu:36-106): the exact inverse of the operator with the mean mu, lam and
+   screen, per wave number w (Nyquist modes zeroed): s1 = -screen - mu |w|^2, s2 = lam + mu,
+   s3 = 1 / (1 - |w|^2 s2 / s1), u = f / s1 + (s2 s3 / s1^2) w (w.f) */
+void orc_el_prec(orc_t *o, const double *avg, const double *f, double *u) {
+  const int64_t ng = o->ng, nh = o->nh;
+  const int n1 = o->n[1], n2h = o->n2h;
+  cplx *F = (cplx *)malloc(sizeof(cplx) * nh * 3);
+  for (int d = 0; d < 3; d++) orc_r2c(o, f + d * ng, F + d * nh);
+  const double mu = avg[0], lam = avg[1], scr = avg[2];
+  const double factor = 1.0 / ((double)o->n[0] * o->n[1] * o->n[2]);
+  const double s2 = lam + mu;
+  for (int64_t idx = 0; idx < nh; idx++) {
+    const int64_t x = idx / ((int64_t)n1 * n2h), y = (idx / n2h) % n1, z = idx % n2h;
+    const double w[3] = {wnum(x, o->n[0]), wnum(y, o->n[1]), wnum(z, o->n[2])};
+    const double wTw = -(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    const double s1 = -scr + mu * wTw;
+    const double s3 = 1.0 / (1.0 + (wTw * s2) / s1);
+    const cplx wf = w[0] * F[idx] + w[1] * F[nh + idx] + w[2] * F[2 * nh + idx];
+    for (int d = 0; d < 3; d++) {
+      const cplx scale = -w[d] * wf;
+      F[d * nh + idx] = factor * (F[d * nh + idx] / s1 - (1.0 / (s1 * s1)) * s2 * s3 * scale);
+    }
+  }
+  for (int d = 0; d < 3; d++) orc_c2r(o, F + d * nh, u + d * ng);
+  free(F);
+}
+
+static double el_dot(int64_t n, const double *a, const double *b) {
+  double s = 0;
+  for (int64_t i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+/* VariableLinearElasticitySolver::solve (src/pde/ElasticitySolver.cpp:471-533): KSPCG with the
+   shell preconditioner, rtol 1e-3, PETSc's default atol 1e-50 and dtol 1e5, maxit 100, nonzero
+   initial guess u; the loop and its stopping rules are those of ksp_cg above.  Returns the
+   iteration count; *reason_out the KSPConvergedReason; *rnorm_out the last preconditioned
+   residual norm. */
+int orc_el_solve(orc_t *o, const double *mu, const double *lam, const double *screen, const double *avg,
+                 const double *rhs, double *u, int *reason_out, double *rnorm_out) {
+  const int64_t N = 3 * o->ng;
+  const double rtol = 1e-3, atol = 1e-50, dtol = 1e5;
+  const int maxit = 100;
+  double *r = (double *)malloc(sizeof(double) * N * 4), *z = r + N, *p = r + 2 * N, *wv = r + 3 * N;
+  double dp, beta = 0, betaold = 0, dpi = 0, dpiold, a, bb, rnorm0, ttol;
+  int its = 0, reason = 0, i;
+  orc_el_matvec(o, mu, lam, screen, u, r);
+  for (int64_t k = 0; k < N; k++) r[k] = rhs[k] - r[k];
+  orc_el_prec(o, avg, r, z);
+  dp = sqrt(el_dot(N, z, z));
+  orc_el_prec(o, avg, rhs, wv);
+  rnorm0 = sqrt(el_dot(N, wv, wv));
+  if (rnorm0 == 0) rnorm0 = dp;
+  ttol = fmax(rtol * rnorm0, atol);
+#define EL_CONVERGED(rn)                                                          \
+  ((isnan(rn) || isinf(rn)) ? KSP_DIVERGED_NANORINF                               \
+   : (rn) <= ttol          ? (((rn) < atol) ? KSP_CONVERGED_ATOL : KSP_CONVERGED_RTOL) \
+   : (rn) >= dtol * rnorm0 ? KSP_DIVERGED_DTOL                                    \
+                           : 0)
+  reason = EL_CONVERGED(dp);
+  if (!reason) {
+    beta = el_dot(N, z, r);
+    i = 0;
+    do {
+      its = i + 1;
+      if (beta == 0.0) { reason = KSP_CONVERGED_ATOL; break; }
+      else if (i > 0 && beta * betaold < 0.0) { reason = KSP_DIVERGED_INDEFINITE_PC; break; }
+      if (!i) {
+        memcpy(p, z, sizeof(double) * N);
+      } else {
+        bb = beta / betaold;
+        for (int64_t k = 0; k < N; k++) p[k] = z[k] + bb * p[k];
+      }
+      dpiold = dpi;
+      orc_el_matvec(o, mu, lam, screen, p, wv);
+      dpi = el_dot(N, p, wv);
+      betaold = beta;
+      if (isnan(dpi) || isinf(dpi)) { reason = KSP_DIVERGED_NANORINF; break; }
+      if ((dpi == 0.0) || ((i > 0) && (dpi * dpiold <= 0.0))) { reason = KSP_DIVERGED_INDEFINITE_MAT; break; }
+      a = beta / dpi;
+      for (int64_t k = 0; k < N; k++) { u[k] += a * p[k]; r[k] -= a * wv[k]; }
+      orc_el_prec(o, avg, r, z);
+      dp = sqrt(el_dot(N, z, z));
+      reason = EL_CONVERGED(dp);
+      if (reason) break;
+      beta = el_dot(N, z, r);
+      if (isnan(beta) || isinf(beta)) { reason = KSP_DIVERGED_NANORINF; break; }
+      i++;
+    } while (i < maxit);
+    if (i >= maxit) reason = KSP_DIVERGED_ITS;
+  }
+#undef EL_CONVERGED
+  *reason_out = reason;
+  *rnorm_out = dp;
+  free(r);
+  return its;
+}

@@ -44,6 +44,11 @@ def _load():
         "orc_grad": (None, [vp, _dp, _dp, _dp, _dp]),
         "orc_div": (None, [vp, _dp, _dp, _dp, _dp]),
         "orc_sl_interp": (None, [vp, _dp, _dp, _dp, _dp, _dp]),
+        "orc_el_material": (None, [vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "orc_el_matvec": (None, [vp, _dp, _dp, _dp, _dp, _dp]),
+        "orc_el_prec": (None, [vp, _dp, _dp, _dp]),
+        "orc_el_solve": (i, [vp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int),
+                             ctypes.POINTER(ctypes.c_double)]),
         "orc_sl_trajectories": (None, [vp, _dp, _dp, _dp, d, _dp, _dp, _dp]),
         "orc_sl_solve": (None, [vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, d, i]),
         "orc_smooth": (None, [vp, _dp, _dp, d]),
@@ -176,6 +181,42 @@ class Oracle:
         out = self._new()
         self.L.orc_div(self.h, self._f(dx), self._f(dy), self._f(dz), out)
         return out.reshape(self.shape)
+
+    # elasticity: vectors are three fields stacked as (3,) + shape
+    EL_DEFAULTS = (2100.0, 0.4, 8000.0, 0.45, 15000.0, 0.45, 100.0, 0.1, 1e-2, 1e2)
+
+    def el_material(self, c, wm, gm, csf, vt, bg, prm=None):
+        """computeMaterialProperties: (mu, lam, screen, avg[3])."""
+        prm = np.array(self.EL_DEFAULTS if prm is None else prm, dtype=np.float64)
+        mu, lam, scr = self._new(), self._new(), self._new()
+        avg = np.zeros(3)
+        self.L.orc_el_material(self.h, *[self._f(a) for a in (c, wm, gm, csf, vt, bg)], prm, mu, lam, scr, avg)
+        return mu.reshape(self.shape), lam.reshape(self.shape), scr.reshape(self.shape), avg
+
+    def _vec(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        if u.size != 3 * self.ng:
+            raise ValueError("vector field has %d entries, grid needs %d" % (u.size, 3 * self.ng))
+        return u
+
+    def el_matvec(self, mu, lam, scr, u):
+        f = np.zeros(3 * self.ng)
+        self.L.orc_el_matvec(self.h, self._f(mu), self._f(lam), self._f(scr), self._vec(u), f)
+        return f.reshape((3,) + self.shape)
+
+    def el_prec(self, avg, f):
+        u = np.zeros(3 * self.ng)
+        self.L.orc_el_prec(self.h, np.ascontiguousarray(avg, dtype=np.float64), self._vec(f), u)
+        return u.reshape((3,) + self.shape)
+
+    def el_solve(self, mu, lam, scr, avg, rhs, u0):
+        """KSPCG on the elasticity operator: (u, iterations, reason, last preconditioned residual norm)."""
+        u = self._vec(u0).copy()
+        reason, rn = ctypes.c_int(), ctypes.c_double()
+        its = self.L.orc_el_solve(self.h, self._f(mu), self._f(lam), self._f(scr),
+                                  np.ascontiguousarray(avg, dtype=np.float64), self._vec(rhs), u,
+                                  ctypes.byref(reason), ctypes.byref(rn))
+        return u.reshape((3,) + self.shape), its, reason.value, rn.value
 
     def sl_interp(self, f, q):
         """f at the points x + q (q: three offset fields in grid units), cubic Lagrange, periodic."""
