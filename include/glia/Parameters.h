@@ -29,6 +29,10 @@ struct Grid {
 struct TumorParameters {
   int model_ = 1;
   bool adv_velocity_set_ = false;
+  // elasticity of the mass-effect models (include/Parameters.h:175-184)
+  ScalarType E_csf_ = 100, E_healthy_ = 2100, E_tumor_ = 8000, E_bg_ = 15000;
+  ScalarType nu_csf_ = 0.1, nu_healthy_ = 0.4, nu_tumor_ = 0.45, nu_bg_ = 0.45;
+  ScalarType screen_low_ = 1E-2, screen_high_ = 1E2;
   int np_ = 1, nk_ = 1, nr_ = 1;
   int nt_ = 1;
   ScalarType dt_ = 0.5;

@@ -4,6 +4,7 @@
 //
 //   "Running diffusion solver"       src/test/pdesolver.cpp:7-57 (||c||_2 = 2.0487, 5 CG iterations)
 //   "Running advection solver"       src/test/pdesolver.cpp:59-108 (round trip below 1%)
+//   VariableLinearElasticitySolver: the solve meets its stopping test through the interface
 //   "Evaluating objective function"  src/test/grad.cpp:7-80 (J = 0 at the truth, J > 0 otherwise)
 //
 //   TILOptimizer (quasi- and Gauss-Newton), RDOptimizer, SparseTILOptimizer on exact data (src/opt/*.cpp through include/glia/Optimizer.h)
@@ -148,6 +149,71 @@ static void test_advection_solver() {
   REQUIRE(norm / norm_c < 1E-2);
   VecDestroy(&c);
   VecDestroy(&c_copy);
+}
+
+// ---- VariableLinearElasticitySolver (src/pde/ElasticitySolver.cpp:6-533) ---------------------------
+static void test_elasticity_solver() {
+  std::printf("TEST VariableLinearElasticitySolver through the interface\n");
+  auto params = std::make_shared<Parameters>();
+  auto spec_ops = std::make_shared<SpectralOperators>(CUFFT);
+  initializeGrid(32, params);
+  params->tu_->model_ = 4;
+  auto tumor = std::make_shared<Tumor>(params, spec_ops);
+  tumor->mat_prop_ = std::make_shared<MatProp>(params, spec_ops);
+  REQUIRE(tumor->mat_prop_->setValuesSinusoidal(params) == 0);
+  createTestFunction(tumor->c_t_, params);
+  // force along the tumor gradient: f = -100 grad c for c = exp(-|x - pi|^2 / R^2)
+  const int64_t ng = params->grid_->ng_;
+  auto force = std::make_shared<VecField>(ng, ng), disp = std::make_shared<VecField>(ng, ng);
+  disp->set(0);
+  {
+    const Grid& g = *params->grid_;
+    const double R = std::sqrt(2.) * (2 * M_PI) / 64;
+    ScalarType *fx, *fy, *fz;
+    VecGetArray(force->x_, &fx);
+    VecGetArray(force->y_, &fy);
+    VecGetArray(force->z_, &fz);
+    for (int X = 0; X < g.n_[0]; X++)
+      for (int Y = 0; Y < g.n_[1]; Y++)
+        for (int Z = 0; Z < g.n_[2]; Z++) {
+          const int64_t i = ((int64_t)X * g.n_[1] + Y) * g.n_[2] + Z;
+          const double d[3] = {g.h_[0] * X - M_PI, g.h_[1] * Y - M_PI, g.h_[2] * Z - M_PI};
+          const double c = std::exp(-(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) / (R * R));
+          fx[i] = 100 * 2 * d[0] / (R * R) * c;
+          fy[i] = 100 * 2 * d[1] / (R * R) * c;
+          fz[i] = 100 * 2 * d[2] / (R * R) * c;
+        }
+    VecRestoreArray(force->x_, &fx);
+    VecRestoreArray(force->y_, &fy);
+    VecRestoreArray(force->z_, &fz);
+  }
+  VariableLinearElasticitySolver el(params, tumor, spec_ops);
+  REQUIRE(el.solve(disp, force) == 0);
+  // the solve's stopping test, checked with the interface's own operator and preconditioner
+  auto r = std::make_shared<VecField>(ng, ng), z = std::make_shared<VecField>(ng, ng), bf = std::make_shared<VecField>(ng, ng);
+  REQUIRE(el.applyOperator(disp, r) == 0);
+  VecScale(r->x_, -1);
+  VecScale(r->y_, -1);
+  VecScale(r->z_, -1);
+  VecAXPY(r->x_, 1, force->x_);
+  VecAXPY(r->y_, 1, force->y_);
+  VecAXPY(r->z_, 1, force->z_);
+  REQUIRE(el.applyPreconditioner(r, z) == 0);
+  REQUIRE(el.applyPreconditioner(force, bf) == 0);
+  auto norm3 = [](std::shared_ptr<VecField> v) {
+    ScalarType a, b, c;
+    VecNorm(v->x_, NORM_2, &a);
+    VecNorm(v->y_, NORM_2, &b);
+    VecNorm(v->z_, NORM_2, &c);
+    return std::sqrt(a * a + b * b + c * c);
+  };
+  const double rz = norm3(z), rb = norm3(bf), un = norm3(disp);
+  std::printf("  %d CG iterations, reason %d, ||B(f - A u)|| / ||B f|| = %.3e, ||u|| = %.3e\n", el.iterations_,
+              el.reason_, rz / rb, un);
+  REQUIRE(el.reason_ == 2);
+  REQUIRE(el.iterations_ >= 1 && el.iterations_ < 100);
+  REQUIRE(rz <= 1.01e-3 * rb);
+  REQUIRE(un > 0);
 }
 
 // ---- src/test/grad.cpp:7-80 ------------------------------------------------------------------
@@ -557,6 +623,7 @@ static void test_sparse_til_optimizer() {
 int main() {
   test_diffusion_solver();
   test_advection_solver();
+  test_elasticity_solver();
   test_objective();
   test_interface_matches_abi_and_fd();
   test_phi_and_apply_d();
