@@ -223,6 +223,19 @@ EL_SIGNATURES = {
 }
 
 
+# include/glia_ms.h (multispecies pointwise terms)
+class MsParams(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ("death_rate", "ox_source", "ox_consumption", "alpha_0", "beta_0",
+                                                 "ox_inv", "ox_hypoxia")]
+
+
+MS_SIGNATURES = {
+    "glia_ms_params_init": (None, [ctypes.POINTER(MsParams)]),
+    "glia_ms_sources": (_i, [ctypes.c_int64] + [_vp] * 8 + [_d, ctypes.POINTER(MsParams), _vp]),
+    "glia_ms_update_coefficients": (_i, [ctypes.c_int64] + [_vp] * 6 + [_d, _d, _d, _d, _vp]),
+}
+
+
 KERNELS = ["xi_pupd", "yi_grad", "z_diff", "yf_div", "xf_matvec", "pcg_update"]
 # algorithmic bytes of one launch per member, in units of one field (4 n0 n1 n2 bytes): every
 # input read once and every output written once
@@ -249,7 +262,7 @@ def load():
             "(hipcc --offload-arch=gfx950); there is no CPU fallback" % LIB_PATH
         )
     lib = ctypes.CDLL(LIB_PATH)
-    tables = [SIGNATURES, IO_SIGNATURES, OPT_SIGNATURES, ADV_SIGNATURES, EL_SIGNATURES]
+    tables = [SIGNATURES, IO_SIGNATURES, OPT_SIGNATURES, ADV_SIGNATURES, EL_SIGNATURES, MS_SIGNATURES]
     for name, (res, args) in [kv for t in tables for kv in t.items()]:
         f = getattr(lib, name)
         f.restype = res

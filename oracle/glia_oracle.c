@@ -1369,3 +1369,49 @@ int orc_el_solve(orc_t *o, const double *mu, const double *lam, const double *sc
   free(r);
   return its;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Multispecies pointwise terms (PdeOperatorsMultiSpecies,                   */
+/* src/pde/PdeOperatorsMultiSpecies.cpp:10-234; src/cuda/PdeOperators.cu:27-124) */
+/* ------------------------------------------------------------------------ */
+
+/* computeReactionRate, computeTransition, computeThesholder and computeSources.  prm =
+   death_rate, ox_source, ox_consumption, alpha_0, beta_0, ox_inv, ox_hypoxia. */
+void orc_ms_sources(int64_t n, double *p, double *iv, double *nec, double *ox, double *gm, double *wm,
+                    const double *rho, const double *di, double dt, const double *prm) {
+  const double death = prm[0], oxs = prm[1], oxc = prm[2], a0 = prm[3], b0 = prm[4], oxinv = prm[5], oxh = prm[6];
+  for (int64_t k = 0; k < n; k++) {
+    const double m = rho[k] * (1 / (1 + exp(-100 * (ox[k] - oxh))));
+    const double al = a0 * (1 / (1 + exp(100 * (ox[k] - oxinv))));
+    const double be = b0 * ox[k];
+    const double h = 1 / (1 + exp(100 * (ox[k] - oxh)));
+    const double pt = p[k], it = iv[k];
+    p[k] += dt * (m * p[k] * (1. - p[k]) - al * p[k] + be * iv[k] - death * h * p[k]);
+    iv[k] += dt * (m * iv[k] * (1. - iv[k]) + al * pt - be * iv[k] - death * h * iv[k]);
+    nec[k] += dt * (h * death * (pt + it + gm[k] + wm[k]));
+    ox[k] += dt * (-oxc * pt + oxs * (1 - ox[k]) * (gm[k] + wm[k]));
+    ox[k] = (ox[k] <= 0.) ? 0. : ox[k];
+    double f1 = 0, f2 = 0;
+    if (gm[k] > 0.01 || wm[k] > 0.01) {
+      f1 = gm[k] / (gm[k] + wm[k]);
+      f2 = wm[k] / (gm[k] + wm[k]);
+    }
+    f1 = isnan(f1) ? 0. : f1;
+    f2 = isnan(f2) ? 0. : f2;
+    const double loss = m * pt * (1. - pt) + m * it * (1. - it) + di[k];
+    gm[k] += -dt * (f1 * loss + h * death * gm[k]);
+    wm[k] += -dt * (f2 * loss + h * death * wm[k]);
+  }
+}
+
+/* updateReacAndDiffCoefficients */
+void orc_ms_update_coefficients(int64_t n, double *rho_f, double *k_f, const double *bg, const double *gm,
+                                const double *vt, const double *csf, double rho, double k, double r_gm_wm,
+                                double k_gm_wm) {
+  for (int64_t i = 0; i < n; i++) {
+    double t = 1 - (bg[i] + (1 - r_gm_wm) * gm[i] + vt[i] + csf[i]);
+    rho_f[i] = (t < 0 ? 0 : t) * rho;
+    t = 1 - (bg[i] + (1 - k_gm_wm) * gm[i] + vt[i] + csf[i]);
+    k_f[i] = (t < 0 ? 0 : t) * k;
+  }
+}

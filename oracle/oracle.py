@@ -44,6 +44,8 @@ def _load():
         "orc_grad": (None, [vp, _dp, _dp, _dp, _dp]),
         "orc_div": (None, [vp, _dp, _dp, _dp, _dp]),
         "orc_sl_interp": (None, [vp, _dp, _dp, _dp, _dp, _dp]),
+        "orc_ms_sources": (None, [ctypes.c_int64] + [_dp] * 8 + [d, _dp]),
+        "orc_ms_update_coefficients": (None, [ctypes.c_int64] + [_dp] * 6 + [d, d, d, d]),
         "orc_el_material": (None, [vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "orc_el_matvec": (None, [vp, _dp, _dp, _dp, _dp, _dp]),
         "orc_el_prec": (None, [vp, _dp, _dp, _dp]),
@@ -392,3 +394,26 @@ class Oracle:
         y = np.zeros(self.np)
         self.L.orc_hessian_p(self.h, _c(ptilde), y)
         return y
+
+
+MS_DEFAULTS = (4.0, 1.0, 5.0, 0.1, 0.01, 0.8, 0.6)  # death_rate, ox_source, ox_consumption, alpha_0, beta_0, ox_inv, ox_hypoxia
+
+
+def ms_sources(p, i, nec, ox, gm, wm, rho, di, dt, prm=MS_DEFAULTS):
+    """computeSources of the multispecies model on flat arrays: the updated (p, i, nec, ox, gm, wm)."""
+    L = _load()
+    out = [np.array(a, dtype=np.float64).reshape(-1).copy() for a in (p, i, nec, ox, gm, wm)]
+    L.orc_ms_sources(out[0].size, *out, _c(rho).reshape(-1), _c(di).reshape(-1), float(dt),
+                     np.ascontiguousarray(prm, dtype=np.float64))
+    return out
+
+
+def ms_update_coefficients(bg, gm, vt, csf, rho, k, r_gm_wm, k_gm_wm):
+    """updateReacAndDiffCoefficients on flat arrays: (rho field, k field)."""
+    L = _load()
+    n = np.asarray(bg).size
+    rf, kf = np.zeros(n), np.zeros(n)
+    L.orc_ms_update_coefficients(n, rf, kf, *[_c(a).reshape(-1) for a in (bg, gm, vt, csf)], float(rho), float(k),
+                                 float(r_gm_wm), float(k_gm_wm))
+    return rf, kf
+
