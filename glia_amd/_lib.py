@@ -220,6 +220,7 @@ EL_SIGNATURES = {
     "glia_el_apply": (_i, [_vp, _vp, _vp, _i]),
     "glia_el_precondition": (_i, [_vp, _vp, _vp, _i]),
     "glia_el_solve": (_i, [_vp, _vp, _vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_d)]),
+    "glia_el_force": (_i, [_vp, _vp, _d, _i, _vp, _i]),
 }
 
 

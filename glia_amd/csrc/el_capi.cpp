@@ -327,6 +327,27 @@ int glia_el_precondition(glia_el_t h, const float* const f[3], float* const u[3]
   return stage_out(h, u, mem_kind, VZ);
 }
 
+int glia_el_force(glia_el_t h, const float* c, double forcing_factor, int use_tanh, float* const f[3], int mem_kind) {
+  if (!h) return GLIA_RD_ERR_ARG;
+  if (!c || !f || !valid_kind(mem_kind)) return fail(h, GLIA_RD_ERR_ARG, "null field or bad mem_kind");
+  float* wm = slot(h, glia::RS_WM);
+  EL_HIP(h, hipMemcpyAsync(wm, c, sizeof(float) * h->ng,
+                           mem_kind == GLIA_RD_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->st));
+  const int n0 = h->e->dims().n0;
+  EL_ENG(h, h->e->smooth_field(0, glia::RS_WM, 2.0 * M_PI / n0));  // sigma_smooth = 2 pi / n0
+  float* cs = h->v[VB][0];                                           // the smoothed c
+  EL_HIP(h, hipMemcpyAsync(cs, wm, sizeof(float) * h->ng, hipMemcpyDeviceToDevice, h->st));
+  EL_ENG(h, h->e->fft_r2c(glia::RS_WM, CU[0]));
+  for (int d = 0; d < 3; d++) {
+    const int s[1] = {CU[0]}, dir[1] = {d};
+    EL_ENG(h, h->e->spec_lin(1, s, dir, CS_C));
+    EL_ENG(h, h->e->fft_c2r(CS_C, glia::RS_GM));
+    EL_HIP(h, glia::el::launch_force_scale(h->st, h->ng, slot(h, glia::RS_GM), cs, (float)forcing_factor,
+                                           use_tanh ? 1 : 0, h->v[VW][d]));
+  }
+  return stage_out(h, f, mem_kind, VW);
+}
+
 int glia_el_solve(glia_el_t h, const float* const f[3], float* const u[3], int mem_kind, int* iterations, int* reason,
                   double* rnorm) {
   if (!h) return GLIA_RD_ERR_ARG;

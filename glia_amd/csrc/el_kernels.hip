@@ -101,12 +101,25 @@ __global__ void __launch_bounds__(256) k_axpby3(int64_t ng, Vec3 p, Vec3C z, dou
       p.p[k][i] = sub ? z.p[k][i] - p.p[k][i] : z.p[k][i] + fb * p.p[k][i];
 }
 
+// computeForce's scaling (src/mat/Tumor.cpp:114-158, nonlinearForceScalingCuda): out = ff g, times
+// tanh(c) when use_tanh
+__global__ void __launch_bounds__(256) k_force_scale(int64_t ng, const float* __restrict__ g, const float* __restrict__ c,
+                                                     float ff, int use_tanh, float* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += stride)
+    out[i] = use_tanh ? ff * g[i] * tanhf(c[i]) : ff * g[i];
+}
+
 int partial_count() { return EB; }
 
 int launch_material(hipStream_t st, int64_t ng, const float* c, const float* wm, const float* gm, const float* csf,
                     const float* vt, const float* bg, const ElMaterial& m, float* mu, float* lam, float* scr,
                     double* part) {
   k_el_material<<<EB, 256, 0, st>>>(ng, c, wm, gm, csf, vt, bg, m, mu, lam, scr, part);
+  return (int)hipGetLastError();
+}
+int launch_force_scale(hipStream_t st, int64_t ng, const float* g, const float* c, float ff, int use_tanh, float* out) {
+  k_force_scale<<<EB, 256, 0, st>>>(ng, g, c, ff, use_tanh, out);
   return (int)hipGetLastError();
 }
 int launch_mul(hipStream_t st, int64_t ng, const float* a, const float* b, float* out) {

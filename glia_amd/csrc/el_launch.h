@@ -22,6 +22,7 @@ int partial_count();
 int launch_material(hipStream_t st, int64_t ng, const float* c, const float* wm, const float* gm, const float* csf,
                     const float* vt, const float* bg, const ElMaterial& m, float* mu, float* lam, float* scr,
                     double* part);
+int launch_force_scale(hipStream_t st, int64_t ng, const float* g, const float* c, float ff, int use_tanh, float* out);
 int launch_mul(hipStream_t st, int64_t ng, const float* a, const float* b, float* out);
 int launch_fnms(hipStream_t st, int64_t ng, const float* a, const float* b, const float* c, float* out);
 int launch_dot3(hipStream_t st, int64_t ng, const Vec3C& x, const Vec3C& y, double* part);

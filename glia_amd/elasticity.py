@@ -94,3 +94,11 @@ class ElasticitySolver:
         self._chk(self.L.glia_el_solve(self.h, self._ptrs(fin), self._ptrs(u), _lib.HOST, ctypes.byref(its),
                                        ctypes.byref(why), ctypes.byref(rn)))
         return np.stack(u).reshape((3,) + self.shape), its.value, REASONS.get(why.value, why.value), rn.value
+
+    def force(self, c, forcing_factor=1e5, use_tanh=True):
+        """Tumor::computeForce: forcing_factor grad(S c) (times tanh(S c)), shape (3,) + grid."""
+        cin, out = self._f(c), [np.empty(self.ng, np.float32) for _ in range(3)]
+        self._chk(self.L.glia_el_force(self.h, cin.ctypes.data_as(ctypes.c_void_p), float(forcing_factor),
+                                       int(bool(use_tanh)), self._ptrs(out), _lib.HOST))
+        return np.stack(out).reshape((3,) + self.shape)
+

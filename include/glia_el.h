@@ -54,6 +54,10 @@ int glia_el_get_material(glia_el_t h, float* mu, float* lam, float* screen, doub
 int glia_el_apply(glia_el_t h, const float* const u[3], float* const f[3], int mem_kind);
 /* operatorConstantCoefficients (src/pde/ElasticitySolver.cpp:122-266): u = B f */
 int glia_el_precondition(glia_el_t h, const float* const f[3], float* const u[3], int mem_kind);
+/* Tumor::computeForce (src/mat/Tumor.cpp:114-158): f = forcing_factor grad(S c), times tanh(S c)
+   when use_tanh (use_tanh_scaling_), with S the Weierstrass smoother of sigma 2 pi / n0 and the
+   spectral gradient.  c is one field; f three. */
+int glia_el_force(glia_el_t h, const float* c, double forcing_factor, int use_tanh, float* const f[3], int mem_kind);
 /* VariableLinearElasticitySolver::solve (src/pde/ElasticitySolver.cpp:471-533) after
    glia_el_set_material: u (initial guess in, solution out) for the force f.  iterations, reason
    (KSPConvergedReason values: 2 rtol, 3 atol, -3 max iterations, -4 dtol, -8 indefinite

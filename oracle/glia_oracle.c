@@ -1415,3 +1415,15 @@ void orc_ms_update_coefficients(int64_t n, double *rho_f, double *k_f, const dou
     k_f[i] = (t < 0 ? 0 : t) * k;
   }
 }
+
+/* Tumor::computeForce (src/mat/Tumor.cpp:114-158): f = ff grad(S c) (times tanh(S c) when use_tanh),
+   S the Weierstrass smoother with sigma = 2 pi / n0 */
+void orc_el_force(orc_t *o, const double *c, double ff, int use_tanh, double *f) {
+  const int64_t ng = o->ng;
+  double *cs = (double *)malloc(sizeof(double) * ng);
+  orc_smooth(o, c, cs, 2.0 * M_PI / o->n[0]);
+  orc_grad(o, cs, f, f + ng, f + 2 * ng);
+  for (int d = 0; d < 3; d++)
+    for (int64_t i = 0; i < ng; i++) f[d * ng + i] *= use_tanh ? ff * tanh(cs[i]) : ff;
+  free(cs);
+}

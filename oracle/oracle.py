@@ -48,6 +48,7 @@ def _load():
         "orc_ms_update_coefficients": (None, [ctypes.c_int64] + [_dp] * 6 + [d, d, d, d]),
         "orc_el_material": (None, [vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "orc_el_matvec": (None, [vp, _dp, _dp, _dp, _dp, _dp]),
+        "orc_el_force": (None, [vp, _dp, d, i, _dp]),
         "orc_el_prec": (None, [vp, _dp, _dp, _dp]),
         "orc_el_solve": (i, [vp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int),
                              ctypes.POINTER(ctypes.c_double)]),
@@ -204,6 +205,12 @@ class Oracle:
     def el_matvec(self, mu, lam, scr, u):
         f = np.zeros(3 * self.ng)
         self.L.orc_el_matvec(self.h, self._f(mu), self._f(lam), self._f(scr), self._vec(u), f)
+        return f.reshape((3,) + self.shape)
+
+    def el_force(self, c, ff=1e5, use_tanh=True):
+        """computeForce: ff grad(S c) (times tanh(S c))."""
+        f = np.zeros(3 * self.ng)
+        self.L.orc_el_force(self.h, self._f(c), float(ff), int(bool(use_tanh)), f)
         return f.reshape((3,) + self.shape)
 
     def el_prec(self, avg, f):

@@ -66,3 +66,15 @@ def test_cg_solve_matches_the_oracle(case):
     assert abs(its - itso) <= 1
     assert np.linalg.norm(res) <= 2e-3 * np.linalg.norm(bf)
     assert rel(u, uo) < 1e-5
+
+
+def test_tumor_force_matches_the_oracle(case):
+    """Tumor::computeForce (src/mat/Tumor.cpp:114-158): the smoothed tumor's spectral gradient times
+    the forcing factor, with and without the tanh scaling."""
+    n, o, s, fields, _ = case
+    c = fields[0]
+    for use_tanh in (True, False):
+        f = s.force(c, 1e5, use_tanh)
+        fo = o.el_force(c, 1e5, use_tanh)
+        print("force tanh", use_tanh, "rel", rel(f, fo), flush=True)
+        assert rel(f, fo) < 1e-5
