@@ -234,6 +234,9 @@ MS_SIGNATURES = {
     "glia_ms_params_init": (None, [ctypes.POINTER(MsParams)]),
     "glia_ms_sources": (_i, [ctypes.c_int64] + [_vp] * 8 + [_d, ctypes.POINTER(MsParams), _vp]),
     "glia_ms_update_coefficients": (_i, [ctypes.c_int64] + [_vp] * 6 + [_d, _d, _d, _d, _vp]),
+    "glia_ms_segmentation": (_i, [ctypes.c_int64] + [_vp] * 7 + [_vp]),
+    "glia_ms_clip": (_i, [ctypes.c_int64, _vp, _vp]),
+    "glia_ms_edema_mask": (_i, [ctypes.c_int64, _vp, _d, _vp, _vp]),
 }
 
 

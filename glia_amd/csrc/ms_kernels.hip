@@ -61,6 +61,36 @@ __global__ void __launch_bounds__(256) k_ms_coefficients(int64_t n, float* __res
   }
 }
 
+__global__ void __launch_bounds__(256) k_ms_segmentation(int64_t n, const float* __restrict__ bg,
+                                                         const float* __restrict__ gm, const float* __restrict__ wm,
+                                                         const float* __restrict__ vt, const float* __restrict__ csf,
+                                                         const float* __restrict__ c, float* __restrict__ seg) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const float ck = c[k], a = 1.0f - ck;
+    float mx = bg[k];
+    int ct = 0;
+    if (ck > mx) { mx = ck; ct = 1; }
+    if (wm[k] * a > mx) { mx = wm[k] * a; ct = 6; }
+    if (gm[k] * a > mx) { mx = gm[k] * a; ct = 5; }
+    if (vt[k] * a > mx) { mx = vt[k] * a; ct = 7; }
+    if (csf[k] * a > mx) { mx = csf[k] * a; ct = 8; }
+    seg[k] = (float)ct;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_ms_clip(int64_t n, float* __restrict__ x) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) x[k] = (x[k] <= 0.0f) ? 0.0f : x[k];
+}
+
+__global__ void __launch_bounds__(256) k_ms_edema(int64_t n, const float* __restrict__ iv, float thr, float* __restrict__ ed) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) ed[k] = (iv[k] > thr) ? 1.0f : 0.0f;
+}
+
+static unsigned blocks(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256, 4096); }
+
 }  // namespace ms
 }  // namespace glia
 
@@ -99,6 +129,28 @@ int glia_ms_update_coefficients(int64_t n, float* rho_field, float* k_field, con
   const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
   glia::ms::k_ms_coefficients<<<blocks, 256, 0, (hipStream_t)stream>>>(
       n, rho_field, k_field, bg, gm, vt, csf, (float)rho, (float)k, (float)(1 - r_gm_wm), (float)(1 - k_gm_wm));
+  return (int)hipGetLastError();
+}
+
+int glia_ms_segmentation(int64_t n, const float* bg, const float* gm, const float* wm, const float* vt,
+                         const float* csf, const float* c, float* seg, void* stream) {
+  if (n < 0 || !bg || !gm || !wm || !vt || !csf || !c || !seg) return GLIA_RD_ERR_ARG;
+  if (n == 0) return 0;
+  glia::ms::k_ms_segmentation<<<glia::ms::blocks(n), 256, 0, (hipStream_t)stream>>>(n, bg, gm, wm, vt, csf, c, seg);
+  return (int)hipGetLastError();
+}
+
+int glia_ms_clip(int64_t n, float* x, void* stream) {
+  if (n < 0 || !x) return GLIA_RD_ERR_ARG;
+  if (n == 0) return 0;
+  glia::ms::k_ms_clip<<<glia::ms::blocks(n), 256, 0, (hipStream_t)stream>>>(n, x);
+  return (int)hipGetLastError();
+}
+
+int glia_ms_edema_mask(int64_t n, const float* i, double threshold, float* ed, void* stream) {
+  if (n < 0 || !i || !ed) return GLIA_RD_ERR_ARG;
+  if (n == 0) return 0;
+  glia::ms::k_ms_edema<<<glia::ms::blocks(n), 256, 0, (hipStream_t)stream>>>(n, i, (float)threshold, ed);
   return (int)hipGetLastError();
 }
 

@@ -49,6 +49,18 @@ int glia_ms_update_coefficients(int64_t n, float* rho_field, float* k_field, con
                                 const float* vt, const float* csf, double rho, double k, double r_gm_wm,
                                 double k_gm_wm, void* stream);
 
+/* Tumor::computeSegmentation (src/mat/Tumor.cpp:180-276, computeTumorSegmentation
+   src/cuda/Tumor.cu:14-33): the label of the largest of bg (0), c (1), wm (6), gm (5), vt (7), csf (8),
+   the tissues attenuated by (1 - c), first maximum wins in that order; seg receives the label as a
+   float. */
+int glia_ms_segmentation(int64_t n, const float* bg, const float* gm, const float* wm, const float* vt,
+                         const float* csf, const float* c, float* seg, void* stream);
+/* Tumor::clipTumor / MatProp::clipHealthyTissues (clipVectorCuda): x = max(x, 0), in place */
+int glia_ms_clip(int64_t n, float* x, void* stream);
+/* the mask of Tumor::computeEdema (src/mat/Tumor.cpp:160-178): ed = 1 where i > threshold
+   (invasive_threshold_, 0.001), else 0; the reference then smooths it with sigma 2 pi / n0 */
+int glia_ms_edema_mask(int64_t n, const float* i, double threshold, float* ed, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1427,3 +1427,19 @@ void orc_el_force(orc_t *o, const double *c, double ff, int use_tanh, double *f)
     for (int64_t i = 0; i < ng; i++) f[d * ng + i] *= use_tanh ? ff * tanh(cs[i]) : ff;
   free(cs);
 }
+
+/* Tumor::computeSegmentation (src/mat/Tumor.cpp:180-276): labels bg 0, c 1, wm 6, gm 5, vt 7, csf 8 */
+void orc_ms_segmentation(int64_t n, const double *bg, const double *gm, const double *wm, const double *vt,
+                         const double *csf, const double *c, double *seg) {
+  for (int64_t i = 0; i < n; i++) {
+    double mx = bg[i];
+    int ct = 0;
+    const double w = wm[i] * (1 - c[i]), g = gm[i] * (1 - c[i]), v = vt[i] * (1 - c[i]), cs = csf[i] * (1 - c[i]);
+    if (c[i] > mx) { mx = c[i]; ct = 1; }
+    if (w > mx) { mx = w; ct = 6; }
+    if (g > mx) { mx = g; ct = 5; }
+    if (v > mx) { mx = v; ct = 7; }
+    if (cs > mx) { mx = cs; ct = 8; }
+    seg[i] = ct;
+  }
+}

@@ -46,6 +46,7 @@ def _load():
         "orc_sl_interp": (None, [vp, _dp, _dp, _dp, _dp, _dp]),
         "orc_ms_sources": (None, [ctypes.c_int64] + [_dp] * 8 + [d, _dp]),
         "orc_ms_update_coefficients": (None, [ctypes.c_int64] + [_dp] * 6 + [d, d, d, d]),
+        "orc_ms_segmentation": (None, [ctypes.c_int64] + [_dp] * 7),
         "orc_el_material": (None, [vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "orc_el_matvec": (None, [vp, _dp, _dp, _dp, _dp, _dp]),
         "orc_el_force": (None, [vp, _dp, d, i, _dp]),
@@ -423,4 +424,13 @@ def ms_update_coefficients(bg, gm, vt, csf, rho, k, r_gm_wm, k_gm_wm):
     L.orc_ms_update_coefficients(n, rf, kf, *[_c(a).reshape(-1) for a in (bg, gm, vt, csf)], float(rho), float(k),
                                  float(r_gm_wm), float(k_gm_wm))
     return rf, kf
+
+
+def ms_segmentation(bg, gm, wm, vt, csf, c):
+    """computeSegmentation labels (bg 0, c 1, wm 6, gm 5, vt 7, csf 8) on flat arrays."""
+    L = _load()
+    n = np.asarray(bg).size
+    seg = np.zeros(n)
+    L.orc_ms_segmentation(n, *[_c(a).reshape(-1) for a in (bg, gm, wm, vt, csf, c)], seg)
+    return seg
 

@@ -40,3 +40,17 @@ def test_coefficients_by_hand(oracle_mod):
     rf, kf = oracle_mod.ms_update_coefficients(bg, gm, vt, csf, 10.0, 0.05, 0.2, 0.2)
     assert np.allclose(rf, [10 * (1 - (0.1 + 0.8 * 0.5 + 0.1)), 0.0])
     assert np.allclose(kf, [0.05 * (1 - (0.1 + 0.8 * 0.5 + 0.1)), 0.0])
+
+
+def test_segmentation_labels_by_hand(oracle_mod):
+    """computeSegmentation: the largest of bg, c and the tissues times (1 - c); ties keep the earlier
+    label in the order bg, c, wm, gm, vt, csf."""
+    bg = np.array([0.9, 0.1, 0.1, 0.0, 0.2, 0.3])
+    c = np.array([0.05, 0.8, 0.0, 0.0, 0.0, 0.3])
+    wm = np.array([0.0, 0.1, 0.7, 0.1, 0.0, 0.3 / 0.7])
+    gm = np.array([0.0, 0.0, 0.2, 0.6, 0.0, 0.0])
+    vt = np.array([0.0, 0.0, 0.0, 0.3, 0.7, 0.0])
+    csf = np.array([0.05, 0.0, 0.0, 0.0, 0.1, 0.0])
+    seg = oracle_mod.ms_segmentation(bg, gm, wm, vt, csf, c)
+    # last voxel: bg 0.3, c 0.3, wm (1 - 0.3) * 0.3 / 0.7 = 0.3: bg keeps the first maximum
+    assert seg.tolist() == [0, 1, 6, 5, 7, 0]
