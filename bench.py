@@ -496,9 +496,9 @@ def elastic_config(args):
     src/pde/ElasticitySolver.cpp:471-533) at 160^3, the mass-effect grid: synthetic tissue with a
     background, a Gaussian tumor, material properties from them, and the force along the tumor
     gradient; one step = one CG solve from u = 0 (rtol 1e-3 in the preconditioned norm).
-    Algorithmic bytes per CG iteration: 188 fields of 4 N^3 bytes (operator 116: 20 transforms,
+    Algorithmic bytes per CG iteration: 185 fields of 4 N^3 bytes (operator 116: 20 transforms,
     the spectral combinations, 7 products, 3 screened updates and the staging copies;
-    preconditioner 30; three dot products 15; the CG updates 27)."""
+    preconditioner 30; <p, Ap> and the fused <z, z>, <z, r> 12; the CG updates 27)."""
     import torch
     from glia_amd import elasticity
     sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
@@ -518,7 +518,7 @@ def elastic_config(args):
     t = (time.perf_counter() - t0) / args.steps
     s.close()
     R = 4.0 * n ** 3
-    bytes_it = 188 * R
+    bytes_it = 185 * R
     achieved = bytes_it * np.mean(its) / t / 1e9
     line = {
         "metric": "elasticity solves/s at 160^3 (CG, spectral preconditioner)",

@@ -158,6 +158,23 @@ int dot(glia_el_s* h, const float* const x[3], const float* const y[3], double* 
   return 0;
 }
 
+// <z, z> and <z, r> in one pass and one host read
+int dot_zz_zr(glia_el_s* h, const float* const z[3], const float* const r[3], double* zz, double* zr) {
+  Vec3C a{{z[0], z[1], z[2]}}, b{{r[0], r[1], r[2]}};
+  EL_HIP(h, glia::el::launch_dot3_zz_zr(h->st, h->ng, a, b, h->part));
+  const int np = glia::el::partial_count();
+  EL_HIP(h, hipMemcpyAsync(h->hpart.data(), h->part, sizeof(double) * 2 * np, hipMemcpyDeviceToHost, h->st));
+  EL_HIP(h, hipStreamSynchronize(h->st));
+  double s0 = 0, s1 = 0;
+  for (int i = 0; i < np; i++) {  // fixed order: deterministic
+    s0 += h->hpart[2 * i];
+    s1 += h->hpart[2 * i + 1];
+  }
+  *zz = s0;
+  *zr = s1;
+  return 0;
+}
+
 Vec3 mut(float* const p[3]) { return Vec3{{p[0], p[1], p[2]}}; }
 Vec3C cst(float* const p[3]) { return Vec3C{{p[0], p[1], p[2]}}; }
 
@@ -405,10 +422,13 @@ int glia_el_solve(glia_el_t h, const float* const f[3], float* const u[3], int m
       }
       a = beta / dpi;
       EL_HIP(h, glia::el::launch_cg_update(h->st, h->ng, mut(x), mut(r), cst(p), cst(w), a));
-      if ((rc = apply_prec(h, r, z)) || (rc = dot(h, z, z, &dp))) return rc;
+      // <z, z> for the test and <z, r> for the next beta in one pass (beta is unused when the
+      // test stops the loop)
+      double zr = 0;
+      if ((rc = apply_prec(h, r, z)) || (rc = dot_zz_zr(h, z, r, &dp, &zr))) return rc;
       dp = std::sqrt(dp);
       if ((why = converged(dp))) break;
-      if ((rc = dot(h, z, r, &beta))) return rc;
+      beta = zr;
       if (std::isnan(beta) || std::isinf(beta)) {
         why = KSP_DIVERGED_NANORINF;
         break;

@@ -81,6 +81,30 @@ __global__ void __launch_bounds__(256) k_dot3(int64_t ng, Vec3C x, Vec3C y, doub
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// per-block FP64 partials of <z, z> and <z, r> in one pass (part[2 b], part[2 b + 1])
+__global__ void __launch_bounds__(256) k_dot3_zz_zr(int64_t ng, Vec3C z, Vec3C r, double* __restrict__ part) {
+  double s0 = 0.0, s1 = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int k = 0; k < 3; k++)
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += stride) {
+      const double zi = z.p[k][i];
+      s0 += zi * zi;
+      s1 += zi * (double)r.p[k][i];
+    }
+  __shared__ double red[2][256];
+  red[0][threadIdx.x] = s0;
+  red[1][threadIdx.x] = s1;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 2) part[2 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
 // x += a p and r -= a w (the CG update)
 __global__ void __launch_bounds__(256) k_cg_update(int64_t ng, Vec3 x, Vec3 r, Vec3C p, Vec3C w, double a) {
   const float fa = (float)a;
@@ -132,6 +156,10 @@ int launch_fnms(hipStream_t st, int64_t ng, const float* a, const float* b, cons
 }
 int launch_dot3(hipStream_t st, int64_t ng, const Vec3C& x, const Vec3C& y, double* part) {
   k_dot3<<<EB, 256, 0, st>>>(ng, x, y, part);
+  return (int)hipGetLastError();
+}
+int launch_dot3_zz_zr(hipStream_t st, int64_t ng, const Vec3C& z, const Vec3C& r, double* part) {
+  k_dot3_zz_zr<<<EB, 256, 0, st>>>(ng, z, r, part);
   return (int)hipGetLastError();
 }
 int launch_cg_update(hipStream_t st, int64_t ng, const Vec3& x, const Vec3& r, const Vec3C& p, const Vec3C& w,

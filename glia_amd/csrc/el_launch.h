@@ -26,6 +26,8 @@ int launch_force_scale(hipStream_t st, int64_t ng, const float* g, const float* 
 int launch_mul(hipStream_t st, int64_t ng, const float* a, const float* b, float* out);
 int launch_fnms(hipStream_t st, int64_t ng, const float* a, const float* b, const float* c, float* out);
 int launch_dot3(hipStream_t st, int64_t ng, const Vec3C& x, const Vec3C& y, double* part);
+// <z, z> and <z, r> partials, interleaved
+int launch_dot3_zz_zr(hipStream_t st, int64_t ng, const Vec3C& z, const Vec3C& r, double* part);
 int launch_cg_update(hipStream_t st, int64_t ng, const Vec3& x, const Vec3& r, const Vec3C& p, const Vec3C& w,
                      double a);
 int launch_axpby3(hipStream_t st, int64_t ng, const Vec3& p, const Vec3C& z, double b, int sub);
