@@ -240,6 +240,29 @@ MS_SIGNATURES = {
 }
 
 
+# include/glia_me.h (mass-effect forward of model 4)
+class MeParams(ctypes.Structure):
+    _fields_ = [("forcing_factor", ctypes.c_double), ("use_tanh", ctypes.c_int), ("max_cfl", ctypes.c_double),
+                ("el", ElParams)]
+
+
+class MeInfo(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int), ("cfl_stop", ctypes.c_int), ("cfl_max", ctypes.c_double),
+                ("el_iterations", ctypes.c_int), ("diff_iterations", ctypes.c_int)]
+
+
+ME_SIGNATURES = {
+    "glia_me_params_init": (None, [ctypes.POINTER(MeParams)]),
+    "glia_me_create": (_i, [ctypes.POINTER(Config), ctypes.POINTER(_vp)]),
+    "glia_me_destroy": (_i, [_vp]),
+    "glia_me_last_error": (ctypes.c_char_p, [_vp]),
+    "glia_me_rd": (_vp, [_vp]),
+    "glia_me_el": (_vp, [_vp]),
+    "glia_me_sl": (_vp, [_vp]),
+    "glia_me_solve_state": (_i, [_vp, ctypes.POINTER(MeParams)] + [_vp] * 9 + [ctypes.POINTER(MeInfo)]),
+}
+
+
 KERNELS = ["xi_pupd", "yi_grad", "z_diff", "yf_div", "xf_matvec", "pcg_update"]
 # algorithmic bytes of one launch per member, in units of one field (4 n0 n1 n2 bytes): every
 # input read once and every output written once
@@ -266,7 +289,8 @@ def load():
             "(hipcc --offload-arch=gfx950); there is no CPU fallback" % LIB_PATH
         )
     lib = ctypes.CDLL(LIB_PATH)
-    tables = [SIGNATURES, IO_SIGNATURES, OPT_SIGNATURES, ADV_SIGNATURES, EL_SIGNATURES, MS_SIGNATURES]
+    tables = [SIGNATURES, IO_SIGNATURES, OPT_SIGNATURES, ADV_SIGNATURES, EL_SIGNATURES, MS_SIGNATURES,
+              ME_SIGNATURES]
     for name, (res, args) in [kv for t in tables for kv in t.items()]:
         f = getattr(lib, name)
         f.restype = res

@@ -1,0 +1,208 @@
+// me_capi.cpp -- extern "C" boundary (include/glia_me.h): the mass-effect forward of model 4
+// (PdeOperatorsMassEffect::solveState, src/pde/PdeOperatorsMassEffect.cpp:421-743) over the
+// diffusion engine, the elasticity solver and the advection solver, all on one stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/glia_me.h"
+#include "../../include/glia_ms.h"
+#include "me_launch.h"
+#include "rd_handle.h"
+
+struct glia_me_s {
+  glia_rd_t rd = nullptr;
+  glia_el_t el = nullptr;
+  glia_sl_t sl = nullptr;
+  hipStream_t st = nullptr;
+  int n[3] = {0, 0, 0};
+  int64_t ng = 0;
+  // u_old (3), force (3), advection outputs of gm, wm, vt, csf, mri (5), then the |v| partials
+  float* work = nullptr;
+  std::vector<float> hpart;
+  std::string err;
+};
+
+namespace {
+
+constexpr int kWorkFields = 11;
+
+int fail(glia_me_s* h, int rc, const std::string& msg) {
+  h->err = msg;
+  return rc;
+}
+
+}  // namespace
+
+#define ME_HIP(h, call)                                                                         \
+  do {                                                                                          \
+    const hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess) return fail(h, (int)e_, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// a call into one of the owned solvers: its own message on failure
+#define ME_SUB(h, call, lasterr)                                             \
+  do {                                                                       \
+    const int r_ = (call);                                                   \
+    if (r_) return fail(h, r_, std::string(#call) + ": " + (lasterr));       \
+  } while (0)
+
+extern "C" {
+
+void glia_me_params_init(glia_me_params* p) {
+  if (!p) return;
+  p->forcing_factor = 1e5;
+  p->use_tanh = 1;
+  p->max_cfl = 14.0;
+  glia_el_params_init(&p->el);
+}
+
+int glia_me_destroy(glia_me_t h) {
+  if (!h) return GLIA_RD_ERR_ARG;
+  if (h->st) (void)hipStreamSynchronize(h->st);
+  (void)hipFree(h->work);
+  if (h->sl) glia_sl_destroy(h->sl);
+  if (h->el) glia_el_destroy(h->el);
+  if (h->rd) glia_rd_destroy(h->rd);
+  delete h;
+  return 0;
+}
+
+int glia_me_create(const glia_rd_config* cfg, glia_me_t* out) {
+  if (!cfg || !out) return GLIA_RD_ERR_ARG;
+  *out = nullptr;
+  if (cfg->nmembers != 1) return GLIA_RD_ERR_ARG;
+  glia_me_s* h = new (std::nothrow) glia_me_s();
+  if (!h) return GLIA_RD_ERR_ARG;
+  // as glia_rd_create: a failed creation still hands out the handle, for glia_me_last_error, and
+  // the caller destroys it
+  *out = h;
+  int rc = glia_rd_create(cfg, &h->rd);
+  if (rc) {
+    h->err = std::string("engine: ") + (h->rd ? glia_rd_last_error(h->rd) : "creation failed");
+    return rc;
+  }
+  h->st = h->rd->e->stream();
+  for (int d = 0; d < 3; d++) h->n[d] = cfg->n[d];
+  h->ng = (int64_t)cfg->n[0] * cfg->n[1] * cfg->n[2];
+  if ((rc = glia_el_create(cfg->n, (void*)h->st, &h->el))) {
+    h->err = std::string("elasticity solver: ") + (h->el ? glia_el_last_error(h->el) : "creation failed");
+    return rc;
+  }
+  if ((rc = glia_sl_create(cfg->n, (void*)h->st, &h->sl))) {
+    h->err = std::string("advection solver: ") + (h->sl ? glia_sl_last_error(h->sl) : "creation failed");
+    return rc;
+  }
+  const int np = glia::me::partial_count();
+  h->hpart.resize(np);
+  if (hipMalloc((void**)&h->work, sizeof(float) * (kWorkFields * h->ng + np)) != hipSuccess) {
+    (void)hipGetLastError();
+    h->work = nullptr;
+    return fail(h, (int)hipErrorOutOfMemory, "out of device memory for the work fields");
+  }
+  return 0;
+}
+
+const char* glia_me_last_error(glia_me_t h) { return h ? h->err.c_str() : "null handle"; }
+glia_rd_t glia_me_rd(glia_me_t h) { return h ? h->rd : nullptr; }
+glia_el_t glia_me_el(glia_me_t h) { return h ? h->el : nullptr; }
+glia_sl_t glia_me_sl(glia_me_t h) { return h ? h->sl : nullptr; }
+
+int glia_me_solve_state(glia_me_t h, const glia_me_params* p, float* c, float* wm, float* gm, float* vt, float* csf,
+                        float* mri, const float* bg, float* const u[3], float* const v[3], glia_me_info* info) {
+  if (!h) return GLIA_RD_ERR_ARG;
+  if (!p || !c || !wm || !gm || !vt || !csf || !bg || !u || !v)
+    return fail(h, GLIA_RD_ERR_ARG, "null parameters or field");
+  for (int d = 0; d < 3; d++)
+    if (!u[d] || !v[d]) return fail(h, GLIA_RD_ERR_ARG, "null displacement or velocity component");
+  if (!(p->max_cfl > 0)) return fail(h, GLIA_RD_ERR_ARG, "max_cfl must be positive");
+  glia::RDEnsemble* e = h->rd->e.get();
+  const hipStream_t st = h->st;
+  const int64_t ng = h->ng;
+  const double dt = e->config().dt;
+  const int nt = e->config().nt;
+  const double h0 = 2.0 * M_PI / h->n[0], sigma = 2.0 * M_PI / h->n[0];
+  float* uold[3] = {h->work, h->work + ng, h->work + 2 * ng};
+  float* force[3] = {h->work + 3 * ng, h->work + 4 * ng, h->work + 5 * ng};
+  float* adv[5];
+  for (int k = 0; k < 5; k++) adv[k] = h->work + (6 + k) * ng;
+  float* part = h->work + kWorkFields * ng;
+  float* ct = e->field_ptr(0, glia::RS_CT);
+  float* rho = e->field_ptr(0, glia::RS_RHO);
+  float* kf = e->field_ptr(0, glia::RS_K);
+  float* ws = e->field_ptr(0, glia::RS_CH);
+  const int np = glia::me::partial_count();
+  const size_t fb = sizeof(float) * ng;
+  glia::MemberStats s0{};
+  ME_SUB(h, e->stats(0, &s0), e->error());
+  glia_me_info inf{};
+
+  // filterTumor, computeForce, the elasticity solve into u_old from zero (:456-460)
+  ME_HIP(h, (hipError_t)glia::me::launch_filter(st, ng, c, wm, gm));
+  for (int d = 0; d < 3; d++) ME_HIP(h, hipMemsetAsync(uold[d], 0, fb, st));
+  int its = 0;
+  ME_SUB(h, glia_el_force(h->el, c, p->forcing_factor, p->use_tanh, force, GLIA_RD_DEVICE), glia_el_last_error(h->el));
+  ME_SUB(h, glia_el_set_material(h->el, c, wm, gm, csf, vt, bg, &p->el, GLIA_RD_DEVICE), glia_el_last_error(h->el));
+  ME_SUB(h, glia_el_solve(h->el, force, uold, GLIA_RD_DEVICE, &its, nullptr, nullptr), glia_el_last_error(h->el));
+  inf.el_iterations += its;
+
+  for (int i = 0; i < nt + 1; i++) {
+    // CFL from the current velocity (:489-502)
+    ME_HIP(h, (hipError_t)glia::me::launch_vmax(st, ng, v[0], v[1], v[2], part));
+    ME_HIP(h, hipMemcpyAsync(h->hpart.data(), part, sizeof(float) * np, hipMemcpyDeviceToHost, st));
+    ME_HIP(h, hipStreamSynchronize(st));
+    float vmax = 0.0f;
+    for (int b = 0; b < np; b++) vmax = std::max(vmax, h->hpart[b]);
+    const double cfl = dt * (double)vmax / h0;
+    inf.cfl_max = std::max(inf.cfl_max, cfl);
+    if (cfl >= p->max_cfl) {
+      inf.cfl_stop = 1;
+      break;
+    }
+    // updateReacAndDiffCoefficients and precFactor (:578-582)
+    ME_SUB(h, glia_ms_update_coefficients(ng, rho, kf, bg, gm, vt, csf, e->rho(0), e->kappa(0), e->config().r_gm_wm,
+                                          e->config().k_gm_wm, (void*)st),
+           "coefficient update");
+    ME_SUB(h, e->prec_factor(), e->error());
+    // advection of gm, wm, vt, csf, c and the MRI on one set of trajectories (:607-621); c lands
+    // in the engine's c(T) field, where the diffusion solve and the reaction continue
+    ME_SUB(h, glia_sl_set_velocity(h->sl, v[0], v[1], v[2], GLIA_RD_DEVICE), glia_sl_last_error(h->sl));
+    const float* in[6] = {gm, wm, vt, csf, c, mri};
+    float* outp[6] = {adv[0], adv[1], adv[2], adv[3], ct, adv[4]};
+    const int modes[6] = {GLIA_SL_MASS_CONSERVATION, GLIA_SL_MASS_CONSERVATION, GLIA_SL_PURE_ADVECTION,
+                          GLIA_SL_PURE_ADVECTION,    GLIA_SL_MASS_CONSERVATION, GLIA_SL_PURE_ADVECTION};
+    ME_SUB(h, glia_sl_advect(h->sl, mri ? 6 : 5, in, outp, modes, dt), glia_sl_last_error(h->sl));
+    float* back[5] = {gm, wm, vt, csf, mri};
+    for (int k = 0; k < (mri ? 5 : 4); k++)
+      ME_HIP(h, hipMemcpyAsync(back[k], adv[k], fb, hipMemcpyDeviceToDevice, st));
+    // diffusion over dt and the logistic reaction (:628-631)
+    ME_SUB(h, e->diffusion_solve_field(glia::RS_CT, dt), e->error());
+    ME_HIP(h, (hipError_t)glia::me::launch_reaction(st, ng, ct, rho, (float)dt));
+    ME_HIP(h, hipMemcpyAsync(c, ct, fb, hipMemcpyDeviceToDevice, st));
+    // force, displacement, velocity (:634-664)
+    ME_SUB(h, glia_el_force(h->el, c, p->forcing_factor, p->use_tanh, force, GLIA_RD_DEVICE),
+           glia_el_last_error(h->el));
+    ME_SUB(h, glia_el_set_material(h->el, c, wm, gm, csf, vt, bg, &p->el, GLIA_RD_DEVICE), glia_el_last_error(h->el));
+    ME_SUB(h, glia_el_solve(h->el, force, u, GLIA_RD_DEVICE, &its, nullptr, nullptr), glia_el_last_error(h->el));
+    inf.el_iterations += its;
+    ME_HIP(h, (hipError_t)glia::me::launch_velocity(st, ng, v, u, uold, (float)(1.0 / dt)));
+    for (int d = 0; d < 3; d++) {
+      ME_HIP(h, hipMemcpyAsync(ws, v[d], fb, hipMemcpyDeviceToDevice, st));
+      ME_SUB(h, e->smooth_field(0, glia::RS_CH, sigma), e->error());
+      ME_HIP(h, hipMemcpyAsync(v[d], ws, fb, hipMemcpyDeviceToDevice, st));
+    }
+    inf.steps++;
+  }
+  ME_HIP(h, hipStreamSynchronize(st));
+  glia::MemberStats s1{};
+  ME_SUB(h, e->stats(0, &s1), e->error());
+  inf.diff_iterations = s1.its_total - s0.its_total;
+  if (info) *info = inf;
+  return 0;
+}
+
+}  // extern "C"

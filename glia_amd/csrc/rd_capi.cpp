@@ -10,12 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/glia_rd.h"
-#include "rd_solver.h"
-
-struct glia_rd_s {
-  std::unique_ptr<glia::RDEnsemble> e;
-  std::string err;
-};
+#include "rd_handle.h"
 
 using glia::RDEnsemble;
 

@@ -71,6 +71,8 @@ class RDEnsemble {
     cfg_.r_glm_wm = r_glm;
   }
   double ctx_dt() const { return ctx_dt_; }
+  double kappa(int m) const { return kappa_[m]; }  // TumorParameters::k_ of member m
+  double rho(int m) const { return rho_[m]; }      // TumorParameters::rho_ of member m
   int set_basis(int m, int np, const double* centers, double sigma);
   int set_data(int m, const float* d1, const float* obs_mask, double thr, bool device);
 

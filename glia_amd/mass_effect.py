@@ -1,0 +1,103 @@
+"""Python mirror of the reference's mass-effect forward of model 4 (PdeOperatorsMassEffect::solveState,
+src/pde/PdeOperatorsMassEffect.cpp:421-743) over libglia_rd.so (include/glia_me.h).  Fields are
+numpy arrays on the grid; they are staged through device memory for the call.  There is no CPU
+path: every step runs in the HIP library."""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .rd import GliaError
+
+
+class MassEffectSolver:
+    """The engine (dt, nt, k, rho and their gm/wm ratios), the elasticity solver and the advection
+    solver of one tumor on an n0 x n1 x n2 grid."""
+
+    def __init__(self, n, dt, nt, k=0.0, rho=0.0, k_gm_wm=0.0, r_gm_wm=0.0, rtol=1e-6, maxit=5000):
+        self.L = _lib.load()
+        self.shape = (n, n, n) if np.isscalar(n) else tuple(int(v) for v in n)
+        self.ng = int(np.prod(self.shape))
+        self.k, self.rho = float(k), float(rho)
+        c = _lib.Config()
+        self.L.glia_rd_config_init(ctypes.byref(c))
+        c.n[0], c.n[1], c.n[2] = self.shape
+        c.nmembers, c.dt, c.nt, c.order = 1, float(dt), int(nt), 2
+        c.pcg_rtol, c.pcg_maxit = float(rtol), int(maxit)
+        c.k_gm_wm, c.r_gm_wm = float(k_gm_wm), float(r_gm_wm)
+        self.h = ctypes.c_void_p()
+        rc = self.L.glia_me_create(ctypes.byref(c), ctypes.byref(self.h))
+        if rc != 0:
+            msg = self.L.glia_me_last_error(self.h).decode() if self.h.value else "glia_me_create failed"
+            if self.h.value:
+                self.L.glia_me_destroy(self.h)
+            self.h = None
+            raise GliaError(rc, msg)
+        self.rd = self.L.glia_me_rd(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.glia_me_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _f(self, a):
+        h = np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+        if h.size != self.ng:
+            raise ValueError("field has %d values, grid has %d" % (h.size, self.ng))
+        return h
+
+    def set_tissue(self, wm, gm, csf, vt):
+        """The initial tissue and DiffCoef/ReacCoef::setValues on it (the preconditioner's mean k)."""
+        arrs = [self._f(a) for a in (wm, gm, csf, vt)]
+        rc = self.L.glia_rd_set_tissue(self.rd, 0, *[a.ctypes.data for a in arrs], 0)
+        if rc == 0:
+            rc = self.L.glia_rd_set_coefficients(self.rd, 0, self.k, self.rho)
+        if rc != 0:
+            raise GliaError(rc, self.L.glia_rd_last_error(self.rd).decode())
+
+    def solve_state(self, c0, wm, gm, vt, csf, bg, u=None, v=None, mri=None, forcing_factor=1e5, use_tanh=True,
+                    max_cfl=14.0, **el):
+        """solveState(0) from c0 and the tissue; el overrides glia_el_params fields.  Returns a dict
+        of c, wm, gm, vt, csf, mri (None when not given), u, v (shape (3,) + grid) and info."""
+        import torch
+
+        p = _lib.MeParams()
+        self.L.glia_me_params_init(ctypes.byref(p))
+        p.forcing_factor, p.use_tanh, p.max_cfl = float(forcing_factor), int(bool(use_tanh)), float(max_cfl)
+        for key, val in el.items():
+            if not hasattr(p.el, key):
+                raise KeyError("unknown elasticity parameter %s" % key)
+            setattr(p.el, key, float(val))
+        dev = torch.device("cuda")
+
+        def put(a):
+            return torch.from_numpy(self._f(a).copy()).to(dev)
+
+        f = {k: put(a) for k, a in (("c", c0), ("wm", wm), ("gm", gm), ("vt", vt), ("csf", csf), ("bg", bg))}
+        m = None if mri is None else put(mri)
+        zero = np.zeros((3, self.ng), np.float32)
+        uu = [put(a) for a in (zero if u is None else np.asarray(u, np.float32).reshape(3, -1))]
+        vv = [put(a) for a in (zero if v is None else np.asarray(v, np.float32).reshape(3, -1))]
+        torch.cuda.synchronize()
+        up = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in uu])
+        vp = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in vv])
+        info = _lib.MeInfo()
+        rc = self.L.glia_me_solve_state(self.h, ctypes.byref(p), f["c"].data_ptr(), f["wm"].data_ptr(),
+                                        f["gm"].data_ptr(), f["vt"].data_ptr(), f["csf"].data_ptr(),
+                                        None if m is None else m.data_ptr(), f["bg"].data_ptr(), up, vp,
+                                        ctypes.byref(info))
+        if rc != 0:
+            raise GliaError(rc, self.L.glia_me_last_error(self.h).decode())
+        out = {k: f[k].cpu().numpy().reshape(self.shape) for k in ("c", "wm", "gm", "vt", "csf")}
+        out["mri"] = None if m is None else m.cpu().numpy().reshape(self.shape)
+        out["u"] = np.stack([t.cpu().numpy() for t in uu]).reshape((3,) + self.shape)
+        out["v"] = np.stack([t.cpu().numpy() for t in vv]).reshape((3,) + self.shape)
+        out["info"] = {"steps": info.steps, "cfl_stop": bool(info.cfl_stop), "cfl_max": info.cfl_max,
+                       "el_iterations": info.el_iterations, "diff_iterations": info.diff_iterations}
+        return out

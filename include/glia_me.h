@@ -1,0 +1,79 @@
+/*
+ * glia_me.h -- C ABI of the mass-effect forward of model 4: PdeOperatorsMassEffect::solveState
+ * (src/pde/PdeOperatorsMassEffect.cpp:421-743), the coupled tumor growth, elasticity and tissue
+ * advection.
+ *
+ * A glia_me_t owns one diffusion engine (glia_rd_t, one member), one elasticity solver (glia_el_t)
+ * and one advection solver (glia_sl_t) on one stream.  The engine carries what the reference keeps
+ * in TumorParameters and DiffCoef/ReacCoef: dt and nt (glia_rd_config), k and rho with their gm/wm
+ * ratios, and the preconditioner's mean k, fixed by glia_rd_set_tissue + glia_rd_set_coefficients
+ * on the initial tissue as DiffCoef::setValues fixes it at initialization (the loop rewrites the k
+ * and rho fields but not that mean).
+ *
+ * Conventions are those of glia_rd.h: FP32 fields of n0*n1*n2 values with z fastest, return 0, a
+ * negative GLIA_RD_ERR_* or a positive hipError_t, and glia_me_last_error() for the message.  The
+ * fields of glia_me_solve_state are device arrays.
+ */
+#ifndef GLIA_ME_H_
+#define GLIA_ME_H_
+
+#include "glia_adv.h"
+#include "glia_el.h"
+#include "glia_rd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct glia_me_s* glia_me_t;
+
+/* the mass-effect parameters of TumorParameters (include/Parameters.h) that the loop reads */
+typedef struct glia_me_params {
+  double forcing_factor; /* forcing_factor_ (gamma); 1e5 */
+  int use_tanh;          /* use_tanh_scaling_; 1 */
+  double max_cfl;        /* the loop's CFL limit (src/pde/PdeOperatorsMassEffect.cpp:472); 14 */
+  glia_el_params el;     /* Young's moduli, Poisson ratios, screening */
+} glia_me_params;
+
+void glia_me_params_init(glia_me_params* p);
+
+typedef struct glia_me_info {
+  int steps;          /* time steps completed (nt + 1 unless the CFL test stopped the loop) */
+  int cfl_stop;       /* 1 when CFL = dt max|v| / h0 reached max_cfl at the top of a step */
+  double cfl_max;     /* largest CFL seen */
+  int el_iterations;  /* elasticity CG iterations over all solves */
+  int diff_iterations;/* diffusion CG iterations over all steps */
+} glia_me_info;
+
+/* the engine (glia_rd_create with cfg->nmembers = 1; cfg->stream, or the engine's own stream, is
+   shared by the elasticity and advection solvers) and the two other solvers.  As with
+   glia_rd_create, a failed creation still sets *out (unless cfg or out is NULL or nmembers != 1)
+   so that glia_me_last_error can name the cause; destroy it either way. */
+int glia_me_create(const glia_rd_config* cfg, glia_me_t* out);
+int glia_me_destroy(glia_me_t h);
+const char* glia_me_last_error(glia_me_t h);
+/* the owned handles, valid until glia_me_destroy: the engine takes the tissue and coefficients */
+glia_rd_t glia_me_rd(glia_me_t h);
+glia_el_t glia_me_el(glia_me_t h);
+glia_sl_t glia_me_sl(glia_me_t h);
+
+/* PdeOperatorsMassEffect::solveState(0) without its output and feature branches.
+   c: Tumor::c_0_ in, c_t_ out.  wm, gm, vt, csf: MatProp's tissue in, filtered (filterTumor) and
+   advected out; mri may be NULL (else advected too); bg: MatProp::bg_, fixed.
+   u: Tumor::displacement_ (initial guess of the first step's elasticity solve in, the last
+   displacement out); v: Tumor::velocity_ (in and out; zero for a fresh tumor).
+   Before the loop: filterTumor, computeForce and the elasticity solve into u_old from zero.  Each
+   of the nt + 1 steps: the CFL test (stop when dt max|v| / h0 >= max_cfl), k and rho from the
+   tissue (updateReacAndDiffCoefficients, :98-138), precFactor, semi-Lagrangian steps of gm, wm
+   (mass conservation), vt, csf (pure advection), c (mass conservation) and mri (pure advection) on
+   one set of RK2 trajectories, the diffusion solve of c over dt, the logistic reaction, the force,
+   the elasticity solve into u, v = S((u - u_old) / dt) with S the smoother of sigma 2 pi / n0, and
+   u_old = u.  info may be NULL. */
+int glia_me_solve_state(glia_me_t h, const glia_me_params* p, float* c, float* wm, float* gm, float* vt, float* csf,
+                        float* mri, const float* bg, float* const u[3], float* const v[3], glia_me_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLIA_ME_H_ */

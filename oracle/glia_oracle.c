@@ -1443,3 +1443,76 @@ void orc_ms_segmentation(int64_t n, const double *bg, const double *gm, const do
     seg[i] = ct;
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Mass-effect forward (model 4)                                            */
+/* ------------------------------------------------------------------------ */
+
+/* PdeOperatorsMassEffect::solveState(0) (src/pde/PdeOperatorsMassEffect.cpp:421-743) without its
+   output and feature branches.  The diffusion part is o's: orc_set_tissue + orc_set_k + orc_set_rho
+   fix k_scale, rho_scale, the ratios and kavg as DiffCoef/ReacCoef::setValues do at initialization,
+   orc_set_time fixes dt and nt.  Each of the nt + 1 steps: CFL = dt max|v| / h0 from the velocity
+   (stop when >= max_cfl), updateReacAndDiffCoefficients (:98-138) into o->k and o->rho (kavg stays),
+   precFactor, RK2 trajectories of v, semi-Lagrangian steps of gm, wm (mass conservation), vt, csf
+   (pure advection), c (mass conservation) and mri (pure advection), the diffusion solve of c over
+   dt, the logistic reaction, the force and the elasticity solve, v = (u - u_old) / dt smoothed with
+   sigma 2 pi / n0, u_old = u.  Before the loop: filterTumor (wm, gm *= 1 - c, src/mat/MatProp.cpp:
+   205-224), force and elasticity solve into u_old (from zero).
+   c: c0 in, c(T) out; wm, gm, vt, csf, mri (mri may be NULL): in and advected out; bg fixed;
+   u (3 ng): Tumor::displacement_, the first step's initial guess in and the last displacement out;
+   v (3 ng): Tumor::velocity_, in and out; el_prm: the ten parameters of orc_el_material.
+   info: steps done, 1 if the CFL test stopped the loop, largest CFL seen, elasticity CG iterations,
+   diffusion CG iterations. */
+int orc_me_solve_state(orc_t *o, double *c, double *wm, double *gm, double *vt, double *csf, double *mri,
+                       const double *bg, double *u, double *v, double ff, int use_tanh, const double *el_prm,
+                       double max_cfl, double *info) {
+  const int64_t ng = o->ng;
+  const double dt = o->dt, h0 = 2.0 * M_PI / o->n[0], sigma = 2.0 * M_PI / o->n[0];
+  double *uold = dalloc(3 * ng), *f = dalloc(3 * ng), *q = dalloc(3 * ng), *tmp = dalloc(ng);
+  double *mu = dalloc(ng), *lam = dalloc(ng), *scr = dalloc(ng), avg[3], rn;
+  int reason, steps = 0, stopped = 0, el_its = 0, diff_its = 0;
+  double cfl_max = 0;
+  for (int64_t i = 0; i < ng; i++) {
+    wm[i] *= 1.0 - c[i];
+    gm[i] *= 1.0 - c[i];
+  }
+  orc_el_force(o, c, ff, use_tanh, f);
+  orc_el_material(o, c, wm, gm, csf, vt, bg, el_prm, mu, lam, scr, avg);
+  el_its += orc_el_solve(o, mu, lam, scr, avg, f, uold, &reason, &rn);
+  double *vx = v, *vy = v + ng, *vz = v + 2 * ng;
+  for (int it = 0; it < o->nt + 1; it++) {
+    double vmax = 0;
+    for (int64_t i = 0; i < ng; i++) vmax = fmax(vmax, sqrt(vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]));
+    const double cfl = dt * vmax / h0;
+    cfl_max = fmax(cfl_max, cfl);
+    if (cfl >= max_cfl) { stopped = 1; break; }
+    orc_ms_update_coefficients(ng, o->rho, o->k, bg, gm, vt, csf, o->rho_scale, o->k_scale, o->r_gm_wm, o->k_gm_wm);
+    orc_prec_factor(o);
+    orc_sl_trajectories(o, vx, vy, vz, dt, q, q + ng, q + 2 * ng);
+    orc_sl_solve(o, gm, vx, vy, vz, q, q + ng, q + 2 * ng, dt, 1);
+    orc_sl_solve(o, wm, vx, vy, vz, q, q + ng, q + 2 * ng, dt, 1);
+    orc_sl_solve(o, vt, vx, vy, vz, q, q + ng, q + 2 * ng, dt, 2);
+    orc_sl_solve(o, csf, vx, vy, vz, q, q + ng, q + 2 * ng, dt, 2);
+    orc_sl_solve(o, c, vx, vy, vz, q, q + ng, q + 2 * ng, dt, 1);
+    if (mri) orc_sl_solve(o, mri, vx, vy, vz, q, q + ng, q + 2 * ng, dt, 2);
+    diff_its += orc_diffusion_solve(o, c, dt);
+    orc_reaction(o, c, NULL, dt);
+    orc_el_force(o, c, ff, use_tanh, f);
+    orc_el_material(o, c, wm, gm, csf, vt, bg, el_prm, mu, lam, scr, avg);
+    el_its += orc_el_solve(o, mu, lam, scr, avg, f, u, &reason, &rn);
+    for (int64_t i = 0; i < 3 * ng; i++) v[i] = (u[i] - uold[i]) * (1.0 / dt);
+    for (int d = 0; d < 3; d++) {
+      orc_smooth(o, v + d * ng, tmp, sigma);
+      memcpy(v + d * ng, tmp, sizeof(double) * ng);
+    }
+    memcpy(uold, u, sizeof(double) * 3 * ng);
+    steps++;
+  }
+  info[0] = steps;
+  info[1] = stopped;
+  info[2] = cfl_max;
+  info[3] = el_its;
+  info[4] = diff_its;
+  free(uold); free(f); free(q); free(tmp); free(mu); free(lam); free(scr);
+  return 0;
+}

@@ -56,6 +56,7 @@ def _load():
         "orc_sl_trajectories": (None, [vp, _dp, _dp, _dp, d, _dp, _dp, _dp]),
         "orc_sl_solve": (None, [vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, d, i]),
         "orc_smooth": (None, [vp, _dp, _dp, d]),
+        "orc_me_solve_state": (i, [vp, _dp, _dp, _dp, _dp, _dp, vp, _dp, _dp, _dp, d, i, _dp, d, _dp]),
         "orc_set_tissue": (None, [vp, vp, vp, vp, vp, vp]),
         "orc_set_tissue_sinusoidal": (None, [vp]),
         "orc_set_k": (None, [vp, d, d, d]),
@@ -227,6 +228,30 @@ class Oracle:
                                   np.ascontiguousarray(avg, dtype=np.float64), self._vec(rhs), u,
                                   ctypes.byref(reason), ctypes.byref(rn))
         return u.reshape((3,) + self.shape), its, reason.value, rn.value
+
+    def me_solve_state(self, c0, wm, gm, vt, csf, bg, u=None, v=None, mri=None, ff=1e5, use_tanh=True,
+                       el_prm=None, max_cfl=14.0):
+        """PdeOperatorsMassEffect::solveState(0): the mass-effect forward of model 4 over nt + 1
+        steps, with the diffusion settings of set_tissue/set_k/set_rho/set_time.  Returns a dict of
+        c, wm, gm, vt, csf, mri (None when not given), u and v (3-component arrays) and info
+        (steps, cfl_stop, cfl_max, el_iterations, diff_iterations)."""
+        f = {k: np.array(a, dtype=np.float64, order="C").reshape(-1).copy()
+             for k, a in (("c", c0), ("wm", wm), ("gm", gm), ("vt", vt), ("csf", csf))}
+        m = None if mri is None else np.array(mri, dtype=np.float64, order="C").reshape(-1).copy()
+        uu = np.zeros(3 * self.ng) if u is None else self._vec(u).copy()
+        vv = np.zeros(3 * self.ng) if v is None else self._vec(v).copy()
+        prm = np.array(self.EL_DEFAULTS if el_prm is None else el_prm, dtype=np.float64)
+        info = np.zeros(5)
+        self.L.orc_me_solve_state(self.h, f["c"], f["wm"], f["gm"], f["vt"], f["csf"],
+                                  None if m is None else m.ctypes.data_as(ctypes.c_void_p), self._f(bg), uu, vv,
+                                  float(ff), int(bool(use_tanh)), prm, float(max_cfl), info)
+        out = {k: a.reshape(self.shape) for k, a in f.items()}
+        out["mri"] = None if m is None else m.reshape(self.shape)
+        out["u"] = uu.reshape((3,) + self.shape)
+        out["v"] = vv.reshape((3,) + self.shape)
+        out["info"] = {"steps": int(info[0]), "cfl_stop": bool(info[1]), "cfl_max": float(info[2]),
+                       "el_iterations": int(info[3]), "diff_iterations": int(info[4])}
+        return out
 
     def sl_interp(self, f, q):
         """f at the points x + q (q: three offset fields in grid units), cubic Lagrange, periodic."""

@@ -1,6 +1,6 @@
 """The C-ABI library loads on a machine without a GPU and exports every entry point that
-include/glia_rd.h, include/glia_io.h, include/glia_opt.h, include/glia_adv.h, include/glia_el.h and
-include/glia_ms.h declare (no compute calls here)."""
+include/glia_rd.h, include/glia_io.h, include/glia_opt.h, include/glia_adv.h, include/glia_el.h,
+include/glia_ms.h and include/glia_me.h declare (no compute calls here)."""
 import ctypes
 import os
 import re
@@ -37,6 +37,7 @@ def test_library_exports_every_declared_symbol():
     assert set(declared("glia_adv.h")) <= exported
     assert set(declared("glia_el.h")) <= exported
     assert set(declared("glia_ms.h")) <= exported
+    assert set(declared("glia_me.h")) <= exported
 
 
 def test_python_binding_covers_every_symbol():
@@ -47,6 +48,7 @@ def test_python_binding_covers_every_symbol():
     assert set(declared("glia_adv.h")) == set(_lib.ADV_SIGNATURES)
     assert set(declared("glia_el.h")) == set(_lib.EL_SIGNATURES)
     assert set(declared("glia_ms.h")) == set(_lib.MS_SIGNATURES)
+    assert set(declared("glia_me.h")) == set(_lib.ME_SIGNATURES)
 
 
 def test_config_defaults_without_gpu():

@@ -1,0 +1,47 @@
+"""The mass-effect forward of model 4 on the GPU (include/glia_me.h) against the FP64 oracle
+(PdeOperatorsMassEffect::solveState, src/pde/PdeOperatorsMassEffect.cpp:421-743).  The bar for the
+state is 1e-5 relative L2 on c(T) as for the reaction-diffusion forward; the displacement and the
+velocity come out of CG solves stopped at rtol 1e-3 and carry the FP32 rounding of those iterates,
+so they get 1e-4."""
+import numpy as np
+import pytest
+
+from me_cases import mass_effect_case
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("with_mri,use_tanh", [(True, True), (False, False)])
+def test_forward_matches_the_oracle(oracle_mod, with_mri, use_tanh):
+    from glia_amd.mass_effect import MassEffectSolver
+    n, dt, nt, k, rho, ff = 32, 0.04, 3, 0.01, 8.0, 1.2e5
+    print("case mass effect 32^3 nt 3 mri %s tanh %s" % (with_mri, use_tanh), flush=True)
+    c0, wm, gm, vt, csf, bg = mass_effect_case(n)
+    mri = (0.3 * wm + 0.7 * gm).astype(np.float32).astype(np.float64) if with_mri else None
+    o = oracle_mod.Oracle(n)
+    o.set_tissue(wm, gm, csf, vt, bg)
+    o.set_k(k, 0.2)
+    o.set_rho(rho, 0.2)
+    o.set_time(dt, nt)
+    ref = o.me_solve_state(c0, wm, gm, vt, csf, bg, mri=mri, ff=ff, use_tanh=use_tanh)
+    s = MassEffectSolver(n, dt, nt, k=k, rho=rho, k_gm_wm=0.2, r_gm_wm=0.2)
+    try:
+        s.set_tissue(wm, gm, csf, vt)
+        out = s.solve_state(c0, wm, gm, vt, csf, bg, mri=mri, forcing_factor=ff, use_tanh=use_tanh)
+    finally:
+        s.close()
+    errs = {key: rel(out[key], ref[key]) for key in ("c", "wm", "gm", "vt", "csf", "u", "v")}
+    if with_mri:
+        errs["mri"] = rel(out["mri"], ref["mri"])
+    print("info", out["info"], ref["info"], "rel", errs, flush=True)
+    assert out["info"]["steps"] == ref["info"]["steps"] == nt + 1
+    assert abs(out["info"]["el_iterations"] - ref["info"]["el_iterations"]) <= nt + 2
+    assert abs(out["info"]["cfl_max"] - ref["info"]["cfl_max"]) < 1e-3 * ref["info"]["cfl_max"]
+    for key in ("c", "wm", "gm", "vt", "csf", "mri"):
+        if key in errs:
+            assert errs[key] < 1e-5, key
+    assert errs["u"] < 1e-4 and errs["v"] < 1e-4

@@ -1,0 +1,74 @@
+"""The oracle's mass-effect forward of model 4 (PdeOperatorsMassEffect::solveState,
+src/pde/PdeOperatorsMassEffect.cpp:421-743) against what the mathematics fixes: the closed-form
+logistic growth when nothing moves and nothing diffuses, the CFL stop, and the symmetry of a problem
+invariant under the exchange of two axes.  Its pieces (advection, elasticity, force, diffusion,
+reaction) are pinned by their own tests."""
+import numpy as np
+import pytest
+
+from me_cases import mass_effect_case
+
+N = 32
+
+
+def _oracle(oracle_mod, fields, k, rho, dt, nt, r_gm_wm=0.0):
+    c, wm, gm, vt, csf, bg = fields
+    o = oracle_mod.Oracle(N)
+    o.set_tissue(wm, gm, csf, vt, bg)
+    o.set_k(k)
+    o.set_rho(rho, r_gm_wm)
+    o.set_time(dt, nt)
+    return o
+
+
+def test_logistic_growth_without_force_or_diffusion(oracle_mod):
+    """ff = 0 and k = 0: u = v = 0, the tissue moves not at all, and each of the nt + 1 steps is one
+    logistic step with rho(x) = rho max(0, 1 - (bg + (1 - r) gm + vt + csf)) of the filtered
+    tissue, so c(T) is the logistic solution over (nt + 1) dt."""
+    fields = mass_effect_case(N)
+    c0, wm, gm, vt, csf, bg = fields
+    dt, nt, rho, r = 0.05, 3, 6.0, 0.3
+    o = _oracle(oracle_mod, fields, 0.0, rho, dt, nt, r)
+    out = o.me_solve_state(c0, wm, gm, vt, csf, bg, ff=0.0)
+    assert out["info"]["steps"] == nt + 1 and not out["info"]["cfl_stop"]
+    assert np.abs(out["u"]).max() == 0 and np.abs(out["v"]).max() == 0
+    wmf, gmf = wm * (1 - c0), gm * (1 - c0)
+    assert np.allclose(out["wm"], wmf, rtol=0, atol=1e-15) and np.allclose(out["gm"], gmf, rtol=0, atol=1e-15)
+    assert np.array_equal(out["vt"], vt) and np.array_equal(out["csf"], csf)
+    rf = rho * np.maximum(0.0, 1.0 - (bg + (1 - r) * gmf + vt + csf))
+    T = (nt + 1) * dt
+    expected = c0 * np.exp(rf * T) / (1.0 - c0 + c0 * np.exp(rf * T))
+    assert np.max(np.abs(out["c"] - expected)) < 1e-12
+
+
+def test_cfl_stop_after_the_first_velocity(oracle_mod):
+    """Zero initial velocity passes the first CFL test; the velocity of the first step then stops
+    the loop at any limit below the CFL that step produced."""
+    fields = mass_effect_case(N)
+    c0, wm, gm, vt, csf, bg = fields
+    o = _oracle(oracle_mod, fields, 0.01, 8.0, 0.04, 3)
+    full = o.me_solve_state(c0, wm, gm, vt, csf, bg, ff=1.2e5)
+    assert full["info"]["steps"] == 4 and full["info"]["cfl_max"] > 0
+    o = _oracle(oracle_mod, fields, 0.01, 8.0, 0.04, 3)
+    lim = 0.5 * full["info"]["cfl_max"]
+    cut = o.me_solve_state(c0, wm, gm, vt, csf, bg, ff=1.2e5, max_cfl=lim)
+    assert cut["info"]["cfl_stop"] and 1 <= cut["info"]["steps"] < 4
+    assert cut["info"]["cfl_max"] >= lim
+
+
+def test_axis_exchange_symmetry(oracle_mod):
+    """Tissue and tumor invariant under x <-> y: c(T) and the tissue stay invariant and the
+    displacement's x and y components exchange (u_x(x, y, z) = u_y(y, x, z))."""
+    fields = mass_effect_case(N, symmetric=True)
+    c0, wm, gm, vt, csf, bg = fields
+    assert np.array_equal(c0, c0.transpose(1, 0, 2))
+    o = _oracle(oracle_mod, fields, 0.01, 8.0, 0.04, 2)
+    out = o.me_solve_state(c0, wm, gm, vt, csf, bg, ff=1.2e5)
+    sw = lambda a: a.transpose(1, 0, 2)  # noqa: E731
+    scale = np.abs(out["u"]).max()
+    assert scale > 0
+    for k in ("c", "wm", "gm", "vt", "csf"):
+        assert np.max(np.abs(out[k] - sw(out[k]))) < 1e-9 * max(1.0, np.abs(out[k]).max()), k
+    assert np.max(np.abs(out["u"][0] - sw(out["u"][1]))) < 1e-8 * scale
+    assert np.max(np.abs(out["u"][2] - sw(out["u"][2]))) < 1e-8 * scale
+    assert np.max(np.abs(out["v"][0] - sw(out["v"][1]))) < 1e-8 * np.abs(out["v"]).max()
