@@ -595,6 +595,58 @@ def masseffect_config(args):
     return 0
 
 
+def multispecies_config(args):
+    """The multispecies mass-effect forward of model 5 (PdeOperatorsMultiSpecies::solveState,
+    src/pde/PdeOperatorsMultiSpecies.cpp:235-508) at 160^3 with 40 steps of dt = 0.025: the tissue,
+    tumor seed and coefficients of the model-4 configuration, the reference's species parameters,
+    forcing factor 6e4 and the model's CFL limit 8; one step = one whole forward (41 time steps).
+    Algorithmic bytes per time step, in fields of 4 N^3 bytes: 185 per elasticity CG iteration,
+    55 + 36 m per diffusion solve of m CG iterations, and 330 for the rest (the 270 of the model-4
+    step, two more advected fields 6, k and its mean 10, clipping 14, the smoothing of i 8, D i 22)."""
+    from glia_amd.mass_effect import MassEffectSolver
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
+    from me_cases import mass_effect_case
+    n, dt, nt, FF = 160, 0.025, 40, 6e4
+    c0, wm, gm, vt, csf, bg = [a.astype(np.float32) for a in mass_effect_case(n)]
+    s = MassEffectSolver(n, dt, nt, k=0.01, rho=8.0)
+    s.set_tissue(wm, gm, csf, vt)
+    for _ in range(args.warmup):
+        s.solve_state_multispecies(c0, wm, gm, vt, csf, bg, forcing_factor=FF)
+    times, infos = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        out = s.solve_state_multispecies(c0, wm, gm, vt, csf, bg, forcing_factor=FF)
+        times.append(time.perf_counter() - t0)
+        infos.append(out["info"])
+        print("forward %.3f s info %s" % (times[-1], out["info"]), file=sys.stderr, flush=True)
+    s.close()
+    t = float(np.mean(times))
+    info = infos[-1]
+    steps = info["steps"]
+    R = 4.0 * n ** 3
+    m_diff = info["diff_iterations"] / max(steps, 1)
+    bytes_fwd = R * (185.0 * info["el_iterations"] + steps * (55.0 + 36.0 * m_diff + 330.0))
+    achieved = bytes_fwd / t / 1e9
+    line = {
+        "metric": "multispecies mass-effect forward time steps/s at 160^3 (model 5)",
+        "value": round(steps / t, 3), "unit": "time steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t * 1000, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "multispecies mass-effect forward of model 5 at 160^3: 40 steps of dt 0.025 (41 "
+                               "with the reference's nt + 1 loop), k 0.01, rho 8, forcing factor 6e4, default "
+                               "species parameters, synthetic tissue and Gaussian tumor; one step here = one "
+                               "whole forward", "n": n, "nt": nt, "dt": dt},
+        "time_steps": steps, "elasticity_cg_iterations": info["el_iterations"],
+        "diffusion_cg_iterations": info["diff_iterations"], "cfl_max": round(info["cfl_max"], 4),
+        "roofline": {"bound": "hbm", "kernel": "whole forward", "achieved": round(achieved, 1), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(achieved / 8000.0, 4),
+                     "algorithmic_bytes_per_forward": bytes_fwd, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -611,7 +663,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--config", choices=["ensemble256", "batch", "diff", "invert", "advect", "elastic",
-                                         "masseffect"],
+                                         "masseffect", "multispecies"],
                     default="ensemble256",
                     help="ensemble256: the 256^3 members (default, the headline metric); batch: the "
                          "uneven-grid batching configuration; diff: the 256^3 differentiation "
@@ -619,7 +671,8 @@ def main():
                          "inversion of 16 members at 128^3 through the inversion loop; advect: the "
                          "semi-Lagrangian step of the mass-effect models at 256^3; elastic: the "
                          "elasticity solve of the mass-effect model at 160^3; masseffect: the whole "
-                         "mass-effect forward of model 4 at 160^3, 40 steps")
+                         "mass-effect forward of model 4 at 160^3, 40 steps; multispecies: the "
+                         "multispecies forward of model 5 at 160^3, 40 steps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device-to-device copy first (the reference dispatch for "
                          "scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -649,6 +702,8 @@ def main():
         return elastic_config(args)
     if args.config == "masseffect":
         return masseffect_config(args)
+    if args.config == "multispecies":
+        return multispecies_config(args)
     n, M, nt, dt = args.n, args.members, args.nt, args.dt
     if args.copy_calibration:
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")

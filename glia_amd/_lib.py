@@ -260,6 +260,8 @@ ME_SIGNATURES = {
     "glia_me_el": (_vp, [_vp]),
     "glia_me_sl": (_vp, [_vp]),
     "glia_me_solve_state": (_i, [_vp, ctypes.POINTER(MeParams)] + [_vp] * 9 + [ctypes.POINTER(MeInfo)]),
+    "glia_me_solve_state_multispecies": (_i, [_vp, ctypes.POINTER(MeParams), ctypes.POINTER(MsParams)] + [_vp] * 12
+                                         + [ctypes.POINTER(MeInfo)]),
 }
 
 

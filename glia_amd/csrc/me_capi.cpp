@@ -1,5 +1,6 @@
-// me_capi.cpp -- extern "C" boundary (include/glia_me.h): the mass-effect forward of model 4
-// (PdeOperatorsMassEffect::solveState, src/pde/PdeOperatorsMassEffect.cpp:421-743) over the
+// me_capi.cpp -- extern "C" boundary (include/glia_me.h): the mass-effect forwards of model 4
+// (PdeOperatorsMassEffect::solveState, src/pde/PdeOperatorsMassEffect.cpp:421-743) and model 5
+// (PdeOperatorsMultiSpecies::solveState, src/pde/PdeOperatorsMultiSpecies.cpp:235-508) over the
 // diffusion engine, the elasticity solver and the advection solver, all on one stream.
 #include <hip/hip_runtime.h>
 
@@ -21,7 +22,8 @@ struct glia_me_s {
   hipStream_t st = nullptr;
   int n[3] = {0, 0, 0};
   int64_t ng = 0;
-  // u_old (3), force (3), advection outputs of gm, wm, vt, csf, mri (5), then the |v| partials
+  // u_old (3), force (3), advection outputs (6: gm, wm, vt, csf and mri or p, nec), the tumor core
+  // of the multispecies model (1), then the |v| partials
   float* work = nullptr;
   std::vector<float> hpart;
   std::string err;
@@ -29,7 +31,7 @@ struct glia_me_s {
 
 namespace {
 
-constexpr int kWorkFields = 11;
+constexpr int kWorkFields = 13;
 
 int fail(glia_me_s* h, int rc, const std::string& msg) {
   h->err = msg;
@@ -194,6 +196,130 @@ int glia_me_solve_state(glia_me_t h, const glia_me_params* p, float* c, float* w
       ME_HIP(h, hipMemcpyAsync(ws, v[d], fb, hipMemcpyDeviceToDevice, st));
       ME_SUB(h, e->smooth_field(0, glia::RS_CH, sigma), e->error());
       ME_HIP(h, hipMemcpyAsync(v[d], ws, fb, hipMemcpyDeviceToDevice, st));
+    }
+    inf.steps++;
+  }
+  ME_HIP(h, hipStreamSynchronize(st));
+  glia::MemberStats s1{};
+  ME_SUB(h, e->stats(0, &s1), e->error());
+  inf.diff_iterations = s1.its_total - s0.its_total;
+  if (info) *info = inf;
+  return 0;
+}
+
+int glia_me_solve_state_multispecies(glia_me_t h, const glia_me_params* p, const glia_ms_params* ms, const float* c0,
+                                     float* prolif, float* infil, float* nec, float* ox, float* wm, float* gm, float* vt,
+                                     float* csf, const float* bg, float* const u[3], float* const v[3],
+                                     glia_me_info* info) {
+  if (!h) return GLIA_RD_ERR_ARG;
+  if (!p || !ms || !c0 || !prolif || !infil || !nec || !ox || !wm || !gm || !vt || !csf || !bg || !u || !v)
+    return fail(h, GLIA_RD_ERR_ARG, "null parameters or field");
+  for (int d = 0; d < 3; d++)
+    if (!u[d] || !v[d]) return fail(h, GLIA_RD_ERR_ARG, "null displacement or velocity component");
+  if (!(p->max_cfl > 0)) return fail(h, GLIA_RD_ERR_ARG, "max_cfl must be positive");
+  glia::RDEnsemble* e = h->rd->e.get();
+  const hipStream_t st = h->st;
+  const int64_t ng = h->ng;
+  const double dt = e->config().dt;
+  const int nt = e->config().nt;
+  const bool moving = p->forcing_factor > 0;
+  const double h0 = 2.0 * M_PI / h->n[0], sigma = 2.0 * M_PI / h->n[0];
+  float* uold[3] = {h->work, h->work + ng, h->work + 2 * ng};
+  float* force[3] = {h->work + 3 * ng, h->work + 4 * ng, h->work + 5 * ng};
+  float* adv[6];
+  for (int k = 0; k < 6; k++) adv[k] = h->work + (6 + k) * ng;
+  float* core = h->work + 12 * ng;
+  float* part = h->work + kWorkFields * ng;
+  float* ib = e->field_ptr(0, glia::RS_CT);  // i through advection, clipping, smoothing and diffusion
+  float* rho = e->field_ptr(0, glia::RS_RHO);
+  float* kf = e->field_ptr(0, glia::RS_K);
+  float* ws = e->field_ptr(0, glia::RS_CH);  // D i for the sources, then the velocity smoothing
+  float* tissue[4] = {e->field_ptr(0, glia::RS_WM), e->field_ptr(0, glia::RS_GM), e->field_ptr(0, glia::RS_CSF),
+                      e->field_ptr(0, glia::RS_VT)};
+  const int np = glia::me::partial_count();
+  const size_t fb = sizeof(float) * ng;
+  glia::MemberStats s0{};
+  ME_SUB(h, e->stats(0, &s0), e->error());
+  glia_me_info inf{};
+  int its = 0;
+
+  // start (:251-283): species, clipTumor, filterTumor with p + i, the force of p and the
+  // elasticity solve into u, u_old = u
+  ME_HIP(h, (hipError_t)glia::me::launch_species_init(st, ng, c0, prolif, infil, ox));
+  ME_SUB(h, glia_ms_clip(ng, nec, (void*)st), "clip");
+  ME_HIP(h, (hipError_t)glia::me::launch_sum(st, ng, prolif, infil, nullptr, core));
+  ME_HIP(h, (hipError_t)glia::me::launch_filter(st, ng, core, wm, gm));
+  if (moving) {
+    ME_SUB(h, glia_el_force(h->el, prolif, p->forcing_factor, p->use_tanh, force, GLIA_RD_DEVICE),
+           glia_el_last_error(h->el));
+    ME_SUB(h, glia_el_set_material(h->el, prolif, wm, gm, csf, vt, bg, &p->el, GLIA_RD_DEVICE),
+           glia_el_last_error(h->el));
+    ME_SUB(h, glia_el_solve(h->el, force, u, GLIA_RD_DEVICE, &its, nullptr, nullptr), glia_el_last_error(h->el));
+    inf.el_iterations += its;
+    for (int d = 0; d < 3; d++) ME_HIP(h, hipMemcpyAsync(uold[d], u[d], fb, hipMemcpyDeviceToDevice, st));
+  }
+
+  for (int i = 0; i < nt + 1; i++) {
+    // CFL from the current velocity (:306-321)
+    ME_HIP(h, (hipError_t)glia::me::launch_vmax(st, ng, v[0], v[1], v[2], part));
+    ME_HIP(h, hipMemcpyAsync(h->hpart.data(), part, sizeof(float) * np, hipMemcpyDeviceToHost, st));
+    ME_HIP(h, hipStreamSynchronize(st));
+    float vmax = 0.0f;
+    for (int b = 0; b < np; b++) vmax = std::max(vmax, h->hpart[b]);
+    const double cfl = dt * (double)vmax / h0;
+    inf.cfl_max = std::max(inf.cfl_max, cfl);
+    if (cfl >= p->max_cfl) {
+      inf.cfl_stop = 1;
+      break;
+    }
+    // rho from the tissue, then k and its mean over the initial brain mask from the current wm,
+    // gm and csf (updateIsotropicCoefficients), precFactor (:412-416)
+    ME_SUB(h, glia_ms_update_coefficients(ng, rho, kf, bg, gm, vt, csf, e->rho(0), e->kappa(0), e->config().r_gm_wm,
+                                          e->config().k_gm_wm, (void*)st),
+           "coefficient update");
+    const float* cur[4] = {wm, gm, csf, vt};
+    for (int k = 0; k < 4; k++) ME_HIP(h, hipMemcpyAsync(tissue[k], cur[k], fb, hipMemcpyDeviceToDevice, st));
+    ME_SUB(h, e->set_kappa(0, e->kappa(0)), e->error());
+    ME_SUB(h, e->prec_factor(), e->error());
+    // advection (:418-431): gm, wm, p, i, nec conserve mass, vt and csf are transported
+    if (moving) {
+      ME_SUB(h, glia_sl_set_velocity(h->sl, v[0], v[1], v[2], GLIA_RD_DEVICE), glia_sl_last_error(h->sl));
+      const float* in[7] = {gm, wm, vt, csf, prolif, infil, nec};
+      float* outp[7] = {adv[0], adv[1], adv[2], adv[3], adv[4], ib, adv[5]};
+      const int modes[7] = {GLIA_SL_MASS_CONSERVATION, GLIA_SL_MASS_CONSERVATION, GLIA_SL_PURE_ADVECTION,
+                            GLIA_SL_PURE_ADVECTION,    GLIA_SL_MASS_CONSERVATION, GLIA_SL_MASS_CONSERVATION,
+                            GLIA_SL_MASS_CONSERVATION};
+      ME_SUB(h, glia_sl_advect(h->sl, 7, in, outp, modes, dt), glia_sl_last_error(h->sl));
+      float* back[6] = {gm, wm, vt, csf, prolif, nec};
+      for (int k = 0; k < 6; k++) ME_HIP(h, hipMemcpyAsync(back[k], adv[k], fb, hipMemcpyDeviceToDevice, st));
+    } else {
+      ME_HIP(h, hipMemcpyAsync(ib, infil, fb, hipMemcpyDeviceToDevice, st));
+    }
+    // clipHealthyTissues, clipTumor (:435-437)
+    float* clip[7] = {gm, wm, csf, vt, prolif, ib, nec};
+    for (int k = 0; k < 7; k++) ME_SUB(h, glia_ms_clip(ng, clip[k], (void*)st), "clip");
+    // i smoothed, D i, the diffusion solve of i (:440-447)
+    ME_SUB(h, e->smooth_field(0, glia::RS_CT, sigma), e->error());
+    ME_SUB(h, e->apply_d_field(glia::RS_CT, glia::RS_CH), e->error());
+    ME_SUB(h, e->diffusion_solve_field(glia::RS_CT, dt), e->error());
+    ME_HIP(h, hipMemcpyAsync(infil, ib, fb, hipMemcpyDeviceToDevice, st));
+    // computeSources and the tumor core (:452-456)
+    ME_SUB(h, glia_ms_sources(ng, prolif, infil, nec, ox, gm, wm, rho, ws, dt, ms, (void*)st), "sources");
+    ME_HIP(h, (hipError_t)glia::me::launch_sum(st, ng, prolif, nec, infil, core));
+    // force, displacement, velocity (:458-489)
+    if (moving) {
+      ME_SUB(h, glia_el_force(h->el, core, p->forcing_factor, p->use_tanh, force, GLIA_RD_DEVICE),
+             glia_el_last_error(h->el));
+      ME_SUB(h, glia_el_set_material(h->el, core, wm, gm, csf, vt, bg, &p->el, GLIA_RD_DEVICE),
+             glia_el_last_error(h->el));
+      ME_SUB(h, glia_el_solve(h->el, force, u, GLIA_RD_DEVICE, &its, nullptr, nullptr), glia_el_last_error(h->el));
+      inf.el_iterations += its;
+      ME_HIP(h, (hipError_t)glia::me::launch_velocity(st, ng, v, u, uold, (float)(1.0 / dt)));
+      for (int d = 0; d < 3; d++) {
+        ME_HIP(h, hipMemcpyAsync(ws, v[d], fb, hipMemcpyDeviceToDevice, st));
+        ME_SUB(h, e->smooth_field(0, glia::RS_CH, sigma), e->error());
+        ME_HIP(h, hipMemcpyAsync(v[d], ws, fb, hipMemcpyDeviceToDevice, st));
+      }
     }
     inf.steps++;
   }

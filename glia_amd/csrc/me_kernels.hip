@@ -69,7 +69,35 @@ __global__ void __launch_bounds__(256) k_velocity(int64_t ng, P3 v, P3C u, P3 uo
     }
 }
 
+// the multispecies start: p = c0 clipped at zero (clipTumor), i = 0, ox = 1
+__global__ void __launch_bounds__(256) k_species_init(int64_t ng, const float* __restrict__ c0, float* __restrict__ p,
+                                                      float* __restrict__ iv, float* __restrict__ ox) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < ng; k += stride) {
+    const float c = c0[k];
+    p[k] = c <= 0.0f ? 0.0f : c;
+    iv[k] = 0.0f;
+    ox[k] = 1.0f;
+  }
+}
+
+// out = a + b (+ c): the tumor core p + nec + i and the start's p + i
+__global__ void __launch_bounds__(256) k_sum(int64_t ng, const float* __restrict__ a, const float* __restrict__ b,
+                                             const float* __restrict__ c, float* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < ng; k += stride)
+    out[k] = c ? a[k] + b[k] + c[k] : a[k] + b[k];
+}
+
 int partial_count() { return MB; }
+int launch_species_init(hipStream_t st, int64_t ng, const float* c0, float* p, float* i, float* ox) {
+  k_species_init<<<MB, 256, 0, st>>>(ng, c0, p, i, ox);
+  return (int)hipGetLastError();
+}
+int launch_sum(hipStream_t st, int64_t ng, const float* a, const float* b, const float* c, float* out) {
+  k_sum<<<MB, 256, 0, st>>>(ng, a, b, c, out);
+  return (int)hipGetLastError();
+}
 
 int launch_filter(hipStream_t st, int64_t ng, const float* c, float* wm, float* gm) {
   k_filter<<<MB, 256, 0, st>>>(ng, c, wm, gm);

@@ -18,5 +18,10 @@ int launch_reaction(hipStream_t st, int64_t ng, float* c, const float* rho, floa
 int launch_velocity(hipStream_t st, int64_t ng, float* const v[3], const float* const u[3], float* const uold[3],
                     float inv_dt);
 
+// the multispecies start: p = max(c0, 0), i = 0, ox = 1 (src/pde/PdeOperatorsMultiSpecies.cpp:253-266)
+int launch_species_init(hipStream_t st, int64_t ng, const float* c0, float* p, float* i, float* ox);
+// out = a + b (+ c when c is not null)
+int launch_sum(hipStream_t st, int64_t ng, const float* a, const float* b, const float* c, float* out);
+
 }  // namespace me
 }  // namespace glia

@@ -1,5 +1,6 @@
-"""Python mirror of the reference's mass-effect forward of model 4 (PdeOperatorsMassEffect::solveState,
-src/pde/PdeOperatorsMassEffect.cpp:421-743) over libglia_rd.so (include/glia_me.h).  Fields are
+"""Python mirror of the reference's mass-effect forwards over libglia_rd.so (include/glia_me.h): model 4
+(PdeOperatorsMassEffect::solveState, src/pde/PdeOperatorsMassEffect.cpp:421-743) and the multispecies
+model 5 (PdeOperatorsMultiSpecies::solveState, src/pde/PdeOperatorsMultiSpecies.cpp:235-508).  Fields are
 numpy arrays on the grid; they are staged through device memory for the call.  There is no CPU
 path: every step runs in the HIP library."""
 import ctypes
@@ -96,6 +97,55 @@ class MassEffectSolver:
             raise GliaError(rc, self.L.glia_me_last_error(self.h).decode())
         out = {k: f[k].cpu().numpy().reshape(self.shape) for k in ("c", "wm", "gm", "vt", "csf")}
         out["mri"] = None if m is None else m.cpu().numpy().reshape(self.shape)
+        out["u"] = np.stack([t.cpu().numpy() for t in uu]).reshape((3,) + self.shape)
+        out["v"] = np.stack([t.cpu().numpy() for t in vv]).reshape((3,) + self.shape)
+        out["info"] = {"steps": info.steps, "cfl_stop": bool(info.cfl_stop), "cfl_max": info.cfl_max,
+                       "el_iterations": info.el_iterations, "diff_iterations": info.diff_iterations}
+        return out
+
+    def solve_state_multispecies(self, c0, wm, gm, vt, csf, bg, nec=None, u=None, v=None, forcing_factor=1e5,
+                                 use_tanh=True, max_cfl=8.0, ms=None, **el):
+        """PdeOperatorsMultiSpecies::solveState(0) from c0 and the tissue; ms overrides glia_ms_params
+        fields (death_rate, ox_source, ...), el the glia_el_params fields.  Returns a dict of p, i,
+        nec, ox, wm, gm, vt, csf, u, v and info."""
+        import torch
+
+        p = _lib.MeParams()
+        self.L.glia_me_params_init(ctypes.byref(p))
+        p.forcing_factor, p.use_tanh, p.max_cfl = float(forcing_factor), int(bool(use_tanh)), float(max_cfl)
+        for key, val in el.items():
+            if not hasattr(p.el, key):
+                raise KeyError("unknown elasticity parameter %s" % key)
+            setattr(p.el, key, float(val))
+        mp = _lib.MsParams()
+        self.L.glia_ms_params_init(ctypes.byref(mp))
+        for key, val in (ms or {}).items():
+            if not hasattr(mp, key):
+                raise KeyError("unknown multispecies parameter %s" % key)
+            setattr(mp, key, float(val))
+        dev = torch.device("cuda")
+
+        def put(a):
+            return torch.from_numpy(self._f(a).copy()).to(dev)
+
+        f = {k: put(a) for k, a in (("c0", c0), ("wm", wm), ("gm", gm), ("vt", vt), ("csf", csf), ("bg", bg))}
+        for k in ("p", "i", "ox"):
+            f[k] = torch.zeros(self.ng, dtype=torch.float32, device=dev)
+        f["nec"] = torch.zeros(self.ng, dtype=torch.float32, device=dev) if nec is None else put(nec)
+        zero = np.zeros((3, self.ng), np.float32)
+        uu = [put(a) for a in (zero if u is None else np.asarray(u, np.float32).reshape(3, -1))]
+        vv = [put(a) for a in (zero if v is None else np.asarray(v, np.float32).reshape(3, -1))]
+        torch.cuda.synchronize()
+        up = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in uu])
+        vp = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in vv])
+        info = _lib.MeInfo()
+        rc = self.L.glia_me_solve_state_multispecies(
+            self.h, ctypes.byref(p), ctypes.byref(mp), *[f[k].data_ptr() for k in
+                                                         ("c0", "p", "i", "nec", "ox", "wm", "gm", "vt", "csf", "bg")],
+            up, vp, ctypes.byref(info))
+        if rc != 0:
+            raise GliaError(rc, self.L.glia_me_last_error(self.h).decode())
+        out = {k: f[k].cpu().numpy().reshape(self.shape) for k in ("p", "i", "nec", "ox", "wm", "gm", "vt", "csf")}
         out["u"] = np.stack([t.cpu().numpy() for t in uu]).reshape((3,) + self.shape)
         out["v"] = np.stack([t.cpu().numpy() for t in vv]).reshape((3,) + self.shape)
         out["info"] = {"steps": info.steps, "cfl_stop": bool(info.cfl_stop), "cfl_max": info.cfl_max,

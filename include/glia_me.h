@@ -1,7 +1,9 @@
 /*
- * glia_me.h -- C ABI of the mass-effect forward of model 4: PdeOperatorsMassEffect::solveState
+ * glia_me.h -- C ABI of the mass-effect forwards: model 4, PdeOperatorsMassEffect::solveState
  * (src/pde/PdeOperatorsMassEffect.cpp:421-743), the coupled tumor growth, elasticity and tissue
- * advection.
+ * advection; and model 5, PdeOperatorsMultiSpecies::solveState
+ * (src/pde/PdeOperatorsMultiSpecies.cpp:235-508), the same coupling with proliferative,
+ * infiltrative and necrotic species and oxygen.
  *
  * A glia_me_t owns one diffusion engine (glia_rd_t, one member), one elasticity solver (glia_el_t)
  * and one advection solver (glia_sl_t) on one stream.  The engine carries what the reference keeps
@@ -19,6 +21,7 @@
 
 #include "glia_adv.h"
 #include "glia_el.h"
+#include "glia_ms.h"
 #include "glia_rd.h"
 
 #ifdef __cplusplus
@@ -71,6 +74,27 @@ glia_sl_t glia_me_sl(glia_me_t h);
    u_old = u.  info may be NULL. */
 int glia_me_solve_state(glia_me_t h, const glia_me_params* p, float* c, float* wm, float* gm, float* vt, float* csf,
                         float* mri, const float* bg, float* const u[3], float* const v[3], glia_me_info* info);
+
+/* PdeOperatorsMultiSpecies::solveState(0) (src/pde/PdeOperatorsMultiSpecies.cpp:235-508), the
+   multispecies forward of model 5, without its output, edema and segmentation branches (they feed
+   only the output).  The reference's CFL limit for this model is 8 (p->max_cfl).
+   c0: Tumor::c_0_ (in); prolif, infil, ox: the proliferative, infiltrative and oxygen species
+   (out; set to c0 clipped at zero, 0 and 1 at the start); nec: the necrotic species (in and out,
+   not reset, as in the reference); wm, gm, vt, csf: the tissue (in, filtered and advected out);
+   bg fixed; u, v: Tumor::displacement_ and velocity_ (in and out).
+   Start (:251-287): clipTumor, filterTumor with p + i, and when forcing_factor > 0 the force of p
+   and the elasticity solve into u, u_old = u.  Each of the nt + 1 steps: the CFL test; rho from the
+   tissue (updateReacAndDiffCoefficients); k from the current wm, gm and csf with its mean over the
+   initial brain mask (updateIsotropicCoefficients, DiffCoef::setValues); precFactor; when
+   forcing_factor > 0 the advection of gm, wm, p, i, nec (mass conservation), vt and csf (pure
+   advection); clipHealthyTissues and clipTumor; i smoothed with sigma 2 pi / n0; di = D i; the
+   diffusion solve of i over dt; computeSources (glia_ms_sources); c = p + nec + i; when
+   forcing_factor > 0 the force of c, the elasticity solve, v = S((u - u_old) / dt), u_old = u.
+   info may be NULL. */
+int glia_me_solve_state_multispecies(glia_me_t h, const glia_me_params* p, const glia_ms_params* ms, const float* c0,
+                                     float* prolif, float* infil, float* nec, float* ox, float* wm, float* gm, float* vt,
+                                     float* csf, const float* bg, float* const u[3], float* const v[3],
+                                     glia_me_info* info);
 
 #ifdef __cplusplus
 }

@@ -1516,3 +1516,101 @@ int orc_me_solve_state(orc_t *o, double *c, double *wm, double *gm, double *vt, 
   free(uold); free(f); free(q); free(tmp); free(mu); free(lam); free(scr);
   return 0;
 }
+
+static void clip0(int64_t n, double *x) {
+  for (int64_t i = 0; i < n; i++) x[i] = (x[i] <= 0.) ? 0. : x[i];
+}
+
+/* PdeOperatorsMultiSpecies::solveState(0) (src/pde/PdeOperatorsMultiSpecies.cpp:235-508), the
+   multispecies mass-effect forward of model 5, without its output, edema and segmentation branches
+   (they feed only the output).  The diffusion settings are o's, as for orc_me_solve_state.
+   Start (:251-287): p = c0, i = 0, ox = 1, clipTumor (p, i, nec >= 0), filterTumor with p + i, and
+   when ff > 0 the force of p and the elasticity solve into u (warm start), u_old = u.  Each of the
+   nt + 1 steps: the CFL test, the rho rule of updateReacAndDiffCoefficients, then
+   updateIsotropicCoefficients (k from the current wm, gm and csf and its mean over the initial
+   brain mask: DiffCoef::setValues, src/mat/DiffCoef.cpp:60-132), precFactor; when ff > 0 the
+   advection of gm, wm (mass conservation), vt, csf (pure advection), p, i, nec (mass
+   conservation); clipHealthyTissues and clipTumor; i smoothed with sigma 2 pi / n0; di = D i; the
+   diffusion solve of i; computeSources; c = p + nec + i; when ff > 0 the force of c, the
+   elasticity solve, v = S((u - u_old) / dt), u_old = u.
+   nec: in and out (the reference does not reset it); ox out; ms_prm: death_rate, ox_source,
+   ox_consumption, alpha_0, beta_0, ox_inv, ox_hypoxia.  info as for orc_me_solve_state. */
+int orc_ms_solve_state(orc_t *o, const double *c0, double *p, double *iv, double *nec, double *ox, double *wm,
+                       double *gm, double *vt, double *csf, const double *bg, double *u, double *v, double ff,
+                       int use_tanh, const double *el_prm, const double *ms_prm, double max_cfl, double *info) {
+  const int64_t ng = o->ng;
+  const double dt = o->dt, h0 = 2.0 * M_PI / o->n[0], sigma = 2.0 * M_PI / o->n[0];
+  double *uold = dalloc(3 * ng), *f = dalloc(3 * ng), *q = dalloc(3 * ng), *tmp = dalloc(ng);
+  double *ct = dalloc(ng), *di = dalloc(ng), *mu = dalloc(ng), *lam = dalloc(ng), *scr = dalloc(ng), avg[3], rn;
+  int reason, steps = 0, stopped = 0, el_its = 0, diff_its = 0;
+  double cfl_max = 0;
+  double *vx = v, *vy = v + ng, *vz = v + 2 * ng;
+  for (int64_t i = 0; i < ng; i++) {
+    p[i] = c0[i];
+    iv[i] = 0.0;
+    ox[i] = 1.0;
+  }
+  clip0(ng, p); clip0(ng, iv); clip0(ng, nec);
+  for (int64_t i = 0; i < ng; i++) {
+    ct[i] = p[i] + iv[i];
+    wm[i] *= 1.0 - ct[i];
+    gm[i] *= 1.0 - ct[i];
+  }
+  if (ff > 0) {
+    orc_el_force(o, p, ff, use_tanh, f);
+    orc_el_material(o, p, wm, gm, csf, vt, bg, el_prm, mu, lam, scr, avg);
+    el_its += orc_el_solve(o, mu, lam, scr, avg, f, u, &reason, &rn);
+    memcpy(uold, u, sizeof(double) * 3 * ng);
+  }
+  for (int it = 0; it < o->nt + 1; it++) {
+    double vmax = 0;
+    for (int64_t i = 0; i < ng; i++) vmax = fmax(vmax, sqrt(vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]));
+    const double cfl = dt * vmax / h0;
+    cfl_max = fmax(cfl_max, cfl);
+    if (cfl >= max_cfl) { stopped = 1; break; }
+    orc_ms_update_coefficients(ng, o->rho, o->k, bg, gm, vt, csf, o->rho_scale, o->k_scale, o->r_gm_wm, o->k_gm_wm);
+    memcpy(o->wm, wm, sizeof(double) * ng);
+    memcpy(o->gm, gm, sizeof(double) * ng);
+    memcpy(o->csf, csf, sizeof(double) * ng);
+    memcpy(o->vt, vt, sizeof(double) * ng);
+    orc_set_k(o, o->k_scale, o->k_gm_wm, o->k_glm_wm);
+    orc_prec_factor(o);
+    if (ff > 0) {
+      orc_sl_trajectories(o, vx, vy, vz, dt, q, q + ng, q + 2 * ng);
+      orc_sl_solve(o, gm, vx, vy, vz, q, q + ng, q + 2 * ng, dt, 1);
+      orc_sl_solve(o, wm, vx, vy, vz, q, q + ng, q + 2 * ng, dt, 1);
+      orc_sl_solve(o, vt, vx, vy, vz, q, q + ng, q + 2 * ng, dt, 2);
+      orc_sl_solve(o, csf, vx, vy, vz, q, q + ng, q + 2 * ng, dt, 2);
+      orc_sl_solve(o, p, vx, vy, vz, q, q + ng, q + 2 * ng, dt, 1);
+      orc_sl_solve(o, iv, vx, vy, vz, q, q + ng, q + 2 * ng, dt, 1);
+      orc_sl_solve(o, nec, vx, vy, vz, q, q + ng, q + 2 * ng, dt, 1);
+    }
+    clip0(ng, gm); clip0(ng, wm); clip0(ng, csf); clip0(ng, vt);
+    clip0(ng, p); clip0(ng, iv); clip0(ng, nec);
+    orc_smooth(o, iv, tmp, sigma);
+    memcpy(iv, tmp, sizeof(double) * ng);
+    orc_apply_d(o, iv, di);
+    diff_its += orc_diffusion_solve(o, iv, dt);
+    orc_ms_sources(ng, p, iv, nec, ox, gm, wm, o->rho, di, dt, ms_prm);
+    for (int64_t i = 0; i < ng; i++) ct[i] = p[i] + nec[i] + iv[i];
+    if (ff > 0) {
+      orc_el_force(o, ct, ff, use_tanh, f);
+      orc_el_material(o, ct, wm, gm, csf, vt, bg, el_prm, mu, lam, scr, avg);
+      el_its += orc_el_solve(o, mu, lam, scr, avg, f, u, &reason, &rn);
+      for (int64_t i = 0; i < 3 * ng; i++) v[i] = (u[i] - uold[i]) * (1.0 / dt);
+      for (int d = 0; d < 3; d++) {
+        orc_smooth(o, v + d * ng, tmp, sigma);
+        memcpy(v + d * ng, tmp, sizeof(double) * ng);
+      }
+      memcpy(uold, u, sizeof(double) * 3 * ng);
+    }
+    steps++;
+  }
+  info[0] = steps;
+  info[1] = stopped;
+  info[2] = cfl_max;
+  info[3] = el_its;
+  info[4] = diff_its;
+  free(uold); free(f); free(q); free(tmp); free(ct); free(di); free(mu); free(lam); free(scr);
+  return 0;
+}

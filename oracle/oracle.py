@@ -57,6 +57,7 @@ def _load():
         "orc_sl_solve": (None, [vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, d, i]),
         "orc_smooth": (None, [vp, _dp, _dp, d]),
         "orc_me_solve_state": (i, [vp, _dp, _dp, _dp, _dp, _dp, vp, _dp, _dp, _dp, d, i, _dp, d, _dp]),
+        "orc_ms_solve_state": (i, [vp] + [_dp] * 12 + [d, i, _dp, _dp, d, _dp]),
         "orc_set_tissue": (None, [vp, vp, vp, vp, vp, vp]),
         "orc_set_tissue_sinusoidal": (None, [vp]),
         "orc_set_k": (None, [vp, d, d, d]),
@@ -247,6 +248,32 @@ class Oracle:
                                   float(ff), int(bool(use_tanh)), prm, float(max_cfl), info)
         out = {k: a.reshape(self.shape) for k, a in f.items()}
         out["mri"] = None if m is None else m.reshape(self.shape)
+        out["u"] = uu.reshape((3,) + self.shape)
+        out["v"] = vv.reshape((3,) + self.shape)
+        out["info"] = {"steps": int(info[0]), "cfl_stop": bool(info[1]), "cfl_max": float(info[2]),
+                       "el_iterations": int(info[3]), "diff_iterations": int(info[4])}
+        return out
+
+    MS_DEFAULTS = (4.0, 1.0, 5.0, 0.1, 0.01, 0.8, 0.6)
+
+    def ms_solve_state(self, c0, wm, gm, vt, csf, bg, nec=None, u=None, v=None, ff=1e5, use_tanh=True,
+                       el_prm=None, ms_prm=None, max_cfl=8.0):
+        """PdeOperatorsMultiSpecies::solveState(0): the multispecies mass-effect forward of model 5
+        over nt + 1 steps.  Returns a dict of p, i, nec, ox, wm, gm, vt, csf, u, v and info."""
+        f = {k: np.array(a, dtype=np.float64, order="C").reshape(-1).copy()
+             for k, a in (("wm", wm), ("gm", gm), ("vt", vt), ("csf", csf))}
+        for k in ("p", "i", "ox"):
+            f[k] = np.zeros(self.ng)
+        f["nec"] = np.zeros(self.ng) if nec is None else np.array(nec, dtype=np.float64).reshape(-1).copy()
+        uu = np.zeros(3 * self.ng) if u is None else self._vec(u).copy()
+        vv = np.zeros(3 * self.ng) if v is None else self._vec(v).copy()
+        prm = np.array(self.EL_DEFAULTS if el_prm is None else el_prm, dtype=np.float64)
+        mprm = np.array(self.MS_DEFAULTS if ms_prm is None else ms_prm, dtype=np.float64)
+        info = np.zeros(5)
+        self.L.orc_ms_solve_state(self.h, self._f(c0), f["p"], f["i"], f["nec"], f["ox"], f["wm"], f["gm"], f["vt"],
+                                  f["csf"], self._f(bg), uu, vv, float(ff), int(bool(use_tanh)), prm, mprm,
+                                  float(max_cfl), info)
+        out = {k: a.reshape(self.shape) for k, a in f.items()}
         out["u"] = uu.reshape((3,) + self.shape)
         out["v"] = vv.reshape((3,) + self.shape)
         out["info"] = {"steps": int(info[0]), "cfl_stop": bool(info[1]), "cfl_max": float(info[2]),

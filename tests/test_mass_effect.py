@@ -72,3 +72,46 @@ def test_axis_exchange_symmetry(oracle_mod):
     assert np.max(np.abs(out["u"][0] - sw(out["u"][1]))) < 1e-8 * scale
     assert np.max(np.abs(out["u"][2] - sw(out["u"][2]))) < 1e-8 * scale
     assert np.max(np.abs(out["v"][0] - sw(out["v"][1]))) < 1e-8 * np.abs(out["v"]).max()
+
+
+def _full_tissue(n):
+    """wm + gm = 1 everywhere (no background), so every growth and death transfer of the
+    multispecies sources stays inside p + i + nec + gm + wm."""
+    x = np.arange(n) * 2 * np.pi / n
+    wm = 0.5 + 0.3 * np.sin(x)[:, None, None] * np.sin(x)[None, :, None] * np.ones(n)[None, None, :]
+    gm = 1.0 - wm
+    z = np.zeros_like(wm)
+    c0 = mass_effect_case(n)[0]
+    return c0, wm, gm, z, z.copy(), z.copy()
+
+
+def test_multispecies_conserves_cells_without_motion_or_diffusion(oracle_mod):
+    """Model 5 with no force and k = 0: nothing moves, the smoothing of i keeps its total, D i = 0,
+    and with wm + gm = 1 the sources move cells between p, i, nec, gm and wm only, so their grid
+    total stays at its value after filterTumor; i starts at 0 and oxygen at 1."""
+    c0, wm, gm, vt, csf, bg = _full_tissue(N)
+    o = _oracle(oracle_mod, (c0, wm, gm, vt, csf, bg), 0.0, 8.0, 0.02, 3)
+    out = o.ms_solve_state(c0, wm, gm, vt, csf, bg, ff=0.0)
+    assert out["info"]["steps"] == 4 and out["info"]["el_iterations"] == 0
+    assert np.abs(out["u"]).max() == 0 and np.abs(out["v"]).max() == 0
+    start = np.sum(c0 + (wm + gm) * (1 - c0))
+    end = sum(np.sum(out[k]) for k in ("p", "i", "nec", "gm", "wm"))
+    assert abs(end - start) < 1e-10 * start
+    assert out["i"].max() > 0 and out["nec"].max() > 0 and out["ox"].max() <= 1.0
+    # the last source step comes after the last clipping: round-off below zero only
+    assert min(out["p"].min(), out["gm"].min(), out["wm"].min()) > -1e-20
+
+
+def test_multispecies_axis_exchange_symmetry(oracle_mod):
+    """Model 5 on a problem invariant under x <-> y: every species and tissue stays invariant and
+    u_x, u_y exchange."""
+    fields = mass_effect_case(N, symmetric=True)
+    c0, wm, gm, vt, csf, bg = fields
+    o = _oracle(oracle_mod, fields, 0.01, 8.0, 0.04, 2)
+    out = o.ms_solve_state(c0, wm, gm, vt, csf, bg, ff=1.2e5)
+    sw = lambda a: a.transpose(1, 0, 2)  # noqa: E731
+    assert out["info"]["steps"] == 3 and out["info"]["el_iterations"] > 0
+    for k in ("p", "i", "nec", "ox", "wm", "gm", "vt", "csf"):
+        assert np.max(np.abs(out[k] - sw(out[k]))) < 1e-9 * max(1.0, np.abs(out[k]).max()), k
+    scale = np.abs(out["u"]).max()
+    assert np.max(np.abs(out["u"][0] - sw(out["u"][1]))) < 1e-8 * scale
