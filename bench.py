@@ -496,9 +496,10 @@ def elastic_config(args):
     src/pde/ElasticitySolver.cpp:471-533) at 160^3, the mass-effect grid: synthetic tissue with a
     background, a Gaussian tumor, material properties from them, and the force along the tumor
     gradient; one step = one CG solve from u = 0 (rtol 1e-3 in the preconditioned norm).
-    Algorithmic bytes per CG iteration: 185 fields of 4 N^3 bytes (operator 116: 20 transforms,
-    the spectral combinations, 7 products, 3 screened updates and the staging copies;
-    preconditioner 30; <p, Ap> and the fused <z, z>, <z, r> 12; the CG updates 27)."""
+    Algorithmic bytes per CG iteration: 167 fields of 4 N^3 bytes (operator 110: 20 transforms,
+    the spectral combinations, 7 products and 3 screened updates, with the CG direction transformed
+    in place; preconditioner 18: 3 + 3 transforms in place and the spectral inverse; <p, Ap> and the
+    fused <z, z>, <z, r> 12; the CG updates 27)."""
     import torch
     from glia_amd import elasticity
     sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
@@ -518,7 +519,7 @@ def elastic_config(args):
     t = (time.perf_counter() - t0) / args.steps
     s.close()
     R = 4.0 * n ** 3
-    bytes_it = 185 * R
+    bytes_it = 167 * R
     achieved = bytes_it * np.mean(its) / t / 1e9
     line = {
         "metric": "elasticity solves/s at 160^3 (CG, spectral preconditioner)",
@@ -545,7 +546,7 @@ def masseffect_config(args):
     reference's value for the SRI atlas; at 1.2e5 this tumor reaches the CFL limit of 14 after 31
     steps and the loop stops); one step = one whole forward (nt + 1 = 41 time steps, 42 elasticity
     solves).
-    Algorithmic bytes per time step, in fields of 4 N^3 bytes: 185 per elasticity CG iteration,
+    Algorithmic bytes per time step, in fields of 4 N^3 bytes: 167 per elasticity CG iteration,
     55 + 36 m per diffusion solve of m CG iterations, and 270 for the rest (elasticity set-up
     with the initial residual 150: force 10, material 10, operator and preconditioner 130; CFL 3,
     coefficients 6, velocity staging 6, trajectories and div v 14, advection of five fields 14,
@@ -573,7 +574,7 @@ def masseffect_config(args):
     steps = info["steps"]
     R = 4.0 * n ** 3
     m_diff = info["diff_iterations"] / max(steps, 1)
-    bytes_fwd = R * (185.0 * info["el_iterations"] + steps * (55.0 + 36.0 * m_diff + 270.0))
+    bytes_fwd = R * (167.0 * info["el_iterations"] + steps * (55.0 + 36.0 * m_diff + 270.0))
     achieved = bytes_fwd / t / 1e9
     line = {
         "metric": "mass-effect forward time steps/s at 160^3 (model 4)",
@@ -600,7 +601,7 @@ def multispecies_config(args):
     src/pde/PdeOperatorsMultiSpecies.cpp:235-508) at 160^3 with 40 steps of dt = 0.025: the tissue,
     tumor seed and coefficients of the model-4 configuration, the reference's species parameters,
     forcing factor 6e4 and the model's CFL limit 8; one step = one whole forward (41 time steps).
-    Algorithmic bytes per time step, in fields of 4 N^3 bytes: 185 per elasticity CG iteration,
+    Algorithmic bytes per time step, in fields of 4 N^3 bytes: 167 per elasticity CG iteration,
     55 + 36 m per diffusion solve of m CG iterations, and 330 for the rest (the 270 of the model-4
     step, two more advected fields 6, k and its mean 10, clipping 14, the smoothing of i 8, D i 22)."""
     from glia_amd.mass_effect import MassEffectSolver
@@ -625,7 +626,7 @@ def multispecies_config(args):
     steps = info["steps"]
     R = 4.0 * n ** 3
     m_diff = info["diff_iterations"] / max(steps, 1)
-    bytes_fwd = R * (185.0 * info["el_iterations"] + steps * (55.0 + 36.0 * m_diff + 330.0))
+    bytes_fwd = R * (167.0 * info["el_iterations"] + steps * (55.0 + 36.0 * m_diff + 330.0))
     achieved = bytes_fwd / t / 1e9
     line = {
         "metric": "multispecies mass-effect forward time steps/s at 160^3 (model 5)",

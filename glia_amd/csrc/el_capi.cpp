@@ -50,7 +50,9 @@ struct glia_el_s {
   int64_t ng = 0;
   hipStream_t st = nullptr;
   float *mu = nullptr, *lam = nullptr, *scr = nullptr;
-  float* v[6][3] = {};  // CG vectors: x staging, b, r, z, p, w
+  // CG vectors: x staging, b, r, z, p, w.  r, z and p live in real fields of the engine that the
+  // solver does not otherwise use, so the transforms read and write them in place
+  float* v[6][3] = {};
   double* part = nullptr;
   std::vector<double> hpart;
   double avg[3] = {0, 0, 0};
@@ -59,11 +61,12 @@ struct glia_el_s {
   ~glia_el_s() {
     for (float* p : {mu, lam, scr})
       if (p) (void)hipFree(p);
-    for (auto& a : v)
-      for (float* p : a)
-        if (p) (void)hipFree(p);
+    for (int k = 0; k < 6; k++)
+      for (float* p : v[k])
+        if (p && !engine_owned(k)) (void)hipFree(p);
     if (part) (void)hipFree(part);
   }
+  static bool engine_owned(int k) { return k == 2 || k == 3 || k == 4; }  // VR, VZ, VP
 };
 
 namespace {
@@ -89,11 +92,22 @@ int hip_fail(glia_el_s* h, int e, const char* what) {
 
 float* slot(glia_el_s* h, int rs) { return h->e->field_ptr(0, rs); }
 
-// real field (device) -> packed spectrum in complex slot cs, through RS_WM
+// the engine's real field at p, or -1
+int engine_slot(glia_el_s* h, const float* p) {
+  for (int s = 0; s < glia::RS_NUM; s++)
+    if (slot(h, s) == p) return s;
+  return -1;
+}
+
+// real field (device) -> packed spectrum in complex slot cs, in place when the field is one of the
+// engine's, else through RS_WM
 int forward(glia_el_s* h, const float* src, int cs) {
-  if (src != slot(h, glia::RS_WM))
+  int s = engine_slot(h, src);
+  if (s < 0) {
     EL_HIP(h, hipMemcpyAsync(slot(h, glia::RS_WM), src, sizeof(float) * h->ng, hipMemcpyDeviceToDevice, h->st));
-  EL_ENG(h, h->e->fft_r2c(glia::RS_WM, cs));
+    s = glia::RS_WM;
+  }
+  EL_ENG(h, h->e->fft_r2c(s, cs));
   return 0;
 }
 
@@ -140,8 +154,13 @@ int apply_prec(glia_el_s* h, const float* const f[3], float* const u[3]) {
   const int in[3] = {CU[0], CU[1], CU[2]}, out[3] = {CS_S[0], CS_S[1], CS_S[2]};
   EL_ENG(h, h->e->el_prec(in, out, h->avg[0], h->avg[1], h->avg[2]));
   for (int d = 0; d < 3; d++) {
-    EL_ENG(h, h->e->fft_c2r(out[d], glia::RS_GM));
-    EL_HIP(h, hipMemcpyAsync(u[d], slot(h, glia::RS_GM), sizeof(float) * h->ng, hipMemcpyDeviceToDevice, h->st));
+    const int s = engine_slot(h, u[d]);
+    if (s >= 0 && s != glia::RS_WM && s != glia::RS_GM) {
+      EL_ENG(h, h->e->fft_c2r(out[d], s));
+    } else {
+      EL_ENG(h, h->e->fft_c2r(out[d], glia::RS_GM));
+      EL_HIP(h, hipMemcpyAsync(u[d], slot(h, glia::RS_GM), sizeof(float) * h->ng, hipMemcpyDeviceToDevice, h->st));
+    }
   }
   return 0;
 }
@@ -238,9 +257,17 @@ int glia_el_create(const int n[3], void* stream, glia_el_t* out) {
   h->ng = (int64_t)n[0] * n[1] * n[2];
   h->st = h->e->stream();
   const size_t b = sizeof(float) * h->ng;
+  // r, z, p in the engine's real fields that the elasticity solver never stages through (it uses
+  // RS_WM and RS_GM)
+  const int own[3][3] = {{glia::RS_CSF, glia::RS_VT, glia::RS_FILTER},
+                         {glia::RS_OBS, glia::RS_D1, glia::RS_C0},
+                         {glia::RS_CT, glia::RS_A0, glia::RS_CH}};
+  for (int k = 0; k < 3; k++)
+    for (int d = 0; d < 3; d++) h->v[2 + k][d] = slot(h, own[k][d]);
   std::vector<float**> bufs = {&h->mu, &h->lam, &h->scr};
-  for (auto& a : h->v)
-    for (float*& p : a) bufs.push_back(&p);
+  for (int k = 0; k < 6; k++)
+    if (!glia_el_s::engine_owned(k))
+      for (float*& p : h->v[k]) bufs.push_back(&p);
   for (float** p : bufs) {
     const hipError_t e = hipMalloc((void**)p, b);
     if (e != hipSuccess) {
