@@ -212,6 +212,7 @@ int RDEnsemble::upload_members() {
 
 RDEnsemble::~RDEnsemble() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  if (smooth_taps_) (void)hipFree(smooth_taps_);
   for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
   for (auto& r : prof_recs_) {
     (void)hipEventDestroy(r.a);
@@ -1166,12 +1167,26 @@ void RDEnsemble::smoother_taps(double sigma, std::vector<double>& taps, int& kr)
 int RDEnsemble::smooth_field(int m, int rslot, double sigma) {
   if (m < 0 || m >= nm_ || rslot < 0 || rslot >= RS_NUM) return fail(-2, "bad member or field");
   if (!(sigma > 0)) return 0;
-  std::vector<double> taps;
-  int kr = 0;
-  smoother_taps(sigma, taps, kr);
-  double* td = nullptr;
-  GLIA_HIP(hipMalloc((void**)&td, sizeof(double) * taps.size()));
-  int rc = copy_sync(td, taps.data(), sizeof(double) * taps.size(), hipMemcpyHostToDevice);
+  int rc = 0;
+  if (sigma != smooth_sigma_) {  // the mass-effect loops smooth with one width many times
+    std::vector<double> taps;
+    int kr = 0;
+    smoother_taps(sigma, taps, kr);
+    if (taps.size() > smooth_cap_) {
+      GLIA_HIP(hipStreamSynchronize(stream_));
+      if (smooth_taps_) (void)hipFree(smooth_taps_);
+      smooth_taps_ = nullptr;
+      smooth_cap_ = 0;
+      GLIA_HIP(hipMalloc((void**)&smooth_taps_, sizeof(double) * taps.size()));
+      smooth_cap_ = taps.size();
+    }
+    rc = copy_sync(smooth_taps_, taps.data(), sizeof(double) * taps.size(), hipMemcpyHostToDevice);
+    if (rc) return rc;
+    smooth_sigma_ = sigma;
+    smooth_kr_ = kr;
+  }
+  const double* td = smooth_taps_;
+  const int kr = smooth_kr_;
   float* f = mh_[m].r[rslot];
   float* scratch = reinterpret_cast<float*>(mh_[m].c[CS_T7]);  // n0*n1*n2 floats
   LaunchCtx c = ctx();
@@ -1181,7 +1196,6 @@ int RDEnsemble::smooth_field(int m, int rslot, double sigma) {
   if (!rc) rc = launch_smooth_axis(c, f, scratch, 2, td + 2 * stride, kr);
   if (!rc) rc = check(hipMemcpyAsync(f, scratch, sizeof(float) * d_.ng, hipMemcpyDeviceToDevice, stream_), "smooth copy");
   if (!rc) rc = sync();
-  (void)hipFree(td);
   return rc;
 }
 

@@ -186,6 +186,10 @@ class RDEnsemble {
   };
   bool quad_on_ = true;     // adjoint steps accumulate the kappa/rho gradient quadratures
   bool use_graphs_ = true;  // replay CG iterations from captured graphs (not while profiling)
+  double* smooth_taps_ = nullptr;  // device taps of the last smoothing width (three axes)
+  size_t smooth_cap_ = 0;          // doubles allocated at smooth_taps_
+  double smooth_sigma_ = -1.0;
+  int smooth_kr_ = 0;
   std::map<std::pair<int, int>, hipGraphExec_t> graphs_;  // (state slot, flag slot) -> graph
   bool prof_on_ = false;
   std::vector<ProfRec> prof_recs_;
