@@ -115,3 +115,23 @@ def test_multispecies_axis_exchange_symmetry(oracle_mod):
         assert np.max(np.abs(out[k] - sw(out[k]))) < 1e-9 * max(1.0, np.abs(out[k]).max()), k
     scale = np.abs(out["u"]).max()
     assert np.max(np.abs(out["u"][0] - sw(out["u"][1]))) < 1e-8 * scale
+
+
+def test_c_abi_rejects_missing_handles_without_a_gpu():
+    """The mass-effect entry points answer a null handle or config with GLIA_RD_ERR_ARG before any
+    device call, and name a null handle in the error text."""
+    import ctypes
+
+    from glia_amd import _lib
+    L = _lib.load()
+    out = ctypes.c_void_p()
+    assert L.glia_me_create(None, ctypes.byref(out)) == -2 and not out.value
+    assert L.glia_me_destroy(None) == -2
+    assert L.glia_me_last_error(None) == b"null handle"
+    assert L.glia_me_rd(None) is None and L.glia_me_el(None) is None and L.glia_me_sl(None) is None
+    info = _lib.MeInfo()
+    assert L.glia_me_solve_state(None, None, *([None] * 9), ctypes.byref(info)) == -2
+    assert L.glia_me_solve_state_multispecies(None, None, None, *([None] * 12), ctypes.byref(info)) == -2
+    p = _lib.MeParams()
+    L.glia_me_params_init(ctypes.byref(p))
+    assert p.forcing_factor == 1e5 and p.use_tanh == 1 and p.max_cfl == 14.0 and p.el.E_tumor == 8000.0
