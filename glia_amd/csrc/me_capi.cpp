@@ -38,6 +38,16 @@ int fail(glia_me_s* h, int rc, const std::string& msg) {
   return rc;
 }
 
+// a handle whose creation failed part way has no engine, solvers or work fields
+bool complete(const glia_me_s* h) { return h->rd && h->rd->e && h->el && h->sl && h->work; }
+
+// the CFL limit of a call: p->max_cfl when positive, else the reference's limit of the model
+// (14 in PdeOperatorsMassEffect.cpp:472, 8 in PdeOperatorsMultiSpecies.cpp:296); -1 for NaN
+double cfl_limit(const glia_me_params* p, double model_default) {
+  if (std::isnan(p->max_cfl)) return -1.0;
+  return p->max_cfl > 0 ? p->max_cfl : model_default;
+}
+
 }  // namespace
 
 #define ME_HIP(h, call)                                                                         \
@@ -59,14 +69,14 @@ void glia_me_params_init(glia_me_params* p) {
   if (!p) return;
   p->forcing_factor = 1e5;
   p->use_tanh = 1;
-  p->max_cfl = 14.0;
+  p->max_cfl = 0.0;  // the model's own limit: 14 for model 4, 8 for model 5
   glia_el_params_init(&p->el);
 }
 
 int glia_me_destroy(glia_me_t h) {
   if (!h) return GLIA_RD_ERR_ARG;
   if (h->st) (void)hipStreamSynchronize(h->st);
-  (void)hipFree(h->work);
+  if (h->work) (void)hipFree(h->work);
   if (h->sl) glia_sl_destroy(h->sl);
   if (h->el) glia_el_destroy(h->el);
   if (h->rd) glia_rd_destroy(h->rd);
@@ -121,7 +131,9 @@ int glia_me_solve_state(glia_me_t h, const glia_me_params* p, float* c, float* w
     return fail(h, GLIA_RD_ERR_ARG, "null parameters or field");
   for (int d = 0; d < 3; d++)
     if (!u[d] || !v[d]) return fail(h, GLIA_RD_ERR_ARG, "null displacement or velocity component");
-  if (!(p->max_cfl > 0)) return fail(h, GLIA_RD_ERR_ARG, "max_cfl must be positive");
+  if (!complete(h)) return fail(h, GLIA_RD_ERR_STATE, "the handle was not fully created: " + h->err);
+  const double max_cfl = cfl_limit(p, 14.0);
+  if (!(max_cfl > 0)) return fail(h, GLIA_RD_ERR_ARG, "max_cfl is NaN");
   glia::RDEnsemble* e = h->rd->e.get();
   const hipStream_t st = h->st;
   const int64_t ng = h->ng;
@@ -161,7 +173,7 @@ int glia_me_solve_state(glia_me_t h, const glia_me_params* p, float* c, float* w
     for (int b = 0; b < np; b++) vmax = std::max(vmax, h->hpart[b]);
     const double cfl = dt * (double)vmax / h0;
     inf.cfl_max = std::max(inf.cfl_max, cfl);
-    if (cfl >= p->max_cfl) {
+    if (cfl >= max_cfl) {
       inf.cfl_stop = 1;
       break;
     }
@@ -192,11 +204,7 @@ int glia_me_solve_state(glia_me_t h, const glia_me_params* p, float* c, float* w
     ME_SUB(h, glia_el_solve(h->el, force, u, GLIA_RD_DEVICE, &its, nullptr, nullptr), glia_el_last_error(h->el));
     inf.el_iterations += its;
     ME_HIP(h, (hipError_t)glia::me::launch_velocity(st, ng, v, u, uold, (float)(1.0 / dt)));
-    for (int d = 0; d < 3; d++) {
-      ME_HIP(h, hipMemcpyAsync(ws, v[d], fb, hipMemcpyDeviceToDevice, st));
-      ME_SUB(h, e->smooth_field(0, glia::RS_CH, sigma), e->error());
-      ME_HIP(h, hipMemcpyAsync(v[d], ws, fb, hipMemcpyDeviceToDevice, st));
-    }
+    for (int d = 0; d < 3; d++) ME_SUB(h, e->smooth_ptr(v[d], ws, sigma), e->error());
     inf.steps++;
   }
   ME_HIP(h, hipStreamSynchronize(st));
@@ -216,7 +224,9 @@ int glia_me_solve_state_multispecies(glia_me_t h, const glia_me_params* p, const
     return fail(h, GLIA_RD_ERR_ARG, "null parameters or field");
   for (int d = 0; d < 3; d++)
     if (!u[d] || !v[d]) return fail(h, GLIA_RD_ERR_ARG, "null displacement or velocity component");
-  if (!(p->max_cfl > 0)) return fail(h, GLIA_RD_ERR_ARG, "max_cfl must be positive");
+  if (!complete(h)) return fail(h, GLIA_RD_ERR_STATE, "the handle was not fully created: " + h->err);
+  const double max_cfl = cfl_limit(p, 8.0);
+  if (!(max_cfl > 0)) return fail(h, GLIA_RD_ERR_ARG, "max_cfl is NaN");
   glia::RDEnsemble* e = h->rd->e.get();
   const hipStream_t st = h->st;
   const int64_t ng = h->ng;
@@ -268,7 +278,7 @@ int glia_me_solve_state_multispecies(glia_me_t h, const glia_me_params* p, const
     for (int b = 0; b < np; b++) vmax = std::max(vmax, h->hpart[b]);
     const double cfl = dt * (double)vmax / h0;
     inf.cfl_max = std::max(inf.cfl_max, cfl);
-    if (cfl >= p->max_cfl) {
+    if (cfl >= max_cfl) {
       inf.cfl_stop = 1;
       break;
     }
@@ -315,11 +325,7 @@ int glia_me_solve_state_multispecies(glia_me_t h, const glia_me_params* p, const
       ME_SUB(h, glia_el_solve(h->el, force, u, GLIA_RD_DEVICE, &its, nullptr, nullptr), glia_el_last_error(h->el));
       inf.el_iterations += its;
       ME_HIP(h, (hipError_t)glia::me::launch_velocity(st, ng, v, u, uold, (float)(1.0 / dt)));
-      for (int d = 0; d < 3; d++) {
-        ME_HIP(h, hipMemcpyAsync(ws, v[d], fb, hipMemcpyDeviceToDevice, st));
-        ME_SUB(h, e->smooth_field(0, glia::RS_CH, sigma), e->error());
-        ME_HIP(h, hipMemcpyAsync(v[d], ws, fb, hipMemcpyDeviceToDevice, st));
-      }
+      for (int d = 0; d < 3; d++) ME_SUB(h, e->smooth_ptr(v[d], ws, sigma), e->error());
     }
     inf.steps++;
   }

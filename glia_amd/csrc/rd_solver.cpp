@@ -1162,40 +1162,71 @@ void RDEnsemble::smoother_taps(double sigma, std::vector<double>& taps, int& kr)
   }
 }
 
-// SpectralOperators::weierstrassSmoother (src/grad/SpectralOperators.cpp:264-381) of real field
-// rslot of member m, in place: three periodic axis convolutions through a scratch buffer.
+// SpectralOperators::weierstrassSmoother (src/grad/SpectralOperators.cpp:264-381) of a real field,
+// in place: three periodic axis convolutions through a scratch buffer.  Axis a keeps the taps
+// |s| <= kr_a = min(n_a / 2 - 1, ceil(12 sigma / h_a)), so every index wraps at most once and no
+// residue of the axis is counted twice.  The device taps are kept while the width repeats (the
+// mass-effect loops smooth with one width many times).
+int RDEnsemble::smooth_taps_for(double sigma) {
+  if (sigma == smooth_sigma_) return 0;
+  std::vector<double> taps;
+  int kr = 0;
+  smoother_taps(sigma, taps, kr);
+  const int n[3] = {d_.n0, d_.n1, d_.n2};
+  const int stride = 2 * kr + 1;
+  int kra[3];
+  for (int a = 0; a < 3; a++) {
+    kra[a] = std::min(n[a] / 2 - 1, (int)std::ceil(12.0 * sigma / h_[a]));
+    // recentre axis a's taps at kra[a]: entry s + kra[a] for |s| <= kra[a]
+    for (int s = -kra[a]; s <= kra[a]; s++) taps[a * stride + s + kra[a]] = taps[a * stride + s + kr];
+  }
+  if (taps.size() > smooth_cap_) {
+    GLIA_HIP(hipStreamSynchronize(stream_));
+    if (smooth_taps_) (void)hipFree(smooth_taps_);
+    smooth_taps_ = nullptr;
+    smooth_cap_ = 0;
+    GLIA_HIP(hipMalloc((void**)&smooth_taps_, sizeof(double) * taps.size()));
+    smooth_cap_ = taps.size();
+  }
+  const int rc = copy_sync(smooth_taps_, taps.data(), sizeof(double) * taps.size(), hipMemcpyHostToDevice);
+  if (rc) return rc;
+  smooth_sigma_ = sigma;
+  smooth_stride_ = stride;
+  for (int a = 0; a < 3; a++) smooth_kr_[a] = kra[a];
+  return 0;
+}
+
 int RDEnsemble::smooth_field(int m, int rslot, double sigma) {
   if (m < 0 || m >= nm_ || rslot < 0 || rslot >= RS_NUM) return fail(-2, "bad member or field");
   if (!(sigma > 0)) return 0;
-  int rc = 0;
-  if (sigma != smooth_sigma_) {  // the mass-effect loops smooth with one width many times
-    std::vector<double> taps;
-    int kr = 0;
-    smoother_taps(sigma, taps, kr);
-    if (taps.size() > smooth_cap_) {
-      GLIA_HIP(hipStreamSynchronize(stream_));
-      if (smooth_taps_) (void)hipFree(smooth_taps_);
-      smooth_taps_ = nullptr;
-      smooth_cap_ = 0;
-      GLIA_HIP(hipMalloc((void**)&smooth_taps_, sizeof(double) * taps.size()));
-      smooth_cap_ = taps.size();
-    }
-    rc = copy_sync(smooth_taps_, taps.data(), sizeof(double) * taps.size(), hipMemcpyHostToDevice);
-    if (rc) return rc;
-    smooth_sigma_ = sigma;
-    smooth_kr_ = kr;
-  }
+  int rc = smooth_taps_for(sigma);
+  if (rc) return rc;
   const double* td = smooth_taps_;
-  const int kr = smooth_kr_;
+  const int* kr = smooth_kr_;
+  const int stride = smooth_stride_;
   float* f = mh_[m].r[rslot];
   float* scratch = reinterpret_cast<float*>(mh_[m].c[CS_T7]);  // n0*n1*n2 floats
   LaunchCtx c = ctx();
-  const int stride = 2 * kr + 1;
-  if (!rc) rc = launch_smooth_axis(c, f, scratch, 0, td, kr);
-  if (!rc) rc = launch_smooth_axis(c, scratch, f, 1, td + stride, kr);
-  if (!rc) rc = launch_smooth_axis(c, f, scratch, 2, td + 2 * stride, kr);
+  if (!rc) rc = launch_smooth_axis(c, f, scratch, 0, td, kr[0]);
+  if (!rc) rc = launch_smooth_axis(c, scratch, f, 1, td + stride, kr[1]);
+  if (!rc) rc = launch_smooth_axis(c, f, scratch, 2, td + 2 * stride, kr[2]);
   if (!rc) rc = check(hipMemcpyAsync(f, scratch, sizeof(float) * d_.ng, hipMemcpyDeviceToDevice, stream_), "smooth copy");
-  if (!rc) rc = sync();
+  return rc;
+}
+
+int RDEnsemble::smooth_ptr(float* f, float* tmp, double sigma) {
+  if (!f || !tmp || f == tmp) return fail(-2, "smoothing needs two distinct device fields");
+  if (!(sigma > 0)) return 0;
+  int rc = smooth_taps_for(sigma);
+  if (rc) return rc;
+  const double* td = smooth_taps_;
+  const int* kr = smooth_kr_;
+  const int stride = smooth_stride_;
+  float* scratch = reinterpret_cast<float*>(mh_[0].c[CS_T7]);  // n0*n1*n2 floats
+  LaunchCtx c = ctx();
+  if (!rc) rc = launch_smooth_axis(c, f, scratch, 0, td, kr[0]);
+  if (!rc) rc = launch_smooth_axis(c, scratch, tmp, 1, td + stride, kr[1]);
+  if (!rc) rc = launch_smooth_axis(c, tmp, f, 2, td + 2 * stride, kr[2]);
   return rc;
 }
 

@@ -109,6 +109,9 @@ class RDEnsemble {
   int profile_enable(bool on);
   int set_use_graphs(bool on);
   int smooth_field(int m, int rslot, double sigma);  // weierstrassSmoother, in place
+  // the same smoother on any device field f of the grid, in place, through tmp (a distinct field
+  // of the grid) and the spectral work slot CS_T7 of member 0; no copies, no host synchronization
+  int smooth_ptr(float* f, float* tmp, double sigma);
   int refresh_tissue(int m);                         // derived quantities from stored tissue fields
   int profile_get(int kind, double* ms, long long* count);
 
@@ -121,6 +124,7 @@ class RDEnsemble {
   LaunchCtx ctx() const;
 
   void smoother_taps(double sigma, std::vector<double>& taps, int& kr) const;
+  int smooth_taps_for(double sigma);
   int cn_solve(int sx, double tau);
   int launch_iteration(int sx, int slot);                       // the launches of one CG iteration
   int iteration_graph(int sx, int slot, hipGraphExec_t* out);   // the same, captured once and cached
@@ -189,7 +193,8 @@ class RDEnsemble {
   double* smooth_taps_ = nullptr;  // device taps of the last smoothing width (three axes)
   size_t smooth_cap_ = 0;          // doubles allocated at smooth_taps_
   double smooth_sigma_ = -1.0;
-  int smooth_kr_ = 0;
+  int smooth_kr_[3] = {0, 0, 0};   // per-axis radius of the last width
+  int smooth_stride_ = 1;          // doubles per axis in smooth_taps_
   std::map<std::pair<int, int>, hipGraphExec_t> graphs_;  // (state slot, flag slot) -> graph
   bool prof_on_ = false;
   std::vector<ProfRec> prof_recs_;

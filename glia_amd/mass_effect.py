@@ -63,14 +63,16 @@ class MassEffectSolver:
             raise GliaError(rc, self.L.glia_rd_last_error(self.rd).decode())
 
     def solve_state(self, c0, wm, gm, vt, csf, bg, u=None, v=None, mri=None, forcing_factor=1e5, use_tanh=True,
-                    max_cfl=14.0, **el):
-        """solveState(0) from c0 and the tissue; el overrides glia_el_params fields.  Returns a dict
+                    max_cfl=None, **el):
+        """solveState(0) from c0 and the tissue (max_cfl None: the reference's limit 14); el overrides
+        glia_el_params fields.  Returns a dict
         of c, wm, gm, vt, csf, mri (None when not given), u, v (shape (3,) + grid) and info."""
         import torch
 
         p = _lib.MeParams()
         self.L.glia_me_params_init(ctypes.byref(p))
-        p.forcing_factor, p.use_tanh, p.max_cfl = float(forcing_factor), int(bool(use_tanh)), float(max_cfl)
+        p.forcing_factor, p.use_tanh = float(forcing_factor), int(bool(use_tanh))
+        p.max_cfl = 0.0 if max_cfl is None else float(max_cfl)
         for key, val in el.items():
             if not hasattr(p.el, key):
                 raise KeyError("unknown elasticity parameter %s" % key)
@@ -104,15 +106,17 @@ class MassEffectSolver:
         return out
 
     def solve_state_multispecies(self, c0, wm, gm, vt, csf, bg, nec=None, u=None, v=None, forcing_factor=1e5,
-                                 use_tanh=True, max_cfl=8.0, ms=None, **el):
-        """PdeOperatorsMultiSpecies::solveState(0) from c0 and the tissue; ms overrides glia_ms_params
+                                 use_tanh=True, max_cfl=None, ms=None, **el):
+        """PdeOperatorsMultiSpecies::solveState(0) from c0 and the tissue (max_cfl None: the
+        reference's limit 8); ms overrides glia_ms_params
         fields (death_rate, ox_source, ...), el the glia_el_params fields.  Returns a dict of p, i,
         nec, ox, wm, gm, vt, csf, u, v and info."""
         import torch
 
         p = _lib.MeParams()
         self.L.glia_me_params_init(ctypes.byref(p))
-        p.forcing_factor, p.use_tanh, p.max_cfl = float(forcing_factor), int(bool(use_tanh)), float(max_cfl)
+        p.forcing_factor, p.use_tanh = float(forcing_factor), int(bool(use_tanh))
+        p.max_cfl = 0.0 if max_cfl is None else float(max_cfl)
         for key, val in el.items():
             if not hasattr(p.el, key):
                 raise KeyError("unknown elasticity parameter %s" % key)

@@ -34,7 +34,9 @@ typedef struct glia_me_s* glia_me_t;
 typedef struct glia_me_params {
   double forcing_factor; /* forcing_factor_ (gamma); 1e5 */
   int use_tanh;          /* use_tanh_scaling_; 1 */
-  double max_cfl;        /* the loop's CFL limit (src/pde/PdeOperatorsMassEffect.cpp:472); 14 */
+  double max_cfl;        /* the loop's CFL limit; <= 0 (the default) selects the reference's limit of
+                            the model: 14 for model 4 (src/pde/PdeOperatorsMassEffect.cpp:472), 8
+                            for model 5 (src/pde/PdeOperatorsMultiSpecies.cpp:296) */
   glia_el_params el;     /* Young's moduli, Poisson ratios, screening */
 } glia_me_params;
 
@@ -51,7 +53,8 @@ typedef struct glia_me_info {
 /* the engine (glia_rd_create with cfg->nmembers = 1; cfg->stream, or the engine's own stream, is
    shared by the elasticity and advection solvers) and the two other solvers.  As with
    glia_rd_create, a failed creation still sets *out (unless cfg or out is NULL or nmembers != 1)
-   so that glia_me_last_error can name the cause; destroy it either way. */
+   so that glia_me_last_error can name the cause; destroy it either way.  The solve entry points
+   answer such a partly created handle with GLIA_RD_ERR_STATE. */
 int glia_me_create(const glia_rd_config* cfg, glia_me_t* out);
 int glia_me_destroy(glia_me_t h);
 const char* glia_me_last_error(glia_me_t h);
@@ -77,7 +80,7 @@ int glia_me_solve_state(glia_me_t h, const glia_me_params* p, float* c, float* w
 
 /* PdeOperatorsMultiSpecies::solveState(0) (src/pde/PdeOperatorsMultiSpecies.cpp:235-508), the
    multispecies forward of model 5, without its output, edema and segmentation branches (they feed
-   only the output).  The reference's CFL limit for this model is 8 (p->max_cfl).
+   only the output).  The reference's CFL limit for this model is 8 (p->max_cfl <= 0 selects it).
    c0: Tumor::c_0_ (in); prolif, infil, ox: the proliferative, infiltrative and oxygen species
    (out; set to c0 clipped at zero, 0 and 1 at the start); nec: the necrotic species (in and out,
    not reset, as in the reference); wm, gm, vt, csf: the tissue (in, filtered and advected out);

@@ -134,4 +134,34 @@ def test_c_abi_rejects_missing_handles_without_a_gpu():
     assert L.glia_me_solve_state_multispecies(None, None, None, *([None] * 12), ctypes.byref(info)) == -2
     p = _lib.MeParams()
     L.glia_me_params_init(ctypes.byref(p))
-    assert p.forcing_factor == 1e5 and p.use_tanh == 1 and p.max_cfl == 14.0 and p.el.E_tumor == 8000.0
+    # max_cfl 0 selects each model's reference limit (14 for model 4, 8 for model 5)
+    assert p.forcing_factor == 1e5 and p.use_tanh == 1 and p.max_cfl == 0.0 and p.el.E_tumor == 8000.0
+
+
+def test_c_abi_partly_created_handle_without_a_gpu():
+    """A mass-effect handle whose engine could not be created (order 3 is rejected before any
+    device call) is still handed out with its message; both solve entry points then answer
+    GLIA_RD_ERR_STATE instead of touching the missing engine."""
+    import ctypes
+
+    from glia_amd import _lib
+    L = _lib.load()
+    cfg = _lib.Config()
+    L.glia_rd_config_init(ctypes.byref(cfg))
+    cfg.order = 3
+    h = ctypes.c_void_p()
+    assert L.glia_me_create(ctypes.byref(cfg), ctypes.byref(h)) == -4 and h.value
+    assert b"engine" in L.glia_me_last_error(h)
+    p = _lib.MeParams()
+    L.glia_me_params_init(ctypes.byref(p))
+    mp = _lib.MsParams()
+    L.glia_ms_params_init(ctypes.byref(mp))
+    dummy = ctypes.c_void_p(16)  # never dereferenced: the handle check comes first
+    comp = (ctypes.c_void_p * 3)(dummy, dummy, dummy)
+    info = _lib.MeInfo()
+    rc = L.glia_me_solve_state(h, ctypes.byref(p), *([dummy] * 7), comp, comp, ctypes.byref(info))
+    assert rc == -3 and b"not fully created" in L.glia_me_last_error(h)
+    rc = L.glia_me_solve_state_multispecies(h, ctypes.byref(p), ctypes.byref(mp), *([dummy] * 10), comp, comp,
+                                            ctypes.byref(info))
+    assert rc == -3
+    assert L.glia_me_destroy(h) == 0
