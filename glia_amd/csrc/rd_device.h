@@ -111,7 +111,7 @@ struct MemberDev {
 struct BrickJob {
   int o[3];
   int b[3];
-  int kr;
+  int kr[3];    // smoothing radius per axis (the brick is extended by kr[a] on both sides)
   int member;
   double ctr[3];
   int64_t ext_off;  // offset of this job's extended scratch (doubles)

@@ -1159,14 +1159,14 @@ __global__ void k_brick_src(const BrickJob* __restrict__ jobs, Dims d, const Mem
   // FP64 restatement of Phi::initialize
 #pragma clang fp contract(off)
   const BrickJob J = jobs[blockIdx.y];
-  const int e0 = J.b[0] + 2 * J.kr, e1 = J.b[1] + 2 * J.kr, e2 = J.b[2] + 2 * J.kr;
+  const int e0 = J.b[0] + 2 * J.kr[0], e1 = J.b[1] + 2 * J.kr[1], e2 = J.b[2] + 2 * J.kr[2];
   const int64_t ne = (int64_t)e0 * e1 * e2;
   const float* fl = mem[J.member].r[RS_FILTER];
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += (int64_t)gridDim.x * blockDim.x) {
     const int z = (int)(i % e2);
     const int y = (int)((i / e2) % e1);
     const int x = (int)(i / ((int64_t)e1 * e2));
-    int X = J.o[0] - J.kr + x, Y = J.o[1] - J.kr + y, Z = J.o[2] - J.kr + z;
+    int X = J.o[0] - J.kr[0] + x, Y = J.o[1] - J.kr[1] + y, Z = J.o[2] - J.kr[2] + z;
     X = ((X % d.n0) + d.n0) % d.n0;
     Y = ((Y % d.n1) + d.n1) % d.n1;
     Z = ((Z % d.n2) + d.n2) % d.n2;
@@ -1200,16 +1200,17 @@ __global__ void k_smooth_axis(const float* __restrict__ in, float* __restrict__ 
   }
 }
 
-// one axis of the separable smoothing: the kept extent shrinks by 2*kr along 'axis'
+// one axis of the separable smoothing: the kept extent shrinks by 2*kr[axis] along 'axis'; the
+// axis's taps sit at tp[s + kr[axis]], |s| <= kr[axis]
 __global__ void k_brick_conv(const BrickJob* __restrict__ jobs, int axis, const double* __restrict__ taps, int tap_stride,
                              const double* __restrict__ src, double* __restrict__ dst) {
   const BrickJob J = jobs[blockIdx.y];
-  const int kr = J.kr;
+  const int kr = J.kr[axis];
   int ein[3], eout[3];
   for (int a = 0; a < 3; a++) {
     // axes < axis already reduced to b, axes >= axis still extended
-    ein[a] = (a < axis) ? J.b[a] : J.b[a] + 2 * kr;
-    eout[a] = (a <= axis) ? J.b[a] : J.b[a] + 2 * kr;
+    ein[a] = (a < axis) ? J.b[a] : J.b[a] + 2 * J.kr[a];
+    eout[a] = (a <= axis) ? J.b[a] : J.b[a] + 2 * J.kr[a];
   }
   const int64_t no = (int64_t)eout[0] * eout[1] * eout[2];
   const double* tp = taps + (int64_t)blockIdx.y * tap_stride * 3 + axis * tap_stride;
@@ -1776,8 +1777,8 @@ int launch_bricks(const LaunchCtx& c, const BrickJob* jobs_host, int njobs, cons
   if (njobs <= 0) return 0;
   int64_t ext_total = 0;
   for (int i = 0; i < njobs; i++) {
-    const int64_t ne = (int64_t)(jobs_host[i].b[0] + 2 * jobs_host[i].kr) * (jobs_host[i].b[1] + 2 * jobs_host[i].kr) *
-                       (jobs_host[i].b[2] + 2 * jobs_host[i].kr);
+    const BrickJob& J = jobs_host[i];
+    const int64_t ne = (int64_t)(J.b[0] + 2 * J.kr[0]) * (J.b[1] + 2 * J.kr[1]) * (J.b[2] + 2 * J.kr[2]);
     if (jobs_host[i].ext_off + ne > ext_total) ext_total = jobs_host[i].ext_off + ne;
   }
   BrickJob* jd = nullptr;
