@@ -5,11 +5,13 @@ Workload (default): the 256^3 members of the batching configuration -- an ensemb
 64 Gaussian bases, nt = 50 steps of dt = 0.02, observed data thresholded from a seeded
 ground-truth forward run.  One step = one batched evaluation of J and dJ = [g_p | g_kappa | g_rho]
 for every member (FP32 bulk, FP64 reductions).  value = gradient evaluations per second of the
-whole ensemble.  Also reported: the same evaluation run one member at a time ("plain"), a plain
-FP32 GPU path built the way the reference's CUDA backend solves (timed sample, "refstyle_plain"), the
-relative errors against the FP64 oracle at 64^3 (where the oracle finishes in seconds), the
-achieved bandwidth of the dominant kernel against the MI355X HBM peak, and the oracle timed on
-the host cores on a bounded sample of the same workload.
+whole ensemble, timed on the product path (graph replay, no per-kernel events).  Also reported:
+per-kernel durations from one further evaluation with HIP events around each CG kernel launch;
+the same evaluation run one member at a time ("plain"); a plain FP32 GPU path built the way the
+reference's CUDA backend evaluates a gradient, timed over one whole evaluation
+("refstyle_plain"); the relative errors of c(T), J and dJ against the FP64 oracle on the benched
+grid at a short horizon (nt = 2), from the same oracle evaluation that gives the CPU baseline
+(all host threads), plus a one-thread sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -85,28 +87,16 @@ def xvec(pr):
     return np.concatenate([pr["p"], [pr["kappa"], pr["rho"]]])
 
 
-def accuracy_64():
-    """Relative errors of c(T), J and dJ against the FP64 oracle at 64^3 (nt = 10, dt = 0.02)."""
-    from glia_amd import rd, synth
+def accuracy_and_cpu(n, dt, pr, d1, nt_short):
+    """Relative errors of c(T), J and dJ against the FP64 oracle on the benched grid, and the
+    oracle's own speed, from one whole evaluation (forward + adjoint + gradient, mode 2: p, kappa
+    and rho) of one member at nt_short steps on both sides, with the same d1.  The oracle runs on
+    all host threads; evals/s at nt steps scale by nt_short / nt (the per-step work is the same at
+    every step, setup is excluded on both sides)."""
+    from glia_amd import rd
     from oracle import oracle
-    n, nt, dt = 64, 10, 0.02
-    pr = synth.problem(n, 2, nbases=8, kappa=0.05, rho=8.0)
-    o = oracle.Oracle(n)
-    o.set_tissue(pr["wm"], pr["gm"], pr["csf"], pr["vt"])
-    o.set_time(dt, nt)
-    o.set_k(pr["kappa"], 0.2, 0.0)
-    o.set_rho(pr["rho"], 0.2, 0.0)
-    o.prec_factor()
-    o.set_phi(pr["centers"], pr["sigma"], 1.0, False)
-    o.set_c0(o.phi_apply(pr["p"] * 1.3))
-    o.solve_state(0)
-    d1 = (o.ct() > 0.5).astype(np.float64)
-    o.set_data(d1, None, -0.99)
-    o.set_time(dt, nt)
     x = xvec(pr)
-    Jo, go = o.objective_and_gradient(x, mode=2)
-    cto = o.ct()
-    s = rd.RDSolver(n, dt=dt, nt=nt, k_gm_wm=0.2, r_gm_wm=0.2)
+    s = rd.RDSolver(n, dt=dt, nt=nt_short, k_gm_wm=0.2, r_gm_wm=0.2)
     s.set_tissue(0, pr["wm"], pr["gm"], pr["csf"], pr["vt"])
     s.set_coefficients(0, pr["kappa"], pr["rho"])
     s.set_basis(0, pr["centers"], pr["sigma"])
@@ -114,56 +104,78 @@ def accuracy_64():
     J, g = s.objective_and_gradient([x])
     ct = s.get_field(0, "ct")
     s.close()
-    rl = lambda a, b: float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
-    return {"grid": "64^3, nt=10", "cT": rl(ct, cto), "J": abs(J[0] - Jo) / abs(Jo), "dJ": rl(g[0], go)}
-
-
-def cpu_baseline(n, nt, dt, pr, c0):
-    """Oracle (FP64 CPU restatement of the reference path) on one forward Strang step of the
-    same 256^3 member; a gradient evaluation costs nt forward steps plus nt adjoint steps of three
-    CN solves each, so evals/s = 1 / (nt * t_step * (1 + 1.5))."""
-    from oracle import oracle
     threads = oracle.max_threads()
     o = oracle.Oracle(n)
     o.set_tissue(pr["wm"], pr["gm"], pr["csf"], pr["vt"])
-    o.set_time(dt, 1)
+    o.set_time(dt, nt_short)
     o.set_k(pr["kappa"], 0.2, 0.0)
     o.set_rho(pr["rho"], 0.2, 0.0)
     o.prec_factor()
-    # c(0) = Phi p of the GPU run (the sample times the time stepping only); one warm-up step,
-    # then the median of three timed steps
-    times = []
-    for rep in range(4):
-        o.set_c0(c0)
-        t0 = time.perf_counter()
-        o.solve_state(0)
-        if rep > 0:
-            times.append(time.perf_counter() - t0)
-    t_step = float(np.median(times))
-    its = o.ksp()["state"]
+    o.set_phi(pr["centers"], pr["sigma"], 1.0, False)
+    o.set_data(d1.astype(np.float64), None, -0.99)
+    o.set_time(dt, nt_short)
+    log("oracle: one %d^3 evaluation at nt=%d on %d threads" % (n, nt_short, threads))
+    oracle.set_verbose(True)
+    t0 = time.perf_counter()
+    Jo, go = o.objective_and_gradient(x, mode=2)
+    t_eval = time.perf_counter() - t0
+    oracle.set_verbose(False)
+    ksp = o.ksp()
+    cto = o.ct()
     del o
-    evals = 1.0 / (nt * t_step * 2.5)
-    return {"value": evals, "unit": "grad evals/s", "cores": threads, "kind": "port",
-            "sample": "oracle FP64, one forward Strang step (2 CN solves, %d CG iterations in the last) of one "
-                      "%d^3 member, median of 3 after a warm-up: %.2f s on %d threads; gradient eval = %d forward "
-                      "+ %d adjoint steps (adjoint step = 1.5 forward steps)" % (its, n, t_step, threads, nt, nt)}
+    rl = lambda a, b: float(np.linalg.norm(np.ravel(a).astype(np.float64) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+    acc = {"grid": "%d^3, nt=%d, dt=%g, %d bases" % (n, nt_short, dt, len(pr["p"])), "cT": rl(ct, cto),
+           "J": abs(J[0] - Jo) / abs(Jo), "dJ": rl(g[0], go)}
+    return acc, t_eval, threads, ksp
 
 
-def refstyle_plain(n, nt, dt, k, rho, mask, c0, rtol=1e-6, dev="cuda"):
-    """The plain FP32 GPU path built the way the reference's CUDA backend runs a solve, timed on the
-    same GPU as a comparison leg (torch FFTs; not the engine): real-space vectors, PETSc-KSPCG
-    preconditioned CG with three host-synchronized reductions per iteration, a matvec of eight full
-    3D transforms (computeGradient: 1 R2C + 3 C2R; computeDivergence: 3 R2C + 1 C2R;
+def cpu_one_thread(dt):
+    """The oracle on one thread over one whole evaluation of a 64^3 member at nt = 2 (a bounded
+    sample: the 256^3 evaluation would take tens of minutes on one core)."""
+    from glia_amd import synth
+    from oracle import oracle
+    n, nt_s = 64, 2
+    pr = synth.problem(n, 2, nbases=8, kappa=0.05, rho=8.0)
+    d1 = (pr["wm"] > 0.5).astype(np.float64)
+    prev = oracle.max_threads()
+    oracle.set_threads(1)
+    try:
+        o = oracle.Oracle(n)
+        o.set_tissue(pr["wm"], pr["gm"], pr["csf"], pr["vt"])
+        o.set_time(dt, nt_s)
+        o.set_k(pr["kappa"], 0.2, 0.0)
+        o.set_rho(pr["rho"], 0.2, 0.0)
+        o.prec_factor()
+        o.set_phi(pr["centers"], pr["sigma"], 1.0, False)
+        o.set_data(d1, None, -0.99)
+        o.set_time(dt, nt_s)
+        t0 = time.perf_counter()
+        o.objective_and_gradient(xvec(pr), mode=2)
+        t = time.perf_counter() - t0
+    finally:
+        oracle.set_threads(prev)
+    return t, n, nt_s
+
+
+def refstyle_gradient(nt, dt, k, rho, mask, wm, gm, c0, d1, rtol=1e-6, dev="cuda"):
+    """The plain FP32 GPU path built the way the reference's CUDA backend evaluates a gradient,
+    timed on the same GPU as a comparison leg (torch FFTs; not the engine).  Real-space vectors;
+    PETSc-KSPCG preconditioned CG with three host-synchronized reductions per iteration; a matvec
+    of eight full 3D transforms (computeGradient: 1 R2C + 3 C2R; computeDivergence: 3 R2C + 1 C2R;
     src/mat/DiffCoef.cpp:255-263, src/grad/SpectralOperators.cpp) and a preconditioner of two
-    (applyPC, src/pde/DiffusionSolver.cpp:182).  One forward Strang step of one member (CN half
-    step, logistic reaction, CN half step) is timed, median of 3 after a warm-up, and extrapolated
-    like the CPU baseline: evals/s = 1 / (nt * t_step * 2.5)."""
+    (applyPC, src/pde/DiffusionSolver.cpp:182).  One whole evaluation of one member: the Strang
+    forward storing c_[i] (src/pde/PdeOperators.cpp:235-316), the misfit, the adjoint with the
+    half-step state recomputed by a CN solve (adjoint_store_ = false, :318-420) and the
+    gradDiffusion / gradReaction quadratures (src/grad/DerivativeOperators.cpp:189-321).  c(0) is
+    given; Phi^T (two transforms per basis) is left out, which only flatters this leg.  Returns
+    the timing and (J, g_kappa, g_rho) for a consistency check against the engine."""
     import torch
     f32 = torch.float32
     sync = torch.cuda.synchronize if dev == "cuda" else (lambda: None)
-    kd = torch.as_tensor(k, dtype=f32, device=dev)
-    rd_ = torch.as_tensor(rho, dtype=f32, device=dev)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=f32, device=dev)
+    kd, rd_, wmd, gmd = T(k), T(rho), T(wm), T(gm)
     shape = kd.shape
+    h3 = (2 * np.pi) ** 3 / float(np.prod(shape))
 
     def wn(m, half=False):
         w = torch.fft.rfftfreq(m, 1.0 / m) if half else torch.fft.fftfreq(m, 1.0 / m)
@@ -219,28 +231,53 @@ def refstyle_plain(n, nt, dt, k, rho, mask, c0, rtol=1e-6, dev="cuda"):
             p = z + (beta / betaold) * p
         return x
 
-    def step(c):
-        c = cn(c, 0.5 * dt)
-        e = torch.exp(rd_ * dt)
-        c = c * e / (c * e + 1.0 - c)
-        return cn(c, 0.5 * dt)
+    e = torch.exp(rd_ * dt)
 
-    c0d = torch.as_tensor(c0, dtype=f32, device=dev)
-    times = []
-    for rep in range(4):
-        its[0] = 0
-        sync()
-        t0 = time.perf_counter()
-        step(c0d.clone())
-        sync()
-        if rep > 0:
-            times.append(time.perf_counter() - t0)
-    t_step = float(np.median(times))
-    return {"value": 1.0 / (nt * t_step * 2.5), "unit": "grad evals/s",
+    def evaluate(c0d, d1d, nsteps):
+        C = [c0d.clone()]
+        c = c0d.clone()
+        for _ in range(nsteps):
+            c = cn(c, 0.5 * dt)
+            c = c * e / (c * e + 1.0 - c)
+            c = cn(c, 0.5 * dt)
+            C.append(c)
+        t = c - d1d
+        J = h3 * 0.5 * float(torch.sum(t.double() * t.double()))
+        a = -t
+        A = [None] * (nsteps + 1)
+        A[nsteps] = a
+        for i in range(nsteps):
+            j = nsteps - i - 1
+            a = cn(a, 0.5 * dt)
+            ch = cn(C[j].clone(), 0.5 * dt)
+            den = ch * e + 1.0 - ch
+            a = a * e / (den * den)
+            a = cn(a, 0.5 * dt)
+            A[j] = a
+        Q = torch.zeros_like(c0d)
+        P = torch.zeros_like(c0d)
+        for i in range(nsteps + 1):
+            wgt = dt * (0.5 if i in (0, nsteps) else 1.0)
+            gc, ga = grad(C[i]), grad(A[i])
+            Q += wgt * (gc[0] * ga[0] + gc[1] * ga[1] + gc[2] * ga[2])
+            P += wgt * (C[i] * C[i] - C[i]) * A[i]
+        gk = h3 * (float(torch.sum(wmd.double() * Q.double())) + 0.2 * float(torch.sum(gmd.double() * Q.double())))
+        gr = h3 * (float(torch.sum(wmd.double() * P.double())) + 0.2 * float(torch.sum(gmd.double() * P.double())))
+        return J, gk, gr
+
+    c0d, d1d = T(c0), T(d1)
+    evaluate(c0d, d1d, 1)  # warm-up (plans, allocator)
+    its[0] = 0
+    sync()
+    t0 = time.perf_counter()
+    J, gk, gr = evaluate(c0d, d1d, nt)
+    sync()
+    t_eval = time.perf_counter() - t0
+    return {"value": 1.0 / t_eval, "unit": "grad evals/s",
             "sample": "reference-structured plain FP32 GPU path (torch FFTs, real-space PETSc-style CG, 10 full 3D "
-                      "FFTs and 3 host-synchronized reductions per CG iteration), one forward Strang step of one "
-                      "%d^3 member (%d CG iterations), median of 3: %.1f ms; gradient eval = %d forward + %d adjoint "
-                      "steps (adjoint step = 1.5 forward steps)" % (shape[0], its[0], t_step * 1e3, nt, nt)}
+                      "FFTs and 3 host-synchronized reductions per CG iteration): one whole gradient evaluation "
+                      "(forward, adjoint with recomputed half steps, kappa/rho quadratures; Phi^T not timed) of "
+                      "one %d^3 member at nt=%d, %d CG iterations: %.2f s" % (shape[0], nt, its[0], t_eval)}, (J, gk, gr)
 
 
 def batch_config(args):
@@ -346,6 +383,29 @@ def diff_config(args):
             rec["grad_norm"] = float("%.6g" % gn)
             rec["fd_err_rel_directional"] = [float("%.3g" % e) for e in np.abs(fds - ads) / np.abs(ads)]
             rec["fd_err_over_grad_norm"] = [float("%.3g" % e) for e in np.abs(fds - ads) / gn]
+            # the bar of the 64^3 finite-difference test: |fd - ad| <= 2e-2 |ad| + 1e-3 ||dJ||
+            rec["fd_pass"] = [bool(abs(f - a) <= 2e-2 * abs(a) + 1e-3 * gn) for f, a in zip(fds, ads)]
+            # where a disagreement comes from: a direction inside the p block (there the gradient
+            # is the discrete adjoint up to the CG tolerance) and the kappa and rho axes alone (the
+            # reference's trapezoid quadratures match the discrete derivative to O(dt) only), each
+            # at three step sizes (truncation error falls as h^2, noise in J grows as 1/h)
+            vp = rng.standard_normal(x0.size)
+            vp[-2:] = 0.0
+            vp /= np.linalg.norm(vp)
+            ek = np.zeros(x0.size)
+            ek[-2] = 1.0
+            er = np.zeros(x0.size)
+            er[-1] = 1.0
+            sweeps = {}
+            for name, v, hs in (("p_block", vp, (1e-2, 1e-3, 1e-4)), ("kappa", ek, (1e-3, 1e-4, 1e-5)),
+                                ("rho", er, (1e-2, 1e-3, 1e-4))):
+                ad = float(g @ v)
+                fdh = {}
+                for h in hs:
+                    fdh["%g" % h] = float("%.7g" % ((s.objective([x0 + h * v])[0] - s.objective([x0 - h * v])[0]) / (2 * h)))
+                sweeps[name] = {"adjoint": float("%.7g" % ad), "fd": fdh,
+                                "rel_err": {k: float("%.3g" % (abs(f - ad) / abs(ad))) for k, f in fdh.items()}}
+            rec["fd_sweeps"] = sweeps
             rec["fd_s"] = round(time.perf_counter() - t0, 2)
         out["checkpoint_%d" % slots if slots else "store_all"] = rec
         s.close()
@@ -663,6 +723,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--nt-oracle", type=int, default=2,
+                    help="time steps of the oracle evaluation that gives rel_err_fp64 and cpu_baseline")
     ap.add_argument("--config", choices=["ensemble256", "batch", "diff", "invert", "advect", "elastic",
                                          "masseffect", "multispecies"],
                     default="ensemble256",
@@ -723,8 +785,9 @@ def main():
 
     for _ in range(args.warmup):
         s.objective_and_gradient(xs)
+    # timed region: the product path (CG iterations replayed from captured graphs, no per-kernel
+    # events)
     s.reset_stats()
-    s.profile(True)
     torch.cuda.synchronize()
     s.sync()
     t1 = time.perf_counter()
@@ -733,27 +796,33 @@ def main():
     s.sync()
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    prof = s.profile_get()
-    s.profile(False)
     st = [s.stats(m) for m in range(M)]
     ms_step = (t2 - t1) * 1000.0 / args.steps
     value = M * args.steps / (t2 - t1)
     its = sum(x["pcg_iterations"] for x in st)
     solves = sum(x["cn_solves"] for x in st)
-    log("batched: %.1f ms/step, %.3f grad evals/s, %.2f CG its per CN solve" % (ms_step, value, its / max(solves, 1)))
+    mbar = its / max(solves, 1)
+    log("batched: %.1f ms/step, %.3f grad evals/s, %.2f CG its per CN solve" % (ms_step, value, mbar))
 
-    # dominant kernel and its achieved bandwidth: algorithmic bytes / event-timed duration.  A
-    # launch covers every member, but converged members exit at once, so the bytes of one kernel
-    # over the timed region are (fields per member-iteration) * (CG iterations summed over members).
+    # one more evaluation with HIP events around every CG kernel launch on the solver stream
+    # (graph replay off): the per-kernel durations.  A launch covers every member, but converged
+    # members exit at once, so the bytes of one kernel are (fields per member-iteration) * (CG
+    # iterations summed over members).
+    s.reset_stats()
+    s.profile(True)
+    s.objective_and_gradient(xs)
+    s.sync()
+    prof = s.profile_get()
+    s.profile(False)
+    its_p = sum(s.stats(m)["pcg_iterations"] for m in range(M))
     field_bytes = 4.0 * n ** 3
     dom = max(prof, key=lambda k: prof[k][1])
     cnt, ms = prof[dom]
-    achieved = _lib.KERNEL_FIELDS[dom] * field_bytes * its / (ms / 1000.0) / 1e9
-    kernels = {k: {"launches": v[0], "ms": round(v[1], 3), "member_iterations": its,
-                   "GBs": round(_lib.KERNEL_FIELDS[k] * field_bytes * its / (v[1] / 1000.0) / 1e9, 1)}
+    achieved = _lib.KERNEL_FIELDS[dom] * field_bytes * its_p / (ms / 1000.0) / 1e9
+    kernels = {k: {"launches": v[0], "ms": round(v[1], 3), "member_iterations": its_p,
+                   "GBs": round(_lib.KERNEL_FIELDS[k] * field_bytes * its_p / (v[1] / 1000.0) / 1e9, 1)}
                for k, v in prof.items() if v[0] > 0}
     # whole-evaluation model of algorithmic bytes: B = R [nt (230 + 144 m) + 20 (nt + 1)] per member
-    mbar = its / max(solves, 1)
     b_eval = field_bytes * (nt * (230 + 144 * mbar) + 20 * (nt + 1))
     model_gbs = b_eval * M * args.steps / (t2 - t1) / 1e9
     c0_member0 = s.get_field(0, "c0").astype(np.float64)
@@ -761,32 +830,50 @@ def main():
     del s
 
     plain = None
+    eng0 = None
     if not args.no_plain:
         # the same evaluations, one member at a time (a one-member solver per member)
         tsum = 0.0
         for m in range(M):
             sp = build_solver(members[m:m + 1], n, nt, dt, args.checkpoint_slots)
             sp.set_data(0, members[m]["d1"], None, -0.99)
-            for _ in range(args.warmup):
-                sp.objective_and_gradient(xs[m:m + 1])
+            sp.objective_and_gradient(xs[m:m + 1])  # warm-up (graph capture)
             sp.sync()
             tp0 = time.perf_counter()
-            sp.objective_and_gradient(xs[m:m + 1])
+            Jm, gm_ = sp.objective_and_gradient(xs[m:m + 1])
             sp.sync()
             tsum += time.perf_counter() - tp0
+            if m == 0:
+                eng0 = (Jm[0], gm_[0][-2], gm_[0][-1])
             sp.close()
         plain = M / tsum
         log("plain (one member at a time): %.3f grad evals/s" % plain)
 
     ref = None
     if not args.no_plain:
-        ref = refstyle_plain(n, nt, dt, k0, rho0, mask0, c0_member0.astype(np.float32))
+        pr0 = members[0]
+        ref, (Jr, gkr, grr) = refstyle_gradient(nt, dt, k0, rho0, mask0, pr0["wm"], pr0["gm"],
+                                                c0_member0.astype(np.float32), pr0["d1"])
+        # the comparison leg computes the same quantities as the engine (FP32 both, different CG
+        # iterates): J, g_kappa and g_rho of member 0
+        ref["vs_engine_rel"] = {"J": float("%.3g" % (abs(Jr - eng0[0]) / abs(eng0[0]))),
+                                "g_kappa": float("%.3g" % (abs(gkr - eng0[1]) / abs(eng0[1]))),
+                                "g_rho": float("%.3g" % (abs(grr - eng0[2]) / abs(eng0[2])))}
         log("reference-structured plain path:", ref)
-    acc = None if args.no_accuracy else accuracy_64()
-    if acc:
+
+    acc, cpu = None, None
+    if not (args.no_accuracy and args.no_cpu):
+        acc, t_orc, threads, ksp = accuracy_and_cpu(n, dt, members[0], members[0]["d1"], args.nt_oracle)
         log("accuracy vs FP64 oracle:", acc)
-    cpu = None if args.no_cpu else cpu_baseline(n, nt, dt, members[0], c0_member0)
-    if cpu:
+        cpu = {"value": args.nt_oracle / (t_orc * nt), "unit": "grad evals/s", "cores": threads, "kind": "port",
+               "sample": "oracle FP64 (the reference path restated), one whole gradient evaluation (forward, adjoint "
+                         "with recomputed half steps, Phi^T, kappa/rho quadratures) of one %d^3 member at nt=%d "
+                         "(%d + %d CG iterations): %.1f s on %d threads, scaled to nt=%d by the step count"
+                         % (n, args.nt_oracle, ksp["state"], ksp["adjoint"], t_orc, threads, nt)}
+        if not args.no_cpu:
+            t1t, n1, nt1 = cpu_one_thread(dt)
+            cpu["one_thread"] = {"seconds": round(t1t, 2), "sample": "one whole evaluation of a %d^3 member at nt=%d "
+                                 "on 1 thread (grad evals/s there: %.4g)" % (n1, nt1, 1.0 / t1t)}
         log("cpu baseline:", cpu)
 
     out = {
@@ -806,6 +893,8 @@ def main():
                                "(%d nonzero), FP32 bulk + FP64 reductions" % (M, n, nt, dt, args.nbases, args.nnz),
                    "n": n, "members": M, "nt": nt, "dt": dt, "nbases": args.nbases,
                    "checkpoint_slots": args.checkpoint_slots},
+        "timing": "value: product path (graph replay, no per-kernel events); roofline and kernels: one further "
+                  "evaluation with HIP events around each CG kernel launch (graph replay off)",
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "algorithmic_bytes_per_launch": _lib.KERNEL_FIELDS[dom] * field_bytes * M,
@@ -817,6 +906,7 @@ def main():
         "batched_vs_refstyle_plain": None if ref is None else round(value / ref["value"], 2),
         "cg_iterations_per_solve": round(mbar, 3),
         "model_hbm_GBs": round(model_gbs, 1),
+        "model_hbm_frac": round(model_gbs / HBM_PEAK_GBS, 4),
         "kernels": kernels,
         "rel_err_fp64": acc,
     }

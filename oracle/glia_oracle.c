@@ -204,6 +204,10 @@ static double *dalloc(int64_t n) {
   return p;
 }
 
+/* progress lines on stderr, one per time step of solveState / solveAdjoint (long oracle runs) */
+static int orc_verbose = 0;
+void orc_set_verbose(int v) { orc_verbose = v; }
+
 int orc_max_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();
@@ -746,6 +750,7 @@ void orc_get_ct(orc_t *o, double *c) { memcpy(c, o->c_t, sizeof(double) * o->ng)
 void orc_get_p0(orc_t *o, double *p) { memcpy(p, o->p_0, sizeof(double) * o->ng); }
 void orc_get_c_hist(orc_t *o, int i, double *c) { memcpy(c, o->c_hist[i], sizeof(double) * o->ng); }
 void orc_get_p_hist(orc_t *o, int i, double *p) { memcpy(p, o->p_hist[i], sizeof(double) * o->ng); }
+void orc_get_c_half(orc_t *o, int i, double *c) { memcpy(c, o->c_half[i], sizeof(double) * o->ng); }
 void orc_get_ksp(orc_t *o, int *state, int *adj, int *last) {
   if (state) *state = o->diff_ksp_itr_state;
   if (adj) *adj = o->diff_ksp_itr_adj;
@@ -782,6 +787,7 @@ int orc_solve_state(orc_t *o, int linearized) {
     o->diff_ksp_itr_state += o->ksp_itr;
     if (linearized == 2) solve_incremental(o, i, 2);
     if (linearized == 0) memcpy(o->c_hist[i + 1], o->c_t, sizeof(double) * o->ng);
+    if (orc_verbose) fprintf(stderr, "oracle: state step %d of %d\n", i + 1, o->nt);
   }
   return o->diff_ksp_itr_state;
 }
@@ -813,6 +819,7 @@ int orc_solve_adjoint(orc_t *o, int linearized) {
     orc_diffusion_solve(o, o->p_0, dt / 2.0);
     o->diff_ksp_itr_adj += o->ksp_itr;
     memcpy(o->p_hist[nt - i - 1], o->p_0, sizeof(double) * o->ng);
+    if (orc_verbose) fprintf(stderr, "oracle: adjoint step %d of %d\n", i + 1, nt);
   }
   return o->diff_ksp_itr_adj;
 }

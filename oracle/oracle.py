@@ -39,6 +39,7 @@ def _load():
         "orc_destroy": (None, [vp]),
         "orc_max_threads": (i, []),
         "orc_set_threads": (None, [i]),
+        "orc_set_verbose": (None, [i]),
         "orc_r2c": (None, [vp, _dp, _cp]),
         "orc_c2r": (None, [vp, _cp, _dp]),
         "orc_grad": (None, [vp, _dp, _dp, _dp, _dp]),
@@ -81,6 +82,7 @@ def _load():
         "orc_get_p0": (None, [vp, _dp]),
         "orc_get_c_hist": (None, [vp, i, _dp]),
         "orc_get_p_hist": (None, [vp, i, _dp]),
+        "orc_get_c_half": (None, [vp, i, _dp]),
         "orc_get_ksp": (None, [vp, vp, vp, vp]),
         "orc_solve_state": (i, [vp, i]),
         "orc_solve_adjoint": (i, [vp, i]),
@@ -119,6 +121,11 @@ def max_threads():
 
 def set_threads(n):
     _load().orc_set_threads(int(n))
+
+
+def set_verbose(on):
+    """One stderr line per time step of the state and adjoint solves (progress of long runs)."""
+    _load().orc_set_verbose(int(bool(on)))
 
 
 def phi_mesh(np_, cm, sigma, spacing_factor):
@@ -402,6 +409,14 @@ class Oracle:
 
     def p_hist(self, i):
         return self._get(self.L.orc_get_p_hist, int(i))
+
+    def c_half(self, i):
+        """c_half_[i] (kept with adjoint_store)."""
+        return self._get(self.L.orc_get_c_half, int(i))
+
+    def set_secondary(self, kt):
+        """DiffCoef::setSecondaryCoefficients: k~ = kt (wm + k_gm gm + k_glm csf)."""
+        self.L.orc_set_secondary(self.h, float(kt))
 
     def ksp(self):
         a, b, c = (ctypes.c_int(), ctypes.c_int(), ctypes.c_int())
