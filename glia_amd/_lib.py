@@ -99,6 +99,13 @@ SIGNATURES = {
     "glia_rd_stats": (_i, [_vp, _i, ctypes.POINTER(Stats)]),
     "glia_rd_reset_stats": (_i, [_vp]),
     "glia_rd_smooth_field": (_i, [_vp, _i, _i, _d]),
+    "glia_rd_set_time": (_i, [_vp, _d, _i]),
+    "glia_rd_keep_adjoint_history": (_i, [_vp, _i]),
+    "glia_rd_get_history": (_i, [_vp, _i, _i, _i, _vp, _i]),
+    "glia_rd_set_secondary": (_i, [_vp, _i, _d]),
+    "glia_rd_grad_diffusion": (_i, [_vp, _vp]),
+    "glia_rd_grad_reaction": (_i, [_vp, _vp]),
+    "glia_rd_check_gradient": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "glia_rd_refresh_tissue": (_i, [_vp, _i]),
     "glia_rd_use_graphs": (_i, [_vp, _i]),
     "glia_rd_profile": (_i, [_vp, _i]),

@@ -5,6 +5,8 @@
 #include <chrono>
 #include <string>
 #include <thread>
+#include <cmath>
+#include <random>
 #include <vector>
 
 #include <hip/hip_runtime.h>
@@ -366,6 +368,49 @@ int glia_rd_solve_adjoint(glia_rd_t h, int linearized) {
   return ret(h, rc);
 }
 
+int glia_rd_set_time(glia_rd_t h, double dt, int nt) {
+  GLIA_H(h);
+  return ret(h, h->e->set_time(dt, nt));
+}
+
+int glia_rd_keep_adjoint_history(glia_rd_t h, int enable) {
+  GLIA_H(h);
+  return ret(h, h->e->keep_adjoint_history(enable != 0));
+}
+
+int glia_rd_get_history(glia_rd_t h, int member, int kind, int index, float* dst, int mem_kind) {
+  GLIA_H(h);
+  if (!dst) return arg_err(h, "null destination");
+  return ret(h, h->e->get_history(member, kind, index, dst, mem_kind == GLIA_RD_DEVICE));
+}
+
+int glia_rd_set_secondary(glia_rd_t h, int member, double kt) {
+  GLIA_H(h);
+  int rc = h->e->set_secondary(member, kt);
+  if (!rc) rc = h->e->sync();
+  return ret(h, rc);
+}
+
+int glia_rd_grad_diffusion(glia_rd_t h, double* gk) {
+  GLIA_H(h);
+  if (!gk) return arg_err(h, "null output");
+  for (int m = 0; m < h->e->nmembers(); m++) {
+    const int rc = h->e->grad_kappa_rho(m, gk + m, nullptr);
+    if (rc) return ret(h, rc);
+  }
+  return 0;
+}
+
+int glia_rd_grad_reaction(glia_rd_t h, double* gr) {
+  GLIA_H(h);
+  if (!gr) return arg_err(h, "null output");
+  for (int m = 0; m < h->e->nmembers(); m++) {
+    const int rc = h->e->grad_kappa_rho(m, nullptr, gr + m);
+    if (rc) return ret(h, rc);
+  }
+  return 0;
+}
+
 static int eval(glia_rd_t h, int mode, const double* x, double* J, double* dJ, bool grad) {
   GLIA_H(h);
   std::vector<std::vector<double>> xv, g;
@@ -395,6 +440,45 @@ int glia_rd_gradient(glia_rd_t h, int mode, const double* x, double* dJ) {
 int glia_rd_objective_and_gradient(glia_rd_t h, int mode, const double* x, double* J, double* dJ) {
   if (!J || !dJ) return arg_err(h, "null output pointer");
   return eval(h, mode, x, J, dJ, true);
+}
+
+int glia_rd_check_gradient(glia_rd_t h, int mode, const double* x, const double* dir, double* err, double* gdiff) {
+  GLIA_H(h);
+  if (!x) return arg_err(h, "null x");
+  const int nm = h->e->nmembers();
+  std::vector<std::vector<double>> xv, g;
+  if (!split_x(h->e.get(), x, 2, xv)) return arg_err(h, "x does not match the basis sizes");
+  std::vector<std::vector<double>> dv(nm);
+  if (dir) {
+    if (!split_x(h->e.get(), dir, 2, dv)) return arg_err(h, "dir does not match the basis sizes");
+  } else {
+    std::mt19937_64 gen(0x9e3779b97f4a7c15ull);
+    std::uniform_real_distribution<double> u(0.0, 1.0);
+    for (int m = 0; m < nm; m++) {
+      dv[m].resize(xv[m].size());
+      for (double& e : dv[m]) e = u(gen);
+    }
+  }
+  std::vector<double> J0, Ji;
+  int rc = h->e->objective_and_gradient(mode, xv, true, J0, g);
+  if (rc) return ret(h, rc);
+  std::vector<double> gd(nm, 0.0);
+  for (int m = 0; m < nm; m++)
+    for (size_t k = 0; k < g[m].size(); k++) gd[m] += g[m][k] * dv[m][k];
+  std::vector<std::vector<double>> xn(nm), unused;
+  for (int i = 0; i < 6; i++) {
+    const double hi = std::pow(10.0, -i);
+    for (int m = 0; m < nm; m++) {
+      xn[m] = xv[m];
+      for (size_t k = 0; k < xn[m].size(); k++) xn[m][k] += hi * dv[m][k];
+    }
+    if ((rc = h->e->objective_and_gradient(mode, xn, false, Ji, unused))) return ret(h, rc);
+    for (int m = 0; m < nm; m++) {
+      if (err) err[6 * m + i] = std::fabs(Ji[m] - (J0[m] + gd[m] * hi));
+      if (gdiff) gdiff[6 * m + i] = (Ji[m] - J0[m]) / hi - gd[m];
+    }
+  }
+  return 0;
 }
 
 int glia_rd_hessian(glia_rd_t h, int diffusivity, const double* x, double* y) {

@@ -80,6 +80,13 @@ LaunchCtx RDEnsemble::ctx() const {
   return c;
 }
 
+// launch context of member m alone
+static LaunchCtx member_ctx(LaunchCtx c, const MemberDev* md, int m) {
+  c.mem = md + m;
+  c.nm = 1;
+  return c;
+}
+
 int RDEnsemble::init() {
   if (nm_ < 1) return fail(-2, "nmembers must be >= 1");
   if (!supported_line(d_.n0) || !supported_line(d_.n1) || !supported_z(d_.n2))
@@ -130,6 +137,7 @@ int RDEnsemble::init() {
   mh_.resize(nm_);
   pool_.resize(nm_);
   chalf_.resize(nm_);
+  apool_.resize(nm_);
   hist_dev_.assign(nm_, nullptr);
   chalf_dev_.assign(nm_, nullptr);
   st_.assign(nm_, nullptr);
@@ -183,6 +191,24 @@ int RDEnsemble::alloc_member(int m) {
   nb = std::max<int64_t>(nb, (int64_t)ew_blocks() * 2);
   GLIA_HIP(hipMalloc(&part_[m], sizeof(double) * nb));
   M.part = part_[m];
+  int rc = alloc_history(m);
+  if (rc) return rc;
+  M.kr_gm = (float)cfg_.k_gm_wm;
+  M.rr_gm = (float)cfg_.r_gm_wm;
+  // MatProp::setValues defaults (src/mat/MatProp.cpp:20-33): wm = 1, filter = 1; Obs filter = 1
+  std::vector<float> ones(d_.ng, 1.0f);
+  GLIA_TRYC(copy_sync(M.r[RS_WM], ones.data(), rb, hipMemcpyHostToDevice));
+  GLIA_TRYC(copy_sync(M.r[RS_FILTER], ones.data(), rb, hipMemcpyHostToDevice));
+  GLIA_TRYC(copy_sync(M.r[RS_OBS], ones.data(), rb, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// the time history of member m for the current nt: the spectral state pool (store-all: c_[0..nt];
+// checkpointing: the checkpoint slots), the half-step states c_half_ (adjoint_store_, store-all)
+// and, when kept, the adjoint history p_[0..nt] (PdeOperatorsRD ctor, src/pde/PdeOperators.cpp:26-53)
+int RDEnsemble::alloc_history(int m) {
+  MemberDev& M = mh_[m];
+  const size_t cb = sizeof(float2) * d_.nspec, rb = sizeof(float) * d_.ng;
   pool_[m].assign(npool_, nullptr);
   for (int i = 0; i < npool_; i++) GLIA_HIP(hipMalloc(&pool_[m][i], cb));
   GLIA_HIP(hipMalloc(&hist_dev_[m], sizeof(float2*) * npool_));
@@ -195,13 +221,129 @@ int RDEnsemble::alloc_member(int m) {
     GLIA_TRYC(copy_sync(chalf_dev_[m], chalf_[m].data(), sizeof(float*) * cfg_.nt, hipMemcpyHostToDevice));
   }
   M.chalf = chalf_dev_[m];
-  M.kr_gm = (float)cfg_.k_gm_wm;
-  M.rr_gm = (float)cfg_.r_gm_wm;
-  // MatProp::setValues defaults (src/mat/MatProp.cpp:20-33): wm = 1, filter = 1; Obs filter = 1
-  std::vector<float> ones(d_.ng, 1.0f);
-  GLIA_TRYC(copy_sync(M.r[RS_WM], ones.data(), rb, hipMemcpyHostToDevice));
-  GLIA_TRYC(copy_sync(M.r[RS_FILTER], ones.data(), rb, hipMemcpyHostToDevice));
-  GLIA_TRYC(copy_sync(M.r[RS_OBS], ones.data(), rb, hipMemcpyHostToDevice));
+  if (keep_adj_ && hist_all_) {
+    apool_[m].assign(cfg_.nt + 1, nullptr);
+    for (int i = 0; i <= cfg_.nt; i++) GLIA_HIP(hipMalloc(&apool_[m][i], cb));
+  }
+  return 0;
+}
+
+void RDEnsemble::free_history(int m) {
+  for (auto p : pool_[m]) (void)hipFree(p);
+  for (auto p : chalf_[m]) (void)hipFree(p);
+  for (auto p : apool_[m]) (void)hipFree(p);
+  (void)hipFree(hist_dev_[m]);
+  (void)hipFree(chalf_dev_[m]);
+  pool_[m].clear();
+  chalf_[m].clear();
+  apool_[m].clear();
+  hist_dev_[m] = nullptr;
+  chalf_dev_[m] = nullptr;
+  mh_[m].hist = nullptr;
+  mh_[m].chalf = nullptr;
+}
+
+// TumorParameters dt_ / nt_ and PdeOperatorsRD::resizeTimeHistory (src/pde/PdeOperators.cpp:56-103):
+// the history is reallocated for the new nt; stored states are dropped.  The diffusion solver's
+// context dt is left as it is, as in the reference.
+int RDEnsemble::set_time(double dt, int nt) {
+  if (nt < 1 || !(dt > 0)) return fail(-2, "nt >= 1 and dt > 0 required");
+  GLIA_HIP(hipStreamSynchronize(stream_));
+  cfg_.dt = dt;
+  if (nt == cfg_.nt) return 0;
+  for (int m = 0; m < nm_; m++) free_history(m);
+  cfg_.nt = nt;
+  npool_ = hist_all_ ? cfg_.nt + 1 : std::max(1, cfg_.checkpoint_slots);
+  stored_.clear();
+  free_pool_.clear();
+  for (int m = 0; m < nm_; m++) {
+    const int rc = alloc_history(m);
+    if (rc) return rc;
+  }
+  return upload_members();
+}
+
+// keep the adjoint history p_[0..nt] during solveAdjoint (store-all only): nt + 1 spectral fields
+// per member
+int RDEnsemble::keep_adjoint_history(bool on) {
+  if (on && !hist_all_) return fail(-4, "the adjoint history is kept in store-all mode only (checkpoint_slots = 0)");
+  if (on == keep_adj_) return 0;
+  GLIA_HIP(hipStreamSynchronize(stream_));
+  keep_adj_ = on;
+  const size_t cb = sizeof(float2) * d_.nspec;
+  for (int m = 0; m < nm_; m++) {
+    for (auto p : apool_[m]) (void)hipFree(p);
+    apool_[m].clear();
+    if (on) {
+      apool_[m].assign(cfg_.nt + 1, nullptr);
+      for (int i = 0; i <= cfg_.nt; i++) GLIA_HIP(hipMalloc(&apool_[m][i], cb));
+    }
+  }
+  adj_valid_ = false;
+  return 0;
+}
+
+int RDEnsemble::store_adjoint(int i) {
+  if (!keep_adj_) return 0;
+  const size_t cb = sizeof(float2) * d_.nspec;
+  for (int m = 0; m < nm_; m++)
+    GLIA_HIP(hipMemcpyAsync(apool_[m][i], mh_[m].c[CS_A], cb, hipMemcpyDeviceToDevice, stream_));
+  return 0;
+}
+
+// the history entries the reference keeps as public members of PdeOperatorsRD: c_[index]
+// (0..nt), c_half_[index] (0..nt-1, adjoint_store_) and p_[index] (0..nt, when kept), as a real
+// field of member m.  Under checkpointing only the stored time indices are available.
+int RDEnsemble::get_history(int m, int kind, int index, float* dst, bool device) {
+  if (m < 0 || m >= nm_) return fail(-2, "bad member");
+  const int nt = cfg_.nt;
+  const size_t cb = sizeof(float2) * d_.nspec;
+  LaunchCtx c = member_ctx(ctx(), md_, m);
+  if (kind == HIST_STATE) {
+    if (index < 0 || index > nt) return fail(-2, "state history index must be in [0, nt]");
+    auto it = stored_.find(index);
+    if (it == stored_.end()) return fail(-6, "this time index of the state history is not stored");
+    GLIA_HIP(hipMemcpyAsync(mh_[m].c[CS_T6], pool_[m][it->second], cb, hipMemcpyDeviceToDevice, stream_));
+  } else if (kind == HIST_HALF) {
+    if (index < 0 || index >= nt) return fail(-2, "half-step history index must be in [0, nt)");
+    if (!(cfg_.adjoint_store && hist_all_) || stored_.empty())
+      return fail(-6, "the half-step history is kept with adjoint_store in store-all mode, after solveState");
+    GLIA_HIP(hipMemcpyAsync(dst, chalf_[m][index], sizeof(float) * d_.ng,
+                            device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, stream_));
+    GLIA_HIP(hipStreamSynchronize(stream_));
+    return 0;
+  } else if (kind == HIST_ADJOINT) {
+    if (index < 0 || index > nt) return fail(-2, "adjoint history index must be in [0, nt]");
+    if (!keep_adj_ || !adj_valid_) return fail(-6, "the adjoint history is not kept (keep it, then run solveAdjoint)");
+    GLIA_HIP(hipMemcpyAsync(mh_[m].c[CS_T6], apool_[m][index], cb, hipMemcpyDeviceToDevice, stream_));
+  } else {
+    return fail(-2, "unknown history kind");
+  }
+  GLIA_TRYC(launch_col_plain(c, 0, +1, CS_T6, CS_T5, false));
+  GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T5, CS_T5, false));
+  GLIA_TRYC(launch_z_point(c, ZPM_C2R, CS_T5, 0, RS_CH, -1, 0.f, false));
+  return get_field(m, RS_CH, dst, device);
+}
+
+// DiffCoef::setSecondaryCoefficients: k~ = kt (m_wm + k_gm m_gm + k_glm m_csf) of member m, the
+// coefficient field of the solveState(2) sources
+int RDEnsemble::set_secondary(int m, double kt) {
+  if (m < 0 || m >= nm_) return fail(-2, "bad member");
+  LaunchCtx cm = member_ctx(ctx(), md_, m);
+  GLIA_TRYC(launch_axpby_real(cm, RS_WM, kt, RS_GM, kt * cfg_.k_gm_wm, RS_KS));
+  GLIA_TRYC(launch_axpby_real(cm, RS_KS, 1.0, RS_CSF, kt * cfg_.k_glm_wm, RS_KS));
+  return 0;
+}
+
+// gradDiffusion / gradReaction (src/grad/DerivativeOperators.cpp:189-321) of member m, nk = nr = 1,
+// from the trapezoid sums the last adjoint sweep accumulated
+int RDEnsemble::grad_kappa_rho(int m, double* gk, double* gr) {
+  if (m < 0 || m >= nm_) return fail(-2, "bad member");
+  PcgState s;
+  GLIA_TRYC(copy_sync(&s, st_[m], sizeof(s), hipMemcpyDeviceToHost));
+  const double h3 = lebesgue();
+  if (gk) *gk = h3 * cfg_.dt * s.gk;
+  if (gr) *gr = h3 * cfg_.dt * s.gr;
   return 0;
 }
 
@@ -222,10 +364,7 @@ RDEnsemble::~RDEnsemble() {
   for (int m = 0; m < (int)mh_.size(); m++) {
     for (int s = 0; s < CS_NUM; s++) (void)hipFree(mh_[m].c[s]);
     for (int s = 0; s < RS_NUM; s++) (void)hipFree(mh_[m].r[s]);
-    for (auto p : pool_[m]) (void)hipFree(p);
-    for (auto p : chalf_[m]) (void)hipFree(p);
-    (void)hipFree(hist_dev_[m]);
-    (void)hipFree(chalf_dev_[m]);
+    free_history(m);
     (void)hipFree(st_[m]);
     (void)hipFree(part_[m]);
     (void)hipFree(bricks_dev_[m]);
@@ -250,11 +389,6 @@ int RDEnsemble::sync() {
 // ---------------------------------------------------------------------------------------------
 // setup
 // ---------------------------------------------------------------------------------------------
-static LaunchCtx member_ctx(LaunchCtx c, const MemberDev* md, int m) {
-  c.mem = md + m;
-  c.nm = 1;
-  return c;
-}
 
 int RDEnsemble::set_field(int m, int rslot, const float* src, bool device) {
   if (m < 0 || m >= nm_ || rslot < 0 || rslot >= RS_NUM) return fail(-2, "bad member or field");
@@ -400,10 +534,9 @@ int RDEnsemble::set_basis(int m, int np, const double* centers, double sigma) {
   sigma_[m] = sigma;
   const int n[3] = {d_.n0, d_.n1, d_.n2};
   const double sig_s = cfg_.smoothing_factor * 2.0 * M_PI / d_.n0;
-  int kr = 0;
+  int kr[3] = {0, 0, 0}, tap_stride = 1;
   std::vector<double> taps1;
-  smoother_taps(sig_s, taps1, kr);
-  const int tap_stride = 2 * kr + 1;
+  smoother_taps(sig_s, taps1, kr, tap_stride);
   std::vector<Brick> bk(np);
   std::vector<BrickJob> jobs(np);
   int64_t off = 0;
@@ -423,7 +556,7 @@ int RDEnsemble::set_basis(int m, int np, const double* centers, double sigma) {
     }
     bk[i].off = off;
     jobs[i].out_off = off;
-    jobs[i].kr = kr;
+    for (int a = 0; a < 3; a++) jobs[i].kr[a] = kr[a];
     jobs[i].member = 0;
     off += (int64_t)bk[i].b[0] * bk[i].b[1] * bk[i].b[2];
   }
@@ -441,7 +574,7 @@ int RDEnsemble::set_basis(int m, int np, const double* centers, double sigma) {
     int64_t ext = 0;
     int j1 = j0;
     while (j1 < np) {
-      const int64_t ne = (int64_t)(jobs[j1].b[0] + 2 * kr) * (jobs[j1].b[1] + 2 * kr) * (jobs[j1].b[2] + 2 * kr);
+      const int64_t ne = (int64_t)(jobs[j1].b[0] + 2 * kr[0]) * (jobs[j1].b[1] + 2 * kr[1]) * (jobs[j1].b[2] + 2 * kr[2]);
       if (j1 > j0 && (ext + ne) * 8 > (int64_t)512 << 20) break;
       jobs[j1].ext_off = ext;
       ext += ne;
@@ -656,7 +789,8 @@ int RDEnsemble::forward_step(int sx, int step, bool store_half) {
 
 // PdeOperatorsRD::solveState (src/pde/PdeOperators.cpp:235-316) from c0 in RS_C0
 int RDEnsemble::solve_state(int linearized, bool keep_history) {
-  if (linearized != 0) return fail(-4, "solveState: only linearized = 0 (state equation) is supported");
+  if (linearized == 1 || linearized == 2) return solve_tangent(linearized == 2);
+  if (linearized != 0) return fail(-2, "solveState: linearized must be 0, 1 or 2");
   int rc = forward_transform_real(RS_C0, CS_X);
   if (rc) return rc;
   stored_.clear();
@@ -715,6 +849,19 @@ int RDEnsemble::solve_state(int linearized, bool keep_history) {
   return inverse_to_real(CS_X, CS_T5, RS_CT);
 }
 
+// PdeOperatorsRD::solveState(1) / solveState(2) (src/pde/PdeOperators.cpp:235-316): the tangent-linear
+// forward from c~(0) in RS_C0 around the state history of the last solveState(0) (reaction(1, i)
+// linearized at c_[i]); with 2 also the solveIncremental sources of the secondary coefficients
+// (set_secondary).  c~(T) lands in RS_CT; the state history is left as it is.
+int RDEnsemble::solve_tangent(bool sources) {
+  if (stored_.find(0) == stored_.end()) return fail(-6, "solveState(1|2) linearizes around the history of solveState(0)");
+  int rc = forward_transform_real(RS_C0, CS_A);
+  if (rc) return rc;
+  int ct = CS_X;
+  if ((rc = tangent_forward(CS_A, sources, &ct))) return rc;
+  return inverse_to_real(CS_A, CS_T5, RS_CT);
+}
+
 // alpha(T) = -O^T (O c(T) - d1) and m1 = ||O c(T) - d1||^2 (src/grad/DerivativeOperatorsRD.cpp:154-162)
 int RDEnsemble::misfit_and_final_adjoint(int cslot_c, int out) {
   LaunchCtx c = ctx();
@@ -765,6 +912,7 @@ int RDEnsemble::adjoint_step(int i, int cslot_c) {
   GLIA_TRYC(launch_col_plain(c, 0, -1, CS_A, CS_A, false));
   rc = cn_solve(CS_A, dt / 2.0);
   if (rc) return rc;
+  if ((rc = store_adjoint(i))) return rc;
   return quad_on_ ? quad(cslot_c, CS_A, (i == 0) ? 0.5 : 1.0) : 0;
 }
 
@@ -981,13 +1129,35 @@ int RDEnsemble::reverse(int a, int b, int pa, int free_slots) {
   return reverse(a, mpos, pa, free_slots);
 }
 
-// PdeOperatorsRD::solveAdjoint from alpha(T) of the current data, with the gradient quadrature
+// PdeOperatorsRD::solveAdjoint (src/pde/PdeOperators.cpp:372-420) with the gradient quadratures of
+// gradDiffusion / gradReaction accumulated along the sweep.
+//   linearized 0 or 1: from alpha(T) = -O^T (O c(T) - d1) of the current data and the state history
+//   linearized 2: the incremental adjoint from a~(T) = -O^T O c~(T), c~(T) in RS_CT (after
+//     solveState(1|2)); its i = nt quadrature term uses the adjoint final condition of the last
+//     gradient, as the reference's p_[nt] (which solveAdjoint(2) does not overwrite) does
+// alpha(0) lands in RS_A0; with the adjoint history kept, p_[i] is stored along the way.
 int RDEnsemble::solve_adjoint(int linearized) {
-  if (linearized != 0 && linearized != 1) return fail(-4, "solveAdjoint: linearized must be 0 or 1");
+  if (linearized < 0 || linearized > 2) return fail(-2, "solveAdjoint: linearized must be 0, 1 or 2");
   if (stored_.find(0) == stored_.end()) return fail(-6, "solveAdjoint needs solveState with history first");
   GLIA_TRYC(launch_state_reset(ctx(), 2));
-  int rc = misfit_and_final_adjoint(CS_X);
+  int rc;
+  if (linearized == 2) {
+    int base = CS_X;  // c(T) of the state history
+    if (hist_all_) {
+      base = state_slot(cfg_.nt);
+    } else {
+      GLIA_TRYC(launch_copy(ctx(), state_slot(stored_[0]), CS_X));
+      for (int t = 0; t < cfg_.nt; t++)
+        if ((rc = forward_step(CS_X, t, false))) return rc;
+    }
+    if ((rc = forward_transform_real(RS_CT, CS_A))) return rc;
+    if ((rc = incremental_adjoint(base, true))) return rc;
+    adj_valid_ = keep_adj_;
+    return 0;
+  }
+  rc = misfit_and_final_adjoint(CS_X);
   if (rc) return rc;
+  if ((rc = store_adjoint(cfg_.nt))) return rc;
   rc = quad(CS_X, CS_A, 0.5);
   if (rc) return rc;
   if (hist_all_) {
@@ -1000,6 +1170,7 @@ int RDEnsemble::solve_adjoint(int linearized) {
     rc = reverse(0, cfg_.nt, stored_[0], npool_ - 1);
     if (rc) return rc;
   }
+  adj_valid_ = keep_adj_;
   return inverse_to_real(CS_A, CS_T5, RS_A0);
 }
 
@@ -1136,17 +1307,22 @@ int RDEnsemble::objective_and_gradient(int mode, const std::vector<std::vector<d
 
 // Per-axis taps of the reference's periodic smoothing kernel (computeWeierstrassFilter,
 // src/cuda/SpectralOperators.cu:12-51): g(x) + g(x - 2 pi) per axis, normalized by its discrete
-// sum; the 3D kernel is their product.  Taps beyond 12 sigma are dropped (kr = 0: identity).
-void RDEnsemble::smoother_taps(double sigma, std::vector<double>& taps, int& kr) const {
+// sum; the 3D kernel is their product.  Axis a keeps the taps |s| <= kr[a] = min(n_a / 2 - 1,
+// ceil(12 sigma / h_a)) at taps[a * stride + s + kr[a]], stride = 2 max(kr) + 1: taps beyond 12
+// sigma (< e^-72) are dropped, and an index wraps at most once, so no residue of a short axis is
+// counted twice.  sigma <= 0 gives the identity (kr = 0).
+void RDEnsemble::smoother_taps(double sigma, std::vector<double>& taps, int kr[3], int& stride) const {
   const int n[3] = {d_.n0, d_.n1, d_.n2};
-  kr = 0;
-  for (int a = 0; a < 3; a++)
-    if (sigma > 0) kr = std::max(kr, std::min(n[a] / 2 - 1, (int)std::ceil(12.0 * sigma / h_[a])));
-  const int stride = 2 * kr + 1;
+  int kmax = 0;
+  for (int a = 0; a < 3; a++) {
+    kr[a] = (sigma > 0) ? std::min(n[a] / 2 - 1, (int)std::ceil(12.0 * sigma / h_[a])) : 0;
+    kmax = std::max(kmax, kr[a]);
+  }
+  stride = 2 * kmax + 1;
   taps.assign(3 * stride, 0.0);
   for (int a = 0; a < 3; a++) {
     if (sigma <= 0) {
-      taps[a * stride + kr] = 1.0;
+      taps[a * stride] = 1.0;
       continue;
     }
     std::vector<double> f1(n[a]);
@@ -1158,28 +1334,18 @@ void RDEnsemble::smoother_taps(double sigma, std::vector<double>& taps, int& kr)
       f1[i] = v;
       sum += v;
     }
-    for (int s = -kr; s <= kr; s++) taps[a * stride + s + kr] = f1[((s % n[a]) + n[a]) % n[a]] / sum;
+    for (int s = -kr[a]; s <= kr[a]; s++) taps[a * stride + s + kr[a]] = f1[((s % n[a]) + n[a]) % n[a]] / sum;
   }
 }
 
 // SpectralOperators::weierstrassSmoother (src/grad/SpectralOperators.cpp:264-381) of a real field,
-// in place: three periodic axis convolutions through a scratch buffer.  Axis a keeps the taps
-// |s| <= kr_a = min(n_a / 2 - 1, ceil(12 sigma / h_a)), so every index wraps at most once and no
-// residue of the axis is counted twice.  The device taps are kept while the width repeats (the
-// mass-effect loops smooth with one width many times).
+// in place: three periodic axis convolutions through a scratch buffer.  The device taps are kept
+// while the width repeats (the mass-effect loops smooth with one width many times).
 int RDEnsemble::smooth_taps_for(double sigma) {
   if (sigma == smooth_sigma_) return 0;
   std::vector<double> taps;
-  int kr = 0;
-  smoother_taps(sigma, taps, kr);
-  const int n[3] = {d_.n0, d_.n1, d_.n2};
-  const int stride = 2 * kr + 1;
-  int kra[3];
-  for (int a = 0; a < 3; a++) {
-    kra[a] = std::min(n[a] / 2 - 1, (int)std::ceil(12.0 * sigma / h_[a]));
-    // recentre axis a's taps at kra[a]: entry s + kra[a] for |s| <= kra[a]
-    for (int s = -kra[a]; s <= kra[a]; s++) taps[a * stride + s + kra[a]] = taps[a * stride + s + kr];
-  }
+  int kra[3], stride = 1;
+  smoother_taps(sigma, taps, kra, stride);
   if (taps.size() > smooth_cap_) {
     GLIA_HIP(hipStreamSynchronize(stream_));
     if (smooth_taps_) (void)hipFree(smooth_taps_);

@@ -99,6 +99,17 @@ class RDEnsemble {
   // history of a previous evaluation at the linearization point
   int hessian(const std::vector<std::vector<double>>& xt, bool diffusivity, std::vector<std::vector<double>>& y);
   int stats(int m, MemberStats* s);
+  // TumorParameters dt_/nt_ with PdeOperatorsRD::resizeTimeHistory (the history is reallocated)
+  int set_time(double dt, int nt);
+  // keep the adjoint history p_[0..nt] during solveAdjoint (store-all mode; nt + 1 spectral fields)
+  int keep_adjoint_history(bool on);
+  // c_[i] (HIST_STATE), c_half_[i] (HIST_HALF) or p_[i] (HIST_ADJOINT) of member m as a real field
+  enum { HIST_STATE = 0, HIST_HALF = 1, HIST_ADJOINT = 2 };
+  int get_history(int m, int kind, int index, float* dst, bool device);
+  // DiffCoef::setSecondaryCoefficients for solveState(2): k~ = kt (wm + k_gm gm + k_glm csf)
+  int set_secondary(int m, double kt);
+  // gradDiffusion / gradReaction of member m from the last adjoint sweep's quadratures
+  int grad_kappa_rho(int m, double* gk, double* gr);
   int basis_size(int m) const { return (m >= 0 && m < nm_) ? np_[m] : -1; }
   int reset_counters();
   int sync();
@@ -117,13 +128,17 @@ class RDEnsemble {
 
  private:
   int alloc_member(int m);
+  int alloc_history(int m);
+  void free_history(int m);
+  int store_adjoint(int i);
+  int solve_tangent(bool sources);
   int upload_members();
   int check(hipError_t e, const char* what);
   int copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind);
   int fail(int code, const std::string& msg);
   LaunchCtx ctx() const;
 
-  void smoother_taps(double sigma, std::vector<double>& taps, int& kr) const;
+  void smoother_taps(double sigma, std::vector<double>& taps, int kr[3], int& stride) const;
   int smooth_taps_for(double sigma);
   int cn_solve(int sx, double tau);
   int launch_iteration(int sx, int slot);                       // the launches of one CG iteration
@@ -158,6 +173,9 @@ class RDEnsemble {
   MemberDev* md_ = nullptr;             // device array
   std::vector<std::vector<float2*>> pool_;   // spectral state pool (history / checkpoints)
   std::vector<std::vector<float*>> chalf_;   // real half-step states (adjoint_store)
+  std::vector<std::vector<float2*>> apool_;  // adjoint history p_[0..nt] when kept
+  bool keep_adj_ = false;
+  bool adj_valid_ = false;
   std::vector<float2**> hist_dev_;
   std::vector<float**> chalf_dev_;
   std::vector<PcgState*> st_;

@@ -243,6 +243,54 @@ class RDSolver:
                                          y.ctypes.data_as(ctypes.c_void_p)))
         return self._split(y)
 
+    # ---- time history (resizeTimeHistory, public c_ / p_ / c_half_) ------------------------
+    def set_time(self, dt, nt):
+        """TumorParameters dt_ / nt_ with PdeOperatorsRD::resizeTimeHistory: the history is
+        reallocated for nt; stored states are dropped."""
+        self._chk(self.L.glia_rd_set_time(self.h, float(dt), int(nt)))
+        self.dt, self.nt = float(dt), int(nt)
+
+    def keep_adjoint_history(self, enable=True):
+        """Keep p_[0..nt] during solveAdjoint (store-all mode only)."""
+        self._chk(self.L.glia_rd_keep_adjoint_history(self.h, int(bool(enable))))
+
+    def history(self, member, kind, index):
+        """c_[index] (kind "state"), c_half_[index] ("half") or p_[index] ("adjoint") of a member."""
+        k = {"state": 0, "half": 1, "adjoint": 2}[kind]
+        o = np.empty(self.ng, dtype=np.float32)
+        self._chk(self.L.glia_rd_get_history(self.h, int(member), k, int(index), o.ctypes.data_as(ctypes.c_void_p),
+                                             _lib.HOST))
+        return o.reshape(self.shape)
+
+    def set_secondary(self, member, kt):
+        """DiffCoef::setSecondaryCoefficients: k~ = kt (wm + k_gm gm + k_glm csf), for solveState(2)."""
+        self._chk(self.L.glia_rd_set_secondary(self.h, int(member), float(kt)))
+
+    def grad_diffusion(self):
+        """gradDiffusion of every member from the last adjoint sweep (g_kappa)."""
+        out = np.zeros(self.nmembers)
+        self._chk(self.L.glia_rd_grad_diffusion(self.h, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def grad_reaction(self):
+        """gradReaction of every member from the last adjoint sweep (g_rho)."""
+        out = np.zeros(self.nmembers)
+        self._chk(self.L.glia_rd_grad_reaction(self.h, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def check_gradient(self, x, mode=_lib.MODE_ALL, direction=None):
+        """DerivativeOperators::checkGradient: the Taylor test at x along direction (per-member
+        vectors like x; None: seeded uniform [0, 1) entries) for h = 1 .. 1e-5.  Returns per member
+        (|J(x + h d) - J(x) - h <dJ, d>|, (J(x + h d) - J(x)) / h - <dJ, d>), six values each."""
+        flat = self._flat_x(x)
+        d = None if direction is None else self._flat_x(direction)
+        err = np.zeros(6 * self.nmembers)
+        gd = np.zeros(6 * self.nmembers)
+        self._chk(self.L.glia_rd_check_gradient(self.h, int(mode), flat.ctypes.data_as(ctypes.c_void_p),
+                                                None if d is None else d.ctypes.data_as(ctypes.c_void_p),
+                                                err.ctypes.data_as(ctypes.c_void_p), gd.ctypes.data_as(ctypes.c_void_p)))
+        return err.reshape(self.nmembers, 6), gd.reshape(self.nmembers, 6)
+
     def stats(self, member=0):
         s = _lib.Stats()
         self._chk(self.L.glia_rd_stats(self.h, member, ctypes.byref(s)))
@@ -332,13 +380,45 @@ class DiffusionSolver:
         return self.s.get_field(0, self.field)
 
 
-class PdeOperatorsRD:
-    """PdeOperatorsRD (src/pde/PdeOperators.cpp): solveState / solveAdjoint on the handle's c0."""
+class _History:
+    """A read-only view of one history member (c_, c_half_ or p_) of member 0: h[i] is a field."""
 
-    def __init__(self, solver):
+    def __init__(self, solver, kind, length):
+        self.s, self.kind, self.length = solver, kind, length
+
+    def __len__(self):
+        return self.length()
+
+    def __getitem__(self, i):
+        n = self.length()
+        if i < 0:
+            i += n
+        if not 0 <= i < n:
+            raise IndexError(i)
+        return self.s.history(0, self.kind, i)
+
+
+class PdeOperatorsRD:
+    """PdeOperatorsRD (src/pde/PdeOperators.cpp): solveState / solveAdjoint on the handle's c0,
+    the public time history c_, p_, c_half_ (views; p_ kept in store-all mode), resizeTimeHistory
+    and reset."""
+
+    def __init__(self, solver, keep_adjoint=True):
         self.s = solver
         self.diff_ksp_itr_state_ = 0
         self.diff_ksp_itr_adj_ = 0
+        if keep_adjoint and solver.cfg.checkpoint_slots == 0:
+            solver.keep_adjoint_history(True)  # the reference always keeps p_
+        self.c_ = _History(solver, "state", lambda: self.s.nt + 1)
+        self.p_ = _History(solver, "adjoint", lambda: self.s.nt + 1)
+        self.c_half_ = _History(solver, "half", lambda: self.s.nt if self.s.cfg.adjoint_store else 0)
+
+    def resizeTimeHistory(self, dt, nt):
+        self.s.set_time(dt, nt)
+
+    def reset(self, dt, nt):
+        """PdeOperatorsRD::reset (src/pde/PdeOperators.cpp:6-15): resizeTimeHistory."""
+        self.resizeTimeHistory(dt, nt)
 
     def solveState(self, linearized=0):
         self.s.reset_stats()
@@ -372,3 +452,26 @@ class DerivativeOperatorsRD:
     def evaluateObjectiveAndGradient(self, x):
         J, dJ = self.s.objective_and_gradient(self._x(x), self.mode)
         return (J[0], dJ[0]) if self.s.nmembers == 1 else (J, dJ)
+
+    def gradDiffusion(self):
+        """g_kappa of the last adjoint sweep (src/grad/DerivativeOperators.cpp:189-256)."""
+        g = self.s.grad_diffusion()
+        return g[0] if self.s.nmembers == 1 else g
+
+    def gradReaction(self):
+        """g_rho of the last adjoint sweep (src/grad/DerivativeOperators.cpp:258-321)."""
+        g = self.s.grad_reaction()
+        return g[0] if self.s.nmembers == 1 else g
+
+    def checkGradient(self, x, direction=None):
+        """The Taylor test of src/grad/DerivativeOperators.cpp:369-450 (see RDSolver.check_gradient)."""
+        d = None if direction is None else self._x(direction)
+        err, gd = self.s.check_gradient(self._x(x), self.mode, d)
+        return (err[0], gd[0]) if self.s.nmembers == 1 else (err, gd)
+
+    def reset(self, solver=None, mode=None):
+        """DerivativeOperators::reset (src/grad/DerivativeOperators.cpp:168-186): adopt new operators."""
+        if solver is not None:
+            self.s = solver
+        if mode is not None:
+            self.mode = mode

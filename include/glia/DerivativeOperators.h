@@ -12,7 +12,10 @@
 #ifndef GLIA_DERIVATIVEOPERATORS_H_
 #define GLIA_DERIVATIVEOPERATORS_H_
 
+#include <cmath>
+#include <cstdio>
 #include <memory>
+#include <random>
 #include <vector>
 
 #include "PdeOperators.h"
@@ -30,6 +33,18 @@ class DerivativeOperators {
   virtual PetscErrorCode evaluateObjectiveAndGradient(PetscReal* J, Vec dJ, Vec x, std::shared_ptr<Data> data) = 0;
   virtual PetscErrorCode evaluateHessian(Vec y, Vec x) = 0;
   virtual PetscErrorCode updateReactionAndDiffusion(Vec x) = 0;
+  virtual PetscErrorCode gradDiffusion(Vec dJ) = 0;
+  virtual PetscErrorCode gradReaction(Vec dJ) = 0;
+  virtual PetscErrorCode checkGradient(Vec p, std::shared_ptr<Data> data) = 0;
+  /* reset (src/grad/DerivativeOperators.cpp:168-186): adopt the new operators, parameters, tumor */
+  virtual PetscErrorCode reset(Vec p, std::shared_ptr<PdeOperators> pde_operators, std::shared_ptr<Parameters> params,
+                               std::shared_ptr<Tumor> tumor) {
+    if (!p) return GLIA_RD_ERR_ARG;
+    pde_operators_ = pde_operators;
+    params_ = params;
+    tumor_ = tumor;
+    return 0;
+  }
 
   std::shared_ptr<PdeOperators> pde_operators_;
   std::shared_ptr<Parameters> params_;
@@ -87,8 +102,69 @@ class DerivativeOperatorsRD : public DerivativeOperators {
     return from_abi(ay, y);
   }
 
+  /* gradDiffusion (src/grad/DerivativeOperators.cpp:189-256): dJ[np] = g_kappa over the history of
+     the last solveState / solveAdjoint, when kappa is inverted */
+  PetscErrorCode gradDiffusion(Vec dJ) override {
+    if (rc_) return rc_;
+    const auto& opt = *params_->opt_;
+    if (!dJ || dJ->n <= params_->tu_->np_) return GLIA_RD_ERR_ARG;
+    if (!(opt.diffusivity_inversion_ || opt.flag_reaction_inv_)) return 0;
+    double gk = 0;
+    PetscErrorCode rc = glia_rd_grad_diffusion(h_, &gk);
+    if (rc || (rc = dJ->pull())) return rc;
+    dJ->v[params_->tu_->np_] = gk;
+    return dJ->push();
+  }
+  /* gradReaction (:258-321): dJ[np + nk] = g_rho when rho is inverted */
+  PetscErrorCode gradReaction(Vec dJ) override {
+    if (rc_) return rc_;
+    const auto& tu = *params_->tu_;
+    if (!dJ || dJ->n <= tu.np_ + tu.nk_) return GLIA_RD_ERR_ARG;
+    if (!params_->opt_->flag_reaction_inv_) return 0;
+    double gr = 0;
+    PetscErrorCode rc = glia_rd_grad_reaction(h_, &gr);
+    if (rc || (rc = dJ->pull())) return rc;
+    dJ->v[tu.np_ + tu.nk_] = gr;
+    return dJ->push();
+  }
+  /* checkGradient (:369-450): the Taylor test at p along a random direction with entries uniform
+     in [0, 1) (seeded), h = 10^0 .. 10^-5; prints the reference's lines and keeps
+     |J - J_taylor| in taylor_err_ and (J - J_p) / h - <dJ, p~> in taylor_gdiff_ */
+  PetscErrorCode checkGradient(Vec p, std::shared_ptr<Data> data) override {
+    if (rc_) return rc_;
+    PetscErrorCode rc = check_layout(p);
+    if (rc) return rc;
+    std::vector<double> a;
+    if ((rc = to_abi(p, a)) || (rc = place_data(data))) return rc;
+    // the direction over the inverted entries of the reference's x, mapped to the ABI's layout
+    const auto& tu = *params_->tu_;
+    const auto& opt = *params_->opt_;
+    const int np = abi_np();
+    std::mt19937_64 gen(20260101);
+    std::uniform_real_distribution<double> u(0.0, 1.0);
+    std::vector<double> ref(p->n), d(a.size(), 0.0);
+    for (double& e : ref) e = u(gen);
+    if (mode() == GLIA_RD_MODE_P_KAPPA)
+      for (int i = 0; i < np && i < tu.np_; i++) d[i] = ref[i];
+    if ((opt.diffusivity_inversion_ || opt.flag_reaction_inv_) && p->n > tu.np_) d[np] = ref[tu.np_];
+    if (opt.flag_reaction_inv_ && p->n > tu.np_ + tu.nk_) d[np + 1] = ref[tu.np_ + tu.nk_];
+    double norm = 0;
+    for (double e : p->v) norm += e * e;
+    std::printf(" ----- Gradient check with taylor expansion ----- \nGradient check performed at x with norm: %g\n",
+                std::sqrt(norm));
+    taylor_err_.assign(6, 0.0);
+    taylor_gdiff_.assign(6, 0.0);
+    if ((rc = glia_rd_check_gradient(h_, mode(), a.data(), d.data(), taylor_err_.data(), taylor_gdiff_.data())))
+      return rc;
+    for (int i = 0; i < 6; i++)
+      std::printf("h[i]: %g |J - J_taylor|: %g  log10(diff) : %g g_fd - xg_dot: %g\n", std::pow(10.0, -i),
+                  taylor_err_[i], std::log10(taylor_err_[i]), taylor_gdiff_[i]);
+    return 0;
+  }
+
   glia_rd_t h_ = nullptr;
   PetscErrorCode rc_ = 0;
+  std::vector<double> taylor_err_, taylor_gdiff_;
 
  private:
   int abi_np() const { return glia_rd_basis_size(h_, 0); }

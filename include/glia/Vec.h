@@ -35,6 +35,7 @@ struct _p_Vec {
   std::vector<ScalarType> v;   // allocated on first use
   glia_rd_t h = nullptr;        // bound field: handle, member, field id
   int member = 0, which = -1;
+  int hist_kind = -1, hist_index = 0;  // bound history entry (GLIA_RD_HIST_*), read-only
   std::vector<float> stage;
 
   void ensure() {
@@ -44,13 +45,15 @@ struct _p_Vec {
     ensure();
     if (!h) return 0;
     stage.resize(n);
-    const int rc = glia_rd_get_field(h, member, which, stage.data(), GLIA_RD_HOST);
+    const int rc = (hist_kind >= 0)
+                       ? glia_rd_get_history(h, member, hist_kind, hist_index, stage.data(), GLIA_RD_HOST)
+                       : glia_rd_get_field(h, member, which, stage.data(), GLIA_RD_HOST);
     if (rc) return rc;
     std::copy(stage.begin(), stage.end(), v.begin());
     return 0;
   }
   PetscErrorCode push() {
-    if (!h) return 0;
+    if (!h || hist_kind >= 0) return 0;  // history views are read-only: writes stay on the host
     ensure();
     stage.assign(v.begin(), v.end());
     return glia_rd_set_field(h, member, which, stage.data(), GLIA_RD_HOST);
@@ -72,6 +75,17 @@ inline PetscErrorCode VecCreateField(glia_rd_t h, int member, int which, int64_t
   (*x)->h = h;
   (*x)->member = member;
   (*x)->which = which;
+  return 0;
+}
+/* a read-only view of history entry 'index' of kind GLIA_RD_HIST_* (c_, c_half_, p_) of a member */
+inline PetscErrorCode VecCreateHistory(glia_rd_t h, int member, int kind, int index, int64_t n, Vec* x) {
+  if (!x || !h || n < 0) return GLIA_RD_ERR_ARG;
+  *x = new _p_Vec();
+  (*x)->n = n;
+  (*x)->h = h;
+  (*x)->member = member;
+  (*x)->hist_kind = kind;
+  (*x)->hist_index = index;
   return 0;
 }
 inline PetscErrorCode VecDuplicate(Vec x, Vec* y) {

@@ -165,12 +165,52 @@ int glia_rd_apply_d(glia_rd_t h, int which_in, int which_out);
 int glia_rd_fft_r2c(glia_rd_t h, int member, const float* f, float* fhat);
 int glia_rd_fft_c2r(glia_rd_t h, int member, const float* fhat, float* f);
 
-/* PdeOperatorsRD::solveState (src/pde/PdeOperators.cpp:235-316) from c0, keeping the history the
-   checkpoint policy asks for; linearized must be 0 */
+/* PdeOperatorsRD::solveState (src/pde/PdeOperators.cpp:235-316).
+   linearized 0: the state equation from c0, keeping the history the checkpoint policy asks for.
+   linearized 1: the tangent-linear equation from c~0 (the c0 field) around the state history of
+     the last solveState(0), reaction(1, i) linearized at c_[i]; c~(T) in the ct field.
+   linearized 2: as 1, with the solveIncremental sources (:192-233) of the secondary coefficients
+     set by glia_rd_set_secondary (the Gauss-Newton Hessian with diffusivity inversion).
+   1 and 2 leave the state history as it is. */
 int glia_rd_solve_state(glia_rd_t h, int linearized);
-/* PdeOperatorsRD::solveAdjoint (src/pde/PdeOperators.cpp:372-420) with the final condition
-   -O^T(O c(T) - d1) of DerivativeOperatorsRD (src/grad/DerivativeOperatorsRD.cpp:154-162) */
+/* PdeOperatorsRD::solveAdjoint (src/pde/PdeOperators.cpp:372-420), accumulating the quadratures of
+   gradDiffusion / gradReaction along the sweep; alpha(0) in the alpha0 field.
+   linearized 0 or 1: from -O^T(O c(T) - d1) of DerivativeOperatorsRD
+     (src/grad/DerivativeOperatorsRD.cpp:154-162) and the state history.
+   linearized 2: the incremental adjoint from -O^T O c~(T), c~(T) = the ct field after
+     solveState(1|2) (evaluateHessian, :404-420); the i = nt quadrature term uses the last
+     gradient's final condition, as the reference's p_[nt] does. */
 int glia_rd_solve_adjoint(glia_rd_t h, int linearized);
+/* TumorParameters dt_ / nt_ with PdeOperatorsRD::resizeTimeHistory (src/pde/PdeOperators.cpp:56-103,
+   called through PdeOperatorsRD::reset, :6-15): the history is reallocated for the new nt and the
+   stored states dropped; the diffusion solver's context dt is unchanged. */
+int glia_rd_set_time(glia_rd_t h, double dt, int nt);
+/* Keep the adjoint history p_[0..nt] during solveAdjoint (enable = 1; store-all mode only: nt + 1
+   more spectral fields per member) or release it (0).  The reference always keeps it
+   (src/pde/PdeOperators.cpp:26-44); the gradient evaluation here does not need it. */
+int glia_rd_keep_adjoint_history(glia_rd_t h, int enable);
+/* The reference's public history members of PdeOperators (include/pde/PdeOperators.h:24-29) as a
+   real field of one member: kind GLIA_RD_HIST_STATE c_[index] (0..nt; under checkpointing only
+   the stored indices), GLIA_RD_HIST_HALF c_half_[index] (0..nt-1, adjoint_store in store-all
+   mode), GLIA_RD_HIST_ADJOINT p_[index] (0..nt, when kept).  GLIA_RD_ERR_NOHISTORY when the entry
+   is not held.  Overwrites the work field. */
+enum { GLIA_RD_HIST_STATE = 0, GLIA_RD_HIST_HALF = 1, GLIA_RD_HIST_ADJOINT = 2 };
+int glia_rd_get_history(glia_rd_t h, int member, int kind, int index, float* dst, int mem_kind);
+/* DiffCoef::setSecondaryCoefficients (src/mat/DiffCoef.cpp:134-168) for solveState(2):
+   k~ = kt (m_wm + k_gm_wm m_gm + k_glm_wm m_csf) of one member */
+int glia_rd_set_secondary(glia_rd_t h, int member, double kt);
+/* DerivativeOperators::gradDiffusion / gradReaction (src/grad/DerivativeOperators.cpp:189-321),
+   nk = nr = 1: g_kappa = h^3 sum_i w_i dt <m_wm + k_gm_wm m_gm, grad c_i . grad alpha_i> and
+   g_rho = h^3 sum_i w_i dt <m_wm + r_gm_wm m_gm, (c_i^2 - c_i) alpha_i> over the history of the
+   last solveState(0) / solveAdjoint; one value per member */
+int glia_rd_grad_diffusion(glia_rd_t h, double* gk);
+int glia_rd_grad_reaction(glia_rd_t h, double* gr);
+/* DerivativeOperators::checkGradient (src/grad/DerivativeOperators.cpp:369-450): the Taylor test
+   at x along dir (laid out like x; NULL: entries uniform in [0, 1) from a fixed seed, as the
+   reference's VecSetRandom draws them): with J0, dJ the objective and gradient at x and
+   J_i = J(x + h_i dir), h_i = 10^-i for i = 0..5, err[6 m + i] = |J_i - J0 - h_i <dJ, dir>| and
+   gdiff[6 m + i] = (J_i - J0) / h_i - <dJ, dir> for member m (either output may be NULL). */
+int glia_rd_check_gradient(glia_rd_t h, int mode, const double* x, const double* dir, double* err, double* gdiff);
 
 /* DerivativeOperatorsRD::evaluateObjective / evaluateGradient / evaluateObjectiveAndGradient
    (src/grad/DerivativeOperatorsRD.cpp:7-226).  x holds [p (np) | kappa | rho] per member,

@@ -422,6 +422,100 @@ static void test_interface_matches_abi_and_fd() {
   VecDestroy(&s.d);
 }
 
+// ---- the rest of the PdeOperators / DerivativeOperators surface -------------------------------
+// gradDiffusion after an evaluation; the public history c_, p_ (views of the device history);
+// solveState(1) against a finite difference of solveState(0); resizeTimeHistory through reset;
+// checkGradient's Taylor remainders (src/grad/DerivativeOperators.cpp:369-450)
+static void test_boundary_methods() {
+  std::printf("TEST PdeOperators / DerivativeOperators boundary methods\n");
+  TilSetup s;
+  make_til(s);
+  const int n = NP + 2, nt = s.params->tu_->nt_;
+  Vec x, g, g2;
+  VecCreateSeq(PETSC_COMM_SELF, n, &x);
+  VecCreateSeq(PETSC_COMM_SELF, n, &g);
+  VecCreateSeq(PETSC_COMM_SELF, n, &g2);
+  const double x0[n] = {0.6, 0.6, 0.2, 0.4, 0.05, 8.0};
+  double* xp;
+  VecGetArray(x, &xp);
+  for (int i = 0; i < n; i++) xp[i] = x0[i];
+  VecRestoreArray(x, &xp);
+  double J = 0;
+  REQUIRE(s.der->evaluateObjectiveAndGradient(&J, g, x, s.data) == 0);
+  VecSet(g2, 0.0);
+  REQUIRE(s.der->gradDiffusion(g2) == 0);
+  double *gp, *g2p;
+  VecGetArray(g, &gp);
+  VecGetArray(g2, &g2p);
+  std::printf("  gradDiffusion %.9e, evaluation g_kappa %.9e\n", g2p[NP], gp[NP]);
+  REQUIRE(g2p[NP] == gp[NP]);
+  VecRestoreArray(g, &gp);
+  VecRestoreArray(g2, &g2p);
+  REQUIRE(s.der->gradReaction(g2) == 0);  // no rho inversion here: dJ is left as it is
+  // history views: c_[nt] is c(T), p_[0] is alpha(0)
+  REQUIRE((int)s.pde->c_.size() == nt + 1 && (int)s.pde->p_.size() == nt + 1);
+  double a = 0, b = 0;
+  VecNorm(s.pde->c_[nt], NORM_2, &a);
+  VecNorm(s.tumor->c_t_, NORM_2, &b);
+  REQUIRE(a == b && a > 0);
+  VecNorm(s.pde->p_[0], NORM_2, &a);
+  VecNorm(s.tumor->p_0_, NORM_2, &b);
+  REQUIRE(a == b && a > 0);
+  VecNorm(s.pde->c_[0], NORM_2, &a);
+  VecNorm(s.tumor->c_0_, NORM_2, &b);
+  REQUIRE(std::fabs(a - b) <= 1e-6 * b);
+  // solveState(1) from c~0 = c(0) against (c(T; (1 + e) c0) - c(T; c0)) / e
+  const int64_t ng = s.params->grid_->ng_;
+  Vec c0, ct_base, ct_lin;
+  VecDuplicate(s.tumor->c_0_, &c0);
+  VecDuplicate(s.tumor->c_0_, &ct_base);
+  VecDuplicate(s.tumor->c_0_, &ct_lin);
+  VecCopy(s.tumor->c_0_, c0);
+  VecCopy(s.tumor->c_t_, ct_base);
+  REQUIRE(s.pde->solveState(1) == 0);
+  VecCopy(s.tumor->c_t_, ct_lin);
+  const double e = 1e-2;
+  VecScale(c0, 1.0 + e);
+  VecCopy(c0, s.tumor->c_0_);
+  REQUIRE(s.pde->solveState(0) == 0);
+  double *pl, *pb, *pp;
+  VecGetArray(ct_lin, &pl);
+  VecGetArray(ct_base, &pb);
+  VecGetArray(s.tumor->c_t_, &pp);
+  double err = 0, nrm = 0;
+  for (int64_t i = 0; i < ng; i++) {
+    const double fd = (pp[i] - pb[i]) / e;
+    err += (fd - pl[i]) * (fd - pl[i]);
+    nrm += pl[i] * pl[i];
+  }
+  std::printf("  solveState(1) vs finite difference of solveState(0): rel %.2e\n", std::sqrt(err / nrm));
+  REQUIRE(std::sqrt(err / nrm) < 2e-2);
+  VecDestroy(&c0);
+  VecDestroy(&ct_base);
+  VecDestroy(&ct_lin);
+  // checkGradient: Taylor remainders fall for h = 1, 0.1, 0.01
+  REQUIRE(s.der->checkGradient(x, s.data) == 0);
+  auto* dro = dynamic_cast<DerivativeOperatorsRD*>(s.der.get());
+  REQUIRE(dro && dro->taylor_err_.size() == 6);
+  if (dro && dro->taylor_err_.size() == 6)
+    REQUIRE(dro->taylor_err_[1] < 0.2 * dro->taylor_err_[0] && dro->taylor_err_[2] < 0.2 * dro->taylor_err_[1]);
+  // resizeTimeHistory through reset: the views follow the new nt and evaluations run
+  s.params->tu_->nt_ = 8;
+  REQUIRE(s.pde->reset(s.params, s.tumor) == 0);
+  REQUIRE((int)s.pde->c_.size() == 9);
+  REQUIRE(s.der->reset(x, s.pde, s.params, s.tumor) == 0);
+  double J8 = 0;
+  REQUIRE(s.der->evaluateObjectiveAndGradient(&J8, g, x, s.data) == 0);
+  VecNorm(s.pde->c_[8], NORM_2, &a);
+  VecNorm(s.tumor->c_t_, NORM_2, &b);
+  std::printf("  nt 5 -> 8: J %.6e -> %.6e\n", J, J8);
+  REQUIRE(a == b && J8 != J);
+  VecDestroy(&x);
+  VecDestroy(&g);
+  VecDestroy(&g2);
+  VecDestroy(&s.d);
+}
+
 // ---- Phi apply / transpose and the diffusion operator through the interface -------------------
 static void test_phi_and_apply_d() {
   std::printf("TEST Phi and DiffCoef::applyD through the interface\n");
@@ -626,6 +720,7 @@ int main() {
   test_elasticity_solver();
   test_objective();
   test_interface_matches_abi_and_fd();
+  test_boundary_methods();
   test_phi_and_apply_d();
   test_til_optimizer(QUASINEWTON);
   test_til_optimizer(GAUSSNEWTON);

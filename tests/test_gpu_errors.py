@@ -42,8 +42,11 @@ def test_misuse_returns_typed_errors_and_the_handle_survives():
     err(L.glia_rd_get_field(h, 0, 99, buf.ctypes.data_as(vp), _lib.HOST), _lib.ERR_ARG, "field")
     err(L.glia_rd_get_field(h, 5, _lib.FIELD["ct"], buf.ctypes.data_as(vp), _lib.HOST), _lib.ERR_ARG)
     err(L.glia_rd_objective(h, 3, x.ctypes.data_as(vp), J.ctypes.data_as(vp)), _lib.ERR_ARG)
-    err(L.glia_rd_solve_state(h, 1), _lib.ERR_UNSUPPORTED)
-    err(L.glia_rd_solve_adjoint(h, 2), _lib.ERR_UNSUPPORTED)
+    err(L.glia_rd_solve_state(h, 3), _lib.ERR_ARG, "linearized")
+    err(L.glia_rd_solve_adjoint(h, 3), _lib.ERR_ARG, "linearized")
+    # the tangent-linear equations linearize around a state history
+    err(L.glia_rd_solve_state(h, 1), _lib.ERR_NOHISTORY)
+    err(L.glia_rd_solve_adjoint(h, 2), _lib.ERR_NOHISTORY)
     centers = np.ascontiguousarray(pr["centers"]).reshape(-1)
     err(L.glia_rd_set_basis(h, 0, 4, centers.ctypes.data_as(vp), -1.0), _lib.ERR_ARG)
     err(L.glia_rd_set_beta(h, -1.0), _lib.ERR_ARG, "beta")
