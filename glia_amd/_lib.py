@@ -99,6 +99,7 @@ SIGNATURES = {
     "glia_rd_stats": (_i, [_vp, _i, ctypes.POINTER(Stats)]),
     "glia_rd_reset_stats": (_i, [_vp]),
     "glia_rd_smooth_field": (_i, [_vp, _i, _i, _d]),
+    "glia_rd_smooth_noalloc": (_i, [ctypes.POINTER(ctypes.c_int), _vp, _vp, _d, _vp, ctypes.c_char_p, _i]),
     "glia_rd_set_time": (_i, [_vp, _d, _i]),
     "glia_rd_keep_adjoint_history": (_i, [_vp, _i]),
     "glia_rd_get_history": (_i, [_vp, _i, _i, _i, _vp, _i]),

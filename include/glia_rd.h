@@ -139,6 +139,17 @@ int glia_rd_set_ratios(glia_rd_t h, double k_gm_wm, double k_glm_wm, double r_gm
    tissue with sigma = smoothing_factor * 2 pi / n0, SolverInterface::readAtlas,
    src/SolverInterface.cpp:502-575) */
 int glia_rd_smooth_field(glia_rd_t h, int member, int which, double sigma);
+/* The smoother of glia_rd_smooth_field on caller-owned device fields, allocating nothing.  It has
+   no counterpart in the reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops).  in and out are distinct device fields of
+   n[0]*n[1]*n[2] floats (any lengths >= 4, at most 2^23 = 8,388,608 points); sigma > 0 with a
+   smoother radius (ceil(12 sigma / h) per axis) of at most 127 points; stream may be NULL (the
+   null stream).  The taps and the intermediate field live in one 32 MiB device array per GPU
+   that every call on that GPU shares: calls that overlap on one GPU (other streams, other host
+   threads) race.  Returns 0, GLIA_RD_ERR_ARG, GLIA_RD_ERR_UNSUPPORTED (grid or radius above the
+   limits) or a hipError_t; err (may be NULL) receives the message. */
+int glia_rd_smooth_noalloc(const int n[3], const float* in, float* out, double sigma, void* stream, char* err,
+                           int err_len);
 /* MatProp::setValuesCustom on the tissue fields already stored (e.g. after glia_rd_set_field and
    glia_rd_smooth_field): mask, k, rho, preconditioner and Phi of the member */
 int glia_rd_refresh_tissue(glia_rd_t h, int member);
