@@ -23,15 +23,17 @@ struct glia_me_s {
   int n[3] = {0, 0, 0};
   int64_t ng = 0;
   // u_old (3), force (3), advection outputs (6: gm, wm, vt, csf and mri or p, nec), the tumor core
-  // of the multispecies model (1), then the |v| partials
+  // of the multispecies model (1), its edema and segmentation outputs (2), then the |v| partials
   float* work = nullptr;
+  bool have_ms_outputs = false;  // edema and segmentation of a multispecies solve are in work
   std::vector<float> hpart;
   std::string err;
 };
 
 namespace {
 
-constexpr int kWorkFields = 13;
+constexpr int kWorkFields = 15;
+constexpr int kEdema = 13, kSeg = 14;  // work fields of the multispecies outputs
 
 int fail(glia_me_s* h, int rc, const std::string& msg) {
   h->err = msg;
@@ -239,6 +241,8 @@ int glia_me_solve_state_multispecies(glia_me_t h, const glia_me_params* p, const
   float* adv[6];
   for (int k = 0; k < 6; k++) adv[k] = h->work + (6 + k) * ng;
   float* core = h->work + 12 * ng;
+  float* edema = h->work + kEdema * ng;
+  float* seg = h->work + kSeg * ng;
   float* part = h->work + kWorkFields * ng;
   float* ib = e->field_ptr(0, glia::RS_CT);  // i through advection, clipping, smoothing and diffusion
   float* rho = e->field_ptr(0, glia::RS_RHO);
@@ -269,8 +273,16 @@ int glia_me_solve_state_multispecies(glia_me_t h, const glia_me_params* p, const
     for (int d = 0; d < 3; d++) ME_HIP(h, hipMemcpyAsync(uold[d], u[d], fb, hipMemcpyDeviceToDevice, st));
   }
 
+  h->have_ms_outputs = false;
   for (int i = 0; i < nt + 1; i++) {
-    // CFL from the current velocity (:306-321)
+    // Tumor::computeEdema and computeSegmentation at the top of the step (:307-308): the edema
+    // mask of i smoothed with sigma 2 pi / n0 (src/mat/Tumor.cpp:160-178) and the labels of the
+    // current tissue and tumor c_t_ (p at the first step, then p + nec + i)
+    ME_SUB(h, glia_ms_edema_mask(ng, infil, ms->invasive_threshold, edema, (void*)st), "edema mask");
+    ME_SUB(h, e->smooth_ptr(edema, ws, sigma), e->error());
+    ME_SUB(h, glia_ms_segmentation(ng, bg, gm, wm, vt, csf, i == 0 ? prolif : core, seg, (void*)st), "segmentation");
+    h->have_ms_outputs = true;
+    // CFL from the current velocity (:309-321)
     ME_HIP(h, (hipError_t)glia::me::launch_vmax(st, ng, v[0], v[1], v[2], part));
     ME_HIP(h, hipMemcpyAsync(h->hpart.data(), part, sizeof(float) * np, hipMemcpyDeviceToHost, st));
     ME_HIP(h, hipStreamSynchronize(st));
@@ -334,6 +346,20 @@ int glia_me_solve_state_multispecies(glia_me_t h, const glia_me_params* p, const
   ME_SUB(h, e->stats(0, &s1), e->error());
   inf.diff_iterations = s1.its_total - s0.its_total;
   if (info) *info = inf;
+  return 0;
+}
+
+int glia_me_multispecies_outputs(glia_me_t h, float* edema, float* seg, int mem_kind) {
+  if (!h) return GLIA_RD_ERR_ARG;
+  if (!edema && !seg) return fail(h, GLIA_RD_ERR_ARG, "no output requested");
+  if (mem_kind != GLIA_RD_HOST && mem_kind != GLIA_RD_DEVICE) return fail(h, GLIA_RD_ERR_ARG, "bad mem_kind");
+  if (!complete(h) || !h->have_ms_outputs)
+    return fail(h, GLIA_RD_ERR_STATE, "no multispecies solve has computed the edema and segmentation");
+  const hipMemcpyKind k = mem_kind == GLIA_RD_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  const size_t fb = sizeof(float) * h->ng;
+  if (edema) ME_HIP(h, hipMemcpyAsync(edema, h->work + kEdema * h->ng, fb, k, h->st));
+  if (seg) ME_HIP(h, hipMemcpyAsync(seg, h->work + kSeg * h->ng, fb, k, h->st));
+  ME_HIP(h, hipStreamSynchronize(h->st));
   return 0;
 }
 

@@ -105,6 +105,7 @@ void glia_ms_params_init(glia_ms_params* p) {
   p->beta_0 = 0.01;
   p->ox_inv = 0.8;
   p->ox_hypoxia = 0.6;
+  p->invasive_threshold = 0.001;
 }
 
 int glia_ms_sources(int64_t n, float* p, float* i, float* nec, float* ox, float* gm, float* wm, const float* rho,

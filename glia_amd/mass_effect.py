@@ -110,7 +110,8 @@ class MassEffectSolver:
         """PdeOperatorsMultiSpecies::solveState(0) from c0 and the tissue (max_cfl None: the
         reference's limit 8); ms overrides glia_ms_params
         fields (death_rate, ox_source, ...), el the glia_el_params fields.  Returns a dict of p, i,
-        nec, ox, wm, gm, vt, csf, u, v and info."""
+        nec, ox, wm, gm, vt, csf, u, v, edema and seg (Tumor::species_["edema"] and Tumor::seg_ as
+        the loop leaves them) and info."""
         import torch
 
         p = _lib.MeParams()
@@ -150,6 +151,14 @@ class MassEffectSolver:
         if rc != 0:
             raise GliaError(rc, self.L.glia_me_last_error(self.h).decode())
         out = {k: f[k].cpu().numpy().reshape(self.shape) for k in ("p", "i", "nec", "ox", "wm", "gm", "vt", "csf")}
+        ed = np.empty(self.ng, np.float32)
+        seg = np.empty(self.ng, np.float32)
+        rc = self.L.glia_me_multispecies_outputs(self.h, ed.ctypes.data_as(ctypes.c_void_p),
+                                                 seg.ctypes.data_as(ctypes.c_void_p), _lib.HOST)
+        if rc != 0:
+            raise GliaError(rc, self.L.glia_me_last_error(self.h).decode())
+        out["edema"] = ed.reshape(self.shape)
+        out["seg"] = seg.reshape(self.shape)
         out["u"] = np.stack([t.cpu().numpy() for t in uu]).reshape((3,) + self.shape)
         out["v"] = np.stack([t.cpu().numpy() for t in vv]).reshape((3,) + self.shape)
         out["info"] = {"steps": info.steps, "cfl_stop": bool(info.cfl_stop), "cfl_max": info.cfl_max,

@@ -79,8 +79,8 @@ int glia_me_solve_state(glia_me_t h, const glia_me_params* p, float* c, float* w
                         float* mri, const float* bg, float* const u[3], float* const v[3], glia_me_info* info);
 
 /* PdeOperatorsMultiSpecies::solveState(0) (src/pde/PdeOperatorsMultiSpecies.cpp:235-508), the
-   multispecies forward of model 5, without its output, edema and segmentation branches (they feed
-   only the output).  The reference's CFL limit for this model is 8 (p->max_cfl <= 0 selects it).
+   multispecies forward of model 5 without its file output branch; the edema and segmentation it
+   computes at every step are read with glia_me_multispecies_outputs.  The reference's CFL limit for this model is 8 (p->max_cfl <= 0 selects it).
    c0: Tumor::c_0_ (in); prolif, infil, ox: the proliferative, infiltrative and oxygen species
    (out; set to c0 clipped at zero, 0 and 1 at the start); nec: the necrotic species (in and out,
    not reset, as in the reference); wm, gm, vt, csf: the tissue (in, filtered and advected out);
@@ -98,6 +98,12 @@ int glia_me_solve_state_multispecies(glia_me_t h, const glia_me_params* p, const
                                      float* prolif, float* infil, float* nec, float* ox, float* wm, float* gm, float* vt,
                                      float* csf, const float* bg, float* const u[3], float* const v[3],
                                      glia_me_info* info);
+/* The outputs the multispecies loop computes at the top of every step (:307-308), as they stand
+   after the last solve: Tumor::species_["edema"] (computeEdema, src/mat/Tumor.cpp:160-178: the mask
+   1[i > ms->invasive_threshold] smoothed with sigma 2 pi / n0) and Tumor::seg_
+   (computeSegmentation, labels of glia_ms_segmentation with c_t_ = p at the first step, then
+   p + nec + i).  Either output may be NULL; GLIA_RD_ERR_STATE before a multispecies solve. */
+int glia_me_multispecies_outputs(glia_me_t h, float* edema, float* seg, int mem_kind);
 
 #ifdef __cplusplus
 }

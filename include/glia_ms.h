@@ -27,6 +27,7 @@ typedef struct glia_ms_params {
   double beta_0;         /* 0.01: infiltrative -> proliferative */
   double ox_inv;         /* 0.8 */
   double ox_hypoxia;     /* 0.6 */
+  double invasive_threshold; /* 0.001: TumorParameters::invasive_threshold_, the edema mask level */
 } glia_ms_params;
 
 void glia_ms_params_init(glia_ms_params* p);

@@ -1542,9 +1542,14 @@ static void clip0(int64_t n, double *x) {
    elasticity solve, v = S((u - u_old) / dt), u_old = u.
    nec: in and out (the reference does not reset it); ox out; ms_prm: death_rate, ox_source,
    ox_consumption, alpha_0, beta_0, ox_inv, ox_hypoxia.  info as for orc_me_solve_state. */
+/* ed and seg (may be NULL) receive Tumor::species_["edema"] and Tumor::seg_ as the loop leaves them:
+   computeEdema (src/mat/Tumor.cpp:160-178: 1[i > ms_prm[7]] smoothed with sigma 2 pi / n0) and
+   computeSegmentation at the top of every step (src/pde/PdeOperatorsMultiSpecies.cpp:307-308), with
+   c_t_ = p at the first step and p + nec + i afterwards. */
 int orc_ms_solve_state(orc_t *o, const double *c0, double *p, double *iv, double *nec, double *ox, double *wm,
                        double *gm, double *vt, double *csf, const double *bg, double *u, double *v, double ff,
-                       int use_tanh, const double *el_prm, const double *ms_prm, double max_cfl, double *info) {
+                       int use_tanh, const double *el_prm, const double *ms_prm, double max_cfl, double *info,
+                       double *ed, double *seg) {
   const int64_t ng = o->ng;
   const double dt = o->dt, h0 = 2.0 * M_PI / o->n[0], sigma = 2.0 * M_PI / o->n[0];
   double *uold = dalloc(3 * ng), *f = dalloc(3 * ng), *q = dalloc(3 * ng), *tmp = dalloc(ng);
@@ -1570,6 +1575,11 @@ int orc_ms_solve_state(orc_t *o, const double *c0, double *p, double *iv, double
     memcpy(uold, u, sizeof(double) * 3 * ng);
   }
   for (int it = 0; it < o->nt + 1; it++) {
+    if (ed) {
+      for (int64_t i = 0; i < ng; i++) tmp[i] = (iv[i] > ms_prm[7]) ? 1.0 : 0.0;
+      orc_smooth(o, tmp, ed, sigma);
+    }
+    if (seg) orc_ms_segmentation(ng, bg, gm, wm, vt, csf, it == 0 ? p : ct, seg);
     double vmax = 0;
     for (int64_t i = 0; i < ng; i++) vmax = fmax(vmax, sqrt(vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]));
     const double cfl = dt * vmax / h0;

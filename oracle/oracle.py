@@ -58,7 +58,7 @@ def _load():
         "orc_sl_solve": (None, [vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, d, i]),
         "orc_smooth": (None, [vp, _dp, _dp, d]),
         "orc_me_solve_state": (i, [vp, _dp, _dp, _dp, _dp, _dp, vp, _dp, _dp, _dp, d, i, _dp, d, _dp]),
-        "orc_ms_solve_state": (i, [vp] + [_dp] * 12 + [d, i, _dp, _dp, d, _dp]),
+        "orc_ms_solve_state": (i, [vp] + [_dp] * 12 + [d, i, _dp, _dp, d, _dp, _dp, _dp]),
         "orc_set_tissue": (None, [vp, vp, vp, vp, vp, vp]),
         "orc_set_tissue_sinusoidal": (None, [vp]),
         "orc_set_k": (None, [vp, d, d, d]),
@@ -266,7 +266,7 @@ class Oracle:
     def ms_solve_state(self, c0, wm, gm, vt, csf, bg, nec=None, u=None, v=None, ff=1e5, use_tanh=True,
                        el_prm=None, ms_prm=None, max_cfl=8.0):
         """PdeOperatorsMultiSpecies::solveState(0): the multispecies mass-effect forward of model 5
-        over nt + 1 steps.  Returns a dict of p, i, nec, ox, wm, gm, vt, csf, u, v and info."""
+        over nt + 1 steps.  Returns a dict of p, i, nec, ox, wm, gm, vt, csf, u, v, edema, seg and info."""
         f = {k: np.array(a, dtype=np.float64, order="C").reshape(-1).copy()
              for k, a in (("wm", wm), ("gm", gm), ("vt", vt), ("csf", csf))}
         for k in ("p", "i", "ox"):
@@ -275,12 +275,18 @@ class Oracle:
         uu = np.zeros(3 * self.ng) if u is None else self._vec(u).copy()
         vv = np.zeros(3 * self.ng) if v is None else self._vec(v).copy()
         prm = np.array(self.EL_DEFAULTS if el_prm is None else el_prm, dtype=np.float64)
-        mprm = np.array(self.MS_DEFAULTS if ms_prm is None else ms_prm, dtype=np.float64)
+        mprm = list(self.MS_DEFAULTS if ms_prm is None else ms_prm)
+        if len(mprm) == 7:
+            mprm.append(0.001)  # TumorParameters::invasive_threshold_
+        mprm = np.array(mprm, dtype=np.float64)
         info = np.zeros(5)
+        ed, seg = np.zeros(self.ng), np.zeros(self.ng)
         self.L.orc_ms_solve_state(self.h, self._f(c0), f["p"], f["i"], f["nec"], f["ox"], f["wm"], f["gm"], f["vt"],
                                   f["csf"], self._f(bg), uu, vv, float(ff), int(bool(use_tanh)), prm, mprm,
-                                  float(max_cfl), info)
+                                  float(max_cfl), info, ed, seg)
         out = {k: a.reshape(self.shape) for k, a in f.items()}
+        out["edema"] = ed.reshape(self.shape)
+        out["seg"] = seg.reshape(self.shape)
         out["u"] = uu.reshape((3,) + self.shape)
         out["v"] = vv.reshape((3,) + self.shape)
         out["info"] = {"steps": int(info[0]), "cfl_stop": bool(info[1]), "cfl_max": float(info[2]),

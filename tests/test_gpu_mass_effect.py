@@ -60,11 +60,13 @@ def test_multispecies_forward_matches_the_oracle(oracle_mod):
     o.set_k(k, 0.2)
     o.set_rho(rho, 0.2)
     o.set_time(dt, nt)
-    ref = o.ms_solve_state(c0, wm, gm, vt, csf, bg, ff=ff)
+    # the edema mask level (invasive_threshold_, 1e-3 by default) is set to 2e-4, well inside the
+    # range of i here (up to about 1e-3), so that the mask has a body of voxels to compare
+    ref = o.ms_solve_state(c0, wm, gm, vt, csf, bg, ff=ff, ms_prm=(4.0, 1.0, 5.0, 0.1, 0.01, 0.8, 0.6, 2e-4))
     s = MassEffectSolver(n, dt, nt, k=k, rho=rho, k_gm_wm=0.2, r_gm_wm=0.2)
     try:
         s.set_tissue(wm, gm, csf, vt)
-        out = s.solve_state_multispecies(c0, wm, gm, vt, csf, bg, forcing_factor=ff)
+        out = s.solve_state_multispecies(c0, wm, gm, vt, csf, bg, forcing_factor=ff, ms={"invasive_threshold": 2e-4})
     finally:
         s.close()
     errs = {key: rel(out[key], ref[key]) for key in ("p", "i", "nec", "ox", "wm", "gm", "vt", "csf", "u", "v")}
@@ -75,3 +77,10 @@ def test_multispecies_forward_matches_the_oracle(oracle_mod):
         assert errs[key] < 1e-5, key
     for key in ("i", "nec", "u", "v"):
         assert errs[key] < 1e-4, key
+    # the edema (smoothed mask of i) and segmentation outputs of the loop's last step
+    e_ed = rel(out["edema"], ref["edema"])
+    seg_diff = int(np.sum(out["seg"] != ref["seg"]))
+    print("edema rel %.2e, segmentation labels differing: %d of %d" % (e_ed, seg_diff, out["seg"].size), flush=True)
+    assert ref["edema"].max() > 0
+    assert e_ed < 1e-2
+    assert seg_diff <= 1e-4 * out["seg"].size
