@@ -99,14 +99,16 @@ SIGNATURES = {
     "glia_rd_stats": (_i, [_vp, _i, ctypes.POINTER(Stats)]),
     "glia_rd_reset_stats": (_i, [_vp]),
     "glia_rd_smooth_field": (_i, [_vp, _i, _i, _d]),
-    "glia_rd_smooth_noalloc": (_i, [ctypes.POINTER(ctypes.c_int), _vp, _vp, _d, _vp, ctypes.c_char_p, _i]),
     "glia_rd_set_time": (_i, [_vp, _d, _i]),
+    "glia_rd_autotune": (_i, [_vp, ctypes.c_char_p, _vp]),
+    "glia_rd_step_info": (_i, [_vp, _vp, _vp, _vp]),
     "glia_rd_keep_adjoint_history": (_i, [_vp, _i]),
     "glia_rd_get_history": (_i, [_vp, _i, _i, _i, _vp, _i]),
     "glia_rd_set_secondary": (_i, [_vp, _i, _d]),
     "glia_rd_grad_diffusion": (_i, [_vp, _vp]),
     "glia_rd_grad_reaction": (_i, [_vp, _vp]),
     "glia_rd_check_gradient": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "glia_rd_smooth_noalloc": (_i, [ctypes.POINTER(ctypes.c_int), _vp, _vp, _d, _vp, ctypes.c_char_p, _i]),
     "glia_rd_refresh_tissue": (_i, [_vp, _i]),
     "glia_rd_use_graphs": (_i, [_vp, _i]),
     "glia_rd_profile": (_i, [_vp, _i]),
@@ -235,7 +237,7 @@ EL_SIGNATURES = {
 # include/glia_ms.h (multispecies pointwise terms)
 class MsParams(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in ("death_rate", "ox_source", "ox_consumption", "alpha_0", "beta_0",
-                                                 "ox_inv", "ox_hypoxia")]
+                                                 "ox_inv", "ox_hypoxia", "invasive_threshold")]
 
 
 MS_SIGNATURES = {
@@ -270,6 +272,7 @@ ME_SIGNATURES = {
     "glia_me_solve_state": (_i, [_vp, ctypes.POINTER(MeParams)] + [_vp] * 9 + [ctypes.POINTER(MeInfo)]),
     "glia_me_solve_state_multispecies": (_i, [_vp, ctypes.POINTER(MeParams), ctypes.POINTER(MsParams)] + [_vp] * 12
                                          + [ctypes.POINTER(MeInfo)]),
+    "glia_me_multispecies_outputs": (_i, [_vp, _vp, _vp, _i]),
 }
 
 

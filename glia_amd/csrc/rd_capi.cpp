@@ -1,4 +1,5 @@
 // rd_capi.cpp -- extern "C" boundary (include/glia_rd.h) over glia::RDEnsemble.
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -107,7 +108,10 @@ int glia_rd_create(const glia_rd_config* cfg, glia_rd_t* out) {
     delete h;
     return GLIA_RD_ERR_ARG;
   }
-  const int rc = h->e->init();
+  int rc = h->e->init();
+  // GLIA_AUTOTUNE=1 tunes every new handle (GLIA_TUNE_CACHE: the choice cache file)
+  const char* tune = std::getenv("GLIA_AUTOTUNE");
+  if (rc == 0 && tune && std::string(tune) == "1") rc = h->e->autotune(std::getenv("GLIA_TUNE_CACHE"), nullptr);
   if (rc != 0) {
     // keep the handle so the caller can read the message, then destroy it
     h->err = h->e->error();
@@ -366,6 +370,18 @@ int glia_rd_solve_adjoint(glia_rd_t h, int linearized) {
   int rc = h->e->solve_adjoint(linearized);
   if (!rc) rc = h->e->sync();
   return ret(h, rc);
+}
+
+int glia_rd_autotune(glia_rd_t h, const char* cache_path, int* choice) {
+  GLIA_H(h);
+  return ret(h, h->e->autotune(cache_path, choice));
+}
+
+int glia_rd_step_info(glia_rd_t h, long long* launched, long long* reruns, int* cached) {
+  GLIA_H(h);
+  if (!launched || !reruns || !cached) return arg_err(h, "null output");
+  h->e->step_info(launched, reruns, cached);
+  return 0;
 }
 
 int glia_rd_set_time(glia_rd_t h, double dt, int nt) {

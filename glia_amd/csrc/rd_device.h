@@ -21,6 +21,7 @@ enum CSlot {
   CS_T5,
   CS_T6,
   CS_T7,
+  CS_BK,      // copy of the adjoint state at the start of a captured adjoint step (re-run source)
   CS_NUM
 };
 
@@ -76,6 +77,10 @@ struct PcgState {
   double c0sq;     // ||c0||^2
   double gk, gr;   // trapezoid sums of <m_k, grad c . grad alpha> and <m_rho, (c^2 - c) alpha>
   double red[4];   // scratch results of generic reductions
+  // accumulators and counters at the start of a captured time step, restored when the step runs
+  // again (one of its solves needed more iterations than the capture allowed)
+  double gk_snap, gr_snap;
+  int its_snap, solves_snap, worst_snap, pad1;
 };
 
 struct Brick {

@@ -477,6 +477,151 @@ k_xf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Split-group variants of the two divergence passes: each field's transform runs on its own group
+// of 16 * NT threads with its own exchange area, so a thread holds one field's values instead of
+// two or three; the groups meet once more in LDS to combine.  The arithmetic of every output value
+// is that of k_yf_div / k_xf_div (same operations in the same order), so the results are the same
+// bit for bit.  Groups are whole waves (16 * NT is a multiple of 64 for the plans used), and every
+// group passes the same barriers.
+// ---------------------------------------------------------------------------------------------
+template <int L>
+struct CKS {
+  using K = CK<L>;
+  static constexpr bool OK = (K::TPB % 64) == 0;  // groups must be whole waves
+};
+
+// Y pass, forward, divergence in three groups: Hx -> Yx = FFT_y(Hx); Hy -> i w_y FFT_y(Hy) into
+// LDS; Q -> Rr = FFT_y(Q) + (that term).
+template <int L>
+__global__ void __launch_bounds__(3 * CK<L>::TPB)
+k_yf_div_s(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_hx, int s_hy, int s_q, int s_yx,
+           int s_rr) {
+  static_assert(CKS<L>::OK, "thread groups must be whole waves");
+  const MemberDev& M = mem[blockIdx.y];
+  if (M.st->done) return;
+  using K = CK<L>;
+  constexpr int R1 = K::R1, R2 = K::R2, NT = K::NT;
+  __shared__ float2 ex[3][16 * K::CP];
+  const int grp = (int)threadIdx.x / K::TPB;
+  const int lt = (int)threadIdx.x - grp * K::TPB;
+  const int col = lt & 15, t = lt >> 4;
+  float2* exc = ex[grp] + col * K::CP;
+  const Tile tl = col_tile(1, d);
+  GPtr In = cptr(M, grp == 0 ? s_hx : (grp == 1 ? s_hy : s_q)) + tl.base;
+  float2 a[K::VPT];
+#pragma unroll
+  for (int j = 0; j < K::VIN; j++) a[j] = In[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
+  __shared__ float2 twl[L];
+  stage_twiddles<L, 3 * K::TPB>(twl, tw);
+  fft2s<R1, R2, NT, -1>(a, exc, t, twl);
+  __syncthreads();  // every exchange area is free again
+  if (grp == 0) {
+    GPtr Yx = cptr(M, s_yx) + tl.base;
+#pragma unroll
+    for (int j = 0; j < K::VOUT; j++) if (K::out_ok(t, j)) Yx[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
+  } else if (grp == 1) {
+#pragma unroll
+    for (int j = 0; j < K::VOUT; j++) {
+      if (!K::out_ok(t, j)) continue;
+      const int row = out_row<R1, R2, NT>(t, j);
+      exc[xpad(row)] = mul_i(cscale(a[j], wnum(row, d.n1)));
+    }
+  }
+  __syncthreads();
+  if (grp == 2) {
+    GPtr Rr = cptr(M, s_rr) + tl.base;
+    const float2* exh = ex[1] + col * K::CP;
+#pragma unroll
+    for (int j = 0; j < K::VOUT; j++) {
+      if (!K::out_ok(t, j)) continue;
+      const int row = out_row<R1, R2, NT>(t, j);
+      Rr[(unsigned)row * tl.rs + col] = cadd(a[j], exh[xpad(row)]);
+    }
+  }
+}
+
+// X pass, forward, closing the divergence in two groups: Yx -> i w_x FFT_x(Yx) into LDS; Rr ->
+// D = (that term) + FFT_x(Rr), then the MODE's output and partial sums as in k_xf_div.
+template <int L, int MODE>
+__global__ void __launch_bounds__(2 * CK<L>::TPB)
+k_xf_div_s(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_yx, int s_rr, int s_p,
+           int s_out) {
+  static_assert(CKS<L>::OK, "thread groups must be whole waves");
+  const MemberDev& M = mem[blockIdx.y];
+  const PcgState* st = M.st;
+  if (st->done) return;
+  using K = CK<L>;
+  constexpr int R1 = K::R1, R2 = K::R2, NT = K::NT;
+  __shared__ float2 ex[2][16 * K::CP];
+  const int grp = (int)threadIdx.x / K::TPB;
+  const int lt = (int)threadIdx.x - grp * K::TPB;
+  const int col = lt & 15, t = lt >> 4;
+  float2* exc = ex[grp] + col * K::CP;
+  const Tile tl = col_tile(0, d);
+  GPtr In = cptr(M, grp == 0 ? s_yx : s_rr) + tl.base;
+  GPtr P = cptr(M, s_p) + tl.base;
+  GPtr O = cptr(M, s_out) + tl.base;
+  const float tau = (float)st->tau;
+  const float pc = (float)st->pc;
+  const float wy = wnum(tl.o, d.n1), wz = (float)(tl.c0 + col);
+  const double wgt = (tl.c0 + col == 0) ? 1.0 : 2.0;
+  float2 a[K::VPT], pv[K::VPT];
+#pragma unroll
+  for (int j = 0; j < K::VIN; j++) a[j] = In[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
+  if (MODE != XF_D && grp == 1) {
+#pragma unroll
+    for (int j = 0; j < K::VOUT; j++) if (K::out_ok(t, j)) pv[j] = P[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col];
+  }
+  __shared__ float2 twl[L];
+  stage_twiddles<L, 2 * K::TPB>(twl, tw);
+  fft2s<R1, R2, NT, -1>(a, exc, t, twl);
+  __syncthreads();  // both exchange areas are free again
+  if (grp == 0) {
+#pragma unroll
+    for (int j = 0; j < K::VOUT; j++) {
+      if (!K::out_ok(t, j)) continue;
+      const int row = out_row<R1, R2, NT>(t, j);
+      exc[xpad(row)] = mul_i(cscale(a[j], wnum(row, d.n0)));
+    }
+  }
+  __syncthreads();
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (grp == 1) {
+    const float2* exa = ex[0] + col * K::CP;
+#pragma unroll
+    for (int j = 0; j < K::VOUT; j++) {
+      if (!K::out_ok(t, j)) continue;
+      const int row = out_row<R1, R2, NT>(t, j);
+      const unsigned g = (unsigned)row * tl.rs + col;
+      const float2 D = cadd(exa[xpad(row)], a[j]);
+      if (MODE == XF_MATVEC) {
+        const float2 p = pv[j];
+        const float2 w = make_float2(p.x - 0.5f * tau * D.x, p.y - 0.5f * tau * D.y);
+        O[g] = w;
+        acc[0] += wgt * ((double)p.x * w.x + (double)p.y * w.y);
+      } else if (MODE == XF_SETUP) {
+        const float2 x = pv[j];
+        const float2 bb = make_float2(x.x + 0.5f * tau * D.x, x.y + 0.5f * tau * D.y);
+        const float2 r = cscale(D, tau);
+        O[g] = r;
+        const float f = precf(pc, wnum(row, d.n0), wy, wz);
+        const double r2 = (double)r.x * r.x + (double)r.y * r.y;
+        acc[0] += wgt * (double)f * f * ((double)bb.x * bb.x + (double)bb.y * bb.y);
+        acc[1] += wgt * (double)f * f * r2;
+        acc[2] += wgt * (double)f * r2;
+      } else {
+        O[g] = D;
+      }
+    }
+  }
+  if (MODE == XF_SETUP) block_reduce_store<3>(acc, M.part + (int64_t)blockIdx.x * 3);
+  if (MODE == XF_MATVEC) {
+    double a1[1] = {acc[0]};
+    block_reduce_store<1>(a1, M.part + blockIdx.x);
+  }
+}
+
 // plain column transform: out = FFT_dir(in) along axis DIR (S = -1 forward, +1 inverse)
 template <int L, int DIR, int S>
 __global__ void __launch_bounds__(CK<L>::TPB, GLIA_COL_MINW)
@@ -959,8 +1104,8 @@ __global__ void k_pcg_fin_setup(const MemberDev* __restrict__ mem, Dims d, int n
         }
       }
     }
-    if (hflag) {
-      hflag[hslot * nm + blockIdx.x] = s->done;
+    if (hflag) {  // 0 while the solve runs, 1 + its CG iterations once it is done
+      hflag[hslot * nm + blockIdx.x] = s->done ? s->its + 1 : 0;
       __threadfence_system();
     }
   }
@@ -1018,8 +1163,8 @@ __global__ void k_pcg_fin_update(const MemberDev* __restrict__ mem, Dims d, int 
         }
       }
     }
-    if (hflag) {
-      hflag[hslot * nm + blockIdx.x] = s->done;
+    if (hflag) {  // 0 while the solve runs, 1 + its CG iterations once it is done
+      hflag[hslot * nm + blockIdx.x] = s->done ? s->its + 1 : 0;
       __threadfence_system();
     }
   }
@@ -1143,6 +1288,20 @@ __global__ void k_state_reset(const MemberDev* __restrict__ mem, int what) {
   PcgState* s = mem[blockIdx.x].st;
   if (what & 1) { s->its_total = 0; s->solves = 0; s->worst_reason = 0; }
   if (what & 2) { s->gk = 0; s->gr = 0; s->m1 = 0; }
+  if (what & 4) {  // snapshot at the start of a captured step
+    s->gk_snap = s->gk;
+    s->gr_snap = s->gr;
+    s->its_snap = s->its_total;
+    s->solves_snap = s->solves;
+    s->worst_snap = s->worst_reason;
+  }
+  if (what & 8) {  // back to the snapshot before the step runs again
+    s->gk = s->gk_snap;
+    s->gr = s->gr_snap;
+    s->its_total = s->its_snap;
+    s->solves = s->solves_snap;
+    s->worst_reason = s->worst_snap;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1512,7 +1671,10 @@ template <int L>
 struct RunYfDiv {
   static int run(const LaunchCtx& c, int s_hx, int s_hy, int s_q, int s_yx, int s_rr) {
     dim3 g(c.d.n0 * (c.d.m2 / 16), c.nm);
-    k_yf_div<L><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw1, s_hx, s_hy, s_q, s_yx, s_rr);
+    if (c.yf_split)
+      k_yf_div_s<L><<<g, 3 * CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw1, s_hx, s_hy, s_q, s_yx, s_rr);
+    else
+      k_yf_div<L><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw1, s_hx, s_hy, s_q, s_yx, s_rr);
     GLIA_CHECK_LAUNCH();
     return 0;
   }
@@ -1525,7 +1687,14 @@ template <int L>
 struct RunXfDiv {
   static int run(const LaunchCtx& c, int mode, int s_yx, int s_rr, int s_p, int s_out) {
     dim3 g(c.d.n1 * (c.d.m2 / 16), c.nm);
-    if (mode == XF_MATVEC)
+    const int tb = 2 * CK<L>::TPB;
+    if (c.xf_split && mode == XF_MATVEC)
+      k_xf_div_s<L, XF_MATVEC><<<g, tb, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
+    else if (c.xf_split && mode == XF_SETUP)
+      k_xf_div_s<L, XF_SETUP><<<g, tb, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
+    else if (c.xf_split)
+      k_xf_div_s<L, XF_D><<<g, tb, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
+    else if (mode == XF_MATVEC)
       k_xf_div<L, XF_MATVEC><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
     else if (mode == XF_SETUP)
       k_xf_div<L, XF_SETUP><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);

@@ -16,6 +16,9 @@ struct LaunchCtx {
   const float2* tw1;  // exp(-2 pi i m / n1)
   const float2* twm;  // exp(-2 pi i m / (n2/2))
   const float2* tw2;  // exp(-2 pi i m / n2), m < n2/2
+  // variant of the forward divergence passes: split-group kernels (one field per thread group)
+  // instead of one group holding every field; chosen per handle (autotuned or by default)
+  int yf_split = 0, xf_split = 0;
 };
 
 // modes of launch_xf_div and launch_z_point (values match the kernel enums)

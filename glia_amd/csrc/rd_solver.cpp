@@ -13,7 +13,10 @@
 #include "rd_solver.h"
 
 #include <algorithm>
+#include <fstream>
+#include <functional>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <sstream>
 
@@ -77,6 +80,8 @@ LaunchCtx RDEnsemble::ctx() const {
   c.tw1 = tw_[1];
   c.twm = tw_[2];
   c.tw2 = tw_[3];
+  c.yf_split = yf_split_;
+  c.xf_split = xf_split_;
   return c;
 }
 
@@ -89,6 +94,18 @@ static LaunchCtx member_ctx(LaunchCtx c, const MemberDev* md, int m) {
 
 int RDEnsemble::init() {
   if (nm_ < 1) return fail(-2, "nmembers must be >= 1");
+  // captured steps unroll each solve's CG loop to the last count of its kind plus this margin;
+  // GLIA_STEP_MARGIN < 0 makes every captured step run again eagerly (a check of that path)
+  if (const char* v = std::getenv("GLIA_STEP_MARGIN")) margin_ = std::atoi(v);
+  // forward divergence variants: the X pass in two thread groups (measured faster at 256 points),
+  // the Y pass in one; GLIA_SPLIT_DIV (none, xf, yf, both) overrides, glia_rd_autotune measures
+  xf_split_ = 1;
+  yf_split_ = 0;
+  if (const char* v = std::getenv("GLIA_SPLIT_DIV")) {
+    const std::string sv(v);
+    yf_split_ = (sv == "yf" || sv == "both") ? 1 : 0;
+    xf_split_ = (sv == "xf" || sv == "both") ? 1 : 0;
+  }
   if (!supported_line(d_.n0) || !supported_line(d_.n1) || !supported_z(d_.n2))
     return fail(-4, "unsupported grid: n0, n1 in {32,48,64,80,96,128,160,192,256}, n2 in {32,64,96,128,160,192,256}");
   if (z_blocks(d_) <= 0 || ((int64_t)d_.n0 * d_.n1) % z_lines_per_block(d_.m2) != 0)
@@ -122,13 +139,15 @@ int RDEnsemble::init() {
     GLIA_TRYC(copy_sync(twbuf_, h.data(), sizeof(float2) * tot, hipMemcpyHostToDevice));
     for (int i = 0; i < 4; i++) tw_[i] = twbuf_ + offs[i];
   }
-  GLIA_HIP(hipHostMalloc((void**)&hflag_host_, sizeof(int) * kFlagSlots * nm_, hipHostMallocMapped | hipHostMallocCoherent));
+  GLIA_HIP(hipHostMalloc((void**)&hflag_host_, sizeof(int) * (kFlagSlots + kStepFlags) * nm_,
+                         hipHostMallocMapped | hipHostMallocCoherent));
   GLIA_HIP(hipHostGetDevicePointer((void**)&hflag_dev_, hflag_host_, 0));
   // The host reads the convergence flags from coherent host memory that the finalize kernels
   // write with a system-scope fence, so these events only need a device-scope release (a
   // system-scope release would write back L2 on every CG iteration).
   for (int i = 0; i < kFlagSlots; i++)
     GLIA_HIP(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming | hipEventReleaseToDevice));
+  GLIA_HIP(hipEventCreateWithFlags(&ev_step_, hipEventDisableTiming | hipEventReleaseToDevice));
   ev_ok_ = true;
   GLIA_HIP(hipMalloc(&coefs_dev_, sizeof(double) * 3 * nm_));
 
@@ -376,8 +395,11 @@ RDEnsemble::~RDEnsemble() {
   (void)hipFree(twbuf_);
   (void)hipFree(coefs_dev_);
   if (hflag_host_) (void)hipHostFree(hflag_host_);
-  if (ev_ok_)
+  for (auto& kv : step_graphs_) (void)hipGraphExecDestroy(kv.second);
+  if (ev_ok_) {
     for (int i = 0; i < kFlagSlots; i++) (void)hipEventDestroy(ev_[i]);
+    (void)hipEventDestroy(ev_step_);
+  }
   if (own_stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -671,8 +693,9 @@ int RDEnsemble::matvec_setup(int sx) {
   return 0;
 }
 
-// DiffusionSolver::solve(c, tau) on the spectral state in slot sx, in place
-int RDEnsemble::cn_solve(int sx, double tau) {
+// DiffusionSolver::solve(c, tau) on the spectral state in slot sx, in place.  kind >= 0 names the
+// solve's place in a time step (SK_*): its iteration count becomes the budget of the next capture.
+int RDEnsemble::cn_solve(int sx, double tau, int kind) {
   ctx_dt_ = tau;  // ctx->dt_ = dt (src/pde/DiffusionSolver.cpp:226)
   LaunchCtx c = ctx();
   const int nbx = xf_div_blocks(d_);
@@ -684,11 +707,15 @@ int RDEnsemble::cn_solve(int sx, double tau) {
   GLIA_HIP(hipEventRecord(ev_[stage % kFlagSlots], stream_));
   int checked = -1;
   bool all_done = false;
+  int its_max = 0;
   auto check_stage = [&](int s) -> int {
     GLIA_HIP(hipEventSynchronize(ev_[s % kFlagSlots]));
     volatile int* f = hflag_host_ + (s % kFlagSlots) * nm_;
     bool all = true;
-    for (int m = 0; m < nm_; m++) all = all && (f[m] != 0);
+    for (int m = 0; m < nm_; m++) {
+      all = all && (f[m] != 0);
+      its_max = std::max(its_max, f[m] - 1);
+    }
     if (all) all_done = true;
     return 0;
   };
@@ -710,7 +737,168 @@ int RDEnsemble::cn_solve(int sx, double tau) {
       if (rc) return rc;
     }
   }
+  if (kind >= 0 && all_done) budget_[kind] = its_max;
   return 0;
+}
+
+// The launches of cn_solve with the iteration loop unrolled K times, for capture: CG iterations
+// past convergence exit at once, and the last iteration's convergence flag (0 = still running,
+// 1 + its otherwise, per member) lands in host flag slot fslot.
+int RDEnsemble::capture_solve(int sx, double tau, int K, int fslot) {
+  ctx_dt_ = tau;
+  LaunchCtx c = ctx();
+  GLIA_TRYC(launch_pcg_begin(c, tau));
+  int rc = matvec_setup(sx);
+  if (rc) return rc;
+  GLIA_TRYC(launch_pcg_fin_setup(c, xf_div_blocks(d_), hflag_dev_, K == 0 ? fslot : 0));
+  for (int k = 1; k <= K; k++)
+    if ((rc = launch_iteration(sx, k == K ? fslot : k % kFlagSlots))) return rc;
+  return 0;
+}
+
+bool RDEnsemble::step_graphs_on() const { return use_graphs_ && !prof_on_ && hist_all_; }
+
+// capture the launches of 'body' into an executable graph cached under key
+int RDEnsemble::step_graph(const StepKey& key, const std::function<int()>& body, hipGraphExec_t* out) {
+  auto it = step_graphs_.find(key);
+  if (it != step_graphs_.end()) {
+    *out = it->second;
+    return 0;
+  }
+  if (step_graphs_.size() >= (size_t)(8 * cfg_.nt + 64)) {  // budgets moved: drop stale captures
+    GLIA_HIP(hipStreamSynchronize(stream_));
+    for (auto& kv : step_graphs_) (void)hipGraphExecDestroy(kv.second);
+    step_graphs_.clear();
+  }
+  GLIA_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  const int rc = body();
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(stream_, &g);
+  if (rc) {
+    if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  GLIA_HIP(e);
+  hipGraphExec_t ex = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  GLIA_HIP(ei);
+  step_graphs_[key] = ex;
+  *out = ex;
+  return 0;
+}
+
+// after a step graph: wait for it and read the flags of its nsolve budgeted solves; ok when every
+// member of every solve converged within the budget, its[j] the largest iteration count of solve j
+int RDEnsemble::read_step_flags(int nsolve, int* its, bool* ok) {
+  GLIA_HIP(hipEventRecord(ev_step_, stream_));
+  GLIA_HIP(hipEventSynchronize(ev_step_));
+  *ok = true;
+  for (int j = 0; j < nsolve; j++) {
+    volatile int* f = hflag_host_ + (kFlagSlots + j) * nm_;
+    its[j] = 0;
+    for (int m = 0; m < nm_; m++) {
+      if (f[m] == 0) *ok = false;
+      its[j] = std::max(its[j], f[m] - 1);
+    }
+  }
+  return 0;
+}
+
+// one forward Strang step (forward_step) from c_i in CS_X, with c_{i+1} stored in the history:
+// replayed from a graph captured with iteration budgets one above (margin_) the last solves' counts; when
+// a solve needs more, the counters return to the step's start and the step runs again eagerly
+// from c_i (stored in the history), so the result is that of forward_step
+int RDEnsemble::forward_step_stored(int i, bool store_half) {
+  const double dt = cfg_.dt;
+  int rc;
+  if (step_graphs_on() && budget_[SK_FWD1] >= 0 && budget_[SK_FWD2] >= 0) {
+    const int K1 = std::max(0, budget_[SK_FWD1] + margin_), K2 = std::max(0, budget_[SK_FWD2] + margin_);
+    const StepKey key{0, i, K1, K2, 0, store_half ? 1 : 0};
+    hipGraphExec_t g = nullptr;
+    rc = step_graph(key, [&]() -> int {
+      LaunchCtx c = ctx();
+      GLIA_TRYC(launch_state_reset(c, 4));
+      int r = capture_solve(CS_X, dt / 2.0, K1, kFlagSlots + 0);
+      if (r) return r;
+      GLIA_TRYC(launch_col_plain(c, 0, +1, CS_X, CS_X, false));
+      GLIA_TRYC(launch_col_plain(c, 1, +1, CS_X, CS_X, false));
+      GLIA_TRYC(launch_z_point(c, ZPM_REACT, CS_X, CS_X, -1, store_half ? i : -1, (float)dt, false));
+      GLIA_TRYC(launch_col_plain(c, 1, -1, CS_X, CS_X, false));
+      GLIA_TRYC(launch_col_plain(c, 0, -1, CS_X, CS_X, false));
+      if ((r = capture_solve(CS_X, dt / 2.0, K2, kFlagSlots + 1))) return r;
+      GLIA_TRYC(launch_copy(c, CS_X, state_slot(i + 1)));
+      return 0;
+    }, &g);
+    if (rc) return rc;
+    GLIA_HIP(hipGraphLaunch(g, stream_));
+    step_launches_++;
+    int its[2];
+    bool ok = false;
+    if ((rc = read_step_flags(2, its, &ok))) return rc;
+    if (ok) {
+      budget_[SK_FWD1] = its[0];
+      budget_[SK_FWD2] = its[1];
+      return 0;
+    }
+    step_reruns_++;
+    GLIA_TRYC(launch_state_reset(ctx(), 8));
+    GLIA_TRYC(launch_copy(ctx(), state_slot(i), CS_X));
+  }
+  if ((rc = forward_step(CS_X, i, store_half))) return rc;
+  GLIA_TRYC(launch_copy(ctx(), CS_X, state_slot(i + 1)));
+  return 0;
+}
+
+// one backward step of the adjoint (adjoint_step at time i, store-all history) replayed from a
+// captured graph in the same way; the adjoint state at the step's start is kept in CS_BK
+int RDEnsemble::adjoint_step_stored(int i) {
+  const double dt = cfg_.dt;
+  const bool half = cfg_.adjoint_store != 0;  // c_half_ stored, no recomputation solve
+  int rc;
+  if (step_graphs_on() && budget_[SK_ADJ1] >= 0 && budget_[SK_ADJ2] >= 0 && (half || budget_[SK_HALF] >= 0)) {
+    const int K1 = std::max(0, budget_[SK_ADJ1] + margin_), K2 = std::max(0, budget_[SK_ADJ2] + margin_);
+    const int Kh = half ? 0 : std::max(0, budget_[SK_HALF] + margin_);
+    const int flags = (half ? 1 : 0) | (quad_on_ ? 2 : 0) | (keep_adj_ ? 4 : 0);
+    const StepKey key{1, i, K1, K2, Kh, flags};
+    hipGraphExec_t g = nullptr;
+    rc = step_graph(key, [&]() -> int {
+      LaunchCtx c = ctx();
+      GLIA_TRYC(launch_state_reset(c, 4));
+      GLIA_TRYC(launch_copy(c, CS_A, CS_BK));
+      int r = capture_solve(CS_A, dt / 2.0, K1, kFlagSlots + 0);
+      if (r) return r;
+      if (!half) {
+        GLIA_TRYC(launch_copy(c, state_slot(i), CS_H));
+        if ((r = capture_solve(CS_H, dt / 2.0, Kh, kFlagSlots + 2))) return r;
+        if ((r = inverse_to_real(CS_H, CS_H, RS_CH))) return r;
+      }
+      GLIA_TRYC(launch_col_plain(c, 0, +1, CS_A, CS_A, false));
+      GLIA_TRYC(launch_col_plain(c, 1, +1, CS_A, CS_A, false));
+      GLIA_TRYC(launch_z_point(c, ZPM_REACT_ADJ, CS_A, CS_A, half ? -1 : RS_CH, half ? i : -1, (float)dt, false));
+      GLIA_TRYC(launch_col_plain(c, 1, -1, CS_A, CS_A, false));
+      GLIA_TRYC(launch_col_plain(c, 0, -1, CS_A, CS_A, false));
+      if ((r = capture_solve(CS_A, dt / 2.0, K2, kFlagSlots + 1))) return r;
+      if ((r = store_adjoint(i))) return r;
+      return quad_on_ ? quad(state_slot(i), CS_A, (i == 0) ? 0.5 : 1.0) : 0;
+    }, &g);
+    if (rc) return rc;
+    GLIA_HIP(hipGraphLaunch(g, stream_));
+    step_launches_++;
+    int its[3] = {0, 0, 0};
+    bool ok = false;
+    if ((rc = read_step_flags(half ? 2 : 3, its, &ok))) return rc;
+    if (ok) {
+      budget_[SK_ADJ1] = its[0];
+      budget_[SK_ADJ2] = its[1];
+      if (!half) budget_[SK_HALF] = its[2];
+      return 0;
+    }
+    step_reruns_++;
+    GLIA_TRYC(launch_state_reset(ctx(), 8));
+    GLIA_TRYC(launch_copy(ctx(), CS_BK, CS_A));
+  }
+  return adjoint_step(i, state_slot(i));
 }
 
 // One CG iteration: matvec w = A p (five transform kernels), alpha, x/r update, the PETSc
@@ -776,7 +964,7 @@ int RDEnsemble::set_use_graphs(bool on) {
 // one Strang step (src/pde/PdeOperators.cpp:264-300): D_{dt/2}, R_dt, D_{dt/2}
 int RDEnsemble::forward_step(int sx, int step, bool store_half) {
   const double dt = cfg_.dt;
-  int rc = cn_solve(sx, dt / 2.0);
+  int rc = cn_solve(sx, dt / 2.0, sx == CS_X ? SK_FWD1 : -1);
   if (rc) return rc;
   LaunchCtx c = ctx();
   GLIA_TRYC(launch_col_plain(c, 0, +1, sx, sx, false));
@@ -784,7 +972,7 @@ int RDEnsemble::forward_step(int sx, int step, bool store_half) {
   GLIA_TRYC(launch_z_point(c, ZPM_REACT, sx, sx, -1, store_half ? step : -1, (float)dt, false));
   GLIA_TRYC(launch_col_plain(c, 1, -1, sx, sx, false));
   GLIA_TRYC(launch_col_plain(c, 0, -1, sx, sx, false));
-  return cn_solve(sx, dt / 2.0);
+  return cn_solve(sx, dt / 2.0, sx == CS_X ? SK_FWD2 : -1);
 }
 
 // PdeOperatorsRD::solveState (src/pde/PdeOperators.cpp:235-316) from c0 in RS_C0
@@ -831,6 +1019,11 @@ int RDEnsemble::solve_state(int linearized, bool keep_history) {
   size_t sp = 0;
   for (int i = 0; i < cfg_.nt; i++) {
     const bool store_half = keep_history && hist_all_ && cfg_.adjoint_store;
+    if (keep_history && hist_all_) {
+      if ((rc = forward_step_stored(i, store_half))) return rc;
+      stored_[i + 1] = i + 1;
+      continue;
+    }
     rc = forward_step(CS_X, i, store_half);
     if (rc) return rc;
     if (keep_history) {
@@ -892,7 +1085,7 @@ int RDEnsemble::quad(int cslot_c, int cslot_a, double w) {
 // reactionAdjoint (:318-370), then the trapezoid term at time i
 int RDEnsemble::adjoint_step(int i, int cslot_c) {
   const double dt = cfg_.dt;
-  int rc = cn_solve(CS_A, dt / 2.0);
+  int rc = cn_solve(CS_A, dt / 2.0, SK_ADJ1);
   if (rc) return rc;
   LaunchCtx c = ctx();
   int ch = -1;
@@ -900,7 +1093,7 @@ int RDEnsemble::adjoint_step(int i, int cslot_c) {
     ch = i;
   } else {
     GLIA_TRYC(launch_copy(c, cslot_c, CS_H));
-    rc = cn_solve(CS_H, dt / 2.0);
+    rc = cn_solve(CS_H, dt / 2.0, SK_HALF);
     if (rc) return rc;
     rc = inverse_to_real(CS_H, CS_H, RS_CH);
     if (rc) return rc;
@@ -910,7 +1103,7 @@ int RDEnsemble::adjoint_step(int i, int cslot_c) {
   GLIA_TRYC(launch_z_point(c, ZPM_REACT_ADJ, CS_A, CS_A, ch >= 0 ? -1 : RS_CH, ch, (float)dt, false));
   GLIA_TRYC(launch_col_plain(c, 1, -1, CS_A, CS_A, false));
   GLIA_TRYC(launch_col_plain(c, 0, -1, CS_A, CS_A, false));
-  rc = cn_solve(CS_A, dt / 2.0);
+  rc = cn_solve(CS_A, dt / 2.0, SK_ADJ2);
   if (rc) return rc;
   if ((rc = store_adjoint(i))) return rc;
   return quad_on_ ? quad(cslot_c, CS_A, (i == 0) ? 0.5 : 1.0) : 0;
@@ -1004,7 +1197,7 @@ int RDEnsemble::incremental_adjoint(int ct_slot, bool quads) {
   quad_on_ = quads;
   rc = 0;
   if (hist_all_) {
-    for (int i = cfg_.nt - 1; i >= 0 && rc == 0; i--) rc = adjoint_step(i, state_slot(i));
+    for (int i = cfg_.nt - 1; i >= 0 && rc == 0; i--) rc = adjoint_step_stored(i);
   } else {
     rc = reverse(0, cfg_.nt, stored_[0], npool_ - 1);
   }
@@ -1162,7 +1355,7 @@ int RDEnsemble::solve_adjoint(int linearized) {
   if (rc) return rc;
   if (hist_all_) {
     for (int i = cfg_.nt - 1; i >= 0; i--) {
-      rc = adjoint_step(i, state_slot(i));
+      rc = adjoint_step_stored(i);
       if (rc) return rc;
     }
   } else {
@@ -1172,6 +1365,86 @@ int RDEnsemble::solve_adjoint(int linearized) {
   }
   adj_valid_ = keep_adj_;
   return inverse_to_real(CS_A, CS_T5, RS_A0);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Autotuning of the kernel variants
+// ---------------------------------------------------------------------------------------------
+// The variants of the two forward divergence passes (one thread group holding every field, or one
+// group per field) are timed on this handle's own grid and members with HIP events: two warm-up
+// launches, then the median of five.  The choice is kept in a text cache (one line per device,
+// grid and member count: "<arch> n0 n1 n2 nm xf_split yf_split xf_ms0 xf_ms1 yf_ms0 yf_ms1") that
+// later handles read instead of timing again.
+int RDEnsemble::autotune(const char* cache_path, int* choice) {
+  hipDeviceProp_t prop;
+  int dev = 0;
+  GLIA_HIP(hipGetDevice(&dev));
+  GLIA_HIP(hipGetDeviceProperties(&prop, dev));
+  std::string arch(prop.gcnArchName);
+  for (char& ch : arch)
+    if (ch == ' ' || ch == ':') ch = '_';
+  std::ostringstream key;
+  key << arch << ' ' << d_.n0 << ' ' << d_.n1 << ' ' << d_.n2 << ' ' << nm_;
+  if (cache_path) {
+    std::ifstream in(cache_path);
+    std::string line;
+    while (std::getline(in, line)) {
+      if (line.compare(0, key.str().size() + 1, key.str() + " ") != 0) continue;
+      std::istringstream ls(line.substr(key.str().size() + 1));
+      int xf = -1, yf = -1;
+      if (ls >> xf >> yf && (xf == 0 || xf == 1) && (yf == 0 || yf == 1)) {
+        xf_split_ = xf;
+        yf_split_ = yf;
+        if (choice) { choice[0] = xf; choice[1] = yf; }
+        return 0;
+      }
+    }
+  }
+  int rc = clear_done();  // every member's kernels run during the timing
+  if (rc) return rc;
+  hipEvent_t a, b;
+  GLIA_HIP(hipEventCreate(&a));
+  GLIA_HIP(hipEventCreate(&b));
+  auto time_it = [&](const std::function<int()>& launch, double* ms) -> int {
+    for (int w = 0; w < 2; w++)
+      if (int r = launch()) return r;
+    std::vector<float> t;
+    for (int k = 0; k < 5; k++) {
+      GLIA_HIP(hipEventRecord(a, stream_));
+      if (int r = launch()) return r;
+      GLIA_HIP(hipEventRecord(b, stream_));
+      GLIA_HIP(hipEventSynchronize(b));
+      float x = 0;
+      GLIA_HIP(hipEventElapsedTime(&x, a, b));
+      t.push_back(x);
+    }
+    std::sort(t.begin(), t.end());
+    *ms = t[2];
+    return 0;
+  };
+  double xf_ms[2], yf_ms[2];
+  for (int v = 0; v < 2 && !rc; v++) {
+    xf_split_ = v;
+    rc = time_it([&]() { return launch_xf_div(ctx(), XFM_MATVEC, CS_T0, CS_T1, CS_P, CS_W); }, &xf_ms[v]);
+  }
+  for (int v = 0; v < 2 && !rc; v++) {
+    yf_split_ = v;
+    if (!rc) rc = time_it([&]() { return launch_yf_div(ctx(), CS_T2, CS_T3, CS_T4, CS_T0, CS_T1); }, &yf_ms[v]);
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (!rc) GLIA_TRYC(launch_set_done(ctx(), 1));  // no solve in progress
+  if (!rc) rc = sync();
+  if (rc) return rc;
+  xf_split_ = xf_ms[1] < xf_ms[0] ? 1 : 0;
+  yf_split_ = yf_ms[1] < yf_ms[0] ? 1 : 0;
+  if (choice) { choice[0] = xf_split_; choice[1] = yf_split_; }
+  if (cache_path) {
+    std::ofstream out(cache_path, std::ios::app);
+    out << key.str() << ' ' << xf_split_ << ' ' << yf_split_ << ' ' << xf_ms[0] << ' ' << xf_ms[1] << ' ' << yf_ms[0] << ' '
+        << yf_ms[1] << '\n';
+  }
+  return 0;
 }
 
 // ---------------------------------------------------------------------------------------------

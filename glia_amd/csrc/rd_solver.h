@@ -5,6 +5,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -119,6 +121,10 @@ class RDEnsemble {
   // profiling mode: per-kind launch count and summed event time (ms) since enabled
   int profile_enable(bool on);
   int set_use_graphs(bool on);
+  // time the forward divergence variants on this handle (or read them from the cache file) and
+  // keep the faster; choice (2 ints, may be NULL) receives (xf_split, yf_split)
+  int autotune(const char* cache_path, int* choice);
+  void variants(int* xf, int* yf) const { *xf = xf_split_; *yf = yf_split_; }
   int smooth_field(int m, int rslot, double sigma);  // weierstrassSmoother, in place
   // the same smoother on any device field f of the grid, in place, through tmp (a distinct field
   // of the grid) and the spectral work slot CS_T7 of member 0; no copies, no host synchronization
@@ -140,7 +146,16 @@ class RDEnsemble {
 
   void smoother_taps(double sigma, std::vector<double>& taps, int kr[3], int& stride) const;
   int smooth_taps_for(double sigma);
-  int cn_solve(int sx, double tau);
+  // the solves of a time step whose iteration counts budget its capture
+  enum { SK_FWD1 = 0, SK_FWD2, SK_ADJ1, SK_HALF, SK_ADJ2, SK_NUM };
+  int cn_solve(int sx, double tau, int kind = -1);
+  int capture_solve(int sx, double tau, int K, int fslot);
+  using StepKey = std::array<int, 6>;  // phase, step, budgets, variant flags
+  bool step_graphs_on() const;
+  int step_graph(const StepKey& key, const std::function<int()>& body, hipGraphExec_t* out);
+  int read_step_flags(int nsolve, int* its, bool* ok);
+  int forward_step_stored(int i, bool store_half);
+  int adjoint_step_stored(int i);
   int launch_iteration(int sx, int slot);                       // the launches of one CG iteration
   int iteration_graph(int sx, int slot, hipGraphExec_t* out);   // the same, captured once and cached
   int matvec_setup(int sx);
@@ -198,6 +213,22 @@ class RDEnsemble {
   int* hflag_host_ = nullptr;
   int* hflag_dev_ = nullptr;
   static constexpr int kFlagSlots = 4;
+  static constexpr int kStepFlags = 3;  // host flag slots of the budgeted solves of a captured step
+  hipEvent_t ev_step_ = nullptr;
+  int budget_[SK_NUM] = {-1, -1, -1, -1, -1};  // iterations of the last solve of each kind (-1: none yet)
+  std::map<StepKey, hipGraphExec_t> step_graphs_;
+  long long step_reruns_ = 0;  // captured steps that ran again eagerly
+  long long step_launches_ = 0;  // captured steps launched
+  int margin_ = 1;             // CG iterations captured beyond the last count of a solve kind
+
+ public:
+  void step_info(long long* launched, long long* reruns, int* cached) const {
+    *launched = step_launches_;
+    *reruns = step_reruns_;
+    *cached = (int)step_graphs_.size();
+  }
+
+ private:
   static constexpr int kLookahead = 2;
   hipEvent_t ev_[kFlagSlots];
   bool ev_ok_ = false;
@@ -206,6 +237,7 @@ class RDEnsemble {
     int kind;
     hipEvent_t a, b;
   };
+  int yf_split_ = 0, xf_split_ = 0;  // split-group variants of the forward divergence passes
   bool quad_on_ = true;     // adjoint steps accumulate the kappa/rho gradient quadratures
   bool use_graphs_ = true;  // replay CG iterations from captured graphs (not while profiling)
   double* smooth_taps_ = nullptr;  // device taps of the last smoothing width (three axes)

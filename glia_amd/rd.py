@@ -304,6 +304,21 @@ class RDSolver:
         """Mask, k, rho, preconditioner and Phi from the stored tissue fields."""
         self._chk(self.L.glia_rd_refresh_tissue(self.h, int(member)))
 
+    def autotune(self, cache_path=None):
+        """Time the variants of the forward divergence passes on this handle and keep the faster
+        (cache_path: a text file of earlier choices, read first and appended to).  Returns
+        (x_pass_split, y_pass_split)."""
+        ch = (ctypes.c_int * 2)()
+        path = None if cache_path is None else str(cache_path).encode()
+        self._chk(self.L.glia_rd_autotune(self.h, path, ctypes.cast(ch, ctypes.c_void_p)))
+        return int(ch[0]), int(ch[1])
+
+    def step_info(self):
+        """(steps launched from captured graphs, steps run again eagerly, graphs cached)."""
+        a, b, c = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_int()
+        self._chk(self.L.glia_rd_step_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
     def use_graphs(self, enable=True):
         """Replay CG iterations from captured HIP graphs (default) or launch kernel by kernel."""
         self._chk(self.L.glia_rd_use_graphs(self.h, int(bool(enable))))

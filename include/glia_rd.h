@@ -271,6 +271,19 @@ enum {
    flag slot, built on first use).  Profiling runs always launch kernel by kernel.  No reference
    counterpart: the reference launches every kernel from the host. */
 int glia_rd_use_graphs(glia_rd_t h, int enable);
+/* Autotuning of the kernel variants (no reference counterpart: the reference leaves plans to
+   cuFFT): the two variants of each forward divergence pass (one thread group holding every
+   field, or one group per field) are timed on this handle's grid and members and the faster is
+   kept.  cache_path (may be NULL) names a text file of earlier choices, read first and appended
+   to.  choice (2 ints, may be NULL) receives (X-pass split, Y-pass split).  Call before the
+   first solve; the results do not change beyond FP32 rounding. */
+int glia_rd_autotune(glia_rd_t h, const char* cache_path, int* choice);
+/* Time steps replayed from captured graphs (glia_rd_use_graphs on, store-all history): each
+   forward and adjoint step runs as one graph whose CG loops are unrolled to the last iteration
+   counts plus one, and the host reads each solve's end-of-solve flag once per step.  A step
+   whose solve needed more iterations runs again eagerly from its stored start.  launched and
+   reruns count such steps since creation, cached the graphs held. */
+int glia_rd_step_info(glia_rd_t h, long long* launched, long long* reruns, int* cached);
 int glia_rd_profile(glia_rd_t h, int enable);
 int glia_rd_profile_get(glia_rd_t h, int kind, double* ms, long long* launches);
 
