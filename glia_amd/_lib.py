@@ -95,6 +95,7 @@ SIGNATURES = {
     "glia_rd_gradient": (_i, [_vp, _i, _vp, _vp]),
     "glia_rd_objective_and_gradient": (_i, [_vp, _i, _vp, _vp, _vp]),
     "glia_rd_objective_and_gradient_group": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "glia_rd_objective_and_gradient_lockstep": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "glia_rd_hessian": (_i, [_vp, _i, _vp, _vp]),
     "glia_rd_stats": (_i, [_vp, _i, ctypes.POINTER(Stats)]),
     "glia_rd_reset_stats": (_i, [_vp]),

@@ -1,4 +1,5 @@
 // rd_capi.cpp -- extern "C" boundary (include/glia_rd.h) over glia::RDEnsemble.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -7,6 +8,8 @@
 #include <string>
 #include <thread>
 #include <cmath>
+#include <cstdint>
+#include <map>
 #include <random>
 #include <vector>
 
@@ -540,6 +543,201 @@ int glia_rd_objective_and_gradient_group(int nh, const glia_rd_t* handles, const
   for (int i = 0; i < nh; i++)
     if (rc[i]) return rc[i];
   return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Lockstep evaluation of several handles: time step i of every handle runs as one graph
+// ---------------------------------------------------------------------------------------------
+extern "C++" {
+namespace {
+
+struct SpanGraph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ex = nullptr;
+};
+
+// graphs spanning handles, keyed by (handle id, step-cache epoch, cached step address) per handle.
+// Per thread, so that evicting never meets a graph that another thread has in flight (every step
+// of a lockstep call has completed before the next one is prepared).  Never destroyed at exit:
+// the runtime may be gone by then.
+std::map<std::vector<uint64_t>, SpanGraph>& span_cache() {
+  thread_local auto* cache = new std::map<std::vector<uint64_t>, SpanGraph>();
+  return *cache;
+}
+constexpr size_t kSpanCacheMax = 512;
+
+int span_launch(const std::vector<RDEnsemble*>& e, const std::vector<const RDEnsemble::StepGraph*>& sg,
+                const std::vector<hipEvent_t>& ev) {
+  std::vector<uint64_t> key;
+  key.reserve(3 * e.size());
+  for (size_t a = 0; a < e.size(); a++) {
+    key.push_back(e[a]->uid());
+    key.push_back(e[a]->graph_epoch());
+    key.push_back((uint64_t)(uintptr_t)sg[a]);
+  }
+  auto& cache = span_cache();
+  auto it = cache.find(key);
+  if (it == cache.end()) {
+    if (cache.size() >= kSpanCacheMax) {
+      for (auto& kv : cache) {
+        (void)hipGraphExecDestroy(kv.second.ex);
+        (void)hipGraphDestroy(kv.second.g);
+      }
+      cache.clear();
+    }
+    SpanGraph s;
+    hipError_t err = hipGraphCreate(&s.g, 0);
+    if (err != hipSuccess) return (int)err;
+    for (size_t a = 0; a < e.size() && err == hipSuccess; a++) {
+      hipGraphNode_t node;
+      err = hipGraphAddChildGraphNode(&node, s.g, nullptr, 0, sg[a]->g);  // independent branches
+    }
+    if (err == hipSuccess) err = hipGraphInstantiate(&s.ex, s.g, nullptr, nullptr, 0);
+    if (err != hipSuccess) {
+      (void)hipGraphDestroy(s.g);
+      return (int)err;
+    }
+    it = cache.emplace(std::move(key), s).first;
+  }
+  // the graph runs on the first handle's stream after the work queued on every handle's stream,
+  // and every handle's stream continues after it
+  hipStream_t s0 = e[0]->stream();
+  for (size_t a = 1; a < e.size(); a++) {
+    if (e[a]->stream() == s0) continue;
+    hipError_t err = hipEventRecord(ev[a], e[a]->stream());
+    if (err == hipSuccess) err = hipStreamWaitEvent(s0, ev[a], 0);
+    if (err != hipSuccess) return (int)err;
+  }
+  hipError_t err = hipGraphLaunch(it->second.ex, s0);
+  if (err == hipSuccess) err = hipEventRecord(ev[0], s0);
+  for (size_t a = 1; a < e.size() && err == hipSuccess; a++)
+    if (e[a]->stream() != s0) err = hipStreamWaitEvent(e[a]->stream(), ev[0], 0);
+  return (int)err;
+}
+
+// one time step of the active handles (step[a] = its index on handle a): every handle's captured
+// step in one graph when all have one, else each captured step on its own stream and the eager
+// steps in the finish calls
+template <class Prepare, class Finish>
+int lockstep_step(const std::vector<glia_rd_t>& hs, const std::vector<int>& step, const std::vector<hipEvent_t>& ev,
+                  Prepare prepare, Finish finish, long long* spanned) {
+  const size_t na = hs.size();
+  std::vector<const RDEnsemble::StepGraph*> sg(na, nullptr);
+  std::vector<RDEnsemble*> e(na);
+  bool all = true;
+  for (size_t a = 0; a < na; a++) {
+    e[a] = hs[a]->e.get();
+    const int rc = prepare(e[a], step[a], &sg[a]);
+    if (rc) return ret(hs[a], rc);
+    all = all && sg[a] != nullptr;
+  }
+  if (all && na > 1) {
+    const int rc = span_launch(e, sg, ev);
+    if (rc) return rc;
+    (*spanned)++;
+  } else {
+    for (size_t a = 0; a < na; a++)
+      if (sg[a]) {
+        const hipError_t err = hipGraphLaunch(sg[a]->ex, e[a]->stream());
+        if (err != hipSuccess) return (int)err;
+      }
+  }
+  for (size_t a = 0; a < na; a++) {
+    const int rc = finish(e[a], step[a], sg[a] != nullptr);
+    if (rc) return ret(hs[a], rc);
+  }
+  return 0;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int glia_rd_objective_and_gradient_lockstep(int nh, const glia_rd_t* handles, const int* modes,
+                                            const double* const* x, double* const* J, double* const* dJ,
+                                            long long* spanned_steps) {
+  if (nh <= 0 || !handles || !modes || !x || !J || !dJ) return GLIA_RD_ERR_ARG;
+  bool phased = true;
+  for (int i = 0; i < nh; i++) {
+    if (!handles[i] || !handles[i]->e || !x[i] || !J[i] || !dJ[i]) return GLIA_RD_ERR_ARG;
+    for (int k = 0; k < i; k++)
+      if (handles[k] == handles[i]) return GLIA_RD_ERR_ARG;
+    phased = phased && handles[i]->e->store_all();
+  }
+  if (spanned_steps) *spanned_steps = 0;
+  if (!phased) {  // checkpointed histories recompute their steps: one handle after another
+    for (int i = 0; i < nh; i++) {
+      const int rc = eval(handles[i], modes[i], x[i], J[i], dJ[i], true);
+      if (rc) return rc;
+    }
+    return 0;
+  }
+  std::vector<std::vector<std::vector<double>>> xv(nh);
+  int ntmax = 0;
+  for (int i = 0; i < nh; i++) {
+    if (!split_x(handles[i]->e.get(), x[i], 2, xv[i])) return arg_err(handles[i], "x does not match the basis sizes");
+    ntmax = std::max(ntmax, handles[i]->e->nt());
+  }
+  std::vector<hipEvent_t> ev(nh, nullptr);
+  for (int i = 0; i < nh; i++)
+    if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) {
+      for (auto v : ev)
+        if (v) (void)hipEventDestroy(v);
+      return GLIA_RD_ERR_STATE;
+    }
+  long long spanned = 0;
+  auto run = [&]() -> int {
+    int rc;
+    for (int i = 0; i < nh; i++)
+      if ((rc = handles[i]->e->eval_begin(modes[i], xv[i]))) return ret(handles[i], rc);
+    std::vector<glia_rd_t> hs;
+    std::vector<int> step;
+    std::vector<hipEvent_t> eva;
+    // forward: step k of every handle with nt > k; adjoint: the k-th step back from each one's end
+    for (int dir = 0; dir < 2; dir++) {
+      for (int k = 0; k < ntmax; k++) {
+        hs.clear();
+        step.clear();
+        eva.clear();
+        for (int i = 0; i < nh; i++) {
+          const int nt = handles[i]->e->nt();
+          if (k >= nt) continue;
+          hs.push_back(handles[i]);
+          step.push_back(dir == 0 ? k : nt - 1 - k);
+          eva.push_back(ev[i]);
+        }
+        if (dir == 0)
+          rc = lockstep_step(
+              hs, step, eva,
+              [](RDEnsemble* e, int i, const RDEnsemble::StepGraph** g) { return e->fwd_prepare(i, e->store_half(), g); },
+              [](RDEnsemble* e, int i, bool l) { return e->fwd_finish(i, e->store_half(), l); }, &spanned);
+        else
+          rc = lockstep_step(
+              hs, step, eva, [](RDEnsemble* e, int i, const RDEnsemble::StepGraph** g) { return e->adj_prepare(i, g); },
+              [](RDEnsemble* e, int i, bool l) { return e->adj_finish(i, l); }, &spanned);
+        if (rc) return rc;
+      }
+      if (dir == 0)
+        for (int i = 0; i < nh; i++)
+          if ((rc = handles[i]->e->eval_middle())) return ret(handles[i], rc);
+    }
+    for (int i = 0; i < nh; i++) {
+      std::vector<double> Jv;
+      std::vector<std::vector<double>> g;
+      if ((rc = handles[i]->e->eval_end(Jv, g))) return ret(handles[i], rc);
+      int64_t off = 0;
+      for (int m = 0; m < handles[i]->e->nmembers(); m++) {
+        J[i][m] = Jv[m];
+        std::memcpy(dJ[i] + off, g[m].data(), sizeof(double) * g[m].size());
+        off += (int64_t)g[m].size();
+      }
+    }
+    return 0;
+  };
+  const int rc = run();
+  for (int i = 0; i < nh; i++) (void)hipStreamSynchronize(handles[i]->e->stream());
+  for (auto v : ev) (void)hipEventDestroy(v);
+  if (spanned_steps) *spanned_steps = spanned;
+  return rc;
 }
 
 int glia_rd_stats(glia_rd_t h, int member, glia_rd_stats_t* out) {

@@ -13,6 +13,7 @@
 #include "rd_solver.h"
 
 #include <algorithm>
+#include <atomic>
 #include <fstream>
 #include <functional>
 #include <cmath>
@@ -38,6 +39,8 @@ static bool supported_z(int n2) {
 }
 
 RDEnsemble::RDEnsemble(const SolverConfig& cfg) : cfg_(cfg) {
+  static std::atomic<unsigned long long> next_uid{1};
+  uid_ = next_uid++;
   nm_ = cfg.nmembers;
   d_.n0 = cfg.n[0];
   d_.n1 = cfg.n[1];
@@ -395,7 +398,7 @@ RDEnsemble::~RDEnsemble() {
   (void)hipFree(twbuf_);
   (void)hipFree(coefs_dev_);
   if (hflag_host_) (void)hipHostFree(hflag_host_);
-  for (auto& kv : step_graphs_) (void)hipGraphExecDestroy(kv.second);
+  drop_step_graphs();
   if (ev_ok_) {
     for (int i = 0; i < kFlagSlots; i++) (void)hipEventDestroy(ev_[i]);
     (void)hipEventDestroy(ev_step_);
@@ -758,17 +761,16 @@ int RDEnsemble::capture_solve(int sx, double tau, int K, int fslot) {
 
 bool RDEnsemble::step_graphs_on() const { return use_graphs_ && !prof_on_ && hist_all_; }
 
-// capture the launches of 'body' into an executable graph cached under key
-int RDEnsemble::step_graph(const StepKey& key, const std::function<int()>& body, hipGraphExec_t* out) {
+// capture the launches of 'body' into a graph cached under key (the graph and its executable form)
+int RDEnsemble::step_graph(const StepKey& key, const std::function<int()>& body, const StepGraph** out) {
   auto it = step_graphs_.find(key);
   if (it != step_graphs_.end()) {
-    *out = it->second;
+    *out = &it->second;
     return 0;
   }
   if (step_graphs_.size() >= (size_t)(8 * cfg_.nt + 64)) {  // budgets moved: drop stale captures
     GLIA_HIP(hipStreamSynchronize(stream_));
-    for (auto& kv : step_graphs_) (void)hipGraphExecDestroy(kv.second);
-    step_graphs_.clear();
+    drop_step_graphs();
   }
   GLIA_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
   const int rc = body();
@@ -781,11 +783,22 @@ int RDEnsemble::step_graph(const StepKey& key, const std::function<int()>& body,
   GLIA_HIP(e);
   hipGraphExec_t ex = nullptr;
   const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) (void)hipGraphDestroy(g);
   GLIA_HIP(ei);
-  step_graphs_[key] = ex;
-  *out = ex;
+  StepGraph& sg = step_graphs_[key];
+  sg.g = g;  // kept: a group evaluation adds it to one graph of several handles' steps
+  sg.ex = ex;
+  *out = &sg;
   return 0;
+}
+
+void RDEnsemble::drop_step_graphs() {
+  for (auto& kv : step_graphs_) {
+    (void)hipGraphExecDestroy(kv.second.ex);
+    (void)hipGraphDestroy(kv.second.g);
+  }
+  step_graphs_.clear();
+  graph_epoch_++;
 }
 
 // after a step graph: wait for it and read the flags of its nsolve budgeted solves; ok when every
@@ -808,30 +821,34 @@ int RDEnsemble::read_step_flags(int nsolve, int* its, bool* ok) {
 // one forward Strang step (forward_step) from c_i in CS_X, with c_{i+1} stored in the history:
 // replayed from a graph captured with iteration budgets one above (margin_) the last solves' counts; when
 // a solve needs more, the counters return to the step's start and the step runs again eagerly
-// from c_i (stored in the history), so the result is that of forward_step
-int RDEnsemble::forward_step_stored(int i, bool store_half) {
+// from c_i (stored in the history), so the result is that of forward_step.  fwd_prepare gives the
+// step's graph (null when the step runs eagerly: no budgets yet, or graphs off); fwd_finish runs
+// after its launch, or in place of it.
+int RDEnsemble::fwd_prepare(int i, bool store_half, const StepGraph** out) {
+  *out = nullptr;
+  if (!(step_graphs_on() && budget_[SK_FWD1] >= 0 && budget_[SK_FWD2] >= 0)) return 0;
   const double dt = cfg_.dt;
+  const int K1 = std::max(0, budget_[SK_FWD1] + margin_), K2 = std::max(0, budget_[SK_FWD2] + margin_);
+  const StepKey key{0, i, K1, K2, 0, store_half ? 1 : 0};
+  return step_graph(key, [&]() -> int {
+    LaunchCtx c = ctx();
+    GLIA_TRYC(launch_state_reset(c, 4));
+    int r = capture_solve(CS_X, dt / 2.0, K1, kFlagSlots + 0);
+    if (r) return r;
+    GLIA_TRYC(launch_col_plain(c, 0, +1, CS_X, CS_X, false));
+    GLIA_TRYC(launch_col_plain(c, 1, +1, CS_X, CS_X, false));
+    GLIA_TRYC(launch_z_point(c, ZPM_REACT, CS_X, CS_X, -1, store_half ? i : -1, (float)dt, false));
+    GLIA_TRYC(launch_col_plain(c, 1, -1, CS_X, CS_X, false));
+    GLIA_TRYC(launch_col_plain(c, 0, -1, CS_X, CS_X, false));
+    if ((r = capture_solve(CS_X, dt / 2.0, K2, kFlagSlots + 1))) return r;
+    GLIA_TRYC(launch_copy(c, CS_X, state_slot(i + 1)));
+    return 0;
+  }, out);
+}
+
+int RDEnsemble::fwd_finish(int i, bool store_half, bool launched) {
   int rc;
-  if (step_graphs_on() && budget_[SK_FWD1] >= 0 && budget_[SK_FWD2] >= 0) {
-    const int K1 = std::max(0, budget_[SK_FWD1] + margin_), K2 = std::max(0, budget_[SK_FWD2] + margin_);
-    const StepKey key{0, i, K1, K2, 0, store_half ? 1 : 0};
-    hipGraphExec_t g = nullptr;
-    rc = step_graph(key, [&]() -> int {
-      LaunchCtx c = ctx();
-      GLIA_TRYC(launch_state_reset(c, 4));
-      int r = capture_solve(CS_X, dt / 2.0, K1, kFlagSlots + 0);
-      if (r) return r;
-      GLIA_TRYC(launch_col_plain(c, 0, +1, CS_X, CS_X, false));
-      GLIA_TRYC(launch_col_plain(c, 1, +1, CS_X, CS_X, false));
-      GLIA_TRYC(launch_z_point(c, ZPM_REACT, CS_X, CS_X, -1, store_half ? i : -1, (float)dt, false));
-      GLIA_TRYC(launch_col_plain(c, 1, -1, CS_X, CS_X, false));
-      GLIA_TRYC(launch_col_plain(c, 0, -1, CS_X, CS_X, false));
-      if ((r = capture_solve(CS_X, dt / 2.0, K2, kFlagSlots + 1))) return r;
-      GLIA_TRYC(launch_copy(c, CS_X, state_slot(i + 1)));
-      return 0;
-    }, &g);
-    if (rc) return rc;
-    GLIA_HIP(hipGraphLaunch(g, stream_));
+  if (launched) {
     step_launches_++;
     int its[2];
     bool ok = false;
@@ -850,44 +867,56 @@ int RDEnsemble::forward_step_stored(int i, bool store_half) {
   return 0;
 }
 
+int RDEnsemble::forward_step_stored(int i, bool store_half) {
+  const StepGraph* sg = nullptr;
+  int rc = fwd_prepare(i, store_half, &sg);
+  if (rc) return rc;
+  if (sg) GLIA_HIP(hipGraphLaunch(sg->ex, stream_));
+  return fwd_finish(i, store_half, sg != nullptr);
+}
+
 // one backward step of the adjoint (adjoint_step at time i, store-all history) replayed from a
 // captured graph in the same way; the adjoint state at the step's start is kept in CS_BK
-int RDEnsemble::adjoint_step_stored(int i) {
-  const double dt = cfg_.dt;
+int RDEnsemble::adj_prepare(int i, const StepGraph** out) {
+  *out = nullptr;
   const bool half = cfg_.adjoint_store != 0;  // c_half_ stored, no recomputation solve
-  int rc;
-  if (step_graphs_on() && budget_[SK_ADJ1] >= 0 && budget_[SK_ADJ2] >= 0 && (half || budget_[SK_HALF] >= 0)) {
-    const int K1 = std::max(0, budget_[SK_ADJ1] + margin_), K2 = std::max(0, budget_[SK_ADJ2] + margin_);
-    const int Kh = half ? 0 : std::max(0, budget_[SK_HALF] + margin_);
-    const int flags = (half ? 1 : 0) | (quad_on_ ? 2 : 0) | (keep_adj_ ? 4 : 0);
-    const StepKey key{1, i, K1, K2, Kh, flags};
-    hipGraphExec_t g = nullptr;
-    rc = step_graph(key, [&]() -> int {
-      LaunchCtx c = ctx();
-      GLIA_TRYC(launch_state_reset(c, 4));
-      GLIA_TRYC(launch_copy(c, CS_A, CS_BK));
-      int r = capture_solve(CS_A, dt / 2.0, K1, kFlagSlots + 0);
-      if (r) return r;
-      if (!half) {
-        GLIA_TRYC(launch_copy(c, state_slot(i), CS_H));
-        if ((r = capture_solve(CS_H, dt / 2.0, Kh, kFlagSlots + 2))) return r;
-        if ((r = inverse_to_real(CS_H, CS_H, RS_CH))) return r;
-      }
-      GLIA_TRYC(launch_col_plain(c, 0, +1, CS_A, CS_A, false));
-      GLIA_TRYC(launch_col_plain(c, 1, +1, CS_A, CS_A, false));
-      GLIA_TRYC(launch_z_point(c, ZPM_REACT_ADJ, CS_A, CS_A, half ? -1 : RS_CH, half ? i : -1, (float)dt, false));
-      GLIA_TRYC(launch_col_plain(c, 1, -1, CS_A, CS_A, false));
-      GLIA_TRYC(launch_col_plain(c, 0, -1, CS_A, CS_A, false));
-      if ((r = capture_solve(CS_A, dt / 2.0, K2, kFlagSlots + 1))) return r;
-      if ((r = store_adjoint(i))) return r;
-      return quad_on_ ? quad(state_slot(i), CS_A, (i == 0) ? 0.5 : 1.0) : 0;
-    }, &g);
-    if (rc) return rc;
-    GLIA_HIP(hipGraphLaunch(g, stream_));
+  if (!(step_graphs_on() && budget_[SK_ADJ1] >= 0 && budget_[SK_ADJ2] >= 0 && (half || budget_[SK_HALF] >= 0)))
+    return 0;
+  const double dt = cfg_.dt;
+  const int K1 = std::max(0, budget_[SK_ADJ1] + margin_), K2 = std::max(0, budget_[SK_ADJ2] + margin_);
+  const int Kh = half ? 0 : std::max(0, budget_[SK_HALF] + margin_);
+  const int flags = (half ? 1 : 0) | (quad_on_ ? 2 : 0) | (keep_adj_ ? 4 : 0);
+  const StepKey key{1, i, K1, K2, Kh, flags};
+  return step_graph(key, [&]() -> int {
+    LaunchCtx c = ctx();
+    GLIA_TRYC(launch_state_reset(c, 4));
+    GLIA_TRYC(launch_copy(c, CS_A, CS_BK));
+    int r = capture_solve(CS_A, dt / 2.0, K1, kFlagSlots + 0);
+    if (r) return r;
+    if (!half) {
+      GLIA_TRYC(launch_copy(c, state_slot(i), CS_H));
+      if ((r = capture_solve(CS_H, dt / 2.0, Kh, kFlagSlots + 2))) return r;
+      if ((r = inverse_to_real(CS_H, CS_H, RS_CH))) return r;
+    }
+    GLIA_TRYC(launch_col_plain(c, 0, +1, CS_A, CS_A, false));
+    GLIA_TRYC(launch_col_plain(c, 1, +1, CS_A, CS_A, false));
+    GLIA_TRYC(launch_z_point(c, ZPM_REACT_ADJ, CS_A, CS_A, half ? -1 : RS_CH, half ? i : -1, (float)dt, false));
+    GLIA_TRYC(launch_col_plain(c, 1, -1, CS_A, CS_A, false));
+    GLIA_TRYC(launch_col_plain(c, 0, -1, CS_A, CS_A, false));
+    if ((r = capture_solve(CS_A, dt / 2.0, K2, kFlagSlots + 1))) return r;
+    if ((r = store_adjoint(i))) return r;
+    return quad_on_ ? quad(state_slot(i), CS_A, (i == 0) ? 0.5 : 1.0) : 0;
+  }, out);
+}
+
+int RDEnsemble::adj_finish(int i, bool launched) {
+  if (launched) {
     step_launches_++;
+    const bool half = cfg_.adjoint_store != 0;
     int its[3] = {0, 0, 0};
     bool ok = false;
-    if ((rc = read_step_flags(half ? 2 : 3, its, &ok))) return rc;
+    int rc = read_step_flags(half ? 2 : 3, its, &ok);
+    if (rc) return rc;
     if (ok) {
       budget_[SK_ADJ1] = its[0];
       budget_[SK_ADJ2] = its[1];
@@ -899,6 +928,14 @@ int RDEnsemble::adjoint_step_stored(int i) {
     GLIA_TRYC(launch_copy(ctx(), CS_BK, CS_A));
   }
   return adjoint_step(i, state_slot(i));
+}
+
+int RDEnsemble::adjoint_step_stored(int i) {
+  const StepGraph* sg = nullptr;
+  int rc = adj_prepare(i, &sg);
+  if (rc) return rc;
+  if (sg) GLIA_HIP(hipGraphLaunch(sg->ex, stream_));
+  return adj_finish(i, sg != nullptr);
 }
 
 // One CG iteration: matvec w = A p (five transform kernels), alpha, x/r update, the PETSc
@@ -975,18 +1012,27 @@ int RDEnsemble::forward_step(int sx, int step, bool store_half) {
   return cn_solve(sx, dt / 2.0, sx == CS_X ? SK_FWD2 : -1);
 }
 
-// PdeOperatorsRD::solveState (src/pde/PdeOperators.cpp:235-316) from c0 in RS_C0
-int RDEnsemble::solve_state(int linearized, bool keep_history) {
-  if (linearized == 1 || linearized == 2) return solve_tangent(linearized == 2);
-  if (linearized != 0) return fail(-2, "solveState: linearized must be 0, 1 or 2");
+// the start of solveState(0): c(0) from RS_C0 into CS_X, and into the history at index 0
+int RDEnsemble::state_begin(bool keep_history) {
   int rc = forward_transform_real(RS_C0, CS_X);
   if (rc) return rc;
   stored_.clear();
   free_pool_.clear();
-  std::vector<int> spine;  // time indices stored during the first sweep (checkpoint mode)
   if (keep_history) {
     GLIA_TRYC(launch_copy(ctx(), CS_X, state_slot(0)));
     stored_[0] = 0;
+  }
+  return 0;
+}
+
+// PdeOperatorsRD::solveState (src/pde/PdeOperators.cpp:235-316) from c0 in RS_C0
+int RDEnsemble::solve_state(int linearized, bool keep_history) {
+  if (linearized == 1 || linearized == 2) return solve_tangent(linearized == 2);
+  if (linearized != 0) return fail(-2, "solveState: linearized must be 0, 1 or 2");
+  int rc = state_begin(keep_history);
+  if (rc) return rc;
+  std::vector<int> spine;  // time indices stored during the first sweep (checkpoint mode)
+  if (keep_history) {
     if (!hist_all_) {
       for (int i = npool_ - 1; i >= 1; i--) free_pool_.push_back(i);
       // the right spine of the recursive schedule: positions the reversal will checkpoint first
@@ -1348,11 +1394,7 @@ int RDEnsemble::solve_adjoint(int linearized) {
     adj_valid_ = keep_adj_;
     return 0;
   }
-  rc = misfit_and_final_adjoint(CS_X);
-  if (rc) return rc;
-  if ((rc = store_adjoint(cfg_.nt))) return rc;
-  rc = quad(CS_X, CS_A, 0.5);
-  if (rc) return rc;
+  if ((rc = adjoint_begin())) return rc;
   if (hist_all_) {
     for (int i = cfg_.nt - 1; i >= 0; i--) {
       rc = adjoint_step_stored(i);
@@ -1363,6 +1405,19 @@ int RDEnsemble::solve_adjoint(int linearized) {
     rc = reverse(0, cfg_.nt, stored_[0], npool_ - 1);
     if (rc) return rc;
   }
+  return adjoint_end();
+}
+
+// the start of solveAdjoint(0|1) from c(T) in CS_X: alpha(T), the misfit and the quadrature at T
+int RDEnsemble::adjoint_begin() {
+  int rc = misfit_and_final_adjoint(CS_X);
+  if (rc) return rc;
+  if ((rc = store_adjoint(cfg_.nt))) return rc;
+  return quad(CS_X, CS_A, 0.5);
+}
+
+// the end of solveAdjoint(0|1): alpha(0) as a real field in RS_A0
+int RDEnsemble::adjoint_end() {
   adj_valid_ = keep_adj_;
   return inverse_to_real(CS_A, CS_T5, RS_A0);
 }
@@ -1528,12 +1583,25 @@ int RDEnsemble::reset_counters() {
 //   mode 2: p, kappa and rho together.  x[m] = [p (np) | kappa | rho].
 int RDEnsemble::objective_and_gradient(int mode, const std::vector<std::vector<double>>& x, bool want_grad,
                                        std::vector<double>& J, std::vector<std::vector<double>>& dJ) {
+  int rc = eval_setup(mode, x);
+  if (rc) return rc;
+  if ((rc = solve_state(0, want_grad))) return rc;
+  if (want_grad) {
+    if ((rc = solve_adjoint(1))) return rc;
+  } else {
+    if ((rc = misfit_and_final_adjoint(CS_X))) return rc;
+  }
+  return eval_result(mode, want_grad, J, dJ);
+}
+
+// the start of an evaluation: updateReactionAndDiffusion (src/grad/DerivativeOperators.cpp:323-367,
+// nk = nr = 1), the preconditioner, c(0) = Phi p (or ||c(0)||^2 for mode 1)
+int RDEnsemble::eval_setup(int mode, const std::vector<std::vector<double>>& x) {
   if (mode < 0 || mode > 2) return fail(-2, "mode must be 0, 1 or 2");
   if ((int)x.size() != nm_) return fail(-2, "x must hold one vector per member");
   for (int m = 0; m < nm_; m++)
     if ((int)x[m].size() != np_[m] + 2) return fail(-2, "x[m] must have np + 2 entries");
   int rc;
-  // updateReactionAndDiffusion (src/grad/DerivativeOperators.cpp:323-367), nk = nr = 1
   for (int m = 0; m < nm_; m++) {
     if ((rc = set_kappa(m, x[m][np_[m]] * cfg_.k_scale))) return rc;
     if (mode != 0 && (rc = set_rho(m, x[m][np_[m] + 1] * cfg_.rho_scale))) return rc;
@@ -1547,16 +1615,17 @@ int RDEnsemble::objective_and_gradient(int mode, const std::vector<std::vector<d
     GLIA_TRYC(launch_fin_state(c, ew_blocks(), FINK_C0SQ, 0.0));
   }
   GLIA_TRYC(launch_state_reset(c, 2));  // gradient accumulators; CG counters reset by the caller
-  if ((rc = solve_state(0, want_grad))) return rc;
+  return 0;
+}
+
+// the end of an evaluation: Phi^T (alpha(0) - beta c(0)) for the p block, then J and dJ from the
+// members' reductions
+int RDEnsemble::eval_result(int mode, bool want_grad, std::vector<double>& J, std::vector<std::vector<double>>& dJ) {
+  int rc;
   std::vector<std::vector<double>> phit;
-  if (want_grad) {
-    if ((rc = solve_adjoint(1))) return rc;
-    if (mode != 1) {
-      GLIA_TRYC(launch_axpby_real(c, RS_A0, 1.0, RS_C0, -cfg_.beta, RS_CH));
-      if ((rc = phi_apply_transpose(RS_CH, phit))) return rc;
-    }
-  } else {
-    if ((rc = misfit_and_final_adjoint(CS_X))) return rc;
+  if (want_grad && mode != 1) {
+    GLIA_TRYC(launch_axpby_real(ctx(), RS_A0, 1.0, RS_C0, -cfg_.beta, RS_CH));
+    if ((rc = phi_apply_transpose(RS_CH, phit))) return rc;
   }
   if ((rc = sync())) return rc;
   const double h3 = lebesgue();
@@ -1576,6 +1645,32 @@ int RDEnsemble::objective_and_gradient(int mode, const std::vector<std::vector<d
     }
   }
   return 0;
+}
+
+// objective_and_gradient(mode, x, want_grad = true) of a store-all handle in phases, so that the
+// time steps of several handles can run as one graph: eval_begin; for i < nt: the forward step i
+// (fwd_prepare, launch, fwd_finish); eval_middle; for i = nt - 1 .. 0: the adjoint step i
+// (adj_prepare, launch, adj_finish); eval_end.  The launches are those of objective_and_gradient.
+int RDEnsemble::eval_begin(int mode, const std::vector<std::vector<double>>& x) {
+  if (!hist_all_) return fail(-6, "a phased evaluation needs the store-all history");
+  int rc = eval_setup(mode, x);
+  if (rc) return rc;
+  eval_mode_ = mode;
+  return state_begin(true);
+}
+
+int RDEnsemble::eval_middle() {
+  for (int i = 1; i <= cfg_.nt; i++) stored_[i] = i;
+  int rc = inverse_to_real(CS_X, CS_T5, RS_CT);  // the end of solveState(0)
+  if (rc) return rc;
+  GLIA_TRYC(launch_state_reset(ctx(), 2));        // the start of solveAdjoint(1)
+  return adjoint_begin();
+}
+
+int RDEnsemble::eval_end(std::vector<double>& J, std::vector<std::vector<double>>& dJ) {
+  int rc = adjoint_end();
+  if (rc) return rc;
+  return eval_result(eval_mode_, true, J, dJ);
 }
 
 // Per-axis taps of the reference's periodic smoothing kernel (computeWeierstrassFilter,

@@ -132,7 +132,43 @@ class RDEnsemble {
   int refresh_tissue(int m);                         // derived quantities from stored tissue fields
   int profile_get(int kind, double* ms, long long* count);
 
+  // --- captured time steps and the phases of a gradient evaluation (store-all history) ---
+  using StepKey = std::array<int, 6>;  // phase, step, budgets, variant flags
+  struct StepGraph {
+    hipGraph_t g = nullptr;       // the captured step, for graphs of several handles
+    hipGraphExec_t ex = nullptr;  // its executable form, replayed alone
+  };
+  bool step_graphs_on() const;
+  // one forward step / adjoint step: prepare gives the captured step to launch (null: the step
+  // runs eagerly in finish), finish reads the budgets' flags after the launch and runs the step
+  // again eagerly when a solve needed more iterations
+  int fwd_prepare(int i, bool store_half, const StepGraph** out);
+  int fwd_finish(int i, bool store_half, bool launched);
+  int adj_prepare(int i, const StepGraph** out);
+  int adj_finish(int i, bool launched);
+  // objective_and_gradient (want_grad, store-all) in phases: begin runs up to c(0) in the history,
+  // middle from c(T) through alpha(T) and the quadrature at T, end from alpha(0) to J and dJ
+  int eval_begin(int mode, const std::vector<std::vector<double>>& x);
+  int eval_middle();
+  int eval_end(std::vector<double>& J, std::vector<std::vector<double>>& dJ);
+  bool store_half() const { return hist_all_ && cfg_.adjoint_store != 0; }
+  bool store_all() const { return hist_all_; }
+  int nt() const { return cfg_.nt; }
+  // process-unique handle number and the count of step-graph cache flushes: with the address of
+  // a cached step they identify it in the key of a graph spanning handles
+  unsigned long long uid() const { return uid_; }
+  unsigned long long graph_epoch() const { return graph_epoch_; }
+
  private:
+  unsigned long long uid_ = 0;
+  unsigned long long graph_epoch_ = 0;
+  int eval_mode_ = 2;
+  int eval_setup(int mode, const std::vector<std::vector<double>>& x);
+  int eval_result(int mode, bool want_grad, std::vector<double>& J, std::vector<std::vector<double>>& dJ);
+  int state_begin(bool keep_history);
+  int adjoint_begin();
+  int adjoint_end();
+  void drop_step_graphs();
   int alloc_member(int m);
   int alloc_history(int m);
   void free_history(int m);
@@ -150,9 +186,7 @@ class RDEnsemble {
   enum { SK_FWD1 = 0, SK_FWD2, SK_ADJ1, SK_HALF, SK_ADJ2, SK_NUM };
   int cn_solve(int sx, double tau, int kind = -1);
   int capture_solve(int sx, double tau, int K, int fslot);
-  using StepKey = std::array<int, 6>;  // phase, step, budgets, variant flags
-  bool step_graphs_on() const;
-  int step_graph(const StepKey& key, const std::function<int()>& body, hipGraphExec_t* out);
+  int step_graph(const StepKey& key, const std::function<int()>& body, const StepGraph** out);
   int read_step_flags(int nsolve, int* its, bool* ok);
   int forward_step_stored(int i, bool store_half);
   int adjoint_step_stored(int i);
@@ -216,7 +250,7 @@ class RDEnsemble {
   static constexpr int kStepFlags = 3;  // host flag slots of the budgeted solves of a captured step
   hipEvent_t ev_step_ = nullptr;
   int budget_[SK_NUM] = {-1, -1, -1, -1, -1};  // iterations of the last solve of each kind (-1: none yet)
-  std::map<StepKey, hipGraphExec_t> step_graphs_;
+  std::map<StepKey, StepGraph> step_graphs_;
   long long step_reruns_ = 0;  // captured steps that ran again eagerly
   long long step_launches_ = 0;  // captured steps launched
   int margin_ = 1;             // CG iterations captured beyond the last count of a solve kind

@@ -372,6 +372,35 @@ def objective_and_gradient_group(solvers, xs, modes=None):
     return Js, [s._split(d) for s, d in zip(solvers, dJs)], ms
 
 
+def objective_and_gradient_lockstep(solvers, xs, modes=None):
+    """The same evaluation of several store-all handles in lockstep from this thread: time step i of
+    every handle runs as one graph launch once each handle has its steps captured.  J and dJ equal
+    each handle's own objective_and_gradient bit for bit.  Returns (J list, dJ list, spanned) with
+    spanned the number of time steps launched as one graph across handles."""
+    nh = len(solvers)
+    if nh == 0 or len(xs) != nh:
+        raise ValueError("one x list per solver")
+    L = solvers[0].L
+    modes = [_lib.MODE_ALL] * nh if modes is None else list(modes)
+    flats = [s._flat_x(x) for s, x in zip(solvers, xs)]
+    Js = [np.zeros(s.nmembers) for s in solvers]
+    dJs = [np.zeros_like(f) for f in flats]
+    spanned = ctypes.c_longlong(0)
+    vp = ctypes.c_void_p
+    H = (vp * nh)(*[s.h.value for s in solvers])
+    M = (ctypes.c_int * nh)(*modes)
+    X = (vp * nh)(*[f.ctypes.data for f in flats])
+    JP = (vp * nh)(*[j.ctypes.data for j in Js])
+    DP = (vp * nh)(*[d.ctypes.data for d in dJs])
+    rc = L.glia_rd_objective_and_gradient_lockstep(nh, ctypes.cast(H, vp), ctypes.cast(M, vp), ctypes.cast(X, vp),
+                                                   ctypes.cast(JP, vp), ctypes.cast(DP, vp),
+                                                   ctypes.cast(ctypes.byref(spanned), vp))
+    if rc != 0:
+        msgs = [s.L.glia_rd_last_error(s.h).decode() for s in solvers]
+        raise GliaError(rc, "; ".join(m for m in msgs if m))
+    return Js, [s._split(d) for s, d in zip(solvers, dJs)], spanned.value
+
+
 class DiffusionSolver:
     """DiffusionSolver (src/pde/DiffusionSolver.cpp): CN half step by PCG on one field."""
 

@@ -252,6 +252,20 @@ int glia_rd_hessian(glia_rd_t h, int diffusivity, const double* x, double* y);
    over independent problems. */
 int glia_rd_objective_and_gradient_group(int nh, const glia_rd_t* handles, const int* modes, const double* const* x,
                                          double* const* J, double* const* dJ, double* ms);
+/* The same evaluation for several store-all handles in lockstep, from one host thread: time step i
+   of every handle (forward, then adjoint) is one graph launch holding each handle's captured step
+   as an independent branch, where every handle has its step captured (from its second evaluation
+   on; the first steps of a handle run eagerly).  A step whose CG solve needs more iterations than
+   its capture holds runs again eagerly on that handle alone, so J and dJ equal those of
+   glia_rd_objective_and_gradient on each handle bit for bit.  Handles with different nt are
+   aligned at the start (forward) and at the end (adjoint).  Checkpointed handles are evaluated one
+   after another.  spanned_steps (may be NULL) receives the number of steps launched as one graph
+   across handles.  After an error the handles hold a partial evaluation.  No single reference
+   counterpart (it batches evaluateObjectiveAndGradient, src/grad/DerivativeOperatorsRD.cpp:130-226,
+   over independent problems). */
+int glia_rd_objective_and_gradient_lockstep(int nh, const glia_rd_t* handles, const int* modes,
+                                            const double* const* x, double* const* J, double* const* dJ,
+                                            long long* spanned_steps);
 
 int glia_rd_stats(glia_rd_t h, int member, glia_rd_stats_t* out);
 int glia_rd_reset_stats(glia_rd_t h);

@@ -133,3 +133,84 @@ def test_autotune_choice_cache_and_results(members, tmp_path, oracle_mod):
     Jo, go = o.objective_and_gradient(x[0], mode=2)
     assert abs(Jt[0] - Jo) / abs(Jo) < 1e-5
     assert rel(gt[0], go) < 1e-4
+
+
+def test_lockstep_group_equals_separate_bitwise():
+    """Handles of different grids, member counts, nt and history modes evaluated in lockstep
+    (glia_rd_objective_and_gradient_lockstep: time step i of every handle in one graph) give J and
+    dJ bit for bit equal to each handle evaluated alone, on three evaluations in a row; from the
+    second evaluation on every time step runs as one graph across handles.  A handle whose captured
+    steps all run again eagerly (GLIA_STEP_MARGIN = -1) keeps the equality, and a checkpointed
+    handle makes the call evaluate one handle after another."""
+    print("case lockstep", flush=True)
+    from glia_amd import rd, synth
+    specs = (  # n, seed, members, nt, adjoint_store, env
+        (32, 5, 2, 6, False, None),
+        (64, 6, 1, 6, True, None),
+        (96, 7, 1, 4, False, {"GLIA_STEP_MARGIN": "-1"}),
+    )
+    problems = [[synth.problem(n, seed + m, nbases=4, nnz=None) for m in range(nm)] for n, seed, nm, *_ in specs]
+    xs = [[np.concatenate([pr["p"], [pr["kappa"], pr["rho"]]]) for pr in members] for members in problems]
+
+    def handles(checkpoint_last=False):
+        out = []
+        for k, ((n, _, nm, nt, ast, env), members) in enumerate(zip(specs, problems)):
+            old = {}
+            for key, v in (env or {}).items():
+                old[key] = os.environ.get(key)
+                os.environ[key] = v
+            try:
+                cp = 3 if (checkpoint_last and k == len(specs) - 1) else 0
+                s = rd.RDSolver(n, nmembers=nm, dt=0.05, nt=nt, k_gm_wm=0.2, r_gm_wm=0.2, adjoint_store=ast,
+                                checkpoint_slots=cp)
+            finally:
+                for key, v in old.items():
+                    if v is None:
+                        os.environ.pop(key, None)
+                    else:
+                        os.environ[key] = v
+            for m, pr in enumerate(members):
+                s.set_tissue(m, pr["wm"], pr["gm"], pr["csf"], pr["vt"])
+                s.set_coefficients(m, pr["kappa"], pr["rho"])
+                s.set_basis(m, pr["centers"], pr["sigma"])
+                s.set_data(m, (pr["wm"] > 0.5).astype(np.float32), None, -0.99)
+            out.append(s)
+        return out
+
+    def check(ref, got):
+        for (J0, g0), (J1, g1) in zip(ref, got):
+            assert np.array_equal(J0, J1)
+            for a, b in zip(g0, g1):
+                assert np.array_equal(a, b)
+
+    # each handle alone, three evaluations (x moved a little each time)
+    def x_at(t):
+        return [[x * (1.0 + 0.01 * t) for x in xl] for xl in xs]
+
+    alone = [[] for _ in specs]
+    for k, s in enumerate(handles()):
+        for t in range(3):
+            alone[k].append(s.objective_and_gradient(x_at(t)[k]))
+        s.close()
+    group = handles()
+    spans = []
+    for t in range(3):
+        Jl, gl, spanned = rd.objective_and_gradient_lockstep(group, x_at(t))
+        spans.append(spanned)
+        print("evaluation", t, "steps spanning handles", spanned, flush=True)
+        for k in range(len(specs)):
+            check([alone[k][t]], [(Jl[k], gl[k])])
+    launched, reruns, _ = group[2].step_info()
+    print("margin -1 handle: launched", launched, "reruns", reruns, flush=True)
+    assert reruns > 0
+    assert spans[0] < 2 * 6 and spans[1] == 2 * 6 and spans[2] == 2 * 6
+    for s in group:
+        s.close()
+    # a checkpointed handle: one handle after another, the same results
+    group = handles(checkpoint_last=True)
+    Jl, gl, spanned = rd.objective_and_gradient_lockstep(group, x_at(0))
+    assert spanned == 0
+    for k in range(2):
+        check([alone[k][0]], [(Jl[k], gl[k])])
+    for s in group:
+        s.close()
