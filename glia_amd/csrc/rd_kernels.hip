@@ -541,6 +541,61 @@ k_yf_div_s(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__
   }
 }
 
+// Y pass, forward, divergence split over blocks: block 2 i transforms Hx -> Yx = FFT_y(Hx) for tile
+// i, block 2 i + 1 computes Rr = FFT_y(Q) + i w_y FFT_y(Hy) for the same tile.  A thread holds at
+// most two fields' values (k_yf_div holds three), so more waves fit per SIMD, and no exchange
+// between the halves is needed.  Each output value takes the operations of k_yf_div.  MINW is the
+// occupancy asked of the compiler (at 256 points, 3 waves per SIMD costs a few spilled registers;
+// the autotuner times both bounds).
+template <int L, int MINW>
+__global__ void __launch_bounds__(CK<L>::TPB, CK<L>::VPT <= 16 ? MINW : 2)
+k_yf_div_b(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_hx, int s_hy, int s_q, int s_yx,
+           int s_rr) {
+  const MemberDev& M = mem[blockIdx.y];
+  if (M.st->done) return;
+  GLIA_COL_PRELUDE(L)
+  const Tile tl = col_tile_at(1, d, (int)(blockIdx.x >> 1));
+  __shared__ float2 twl[L];
+  if ((blockIdx.x & 1) == 0) {  // uniform per block
+    GPtr Hx = cptr(M, s_hx) + tl.base;
+    GPtr Yx = cptr(M, s_yx) + tl.base;
+    float2 a[K::VPT];
+#pragma unroll
+    for (int j = 0; j < K::VIN; j++) a[j] = Hx[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
+    stage_twiddles<L, K::TPB>(twl, tw);
+    fft2s<R1, R2, NT, -1>(a, exc, t, twl);
+#pragma unroll
+    for (int j = 0; j < K::VOUT; j++) if (K::out_ok(t, j)) Yx[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
+    return;
+  }
+  GPtr Hy = cptr(M, s_hy) + tl.base;
+  GPtr Q = cptr(M, s_q) + tl.base;
+  GPtr Rr = cptr(M, s_rr) + tl.base;
+  float2 a[K::VPT], b[K::VPT];
+  // Hy is loaded up front for short plans, else after the transform of Q (fewer live values)
+  constexpr bool early = K::VPT <= 8;
+#pragma unroll
+  for (int j = 0; j < K::VIN; j++) {
+    const unsigned g = (unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col;
+    a[j] = Q[g];
+    if (early) b[j] = Hy[g];
+  }
+  stage_twiddles<L, K::TPB>(twl, tw);
+  fft2s<R1, R2, NT, -1>(a, exc, t, twl);
+  if (!early) {
+#pragma unroll
+    for (int j = 0; j < K::VIN; j++) b[j] = Hy[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
+  }
+  __syncthreads();
+  fft2s<R1, R2, NT, -1>(b, exc, t, twl);
+#pragma unroll
+  for (int j = 0; j < K::VOUT; j++) {
+    if (!K::out_ok(t, j)) continue;
+    const int row = out_row<R1, R2, NT>(t, j);
+    Rr[(unsigned)row * tl.rs + col] = cadd(a[j], mul_i(cscale(b[j], wnum(row, d.n1))));
+  }
+}
+
 // X pass, forward, closing the divergence in two groups: Yx -> i w_x FFT_x(Yx) into LDS; Rr ->
 // D = (that term) + FFT_x(Rr), then the MODE's output and partial sums as in k_xf_div.
 template <int L, int MODE>
@@ -1671,9 +1726,15 @@ template <int L>
 struct RunYfDiv {
   static int run(const LaunchCtx& c, int s_hx, int s_hy, int s_q, int s_yx, int s_rr) {
     dim3 g(c.d.n0 * (c.d.m2 / 16), c.nm);
-    if (c.yf_split)
+    if (c.yf_split == 1) {
       k_yf_div_s<L><<<g, 3 * CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw1, s_hx, s_hy, s_q, s_yx, s_rr);
-    else
+    } else if (c.yf_split == 2) {
+      dim3 g2(2 * g.x, g.y);
+      k_yf_div_b<L, 3><<<g2, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw1, s_hx, s_hy, s_q, s_yx, s_rr);
+    } else if (c.yf_split == 3) {
+      dim3 g2(2 * g.x, g.y);
+      k_yf_div_b<L, 2><<<g2, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw1, s_hx, s_hy, s_q, s_yx, s_rr);
+    } else
       k_yf_div<L><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw1, s_hx, s_hy, s_q, s_yx, s_rr);
     GLIA_CHECK_LAUNCH();
     return 0;

@@ -101,13 +101,21 @@ int RDEnsemble::init() {
   // GLIA_STEP_MARGIN < 0 makes every captured step run again eagerly (a check of that path)
   if (const char* v = std::getenv("GLIA_STEP_MARGIN")) margin_ = std::atoi(v);
   // forward divergence variants: the X pass in two thread groups (measured faster at 256 points),
-  // the Y pass in one; GLIA_SPLIT_DIV (none, xf, yf, both) overrides, glia_rd_autotune measures
+  // the Y pass in one.  GLIA_SPLIT_DIV overrides with a comma-separated list of none, xf (X pass
+  // in two groups), yf (Y pass in three groups), yfb / yfb2 (Y pass split over blocks at 3 / 2
+  // waves per SIMD), both (xf and yf); glia_rd_autotune measures them.
   xf_split_ = 1;
   yf_split_ = 0;
   if (const char* v = std::getenv("GLIA_SPLIT_DIV")) {
-    const std::string sv(v);
-    yf_split_ = (sv == "yf" || sv == "both") ? 1 : 0;
-    xf_split_ = (sv == "xf" || sv == "both") ? 1 : 0;
+    xf_split_ = 0;
+    std::istringstream toks(v);
+    std::string tok;
+    while (std::getline(toks, tok, ',')) {
+      if (tok == "xf" || tok == "both") xf_split_ = 1;
+      if (tok == "yf" || tok == "both") yf_split_ = 1;
+      if (tok == "yfb") yf_split_ = 2;
+      if (tok == "yfb2") yf_split_ = 3;
+    }
   }
   if (!supported_line(d_.n0) || !supported_line(d_.n1) || !supported_z(d_.n2))
     return fail(-4, "unsupported grid: n0, n1 in {32,48,64,80,96,128,160,192,256}, n2 in {32,64,96,128,160,192,256}");
@@ -1425,11 +1433,12 @@ int RDEnsemble::adjoint_end() {
 // ---------------------------------------------------------------------------------------------
 // Autotuning of the kernel variants
 // ---------------------------------------------------------------------------------------------
-// The variants of the two forward divergence passes (one thread group holding every field, or one
-// group per field) are timed on this handle's own grid and members with HIP events: two warm-up
-// launches, then the median of five.  The choice is kept in a text cache (one line per device,
-// grid and member count: "<arch> n0 n1 n2 nm xf_split yf_split xf_ms0 xf_ms1 yf_ms0 yf_ms1") that
-// later handles read instead of timing again.
+// The variants of the two forward divergence passes are timed on this handle's own grid and
+// members with HIP events: two warm-up launches, then the median of five.  X pass: one thread group
+// holding both fields (0) or one group per field (1).  Y pass: one group holding three fields (0),
+// one group per field (1), or the fields split over blocks at 3 (2) or 2 (3) waves per SIMD.  The
+// choice is kept in a text cache (one line per device, grid and member count: "<arch> n0 n1 n2 nm
+// xf yf xf_ms0 xf_ms1 yf_ms0 yf_ms1 yf_ms2 yf_ms3") that later handles read instead of timing again.
 int RDEnsemble::autotune(const char* cache_path, int* choice) {
   hipDeviceProp_t prop;
   int dev = 0;
@@ -1447,7 +1456,7 @@ int RDEnsemble::autotune(const char* cache_path, int* choice) {
       if (line.compare(0, key.str().size() + 1, key.str() + " ") != 0) continue;
       std::istringstream ls(line.substr(key.str().size() + 1));
       int xf = -1, yf = -1;
-      if (ls >> xf >> yf && (xf == 0 || xf == 1) && (yf == 0 || yf == 1)) {
+      if (ls >> xf >> yf && (xf == 0 || xf == 1) && yf >= 0 && yf < kYfVariants) {
         xf_split_ = xf;
         yf_split_ = yf;
         if (choice) { choice[0] = xf; choice[1] = yf; }
@@ -1477,14 +1486,14 @@ int RDEnsemble::autotune(const char* cache_path, int* choice) {
     *ms = t[2];
     return 0;
   };
-  double xf_ms[2], yf_ms[2];
+  double xf_ms[2], yf_ms[kYfVariants];
   for (int v = 0; v < 2 && !rc; v++) {
     xf_split_ = v;
     rc = time_it([&]() { return launch_xf_div(ctx(), XFM_MATVEC, CS_T0, CS_T1, CS_P, CS_W); }, &xf_ms[v]);
   }
-  for (int v = 0; v < 2 && !rc; v++) {
+  for (int v = 0; v < kYfVariants && !rc; v++) {
     yf_split_ = v;
-    if (!rc) rc = time_it([&]() { return launch_yf_div(ctx(), CS_T2, CS_T3, CS_T4, CS_T0, CS_T1); }, &yf_ms[v]);
+    rc = time_it([&]() { return launch_yf_div(ctx(), CS_T2, CS_T3, CS_T4, CS_T0, CS_T1); }, &yf_ms[v]);
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
@@ -1492,12 +1501,13 @@ int RDEnsemble::autotune(const char* cache_path, int* choice) {
   if (!rc) rc = sync();
   if (rc) return rc;
   xf_split_ = xf_ms[1] < xf_ms[0] ? 1 : 0;
-  yf_split_ = yf_ms[1] < yf_ms[0] ? 1 : 0;
+  yf_split_ = (int)(std::min_element(yf_ms, yf_ms + kYfVariants) - yf_ms);
   if (choice) { choice[0] = xf_split_; choice[1] = yf_split_; }
   if (cache_path) {
     std::ofstream out(cache_path, std::ios::app);
-    out << key.str() << ' ' << xf_split_ << ' ' << yf_split_ << ' ' << xf_ms[0] << ' ' << xf_ms[1] << ' ' << yf_ms[0] << ' '
-        << yf_ms[1] << '\n';
+    out << key.str() << ' ' << xf_split_ << ' ' << yf_split_ << ' ' << xf_ms[0] << ' ' << xf_ms[1];
+    for (int v = 0; v < kYfVariants; v++) out << ' ' << yf_ms[v];
+    out << '\n';
   }
   return 0;
 }

@@ -286,11 +286,13 @@ enum {
    counterpart: the reference launches every kernel from the host. */
 int glia_rd_use_graphs(glia_rd_t h, int enable);
 /* Autotuning of the kernel variants (no reference counterpart: the reference leaves plans to
-   cuFFT): the two variants of each forward divergence pass (one thread group holding every
-   field, or one group per field) are timed on this handle's grid and members and the faster is
-   kept.  cache_path (may be NULL) names a text file of earlier choices, read first and appended
-   to.  choice (2 ints, may be NULL) receives (X-pass split, Y-pass split).  Call before the
-   first solve; the results do not change beyond FP32 rounding. */
+   cuFFT): the variants of each forward divergence pass are timed on this handle's grid and
+   members and the fastest is kept.  X pass: 0 = one thread group holding both fields, 1 = one
+   group per field.  Y pass: 0 = one group holding three fields, 1 = one group per field, 2 and 3 =
+   the fields split over blocks at 3 and 2 waves per SIMD.  cache_path (may be NULL) names a text
+   file of earlier choices, read first and appended to.  choice (2 ints, may be NULL) receives
+   (X-pass variant, Y-pass variant).  Call before the first solve; the results do not change
+   beyond FP32 rounding. */
 int glia_rd_autotune(glia_rd_t h, const char* cache_path, int* choice);
 /* Time steps replayed from captured graphs (glia_rd_use_graphs on, store-all history): each
    forward and adjoint step runs as one graph whose CG loops are unrolled to the last iteration

@@ -169,3 +169,46 @@ def test_headline_grid_gradient_vs_oracle_fixture(rd, name):
     go = gd["dJ"]
     for k in (-2, -1):
         assert abs(g[0][k] - go[k]) / max(abs(go[k]), 1e-2 * np.linalg.norm(go)) < 1e-4
+
+
+@pytest.mark.parametrize("split", ["none", "xf", "yf", "yfb", "yfb2"])
+@pytest.mark.parametrize("n", [(32, 256, 32), (256, 32, 32), (32, 128, 32)])
+def test_divergence_variants_headline_plans_vs_oracle(rd, oracle_mod, n, split):
+    """Every variant of the forward divergence passes the autotuner may choose (GLIA_SPLIT_DIV) on
+    the long-axis grids that run it at 128 or 256 points: applyD within 1e-5 and J, dJ within the
+    oracle bars (nt = 2)."""
+    import os
+    print("case variant", n, split, flush=True)
+    dt, nt = 0.02, 2
+    pr = problem(n, 25)
+    o = oracle_for(oracle_mod, pr, dt, nt)
+    rng = np.random.default_rng(6)
+    c = o.phi_apply(rng.uniform(0.2, 1.0, 6))
+    ref_d = o.apply_d(c).reshape(n)
+    o.set_c0(o.phi_apply(pr["p"] * 1.3))
+    o.solve_state(0)
+    d1 = (o.ct() > 0.4).astype(np.float64)
+    x = np.concatenate([pr["p"], [pr["kappa"], pr["rho"]]])
+    o.set_data(d1, None, -0.99)
+    o.set_opt(beta=1e-3)
+    o.set_time(dt, nt)
+    Jo, go = o.objective_and_gradient(x, mode=2)
+    old = os.environ.get("GLIA_SPLIT_DIV")
+    os.environ["GLIA_SPLIT_DIV"] = split
+    try:
+        s = solver_for(rd, pr, dt, nt, beta=1e-3)
+    finally:
+        if old is None:
+            os.environ.pop("GLIA_SPLIT_DIV", None)
+        else:
+            os.environ["GLIA_SPLIT_DIV"] = old
+    s.set_field(0, "work", c)
+    s.apply_d("work", "ct")
+    ed = rel(s.get_field(0, "ct"), ref_d)
+    s.set_data(0, d1, None, -0.99)
+    J, g = s.objective_and_gradient([x])
+    s.close()
+    print("applyD %.2e J %.2e dJ %.2e" % (ed, abs(J[0] - Jo) / abs(Jo), rel(g[0], go)), flush=True)
+    assert ed < 1e-5
+    assert abs(J[0] - Jo) / abs(Jo) < 1e-5
+    assert rel(g[0], go) < 1e-4

@@ -102,16 +102,16 @@ def test_autotune_choice_cache_and_results(members, tmp_path, oracle_mod):
     s = make(members)
     choice = s.autotune(str(cache))
     print("choice", choice, open(cache).read(), flush=True)
-    assert choice[0] in (0, 1) and choice[1] in (0, 1)
+    assert choice[0] in (0, 1) and choice[1] in (0, 1, 2, 3)
     lines = open(cache).read().splitlines()
-    assert len(lines) == 1 and len(lines[0].split()) == 11
+    assert len(lines) == 1 and len(lines[0].split()) == 13
     Jt, gt = s.objective_and_gradient(x)
     s.close()
     s = make(members)
     assert s.autotune(str(cache)) == choice
     assert len(open(cache).read().splitlines()) == 1
     s.close()
-    for split in ("none", "xf", "yf", "both"):
+    for split in ("none", "xf", "yf", "both", "yfb", "xf,yfb2"):
         s = make(members, {"GLIA_SPLIT_DIV": split})
         J, g = s.objective_and_gradient(x)
         s.close()

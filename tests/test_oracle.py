@@ -291,6 +291,62 @@ def test_phi_unsmoothed_is_masked_truncated_gaussian(oracle_mod):
     assert abs(o.phi(0).max() - 1.0) < 1e-15
 
 
+def test_phi_on_the_fly_normalization_closed_form(oracle_mod):
+    """Phi::apply without phi_store divides by the largest maximum among the ACTIVE bases and
+    applyTranspose by the largest among all bases (src/mat/Phi.cpp:313-439).  Unsmoothed bases with
+    hand-known maxima pin both: A on a grid point (max 1), B half a cell off (max e^(-1/32)), C
+    inside a 3-cell hole of the mask, so its largest value sits 2 cells out (max e^(-1/2))."""
+    n = 32
+    h = 2 * np.pi / n
+    sig = 2 * h
+    o = oracle_mod.Oracle(n)
+    wm = np.ones((n, n, n))
+    wm[23:26, 23:26, 23:26] = 0.0  # mask hole around C
+    o.set_tissue(wm, None, None, None)
+    ctr = np.array([[16 * h, 16 * h, 16 * h], [8 * h, 8.5 * h, 8 * h], [24 * h, 24 * h, 24 * h]])
+    o.set_phi(ctr, sig, 0.0, False)
+    X = np.arange(n) * h
+    mask = (wm > 0.1).astype(np.float64)
+    g = []
+    for c in ctr:
+        r = np.sqrt((X[:, None, None] - c[0]) ** 2 + (X[None, :, None] - c[1]) ** 2 + (X[None, None, :] - c[2]) ** 2)
+        gi = np.exp(-r ** 2 / (2 * sig ** 2)) * mask
+        gi[r / sig > 5] = 0
+        g.append(gi)
+    maxima = (1.0, np.exp(-1.0 / 32.0), np.exp(-0.5))
+    for gi, m in zip(g, maxima):
+        assert abs(gi.max() - m) < 1e-15
+    for p, norm in (([0, 0, 0.7], maxima[2]), ([0, 0.4, 0.7], maxima[1]), ([0.2, 0.4, 0.7], 1.0)):
+        want = sum(pi * gi for pi, gi in zip(p, g)) / norm
+        assert rel(o.phi_apply(np.array(p, dtype=np.float64)), want) < 1e-13
+    assert np.all(o.phi_apply(np.zeros(3)) == 0.0)
+    v = np.random.default_rng(2).standard_normal((n, n, n))
+    want_t = np.array([np.sum(gi * v) for gi in g]) / 1.0
+    assert rel(o.phi_applyT(v), want_t) < 1e-12
+    # without basis A the largest maximum over all bases is B's
+    o.set_phi(ctr[1:], sig, 0.0, False)
+    assert rel(o.phi_applyT(v), want_t[1:] / maxima[1]) < 1e-12
+
+
+def test_kbar_is_sum_of_k_over_mask_count(oracle_mod):
+    """DiffCoef::setValues' kxx_avg_ is sum(k) over the whole grid divided by the number of mask
+    voxels (src/mat/DiffCoef.cpp:103-116), so k outside the mask still counts in the numerator.  Half
+    the grid white matter (mask 1, k = ks), a quarter ventricle with CSF 0.5 (mask 0 as vt >= 0.8,
+    k = ks * k_glm * 0.5), a quarter empty: kbar = ks (1 + 0.25 k_glm)."""
+    n = 16
+    o = oracle_mod.Oracle(n)
+    wm = np.zeros((n, n, n)); csf = np.zeros((n, n, n)); vt = np.zeros((n, n, n))
+    wm[: n // 2] = 1.0
+    csf[n // 2: 3 * n // 4] = 0.5
+    vt[n // 2: 3 * n // 4] = 0.9
+    o.set_tissue(wm, None, csf, vt)
+    ks, kglm = 0.1, 0.2
+    o.set_k(ks, 0.0, kglm)
+    f = o.fields()
+    assert f["filter"].sum() == n ** 3 // 2
+    assert abs(f["kavg"] - ks * (1 + 0.25 * kglm)) < 1e-15
+
+
 def test_phi_mesh_counts(oracle_mod):
     for np_, want in ((1, 1), (8, 8), (27, 27), (64, 64)):
         c = oracle_mod.phi_mesh(np_, np.array([3.0, 3.0, 3.0]), 0.1, 1.5)
