@@ -259,7 +259,7 @@ class MeParams(ctypes.Structure):
 
 class MeInfo(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int), ("cfl_stop", ctypes.c_int), ("cfl_max", ctypes.c_double),
-                ("el_iterations", ctypes.c_int), ("diff_iterations", ctypes.c_int)]
+                ("el_iterations", ctypes.c_int), ("diff_iterations", ctypes.c_int), ("volume_stop", ctypes.c_int)]
 
 
 ME_SIGNATURES = {
@@ -271,6 +271,9 @@ ME_SIGNATURES = {
     "glia_me_el": (_vp, [_vp]),
     "glia_me_sl": (_vp, [_vp]),
     "glia_me_solve_state": (_i, [_vp, ctypes.POINTER(MeParams)] + [_vp] * 9 + [ctypes.POINTER(MeInfo)]),
+    "glia_me_solve_state_features": (_i, [_vp, ctypes.POINTER(MeParams)] + [_vp] * 9 + [ctypes.POINTER(MeInfo), _vp, _i,
+                                                                                        ctypes.POINTER(_i)]),
+    "glia_me_feature_name": (ctypes.c_char_p, [_i]),
     "glia_me_solve_state_multispecies": (_i, [_vp, ctypes.POINTER(MeParams), ctypes.POINTER(MsParams)] + [_vp] * 12
                                          + [ctypes.POINTER(MeInfo)]),
     "glia_me_multispecies_outputs": (_i, [_vp, _vp, _vp, _i]),

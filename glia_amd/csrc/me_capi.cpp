@@ -12,6 +12,7 @@
 
 #include "../../include/glia_me.h"
 #include "../../include/glia_ms.h"
+#include "me_features.h"
 #include "me_launch.h"
 #include "rd_handle.h"
 
@@ -27,6 +28,12 @@ struct glia_me_s {
   float* work = nullptr;
   bool have_ms_outputs = false;  // edema and segmentation of a multispecies solve are in work
   std::vector<float> hpart;
+  // biophysical features (allocated on the first call that asks for them): kFeatFields fields,
+  // the device sums of one step, and the scratch of the reductions and sorts
+  float* feat = nullptr;
+  double* acc = nullptr;
+  void* fscratch = nullptr;
+  std::vector<double> hacc;
   std::string err;
 };
 
@@ -50,6 +57,12 @@ double cfl_limit(const glia_me_params* p, double model_default) {
   return p->max_cfl > 0 ? p->max_cfl : model_default;
 }
 
+// feature fields: seg, three components (grad c, then c v), |grad c|, D c, dc/dt, |u|, |v|, grad u
+// (9), mu, lam, det F, tr S, max shear, two sort key arrays, c* (one value)
+enum { FS_SEG = 0, FS_G = 1, FS_GRADC = 4, FS_DC, FS_DCDT, FS_UMAG, FS_VMAG, FS_GU = 9, FS_MU = 18, FS_LAM, FS_JAC,
+       FS_TR, FS_TAU, FS_KEYS, FS_KEYS2, FS_CSTAR, kFeatFields };
+constexpr int kFeatures = 8;  // c, dcdt, gradc, u, jac, trT, tau, vel
+
 }  // namespace
 
 #define ME_HIP(h, call)                                                                         \
@@ -65,6 +78,127 @@ double cfl_limit(const glia_me_params* p, double model_default) {
     if (r_) return fail(h, r_, std::string(#call) + ": " + (lasterr));       \
   } while (0)
 
+namespace {
+
+// computeGradient of a device field through the engine's transforms (member 0; the engine's
+// alpha(0) field and its transform work slots serve as scratch)
+int feat_gradient(glia_me_s* h, const float* f, float* const g[3]) {
+  glia::RDEnsemble* e = h->rd->e.get();
+  float* ra = e->field_ptr(0, glia::RS_A0);
+  const size_t fb = sizeof(float) * h->ng;
+  ME_HIP(h, hipMemcpyAsync(ra, f, fb, hipMemcpyDeviceToDevice, h->st));
+  ME_SUB(h, e->fft_r2c(glia::RS_A0, glia::CS_T0), e->error());
+  const int slot = glia::CS_T0;
+  for (int d = 0; d < 3; d++) {
+    ME_SUB(h, e->spec_lin(1, &slot, &d, glia::CS_T1), e->error());
+    ME_SUB(h, e->fft_c2r(glia::CS_T1, glia::RS_A0), e->error());
+    ME_HIP(h, hipMemcpyAsync(g[d], ra, fb, hipMemcpyDeviceToDevice, h->st));
+  }
+  return 0;
+}
+
+// computeDivergence of three device fields into out (out may be x[0])
+int feat_divergence(glia_me_s* h, const float* const x[3], float* out) {
+  glia::RDEnsemble* e = h->rd->e.get();
+  float* ra = e->field_ptr(0, glia::RS_A0);
+  const size_t fb = sizeof(float) * h->ng;
+  const int slots[3] = {glia::CS_T0, glia::CS_T1, glia::CS_T2}, dirs[3] = {0, 1, 2};
+  for (int d = 0; d < 3; d++) {
+    ME_HIP(h, hipMemcpyAsync(ra, x[d], fb, hipMemcpyDeviceToDevice, h->st));
+    ME_SUB(h, e->fft_r2c(glia::RS_A0, slots[d]), e->error());
+  }
+  ME_SUB(h, e->spec_lin(3, slots, dirs, glia::CS_T3), e->error());
+  ME_SUB(h, e->fft_c2r(glia::CS_T3, glia::RS_A0), e->error());
+  ME_HIP(h, hipMemcpyAsync(out, ra, fb, hipMemcpyDeviceToDevice, h->st));
+  return 0;
+}
+
+int feat_alloc(glia_me_s* h) {
+  if (h->feat) return 0;
+  const int64_t ng = h->ng;
+  if (hipMalloc((void**)&h->feat, sizeof(float) * ((int64_t)(kFeatFields - 1) * ng + 64)) != hipSuccess ||
+      hipMalloc((void**)&h->acc, sizeof(double) * kFeatures * glia::mef::kPerField) != hipSuccess ||
+      hipMalloc(&h->fscratch, glia::mef::scratch_bytes(ng)) != hipSuccess) {
+    (void)hipGetLastError();
+    if (h->feat) (void)hipFree(h->feat);
+    if (h->acc) (void)hipFree(h->acc);
+    h->feat = nullptr;
+    h->acc = nullptr;
+    h->fscratch = nullptr;
+    return fail(h, (int)hipErrorOutOfMemory, "out of device memory for the feature fields");
+  }
+  h->hacc.assign(kFeatures * glia::mef::kPerField, 0.0);
+  return 0;
+}
+
+// PdeOperatorsMassEffect::computeBiophysicalFeatures (src/pde/PdeOperatorsMassEffect.cpp:323-418) at
+// time step i: one row of GLIA_ME_NFEATURES values (the reference's CSV row) and the tumor ratio
+// tc / (tc + healthy) in single precision, as the loop's stopping rule reads it
+int feat_step(glia_me_s* h, int i, double dt, const float* c, const float* wm, const float* gm, const float* vt,
+              const float* csf, const float* bg, float* const u[3], float* const v[3], double* row, float* ratio) {
+  int rc = feat_alloc(h);
+  if (rc) return rc;
+  glia::RDEnsemble* e = h->rd->e.get();
+  const int64_t ng = h->ng;
+  const hipStream_t st = h->st;
+  auto F = [&](int k) { return h->feat + (int64_t)k * ng; };
+  float* G[3] = {F(FS_G), F(FS_G + 1), F(FS_G + 2)};
+  float* cstar = F(FS_CSTAR);
+  // Tumor::computeSegmentation, then tc = [seg == 1] and healthy = [seg not in {0, 1}]
+  ME_SUB(h, glia_ms_segmentation(ng, bg, gm, wm, vt, csf, c, F(FS_SEG), (void*)st), "segmentation");
+  // computeTumorQuants: |grad c|, dc/dt = D c + rho (c - c^2) - div(c v), c* (:214-259)
+  if ((rc = feat_gradient(h, c, G))) return rc;
+  ME_HIP(h, (hipError_t)glia::mef::launch_mag3(st, ng, G[0], G[1], G[2], F(FS_GRADC)));
+  float* ra = e->field_ptr(0, glia::RS_A0);
+  ME_HIP(h, hipMemcpyAsync(ra, c, sizeof(float) * ng, hipMemcpyDeviceToDevice, st));
+  ME_SUB(h, e->apply_d_field(glia::RS_A0, glia::RS_A0), e->error());
+  ME_HIP(h, hipMemcpyAsync(F(FS_DC), ra, sizeof(float) * ng, hipMemcpyDeviceToDevice, st));
+  ME_HIP(h, (hipError_t)glia::mef::launch_flux(st, ng, c, v, G));
+  if ((rc = feat_divergence(h, G, G[0]))) return rc;
+  ME_HIP(h, (hipError_t)glia::mef::launch_dcdt(st, ng, F(FS_DC), c, e->field_ptr(0, glia::RS_RHO), G[0], F(FS_DCDT)));
+  ME_HIP(h, (hipError_t)glia::mef::launch_cstar(st, ng, F(FS_GRADC), F(FS_SEG), c, h->fscratch, cstar));
+  // |u|, |v|, grad u and the stress quantities with the elasticity solver's current mu and lam
+  ME_HIP(h, (hipError_t)glia::mef::launch_mag3(st, ng, u[0], u[1], u[2], F(FS_UMAG)));
+  ME_HIP(h, (hipError_t)glia::mef::launch_mag3(st, ng, v[0], v[1], v[2], F(FS_VMAG)));
+  glia::mef::StressIn gi;
+  for (int a = 0; a < 3; a++) {
+    float* ga[3] = {F(FS_GU + 3 * a), F(FS_GU + 3 * a + 1), F(FS_GU + 3 * a + 2)};
+    if ((rc = feat_gradient(h, u[a], ga))) return rc;
+    for (int b = 0; b < 3; b++) gi.d[3 * a + b] = ga[b];
+  }
+  ME_SUB(h, glia_el_get_material(h->el, F(FS_MU), F(FS_LAM), nullptr, nullptr, GLIA_RD_DEVICE), glia_el_last_error(h->el));
+  ME_HIP(h, (hipError_t)glia::mef::launch_stress(st, ng, gi, F(FS_MU), F(FS_LAM), F(FS_JAC), F(FS_TR), F(FS_TAU)));
+  // writeStats of c, dcdt, |grad c|, |u|, det F, tr S, max shear, |v|
+  const float* fields[kFeatures] = {c, F(FS_DCDT), F(FS_GRADC), F(FS_UMAG), F(FS_JAC), F(FS_TR), F(FS_TAU), F(FS_VMAG)};
+  for (int k = 0; k < kFeatures; k++)
+    ME_HIP(h, (hipError_t)glia::mef::launch_stats(st, ng, fields[k], F(FS_SEG), c, cstar, h->fscratch, F(FS_KEYS),
+                                                   F(FS_KEYS2), h->acc, k * glia::mef::kPerField));
+  ME_HIP(h, hipMemcpyAsync(h->hacc.data(), h->acc, sizeof(double) * h->hacc.size(), hipMemcpyDeviceToHost, st));
+  ME_HIP(h, hipStreamSynchronize(st));
+  const double h3 = (2.0 * M_PI / h->n[0]) * (2.0 * M_PI / h->n[1]) * (2.0 * M_PI / h->n[2]);
+  const double sa_measure = h3 * h->n[0] / (2.0 * M_PI);
+  const double* a0 = h->hacc.data();
+  row[0] = i * dt;
+  row[1] = a0[4] / (a0[5] + a0[4]);
+  *ratio = (float)a0[4] / ((float)a0[5] + (float)a0[4]);
+  for (int k = 0; k < kFeatures; k++) {
+    const double* a = a0 + k * glia::mef::kPerField;
+    double* r = row + 2 + 16 * k;
+    r[0] = a[0] * h3;
+    r[1] = a[1] * h3;
+    r[2] = a[2] * h3;
+    r[3] = a[3] * sa_measure;
+    r[4] = std::sqrt(a[6] / (double)(ng - 1));
+    r[5] = std::sqrt(a[7] / (a[4] - 1.0));
+    r[6] = std::sqrt(a[8] / (a[5] - 1.0));
+    for (int q = 0; q < 3; q++)
+      for (int reg = 0; reg < 3; reg++) r[7 + 3 * q + reg] = a[9 + 3 * reg + q];
+  }
+  return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 void glia_me_params_init(glia_me_params* p) {
@@ -79,6 +213,9 @@ int glia_me_destroy(glia_me_t h) {
   if (!h) return GLIA_RD_ERR_ARG;
   if (h->st) (void)hipStreamSynchronize(h->st);
   if (h->work) (void)hipFree(h->work);
+  if (h->feat) (void)hipFree(h->feat);
+  if (h->acc) (void)hipFree(h->acc);
+  if (h->fscratch) (void)hipFree(h->fscratch);
   if (h->sl) glia_sl_destroy(h->sl);
   if (h->el) glia_el_destroy(h->el);
   if (h->rd) glia_rd_destroy(h->rd);
@@ -128,7 +265,36 @@ glia_sl_t glia_me_sl(glia_me_t h) { return h ? h->sl : nullptr; }
 
 int glia_me_solve_state(glia_me_t h, const glia_me_params* p, float* c, float* wm, float* gm, float* vt, float* csf,
                         float* mri, const float* bg, float* const u[3], float* const v[3], glia_me_info* info) {
+  return glia_me_solve_state_features(h, p, c, wm, gm, vt, csf, mri, bg, u, v, info, nullptr, 0, nullptr);
+}
+
+const char* glia_me_feature_name(int k) {
+  static const std::vector<std::string> names = []() {
+    std::vector<std::string> out = {"t", "volc/volb"};
+    const char* feats[kFeatures] = {"c", "dcdt", "gradc", "u", "jac", "trT", "tau", "vel"};
+    const char* stats[6] = {"vol", "sa", "std", "quartmed", "95", "99"};
+    for (const char* f : feats)
+      for (const char* st : stats) {
+        if (std::string(st) == "sa") {
+          out.push_back(std::string(st) + "_" + f);
+        } else {
+          out.push_back(std::string(st) + "_" + f);
+          out.push_back(std::string(st) + "_tc_" + f);
+          out.push_back(std::string(st) + "_b_" + f);
+        }
+      }
+    return out;
+  }();
+  return (k >= 0 && k < (int)names.size()) ? names[k].c_str() : nullptr;
+}
+
+int glia_me_solve_state_features(glia_me_t h, const glia_me_params* p, float* c, float* wm, float* gm, float* vt,
+                                 float* csf, float* mri, const float* bg, float* const u[3], float* const v[3],
+                                 glia_me_info* info, double* features, int max_rows, int* rows) {
   if (!h) return GLIA_RD_ERR_ARG;
+  if (rows) *rows = 0;
+  if (features && (!rows || max_rows < h->rd->e->config().nt + 1))
+    return fail(h, GLIA_RD_ERR_ARG, "features need rows and max_rows >= nt + 1");
   if (!p || !c || !wm || !gm || !vt || !csf || !bg || !u || !v)
     return fail(h, GLIA_RD_ERR_ARG, "null parameters or field");
   for (int d = 0; d < 3; d++)
@@ -184,6 +350,18 @@ int glia_me_solve_state(glia_me_t h, const glia_me_params* p, float* c, float* w
                                           e->config().k_gm_wm, (void*)st),
            "coefficient update");
     ME_SUB(h, e->prec_factor(), e->error());
+    // feature_compute_ (:584-604): the features of this step, then the stop once the tumor core
+    // holds about 5% of the brain (ratio within 5e-3 of 0.05, or above) after t = 1
+    if (features) {
+      float ratio = 0.0f;
+      int rc = feat_step(h, i, dt, c, wm, gm, vt, csf, bg, u, v, features + (size_t)i * GLIA_ME_NFEATURES, &ratio);
+      if (rc) return rc;
+      (*rows)++;
+      if ((std::fabs(ratio - 0.05f) < 5e-3f || ratio > 0.05f) && (float)i * (float)dt >= 1.0f) {
+        inf.volume_stop = 1;
+        break;
+      }
+    }
     // advection of gm, wm, vt, csf, c and the MRI on one set of trajectories (:607-621); c lands
     // in the engine's c(T) field, where the diffusion solve and the reaction continue
     ME_SUB(h, glia_sl_set_velocity(h->sl, v[0], v[1], v[2], GLIA_RD_DEVICE), glia_sl_last_error(h->sl));

@@ -10,6 +10,14 @@ import numpy as np
 from . import _lib
 from .rd import GliaError
 
+NFEATURES = 130  # GLIA_ME_NFEATURES: t, the tumor ratio, 16 statistics of 8 fields
+
+
+def feature_names(L=None):
+    """The column names of a feature row (glia_me_feature_name), as the reference's CSV header."""
+    L = L or _lib.load()
+    return [L.glia_me_feature_name(k).decode() for k in range(NFEATURES)]
+
 
 class MassEffectSolver:
     """The engine (dt, nt, k, rho and their gm/wm ratios), the elasticity solver and the advection
@@ -20,6 +28,7 @@ class MassEffectSolver:
         self.shape = (n, n, n) if np.isscalar(n) else tuple(int(v) for v in n)
         self.ng = int(np.prod(self.shape))
         self.k, self.rho = float(k), float(rho)
+        self.nt = int(nt)
         c = _lib.Config()
         self.L.glia_rd_config_init(ctypes.byref(c))
         c.n[0], c.n[1], c.n[2] = self.shape
@@ -63,10 +72,12 @@ class MassEffectSolver:
             raise GliaError(rc, self.L.glia_rd_last_error(self.rd).decode())
 
     def solve_state(self, c0, wm, gm, vt, csf, bg, u=None, v=None, mri=None, forcing_factor=1e5, use_tanh=True,
-                    max_cfl=None, **el):
+                    max_cfl=None, features=False, **el):
         """solveState(0) from c0 and the tissue (max_cfl None: the reference's limit 14); el overrides
         glia_el_params fields.  Returns a dict
-        of c, wm, gm, vt, csf, mri (None when not given), u, v (shape (3,) + grid) and info."""
+        of c, wm, gm, vt, csf, mri (None when not given), u, v (shape (3,) + grid) and info.  With
+        features the loop runs its feature branch (glia_me_solve_state_features): the dict also holds
+        "features", one row of GLIA_ME_NFEATURES values per step, and "feature_names"."""
         import torch
 
         p = _lib.MeParams()
@@ -91,10 +102,16 @@ class MassEffectSolver:
         up = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in uu])
         vp = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in vv])
         info = _lib.MeInfo()
-        rc = self.L.glia_me_solve_state(self.h, ctypes.byref(p), f["c"].data_ptr(), f["wm"].data_ptr(),
-                                        f["gm"].data_ptr(), f["vt"].data_ptr(), f["csf"].data_ptr(),
-                                        None if m is None else m.data_ptr(), f["bg"].data_ptr(), up, vp,
-                                        ctypes.byref(info))
+        args = (self.h, ctypes.byref(p), f["c"].data_ptr(), f["wm"].data_ptr(), f["gm"].data_ptr(),
+                f["vt"].data_ptr(), f["csf"].data_ptr(), None if m is None else m.data_ptr(), f["bg"].data_ptr(),
+                up, vp, ctypes.byref(info))
+        feats, rows = None, ctypes.c_int(0)
+        if features:
+            feats = np.zeros((self.nt + 1, NFEATURES))
+            rc = self.L.glia_me_solve_state_features(*args, feats.ctypes.data_as(ctypes.c_void_p), self.nt + 1,
+                                                     ctypes.byref(rows))
+        else:
+            rc = self.L.glia_me_solve_state(*args)
         if rc != 0:
             raise GliaError(rc, self.L.glia_me_last_error(self.h).decode())
         out = {k: f[k].cpu().numpy().reshape(self.shape) for k in ("c", "wm", "gm", "vt", "csf")}
@@ -102,7 +119,11 @@ class MassEffectSolver:
         out["u"] = np.stack([t.cpu().numpy() for t in uu]).reshape((3,) + self.shape)
         out["v"] = np.stack([t.cpu().numpy() for t in vv]).reshape((3,) + self.shape)
         out["info"] = {"steps": info.steps, "cfl_stop": bool(info.cfl_stop), "cfl_max": info.cfl_max,
-                       "el_iterations": info.el_iterations, "diff_iterations": info.diff_iterations}
+                       "el_iterations": info.el_iterations, "diff_iterations": info.diff_iterations,
+                       "volume_stop": bool(info.volume_stop)}
+        if features:
+            out["features"] = feats[: rows.value]
+            out["feature_names"] = feature_names(self.L)
         return out
 
     def solve_state_multispecies(self, c0, wm, gm, vt, csf, bg, nec=None, u=None, v=None, forcing_factor=1e5,

@@ -48,6 +48,8 @@ typedef struct glia_me_info {
   double cfl_max;     /* largest CFL seen */
   int el_iterations;  /* elasticity CG iterations over all solves */
   int diff_iterations;/* diffusion CG iterations over all steps */
+  int volume_stop;    /* 1 when the feature branch stopped the loop: the tumor core reached about 5%
+                         of the brain (ratio within 5e-3 of 0.05, or above) at t >= 1 */
 } glia_me_info;
 
 /* the engine (glia_rd_create with cfg->nmembers = 1; cfg->stream, or the engine's own stream, is
@@ -63,7 +65,8 @@ glia_rd_t glia_me_rd(glia_me_t h);
 glia_el_t glia_me_el(glia_me_t h);
 glia_sl_t glia_me_sl(glia_me_t h);
 
-/* PdeOperatorsMassEffect::solveState(0) without its output and feature branches.
+/* PdeOperatorsMassEffect::solveState(0) without its output and feature branches (the feature
+   branch is glia_me_solve_state_features).
    c: Tumor::c_0_ in, c_t_ out.  wm, gm, vt, csf: MatProp's tissue in, filtered (filterTumor) and
    advected out; mri may be NULL (else advected too); bg: MatProp::bg_, fixed.
    u: Tumor::displacement_ (initial guess of the first step's elasticity solve in, the last
@@ -77,6 +80,31 @@ glia_sl_t glia_me_sl(glia_me_t h);
    u_old = u.  info may be NULL. */
 int glia_me_solve_state(glia_me_t h, const glia_me_params* p, float* c, float* wm, float* gm, float* vt, float* csf,
                         float* mri, const float* bg, float* const u[3], float* const v[3], glia_me_info* info);
+
+/* The columns of a feature row: GLIA_ME_NFEATURES values, named as the reference's CSV header
+   (biophysical_features.csv) names them. */
+#define GLIA_ME_NFEATURES 130
+/* name of column k (0 <= k < GLIA_ME_NFEATURES), else NULL */
+const char* glia_me_feature_name(int k);
+/* glia_me_solve_state with feature_compute_ on (src/pde/PdeOperatorsMassEffect.cpp:584-604): at
+   every step, after the coefficient update and before the advection,
+   computeBiophysicalFeatures (:323-418) writes row i of features (row-major, GLIA_ME_NFEATURES
+   doubles per row, max_rows >= nt + 1; *rows receives the rows written):
+     t = i dt, then tc / (tc + healthy) with tc the voxels labelled 1 by computeSegmentation and
+     healthy those labelled neither 0 nor 1; then for each of c, dc/dt = D c + rho (c - c^2) -
+     div(c v), |grad c|, |u|, det(I + grad u), tr S, the maximum shear stress (S = lam tr(E) I +
+     2 mu E, E the linearized strain, mu and lam the elasticity solver's current material) and |v|,
+     16 statistics: the integrals over the grid, tc and healthy; the surface integral
+     (n0 / 2 pi) h^3 sum I x with I = [|c - c*| < 0.1], c* the value of c where |grad c| is largest
+     outside tc; the standard deviations (unbiased, two-pass) over the grid, tc and healthy; and
+     the 0.875, 0.95 and 0.99 quantiles (entry floor(q m) of the m sorted values) over the grid,
+     tc and healthy, in the reference's column order.  An empty region's quantiles are 0.
+   After the features of step i, the loop stops (info->volume_stop) when the ratio is within 5e-3
+   of 0.05 or above and i dt >= 1.  With features NULL this is glia_me_solve_state.  The feature
+   computation uses the engine's alpha(0) field as scratch. */
+int glia_me_solve_state_features(glia_me_t h, const glia_me_params* p, float* c, float* wm, float* gm, float* vt,
+                                 float* csf, float* mri, const float* bg, float* const u[3], float* const v[3],
+                                 glia_me_info* info, double* features, int max_rows, int* rows);
 
 /* PdeOperatorsMultiSpecies::solveState(0) (src/pde/PdeOperatorsMultiSpecies.cpp:235-508), the
    multispecies forward of model 5 without its file output branch; the edema and segmentation it

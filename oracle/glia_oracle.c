@@ -1452,6 +1452,162 @@ void orc_ms_segmentation(int64_t n, const double *bg, const double *gm, const do
 }
 
 /* ------------------------------------------------------------------------ */
+/* Biophysical features of the model-4 loop                                 */
+/* ------------------------------------------------------------------------ */
+
+/* eigenvalues of a symmetric 3x3 matrix a (row-major) by cyclic Jacobi rotations (Golub & Van Loan,
+   Matrix Computations, 8.5): sweeps until the off-diagonal part is negligible */
+static void sym3_eig_jacobi(const double *a_in, double ev[3]) {
+  double a[3][3];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) a[r][c] = a_in[3 * r + c];
+  for (int sweep = 0; sweep < 50; sweep++) {
+    const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+    const double dia = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+    if (off <= 1e-30 * dia || off == 0) break;
+    for (int p = 0; p < 2; p++)
+      for (int q = p + 1; q < 3; q++) {
+        if (a[p][q] == 0) continue;
+        const double theta = (a[q][q] - a[p][p]) / (2 * a[p][q]);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+        const double cs = 1 / sqrt(t * t + 1), sn = t * cs;
+        for (int k = 0; k < 3; k++) {  /* A <- A J (columns p, q) */
+          const double akp = a[k][p], akq = a[k][q];
+          a[k][p] = cs * akp - sn * akq;
+          a[k][q] = sn * akp + cs * akq;
+        }
+        for (int k = 0; k < 3; k++) {  /* A <- J^T A (rows p, q) */
+          const double apk = a[p][k], aqk = a[q][k];
+          a[p][k] = cs * apk - sn * aqk;
+          a[q][k] = sn * apk + cs * aqk;
+        }
+      }
+  }
+  for (int k = 0; k < 3; k++) ev[k] = a[k][k];
+}
+
+/* computeStressQuants (src/pde/PdeOperatorsMassEffect.cpp:140-212): at one voxel, from the nine
+   gradient entries g[3 a + b] = d u_a / d x_b, det(I + grad u), the trace of S = lam tr(E) I + 2 mu E
+   with the linearized strain E = (grad u + grad u^T) / 2, and half the spread of S's eigenvalues */
+void orc_stress_point(const double *g, double mu, double lam, double *jac, double *tr, double *tau) {
+  double f[9], e[9], sm[9], ev[3];
+  for (int k = 0; k < 9; k++) f[k] = g[k];
+  f[0] += 1; f[4] += 1; f[8] += 1;
+  *jac = f[0] * (f[4] * f[8] - f[7] * f[5]) - f[1] * (f[3] * f[8] - f[6] * f[5]) + f[2] * (f[3] * f[7] - f[6] * f[4]);
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) e[3 * r + c] = 0.5 * (g[3 * r + c] + g[3 * c + r]);
+  const double tre = e[0] + e[4] + e[8];
+  for (int k = 0; k < 9; k++) sm[k] = 2 * mu * e[k];
+  sm[0] += lam * tre; sm[4] += lam * tre; sm[8] += lam * tre;
+  *tr = sm[0] + sm[4] + sm[8];
+  sym3_eig_jacobi(sm, ev);
+  const double lo = fmin(ev[0], fmin(ev[1], ev[2])), hi = fmax(ev[0], fmax(ev[1], ev[2]));
+  *tau = 0.5 * (hi - lo);
+}
+
+static int cmp_dbl(const void *a, const void *b) {
+  const double x = *(const double *)a, y = *(const double *)b;
+  return (x < y) ? -1 : (x > y ? 1 : 0);
+}
+
+/* computeQuantile (src/utils/Utils.cpp:828-880): entry floor(q m) of the m sorted values, q m in
+   single precision as the reference multiplies a float quantile by the count; 0 when m = 0 */
+static double quantile_of(double *vals, int64_t m, float q) {
+  if (m <= 0) return 0.0;
+  qsort(vals, (size_t)m, sizeof(double), cmp_dbl);
+  int64_t idx = (int64_t)floorf(q * (float)m);
+  if (idx >= m) idx = m - 1;
+  return vals[idx];
+}
+
+/* writeStats (src/pde/PdeOperatorsMassEffect.cpp:261-321) of x: 16 values into r */
+static void feature_stats(orc_t *o, const double *x, const double *tc, const double *hb, const double *ind,
+                          double ntc, double nh, double *r, double *buf) {
+  const int64_t ng = o->ng;
+  const double h3 = o->h[0] * o->h[1] * o->h[2];
+  double s = 0, st = 0, sh = 0, si = 0;
+  for (int64_t i = 0; i < ng; i++) {
+    s += x[i];
+    st += tc[i] * x[i];
+    sh += hb[i] * x[i];
+    si += ind[i] * x[i];
+  }
+  r[0] = s * h3; r[1] = st * h3; r[2] = sh * h3;
+  r[3] = si * h3 * o->n[0] / (2.0 * M_PI);
+  const double m = s / ng, mt = st / ntc, mh = sh / nh;
+  double q = 0, qt = 0, qh = 0;
+  for (int64_t i = 0; i < ng; i++) {
+    const double a = x[i] - m, b = (x[i] - mt) * tc[i], c = (x[i] - mh) * hb[i];
+    q += a * a; qt += b * b; qh += c * c;
+  }
+  r[4] = sqrt(q / (double)(ng - 1)); r[5] = sqrt(qt / (ntc - 1)); r[6] = sqrt(qh / (nh - 1));
+  const float qs[3] = {0.875f, 0.95f, 0.99f};
+  for (int reg = 0; reg < 3; reg++) {
+    int64_t m = 0;
+    for (int64_t i = 0; i < ng; i++)
+      if (reg == 0 || (reg == 1 ? tc[i] != 0 : hb[i] != 0)) buf[m++] = x[i];
+    for (int k = 0; k < 3; k++) {
+      r[7 + 3 * k + reg] = quantile_of(buf, m, qs[k]);  /* buf stays sorted after the first call */
+    }
+  }
+}
+
+/* PdeOperatorsMassEffect::computeBiophysicalFeatures (src/pde/PdeOperatorsMassEffect.cpp:323-418) at
+   step it: row[0] = it dt, row[1] = tc / (tc + healthy), then 16 statistics (feature_stats) of each
+   of c, dc/dt, |grad c|, |u|, det F, tr S, max shear and |v|.  tc = [seg == 1], healthy =
+   [seg not in {0, 1}] with seg from computeSegmentation; dc/dt = D c + rho (c - c^2) - div(c v)
+   with o->k and o->rho as the loop last set them (computeTumorQuants, :214-259); the indicator of
+   the surface statistic is |c - c*| < 0.1 with c* = c where |grad c| (1 - tc) is largest (first
+   index).  u, v: 3 ng; mu, lam: the elasticity material. */
+void orc_me_features(orc_t *o, int it, double dt, const double *c, const double *wm, const double *gm,
+                     const double *vt, const double *csf, const double *bg, const double *u, const double *v,
+                     const double *mu, const double *lam, double *row) {
+  const int64_t ng = o->ng;
+  double *seg = dalloc(ng), *tc = dalloc(ng), *hb = dalloc(ng), *ind = dalloc(ng), *buf = dalloc(ng);
+  double *g = dalloc(3 * ng), *gu = dalloc(9 * ng), *fl = dalloc(3 * ng), *dv = dalloc(ng), *dc = dalloc(ng);
+  double *fld = dalloc(8 * ng);
+  double *fc = fld, *fdcdt = fld + ng, *fgrad = fld + 2 * ng, *fu = fld + 3 * ng, *fjac = fld + 4 * ng,
+         *ftr = fld + 5 * ng, *ftau = fld + 6 * ng, *fv = fld + 7 * ng;
+  orc_ms_segmentation(ng, bg, gm, wm, vt, csf, c, seg);
+  double ntc = 0, nh = 0;
+  for (int64_t i = 0; i < ng; i++) {
+    tc[i] = (seg[i] == 1) ? 1.0 : 0.0;
+    hb[i] = (seg[i] != 0 && seg[i] != 1) ? 1.0 : 0.0;
+    ntc += tc[i];
+    nh += hb[i];
+  }
+  orc_grad(o, c, g, g + ng, g + 2 * ng);
+  for (int64_t i = 0; i < ng; i++)
+    fgrad[i] = sqrt(g[i] * g[i] + g[ng + i] * g[ng + i] + g[2 * ng + i] * g[2 * ng + i]);
+  orc_apply_d(o, c, dc);
+  for (int64_t i = 0; i < ng; i++)
+    for (int d = 0; d < 3; d++) fl[d * ng + i] = c[i] * v[d * ng + i];
+  orc_div(o, fl, fl + ng, fl + 2 * ng, dv);
+  for (int64_t i = 0; i < ng; i++) fdcdt[i] = dc[i] + o->rho[i] * (c[i] - c[i] * c[i]) - dv[i];
+  int64_t imax = 0;
+  double best = -INFINITY;
+  for (int64_t i = 0; i < ng; i++) {
+    const double w = fgrad[i] * (1 - tc[i]);
+    if (w > best) { best = w; imax = i; }
+  }
+  const double cstar = c[imax];
+  for (int64_t i = 0; i < ng; i++) ind[i] = (fabs(c[i] - cstar) < 0.1) ? 1.0 : 0.0;
+  for (int a = 0; a < 3; a++) orc_grad(o, u + a * ng, gu + (3 * a) * ng, gu + (3 * a + 1) * ng, gu + (3 * a + 2) * ng);
+  for (int64_t i = 0; i < ng; i++) {
+    fc[i] = c[i];
+    fu[i] = sqrt(u[i] * u[i] + u[ng + i] * u[ng + i] + u[2 * ng + i] * u[2 * ng + i]);
+    fv[i] = sqrt(v[i] * v[i] + v[ng + i] * v[ng + i] + v[2 * ng + i] * v[2 * ng + i]);
+    double gi[9];
+    for (int k = 0; k < 9; k++) gi[k] = gu[k * ng + i];
+    orc_stress_point(gi, mu[i], lam[i], &fjac[i], &ftr[i], &ftau[i]);
+  }
+  row[0] = it * dt;
+  row[1] = ntc / (nh + ntc);
+  for (int k = 0; k < 8; k++) feature_stats(o, fld + k * ng, tc, hb, ind, ntc, nh, row + 2 + 16 * k, buf);
+  free(seg); free(tc); free(hb); free(ind); free(buf); free(g); free(gu); free(fl); free(dv); free(dc); free(fld);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Mass-effect forward (model 4)                                            */
 /* ------------------------------------------------------------------------ */
 
@@ -1470,9 +1626,27 @@ void orc_ms_segmentation(int64_t n, const double *bg, const double *gm, const do
    v (3 ng): Tumor::velocity_, in and out; el_prm: the ten parameters of orc_el_material.
    info: steps done, 1 if the CFL test stopped the loop, largest CFL seen, elasticity CG iterations,
    diffusion CG iterations. */
+/* With features (may be NULL; max_rows >= nt + 1 rows of 130 values), the loop runs its feature
+   branch (:584-604): after precFactor, orc_me_features of the step into row it, then the stop when
+   the tumor ratio (single precision) is within 5e-3 of 0.05 or above and it dt >= 1; *rows receives
+   the rows written and info[5] is 1 for that stop. */
+int orc_me_solve_state_f(orc_t *o, double *c, double *wm, double *gm, double *vt, double *csf, double *mri,
+                         const double *bg, double *u, double *v, double ff, int use_tanh, const double *el_prm,
+                         double max_cfl, double *info, double *features, int max_rows, int *rows);
+
 int orc_me_solve_state(orc_t *o, double *c, double *wm, double *gm, double *vt, double *csf, double *mri,
                        const double *bg, double *u, double *v, double ff, int use_tanh, const double *el_prm,
                        double max_cfl, double *info) {
+  return orc_me_solve_state_f(o, c, wm, gm, vt, csf, mri, bg, u, v, ff, use_tanh, el_prm, max_cfl, info, NULL, 0,
+                              NULL);
+}
+
+int orc_me_solve_state_f(orc_t *o, double *c, double *wm, double *gm, double *vt, double *csf, double *mri,
+                         const double *bg, double *u, double *v, double ff, int use_tanh, const double *el_prm,
+                         double max_cfl, double *info, double *features, int max_rows, int *rows) {
+  if (rows) *rows = 0;
+  if (features && (!rows || max_rows < o->nt + 1)) return -2;
+  int vstop = 0;
   const int64_t ng = o->ng;
   const double dt = o->dt, h0 = 2.0 * M_PI / o->n[0], sigma = 2.0 * M_PI / o->n[0];
   double *uold = dalloc(3 * ng), *f = dalloc(3 * ng), *q = dalloc(3 * ng), *tmp = dalloc(ng);
@@ -1495,6 +1669,16 @@ int orc_me_solve_state(orc_t *o, double *c, double *wm, double *gm, double *vt, 
     if (cfl >= max_cfl) { stopped = 1; break; }
     orc_ms_update_coefficients(ng, o->rho, o->k, bg, gm, vt, csf, o->rho_scale, o->k_scale, o->r_gm_wm, o->k_gm_wm);
     orc_prec_factor(o);
+    if (features) {
+      double *row = features + (size_t)it * 130;
+      orc_me_features(o, it, dt, c, wm, gm, vt, csf, bg, u, v, mu, lam, row);
+      (*rows)++;
+      const float ratio = (float)row[1];
+      if ((fabsf(ratio - 0.05f) < 5e-3f || ratio > 0.05f) && (float)it * (float)dt >= 1.0f) {
+        vstop = 1;
+        break;
+      }
+    }
     orc_sl_trajectories(o, vx, vy, vz, dt, q, q + ng, q + 2 * ng);
     orc_sl_solve(o, gm, vx, vy, vz, q, q + ng, q + 2 * ng, dt, 1);
     orc_sl_solve(o, wm, vx, vy, vz, q, q + ng, q + 2 * ng, dt, 1);
@@ -1520,6 +1704,7 @@ int orc_me_solve_state(orc_t *o, double *c, double *wm, double *gm, double *vt, 
   info[2] = cfl_max;
   info[3] = el_its;
   info[4] = diff_its;
+  info[5] = vstop;
   free(uold); free(f); free(q); free(tmp); free(mu); free(lam); free(scr);
   return 0;
 }

@@ -58,6 +58,11 @@ def _load():
         "orc_sl_solve": (None, [vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, d, i]),
         "orc_smooth": (None, [vp, _dp, _dp, d]),
         "orc_me_solve_state": (i, [vp, _dp, _dp, _dp, _dp, _dp, vp, _dp, _dp, _dp, d, i, _dp, d, _dp]),
+        "orc_me_solve_state_f": (i, [vp, _dp, _dp, _dp, _dp, _dp, vp, _dp, _dp, _dp, d, i, _dp, d, _dp, vp, i,
+                                     ctypes.POINTER(ctypes.c_int)]),
+        "orc_me_features": (None, [vp, i, d] + [_dp] * 11),
+        "orc_stress_point": (None, [_dp, d, d, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double)]),
         "orc_ms_solve_state": (i, [vp] + [_dp] * 12 + [d, i, _dp, _dp, d, _dp, _dp, _dp]),
         "orc_set_tissue": (None, [vp, vp, vp, vp, vp, vp]),
         "orc_set_tissue_sinusoidal": (None, [vp]),
@@ -133,6 +138,14 @@ def phi_mesh(np_, cm, sigma, spacing_factor):
     out = np.zeros(3 * max(np_, 1) * 2, dtype=np.float64)
     k = _load().orc_phi_mesh(int(np_), _c(cm), float(sigma), float(spacing_factor), out)
     return out[: 3 * k].reshape(k, 3)
+
+
+def stress_point(g, mu, lam):
+    """computeStressQuants at one voxel (orc_stress_point): g[3 a + b] = d u_a / d x_b; returns
+    (det(I + grad u), tr S, max shear) with S = lam tr(E) I + 2 mu E, E the linearized strain."""
+    jac, tr, tau = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    _load().orc_stress_point(_c(g), float(mu), float(lam), ctypes.byref(jac), ctypes.byref(tr), ctypes.byref(tau))
+    return jac.value, tr.value, tau.value
 
 
 class Oracle:
@@ -238,28 +251,45 @@ class Oracle:
         return u.reshape((3,) + self.shape), its, reason.value, rn.value
 
     def me_solve_state(self, c0, wm, gm, vt, csf, bg, u=None, v=None, mri=None, ff=1e5, use_tanh=True,
-                       el_prm=None, max_cfl=14.0):
+                       el_prm=None, max_cfl=14.0, features=False):
         """PdeOperatorsMassEffect::solveState(0): the mass-effect forward of model 4 over nt + 1
         steps, with the diffusion settings of set_tissue/set_k/set_rho/set_time.  Returns a dict of
         c, wm, gm, vt, csf, mri (None when not given), u and v (3-component arrays) and info
-        (steps, cfl_stop, cfl_max, el_iterations, diff_iterations)."""
+        (steps, cfl_stop, cfl_max, el_iterations, diff_iterations, volume_stop).  With features the
+        loop runs its feature branch and the dict holds "features" (one row of 130 per step)."""
         f = {k: np.array(a, dtype=np.float64, order="C").reshape(-1).copy()
              for k, a in (("c", c0), ("wm", wm), ("gm", gm), ("vt", vt), ("csf", csf))}
         m = None if mri is None else np.array(mri, dtype=np.float64, order="C").reshape(-1).copy()
         uu = np.zeros(3 * self.ng) if u is None else self._vec(u).copy()
         vv = np.zeros(3 * self.ng) if v is None else self._vec(v).copy()
         prm = np.array(self.EL_DEFAULTS if el_prm is None else el_prm, dtype=np.float64)
-        info = np.zeros(5)
-        self.L.orc_me_solve_state(self.h, f["c"], f["wm"], f["gm"], f["vt"], f["csf"],
-                                  None if m is None else m.ctypes.data_as(ctypes.c_void_p), self._f(bg), uu, vv,
-                                  float(ff), int(bool(use_tanh)), prm, float(max_cfl), info)
+        info = np.zeros(6)
+        feats, rows = None, ctypes.c_int(0)
+        if features:
+            feats = np.zeros((self.nt + 1, 130))
+        self.L.orc_me_solve_state_f(self.h, f["c"], f["wm"], f["gm"], f["vt"], f["csf"],
+                                    None if m is None else m.ctypes.data_as(ctypes.c_void_p), self._f(bg), uu, vv,
+                                    float(ff), int(bool(use_tanh)), prm, float(max_cfl), info,
+                                    None if feats is None else feats.ctypes.data_as(ctypes.c_void_p),
+                                    0 if feats is None else self.nt + 1, ctypes.byref(rows))
         out = {k: a.reshape(self.shape) for k, a in f.items()}
         out["mri"] = None if m is None else m.reshape(self.shape)
         out["u"] = uu.reshape((3,) + self.shape)
         out["v"] = vv.reshape((3,) + self.shape)
         out["info"] = {"steps": int(info[0]), "cfl_stop": bool(info[1]), "cfl_max": float(info[2]),
-                       "el_iterations": int(info[3]), "diff_iterations": int(info[4])}
+                       "el_iterations": int(info[3]), "diff_iterations": int(info[4]),
+                       "volume_stop": bool(info[5])}
+        if features:
+            out["features"] = feats[: rows.value]
         return out
+
+    def me_features(self, it, dt, c, wm, gm, vt, csf, bg, u, v, mu, lam):
+        """computeBiophysicalFeatures of one state (orc_me_features): a row of 130 values.  D and
+        rho are the oracle's current k and rho fields."""
+        row = np.zeros(130)
+        self.L.orc_me_features(self.h, int(it), float(dt), self._f(c), self._f(wm), self._f(gm), self._f(vt),
+                               self._f(csf), self._f(bg), self._vec(u), self._vec(v), self._f(mu), self._f(lam), row)
+        return row
 
     MS_DEFAULTS = (4.0, 1.0, 5.0, 0.1, 0.01, 0.8, 0.6)
 

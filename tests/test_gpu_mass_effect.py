@@ -84,3 +84,52 @@ def test_multispecies_forward_matches_the_oracle(oracle_mod):
     assert ref["edema"].max() > 0
     assert e_ed < 1e-2
     assert seg_diff <= 1e-4 * out["seg"].size
+
+
+def test_biophysical_features_match_the_oracle(oracle_mod):
+    """The feature branch of model 4 (computeBiophysicalFeatures, src/pde/PdeOperatorsMassEffect.cpp:
+    323-418) at 32^3, nt = 3: every row against the oracle's.  Integrals and surface terms within
+    1e-4 of their column's largest value; standard deviations and quantiles within 1e-4 of the
+    feature's scale (its largest standard deviation or quantile over the rows: a grid-wide quantile
+    that falls among near-zero values is compared in the feature's units, not its own).  The step
+    count, tumor ratio and time column match exactly or to rounding."""
+    from glia_amd.mass_effect import MassEffectSolver, NFEATURES, feature_names
+    n, dt, nt, k, rho, ff = 32, 0.04, 3, 0.01, 8.0, 1.2e5
+    print("case mass effect features 32^3 nt 3", flush=True)
+    c0, wm, gm, vt, csf, bg = mass_effect_case(n)
+    o = oracle_mod.Oracle(n)
+    o.set_tissue(wm, gm, csf, vt, bg)
+    o.set_k(k, 0.2)
+    o.set_rho(rho, 0.2)
+    o.set_time(dt, nt)
+    ref = o.me_solve_state(c0, wm, gm, vt, csf, bg, ff=ff, features=True)
+    s = MassEffectSolver(n, dt, nt, k=k, rho=rho, k_gm_wm=0.2, r_gm_wm=0.2)
+    try:
+        s.set_tissue(wm, gm, csf, vt)
+        out = s.solve_state(c0, wm, gm, vt, csf, bg, forcing_factor=ff, features=True)
+        plain = s.solve_state(c0, wm, gm, vt, csf, bg, forcing_factor=ff)
+    finally:
+        s.close()
+    G, R = out["features"], ref["features"]
+    names = out["feature_names"]
+    assert names == feature_names() and len(names) == NFEATURES
+    assert names[:4] == ["t", "volc/volb", "vol_c", "vol_tc_c"] and names[5] == "sa_c" and names[-1] == "99_b_vel"
+    assert G.shape == R.shape == (nt + 1, NFEATURES)
+    assert out["info"]["steps"] == ref["info"]["steps"] == nt + 1
+    assert not out["info"]["volume_stop"] and not ref["info"]["volume_stop"]
+    assert np.allclose(G[:, 0], R[:, 0], rtol=1e-12, atol=0)
+    assert np.allclose(G[:, 1], R[:, 1], rtol=1e-12, atol=0)  # voxel counts
+    worst = 0.0
+    for f in range(8):
+        blk = slice(2 + 16 * f, 2 + 16 * (f + 1))
+        g, r = G[:, blk], R[:, blk]
+        scale = max(np.max(np.abs(r[:, 4:])), 1e-30)
+        for j in range(16):
+            den = max(np.max(np.abs(r[:, j])), 1e-30) if j < 4 else scale
+            e = float(np.max(np.abs(g[:, j] - r[:, j])) / den)
+            worst = max(worst, e)
+            assert e < 1e-4, (names[2 + 16 * f + j], e, g[:, j], r[:, j])
+    print("worst feature error %.2e" % worst, flush=True)
+    # the feature branch does not change the state the loop produces
+    for key in ("c", "u", "v"):
+        assert np.array_equal(out[key], plain[key]), key
