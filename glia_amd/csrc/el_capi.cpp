@@ -1,14 +1,21 @@
 // el_capi.cpp -- extern "C" boundary of the linear-elasticity solver (include/glia_el.h).
 //
-// The operator runs in spectral space on a one-member RD engine: the three displacement spectra
-// U_d are formed once, then
-//   div u -> lam div u -> its spectrum L,
-//   the six strain terms mu (d_j u_i + d_i u_j) -> their spectra S_ij,
-//   f_i = inverse of i (sum_j w_j S_ij + w_i L), minus screen u_i,
-// i.e. 20 transforms per application (the reference's sequence of gradient and divergence calls
-// makes 48).  The engine's real slots RS_WM (forward input) and RS_GM (inverse output) stage the
-// transforms; its complex slots hold U (CS_X, CS_A, CS_H), L (CS_P), S (CS_T0..CS_T5) and one
-// combination (CS_W); CS_T7 is the inverse transform's work slot.
+// The operator runs in spectral space on an RD engine of three members used as lanes, one per
+// displacement component, so that every transform covers the three components (or three strain
+// terms) in one launch.  Per application:
+//   U_d = FFT(u_d) in lane d (one launch),
+//   the strain terms d_j u_i + d_i u_j: the diagonal ones (lanes 0-2) and the off-diagonal ones
+//   (xy, xz, yz in lanes 0-2) as cross-lane spectral combinations, each triple inverted, scaled by
+//   mu and transformed back with the product in the Z pass between the two transforms; lam div u
+//   in lane 0 alone the same way,
+//   f_i = inverse of i (sum_j w_j S_ij + w_i L) in lane i, minus screen u_i in the inverse's Z pass,
+// i.e. 20 transforms per application in 8 transform launches (the reference's sequence of gradient
+// and divergence calls makes 48 single transforms).  Each lane's real slots RS_WM and RS_GM stage
+// the force's transforms, six more hold the lane's component of the CG vectors x, b, r, z, p, w, so
+// the transforms read and write them in place, and three hold the material (mu, lam, screen) where
+// the Z passes read it.  Complex slots: U in CS_X, the combination in CS_W, the diagonal strains in
+// CS_T0, the off-diagonal ones in CS_T1, L in CS_T2 of lane 0, the preconditioned spectra in CS_T3;
+// CS_T7 is the transforms' work slot.
 #include <cmath>
 #include <memory>
 #include <utility>
@@ -35,13 +42,24 @@ enum {
   KSP_DIVERGED_NANORINF = -9,
   KSP_DIVERGED_INDEFINITE_MAT = -10
 };
-constexpr int CU[3] = {glia::CS_X, glia::CS_A, glia::CS_H};
-constexpr int CS_L = glia::CS_P, CS_C = glia::CS_W;
-constexpr int CS_S[6] = {glia::CS_T0, glia::CS_T1, glia::CS_T2, glia::CS_T3, glia::CS_T4, glia::CS_T5};
-// strain pair (i, j), i <= j, -> index into CS_S
-int sidx(int i, int j) {
+constexpr int CU = glia::CS_X, CS_C = glia::CS_W, CS_DIAG = glia::CS_T0, CS_OFF = glia::CS_T1, CS_L = glia::CS_T2,
+              CS_PC = glia::CS_T3;
+// the material in every lane's real slots, where the Z passes read it: mu (all lanes), lam (lane 0),
+// the screening coefficient in RS_RHO (all lanes)
+constexpr int RS_MU = glia::RS_K, RS_LAM = glia::RS_CH;
+// real slot of each CG vector in every lane: x, b, r, z, p, w
+enum { VX = 0, VB, VR, VZ, VP, VW, kVecs };
+constexpr int kVecSlot[kVecs] = {glia::RS_CSF, glia::RS_VT, glia::RS_FILTER, glia::RS_OBS, glia::RS_D1, glia::RS_C0};
+// where the strain spectrum S_ij lives: (lane, slot)
+void strain_at(int i, int j, int* lane, int* slot) {
   if (i > j) std::swap(i, j);
-  return i == 0 ? j : (i == 1 ? 2 + j : 5);
+  if (i == j) {
+    *lane = i;
+    *slot = CS_DIAG;
+  } else {
+    *lane = (i == 0) ? (j == 1 ? 0 : 1) : 2;  // xy -> 0, xz -> 1, yz -> 2
+    *slot = CS_OFF;
+  }
 }
 }  // namespace
 
@@ -50,10 +68,8 @@ struct glia_el_s {
   int64_t ng = 0;
   hipStream_t st = nullptr;
   float *mu = nullptr, *lam = nullptr, *scr = nullptr;
-  // CG vectors: x staging, b, r, z, p, w.  r, z and p live in real fields of the engine that the
-  // solver does not otherwise use, so the transforms read and write them in place
-  float* v[6][3] = {};
-  double* part = nullptr;
+  float* v[kVecs][3] = {};   // lane d's real slot of each CG vector (the engine's memory)
+  double* part = nullptr;    // block partials, then (at part + 3 np) the summed scalars
   std::vector<double> hpart;
   double avg[3] = {0, 0, 0};
   bool have_material = false;
@@ -61,12 +77,8 @@ struct glia_el_s {
   ~glia_el_s() {
     for (float* p : {mu, lam, scr})
       if (p) (void)hipFree(p);
-    for (int k = 0; k < 6; k++)
-      for (float* p : v[k])
-        if (p && !engine_owned(k)) (void)hipFree(p);
     if (part) (void)hipFree(part);
   }
-  static bool engine_owned(int k) { return k == 2 || k == 3 || k == 4; }  // VR, VZ, VP
 };
 
 namespace {
@@ -90,112 +102,98 @@ int hip_fail(glia_el_s* h, int e, const char* what) {
     if (e_ != 0) return fail(h, e_, h->e->error());    \
   } while (0)
 
-float* slot(glia_el_s* h, int rs) { return h->e->field_ptr(0, rs); }
+float* lane_slot(glia_el_s* h, int lane, int rs) { return h->e->field_ptr(lane, rs); }
 
-// the engine's real field at p, or -1
-int engine_slot(glia_el_s* h, const float* p) {
-  for (int s = 0; s < glia::RS_NUM; s++)
-    if (slot(h, s) == p) return s;
-  return -1;
+// lanes 0..k-1 take part in the engine's launches
+struct Lanes {
+  glia::RDEnsemble* e;
+  Lanes(glia::RDEnsemble* en, int k) : e(en) { e->set_lanes(k); }
+  ~Lanes() { e->set_lanes(0); }
+};
+
+Vec3 vec(glia_el_s* h, int k) { return Vec3{{h->v[k][0], h->v[k][1], h->v[k][2]}}; }
+Vec3C cvec(glia_el_s* h, int k) { return Vec3C{{h->v[k][0], h->v[k][1], h->v[k][2]}}; }
+Vec3 lane_field(glia_el_s* h, int rs) { return Vec3{{lane_slot(h, 0, rs), lane_slot(h, 1, rs), lane_slot(h, 2, rs)}}; }
+Vec3C lane_cfield(glia_el_s* h, int rs) {
+  return Vec3C{{lane_slot(h, 0, rs), lane_slot(h, 1, rs), lane_slot(h, 2, rs)}};
 }
 
-// real field (device) -> packed spectrum in complex slot cs, in place when the field is one of the
-// engine's, else through RS_WM
-int forward(glia_el_s* h, const float* src, int cs) {
-  int s = engine_slot(h, src);
-  if (s < 0) {
-    EL_HIP(h, hipMemcpyAsync(slot(h, glia::RS_WM), src, sizeof(float) * h->ng, hipMemcpyDeviceToDevice, h->st));
-    s = glia::RS_WM;
-  }
-  EL_ENG(h, h->e->fft_r2c(s, cs));
+// one triple (or lane 0 alone) of strain-like terms: the cross-lane combination T into CS_C, then
+// FFT(coef IFFT(.)) into slot cs_out with the coefficient applied in the Z pass
+int strain_channel(glia_el_s* h, int nl, const int* nterms, const int (*lane)[4], const int (*slot)[4],
+                   const int (*dir)[4], int coef_rs, int cs_out) {
+  Lanes use(h->e.get(), nl);
+  EL_ENG(h, h->e->lane_lin(nterms, lane, slot, dir, CS_C));
+  EL_ENG(h, h->e->coef_transform(CS_C, coef_rs, cs_out));
   return 0;
 }
 
-// f = A u (device vectors, f must not alias u)
-int apply_op(glia_el_s* h, const float* const u[3], float* const f[3]) {
-  const int64_t ng = h->ng;
-  float* wm = slot(h, glia::RS_WM);
-  float* gmo = slot(h, glia::RS_GM);
+// f = A u on the lane vectors (vector ids; f must differ from u)
+int apply_op(glia_el_s* h, int u, int f) {
   int rc;
-  for (int d = 0; d < 3; d++)
-    if ((rc = forward(h, u[d], CU[d]))) return rc;
-  // lam div u -> L
+  Lanes use(h->e.get(), 3);
+  EL_ENG(h, h->e->fft_r2c(kVecSlot[u], CU));
+  // d_j u_i + d_i u_j, diagonal terms (ii) in lane i, then xy, xz, yz in lanes 0, 1, 2
   {
-    const int s[3] = {CU[0], CU[1], CU[2]}, dir[3] = {0, 1, 2};
-    EL_ENG(h, h->e->spec_lin(3, s, dir, CS_C));
-    EL_ENG(h, h->e->fft_c2r(CS_C, glia::RS_GM));
-    EL_HIP(h, glia::el::launch_mul(h->st, ng, h->lam, gmo, wm));
-    EL_ENG(h, h->e->fft_r2c(glia::RS_WM, CS_L));
+    const int n[3] = {2, 2, 2};
+    const int lane[3][4] = {{0, 0}, {1, 1}, {2, 2}}, slot[3][4] = {{CU, CU}, {CU, CU}, {CU, CU}};
+    const int dir[3][4] = {{0, 0}, {1, 1}, {2, 2}};
+    if ((rc = strain_channel(h, 3, n, lane, slot, dir, RS_MU, CS_DIAG))) return rc;
   }
-  // mu (d_j u_i + d_i u_j) -> S_ij
-  for (int i = 0; i < 3; i++)
-    for (int j = i; j < 3; j++) {
-      const int s[2] = {CU[i], CU[j]}, dir[2] = {j, i};
-      EL_ENG(h, h->e->spec_lin(2, s, dir, CS_C));
-      EL_ENG(h, h->e->fft_c2r(CS_C, glia::RS_GM));
-      EL_HIP(h, glia::el::launch_mul(h->st, ng, h->mu, gmo, wm));
-      EL_ENG(h, h->e->fft_r2c(glia::RS_WM, CS_S[sidx(i, j)]));
+  {
+    const int n[3] = {2, 2, 2};
+    const int lane[3][4] = {{0, 1}, {0, 2}, {1, 2}}, slot[3][4] = {{CU, CU}, {CU, CU}, {CU, CU}};
+    const int dir[3][4] = {{1, 0}, {2, 0}, {2, 1}};
+    if ((rc = strain_channel(h, 3, n, lane, slot, dir, RS_MU, CS_OFF))) return rc;
+  }
+  // lam div u in lane 0
+  {
+    const int n[3] = {3, 0, 0};
+    const int lane[3][4] = {{0, 1, 2}}, slot[3][4] = {{CU, CU, CU}}, dir[3][4] = {{0, 1, 2}};
+    if ((rc = strain_channel(h, 1, n, lane, slot, dir, RS_LAM, CS_L))) return rc;
+  }
+  // f_i = div_j(S_ij) + d_i L - screen u_i in lane i
+  {
+    const int n[3] = {4, 4, 4};
+    int lane[3][4], slot[3][4], dir[3][4];
+    for (int i = 0; i < 3; i++) {
+      for (int j = 0; j < 3; j++) {
+        strain_at(i, j, &lane[i][j], &slot[i][j]);
+        dir[i][j] = j;
+      }
+      lane[i][3] = 0;
+      slot[i][3] = CS_L;
+      dir[i][3] = i;
     }
-  // f_i = div_j(S_ij) + d_i L - screen u_i
-  for (int i = 0; i < 3; i++) {
-    const int s[4] = {CS_S[sidx(i, 0)], CS_S[sidx(i, 1)], CS_S[sidx(i, 2)], CS_L}, dir[4] = {0, 1, 2, i};
-    EL_ENG(h, h->e->spec_lin(4, s, dir, CS_C));
-    EL_ENG(h, h->e->fft_c2r(CS_C, glia::RS_GM));
-    EL_HIP(h, glia::el::launch_fnms(h->st, ng, gmo, h->scr, u[i], f[i]));
+    EL_ENG(h, h->e->lane_lin(n, lane, slot, dir, CS_C));
   }
+  EL_ENG(h, h->e->screen_transform(CS_C, kVecSlot[u], kVecSlot[f]));
   return 0;
 }
 
-// u = B f (device vectors)
-int apply_prec(glia_el_s* h, const float* const f[3], float* const u[3]) {
-  int rc;
-  for (int d = 0; d < 3; d++)
-    if ((rc = forward(h, f[d], CU[d]))) return rc;
-  const int in[3] = {CU[0], CU[1], CU[2]}, out[3] = {CS_S[0], CS_S[1], CS_S[2]};
-  EL_ENG(h, h->e->el_prec(in, out, h->avg[0], h->avg[1], h->avg[2]));
-  for (int d = 0; d < 3; d++) {
-    const int s = engine_slot(h, u[d]);
-    if (s >= 0 && s != glia::RS_WM && s != glia::RS_GM) {
-      EL_ENG(h, h->e->fft_c2r(out[d], s));
-    } else {
-      EL_ENG(h, h->e->fft_c2r(out[d], glia::RS_GM));
-      EL_HIP(h, hipMemcpyAsync(u[d], slot(h, glia::RS_GM), sizeof(float) * h->ng, hipMemcpyDeviceToDevice, h->st));
-    }
-  }
+// u = B f on the lane vectors
+int apply_prec(glia_el_s* h, int f, int u) {
+  Lanes use(h->e.get(), 3);
+  EL_ENG(h, h->e->fft_r2c(kVecSlot[f], CU));
+  EL_ENG(h, h->e->el_prec_lanes(CU, CS_PC, h->avg[0], h->avg[1], h->avg[2]));
+  EL_ENG(h, h->e->fft_c2r(CS_PC, kVecSlot[u]));
   return 0;
 }
 
-int dot(glia_el_s* h, const float* const x[3], const float* const y[3], double* out) {
-  Vec3C a{{x[0], x[1], x[2]}}, b{{y[0], y[1], y[2]}};
-  EL_HIP(h, glia::el::launch_dot3(h->st, h->ng, a, b, h->part));
+// nsum scalars of interleaved block partials, summed on the device, read with one synchronization
+int read_scalars(glia_el_s* h, int nsum, double* out) {
   const int np = glia::el::partial_count();
-  EL_HIP(h, hipMemcpyAsync(h->hpart.data(), h->part, sizeof(double) * np, hipMemcpyDeviceToHost, h->st));
+  double* dst = h->part + 3 * np;
+  EL_HIP(h, glia::el::launch_sum_partials(h->st, h->part, nsum, dst));
+  EL_HIP(h, hipMemcpyAsync(out, dst, sizeof(double) * nsum, hipMemcpyDeviceToHost, h->st));
   EL_HIP(h, hipStreamSynchronize(h->st));
-  double s = 0;
-  for (int i = 0; i < np; i++) s += h->hpart[i];  // fixed order: deterministic
-  *out = s;
   return 0;
 }
 
-// <z, z> and <z, r> in one pass and one host read
-int dot_zz_zr(glia_el_s* h, const float* const z[3], const float* const r[3], double* zz, double* zr) {
-  Vec3C a{{z[0], z[1], z[2]}}, b{{r[0], r[1], r[2]}};
-  EL_HIP(h, glia::el::launch_dot3_zz_zr(h->st, h->ng, a, b, h->part));
-  const int np = glia::el::partial_count();
-  EL_HIP(h, hipMemcpyAsync(h->hpart.data(), h->part, sizeof(double) * 2 * np, hipMemcpyDeviceToHost, h->st));
-  EL_HIP(h, hipStreamSynchronize(h->st));
-  double s0 = 0, s1 = 0;
-  for (int i = 0; i < np; i++) {  // fixed order: deterministic
-    s0 += h->hpart[2 * i];
-    s1 += h->hpart[2 * i + 1];
-  }
-  *zz = s0;
-  *zr = s1;
-  return 0;
+int dot(glia_el_s* h, int x, int y, double* out) {
+  EL_HIP(h, glia::el::launch_dot3(h->st, h->ng, cvec(h, x), cvec(h, y), h->part));
+  return read_scalars(h, 1, out);
 }
-
-Vec3 mut(float* const p[3]) { return Vec3{{p[0], p[1], p[2]}}; }
-Vec3C cst(float* const p[3]) { return Vec3C{{p[0], p[1], p[2]}}; }
 
 bool valid_kind(int k) { return k == GLIA_RD_HOST || k == GLIA_RD_DEVICE; }
 
@@ -217,8 +215,6 @@ int stage_out(glia_el_s* h, float* const dst[3], int kind, int vs) {
   EL_HIP(h, hipStreamSynchronize(h->st));
   return 0;
 }
-
-enum { VX = 0, VB, VR, VZ, VP, VW };
 
 }  // namespace
 
@@ -243,7 +239,7 @@ int glia_el_create(const int n[3], void* stream, glia_el_t* out) {
   *out = nullptr;
   glia::SolverConfig sc;
   for (int i = 0; i < 3; i++) sc.n[i] = n[i];
-  sc.nmembers = 1;
+  sc.nmembers = 3;  // one lane per displacement component
   sc.nt = 1;
   sc.stream = (hipStream_t)stream;
   glia_el_s* h = new (std::nothrow) glia_el_s();
@@ -256,19 +252,10 @@ int glia_el_create(const int n[3], void* stream, glia_el_t* out) {
   }
   h->ng = (int64_t)n[0] * n[1] * n[2];
   h->st = h->e->stream();
+  for (int k = 0; k < kVecs; k++)
+    for (int d = 0; d < 3; d++) h->v[k][d] = lane_slot(h, d, kVecSlot[k]);
   const size_t b = sizeof(float) * h->ng;
-  // r, z, p in the engine's real fields that the elasticity solver never stages through (it uses
-  // RS_WM and RS_GM)
-  const int own[3][3] = {{glia::RS_CSF, glia::RS_VT, glia::RS_FILTER},
-                         {glia::RS_OBS, glia::RS_D1, glia::RS_C0},
-                         {glia::RS_CT, glia::RS_A0, glia::RS_CH}};
-  for (int k = 0; k < 3; k++)
-    for (int d = 0; d < 3; d++) h->v[2 + k][d] = slot(h, own[k][d]);
-  std::vector<float**> bufs = {&h->mu, &h->lam, &h->scr};
-  for (int k = 0; k < 6; k++)
-    if (!glia_el_s::engine_owned(k))
-      for (float*& p : h->v[k]) bufs.push_back(&p);
-  for (float** p : bufs) {
+  for (float** p : {&h->mu, &h->lam, &h->scr}) {
     const hipError_t e = hipMalloc((void**)p, b);
     if (e != hipSuccess) {
       delete h;
@@ -276,7 +263,7 @@ int glia_el_create(const int n[3], void* stream, glia_el_t* out) {
     }
   }
   const int np = glia::el::partial_count();
-  if (hipMalloc((void**)&h->part, sizeof(double) * 3 * np) != hipSuccess) {
+  if (hipMalloc((void**)&h->part, sizeof(double) * (3 * np + 8)) != hipSuccess) {
     delete h;
     return (int)hipErrorOutOfMemory;
   }
@@ -330,6 +317,12 @@ int glia_el_set_material(glia_el_t h, const float* c, const float* wm, const flo
   for (int i = 0; i < np; i++)
     for (int k = 0; k < 3; k++) s[k] += h->hpart[3 * i + k];
   for (int k = 0; k < 3; k++) h->avg[k] = s[k] / (double)h->ng;
+  const size_t fb = sizeof(float) * h->ng;
+  for (int d = 0; d < 3; d++) {
+    EL_HIP(h, hipMemcpyAsync(lane_slot(h, d, RS_MU), h->mu, fb, hipMemcpyDeviceToDevice, h->st));
+    EL_HIP(h, hipMemcpyAsync(lane_slot(h, d, glia::RS_RHO), h->scr, fb, hipMemcpyDeviceToDevice, h->st));
+  }
+  EL_HIP(h, hipMemcpyAsync(lane_slot(h, 0, RS_LAM), h->lam, fb, hipMemcpyDeviceToDevice, h->st));
   h->have_material = true;
   return 0;
 }
@@ -356,8 +349,7 @@ int glia_el_apply(glia_el_t h, const float* const u[3], float* const f[3], int m
   if (!h->have_material) return fail(h, GLIA_RD_ERR_STATE, "material not set");
   int rc;
   if ((rc = stage_in(h, u, mem_kind, VP))) return rc;
-  float* const* out = h->v[VW];
-  if ((rc = apply_op(h, h->v[VP], out))) return rc;
+  if ((rc = apply_op(h, VP, VW))) return rc;
   return stage_out(h, f, mem_kind, VW);
 }
 
@@ -367,27 +359,41 @@ int glia_el_precondition(glia_el_t h, const float* const f[3], float* const u[3]
   if (!h->have_material) return fail(h, GLIA_RD_ERR_STATE, "material not set");
   int rc;
   if ((rc = stage_in(h, f, mem_kind, VR))) return rc;
-  if ((rc = apply_prec(h, h->v[VR], h->v[VZ]))) return rc;
+  if ((rc = apply_prec(h, VR, VZ))) return rc;
   return stage_out(h, u, mem_kind, VZ);
 }
 
+// computeForce (src/mat/Tumor.cpp:114-158): f = ff grad(S c) (times tanh(S c)) with S the smoother
+// of sigma 2 pi / n0; the smoothed c in lane 0's RS_WM, its gradient through the three lanes
 int glia_el_force(glia_el_t h, const float* c, double forcing_factor, int use_tanh, float* const f[3], int mem_kind) {
   if (!h) return GLIA_RD_ERR_ARG;
   if (!c || !f || !valid_kind(mem_kind)) return fail(h, GLIA_RD_ERR_ARG, "null field or bad mem_kind");
-  float* wm = slot(h, glia::RS_WM);
-  EL_HIP(h, hipMemcpyAsync(wm, c, sizeof(float) * h->ng,
+  for (int d = 0; d < 3; d++)
+    if (!f[d]) return fail(h, GLIA_RD_ERR_ARG, "null component");
+  float* cs = lane_slot(h, 0, glia::RS_WM);
+  EL_HIP(h, hipMemcpyAsync(cs, c, sizeof(float) * h->ng,
                            mem_kind == GLIA_RD_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->st));
   const int n0 = h->e->dims().n0;
   EL_ENG(h, h->e->smooth_field(0, glia::RS_WM, 2.0 * M_PI / n0));  // sigma_smooth = 2 pi / n0
-  float* cs = h->v[VB][0];                                           // the smoothed c
-  EL_HIP(h, hipMemcpyAsync(cs, wm, sizeof(float) * h->ng, hipMemcpyDeviceToDevice, h->st));
-  EL_ENG(h, h->e->fft_r2c(glia::RS_WM, CU[0]));
-  for (int d = 0; d < 3; d++) {
-    const int s[1] = {CU[0]}, dir[1] = {d};
-    EL_ENG(h, h->e->spec_lin(1, s, dir, CS_C));
+  {
+    Lanes one(h->e.get(), 1);
+    EL_ENG(h, h->e->fft_r2c(glia::RS_WM, CU));
+  }
+  {
+    Lanes use(h->e.get(), 3);
+    const int n[3] = {1, 1, 1};
+    const int lane[3][4] = {{0}, {0}, {0}}, slot[3][4] = {{CU}, {CU}, {CU}}, dir[3][4] = {{0}, {1}, {2}};
+    EL_ENG(h, h->e->lane_lin(n, lane, slot, dir, CS_C));
     EL_ENG(h, h->e->fft_c2r(CS_C, glia::RS_GM));
-    EL_HIP(h, glia::el::launch_force_scale(h->st, h->ng, slot(h, glia::RS_GM), cs, (float)forcing_factor,
-                                           use_tanh ? 1 : 0, h->v[VW][d]));
+  }
+  float* dst[3];
+  for (int d = 0; d < 3; d++) dst[d] = mem_kind == GLIA_RD_DEVICE ? f[d] : h->v[VW][d];
+  for (int d = 0; d < 3; d++)
+    EL_HIP(h, glia::el::launch_force_scale(h->st, h->ng, lane_slot(h, d, glia::RS_GM), cs, (float)forcing_factor,
+                                           use_tanh ? 1 : 0, dst[d]));
+  if (mem_kind == GLIA_RD_DEVICE) {
+    EL_HIP(h, hipStreamSynchronize(h->st));
+    return 0;
   }
   return stage_out(h, f, mem_kind, VW);
 }
@@ -401,19 +407,16 @@ int glia_el_solve(glia_el_t h, const float* const f[3], float* const u[3], int m
   const int maxit = 100;
   int rc;
   if ((rc = stage_in(h, f, mem_kind, VB)) || (rc = stage_in(h, u, mem_kind, VX))) return rc;
-  float* const* x = h->v[VX];
-  float* const* b = h->v[VB];
-  float* const* r = h->v[VR];
-  float* const* z = h->v[VZ];
-  float* const* p = h->v[VP];
-  float* const* w = h->v[VW];
-  // KSPSolve_CG (PETSc, KSP_NORM_PRECONDITIONED, nonzero initial guess) with KSPConvergedDefault
-  if ((rc = apply_op(h, x, r))) return rc;
-  EL_HIP(h, glia::el::launch_axpby3(h->st, h->ng, mut(r), cst(b), 0.0, 1));  // r = b - A x
+  // KSPSolve_CG (PETSc, KSP_NORM_PRECONDITIONED, nonzero initial guess) with KSPConvergedDefault.
+  // One host synchronization per iteration: <p, A p> stays on the device, where the update takes
+  // its step from it (and skips itself when the host is going to stop on it); the host reads it
+  // with <z, z> and <z, r> at the end of the iteration.
+  if ((rc = apply_op(h, VX, VR))) return rc;
+  EL_HIP(h, glia::el::launch_axpby3(h->st, h->ng, vec(h, VR), cvec(h, VB), 0.0, 1));  // r = b - A x
   double rnorm0, dp, beta = 0, betaold = 0, dpi = 0, dpiold, a;
-  if ((rc = apply_prec(h, b, z)) || (rc = dot(h, z, z, &rnorm0))) return rc;  // ||B b||
+  if ((rc = apply_prec(h, VB, VZ)) || (rc = dot(h, VZ, VZ, &rnorm0))) return rc;  // ||B b||
   rnorm0 = std::sqrt(rnorm0);
-  if ((rc = apply_prec(h, r, z)) || (rc = dot(h, z, z, &dp))) return rc;
+  if ((rc = apply_prec(h, VR, VZ)) || (rc = dot(h, VZ, VZ, &dp))) return rc;
   dp = std::sqrt(dp);
   if (rnorm0 == 0) rnorm0 = dp;
   const double ttol = std::fmax(rtol * rnorm0, atol);
@@ -423,9 +426,11 @@ int glia_el_solve(glia_el_t h, const float* const f[3], float* const u[3], int m
     if (rn >= dtol * rnorm0) return (int)KSP_DIVERGED_DTOL;
     return 0;
   };
+  const int np = glia::el::partial_count();
+  double* dpi_dev = h->part + 3 * np;  // read_scalars' destination: <p, A p> at [0], then <z, z>, <z, r>
   int its = 0, why = converged(dp), i = 0;
   if (!why) {
-    if ((rc = dot(h, z, r, &beta))) return rc;
+    if ((rc = dot(h, VZ, VR, &beta))) return rc;
     do {
       its = i + 1;
       if (beta == 0.0) {
@@ -435,10 +440,22 @@ int glia_el_solve(glia_el_t h, const float* const f[3], float* const u[3], int m
         why = KSP_DIVERGED_INDEFINITE_PC;
         break;
       }
-      EL_HIP(h, glia::el::launch_axpby3(h->st, h->ng, mut(p), cst(z), i ? beta / betaold : 0.0, 0));
+      EL_HIP(h, glia::el::launch_axpby3(h->st, h->ng, vec(h, VP), cvec(h, VZ), i ? beta / betaold : 0.0, 0));
       dpiold = dpi;
-      if ((rc = apply_op(h, p, w)) || (rc = dot(h, p, w, &dpi))) return rc;
+      if ((rc = apply_op(h, VP, VW))) return rc;
+      EL_HIP(h, glia::el::launch_dot3(h->st, h->ng, cvec(h, VP), cvec(h, VW), h->part));
+      EL_HIP(h, glia::el::launch_sum_partials(h->st, h->part, 1, dpi_dev));
+      EL_HIP(h, glia::el::launch_cg_update_dev(h->st, h->ng, vec(h, VX), vec(h, VR), cvec(h, VP), cvec(h, VW), dpi_dev,
+                                               beta, dpiold, i == 0 ? 1 : 0));
       betaold = beta;
+      // <z, z> for the test and <z, r> for the next beta in one pass
+      if ((rc = apply_prec(h, VR, VZ))) return rc;
+      EL_HIP(h, glia::el::launch_dot3_zz_zr(h->st, h->ng, cvec(h, VZ), cvec(h, VR), h->part));
+      EL_HIP(h, glia::el::launch_sum_partials(h->st, h->part, 2, dpi_dev + 1));
+      double sc[3];
+      EL_HIP(h, hipMemcpyAsync(sc, dpi_dev, sizeof(sc), hipMemcpyDeviceToHost, h->st));
+      EL_HIP(h, hipStreamSynchronize(h->st));
+      dpi = sc[0];
       if (std::isnan(dpi) || std::isinf(dpi)) {
         why = KSP_DIVERGED_NANORINF;
         break;
@@ -448,14 +465,10 @@ int glia_el_solve(glia_el_t h, const float* const f[3], float* const u[3], int m
         break;
       }
       a = beta / dpi;
-      EL_HIP(h, glia::el::launch_cg_update(h->st, h->ng, mut(x), mut(r), cst(p), cst(w), a));
-      // <z, z> for the test and <z, r> for the next beta in one pass (beta is unused when the
-      // test stops the loop)
-      double zr = 0;
-      if ((rc = apply_prec(h, r, z)) || (rc = dot_zz_zr(h, z, r, &dp, &zr))) return rc;
-      dp = std::sqrt(dp);
+      (void)a;
+      dp = std::sqrt(sc[1]);
       if ((why = converged(dp))) break;
-      beta = zr;
+      beta = sc[2];
       if (std::isnan(beta) || std::isinf(beta)) {
         why = KSP_DIVERGED_NANORINF;
         break;

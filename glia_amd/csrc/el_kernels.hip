@@ -50,20 +50,6 @@ __global__ void __launch_bounds__(256) k_el_material(int64_t ng, const float* __
   if (threadIdx.x < 3) part[blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// out = a * b
-__global__ void __launch_bounds__(256) k_mul(int64_t ng, const float* __restrict__ a, const float* __restrict__ b,
-                                             float* __restrict__ out) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += stride) out[i] = a[i] * b[i];
-}
-
-// out = a - b * c
-__global__ void __launch_bounds__(256) k_fnms(int64_t ng, const float* __restrict__ a, const float* __restrict__ b,
-                                              const float* __restrict__ c, float* __restrict__ out) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += stride) out[i] = a[i] - b[i] * c[i];
-}
-
 // per-block FP64 partials of <x, y> over the three components
 __global__ void __launch_bounds__(256) k_dot3(int64_t ng, Vec3C x, Vec3C y, double* __restrict__ part) {
   double s = 0.0;
@@ -105,9 +91,33 @@ __global__ void __launch_bounds__(256) k_dot3_zz_zr(int64_t ng, Vec3C z, Vec3C r
   if (threadIdx.x < 2) part[2 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// x += a p and r -= a w (the CG update)
-__global__ void __launch_bounds__(256) k_cg_update(int64_t ng, Vec3 x, Vec3 r, Vec3C p, Vec3C w, double a) {
-  const float fa = (float)a;
+// fixed-order sums of nsum interleaved partial arrays (part[nsum b + s], b < np) into out[s]
+__global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ part, int np, int nsum,
+                                                      double* __restrict__ out) {
+  __shared__ double red[256];
+  for (int s = 0; s < nsum; s++) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < np; b += 256) v += part[(int64_t)nsum * b + s];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[s] = red[0];
+    __syncthreads();
+  }
+}
+
+// the CG update with the step a = beta / dpi taken on the device, dpi = <p, A p> from *dpi; the
+// update is skipped when the host will stop on dpi (NaN or infinite, zero, or a sign change from
+// dpiold after the first iteration), so x and r stay as the reference leaves them
+__global__ void __launch_bounds__(256) k_cg_update_dev(int64_t ng, Vec3 x, Vec3 r, Vec3C p, Vec3C w,
+                                                       const double* __restrict__ dpi, double beta, double dpiold,
+                                                       int first) {
+  const double q = *dpi;
+  if (isnan(q) || isinf(q) || q == 0.0 || (!first && q * dpiold <= 0.0)) return;
+  const float fa = (float)(beta / q);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int k = 0; k < 3; k++)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += stride) {
@@ -146,14 +156,6 @@ int launch_force_scale(hipStream_t st, int64_t ng, const float* g, const float* 
   k_force_scale<<<EB, 256, 0, st>>>(ng, g, c, ff, use_tanh, out);
   return (int)hipGetLastError();
 }
-int launch_mul(hipStream_t st, int64_t ng, const float* a, const float* b, float* out) {
-  k_mul<<<EB, 256, 0, st>>>(ng, a, b, out);
-  return (int)hipGetLastError();
-}
-int launch_fnms(hipStream_t st, int64_t ng, const float* a, const float* b, const float* c, float* out) {
-  k_fnms<<<EB, 256, 0, st>>>(ng, a, b, c, out);
-  return (int)hipGetLastError();
-}
 int launch_dot3(hipStream_t st, int64_t ng, const Vec3C& x, const Vec3C& y, double* part) {
   k_dot3<<<EB, 256, 0, st>>>(ng, x, y, part);
   return (int)hipGetLastError();
@@ -162,9 +164,13 @@ int launch_dot3_zz_zr(hipStream_t st, int64_t ng, const Vec3C& z, const Vec3C& r
   k_dot3_zz_zr<<<EB, 256, 0, st>>>(ng, z, r, part);
   return (int)hipGetLastError();
 }
-int launch_cg_update(hipStream_t st, int64_t ng, const Vec3& x, const Vec3& r, const Vec3C& p, const Vec3C& w,
-                     double a) {
-  k_cg_update<<<EB, 256, 0, st>>>(ng, x, r, p, w, a);
+int launch_sum_partials(hipStream_t st, const double* part, int nsum, double* out) {
+  k_sum_partials<<<1, 256, 0, st>>>(part, EB, nsum, out);
+  return (int)hipGetLastError();
+}
+int launch_cg_update_dev(hipStream_t st, int64_t ng, const Vec3& x, const Vec3& r, const Vec3C& p, const Vec3C& w,
+                         const double* dpi, double beta, double dpiold, int first) {
+  k_cg_update_dev<<<EB, 256, 0, st>>>(ng, x, r, p, w, dpi, beta, dpiold, first);
   return (int)hipGetLastError();
 }
 int launch_axpby3(hipStream_t st, int64_t ng, const Vec3& p, const Vec3C& z, double b, int sub) {

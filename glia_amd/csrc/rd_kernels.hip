@@ -895,7 +895,12 @@ k_z_diff(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
 //   ZR_R2C      : packed out = R2C(real in)
 //   ZR_MISFIT   : c = C2R/N^3 -> CT; t = O c - d1; partial sum t^2; a = -O t; out = R2C(a)
 //   ZR_OBS2     : c = C2R/N^3; a = -O (O c); out = R2C(a)  (final condition of the incremental adjoint)
-enum { ZR_REACT = 0, ZR_REACT_ADJ = 1, ZR_C2R = 2, ZR_R2C = 3, ZR_MISFIT = 4, ZR_OBS2 = 5 };
+//   ZR_COEF     : c = C2R/N^3; out = R2C(m c) with m the real slot r_slot (a Lame coefficient times a
+//                 strain term of the elasticity operator)
+//   ZR_SCREEN   : c = C2R/N^3; real slot r_slot = c - s u with s in RS_RHO and u the real slot s_out
+//                 (the screened elasticity operator's output; no forward transform)
+enum { ZR_REACT = 0, ZR_REACT_ADJ = 1, ZR_C2R = 2, ZR_R2C = 3, ZR_MISFIT = 4, ZR_OBS2 = 5, ZR_COEF = 6,
+       ZR_SCREEN = 7 };
 
 template <int M, int MODE>
 __global__ void __launch_bounds__(ZK<M>::TPB, GLIA_Z_MINW)
@@ -906,7 +911,7 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT, VPT = ZK<M>::VPT;
   // pointwise operands are loaded before the transform when the registers fit
   constexpr bool PF = VPT <= 16;
-  constexpr bool TWO = (MODE == ZR_REACT_ADJ || MODE == ZR_MISFIT);
+  constexpr bool TWO = (MODE == ZR_REACT_ADJ || MODE == ZR_MISFIT || MODE == ZR_SCREEN);
   __shared__ float2 lb[ZK<M>::NL * ZK<M>::LP];
   __shared__ float2 twml[M], tw2l[M];
   const int li = threadIdx.x / NT, t = threadIdx.x % NT;
@@ -919,8 +924,8 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   const float s = (float)d.inv_ng;
   double acc[1] = {0.0};
   float2 v[VPT];
-  GPtr op1 = rptr2(Mb, (MODE == ZR_MISFIT || MODE == ZR_OBS2) ? RS_OBS : RS_RHO) + base;
-  GPtr op2 = MODE == ZR_MISFIT ? rptr2(Mb, RS_D1) + base : rf;
+  GPtr op1 = MODE == ZR_COEF ? rf : rptr2(Mb, (MODE == ZR_MISFIT || MODE == ZR_OBS2) ? RS_OBS : RS_RHO) + base;
+  GPtr op2 = MODE == ZR_MISFIT ? rptr2(Mb, RS_D1) + base : (MODE == ZR_SCREEN ? rptr2(Mb, s_out) + base : rf);
   GPtr ctp = rptr2(Mb, RS_CT) + base;
   float2 o1[PF ? VPT : 1], o2[PF && TWO ? VPT : 1];
   if (MODE == ZR_R2C) {
@@ -964,6 +969,11 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
           v[j] = c;
         } else if (MODE == ZR_OBS2) {
           v[j] = make_float2(-(a1.x * (a1.x * c.x)), -(a1.y * (a1.y * c.y)));
+        } else if (MODE == ZR_COEF) {
+          v[j] = make_float2(a1.x * c.x, a1.y * c.y);
+        } else if (MODE == ZR_SCREEN) {
+          const float2 u = PF ? o2[j] : op2[gi];
+          rf[gi] = make_float2(c.x - a1.x * u.x, c.y - a1.y * u.y);
         } else if (MODE == ZR_MISFIT) {
           ctp[gi] = c;
           const float2 dd = PF ? o2[j] : op2[gi];
@@ -974,7 +984,7 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
       }
     }
   }
-  if (MODE != ZR_C2R) zl_forward_store<M>(v, lb, line, t, twml, tw2l, cptr(Mb, s_out), base, false);
+  if (MODE != ZR_C2R && MODE != ZR_SCREEN) zl_forward_store<M>(v, lb, line, t, twml, tw2l, cptr(Mb, s_out), base, false);
   if (MODE == ZR_MISFIT) block_reduce_store<1>(acc, Mb.part + blockIdx.x);
 }
 
@@ -1597,6 +1607,10 @@ __global__ void __launch_bounds__(256) k_spec_div(const MemberDev* __restrict__ 
 struct SpecTerms {
   int slot[4], dir[4], n;
 };
+struct LaneTerms {
+  int n[3];
+  int lane[3][4], slot[3][4], dir[3][4];
+};
 __global__ void __launch_bounds__(256) k_spec_lin(const MemberDev* __restrict__ mem, Dims d, SpecTerms T, int s_out) {
   const MemberDev& M = mem[blockIdx.y];
   float2* o = craw(M, s_out);
@@ -1618,6 +1632,42 @@ __global__ void __launch_bounds__(256) k_spec_lin(const MemberDev* __restrict__ 
   }
 }
 
+// the same combination across members used as lanes (one displacement component per lane): output
+// lane l = blockIdx.y gets i sum_k w_{dir[l][k]} X_k with X_k the slot slot[l][k] of lane lane[l][k];
+// the terms are summed in k_spec_lin's order.  NT terms (every output lane of a launch has as many);
+// the source pointers are taken once per block and the spectral index is split in 32 bits.
+template <int NT>
+__global__ void __launch_bounds__(256) k_lane_lin(const MemberDev* __restrict__ mem, Dims d, LaneTerms T, int s_out) {
+  const int l = blockIdx.y;
+  float2* __restrict__ o = craw(mem[l], s_out);
+  const float2* __restrict__ src[NT];
+  int dir[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    src[t] = craw(mem[T.lane[l][t]], T.slot[l][t]);
+    dir[t] = T.dir[l][t];
+  }
+  const unsigned m2 = (unsigned)d.m2, n1 = (unsigned)d.n1, nspec = (unsigned)d.nspec;
+  const unsigned stride = gridDim.x * blockDim.x;
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < nspec; i += stride) {
+    const unsigned r = i / m2;
+    const int kz = (int)(i - r * m2);
+    const unsigned kxu = r / n1;
+    const int ky = (int)(r - kxu * n1), kx = (int)kxu;
+    const float w[3] = {(2 * kx == d.n0) ? 0.0f : (float)(2 * kx < d.n0 ? kx : kx - d.n0),
+                        (2 * ky == d.n1) ? 0.0f : (float)(2 * ky < d.n1 ? ky : ky - d.n1), (float)kz};
+    float re = 0.0f, im = 0.0f;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const float2 a = src[t][i];
+      const float wt = w[dir[t]];
+      re += wt * a.x;
+      im += wt * a.y;
+    }
+    o[i] = make_float2(-im, re);
+  }
+}
+
 // precFactorElasticity (src/cuda/ElasticitySolver.cu:36-106) on packed spectra, without the 1/N^3
 // that the inverse transform applies: s1 = -screen - mu |w|^2, s2 = lam + mu,
 // s3 = 1 / (1 - |w|^2 s2 / s1), U_d = F_d / s1 + (s2 s3 / s1^2) w_d (w.F).  Out of place.
@@ -1626,6 +1676,31 @@ __global__ void __launch_bounds__(256) k_el_prec(const MemberDev* __restrict__ m
   const MemberDev& M = mem[blockIdx.y];
   const float2* F[3] = {craw(M, sf.x), craw(M, sf.y), craw(M, sf.z)};
   float2* U[3] = {craw(M, su.x), craw(M, su.y), craw(M, su.z)};
+  const float s2 = lam + mu;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.nspec; i += stride) {
+    const int kz = (int)(i % d.m2);
+    const int64_t r = i / d.m2;
+    const int ky = (int)(r % d.n1), kx = (int)(r / d.n1);
+    const float w[3] = {(2 * kx == d.n0) ? 0.0f : (float)(2 * kx < d.n0 ? kx : kx - d.n0),
+                        (2 * ky == d.n1) ? 0.0f : (float)(2 * ky < d.n1 ? ky : ky - d.n1), (float)kz};
+    const float ww = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    const float s1 = -scr - mu * ww;
+    const float s3 = 1.0f / (1.0f - ww * s2 / s1);
+    const float c = s2 * s3 / (s1 * s1);
+    const float2 f0 = F[0][i], f1 = F[1][i], f2 = F[2][i];
+    const float wfx = w[0] * f0.x + w[1] * f1.x + w[2] * f2.x, wfy = w[0] * f0.y + w[1] * f1.y + w[2] * f2.y;
+    const float2 f[3] = {f0, f1, f2};
+#pragma unroll
+    for (int k = 0; k < 3; k++) U[k][i] = make_float2(f[k].x / s1 + c * w[k] * wfx, f[k].y / s1 + c * w[k] * wfy);
+  }
+}
+
+// k_el_prec across lanes: F_d in slot sf of lane d, U_d into slot su of lane d (one pass, grid.y = 1)
+__global__ void __launch_bounds__(256) k_el_prec_lanes(const MemberDev* __restrict__ mem, Dims d, int sf, int su,
+                                                       float mu, float lam, float scr) {
+  const float2* F[3] = {craw(mem[0], sf), craw(mem[1], sf), craw(mem[2], sf)};
+  float2* U[3] = {craw(mem[0], su), craw(mem[1], su), craw(mem[2], su)};
   const float s2 = lam + mu;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.nspec; i += stride) {
@@ -1813,6 +1888,8 @@ struct RunZPoint {
       case ZR_R2C: GLIA_ZP(ZR_R2C); break;
       case ZR_MISFIT: GLIA_ZP(ZR_MISFIT); break;
       case ZR_OBS2: GLIA_ZP(ZR_OBS2); break;
+      case ZR_COEF: GLIA_ZP(ZR_COEF); break;
+      case ZR_SCREEN: GLIA_ZP(ZR_SCREEN); break;
       default: return -1;
     }
 #undef GLIA_ZP
@@ -1980,6 +2057,39 @@ int launch_spec_lin(const LaunchCtx& c, int nterms, const int* slots, const int*
     T.dir[t] = dirs[t];
   }
   k_spec_lin<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d, T, s_out);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+
+int launch_lane_lin(const LaunchCtx& c, const int* nterms, const int (*lane)[4], const int (*slot)[4],
+                    const int (*dir)[4], int s_out) {
+  if (c.nm < 1 || c.nm > 3) return (int)hipErrorInvalidValue;
+  LaneTerms T{};
+  for (int l = 0; l < c.nm; l++) {
+    if (nterms[l] < 1 || nterms[l] > 4) return (int)hipErrorInvalidValue;
+    T.n[l] = nterms[l];
+    for (int t = 0; t < nterms[l]; t++) {
+      T.lane[l][t] = lane[l][t];
+      T.slot[l][t] = slot[l][t];
+      T.dir[l][t] = dir[l][t];
+    }
+  }
+  for (int l = 1; l < c.nm; l++)
+    if (T.n[l] != T.n[0]) return (int)hipErrorInvalidValue;  // one term count per launch
+  if ((int64_t)c.d.nspec >= (int64_t)1 << 31) return (int)hipErrorInvalidValue;
+  const dim3 g(EW_BLOCKS, c.nm);
+  switch (T.n[0]) {
+    case 1: k_lane_lin<1><<<g, 256, 0, c.stream>>>(c.mem, c.d, T, s_out); break;
+    case 2: k_lane_lin<2><<<g, 256, 0, c.stream>>>(c.mem, c.d, T, s_out); break;
+    case 3: k_lane_lin<3><<<g, 256, 0, c.stream>>>(c.mem, c.d, T, s_out); break;
+    default: k_lane_lin<4><<<g, 256, 0, c.stream>>>(c.mem, c.d, T, s_out); break;
+  }
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+
+int launch_el_prec_lanes(const LaunchCtx& c, int sf, int su, double mu, double lam, double scr) {
+  k_el_prec_lanes<<<dim3(EW_BLOCKS, 1), 256, 0, c.stream>>>(c.mem, c.d, sf, su, (float)mu, (float)lam, (float)scr);
   GLIA_CHECK_LAUNCH();
   return 0;
 }

@@ -23,7 +23,8 @@ struct LaunchCtx {
 
 // modes of launch_xf_div and launch_z_point (values match the kernel enums)
 enum { XFM_MATVEC = 0, XFM_SETUP = 1, XFM_D = 2 };
-enum { ZPM_REACT = 0, ZPM_REACT_ADJ = 1, ZPM_C2R = 2, ZPM_R2C = 3, ZPM_MISFIT = 4, ZPM_OBS2 = 5 };
+enum { ZPM_REACT = 0, ZPM_REACT_ADJ = 1, ZPM_C2R = 2, ZPM_R2C = 3, ZPM_MISFIT = 4, ZPM_OBS2 = 5, ZPM_COEF = 6,
+       ZPM_SCREEN = 7 };
 enum { FINK_FILTER = 0, FINK_KSUM = 1, FINK_MISFIT = 2, FINK_C0SQ = 3, FINK_QUAD = 4 };
 
 bool supported_line(int L);
@@ -67,6 +68,12 @@ int launch_axpby_c(const LaunchCtx& c, int s_x, double a, int s_y, double b, int
 int launch_spec_div(const LaunchCtx& c, int s_x, int s_y, int s_z, int s_out);
 int launch_spec_lin(const LaunchCtx& c, int nterms, const int* slots, const int* dirs, int s_out);
 int launch_el_prec(const LaunchCtx& c, int4 sf, int4 su, double mu, double lam, double scr);
+// members as lanes (at most 3, c.nm of them written): output lane l = i sum_t w_{dir[l][t]} X with X the
+// slot slot[l][t] of lane lane[l][t] (1 to 4 terms, the same count for every lane of a launch)
+int launch_lane_lin(const LaunchCtx& c, const int* nterms, const int (*lane)[4], const int (*slot)[4],
+                    const int (*dir)[4], int s_out);
+// the elasticity preconditioner with F_d and U_d in lane d (members 0..2)
+int launch_el_prec_lanes(const LaunchCtx& c, int sf, int su, double mu, double lam, double scr);
 int launch_bricks(const LaunchCtx& c, const BrickJob* jobs_host, int njobs, const double* taps_host, int tap_stride,
                   double h0, double h1, double h2, double sigma, float* store_dev, double* max_out_host);
 

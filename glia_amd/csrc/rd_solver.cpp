@@ -77,7 +77,7 @@ LaunchCtx RDEnsemble::ctx() const {
   LaunchCtx c;
   c.d = d_;
   c.mem = md_;
-  c.nm = nm_;
+  c.nm = (lanes_ > 0 && lanes_ < nm_) ? lanes_ : nm_;
   c.stream = stream_;
   c.tw0 = tw_[0];
   c.tw1 = tw_[1];
@@ -665,6 +665,36 @@ int RDEnsemble::fft_c2r(int cslot, int rslot) { return inverse_to_real(cslot, CS
 
 int RDEnsemble::spec_lin(int nterms, const int* slots, const int* dirs, int cout) {
   GLIA_TRYC(launch_spec_lin(ctx(), nterms, slots, dirs, cout));
+  return 0;
+}
+
+int RDEnsemble::lane_lin(const int* nterms, const int (*lane)[4], const int (*slot)[4], const int (*dir)[4],
+                         int cout) {
+  GLIA_TRYC(launch_lane_lin(ctx(), nterms, lane, slot, dir, cout));
+  return 0;
+}
+
+int RDEnsemble::el_prec_lanes(int cf, int cu, double mu, double lam, double scr) {
+  if (nm_ < 3) return fail(-2, "the lane preconditioner needs three members");
+  GLIA_TRYC(launch_el_prec_lanes(ctx(), cf, cu, mu, lam, scr));
+  return 0;
+}
+
+int RDEnsemble::coef_transform(int cs_in, int coef_rs, int cs_out) {
+  LaunchCtx c = ctx();
+  GLIA_TRYC(launch_col_plain(c, 0, +1, cs_in, CS_T7, false));
+  GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T7, CS_T7, false));
+  GLIA_TRYC(launch_z_point(c, ZPM_COEF, CS_T7, CS_T7, coef_rs, -1, 0.f, false));
+  GLIA_TRYC(launch_col_plain(c, 1, -1, CS_T7, CS_T7, false));
+  GLIA_TRYC(launch_col_plain(c, 0, -1, CS_T7, cs_out, false));
+  return 0;
+}
+
+int RDEnsemble::screen_transform(int cs_in, int u_rs, int out_rs) {
+  LaunchCtx c = ctx();
+  GLIA_TRYC(launch_col_plain(c, 0, +1, cs_in, CS_T7, false));
+  GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T7, CS_T7, false));
+  GLIA_TRYC(launch_z_point(c, ZPM_SCREEN, CS_T7, u_rs, out_rs, -1, 0.f, false));
   return 0;
 }
 

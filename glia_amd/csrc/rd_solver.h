@@ -91,6 +91,16 @@ class RDEnsemble {
   int spec_lin(int nterms, const int* slots, const int* dirs, int cout);
   // the constant-coefficient elasticity preconditioner on packed spectra, (fx, fy, fz) -> (ux, uy, uz)
   int el_prec(const int f[3], const int u[3], double mu, double lam, double scr);
+  // members as lanes: launches cover the first k members (0: all); see launch_lane_lin and
+  // launch_el_prec_lanes for the cross-lane kernels
+  void set_lanes(int k) { lanes_ = k; }
+  int lane_lin(const int* nterms, const int (*lane)[4], const int (*slot)[4], const int (*dir)[4], int cout);
+  int el_prec_lanes(int cf, int cu, double mu, double lam, double scr);
+  // out spectrum cs_out = FFT(m IFFT(cs_in)) with m the real slot coef_rs, the pointwise product in
+  // the Z pass between the two transforms (CS_T7 as work)
+  int coef_transform(int cs_in, int coef_rs, int cs_out);
+  // real slot out_rs = IFFT(cs_in) - s u with s in RS_RHO and u the real slot u_rs (CS_T7 as work)
+  int screen_transform(int cs_in, int u_rs, int out_rs);
   int get_spectrum(int m, int cslot, float2* host);
   int set_spectrum(int m, int cslot, const float2* host);
   int solve_state(int linearized, bool keep_history);
@@ -162,6 +172,7 @@ class RDEnsemble {
  private:
   unsigned long long uid_ = 0;
   unsigned long long graph_epoch_ = 0;
+  int lanes_ = 0;
   int eval_mode_ = 2;
   int eval_setup(int mode, const std::vector<std::vector<double>>& x);
   int eval_result(int mode, bool want_grad, std::vector<double>& J, std::vector<std::vector<double>>& dJ);

@@ -95,6 +95,21 @@ class ElasticitySolver:
                                        ctypes.byref(why), ctypes.byref(rn)))
         return np.stack(u).reshape((3,) + self.shape), its.value, REASONS.get(why.value, why.value), rn.value
 
+    def solve_device(self, f, u):
+        """The same solve on device-resident vectors: f and u are float32 torch tensors on the GPU,
+        each of three contiguous components (shape (3,) + grid); u holds the initial guess and
+        receives the solution.  Returns (iterations, reason, last preconditioned residual norm)."""
+        for t in (f, u):
+            if not (t.is_cuda and t.dtype.is_floating_point and t.element_size() == 4 and t.is_contiguous()
+                    and tuple(t.shape) == (3,) + self.shape):
+                raise ValueError("f and u must be contiguous float32 GPU tensors of shape (3,) + grid")
+        fp = (ctypes.c_void_p * 3)(*[f[d].data_ptr() for d in range(3)])
+        up = (ctypes.c_void_p * 3)(*[u[d].data_ptr() for d in range(3)])
+        its, why, rn = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._chk(self.L.glia_el_solve(self.h, fp, up, _lib.DEVICE, ctypes.byref(its), ctypes.byref(why),
+                                       ctypes.byref(rn)))
+        return its.value, REASONS.get(why.value, why.value), rn.value
+
     def force(self, c, forcing_factor=1e5, use_tanh=True):
         """Tumor::computeForce: forcing_factor grad(S c) (times tanh(S c)), shape (3,) + grid."""
         cin, out = self._f(c), [np.empty(self.ng, np.float32) for _ in range(3)]
