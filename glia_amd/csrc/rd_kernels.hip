@@ -806,6 +806,32 @@ __device__ __forceinline__ void zl_inverse(float2* v, float2* line, int t, const
 
 // forward: v[j] = z[out_row(t, j)] -> packed spectrum of every line, stored block-cooperatively
 // to dst[base ...] (multiplied by i w_z = i k when mulz)
+// real output of a Z line kernel (C2R times scale), stored coalesced: the values go back to the
+// line buffer at their line positions and the block writes its lines in order.  With SCREEN the
+// stored value is c - a u, a and u read in the same order (the screened elasticity operator).
+template <int M, bool SCREEN>
+__device__ __forceinline__ void zl_real_store(const float2* v, float2* lb, float2* line, int t, float scale, GPtr out,
+                                              GPtr ap, GPtr up) {
+  constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT;
+  __syncthreads();  // every thread's inverse has read the line buffer
+#pragma unroll
+  for (int j = 0; j < ZK<M>::VPT; j++) line[xpad(out_row<R1, R2, NT>(t, j))] = cscale(v[j], scale);
+  __syncthreads();
+  constexpr int TPB = ZK<M>::TPB, IT = ZK<M>::NL * M / TPB;
+#pragma unroll 8
+  for (int i = 0; i < IT; i++) {
+    const int e = (int)threadIdx.x + i * TPB;
+    const int li = e / M, k = e - li * M;
+    const float2 c = lb[li * ZK<M>::LP + xpad(k)];
+    if (SCREEN) {
+      const float2 a = ap[e], u = up[e];
+      out[e] = make_float2(c.x - a.x * u.x, c.y - a.y * u.y);
+    } else {
+      out[e] = c;
+    }
+  }
+}
+
 template <int M>
 __device__ __forceinline__ void zl_forward_store(float2* v, float2* lb, float2* line, int t, const float2* __restrict__ twm,
                                                  const float2* __restrict__ tw2, GPtr dst, int64_t base,
@@ -911,7 +937,8 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT, VPT = ZK<M>::VPT;
   // pointwise operands are loaded before the transform when the registers fit
   constexpr bool PF = VPT <= 16;
-  constexpr bool TWO = (MODE == ZR_REACT_ADJ || MODE == ZR_MISFIT || MODE == ZR_SCREEN);
+  constexpr bool TWO = (MODE == ZR_REACT_ADJ || MODE == ZR_MISFIT);
+  constexpr bool REAL_OUT = (MODE == ZR_C2R || MODE == ZR_SCREEN);  // no forward transform
   __shared__ float2 lb[ZK<M>::NL * ZK<M>::LP];
   __shared__ float2 twml[M], tw2l[M];
   const int li = threadIdx.x / NT, t = threadIdx.x % NT;
@@ -929,11 +956,10 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   GPtr ctp = rptr2(Mb, RS_CT) + base;
   float2 o1[PF ? VPT : 1], o2[PF && TWO ? VPT : 1];
   if (MODE == ZR_R2C) {
-#pragma unroll
-    for (int j = 0; j < VPT; j++) v[j] = rf[lo + out_row<R1, R2, NT>(t, j)];
+    zl_load<M>(lb, rf, 0);  // the real lines, coalesced, into the line buffer
   } else {
     zl_load<M>(lb, cptr(Mb, s_in), base);
-    if (PF && MODE != ZR_C2R) {
+    if (PF && !REAL_OUT) {
 #pragma unroll
       for (int j = 0; j < VPT; j++) {
         const int gi = lo + out_row<R1, R2, NT>(t, j);
@@ -944,15 +970,20 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   }
   stage_line_twiddles<M>(twml, tw2l, twm, tw2);
   __syncthreads();
-  if (MODE != ZR_R2C) {
+  if (MODE == ZR_R2C) {
+#pragma unroll
+    for (int j = 0; j < VPT; j++) v[j] = line[xpad(out_row<R1, R2, NT>(t, j))];
+  }
+  if (REAL_OUT) {
+    zl_inverse<M>(v, line, t, twml, tw2l);
+    zl_real_store<M, MODE == ZR_SCREEN>(v, lb, line, t, s, rf, op1, op2);
+  } else if (MODE != ZR_R2C) {
     zl_inverse<M>(v, line, t, twml, tw2l);
 #pragma unroll
     for (int j = 0; j < VPT; j++) {
       const int gi = lo + out_row<R1, R2, NT>(t, j);
       float2 c = cscale(v[j], s);
-      if (MODE == ZR_C2R) {
-        rf[gi] = c;
-      } else {
+      {
         const float2 a1 = PF ? o1[j] : op1[gi];
         if (MODE == ZR_REACT) {
           if (rf) rf[gi] = c;
@@ -971,9 +1002,6 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
           v[j] = make_float2(-(a1.x * (a1.x * c.x)), -(a1.y * (a1.y * c.y)));
         } else if (MODE == ZR_COEF) {
           v[j] = make_float2(a1.x * c.x, a1.y * c.y);
-        } else if (MODE == ZR_SCREEN) {
-          const float2 u = PF ? o2[j] : op2[gi];
-          rf[gi] = make_float2(c.x - a1.x * u.x, c.y - a1.y * u.y);
         } else if (MODE == ZR_MISFIT) {
           ctp[gi] = c;
           const float2 dd = PF ? o2[j] : op2[gi];
@@ -984,7 +1012,7 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
       }
     }
   }
-  if (MODE != ZR_C2R && MODE != ZR_SCREEN) zl_forward_store<M>(v, lb, line, t, twml, tw2l, cptr(Mb, s_out), base, false);
+  if (!REAL_OUT) zl_forward_store<M>(v, lb, line, t, twml, tw2l, cptr(Mb, s_out), base, false);
   if (MODE == ZR_MISFIT) block_reduce_store<1>(acc, Mb.part + blockIdx.x);
 }
 
