@@ -508,8 +508,7 @@ def advect_config(args):
     departure points (8 B) and each field (4 B), and writes each field (4 B)."""
     import torch
     from glia_amd import adv
-    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
-    from adv_cases import reference_velocity
+    from glia_amd.synth import reference_velocity
     n, dt = 256, 0.02
     ng = n ** 3
     torch.manual_seed(11)
@@ -578,8 +577,7 @@ def elastic_config(args):
     fused <z, z>, <z, r> 12; the CG updates 27)."""
     import torch
     from glia_amd import elasticity
-    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
-    from el_cases import tissue_and_tumor, tumor_force
+    from glia_amd.synth import tissue_and_tumor, tumor_force
     n = 160
     fields = [a.astype(np.float32) for a in tissue_and_tumor(n)]
     f = tumor_force(fields[0].astype(np.float64)).astype(np.float32)
@@ -641,8 +639,7 @@ def masseffect_config(args):
     copies back 10, transforms around the diffusion solve 4, reaction 3, velocity 12, its
     smoothing 24, 24 for the engine's staging)."""
     from glia_amd.mass_effect import MassEffectSolver
-    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
-    from me_cases import mass_effect_case
+    from glia_amd.synth import mass_effect_case
     n, dt, nt, FF = 160, 0.025, 40, 6e4
     c0, wm, gm, vt, csf, bg = [a.astype(np.float32) for a in mass_effect_case(n)]
     s = MassEffectSolver(n, dt, nt, k=0.01, rho=8.0)
@@ -693,8 +690,7 @@ def multispecies_config(args):
     55 + 36 m per diffusion solve of m CG iterations, and 330 for the rest (the 270 of the model-4
     step, two more advected fields 6, k and its mean 10, clipping 14, the smoothing of i 8, D i 22)."""
     from glia_amd.mass_effect import MassEffectSolver
-    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
-    from me_cases import mass_effect_case
+    from glia_amd.synth import mass_effect_case
     n, dt, nt, FF = 160, 0.025, 40, 6e4
     c0, wm, gm, vt, csf, bg = [a.astype(np.float32) for a in mass_effect_case(n)]
     s = MassEffectSolver(n, dt, nt, k=0.01, rho=8.0)

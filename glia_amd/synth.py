@@ -106,3 +106,50 @@ def ensemble_params(count, seed):
     """kappa ~ U(0.01, 0.1) and rho ~ U(4, 15) per member."""
     rng = np.random.default_rng([int(seed), 5005])
     return rng.uniform(0.01, 0.1, count), rng.uniform(4.0, 15.0, count)
+
+
+def tissue_and_tumor(n, seed=4):
+    """Tissue maps with a background and a centred Gaussian tumor (the inputs of the elasticity and
+    mass-effect configurations): c, wm, gm, csf, vt, bg on an n (or n0 x n1 x n2) grid."""
+    shape = (n, n, n) if np.isscalar(n) else tuple(int(v) for v in n)
+    pr = problem(shape, seed, nbases=4, kappa=0.05, rho=8.0, sigma=2 * np.pi / 32)
+    wm, gm, csf, vt = (np.asarray(pr[k], np.float64) for k in ("wm", "gm", "csf", "vt"))
+    bg = np.clip(1.0 - (wm + gm + csf + vt), 0.0, 1.0)
+    x = [np.arange(m) * 2 * np.pi / m - np.pi for m in shape]
+    r2 = x[0][:, None, None] ** 2 + x[1][None, :, None] ** 2 + x[2][None, None, :] ** 2
+    c = np.exp(-r2 / 0.3)
+    return c, wm, gm, csf, vt, bg
+
+
+def tumor_force(c):
+    """A force along the tumor gradient, the shape of the mass-effect forcing (3 components)."""
+    h = [2 * np.pi / m for m in c.shape]
+    g = np.gradient(c, *h)
+    return -1e2 * np.stack(g)
+
+
+def mass_effect_case(n, symmetric=False, amplitude=0.5):
+    """Inputs of the mass-effect forwards: tissue with all four classes (part of the CSF map as
+    ventricles), a tumor seed of the given amplitude and a background that closes the partition of
+    unity; c, wm, gm, vt, csf, bg rounded to FP32."""
+    c, wm, gm, csf, vt, bg = tissue_and_tumor(n)
+    vt = 0.4 * csf
+    csf = 0.6 * csf
+    if symmetric:  # invariant under the exchange of the x and y axes
+        sym = lambda a: 0.5 * (a + a.transpose(1, 0, 2))  # noqa: E731
+        wm, gm, csf, vt = sym(wm), sym(gm), sym(csf), sym(vt)
+    bg = np.clip(1.0 - (wm + gm + csf + vt), 0.0, 1.0)
+    c = amplitude * c
+    return [a.astype(np.float32).astype(np.float64) for a in (c, wm, gm, vt, csf, bg)]
+
+
+def reference_velocity(n):
+    """createTestField (src/test/helper.cpp:47-80): frequency-4 sinusoids around (1, 1.5, 1)."""
+    n = (n, n, n) if np.isscalar(n) else tuple(n)
+    h = 2 * np.pi / np.array(n)
+    X, Y, Z = np.meshgrid(*[np.arange(m) for m in n], indexing="ij")
+    a, b, c = 4 * h[0] * X, 4 * h[1] * Y, 4 * h[2] * Z
+    vx = 1 + 0.25 * np.sin(a) * np.sin(b) * np.sin(c)
+    vy = 1.5 + 0.1 * np.sin(a) * np.cos(b) * np.sin(c)
+    vz = 1 + np.cos(a) * np.cos(b) * np.sin(c)
+    return [vx, vy, vz]

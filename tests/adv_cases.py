@@ -1,17 +1,8 @@
-"""Shared inputs of the advection tests: the reference's test velocity and test function."""
+"""Shared inputs of the advection tests: the reference's test velocity (glia_amd.synth, where the
+benchmark configuration takes it too) and test function."""
 import numpy as np
 
-
-def reference_velocity(n):
-    """createTestField (src/test/helper.cpp:47-80): frequency-4 sinusoids around (1, 1.5, 1)."""
-    n = (n, n, n) if np.isscalar(n) else tuple(n)
-    h = 2 * np.pi / np.array(n)
-    X, Y, Z = np.meshgrid(*[np.arange(m) for m in n], indexing="ij")
-    a, b, c = 4 * h[0] * X, 4 * h[1] * Y, 4 * h[2] * Z
-    vx = 1 + 0.25 * np.sin(a) * np.sin(b) * np.sin(c)
-    vy = 1.5 + 0.1 * np.sin(a) * np.cos(b) * np.sin(c)
-    vz = 1 + np.cos(a) * np.cos(b) * np.sin(c)
-    return [vx, vy, vz]
+from glia_amd.synth import reference_velocity  # noqa: F401
 
 
 def gauss_test_function(n):

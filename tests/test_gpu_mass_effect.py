@@ -133,3 +133,43 @@ def test_biophysical_features_match_the_oracle(oracle_mod):
     # the feature branch does not change the state the loop produces
     for key in ("c", "u", "v"):
         assert np.array_equal(out[key], plain[key]), key
+
+
+@pytest.mark.parametrize("shape", [(160, 32, 32), (32, 32, 160)])
+def test_forwards_on_radix5_plans_match_the_oracle(oracle_mod, shape):
+    """Both mass-effect loops on grids that run the radix-5 plans of the 160^3 configuration: the
+    160-point columns (160 x 32 x 32) and the 80-point packed Z lines (32 x 32 x 160), nt = 2, with
+    the bars of the 32^3 tests (state and tissue 1e-5; u, v and the small species 1e-4), the step
+    count and the elasticity CG iterations."""
+    from glia_amd.mass_effect import MassEffectSolver
+    dt, nt, k, rho, ff = 0.04, 2, 0.01, 8.0, 1.2e5
+    print("case mass effect radix-5 %s nt %d" % (shape, nt), flush=True)
+    c0, wm, gm, vt, csf, bg = mass_effect_case(shape)
+    o = oracle_mod.Oracle(shape)
+    o.set_tissue(wm, gm, csf, vt, bg)
+    o.set_k(k, 0.2)
+    o.set_rho(rho, 0.2)
+    o.set_time(dt, nt)
+    ref4 = o.me_solve_state(c0, wm, gm, vt, csf, bg, ff=ff)
+    ref5 = o.ms_solve_state(c0, wm, gm, vt, csf, bg, ff=ff)
+    s = MassEffectSolver(shape, dt, nt, k=k, rho=rho, k_gm_wm=0.2, r_gm_wm=0.2)
+    try:
+        s.set_tissue(wm, gm, csf, vt)
+        out4 = s.solve_state(c0, wm, gm, vt, csf, bg, forcing_factor=ff)
+        out5 = s.solve_state_multispecies(c0, wm, gm, vt, csf, bg, forcing_factor=ff)
+    finally:
+        s.close()
+    e4 = {key: rel(out4[key], ref4[key]) for key in ("c", "wm", "gm", "vt", "csf", "u", "v")}
+    e5 = {key: rel(out5[key], ref5[key]) for key in ("p", "i", "nec", "ox", "wm", "gm", "vt", "csf", "u", "v")}
+    print("model 4", out4["info"], ref4["info"], e4, flush=True)
+    print("model 5", out5["info"], ref5["info"], e5, flush=True)
+    for out, ref in ((out4, ref4), (out5, ref5)):
+        assert out["info"]["steps"] == ref["info"]["steps"] == nt + 1
+        assert abs(out["info"]["el_iterations"] - ref["info"]["el_iterations"]) <= nt + 2
+    for key in ("c", "wm", "gm", "vt", "csf"):
+        assert e4[key] < 1e-5, key
+    assert e4["u"] < 1e-4 and e4["v"] < 1e-4
+    for key in ("p", "ox", "wm", "gm", "vt", "csf"):
+        assert e5[key] < 1e-5, key
+    for key in ("i", "nec", "u", "v"):
+        assert e5[key] < 1e-4, key
