@@ -45,6 +45,25 @@ def traffic_from_profiles(kernel, n, members):
     return None
 
 
+def hbm_copy_rate(mib=1024, reps=20):
+    """What a plain device-to-device copy of mib MiB moves per second (read + write bytes), timed
+    with HIP events after three warm-up copies: the practical HBM rate beside the published peak."""
+    import torch
+    a = torch.ones(mib * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return 2.0 * mib * 1024 * 1024 / (ms * 1e-3) / 1e9
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -886,6 +905,9 @@ def main():
                                 "g_rho": float("%.3g" % (abs(grr - eng0[2]) / abs(eng0[2])))}
         log("reference-structured plain path:", ref)
 
+    copy_gbs = hbm_copy_rate()
+    log("device copy of 1 GiB: %.1f GB/s (read + write)" % copy_gbs)
+
     acc, cpu = None, None
     if not (args.no_accuracy and args.no_cpu):
         acc, t_orc, threads, ksp = accuracy_and_cpu(n, dt, members[0], members[0]["d1"], args.nt_oracle)
@@ -932,6 +954,7 @@ def main():
         "cg_iterations_per_solve": round(mbar, 3),
         "model_hbm_GBs": round(model_gbs, 1),
         "model_hbm_frac": round(model_gbs / HBM_PEAK_GBS, 4),
+        "hbm_copy_GBs": round(copy_gbs, 1),
         "kernels": kernels,
         "rel_err_fp64": acc,
     }
