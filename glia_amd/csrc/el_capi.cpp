@@ -118,13 +118,12 @@ Vec3C lane_cfield(glia_el_s* h, int rs) {
   return Vec3C{{lane_slot(h, 0, rs), lane_slot(h, 1, rs), lane_slot(h, 2, rs)}};
 }
 
-// one triple (or lane 0 alone) of strain-like terms: the cross-lane combination T into CS_C, then
-// FFT(coef IFFT(.)) into slot cs_out with the coefficient applied in the Z pass
+// one triple (or lane 0 alone) of strain-like terms: FFT(coef IFFT(combination)) into slot cs_out,
+// the combination formed in the first inverse pass and the coefficient applied in the Z pass
 int strain_channel(glia_el_s* h, int nl, const int* nterms, const int (*lane)[4], const int (*slot)[4],
                    const int (*dir)[4], int coef_rs, int cs_out) {
   Lanes use(h->e.get(), nl);
-  EL_ENG(h, h->e->lane_lin(nterms, lane, slot, dir, CS_C));
-  EL_ENG(h, h->e->coef_transform(CS_C, coef_rs, cs_out));
+  EL_ENG(h, h->e->lin_coef_transform(nterms, lane, slot, dir, coef_rs, cs_out));
   return 0;
 }
 
@@ -165,9 +164,8 @@ int apply_op(glia_el_s* h, int u, int f) {
       slot[i][3] = CS_L;
       dir[i][3] = i;
     }
-    EL_ENG(h, h->e->lane_lin(n, lane, slot, dir, CS_C));
+    EL_ENG(h, h->e->lin_screen_transform(n, lane, slot, dir, kVecSlot[u], kVecSlot[f]));
   }
-  EL_ENG(h, h->e->screen_transform(CS_C, kVecSlot[u], kVecSlot[f]));
   return 0;
 }
 

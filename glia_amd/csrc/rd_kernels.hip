@@ -234,6 +234,57 @@ __device__ __forceinline__ void stage_twiddles(float2* twl, const float2* __rest
   const int col = threadIdx.x & 15, t = threadIdx.x >> 4;    \
   float2* exc = ex + col * K::CP;
 
+// terms of the cross-lane spectral combinations (members used as lanes, at most 3): output lane l
+// gets i sum_t w_{dir[l][t]} X_t with X_t the slot slot[l][t] of lane lane[l][t]
+struct LaneTerms {
+  int n[3];
+  int lane[3][4], slot[3][4], dir[3][4];
+};
+
+// X pass, inverse, of a cross-lane combination formed as the columns are loaded: lane l = blockIdx.y
+// transforms i sum_t w_{dir[l][t]} X_t (NTM terms, summed in k_lane_lin's order) into s_out.  It
+// replaces a k_lane_lin launch followed by k_col_plain<L, 0, +1> without the intermediate field.
+template <int L, int NTM>
+__global__ void __launch_bounds__(CK<L>::TPB, GLIA_COL_MINW)
+k_col_lin(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, LaneTerms T, int s_out) {
+  const int l = blockIdx.y;
+  const MemberDev& M = mem[l];
+  GLIA_COL_PRELUDE(L)
+  const Tile tl = col_tile(0, d);
+  GPtr src[NTM];
+  int dir[NTM];
+#pragma unroll
+  for (int q = 0; q < NTM; q++) {
+    src[q] = cptr(mem[T.lane[l][q]], T.slot[l][q]) + tl.base;
+    dir[q] = T.dir[l][q];
+  }
+  GPtr O = cptr(M, s_out) + tl.base;
+  // this tile's columns: ky = tile index o, kz = c0 + col; the rows run over kx
+  const int ky = tl.o, kz = tl.c0 + col;
+  const float wy = (2 * ky == d.n1) ? 0.0f : (float)(2 * ky < d.n1 ? ky : ky - d.n1), wz = (float)kz;
+  float2 a[K::VPT];
+#pragma unroll
+  for (int j = 0; j < K::VIN; j++) {
+    const int row = in_row<R1, R2, NT>(t, j);
+    const unsigned g = (unsigned)row * tl.rs + col;
+    const float wx = (2 * row == d.n0) ? 0.0f : (float)(2 * row < d.n0 ? row : row - d.n0);
+    float re = 0.0f, im = 0.0f;
+#pragma unroll
+    for (int q = 0; q < NTM; q++) {
+      const float2 x = src[q][g];
+      const float wt = dir[q] == 0 ? wx : (dir[q] == 1 ? wy : wz);
+      re += wt * x.x;
+      im += wt * x.y;
+    }
+    a[j] = make_float2(-im, re);
+  }
+  __shared__ float2 twl[L];
+  stage_twiddles<L, K::TPB>(twl, tw);
+  fft2s<R1, R2, NT, 1>(a, exc, t, twl);
+#pragma unroll
+  for (int j = 0; j < K::VOUT; j++) if (K::out_ok(t, j)) O[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
+}
+
 // X pass, inverse: U = IFFT_x(p), V = IFFT_x(i w_x p).  With PUPD the search direction is
 // first updated in place: p = B r (i == 0) or p = B r + (beta/betaold) p.
 template <int L, bool PUPD>
@@ -1635,10 +1686,7 @@ __global__ void __launch_bounds__(256) k_spec_div(const MemberDev* __restrict__ 
 struct SpecTerms {
   int slot[4], dir[4], n;
 };
-struct LaneTerms {
-  int n[3];
-  int lane[3][4], slot[3][4], dir[3][4];
-};
+
 __global__ void __launch_bounds__(256) k_spec_lin(const MemberDev* __restrict__ mem, Dims d, SpecTerms T, int s_out) {
   const MemberDev& M = mem[blockIdx.y];
   float2* o = craw(M, s_out);
@@ -1887,6 +1935,43 @@ struct RunColPlain {
     return 0;
   }
 };
+template <int L>
+struct RunColLin {
+  static int run(const LaunchCtx& c, const LaneTerms& T, int s_out) {
+    dim3 g(c.d.n1 * (c.d.m2 / 16), c.nm);
+    switch (T.n[0]) {
+      case 1: k_col_lin<L, 1><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, T, s_out); break;
+      case 2: k_col_lin<L, 2><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, T, s_out); break;
+      case 3: k_col_lin<L, 3><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, T, s_out); break;
+      default: k_col_lin<L, 4><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, T, s_out); break;
+    }
+    GLIA_CHECK_LAUNCH();
+    return 0;
+  }
+};
+static int lane_terms(const LaunchCtx& c, const int* nterms, const int (*lane)[4], const int (*slot)[4],
+                      const int (*dir)[4], LaneTerms* T) {
+  if (c.nm < 1 || c.nm > 3) return (int)hipErrorInvalidValue;
+  *T = LaneTerms{};
+  for (int l = 0; l < c.nm; l++) {
+    if (nterms[l] < 1 || nterms[l] > 4 || nterms[l] != nterms[0]) return (int)hipErrorInvalidValue;
+    T->n[l] = nterms[l];
+    for (int t = 0; t < nterms[l]; t++) {
+      T->lane[l][t] = lane[l][t];
+      T->slot[l][t] = slot[l][t];
+      T->dir[l][t] = dir[l][t];
+    }
+  }
+  return 0;
+}
+int launch_col_lin(const LaunchCtx& c, const int* nterms, const int (*lane)[4], const int (*slot)[4],
+                   const int (*dir)[4], int s_out) {
+  LaneTerms T;
+  const int rc = lane_terms(c, nterms, lane, slot, dir, &T);
+  if (rc) return rc;
+  return dispatch_L<RunColLin>(c.d.n0, c, T, s_out);
+}
+
 int launch_col_plain(const LaunchCtx& c, int dir, int sign, int s_in, int s_out, bool honor_done) {
   return dispatch_L<RunColPlain>(dir == 0 ? c.d.n0 : c.d.n1, c, dir, sign, s_in, s_out, honor_done ? 1 : 0);
 }

@@ -72,6 +72,9 @@ int launch_el_prec(const LaunchCtx& c, int4 sf, int4 su, double mu, double lam, 
 // slot slot[l][t] of lane lane[l][t] (1 to 4 terms, the same count for every lane of a launch)
 int launch_lane_lin(const LaunchCtx& c, const int* nterms, const int (*lane)[4], const int (*slot)[4],
                     const int (*dir)[4], int s_out);
+// the inverse X pass of the same combination, formed as the columns load, into complex slot s_out
+int launch_col_lin(const LaunchCtx& c, const int* nterms, const int (*lane)[4], const int (*slot)[4],
+                   const int (*dir)[4], int s_out);
 // the elasticity preconditioner with F_d and U_d in lane d (members 0..2)
 int launch_el_prec_lanes(const LaunchCtx& c, int sf, int su, double mu, double lam, double scr);
 int launch_bricks(const LaunchCtx& c, const BrickJob* jobs_host, int njobs, const double* taps_host, int tap_stride,

@@ -680,9 +680,10 @@ int RDEnsemble::el_prec_lanes(int cf, int cu, double mu, double lam, double scr)
   return 0;
 }
 
-int RDEnsemble::coef_transform(int cs_in, int coef_rs, int cs_out) {
+int RDEnsemble::lin_coef_transform(const int* nterms, const int (*lane)[4], const int (*slot)[4], const int (*dir)[4],
+                                   int coef_rs, int cs_out) {
   LaunchCtx c = ctx();
-  GLIA_TRYC(launch_col_plain(c, 0, +1, cs_in, CS_T7, false));
+  GLIA_TRYC(launch_col_lin(c, nterms, lane, slot, dir, CS_T7));
   GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T7, CS_T7, false));
   GLIA_TRYC(launch_z_point(c, ZPM_COEF, CS_T7, CS_T7, coef_rs, -1, 0.f, false));
   GLIA_TRYC(launch_col_plain(c, 1, -1, CS_T7, CS_T7, false));
@@ -690,9 +691,10 @@ int RDEnsemble::coef_transform(int cs_in, int coef_rs, int cs_out) {
   return 0;
 }
 
-int RDEnsemble::screen_transform(int cs_in, int u_rs, int out_rs) {
+int RDEnsemble::lin_screen_transform(const int* nterms, const int (*lane)[4], const int (*slot)[4],
+                                     const int (*dir)[4], int u_rs, int out_rs) {
   LaunchCtx c = ctx();
-  GLIA_TRYC(launch_col_plain(c, 0, +1, cs_in, CS_T7, false));
+  GLIA_TRYC(launch_col_lin(c, nterms, lane, slot, dir, CS_T7));
   GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T7, CS_T7, false));
   GLIA_TRYC(launch_z_point(c, ZPM_SCREEN, CS_T7, u_rs, out_rs, -1, 0.f, false));
   return 0;

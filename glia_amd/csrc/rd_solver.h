@@ -96,11 +96,14 @@ class RDEnsemble {
   void set_lanes(int k) { lanes_ = k; }
   int lane_lin(const int* nterms, const int (*lane)[4], const int (*slot)[4], const int (*dir)[4], int cout);
   int el_prec_lanes(int cf, int cu, double mu, double lam, double scr);
-  // out spectrum cs_out = FFT(m IFFT(cs_in)) with m the real slot coef_rs, the pointwise product in
+  // out spectrum cs_out = FFT(m IFFT(X)) with m the real slot coef_rs and X the cross-lane
+  // combination of the terms (launch_lane_lin's), formed in the first inverse pass; the product is in
   // the Z pass between the two transforms (CS_T7 as work)
-  int coef_transform(int cs_in, int coef_rs, int cs_out);
-  // real slot out_rs = IFFT(cs_in) - s u with s in RS_RHO and u the real slot u_rs (CS_T7 as work)
-  int screen_transform(int cs_in, int u_rs, int out_rs);
+  int lin_coef_transform(const int* nterms, const int (*lane)[4], const int (*slot)[4], const int (*dir)[4],
+                         int coef_rs, int cs_out);
+  // real slot out_rs = IFFT(X) - s u with X the combination, s in RS_RHO and u the real slot u_rs
+  int lin_screen_transform(const int* nterms, const int (*lane)[4], const int (*slot)[4], const int (*dir)[4],
+                           int u_rs, int out_rs);
   int get_spectrum(int m, int cslot, float2* host);
   int set_spectrum(int m, int cslot, const float2* host);
   int solve_state(int linearized, bool keep_history);
