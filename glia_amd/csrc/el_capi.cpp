@@ -169,28 +169,21 @@ int apply_op(glia_el_s* h, int u, int f) {
   return 0;
 }
 
-// u = B f on the lane vectors
+// u = B f on the lane vectors; the inverse's Z pass leaves the block partials of <u, u> and <u, f>
 int apply_prec(glia_el_s* h, int f, int u) {
   Lanes use(h->e.get(), 3);
   EL_ENG(h, h->e->fft_r2c(kVecSlot[f], CU));
   EL_ENG(h, h->e->el_prec_lanes(CU, CS_PC, h->avg[0], h->avg[1], h->avg[2]));
-  EL_ENG(h, h->e->fft_c2r(CS_PC, kVecSlot[u]));
+  EL_ENG(h, h->e->c2r_dots(CS_PC, kVecSlot[u], kVecSlot[f]));
   return 0;
 }
 
-// nsum scalars of interleaved block partials, summed on the device, read with one synchronization
-int read_scalars(glia_el_s* h, int nsum, double* out) {
-  const int np = glia::el::partial_count();
-  double* dst = h->part + 3 * np;
-  EL_HIP(h, glia::el::launch_sum_partials(h->st, h->part, nsum, dst));
-  EL_HIP(h, hipMemcpyAsync(out, dst, sizeof(double) * nsum, hipMemcpyDeviceToHost, h->st));
-  EL_HIP(h, hipStreamSynchronize(h->st));
+// the partials the last Z pass left (nq per block over the three lanes), summed on the device into
+// dst[0..nq)
+int lane_sums(glia_el_s* h, int nq, double* dst) {
+  Lanes use(h->e.get(), 3);
+  EL_ENG(h, h->e->lane_part_sum(nq, dst));
   return 0;
-}
-
-int dot(glia_el_s* h, int x, int y, double* out) {
-  EL_HIP(h, glia::el::launch_dot3(h->st, h->ng, cvec(h, x), cvec(h, y), h->part));
-  return read_scalars(h, 1, out);
 }
 
 bool valid_kind(int k) { return k == GLIA_RD_HOST || k == GLIA_RD_DEVICE; }
@@ -406,16 +399,26 @@ int glia_el_solve(glia_el_t h, const float* const f[3], float* const u[3], int m
   int rc;
   if ((rc = stage_in(h, f, mem_kind, VB)) || (rc = stage_in(h, u, mem_kind, VX))) return rc;
   // KSPSolve_CG (PETSc, KSP_NORM_PRECONDITIONED, nonzero initial guess) with KSPConvergedDefault.
-  // One host synchronization per iteration: <p, A p> stays on the device, where the update takes
-  // its step from it (and skips itself when the host is going to stop on it); the host reads it
-  // with <z, z> and <z, r> at the end of the iteration.
+  // One host synchronization per iteration: <p, A p> (from the operator's last Z pass) stays on the
+  // device, where the update takes its step from it (and skips itself when the host is going to stop
+  // on it); the host reads it with <z, z> and <z, r> (from the preconditioner's inverse pass) at the
+  // end of the iteration.
   if ((rc = apply_op(h, VX, VR))) return rc;
   EL_HIP(h, glia::el::launch_axpby3(h->st, h->ng, vec(h, VR), cvec(h, VB), 0.0, 1));  // r = b - A x
-  double rnorm0, dp, beta = 0, betaold = 0, dpi = 0, dpiold, a;
-  if ((rc = apply_prec(h, VB, VZ)) || (rc = dot(h, VZ, VZ, &rnorm0))) return rc;  // ||B b||
-  rnorm0 = std::sqrt(rnorm0);
-  if ((rc = apply_prec(h, VR, VZ)) || (rc = dot(h, VZ, VZ, &dp))) return rc;
-  dp = std::sqrt(dp);
+  double rnorm0, dp, beta = 0, betaold = 0, dpi = 0, dpiold;
+  const int np = glia::el::partial_count();
+  double* sc_dev = h->part + 3 * np;  // device scalars: <p, A p>, then <z, z>, <z, r>
+  double sc[3] = {0, 0, 0};
+  auto read3 = [&]() -> int {
+    EL_HIP(h, hipMemcpyAsync(sc, sc_dev, sizeof(sc), hipMemcpyDeviceToHost, h->st));
+    EL_HIP(h, hipStreamSynchronize(h->st));
+    return 0;
+  };
+  // ||B b||, then ||B r|| and <B r, r>, each from the preconditioner's inverse pass
+  if ((rc = apply_prec(h, VB, VZ)) || (rc = lane_sums(h, 2, sc_dev + 1)) || (rc = read3())) return rc;
+  rnorm0 = std::sqrt(sc[1]);
+  if ((rc = apply_prec(h, VR, VZ)) || (rc = lane_sums(h, 2, sc_dev + 1)) || (rc = read3())) return rc;
+  dp = std::sqrt(sc[1]);
   if (rnorm0 == 0) rnorm0 = dp;
   const double ttol = std::fmax(rtol * rnorm0, atol);
   auto converged = [&](double rn) {
@@ -424,11 +427,10 @@ int glia_el_solve(glia_el_t h, const float* const f[3], float* const u[3], int m
     if (rn >= dtol * rnorm0) return (int)KSP_DIVERGED_DTOL;
     return 0;
   };
-  const int np = glia::el::partial_count();
-  double* dpi_dev = h->part + 3 * np;  // read_scalars' destination: <p, A p> at [0], then <z, z>, <z, r>
+  double* dpi_dev = sc_dev;
   int its = 0, why = converged(dp), i = 0;
   if (!why) {
-    if ((rc = dot(h, VZ, VR, &beta))) return rc;
+    beta = sc[2];
     do {
       its = i + 1;
       if (beta == 0.0) {
@@ -440,19 +442,13 @@ int glia_el_solve(glia_el_t h, const float* const f[3], float* const u[3], int m
       }
       EL_HIP(h, glia::el::launch_axpby3(h->st, h->ng, vec(h, VP), cvec(h, VZ), i ? beta / betaold : 0.0, 0));
       dpiold = dpi;
-      if ((rc = apply_op(h, VP, VW))) return rc;
-      EL_HIP(h, glia::el::launch_dot3(h->st, h->ng, cvec(h, VP), cvec(h, VW), h->part));
-      EL_HIP(h, glia::el::launch_sum_partials(h->st, h->part, 1, dpi_dev));
+      // <p, A p> from the operator's last Z pass
+      if ((rc = apply_op(h, VP, VW)) || (rc = lane_sums(h, 1, dpi_dev))) return rc;
       EL_HIP(h, glia::el::launch_cg_update_dev(h->st, h->ng, vec(h, VX), vec(h, VR), cvec(h, VP), cvec(h, VW), dpi_dev,
                                                beta, dpiold, i == 0 ? 1 : 0));
       betaold = beta;
-      // <z, z> for the test and <z, r> for the next beta in one pass
-      if ((rc = apply_prec(h, VR, VZ))) return rc;
-      EL_HIP(h, glia::el::launch_dot3_zz_zr(h->st, h->ng, cvec(h, VZ), cvec(h, VR), h->part));
-      EL_HIP(h, glia::el::launch_sum_partials(h->st, h->part, 2, dpi_dev + 1));
-      double sc[3];
-      EL_HIP(h, hipMemcpyAsync(sc, dpi_dev, sizeof(sc), hipMemcpyDeviceToHost, h->st));
-      EL_HIP(h, hipStreamSynchronize(h->st));
+      // <z, z> for the test and <z, r> for the next beta from the preconditioner's inverse pass
+      if ((rc = apply_prec(h, VR, VZ)) || (rc = lane_sums(h, 2, dpi_dev + 1)) || (rc = read3())) return rc;
       dpi = sc[0];
       if (std::isnan(dpi) || std::isinf(dpi)) {
         why = KSP_DIVERGED_NANORINF;
@@ -462,8 +458,6 @@ int glia_el_solve(glia_el_t h, const float* const f[3], float* const u[3], int m
         why = KSP_DIVERGED_INDEFINITE_MAT;
         break;
       }
-      a = beta / dpi;
-      (void)a;
       dp = std::sqrt(sc[1]);
       if ((why = converged(dp))) break;
       beta = sc[2];

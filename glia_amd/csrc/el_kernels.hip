@@ -50,65 +50,6 @@ __global__ void __launch_bounds__(256) k_el_material(int64_t ng, const float* __
   if (threadIdx.x < 3) part[blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// per-block FP64 partials of <x, y> over the three components
-__global__ void __launch_bounds__(256) k_dot3(int64_t ng, Vec3C x, Vec3C y, double* __restrict__ part) {
-  double s = 0.0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int k = 0; k < 3; k++)
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += stride)
-      s += (double)x.p[k][i] * (double)y.p[k][i];
-  __shared__ double red[256];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
-
-// per-block FP64 partials of <z, z> and <z, r> in one pass (part[2 b], part[2 b + 1])
-__global__ void __launch_bounds__(256) k_dot3_zz_zr(int64_t ng, Vec3C z, Vec3C r, double* __restrict__ part) {
-  double s0 = 0.0, s1 = 0.0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int k = 0; k < 3; k++)
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += stride) {
-      const double zi = z.p[k][i];
-      s0 += zi * zi;
-      s1 += zi * (double)r.p[k][i];
-    }
-  __shared__ double red[2][256];
-  red[0][threadIdx.x] = s0;
-  red[1][threadIdx.x] = s1;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + w];
-      red[1][threadIdx.x] += red[1][threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x < 2) part[2 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
-}
-
-// fixed-order sums of nsum interleaved partial arrays (part[nsum b + s], b < np) into out[s]
-__global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ part, int np, int nsum,
-                                                      double* __restrict__ out) {
-  __shared__ double red[256];
-  for (int s = 0; s < nsum; s++) {
-    double v = 0.0;
-    for (int b = threadIdx.x; b < np; b += 256) v += part[(int64_t)nsum * b + s];
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[s] = red[0];
-    __syncthreads();
-  }
-}
-
 // the CG update with the step a = beta / dpi taken on the device, dpi = <p, A p> from *dpi; the
 // update is skipped when the host will stop on dpi (NaN or infinite, zero, or a sign change from
 // dpiold after the first iteration), so x and r stay as the reference leaves them
@@ -154,18 +95,6 @@ int launch_material(hipStream_t st, int64_t ng, const float* c, const float* wm,
 }
 int launch_force_scale(hipStream_t st, int64_t ng, const float* g, const float* c, float ff, int use_tanh, float* out) {
   k_force_scale<<<EB, 256, 0, st>>>(ng, g, c, ff, use_tanh, out);
-  return (int)hipGetLastError();
-}
-int launch_dot3(hipStream_t st, int64_t ng, const Vec3C& x, const Vec3C& y, double* part) {
-  k_dot3<<<EB, 256, 0, st>>>(ng, x, y, part);
-  return (int)hipGetLastError();
-}
-int launch_dot3_zz_zr(hipStream_t st, int64_t ng, const Vec3C& z, const Vec3C& r, double* part) {
-  k_dot3_zz_zr<<<EB, 256, 0, st>>>(ng, z, r, part);
-  return (int)hipGetLastError();
-}
-int launch_sum_partials(hipStream_t st, const double* part, int nsum, double* out) {
-  k_sum_partials<<<1, 256, 0, st>>>(part, EB, nsum, out);
   return (int)hipGetLastError();
 }
 int launch_cg_update_dev(hipStream_t st, int64_t ng, const Vec3& x, const Vec3& r, const Vec3C& p, const Vec3C& w,

@@ -23,12 +23,7 @@ int launch_material(hipStream_t st, int64_t ng, const float* c, const float* wm,
                     const float* vt, const float* bg, const ElMaterial& m, float* mu, float* lam, float* scr,
                     double* part);
 int launch_force_scale(hipStream_t st, int64_t ng, const float* g, const float* c, float ff, int use_tanh, float* out);
-int launch_dot3(hipStream_t st, int64_t ng, const Vec3C& x, const Vec3C& y, double* part);
-// <z, z> and <z, r> partials, interleaved
-int launch_dot3_zz_zr(hipStream_t st, int64_t ng, const Vec3C& z, const Vec3C& r, double* part);
 int launch_axpby3(hipStream_t st, int64_t ng, const Vec3& p, const Vec3C& z, double b, int sub);
-// out[s] = sum over the partial_count() blocks of part[nsum b + s], in a fixed order, on the device
-int launch_sum_partials(hipStream_t st, const double* part, int nsum, double* out);
 // x += a p, r -= a w with a = beta / *dpi, skipped when *dpi would stop the iteration
 int launch_cg_update_dev(hipStream_t st, int64_t ng, const Vec3& x, const Vec3& r, const Vec3C& p, const Vec3C& w,
                          const double* dpi, double beta, double dpiold, int first);

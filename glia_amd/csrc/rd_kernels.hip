@@ -860,9 +860,9 @@ __device__ __forceinline__ void zl_inverse(float2* v, float2* line, int t, const
 // real output of a Z line kernel (C2R times scale), stored coalesced: the values go back to the
 // line buffer at their line positions and the block writes its lines in order.  With SCREEN the
 // stored value is c - a u, a and u read in the same order (the screened elasticity operator).
-template <int M, bool SCREEN>
+template <int M, bool SCREEN, int NDOT>
 __device__ __forceinline__ void zl_real_store(const float2* v, float2* lb, float2* line, int t, float scale, GPtr out,
-                                              GPtr ap, GPtr up) {
+                                              GPtr ap, GPtr up, double* acc) {
   constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT;
   __syncthreads();  // every thread's inverse has read the line buffer
 #pragma unroll
@@ -876,9 +876,16 @@ __device__ __forceinline__ void zl_real_store(const float2* v, float2* lb, float
     const float2 c = lb[li * ZK<M>::LP + xpad(k)];
     if (SCREEN) {
       const float2 a = ap[e], u = up[e];
-      out[e] = make_float2(c.x - a.x * u.x, c.y - a.y * u.y);
+      const float2 w = make_float2(c.x - a.x * u.x, c.y - a.y * u.y);
+      out[e] = w;
+      if (NDOT == 1) acc[0] += (double)u.x * (double)w.x + (double)u.y * (double)w.y;  // <u, w>
     } else {
       out[e] = c;
+      if (NDOT == 2) {  // <c, c> and <c, r> with r from up
+        const float2 r = up[e];
+        acc[0] += (double)c.x * (double)c.x + (double)c.y * (double)c.y;
+        acc[1] += (double)c.x * (double)r.x + (double)c.y * (double)r.y;
+      }
     }
   }
 }
@@ -975,9 +982,12 @@ k_z_diff(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
 //   ZR_COEF     : c = C2R/N^3; out = R2C(m c) with m the real slot r_slot (a Lame coefficient times a
 //                 strain term of the elasticity operator)
 //   ZR_SCREEN   : c = C2R/N^3; real slot r_slot = c - s u with s in RS_RHO and u the real slot s_out
-//                 (the screened elasticity operator's output; no forward transform)
+//                 (the screened elasticity operator's output; no forward transform); block partials
+//                 of <u, out> (the CG's <p, A p>)
+//   ZR_C2R_DOT  : real slot r_slot = c = C2R/N^3, with block partials of <c, c> and <c, r>, r the real
+//                 slot s_out (the preconditioned residual's norm and <z, r>)
 enum { ZR_REACT = 0, ZR_REACT_ADJ = 1, ZR_C2R = 2, ZR_R2C = 3, ZR_MISFIT = 4, ZR_OBS2 = 5, ZR_COEF = 6,
-       ZR_SCREEN = 7 };
+       ZR_SCREEN = 7, ZR_C2R_DOT = 8 };
 
 template <int M, int MODE>
 __global__ void __launch_bounds__(ZK<M>::TPB, GLIA_Z_MINW)
@@ -989,7 +999,8 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   // pointwise operands are loaded before the transform when the registers fit
   constexpr bool PF = VPT <= 16;
   constexpr bool TWO = (MODE == ZR_REACT_ADJ || MODE == ZR_MISFIT);
-  constexpr bool REAL_OUT = (MODE == ZR_C2R || MODE == ZR_SCREEN);  // no forward transform
+  constexpr bool REAL_OUT = (MODE == ZR_C2R || MODE == ZR_SCREEN || MODE == ZR_C2R_DOT);  // no forward transform
+  constexpr int NDOT = MODE == ZR_SCREEN ? 1 : (MODE == ZR_C2R_DOT ? 2 : 0);
   __shared__ float2 lb[ZK<M>::NL * ZK<M>::LP];
   __shared__ float2 twml[M], tw2l[M];
   const int li = threadIdx.x / NT, t = threadIdx.x % NT;
@@ -1000,10 +1011,11 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   GPtr rf = gptr(ch_idx >= 0 ? (const void*)Mb.chalf[ch_idx] : (r_slot >= 0 ? (const void*)Mb.r[r_slot] : nullptr));
   if (rf) rf = rf + base;
   const float s = (float)d.inv_ng;
-  double acc[1] = {0.0};
+  double acc[2] = {0.0, 0.0};
   float2 v[VPT];
   GPtr op1 = MODE == ZR_COEF ? rf : rptr2(Mb, (MODE == ZR_MISFIT || MODE == ZR_OBS2) ? RS_OBS : RS_RHO) + base;
-  GPtr op2 = MODE == ZR_MISFIT ? rptr2(Mb, RS_D1) + base : (MODE == ZR_SCREEN ? rptr2(Mb, s_out) + base : rf);
+  GPtr op2 = MODE == ZR_MISFIT ? rptr2(Mb, RS_D1) + base
+                               : ((MODE == ZR_SCREEN || MODE == ZR_C2R_DOT) ? rptr2(Mb, s_out) + base : rf);
   GPtr ctp = rptr2(Mb, RS_CT) + base;
   float2 o1[PF ? VPT : 1], o2[PF && TWO ? VPT : 1];
   if (MODE == ZR_R2C) {
@@ -1027,7 +1039,13 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   }
   if (REAL_OUT) {
     zl_inverse<M>(v, line, t, twml, tw2l);
-    zl_real_store<M, MODE == ZR_SCREEN>(v, lb, line, t, s, rf, op1, op2);
+    zl_real_store<M, MODE == ZR_SCREEN, NDOT>(v, lb, line, t, s, rf, op1, op2, acc);
+    if (NDOT == 1) {
+      double a1[1] = {acc[0]};
+      block_reduce_store<1>(a1, Mb.part + blockIdx.x);
+    } else if (NDOT == 2) {
+      block_reduce_store<2>(acc, Mb.part + (int64_t)blockIdx.x * 2);
+    }
   } else if (MODE != ZR_R2C) {
     zl_inverse<M>(v, line, t, twml, tw2l);
 #pragma unroll
@@ -1064,7 +1082,10 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
     }
   }
   if (!REAL_OUT) zl_forward_store<M>(v, lb, line, t, twml, tw2l, cptr(Mb, s_out), base, false);
-  if (MODE == ZR_MISFIT) block_reduce_store<1>(acc, Mb.part + blockIdx.x);
+  if (MODE == ZR_MISFIT) {
+    double a1[1] = {acc[0]};
+    block_reduce_store<1>(a1, Mb.part + blockIdx.x);
+  }
 }
 
 // Z pass of the gradient quadrature at one time point: from the y-transformed fields of c
@@ -1772,6 +1793,26 @@ __global__ void __launch_bounds__(256) k_el_prec(const MemberDev* __restrict__ m
   }
 }
 
+// fixed-order sums of the Z-pass block partials of the first nl members: out[q] = sum over lanes l
+// and blocks b < nb of part_l[nq b + q] (one block)
+__global__ void __launch_bounds__(256) k_lane_part_sum(const MemberDev* __restrict__ mem, int nl, int nb, int nq,
+                                                       double* __restrict__ out) {
+  __shared__ double red[256];
+  for (int q = 0; q < nq; q++) {
+    double v = 0.0;
+    for (int l = 0; l < nl; l++)
+      for (int b = threadIdx.x; b < nb; b += 256) v += mem[l].part[(int64_t)nq * b + q];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = red[0];
+    __syncthreads();
+  }
+}
+
 // k_el_prec across lanes: F_d in slot sf of lane d, U_d into slot su of lane d (one pass, grid.y = 1)
 __global__ void __launch_bounds__(256) k_el_prec_lanes(const MemberDev* __restrict__ mem, Dims d, int sf, int su,
                                                        float mu, float lam, float scr) {
@@ -2003,6 +2044,7 @@ struct RunZPoint {
       case ZR_OBS2: GLIA_ZP(ZR_OBS2); break;
       case ZR_COEF: GLIA_ZP(ZR_COEF); break;
       case ZR_SCREEN: GLIA_ZP(ZR_SCREEN); break;
+      case ZR_C2R_DOT: GLIA_ZP(ZR_C2R_DOT); break;
       default: return -1;
     }
 #undef GLIA_ZP
@@ -2197,6 +2239,12 @@ int launch_lane_lin(const LaunchCtx& c, const int* nterms, const int (*lane)[4],
     case 3: k_lane_lin<3><<<g, 256, 0, c.stream>>>(c.mem, c.d, T, s_out); break;
     default: k_lane_lin<4><<<g, 256, 0, c.stream>>>(c.mem, c.d, T, s_out); break;
   }
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+
+int launch_lane_part_sum(const LaunchCtx& c, int nq, double* out) {
+  k_lane_part_sum<<<1, 256, 0, c.stream>>>(c.mem, c.nm, z_blocks(c.d), nq, out);
   GLIA_CHECK_LAUNCH();
   return 0;
 }

@@ -24,7 +24,7 @@ struct LaunchCtx {
 // modes of launch_xf_div and launch_z_point (values match the kernel enums)
 enum { XFM_MATVEC = 0, XFM_SETUP = 1, XFM_D = 2 };
 enum { ZPM_REACT = 0, ZPM_REACT_ADJ = 1, ZPM_C2R = 2, ZPM_R2C = 3, ZPM_MISFIT = 4, ZPM_OBS2 = 5, ZPM_COEF = 6,
-       ZPM_SCREEN = 7 };
+       ZPM_SCREEN = 7, ZPM_C2R_DOT = 8 };
 enum { FINK_FILTER = 0, FINK_KSUM = 1, FINK_MISFIT = 2, FINK_C0SQ = 3, FINK_QUAD = 4 };
 
 bool supported_line(int L);
@@ -75,6 +75,8 @@ int launch_lane_lin(const LaunchCtx& c, const int* nterms, const int (*lane)[4],
 // the inverse X pass of the same combination, formed as the columns load, into complex slot s_out
 int launch_col_lin(const LaunchCtx& c, const int* nterms, const int (*lane)[4], const int (*slot)[4],
                    const int (*dir)[4], int s_out);
+// out[q], q < nq: the Z-pass block partials (nq per block) of the c.nm lanes, summed on the device
+int launch_lane_part_sum(const LaunchCtx& c, int nq, double* out);
 // the elasticity preconditioner with F_d and U_d in lane d (members 0..2)
 int launch_el_prec_lanes(const LaunchCtx& c, int sf, int su, double mu, double lam, double scr);
 int launch_bricks(const LaunchCtx& c, const BrickJob* jobs_host, int njobs, const double* taps_host, int tap_stride,

@@ -700,6 +700,19 @@ int RDEnsemble::lin_screen_transform(const int* nterms, const int (*lane)[4], co
   return 0;
 }
 
+int RDEnsemble::c2r_dots(int cs_in, int out_rs, int r_rs) {
+  LaunchCtx c = ctx();
+  GLIA_TRYC(launch_col_plain(c, 0, +1, cs_in, CS_T7, false));
+  GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T7, CS_T7, false));
+  GLIA_TRYC(launch_z_point(c, ZPM_C2R_DOT, CS_T7, r_rs, out_rs, -1, 0.f, false));
+  return 0;
+}
+
+int RDEnsemble::lane_part_sum(int nq, double* out) {
+  GLIA_TRYC(launch_lane_part_sum(ctx(), nq, out));
+  return 0;
+}
+
 int RDEnsemble::el_prec(const int f[3], const int u[3], double mu, double lam, double scr) {
   GLIA_TRYC(launch_el_prec(ctx(), make_int4(f[0], f[1], f[2], 0), make_int4(u[0], u[1], u[2], 0), mu, lam, scr));
   return 0;

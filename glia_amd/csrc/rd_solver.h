@@ -104,6 +104,12 @@ class RDEnsemble {
   // real slot out_rs = IFFT(X) - s u with X the combination, s in RS_RHO and u the real slot u_rs
   int lin_screen_transform(const int* nterms, const int (*lane)[4], const int (*slot)[4], const int (*dir)[4],
                            int u_rs, int out_rs);
+  // real slot out_rs = IFFT(cs_in), with per-block partials of <out, out> and <out, r> (r the real
+  // slot r_rs) left for lane_part_sum (CS_T7 as work)
+  int c2r_dots(int cs_in, int out_rs, int r_rs);
+  // out[q] (device), q < nq: the last Z pass's block partials of the lanes in use, summed in a fixed
+  // order (lin_screen_transform leaves <u, out>, c2r_dots <out, out> and <out, r>)
+  int lane_part_sum(int nq, double* out);
   int get_spectrum(int m, int cslot, float2* host);
   int set_spectrum(int m, int cslot, const float2* host);
   int solve_state(int linearized, bool keep_history);
