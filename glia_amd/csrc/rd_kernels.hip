@@ -57,6 +57,36 @@
 namespace glia {
 using namespace dev;
 
+// Non-temporal bits of the passes (bit 0 loads, bit 1 stores; see GRefT), chosen by timing: on the
+// 8 x 256^3 evaluation the CG passes gain about 5% over none (the first X pass streams its loads
+// only: the next pass reads its outputs at once); on the 160^3 elasticity solve the plain column
+// transforms and the pointwise Z passes gain about 4% from streamed stores alone.
+#ifndef GLIA_NT_XI
+#define GLIA_NT_XI 1
+#endif
+#ifndef GLIA_NT_YI
+#define GLIA_NT_YI 3
+#endif
+#ifndef GLIA_NT_ZD
+#define GLIA_NT_ZD 2
+#endif
+#ifndef GLIA_NT_YF
+#define GLIA_NT_YF 3
+#endif
+#ifndef GLIA_NT_XF
+#define GLIA_NT_XF 3
+#endif
+#ifndef GLIA_NT_PU
+#define GLIA_NT_PU 3
+#endif
+// the plain column transforms and the Z passes with a pointwise stage
+#ifndef GLIA_NT_COL
+#define GLIA_NT_COL 2
+#endif
+#ifndef GLIA_NT_ZP
+#define GLIA_NT_ZP 2
+#endif
+
 // ---------------------------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------------------------
@@ -65,34 +95,71 @@ using namespace dev;
 #define GLIA_GLOBAL __attribute__((address_space(1)))
 typedef float v2f __attribute__((ext_vector_type(2)));
 typedef float v4f __attribute__((ext_vector_type(4)));
-struct GRef {
+// Cache policy of a kernel's global accesses, a template argument of its pointers: bit 0 makes
+// the loads, bit 1 the stores non-temporal (the nt bit: the line streams through the caches instead
+// of being kept for reuse).  Each pass sets its own (GLIA_NT_* below).
+template <int H>
+struct GRefT {
   GLIA_GLOBAL v2f* p;
   __device__ __forceinline__ operator float2() const {
-    const v2f a = *p;
+    v2f a;
+    if constexpr ((H & 1) != 0) {
+      a = __builtin_nontemporal_load(p);
+    } else {
+      a = *p;
+    }
     return make_float2(a.x, a.y);
   }
-  __device__ __forceinline__ GRef& operator=(float2 v) {
+  __device__ __forceinline__ GRefT& operator=(float2 v) {
     v2f a;
     a.x = v.x;
     a.y = v.y;
-    *p = a;
+    if constexpr ((H & 2) != 0) {
+      __builtin_nontemporal_store(a, p);
+    } else {
+      *p = a;
+    }
     return *this;
   }
 };
-struct GPtr {
+template <int H>
+struct GPtrT {
   GLIA_GLOBAL v2f* p;
-  __device__ __forceinline__ GRef operator[](unsigned i) const { return GRef{p + i}; }
-  __device__ __forceinline__ GPtr operator+(int64_t o) const { return GPtr{p + o}; }
+  __device__ __forceinline__ GRefT<H> operator[](unsigned i) const { return GRefT<H>{p + i}; }
+  __device__ __forceinline__ GPtrT operator+(int64_t o) const { return GPtrT{p + o}; }
   __device__ __forceinline__ explicit operator bool() const { return p != nullptr; }
 };
-__device__ __forceinline__ GPtr gptr(const void* q) { return GPtr{(GLIA_GLOBAL v2f*)q}; }
+using GPtr = GPtrT<0>;
+// 16-byte accesses of the elementwise passes under the same policy bits
+template <int H>
+__device__ __forceinline__ v4f ld4(GLIA_GLOBAL v4f* p) {
+  if constexpr ((H & 1) != 0) {
+    return __builtin_nontemporal_load(p);
+  } else {
+    return *p;
+  }
+}
+template <int H>
+__device__ __forceinline__ void st4(GLIA_GLOBAL v4f* p, v4f v) {
+  if constexpr ((H & 2) != 0) {
+    __builtin_nontemporal_store(v, p);
+  } else {
+    *p = v;
+  }
+}
+template <int H = 0>
+__device__ __forceinline__ GPtrT<H> gptr(const void* q) { return GPtrT<H>{(GLIA_GLOBAL v2f*)q}; }
 
 // complex slot s of a member: work slots below CS_NUM, state-pool entries (history/checkpoints) above
-__device__ __forceinline__ GPtr cptr(const MemberDev& M, int s) { return gptr((s < CS_NUM) ? M.c[s] : M.hist[s - CS_NUM]); }
+template <int H = 0>
+__device__ __forceinline__ GPtrT<H> cptr(const MemberDev& M, int s) {
+  return gptr<H>((s < CS_NUM) ? M.c[s] : M.hist[s - CS_NUM]);
+}
 // the same slot as a plain pointer (elementwise kernels)
 __device__ __forceinline__ float2* craw(const MemberDev& M, int s) { return (s < CS_NUM) ? M.c[s] : M.hist[s - CS_NUM]; }
 // a real field viewed as float2 pairs (x[2n], x[2n+1])
-__device__ __forceinline__ GPtr rptr2(const MemberDev& M, int r) { return gptr(M.r[r]); }
+template <int H = 0>
+__device__ __forceinline__ GPtrT<H> rptr2(const MemberDev& M, int r) { return gptr<H>(M.r[r]); }
 
 // wave number with the Nyquist mode zeroed (reference src/cuda/SpectralOperators.cu:63-72)
 __device__ __forceinline__ float wnum(int k, int n) {
@@ -247,18 +314,19 @@ struct LaneTerms {
 template <int L, int NTM>
 __global__ void __launch_bounds__(CK<L>::TPB, GLIA_COL_MINW)
 k_col_lin(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, LaneTerms T, int s_out) {
+  constexpr int H = GLIA_NT_COL;
   const int l = blockIdx.y;
   const MemberDev& M = mem[l];
   GLIA_COL_PRELUDE(L)
   const Tile tl = col_tile(0, d);
-  GPtr src[NTM];
+  GPtrT<H> src[NTM];
   int dir[NTM];
 #pragma unroll
   for (int q = 0; q < NTM; q++) {
-    src[q] = cptr(mem[T.lane[l][q]], T.slot[l][q]) + tl.base;
+    src[q] = cptr<H>(mem[T.lane[l][q]], T.slot[l][q]) + tl.base;
     dir[q] = T.dir[l][q];
   }
-  GPtr O = cptr(M, s_out) + tl.base;
+  GPtrT<H> O = cptr<H>(M, s_out) + tl.base;
   // this tile's columns: ky = tile index o, kz = c0 + col; the rows run over kx
   const int ky = tl.o, kz = tl.c0 + col;
   const float wy = (2 * ky == d.n1) ? 0.0f : (float)(2 * ky < d.n1 ? ky : ky - d.n1), wz = (float)kz;
@@ -290,16 +358,17 @@ k_col_lin(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
 template <int L, bool PUPD>
 __global__ void __launch_bounds__(CK<L>::TPB, CK<L>::VPT <= 16 ? GLIA_XI_MINW : 2)
 k_xi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_in, int s_r, int s_u, int s_v) {
+  constexpr int H = GLIA_NT_XI;
   const MemberDev& M = mem[blockIdx.y];
   const PcgState* st = M.st;
   if (st->done) return;
   GLIA_COL_PRELUDE(L)
   const Tile tl = col_tile(0, d);
   const float wy = wnum(tl.o, d.n1), wz = (float)(tl.c0 + col);
-  GPtr pin = cptr(M, s_in) + tl.base;
-  GPtr rr = cptr(M, s_r) + tl.base;
-  GPtr U = cptr(M, s_u) + tl.base;
-  GPtr V = cptr(M, s_v) + tl.base;
+  GPtrT<H> pin = cptr<H>(M, s_in) + tl.base;
+  GPtrT<H> rr = cptr<H>(M, s_r) + tl.base;
+  GPtrT<H> U = cptr<H>(M, s_u) + tl.base;
+  GPtrT<H> V = cptr<H>(M, s_v) + tl.base;
   bool first = false;
   float bc = 0.f, pc = 0.f;
   if (PUPD) {
@@ -357,17 +426,18 @@ template <int L, bool WITH0>
 __global__ void __launch_bounds__(CK<L>::TPB, CK<L>::VPT <= 16 ? GLIA_YI_MINW : 2)
 k_yi_grad(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_u, int s_v, int s_gx, int s_gy,
           int s_gz, int s_g0) {
+  constexpr int H = GLIA_NT_YI;
   const MemberDev& M = mem[blockIdx.y];
   if (M.st->done) return;
   GLIA_COL_PRELUDE(L)
   const Tile tl = col_tile(1, d);
   const float wz = (float)(tl.c0 + col);
-  GPtr U = cptr(M, s_u) + tl.base;
-  GPtr V = cptr(M, s_v) + tl.base;
-  GPtr Gx = cptr(M, s_gx) + tl.base;
-  GPtr Gy = cptr(M, s_gy) + tl.base;
-  GPtr Gz = cptr(M, s_gz) + tl.base;
-  GPtr G0 = WITH0 ? cptr(M, s_g0) + tl.base : gptr(nullptr);
+  GPtrT<H> U = cptr<H>(M, s_u) + tl.base;
+  GPtrT<H> V = cptr<H>(M, s_v) + tl.base;
+  GPtrT<H> Gx = cptr<H>(M, s_gx) + tl.base;
+  GPtrT<H> Gy = cptr<H>(M, s_gy) + tl.base;
+  GPtrT<H> Gz = cptr<H>(M, s_gz) + tl.base;
+  GPtrT<H> G0 = WITH0 ? cptr<H>(M, s_g0) + tl.base : gptr<H>(nullptr);
   float2 a[K::VPT], b[K::VPT], u[K::PF ? K::VPT : 1];
   // both inputs are loaded up front (when registers allow) so that their latencies overlap
 #pragma unroll
@@ -408,15 +478,16 @@ template <int L>
 __global__ void __launch_bounds__(CK<L>::TPB, CK<L>::VPT <= 16 ? GLIA_YF_MINW : 2)
 k_yf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_hx, int s_hy, int s_q, int s_yx,
          int s_rr) {
+  constexpr int H = GLIA_NT_YF;
   const MemberDev& M = mem[blockIdx.y];
   if (M.st->done) return;
   GLIA_COL_PRELUDE(L)
   const Tile tl = col_tile(1, d);
-  GPtr Hx = cptr(M, s_hx) + tl.base;
-  GPtr Hy = cptr(M, s_hy) + tl.base;
-  GPtr Q = cptr(M, s_q) + tl.base;
-  GPtr Yx = cptr(M, s_yx) + tl.base;
-  GPtr Rr = cptr(M, s_rr) + tl.base;
+  GPtrT<H> Hx = cptr<H>(M, s_hx) + tl.base;
+  GPtrT<H> Hy = cptr<H>(M, s_hy) + tl.base;
+  GPtrT<H> Q = cptr<H>(M, s_q) + tl.base;
+  GPtrT<H> Yx = cptr<H>(M, s_yx) + tl.base;
+  GPtrT<H> Rr = cptr<H>(M, s_rr) + tl.base;
   float2 a[K::VPT], b[K::VPT], q[K::PF ? K::VPT : 1];
   // all three inputs are loaded up front (when registers allow) so that their latencies overlap
 #pragma unroll
@@ -462,15 +533,16 @@ enum { XF_MATVEC = 0, XF_SETUP = 1, XF_D = 2 };
 template <int L, int MODE>
 __global__ void __launch_bounds__(CK<L>::TPB, CK<L>::VPT <= 16 ? GLIA_XF_MINW : 2)
 k_xf_div(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_yx, int s_rr, int s_p, int s_out) {
+  constexpr int H = GLIA_NT_XF;
   const MemberDev& M = mem[blockIdx.y];
   const PcgState* st = M.st;
   if (st->done) return;
   GLIA_COL_PRELUDE(L)
   const Tile tl = col_tile(0, d);
-  GPtr Yx = cptr(M, s_yx) + tl.base;
-  GPtr Rr = cptr(M, s_rr) + tl.base;
-  GPtr P = cptr(M, s_p) + tl.base;
-  GPtr O = cptr(M, s_out) + tl.base;
+  GPtrT<H> Yx = cptr<H>(M, s_yx) + tl.base;
+  GPtrT<H> Rr = cptr<H>(M, s_rr) + tl.base;
+  GPtrT<H> P = cptr<H>(M, s_p) + tl.base;
+  GPtrT<H> O = cptr<H>(M, s_out) + tl.base;
   const float tau = (float)st->tau;
   const float pc = (float)st->pc;
   const float wy = wnum(tl.o, d.n1), wz = (float)(tl.c0 + col);
@@ -548,6 +620,7 @@ template <int L>
 __global__ void __launch_bounds__(3 * CK<L>::TPB)
 k_yf_div_s(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_hx, int s_hy, int s_q, int s_yx,
            int s_rr) {
+  constexpr int H = GLIA_NT_YF;
   static_assert(CKS<L>::OK, "thread groups must be whole waves");
   const MemberDev& M = mem[blockIdx.y];
   if (M.st->done) return;
@@ -559,7 +632,7 @@ k_yf_div_s(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__
   const int col = lt & 15, t = lt >> 4;
   float2* exc = ex[grp] + col * K::CP;
   const Tile tl = col_tile(1, d);
-  GPtr In = cptr(M, grp == 0 ? s_hx : (grp == 1 ? s_hy : s_q)) + tl.base;
+  GPtrT<H> In = cptr<H>(M, grp == 0 ? s_hx : (grp == 1 ? s_hy : s_q)) + tl.base;
   float2 a[K::VPT];
 #pragma unroll
   for (int j = 0; j < K::VIN; j++) a[j] = In[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
@@ -568,7 +641,7 @@ k_yf_div_s(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__
   fft2s<R1, R2, NT, -1>(a, exc, t, twl);
   __syncthreads();  // every exchange area is free again
   if (grp == 0) {
-    GPtr Yx = cptr(M, s_yx) + tl.base;
+    GPtrT<H> Yx = cptr<H>(M, s_yx) + tl.base;
 #pragma unroll
     for (int j = 0; j < K::VOUT; j++) if (K::out_ok(t, j)) Yx[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
   } else if (grp == 1) {
@@ -581,7 +654,7 @@ k_yf_div_s(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__
   }
   __syncthreads();
   if (grp == 2) {
-    GPtr Rr = cptr(M, s_rr) + tl.base;
+    GPtrT<H> Rr = cptr<H>(M, s_rr) + tl.base;
     const float2* exh = ex[1] + col * K::CP;
 #pragma unroll
     for (int j = 0; j < K::VOUT; j++) {
@@ -602,14 +675,15 @@ template <int L, int MINW>
 __global__ void __launch_bounds__(CK<L>::TPB, CK<L>::VPT <= 16 ? MINW : 2)
 k_yf_div_b(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_hx, int s_hy, int s_q, int s_yx,
            int s_rr) {
+  constexpr int H = GLIA_NT_YF;
   const MemberDev& M = mem[blockIdx.y];
   if (M.st->done) return;
   GLIA_COL_PRELUDE(L)
   const Tile tl = col_tile_at(1, d, (int)(blockIdx.x >> 1));
   __shared__ float2 twl[L];
   if ((blockIdx.x & 1) == 0) {  // uniform per block
-    GPtr Hx = cptr(M, s_hx) + tl.base;
-    GPtr Yx = cptr(M, s_yx) + tl.base;
+    GPtrT<H> Hx = cptr<H>(M, s_hx) + tl.base;
+    GPtrT<H> Yx = cptr<H>(M, s_yx) + tl.base;
     float2 a[K::VPT];
 #pragma unroll
     for (int j = 0; j < K::VIN; j++) a[j] = Hx[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
@@ -619,9 +693,9 @@ k_yf_div_b(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__
     for (int j = 0; j < K::VOUT; j++) if (K::out_ok(t, j)) Yx[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col] = a[j];
     return;
   }
-  GPtr Hy = cptr(M, s_hy) + tl.base;
-  GPtr Q = cptr(M, s_q) + tl.base;
-  GPtr Rr = cptr(M, s_rr) + tl.base;
+  GPtrT<H> Hy = cptr<H>(M, s_hy) + tl.base;
+  GPtrT<H> Q = cptr<H>(M, s_q) + tl.base;
+  GPtrT<H> Rr = cptr<H>(M, s_rr) + tl.base;
   float2 a[K::VPT], b[K::VPT];
   // Hy is loaded up front for short plans, else after the transform of Q (fewer live values)
   constexpr bool early = K::VPT <= 8;
@@ -653,6 +727,7 @@ template <int L, int MODE>
 __global__ void __launch_bounds__(2 * CK<L>::TPB)
 k_xf_div_s(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_yx, int s_rr, int s_p,
            int s_out) {
+  constexpr int H = GLIA_NT_XF;
   static_assert(CKS<L>::OK, "thread groups must be whole waves");
   const MemberDev& M = mem[blockIdx.y];
   const PcgState* st = M.st;
@@ -665,9 +740,9 @@ k_xf_div_s(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__
   const int col = lt & 15, t = lt >> 4;
   float2* exc = ex[grp] + col * K::CP;
   const Tile tl = col_tile(0, d);
-  GPtr In = cptr(M, grp == 0 ? s_yx : s_rr) + tl.base;
-  GPtr P = cptr(M, s_p) + tl.base;
-  GPtr O = cptr(M, s_out) + tl.base;
+  GPtrT<H> In = cptr<H>(M, grp == 0 ? s_yx : s_rr) + tl.base;
+  GPtrT<H> P = cptr<H>(M, s_p) + tl.base;
+  GPtrT<H> O = cptr<H>(M, s_out) + tl.base;
   const float tau = (float)st->tau;
   const float pc = (float)st->pc;
   const float wy = wnum(tl.o, d.n1), wz = (float)(tl.c0 + col);
@@ -732,12 +807,13 @@ k_xf_div_s(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__
 template <int L, int DIR, int S>
 __global__ void __launch_bounds__(CK<L>::TPB, GLIA_COL_MINW)
 k_col_plain(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_in, int s_out, int honor_done) {
+  constexpr int H = GLIA_NT_COL;
   const MemberDev& M = mem[blockIdx.y];
   if (honor_done && M.st->done) return;
   GLIA_COL_PRELUDE(L)
   const Tile tl = col_tile(DIR, d);
-  GPtr I = cptr(M, s_in) + tl.base;
-  GPtr O = cptr(M, s_out) + tl.base;
+  GPtrT<H> I = cptr<H>(M, s_in) + tl.base;
+  GPtrT<H> O = cptr<H>(M, s_out) + tl.base;
   float2 a[K::VPT];
 #pragma unroll
   for (int j = 0; j < K::VIN; j++) a[j] = I[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
@@ -784,9 +860,9 @@ struct ZIt {
   static constexpr int value = ZK<M>::NL * M / ZK<M>::TPB;
   static_assert(value * ZK<M>::TPB == ZK<M>::NL * M, "block must tile its lines");
 };
-template <int M>
-__device__ __forceinline__ void zl_fetch(float2 (&tmp)[ZIt<M>::value], GPtr src, int64_t base) {
-  GPtr sb = src + base;
+template <int M, int H>
+__device__ __forceinline__ void zl_fetch(float2 (&tmp)[ZIt<M>::value], GPtrT<H> src, int64_t base) {
+  GPtrT<H> sb = src + base;
 #pragma unroll
   for (int i = 0; i < ZIt<M>::value; i++) tmp[i] = sb[(int)threadIdx.x + i * ZK<M>::TPB];
 }
@@ -799,8 +875,8 @@ __device__ __forceinline__ void zl_put(float2* lb, const float2 (&tmp)[ZIt<M>::v
     lb[li * ZK<M>::LP + xpad(k)] = tmp[i];
   }
 }
-template <int M>
-__device__ __forceinline__ void zl_load(float2* lb, GPtr src, int64_t base) {
+template <int M, int H>
+__device__ __forceinline__ void zl_load(float2* lb, GPtrT<H> src, int64_t base) {
   float2 tmp[ZIt<M>::value];
   zl_fetch<M>(tmp, src, base);
   zl_put<M>(lb, tmp);
@@ -860,9 +936,9 @@ __device__ __forceinline__ void zl_inverse(float2* v, float2* line, int t, const
 // real output of a Z line kernel (C2R times scale), stored coalesced: the values go back to the
 // line buffer at their line positions and the block writes its lines in order.  With SCREEN the
 // stored value is c - a u, a and u read in the same order (the screened elasticity operator).
-template <int M, bool SCREEN, int NDOT>
-__device__ __forceinline__ void zl_real_store(const float2* v, float2* lb, float2* line, int t, float scale, GPtr out,
-                                              GPtr ap, GPtr up, double* acc) {
+template <int M, bool SCREEN, int NDOT, int H>
+__device__ __forceinline__ void zl_real_store(const float2* v, float2* lb, float2* line, int t, float scale, GPtrT<H> out,
+                                              GPtrT<H> ap, GPtrT<H> up, double* acc) {
   constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT;
   __syncthreads();  // every thread's inverse has read the line buffer
 #pragma unroll
@@ -890,11 +966,11 @@ __device__ __forceinline__ void zl_real_store(const float2* v, float2* lb, float
   }
 }
 
-template <int M>
+template <int M, int H>
 __device__ __forceinline__ void zl_forward_store(float2* v, float2* lb, float2* line, int t, const float2* __restrict__ twm,
-                                                 const float2* __restrict__ tw2, GPtr dst, int64_t base,
+                                                 const float2* __restrict__ tw2, GPtrT<H> dst, int64_t base,
                                                  bool mulz) {
-  GPtr db = dst + base;
+  GPtrT<H> db = dst + base;
   constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT;
   __syncthreads();
   fft2s<R2, R1, NT, -1>(v, line, t, twm);
@@ -931,6 +1007,7 @@ template <int M>
 __global__ void __launch_bounds__(ZK<M>::TPB, ZK<M>::VPT <= 16 ? GLIA_ZD_MINW : GLIA_ZD_MINW24)
 k_z_diff(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ twm, const float2* __restrict__ tw2,
           int s_gx, int s_gy, int s_gz, int k_slot) {
+  constexpr int H = GLIA_NT_ZD;
   const MemberDev& Mb = mem[blockIdx.y];
   if (Mb.st->done) return;
   constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT, VPT = ZK<M>::VPT;
@@ -949,12 +1026,12 @@ k_z_diff(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
   const int li = threadIdx.x / NT, t = threadIdx.x % NT;
   const int64_t base = (int64_t)grp * ZK<M>::NL * M;
   float2* line = lb + li * ZK<M>::LP;
-  GPtr kf = rptr2(Mb, k_slot) + base;
+  GPtrT<H> kf = rptr2<H>(Mb, k_slot) + base;
   const int lo = li * M;
   const float s = (float)d.inv_ng;
   const int slot = comp == 0 ? s_gx : (comp == 1 ? s_gy : s_gz);
   float2 nxt[ZIt<M>::value];
-  zl_fetch<M>(nxt, cptr(Mb, slot), base);
+  zl_fetch<M>(nxt, cptr<H>(Mb, slot), base);
   float2 kv[VPT], v[VPT];
 #pragma unroll
   for (int j = 0; j < VPT; j++) {
@@ -967,7 +1044,7 @@ k_z_diff(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ t
   zl_inverse<M>(v, line, t, twml, tw2l);
 #pragma unroll
   for (int j = 0; j < VPT; j++) v[j] = make_float2(kv[j].x * v[j].x, kv[j].y * v[j].y);
-  zl_forward_store<M>(v, lb, line, t, twml, tw2l, cptr(Mb, slot), base, comp == 2);
+  zl_forward_store<M>(v, lb, line, t, twml, tw2l, cptr<H>(Mb, slot), base, comp == 2);
 }
 
 // Z pass for a real-space pointwise stage between an inverse and a forward transform.
@@ -993,6 +1070,7 @@ template <int M, int MODE>
 __global__ void __launch_bounds__(ZK<M>::TPB, GLIA_Z_MINW)
 k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ twm, const float2* __restrict__ tw2,
           int s_in, int s_out, int r_slot, int ch_idx, float dt, int honor_done) {
+  constexpr int H = GLIA_NT_ZP;
   const MemberDev& Mb = mem[blockIdx.y];
   if (honor_done && Mb.st->done) return;
   constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT, VPT = ZK<M>::VPT;
@@ -1008,20 +1086,20 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   const int lo = li * M;
   float2* line = lb + li * ZK<M>::LP;
   // real field: a half-step history entry, a real slot, or none (block-base pointers, uniform)
-  GPtr rf = gptr(ch_idx >= 0 ? (const void*)Mb.chalf[ch_idx] : (r_slot >= 0 ? (const void*)Mb.r[r_slot] : nullptr));
+  GPtrT<H> rf = gptr<H>(ch_idx >= 0 ? (const void*)Mb.chalf[ch_idx] : (r_slot >= 0 ? (const void*)Mb.r[r_slot] : nullptr));
   if (rf) rf = rf + base;
   const float s = (float)d.inv_ng;
   double acc[2] = {0.0, 0.0};
   float2 v[VPT];
-  GPtr op1 = MODE == ZR_COEF ? rf : rptr2(Mb, (MODE == ZR_MISFIT || MODE == ZR_OBS2) ? RS_OBS : RS_RHO) + base;
-  GPtr op2 = MODE == ZR_MISFIT ? rptr2(Mb, RS_D1) + base
-                               : ((MODE == ZR_SCREEN || MODE == ZR_C2R_DOT) ? rptr2(Mb, s_out) + base : rf);
-  GPtr ctp = rptr2(Mb, RS_CT) + base;
+  GPtrT<H> op1 = MODE == ZR_COEF ? rf : rptr2<H>(Mb, (MODE == ZR_MISFIT || MODE == ZR_OBS2) ? RS_OBS : RS_RHO) + base;
+  GPtrT<H> op2 = MODE == ZR_MISFIT ? rptr2<H>(Mb, RS_D1) + base
+                               : ((MODE == ZR_SCREEN || MODE == ZR_C2R_DOT) ? rptr2<H>(Mb, s_out) + base : rf);
+  GPtrT<H> ctp = rptr2<H>(Mb, RS_CT) + base;
   float2 o1[PF ? VPT : 1], o2[PF && TWO ? VPT : 1];
   if (MODE == ZR_R2C) {
     zl_load<M>(lb, rf, 0);  // the real lines, coalesced, into the line buffer
   } else {
-    zl_load<M>(lb, cptr(Mb, s_in), base);
+    zl_load<M>(lb, cptr<H>(Mb, s_in), base);
     if (PF && !REAL_OUT) {
 #pragma unroll
       for (int j = 0; j < VPT; j++) {
@@ -1081,7 +1159,7 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
       }
     }
   }
-  if (!REAL_OUT) zl_forward_store<M>(v, lb, line, t, twml, tw2l, cptr(Mb, s_out), base, false);
+  if (!REAL_OUT) zl_forward_store<M>(v, lb, line, t, twml, tw2l, cptr<H>(Mb, s_out), base, false);
   if (MODE == ZR_MISFIT) {
     double a1[1] = {acc[0]};
     block_reduce_store<1>(a1, Mb.part + blockIdx.x);
@@ -1173,10 +1251,11 @@ k_pcg_update(const MemberDev* __restrict__ mem, Dims d, int s_x) {
   if (st->done) return;
   // a lane handles a pair of adjacent entries along z (m2 is even, so both share kx and ky) with
   // 16-byte accesses, and two pairs per loop trip so that eight loads are in flight per lane
+  constexpr int H = GLIA_NT_PU;
   GLIA_GLOBAL v4f* X = (GLIA_GLOBAL v4f*)craw(M, s_x);
   GLIA_GLOBAL v4f* R = (GLIA_GLOBAL v4f*)M.c[CS_R];
-  const GLIA_GLOBAL v4f* P = (const GLIA_GLOBAL v4f*)M.c[CS_P];
-  const GLIA_GLOBAL v4f* W = (const GLIA_GLOBAL v4f*)M.c[CS_W];
+  GLIA_GLOBAL v4f* P = (GLIA_GLOBAL v4f*)M.c[CS_P];
+  GLIA_GLOBAL v4f* W = (GLIA_GLOBAL v4f*)M.c[CS_W];
   const float a = (float)st->alpha;
   const float pc = (float)st->pc;
   const unsigned npairs = (unsigned)(d.nspec >> 1), hm = (unsigned)(d.m2 >> 1), n1 = (unsigned)d.n1;
@@ -1184,8 +1263,8 @@ k_pcg_update(const MemberDev* __restrict__ mem, Dims d, int s_x) {
   auto finish = [&](unsigned i, v4f x, v4f r, v4f p, v4f w) {
     x += a * p;
     r -= a * w;
-    X[i] = x;
-    R[i] = r;
+    st4<H>(X + i, x);
+    st4<H>(R + i, r);
     const unsigned line = i / hm, kz = 2u * (i - line * hm);
     const unsigned kx = line / n1, ky = line - kx * n1;
     const float wx = wnum((int)kx, d.n0), wy = wnum((int)ky, d.n1);
@@ -1199,12 +1278,12 @@ k_pcg_update(const MemberDev* __restrict__ mem, Dims d, int s_x) {
   unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
   for (; i + stride < npairs; i += 2 * stride) {
     const unsigned j = i + stride;
-    const v4f p0 = P[i], w0 = W[i], x0 = X[i], r0 = R[i];
-    const v4f p1 = P[j], w1 = W[j], x1 = X[j], r1 = R[j];
+    const v4f p0 = ld4<H>(P + i), w0 = ld4<H>(W + i), x0 = ld4<H>(X + i), r0 = ld4<H>(R + i);
+    const v4f p1 = ld4<H>(P + j), w1 = ld4<H>(W + j), x1 = ld4<H>(X + j), r1 = ld4<H>(R + j);
     finish(i, x0, r0, p0, w0);
     finish(j, x1, r1, p1, w1);
   }
-  if (i < npairs) finish(i, X[i], R[i], P[i], W[i]);
+  if (i < npairs) finish(i, ld4<H>(X + i), ld4<H>(R + i), ld4<H>(P + i), ld4<H>(W + i));
   block_reduce_store<2>(acc, M.part + (int64_t)blockIdx.x * 2);
 }
 
