@@ -80,10 +80,10 @@ def make_members(n, count, nbases, nnz, seed0):
     return out
 
 
-def build_solver(members, n, nt, dt, checkpoint_slots=0):
+def build_solver(members, n, nt, dt, checkpoint_slots=0, adjoint_store=False):
     from glia_amd import rd
     s = rd.RDSolver(n, nmembers=len(members), dt=dt, nt=nt, k_gm_wm=0.2, r_gm_wm=0.2,
-                    checkpoint_slots=checkpoint_slots)
+                    checkpoint_slots=checkpoint_slots, adjoint_store=adjoint_store)
     for m, pr in enumerate(members):
         s.set_tissue(m, pr["wm"], pr["gm"], pr["csf"], pr["vt"])
         s.set_coefficients(m, pr["kappa"], pr["rho"])
@@ -766,6 +766,8 @@ def main():
     ap.add_argument("--fd-h", type=float, default=1e-4, help="finite-difference step of --config diff")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-store-leg", action="store_true",
+                    help="skip the evaluations with the forward's half-step states kept for the adjoint")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--nt-oracle", type=int, default=2,
                     help="time steps of the oracle evaluation that gives rel_err_fp64 and cpu_baseline")
@@ -893,6 +895,32 @@ def main():
         plain = M / tsum
         log("plain (one member at a time): %.3f grad evals/s" % plain)
 
+    store = None
+    if not args.no_store_leg:
+        # the same evaluations with the forward's half-step states kept for the adjoint instead of
+        # recomputed by a CN solve per step (adjoint_store_, set in the reference's shipped configs
+        # and scripts; the library default recomputes); the results are the same bit for bit
+        ss = build_solver(members, n, nt, dt, args.checkpoint_slots, adjoint_store=True)
+        for m in range(M):
+            ss.set_data(m, members[m]["d1"], None, -0.99)
+        for _ in range(max(args.warmup, 1)):
+            Js, gs = ss.objective_and_gradient(xs)
+        ss.sync()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        for _ in range(args.steps):
+            Js, gs = ss.objective_and_gradient(xs)
+        ss.sync()
+        torch.cuda.synchronize()
+        t4 = time.perf_counter()
+        same = bool(np.array_equal(np.asarray(Js), np.asarray(J)) and
+                    all(np.array_equal(np.asarray(a), np.asarray(b)) for a, b in zip(gs, g)))
+        store = {"value": round(M * args.steps / (t4 - t3), 4), "ms_per_step": round((t4 - t3) * 1000.0 / args.steps, 2),
+                 "same_result_bitwise": same}
+        ss.close()
+        del ss
+        log("adjoint_store leg: %s" % store)
+
     ref = None
     if not args.no_plain:
         pr0 = members[0]
@@ -951,6 +979,7 @@ def main():
         "batched_vs_plain": None if plain is None else round(value / plain, 3),
         "refstyle_plain": ref,
         "batched_vs_refstyle_plain": None if ref is None else round(value / ref["value"], 2),
+        "adjoint_store": store,
         "cg_iterations_per_solve": round(mbar, 3),
         "model_hbm_GBs": round(model_gbs, 1),
         "model_hbm_frac": round(model_gbs / HBM_PEAK_GBS, 4),
