@@ -127,11 +127,10 @@ int strain_channel(glia_el_s* h, int nl, const int* nterms, const int (*lane)[4]
   return 0;
 }
 
-// f = A u on the lane vectors (vector ids; f must differ from u)
-int apply_op(glia_el_s* h, int u, int f) {
+// f = A u on the lane vectors from U = FFT(u) in CU (vector ids; f must differ from u)
+int apply_op_spectral(glia_el_s* h, int u, int f) {
   int rc;
   Lanes use(h->e.get(), 3);
-  EL_ENG(h, h->e->fft_r2c(kVecSlot[u], CU));
   // d_j u_i + d_i u_j, diagonal terms (ii) in lane i, then xy, xz, yz in lanes 0, 1, 2
   {
     const int n[3] = {2, 2, 2};
@@ -169,12 +168,39 @@ int apply_op(glia_el_s* h, int u, int f) {
   return 0;
 }
 
+// f = A u on the lane vectors (vector ids; f must differ from u)
+int apply_op(glia_el_s* h, int u, int f) {
+  {
+    Lanes use(h->e.get(), 3);
+    EL_ENG(h, h->e->fft_r2c(kVecSlot[u], CU));
+  }
+  return apply_op_spectral(h, u, f);
+}
+
+// the CG's p = z + b p folded into the first pass of w = A p
+int apply_op_new_p(glia_el_s* h, double b) {
+  {
+    Lanes use(h->e.get(), 3);
+    EL_ENG(h, h->e->fft_r2c_axpby(kVecSlot[VP], kVecSlot[VZ], b, CU));
+  }
+  return apply_op_spectral(h, VP, VW);
+}
+
 // u = B f on the lane vectors; the inverse's Z pass leaves the block partials of <u, u> and <u, f>
 int apply_prec(glia_el_s* h, int f, int u) {
   Lanes use(h->e.get(), 3);
   EL_ENG(h, h->e->fft_r2c(kVecSlot[f], CU));
   EL_ENG(h, h->e->el_prec_lanes(CU, CS_PC, h->avg[0], h->avg[1], h->avg[2]));
   EL_ENG(h, h->e->c2r_dots(CS_PC, kVecSlot[u], kVecSlot[f]));
+  return 0;
+}
+
+// the CG's x += t p, r -= t w (t from the device scalar dpi) folded into the first pass of z = B r
+int apply_prec_updated_r(glia_el_s* h, const double* dpi, double beta, double dpiold, bool first) {
+  Lanes use(h->e.get(), 3);
+  EL_ENG(h, h->e->fft_r2c_cgupd(kVecSlot[VX], kVecSlot[VR], kVecSlot[VP], kVecSlot[VW], dpi, beta, dpiold, first, CU));
+  EL_ENG(h, h->e->el_prec_lanes(CU, CS_PC, h->avg[0], h->avg[1], h->avg[2]));
+  EL_ENG(h, h->e->c2r_dots(CS_PC, kVecSlot[VZ], kVecSlot[VR]));
   return 0;
 }
 
@@ -402,7 +428,8 @@ int glia_el_solve(glia_el_t h, const float* const f[3], float* const u[3], int m
   // One host synchronization per iteration: <p, A p> (from the operator's last Z pass) stays on the
   // device, where the update takes its step from it (and skips itself when the host is going to stop
   // on it); the host reads it with <z, z> and <z, r> (from the preconditioner's inverse pass) at the
-  // end of the iteration.
+  // end of the iteration.  The vector updates run inside the first pass of the transform that reads
+  // the vector next: p = z + b p in the operator's, x and r in the preconditioner's.
   if ((rc = apply_op(h, VX, VR))) return rc;
   EL_HIP(h, glia::el::launch_axpby3(h->st, h->ng, vec(h, VR), cvec(h, VB), 0.0, 1));  // r = b - A x
   double rnorm0, dp, beta = 0, betaold = 0, dpi = 0, dpiold;
@@ -440,15 +467,15 @@ int glia_el_solve(glia_el_t h, const float* const f[3], float* const u[3], int m
         why = KSP_DIVERGED_INDEFINITE_PC;
         break;
       }
-      EL_HIP(h, glia::el::launch_axpby3(h->st, h->ng, vec(h, VP), cvec(h, VZ), i ? beta / betaold : 0.0, 0));
       dpiold = dpi;
-      // <p, A p> from the operator's last Z pass
-      if ((rc = apply_op(h, VP, VW)) || (rc = lane_sums(h, 1, dpi_dev))) return rc;
-      EL_HIP(h, glia::el::launch_cg_update_dev(h->st, h->ng, vec(h, VX), vec(h, VR), cvec(h, VP), cvec(h, VW), dpi_dev,
-                                               beta, dpiold, i == 0 ? 1 : 0));
+      // p = z + (beta / betaold) p, w = A p, and <p, A p> from the operator's last Z pass
+      if ((rc = apply_op_new_p(h, i ? beta / betaold : 0.0)) || (rc = lane_sums(h, 1, dpi_dev))) return rc;
       betaold = beta;
-      // <z, z> for the test and <z, r> for the next beta from the preconditioner's inverse pass
-      if ((rc = apply_prec(h, VR, VZ)) || (rc = lane_sums(h, 2, dpi_dev + 1)) || (rc = read3())) return rc;
+      // x, r updated with the step beta / <p, A p>, then <z, z> for the test and <z, r> for the next
+      // beta from the preconditioner's inverse pass
+      if ((rc = apply_prec_updated_r(h, dpi_dev, beta, dpiold, i == 0)) || (rc = lane_sums(h, 2, dpi_dev + 1)) ||
+          (rc = read3()))
+        return rc;
       dpi = sc[0];
       if (std::isnan(dpi) || std::isinf(dpi)) {
         why = KSP_DIVERGED_NANORINF;

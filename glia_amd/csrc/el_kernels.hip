@@ -50,23 +50,6 @@ __global__ void __launch_bounds__(256) k_el_material(int64_t ng, const float* __
   if (threadIdx.x < 3) part[blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// the CG update with the step a = beta / dpi taken on the device, dpi = <p, A p> from *dpi; the
-// update is skipped when the host will stop on dpi (NaN or infinite, zero, or a sign change from
-// dpiold after the first iteration), so x and r stay as the reference leaves them
-__global__ void __launch_bounds__(256) k_cg_update_dev(int64_t ng, Vec3 x, Vec3 r, Vec3C p, Vec3C w,
-                                                       const double* __restrict__ dpi, double beta, double dpiold,
-                                                       int first) {
-  const double q = *dpi;
-  if (isnan(q) || isinf(q) || q == 0.0 || (!first && q * dpiold <= 0.0)) return;
-  const float fa = (float)(beta / q);
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int k = 0; k < 3; k++)
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += stride) {
-      x.p[k][i] += fa * p.p[k][i];
-      r.p[k][i] -= fa * w.p[k][i];
-    }
-}
-
 // p = z + b p (b = 0: p = z); y = x - y when sub (the initial residual b - A x)
 __global__ void __launch_bounds__(256) k_axpby3(int64_t ng, Vec3 p, Vec3C z, double b, int sub) {
   const float fb = (float)b;
@@ -95,11 +78,6 @@ int launch_material(hipStream_t st, int64_t ng, const float* c, const float* wm,
 }
 int launch_force_scale(hipStream_t st, int64_t ng, const float* g, const float* c, float ff, int use_tanh, float* out) {
   k_force_scale<<<EB, 256, 0, st>>>(ng, g, c, ff, use_tanh, out);
-  return (int)hipGetLastError();
-}
-int launch_cg_update_dev(hipStream_t st, int64_t ng, const Vec3& x, const Vec3& r, const Vec3C& p, const Vec3C& w,
-                         const double* dpi, double beta, double dpiold, int first) {
-  k_cg_update_dev<<<EB, 256, 0, st>>>(ng, x, r, p, w, dpi, beta, dpiold, first);
   return (int)hipGetLastError();
 }
 int launch_axpby3(hipStream_t st, int64_t ng, const Vec3& p, const Vec3C& z, double b, int sub) {

@@ -24,9 +24,6 @@ int launch_material(hipStream_t st, int64_t ng, const float* c, const float* wm,
                     double* part);
 int launch_force_scale(hipStream_t st, int64_t ng, const float* g, const float* c, float ff, int use_tanh, float* out);
 int launch_axpby3(hipStream_t st, int64_t ng, const Vec3& p, const Vec3C& z, double b, int sub);
-// x += a p, r -= a w with a = beta / *dpi, skipped when *dpi would stop the iteration
-int launch_cg_update_dev(hipStream_t st, int64_t ng, const Vec3& x, const Vec3& r, const Vec3C& p, const Vec3C& w,
-                         const double* dpi, double beta, double dpiold, int first);
 
 }  // namespace el
 }  // namespace glia

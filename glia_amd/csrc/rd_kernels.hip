@@ -1166,6 +1166,85 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
   }
 }
 
+// Z pass of a forward transform whose real input is first updated in place: the elasticity CG's
+// vector work folded into the transform that reads the vector next.
+//   ZU_AXPBY: p = z + b p (p = z when b = 0), out = R2C(p)            [ra = p, rb = z]
+//   ZU_CGUPD: x += t p, r -= t w with t = beta / q, q = *dpi (<p, A p>); out = R2C(r)
+//             [ra = x, rb = r, rc = p, rd = w].  No update when q is 0, not finite, or of the sign
+//             opposite to dpiold after the first iteration (the host stops there).
+enum { ZU_AXPBY = 0, ZU_CGUPD = 1 };
+
+template <int M, int MODE>
+__global__ void __launch_bounds__(ZK<M>::TPB, GLIA_Z_MINW)
+k_z_upd_r2c(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ twm, const float2* __restrict__ tw2,
+            int ra, int rb, int rc, int rd, const double* __restrict__ dpi, double beta, double dpiold, int first,
+            int s_out) {
+  constexpr int H = GLIA_NT_ZP;
+  const MemberDev& Mb = mem[blockIdx.y];
+  constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT, VPT = ZK<M>::VPT, TPB = ZK<M>::TPB;
+  constexpr int IT = ZIt<M>::value;
+  __shared__ float2 lb[ZK<M>::NL * ZK<M>::LP];
+  __shared__ float2 twml[M], tw2l[M];
+  const int li = threadIdx.x / NT, t = threadIdx.x % NT;
+  const int64_t base = (int64_t)blockIdx.x * ZK<M>::NL * M;
+  float2* line = lb + li * ZK<M>::LP;
+  float cf = 0.0f;
+  bool upd = true;
+  if (MODE == ZU_AXPBY) {
+    cf = (float)beta;
+  } else {
+    const double q = *dpi;
+    upd = !(isnan(q) || isinf(q) || q == 0.0 || (!first && q * dpiold <= 0.0));
+    if (upd) cf = (float)(beta / q);
+  }
+  GPtrT<H> A = rptr2<H>(Mb, ra) + base, B = rptr2<H>(Mb, rb) + base;
+  float2 tmp[IT];
+  if (MODE == ZU_AXPBY) {
+    float2 za[IT], pa[IT];
+#pragma unroll
+    for (int i = 0; i < IT; i++) {
+      za[i] = B[(int)threadIdx.x + i * TPB];
+      if (beta != 0.0) pa[i] = A[(int)threadIdx.x + i * TPB];
+    }
+#pragma unroll
+    for (int i = 0; i < IT; i++) {
+      float2 v = za[i];
+      if (beta != 0.0) v = make_float2(za[i].x + cf * pa[i].x, za[i].y + cf * pa[i].y);
+      A[(int)threadIdx.x + i * TPB] = v;
+      tmp[i] = v;
+    }
+  } else {
+    GPtrT<H> C = rptr2<H>(Mb, rc) + base, D = rptr2<H>(Mb, rd) + base;
+    float2 xa[IT], pa[IT], wa[IT];
+#pragma unroll
+    for (int i = 0; i < IT; i++) {
+      const int e = (int)threadIdx.x + i * TPB;
+      tmp[i] = B[e];
+      if (upd) {
+        xa[i] = A[e];
+        pa[i] = C[e];
+        wa[i] = D[e];
+      }
+    }
+    if (upd) {
+#pragma unroll
+      for (int i = 0; i < IT; i++) {
+        const int e = (int)threadIdx.x + i * TPB;
+        A[e] = make_float2(xa[i].x + cf * pa[i].x, xa[i].y + cf * pa[i].y);
+        tmp[i] = make_float2(tmp[i].x - cf * wa[i].x, tmp[i].y - cf * wa[i].y);
+        B[e] = tmp[i];
+      }
+    }
+  }
+  stage_line_twiddles<M>(twml, tw2l, twm, tw2);
+  zl_put<M>(lb, tmp);
+  __syncthreads();
+  float2 v[VPT];
+#pragma unroll
+  for (int j = 0; j < VPT; j++) v[j] = line[xpad(out_row<R1, R2, NT>(t, j))];
+  zl_forward_store<M>(v, lb, line, t, twml, tw2l, cptr<H>(Mb, s_out), base, false);
+}
+
 // Z pass of the gradient quadrature at one time point: from the y-transformed fields of c
 // (G0, Gx, Gy, Gz in slots s_c) and of alpha (s_a):
 //   q_k   = sum (wm + r_k gm) (grad c . grad alpha)
@@ -2133,6 +2212,28 @@ struct RunZPoint {
 };
 int launch_z_point(const LaunchCtx& c, int mode, int s_in, int s_out, int r_slot, int ch_idx, float dt, bool honor_done) {
   return dispatch_M<RunZPoint>(c.d.m2, c, mode, s_in, s_out, r_slot, ch_idx, dt, honor_done ? 1 : 0);
+}
+template <int M>
+struct RunZUpd {
+  static int run(const LaunchCtx& c, int mode, int ra, int rb, int rc, int rd, const double* dpi, double beta,
+                 double dpiold, int first, int s_out) {
+    dim3 g((unsigned)((int64_t)c.d.n0 * c.d.n1 / ZK<M>::NL), c.nm);
+    if (mode == ZU_AXPBY)
+      k_z_upd_r2c<M, ZU_AXPBY><<<g, ZK<M>::TPB, 0, c.stream>>>(c.mem, c.d, c.twm, c.tw2, ra, rb, rc, rd, dpi, beta,
+                                                                 dpiold, first, s_out);
+    else if (mode == ZU_CGUPD)
+      k_z_upd_r2c<M, ZU_CGUPD><<<g, ZK<M>::TPB, 0, c.stream>>>(c.mem, c.d, c.twm, c.tw2, ra, rb, rc, rd, dpi, beta,
+                                                                 dpiold, first, s_out);
+    else
+      return -1;
+    GLIA_CHECK_LAUNCH();
+    return 0;
+  }
+};
+int launch_z_upd_r2c(const LaunchCtx& c, int mode, int ra, int rb, int rc, int rd, const double* dpi, double beta,
+                     double dpiold, int first, int s_out) {
+  if (mode == ZU_CGUPD && !dpi) return -1;
+  return dispatch_M<RunZUpd>(c.d.m2, c, mode, ra, rb, rc, rd, dpi, beta, dpiold, first, s_out);
 }
 int z_blocks(const Dims& d) {
   switch (d.m2) {

@@ -42,6 +42,12 @@ int launch_xf_div(const LaunchCtx& c, int mode, int s_yx, int s_rr, int s_p, int
 int launch_col_plain(const LaunchCtx& c, int dir, int sign, int s_in, int s_out, bool honor_done);
 int launch_z_diff(const LaunchCtx& c, int s_gx, int s_gy, int s_gz, int k_slot);
 int launch_z_point(const LaunchCtx& c, int mode, int s_in, int s_out, int r_slot, int ch_idx, float dt, bool honor_done);
+// Z pass of a forward transform with the elasticity CG's vector update folded in (k_z_upd_r2c):
+// ZUM_AXPBY: p = z + beta p, out = R2C(p) [ra p, rb z]; ZUM_CGUPD: x += t p, r -= t w with
+// t = beta / *dpi, out = R2C(r) [ra x, rb r, rc p, rd w]
+enum { ZUM_AXPBY = 0, ZUM_CGUPD = 1 };
+int launch_z_upd_r2c(const LaunchCtx& c, int mode, int ra, int rb, int rc, int rd, const double* dpi, double beta,
+                     double dpiold, int first, int s_out);
 int launch_z_gradq(const LaunchCtx& c, int4 sc, int4 sa);
 
 int launch_pcg_begin(const LaunchCtx& c, double tau);

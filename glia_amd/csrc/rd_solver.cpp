@@ -713,6 +713,24 @@ int RDEnsemble::lane_part_sum(int nq, double* out) {
   return 0;
 }
 
+int RDEnsemble::fft_r2c_axpby(int p_rs, int z_rs, double b, int cs_out) {
+  LaunchCtx c = ctx();
+  GLIA_TRYC(launch_z_upd_r2c(c, ZUM_AXPBY, p_rs, z_rs, -1, -1, nullptr, b, 0.0, 0, cs_out));
+  GLIA_TRYC(launch_col_plain(c, 1, -1, cs_out, cs_out, false));
+  GLIA_TRYC(launch_col_plain(c, 0, -1, cs_out, cs_out, false));
+  return 0;
+}
+
+int RDEnsemble::fft_r2c_cgupd(int x_rs, int r_rs, int p_rs, int w_rs, const double* dpi, double beta, double dpiold,
+                              bool first, int cs_out) {
+  if (!dpi) return fail(-2, "fft_r2c_cgupd needs the device scalar <p, A p>");
+  LaunchCtx c = ctx();
+  GLIA_TRYC(launch_z_upd_r2c(c, ZUM_CGUPD, x_rs, r_rs, p_rs, w_rs, dpi, beta, dpiold, first ? 1 : 0, cs_out));
+  GLIA_TRYC(launch_col_plain(c, 1, -1, cs_out, cs_out, false));
+  GLIA_TRYC(launch_col_plain(c, 0, -1, cs_out, cs_out, false));
+  return 0;
+}
+
 int RDEnsemble::el_prec(const int f[3], const int u[3], double mu, double lam, double scr) {
   GLIA_TRYC(launch_el_prec(ctx(), make_int4(f[0], f[1], f[2], 0), make_int4(u[0], u[1], u[2], 0), mu, lam, scr));
   return 0;

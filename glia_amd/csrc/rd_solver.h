@@ -110,6 +110,12 @@ class RDEnsemble {
   // out[q] (device), q < nq: the last Z pass's block partials of the lanes in use, summed in a fixed
   // order (lin_screen_transform leaves <u, out>, c2r_dots <out, out> and <out, r>)
   int lane_part_sum(int nq, double* out);
+  // packed spectrum cs_out of the real slot p_rs after p = z + b p in place (p = z when b = 0)
+  int fft_r2c_axpby(int p_rs, int z_rs, double b, int cs_out);
+  // packed spectrum cs_out of the real slot r_rs after x += t p, r -= t w in place, t = beta / *dpi
+  // (skipped when *dpi is 0, not finite, or of the sign opposite to dpiold after the first iteration)
+  int fft_r2c_cgupd(int x_rs, int r_rs, int p_rs, int w_rs, const double* dpi, double beta, double dpiold, bool first,
+                    int cs_out);
   int get_spectrum(int m, int cslot, float2* host);
   int set_spectrum(int m, int cslot, const float2* host);
   int solve_state(int linearized, bool keep_history);
