@@ -71,6 +71,9 @@ struct glia_el_s {
   float* v[kVecs][3] = {};   // lane d's real slot of each CG vector (the engine's memory)
   double* part = nullptr;    // block partials, then (at part + 3 np) the summed scalars
   std::vector<double> hpart;
+  int* hflag_host = nullptr;  // mapped: set by the device when the CG stops
+  int* hflag_dev = nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};  // ends of the last enqueued iterations
   double avg[3] = {0, 0, 0};
   bool have_material = false;
   std::string err;
@@ -78,6 +81,9 @@ struct glia_el_s {
     for (float* p : {mu, lam, scr})
       if (p) (void)hipFree(p);
     if (part) (void)hipFree(part);
+    if (hflag_host) (void)hipHostFree(hflag_host);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
   }
 };
 
@@ -177,11 +183,11 @@ int apply_op(glia_el_s* h, int u, int f) {
   return apply_op_spectral(h, u, f);
 }
 
-// the CG's p = z + b p folded into the first pass of w = A p
-int apply_op_new_p(glia_el_s* h, double b) {
+// the CG's p = z + (beta / betaold) p folded into the first pass of w = A p
+int apply_op_new_p(glia_el_s* h) {
   {
     Lanes use(h->e.get(), 3);
-    EL_ENG(h, h->e->fft_r2c_axpby(kVecSlot[VP], kVecSlot[VZ], b, CU));
+    EL_ENG(h, h->e->fft_r2c_axpby(kVecSlot[VP], kVecSlot[VZ], CU));
   }
   return apply_op_spectral(h, VP, VW);
 }
@@ -195,10 +201,10 @@ int apply_prec(glia_el_s* h, int f, int u) {
   return 0;
 }
 
-// the CG's x += t p, r -= t w (t from the device scalar dpi) folded into the first pass of z = B r
-int apply_prec_updated_r(glia_el_s* h, const double* dpi, double beta, double dpiold, bool first) {
+// the CG's x += t p, r -= t w (t = beta / sc[0] on the device) folded into the first pass of z = B r
+int apply_prec_updated_r(glia_el_s* h, const double* sc) {
   Lanes use(h->e.get(), 3);
-  EL_ENG(h, h->e->fft_r2c_cgupd(kVecSlot[VX], kVecSlot[VR], kVecSlot[VP], kVecSlot[VW], dpi, beta, dpiold, first, CU));
+  EL_ENG(h, h->e->fft_r2c_cgupd(kVecSlot[VX], kVecSlot[VR], kVecSlot[VP], kVecSlot[VW], sc, CU));
   EL_ENG(h, h->e->el_prec_lanes(CU, CS_PC, h->avg[0], h->avg[1], h->avg[2]));
   EL_ENG(h, h->e->c2r_dots(CS_PC, kVecSlot[VZ], kVecSlot[VR]));
   return 0;
@@ -285,6 +291,16 @@ int glia_el_create(const int n[3], void* stream, glia_el_t* out) {
     return (int)hipErrorOutOfMemory;
   }
   h->hpart.assign(3 * np, 0.0);
+  if (hipHostMalloc((void**)&h->hflag_host, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&h->hflag_dev, h->hflag_host, 0) != hipSuccess) {
+    delete h;
+    return (int)hipErrorOutOfMemory;
+  }
+  for (hipEvent_t& e : h->ev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      delete h;
+      return (int)hipErrorOutOfMemory;
+    }
   *out = h;
   return 0;
 }
@@ -425,14 +441,16 @@ int glia_el_solve(glia_el_t h, const float* const f[3], float* const u[3], int m
   int rc;
   if ((rc = stage_in(h, f, mem_kind, VB)) || (rc = stage_in(h, u, mem_kind, VX))) return rc;
   // KSPSolve_CG (PETSc, KSP_NORM_PRECONDITIONED, nonzero initial guess) with KSPConvergedDefault.
-  // One host synchronization per iteration: <p, A p> (from the operator's last Z pass) stays on the
-  // device, where the update takes its step from it (and skips itself when the host is going to stop
-  // on it); the host reads it with <z, z> and <z, r> (from the preconditioner's inverse pass) at the
-  // end of the iteration.  The vector updates run inside the first pass of the transform that reads
-  // the vector next: p = z + b p in the operator's, x and r in the preconditioner's.
+  // The set-up reads ||B b||, ||B r|| and <B r, r> on the host; the iterations then run without the
+  // host: the scalars stay on the device, where k_el_cg_fin (lane 0's PcgState) makes the stopping
+  // tests of KSPSolve_CG and the update takes its step; the host enqueues iterations two ahead of the
+  // device and stops enqueueing when the device raises the mapped flag.  Iterations enqueued past the
+  // stop return at once in every launch.  The vector updates run inside the first pass of the
+  // transform that reads the vector next: p = z + beta p in the operator's, x and r in the
+  // preconditioner's; <p, A p>, <z, z> and <z, r> come from the Z passes that write w and z.
   if ((rc = apply_op(h, VX, VR))) return rc;
   EL_HIP(h, glia::el::launch_axpby3(h->st, h->ng, vec(h, VR), cvec(h, VB), 0.0, 1));  // r = b - A x
-  double rnorm0, dp, beta = 0, betaold = 0, dpi = 0, dpiold;
+  double rnorm0, dp;
   const int np = glia::el::partial_count();
   double* sc_dev = h->part + 3 * np;  // device scalars: <p, A p>, then <z, z>, <z, r>
   double sc[3] = {0, 0, 0};
@@ -448,53 +466,47 @@ int glia_el_solve(glia_el_t h, const float* const f[3], float* const u[3], int m
   dp = std::sqrt(sc[1]);
   if (rnorm0 == 0) rnorm0 = dp;
   const double ttol = std::fmax(rtol * rnorm0, atol);
-  auto converged = [&](double rn) {
-    if (std::isnan(rn) || std::isinf(rn)) return (int)KSP_DIVERGED_NANORINF;
-    if (rn <= ttol) return (int)(rn < atol ? KSP_CONVERGED_ATOL : KSP_CONVERGED_RTOL);
-    if (rn >= dtol * rnorm0) return (int)KSP_DIVERGED_DTOL;
-    return 0;
-  };
-  double* dpi_dev = sc_dev;
-  int its = 0, why = converged(dp), i = 0;
+  int its = 0, why = 0;
+  if (std::isnan(dp) || std::isinf(dp))
+    why = KSP_DIVERGED_NANORINF;
+  else if (dp <= ttol)
+    why = dp < atol ? KSP_CONVERGED_ATOL : KSP_CONVERGED_RTOL;
+  else if (dp >= dtol * rnorm0)
+    why = KSP_DIVERGED_DTOL;
   if (!why) {
-    beta = sc[2];
-    do {
-      its = i + 1;
-      if (beta == 0.0) {
-        why = KSP_CONVERGED_ATOL;
+    *h->hflag_host = 0;
+    {
+      Lanes use(h->e.get(), 3);
+      EL_ENG(h, h->e->el_cg_begin(sc[2], dp, rnorm0, ttol, atol, dtol, maxit));
+    }
+    h->e->set_honor_lanes(true);
+    constexpr int kAhead = 2;
+    for (int k = 0; k < maxit && !rc; k++) {
+      if ((rc = apply_op_new_p(h)) || (rc = lane_sums(h, 1, sc_dev)) || (rc = apply_prec_updated_r(h, sc_dev)) ||
+          (rc = lane_sums(h, 2, sc_dev + 1)))
         break;
-      } else if (i > 0 && beta * betaold < 0.0) {
-        why = KSP_DIVERGED_INDEFINITE_PC;
+      {
+        Lanes use(h->e.get(), 3);
+        if ((rc = h->e->el_cg_fin(sc_dev, h->hflag_dev))) {
+          fail(h, rc, h->e->error());
+          break;
+        }
+      }
+      hipError_t e = hipEventRecord(h->ev[k % 3], h->st);
+      if (e == hipSuccess && k >= kAhead) e = hipEventSynchronize(h->ev[(k - kAhead) % 3]);
+      if (e != hipSuccess) {
+        rc = hip_fail(h, (int)e, "enqueueing the CG iterations");
         break;
       }
-      dpiold = dpi;
-      // p = z + (beta / betaold) p, w = A p, and <p, A p> from the operator's last Z pass
-      if ((rc = apply_op_new_p(h, i ? beta / betaold : 0.0)) || (rc = lane_sums(h, 1, dpi_dev))) return rc;
-      betaold = beta;
-      // x, r updated with the step beta / <p, A p>, then <z, z> for the test and <z, r> for the next
-      // beta from the preconditioner's inverse pass
-      if ((rc = apply_prec_updated_r(h, dpi_dev, beta, dpiold, i == 0)) || (rc = lane_sums(h, 2, dpi_dev + 1)) ||
-          (rc = read3()))
-        return rc;
-      dpi = sc[0];
-      if (std::isnan(dpi) || std::isinf(dpi)) {
-        why = KSP_DIVERGED_NANORINF;
-        break;
-      }
-      if (dpi == 0.0 || (i > 0 && dpi * dpiold <= 0.0)) {
-        why = KSP_DIVERGED_INDEFINITE_MAT;
-        break;
-      }
-      dp = std::sqrt(sc[1]);
-      if ((why = converged(dp))) break;
-      beta = sc[2];
-      if (std::isnan(beta) || std::isinf(beta)) {
-        why = KSP_DIVERGED_NANORINF;
-        break;
-      }
-      i++;
-    } while (i < maxit);
-    if (i >= maxit) why = KSP_DIVERGED_ITS;
+      if (*(volatile int*)h->hflag_host) break;
+    }
+    {
+      Lanes use(h->e.get(), 3);
+      const int rc2 = h->e->el_cg_state(&its, &why, &dp);
+      const int rc3 = h->e->el_cg_end();
+      if (!rc && (rc2 || rc3)) rc = fail(h, rc2 ? rc2 : rc3, h->e->error());
+    }
+    if (rc) return rc;
   }
   if ((rc = stage_out(h, u, mem_kind, VX))) return rc;
   if (iterations) *iterations = its;

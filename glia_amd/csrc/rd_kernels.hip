@@ -313,10 +313,11 @@ struct LaneTerms {
 // replaces a k_lane_lin launch followed by k_col_plain<L, 0, +1> without the intermediate field.
 template <int L, int NTM>
 __global__ void __launch_bounds__(CK<L>::TPB, GLIA_COL_MINW)
-k_col_lin(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, LaneTerms T, int s_out) {
+k_col_lin(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, LaneTerms T, int s_out, int honor) {
   constexpr int H = GLIA_NT_COL;
   const int l = blockIdx.y;
   const MemberDev& M = mem[l];
+  if (honor && M.st->done) return;
   GLIA_COL_PRELUDE(L)
   const Tile tl = col_tile(0, d);
   GPtrT<H> src[NTM];
@@ -1167,20 +1168,24 @@ k_z_point(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ 
 }
 
 // Z pass of a forward transform whose real input is first updated in place: the elasticity CG's
-// vector work folded into the transform that reads the vector next.
-//   ZU_AXPBY: p = z + b p (p = z when b = 0), out = R2C(p)            [ra = p, rb = z]
-//   ZU_CGUPD: x += t p, r -= t w with t = beta / q, q = *dpi (<p, A p>); out = R2C(r)
+// vector work folded into the transform that reads the vector next.  The CG scalars come from lane
+// 0's PcgState (the elasticity CG's state, kept by k_el_cg_begin / k_el_cg_fin) and sc:
+//   ZU_AXPBY: p = z + b p, b = beta / betaold (p = z in the first iteration), out = R2C(p)
+//             [ra = p, rb = z]
+//   ZU_CGUPD: x += t p, r -= t w with t = beta / q, q = sc[0] (<p, A p>); out = R2C(r)
 //             [ra = x, rb = r, rc = p, rd = w].  No update when q is 0, not finite, or of the sign
-//             opposite to dpiold after the first iteration (the host stops there).
+//             opposite to the previous <p, A p> after the first iteration (the CG stops there).
+// With honor set, a lane whose CG has finished returns at once.
 enum { ZU_AXPBY = 0, ZU_CGUPD = 1 };
 
 template <int M, int MODE>
 __global__ void __launch_bounds__(ZK<M>::TPB, GLIA_Z_MINW)
 k_z_upd_r2c(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ twm, const float2* __restrict__ tw2,
-            int ra, int rb, int rc, int rd, const double* __restrict__ dpi, double beta, double dpiold, int first,
-            int s_out) {
+            int ra, int rb, int rc, int rd, const double* __restrict__ sc, int s_out, int honor) {
   constexpr int H = GLIA_NT_ZP;
   const MemberDev& Mb = mem[blockIdx.y];
+  if (honor && Mb.st->done) return;
+  const PcgState* S = mem[0].st;
   constexpr int R1 = ZK<M>::R1, R2 = ZK<M>::R2, NT = ZK<M>::NT, VPT = ZK<M>::VPT, TPB = ZK<M>::TPB;
   constexpr int IT = ZIt<M>::value;
   __shared__ float2 lb[ZK<M>::NL * ZK<M>::LP];
@@ -1190,12 +1195,14 @@ k_z_upd_r2c(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict_
   float2* line = lb + li * ZK<M>::LP;
   float cf = 0.0f;
   bool upd = true;
+  double beta = 0.0;
   if (MODE == ZU_AXPBY) {
+    beta = S->i ? S->beta / S->betaold : 0.0;
     cf = (float)beta;
   } else {
-    const double q = *dpi;
-    upd = !(isnan(q) || isinf(q) || q == 0.0 || (!first && q * dpiold <= 0.0));
-    if (upd) cf = (float)(beta / q);
+    const double q = sc[0], dpiold = S->dpi;
+    upd = !(isnan(q) || isinf(q) || q == 0.0 || (S->i != 0 && q * dpiold <= 0.0));
+    if (upd) cf = (float)(S->beta / q);
   }
   GPtrT<H> A = rptr2<H>(Mb, ra) + base, B = rptr2<H>(Mb, rb) + base;
   float2 tmp[IT];
@@ -1973,7 +1980,8 @@ __global__ void __launch_bounds__(256) k_lane_part_sum(const MemberDev* __restri
 
 // k_el_prec across lanes: F_d in slot sf of lane d, U_d into slot su of lane d (one pass, grid.y = 1)
 __global__ void __launch_bounds__(256) k_el_prec_lanes(const MemberDev* __restrict__ mem, Dims d, int sf, int su,
-                                                       float mu, float lam, float scr) {
+                                                       float mu, float lam, float scr, int honor) {
+  if (honor && mem[0].st->done) return;
   const float2* F[3] = {craw(mem[0], sf), craw(mem[1], sf), craw(mem[2], sf)};
   float2* U[3] = {craw(mem[0], su), craw(mem[1], su), craw(mem[2], su)};
   const float s2 = lam + mu;
@@ -2136,13 +2144,13 @@ struct RunColPlain {
 };
 template <int L>
 struct RunColLin {
-  static int run(const LaunchCtx& c, const LaneTerms& T, int s_out) {
+  static int run(const LaunchCtx& c, const LaneTerms& T, int s_out, int honor) {
     dim3 g(c.d.n1 * (c.d.m2 / 16), c.nm);
     switch (T.n[0]) {
-      case 1: k_col_lin<L, 1><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, T, s_out); break;
-      case 2: k_col_lin<L, 2><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, T, s_out); break;
-      case 3: k_col_lin<L, 3><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, T, s_out); break;
-      default: k_col_lin<L, 4><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, T, s_out); break;
+      case 1: k_col_lin<L, 1><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, T, s_out, honor); break;
+      case 2: k_col_lin<L, 2><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, T, s_out, honor); break;
+      case 3: k_col_lin<L, 3><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, T, s_out, honor); break;
+      default: k_col_lin<L, 4><<<g, CK<L>::TPB, 0, c.stream>>>(c.mem, c.d, c.tw0, T, s_out, honor); break;
     }
     GLIA_CHECK_LAUNCH();
     return 0;
@@ -2164,11 +2172,11 @@ static int lane_terms(const LaunchCtx& c, const int* nterms, const int (*lane)[4
   return 0;
 }
 int launch_col_lin(const LaunchCtx& c, const int* nterms, const int (*lane)[4], const int (*slot)[4],
-                   const int (*dir)[4], int s_out) {
+                   const int (*dir)[4], int s_out, bool honor_done) {
   LaneTerms T;
   const int rc = lane_terms(c, nterms, lane, slot, dir, &T);
   if (rc) return rc;
-  return dispatch_L<RunColLin>(c.d.n0, c, T, s_out);
+  return dispatch_L<RunColLin>(c.d.n0, c, T, s_out, honor_done ? 1 : 0);
 }
 
 int launch_col_plain(const LaunchCtx& c, int dir, int sign, int s_in, int s_out, bool honor_done) {
@@ -2215,25 +2223,24 @@ int launch_z_point(const LaunchCtx& c, int mode, int s_in, int s_out, int r_slot
 }
 template <int M>
 struct RunZUpd {
-  static int run(const LaunchCtx& c, int mode, int ra, int rb, int rc, int rd, const double* dpi, double beta,
-                 double dpiold, int first, int s_out) {
+  static int run(const LaunchCtx& c, int mode, int ra, int rb, int rc, int rd, const double* sc, int s_out, int honor) {
     dim3 g((unsigned)((int64_t)c.d.n0 * c.d.n1 / ZK<M>::NL), c.nm);
     if (mode == ZU_AXPBY)
-      k_z_upd_r2c<M, ZU_AXPBY><<<g, ZK<M>::TPB, 0, c.stream>>>(c.mem, c.d, c.twm, c.tw2, ra, rb, rc, rd, dpi, beta,
-                                                                 dpiold, first, s_out);
+      k_z_upd_r2c<M, ZU_AXPBY><<<g, ZK<M>::TPB, 0, c.stream>>>(c.mem, c.d, c.twm, c.tw2, ra, rb, rc, rd, sc, s_out,
+                                                                 honor);
     else if (mode == ZU_CGUPD)
-      k_z_upd_r2c<M, ZU_CGUPD><<<g, ZK<M>::TPB, 0, c.stream>>>(c.mem, c.d, c.twm, c.tw2, ra, rb, rc, rd, dpi, beta,
-                                                                 dpiold, first, s_out);
+      k_z_upd_r2c<M, ZU_CGUPD><<<g, ZK<M>::TPB, 0, c.stream>>>(c.mem, c.d, c.twm, c.tw2, ra, rb, rc, rd, sc, s_out,
+                                                                 honor);
     else
       return -1;
     GLIA_CHECK_LAUNCH();
     return 0;
   }
 };
-int launch_z_upd_r2c(const LaunchCtx& c, int mode, int ra, int rb, int rc, int rd, const double* dpi, double beta,
-                     double dpiold, int first, int s_out) {
-  if (mode == ZU_CGUPD && !dpi) return -1;
-  return dispatch_M<RunZUpd>(c.d.m2, c, mode, ra, rb, rc, rd, dpi, beta, dpiold, first, s_out);
+int launch_z_upd_r2c(const LaunchCtx& c, int mode, int ra, int rb, int rc, int rd, const double* sc, int s_out,
+                     bool honor_done) {
+  if (mode == ZU_CGUPD && !sc) return -1;
+  return dispatch_M<RunZUpd>(c.d.m2, c, mode, ra, rb, rc, rd, sc, s_out, honor_done ? 1 : 0);
 }
 int z_blocks(const Dims& d) {
   switch (d.m2) {
@@ -2429,8 +2436,91 @@ int launch_lane_part_sum(const LaunchCtx& c, int nq, double* out) {
   return 0;
 }
 
-int launch_el_prec_lanes(const LaunchCtx& c, int sf, int su, double mu, double lam, double scr) {
-  k_el_prec_lanes<<<dim3(EW_BLOCKS, 1), 256, 0, c.stream>>>(c.mem, c.d, sf, su, (float)mu, (float)lam, (float)scr);
+// The elasticity CG's state machine (KSPSolve_CG with KSP_NORM_PRECONDITIONED and
+// KSPConvergedDefault, as the host loop of el_capi.cpp ran it) in lane 0's PcgState, one thread.
+// k_el_cg_begin starts it from the host's initial scalars; k_el_cg_fin closes an iteration from the
+// summed partials sc = (<p, A p>, <z, z>, <z, r>) and makes the checks of the next iteration's
+// start.  When it stops, every lane is marked done, and the mapped host flag is set.
+__global__ void k_el_cg_begin(const MemberDev* __restrict__ mem, int nl, double beta0, double dp0, double rnorm0,
+                              double ttol, double atol, double dtol, int maxit) {
+  if (threadIdx.x != 0) return;
+  PcgState* S = mem[0].st;
+  S->i = 0;
+  S->its = 1;
+  S->beta = beta0;
+  S->betaold = 0.0;
+  S->dpi = 0.0;
+  S->dpiold = 0.0;
+  S->dp = dp0;
+  S->rnorm0 = rnorm0;
+  S->ttol = ttol;
+  S->atol = atol;
+  S->dtol = dtol;
+  S->maxit = maxit;
+  S->reason = (beta0 == 0.0) ? KSPR_CONVERGED_ATOL : 0;
+  const int done = S->reason != 0;
+  for (int l = 0; l < nl; l++) mem[l].st->done = done;
+}
+
+__global__ void k_el_cg_fin(const MemberDev* __restrict__ mem, int nl, const double* __restrict__ sc, int* hflag) {
+  if (threadIdx.x != 0) return;
+  PcgState* S = mem[0].st;
+  if (S->done) return;
+  const double q = sc[0], zz = sc[1], zr = sc[2];
+  const int i = S->i;
+  const double dpiold = S->dpi;
+  S->betaold = S->beta;
+  S->dpiold = dpiold;
+  S->dpi = q;
+  int reason = 0;
+  if (isnan(q) || isinf(q)) {
+    reason = KSPR_DIVERGED_NANORINF;
+  } else if (q == 0.0 || (i > 0 && q * dpiold <= 0.0)) {
+    reason = KSPR_DIVERGED_INDEFINITE_MAT;
+  } else {
+    S->dp = sqrt(zz);
+    reason = ksp_conv_test(S, S->dp);
+    if (!reason) {
+      S->beta = zr;
+      if (isnan(zr) || isinf(zr)) {
+        reason = KSPR_DIVERGED_NANORINF;
+      } else {
+        S->i = i + 1;
+        if (i + 1 >= S->maxit) {
+          reason = KSPR_DIVERGED_ITS;
+        } else {
+          S->its = i + 2;
+          if (zr == 0.0)
+            reason = KSPR_CONVERGED_ATOL;
+          else if (zr * S->betaold < 0.0)
+            reason = KSPR_DIVERGED_INDEFINITE_PC;
+        }
+      }
+    }
+  }
+  if (reason) {
+    S->reason = reason;
+    for (int l = 0; l < nl; l++) mem[l].st->done = 1;
+    if (hflag) *hflag = 1;
+  }
+}
+
+int launch_el_cg_begin(const LaunchCtx& c, double beta0, double dp0, double rnorm0, double ttol, double atol,
+                       double dtol, int maxit) {
+  k_el_cg_begin<<<1, 64, 0, c.stream>>>(c.mem, c.nm, beta0, dp0, rnorm0, ttol, atol, dtol, maxit);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+
+int launch_el_cg_fin(const LaunchCtx& c, const double* sc, int* hflag) {
+  k_el_cg_fin<<<1, 64, 0, c.stream>>>(c.mem, c.nm, sc, hflag);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
+
+int launch_el_prec_lanes(const LaunchCtx& c, int sf, int su, double mu, double lam, double scr, bool honor_done) {
+  k_el_prec_lanes<<<dim3(EW_BLOCKS, 1), 256, 0, c.stream>>>(c.mem, c.d, sf, su, (float)mu, (float)lam, (float)scr,
+                                                             honor_done ? 1 : 0);
   GLIA_CHECK_LAUNCH();
   return 0;
 }

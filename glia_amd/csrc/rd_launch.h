@@ -42,12 +42,16 @@ int launch_xf_div(const LaunchCtx& c, int mode, int s_yx, int s_rr, int s_p, int
 int launch_col_plain(const LaunchCtx& c, int dir, int sign, int s_in, int s_out, bool honor_done);
 int launch_z_diff(const LaunchCtx& c, int s_gx, int s_gy, int s_gz, int k_slot);
 int launch_z_point(const LaunchCtx& c, int mode, int s_in, int s_out, int r_slot, int ch_idx, float dt, bool honor_done);
-// Z pass of a forward transform with the elasticity CG's vector update folded in (k_z_upd_r2c):
-// ZUM_AXPBY: p = z + beta p, out = R2C(p) [ra p, rb z]; ZUM_CGUPD: x += t p, r -= t w with
-// t = beta / *dpi, out = R2C(r) [ra x, rb r, rc p, rd w]
+// Z pass of a forward transform with the elasticity CG's vector update folded in (k_z_upd_r2c; the
+// scalars from lane 0's PcgState and sc): ZUM_AXPBY: p = z + (beta / betaold) p, out = R2C(p)
+// [ra p, rb z]; ZUM_CGUPD: x += t p, r -= t w with t = beta / sc[0], out = R2C(r) [ra x, rb r, rc p, rd w]
 enum { ZUM_AXPBY = 0, ZUM_CGUPD = 1 };
-int launch_z_upd_r2c(const LaunchCtx& c, int mode, int ra, int rb, int rc, int rd, const double* dpi, double beta,
-                     double dpiold, int first, int s_out);
+int launch_z_upd_r2c(const LaunchCtx& c, int mode, int ra, int rb, int rc, int rd, const double* sc, int s_out,
+                     bool honor_done);
+// the elasticity CG's state machine in lane 0's PcgState (k_el_cg_begin / k_el_cg_fin)
+int launch_el_cg_begin(const LaunchCtx& c, double beta0, double dp0, double rnorm0, double ttol, double atol,
+                       double dtol, int maxit);
+int launch_el_cg_fin(const LaunchCtx& c, const double* sc, int* hflag);
 int launch_z_gradq(const LaunchCtx& c, int4 sc, int4 sa);
 
 int launch_pcg_begin(const LaunchCtx& c, double tau);
@@ -79,12 +83,13 @@ int launch_el_prec(const LaunchCtx& c, int4 sf, int4 su, double mu, double lam, 
 int launch_lane_lin(const LaunchCtx& c, const int* nterms, const int (*lane)[4], const int (*slot)[4],
                     const int (*dir)[4], int s_out);
 // the inverse X pass of the same combination, formed as the columns load, into complex slot s_out
+// (honor_done: lanes whose CG has finished return at once)
 int launch_col_lin(const LaunchCtx& c, const int* nterms, const int (*lane)[4], const int (*slot)[4],
-                   const int (*dir)[4], int s_out);
+                   const int (*dir)[4], int s_out, bool honor_done);
 // out[q], q < nq: the Z-pass block partials (nq per block) of the c.nm lanes, summed on the device
 int launch_lane_part_sum(const LaunchCtx& c, int nq, double* out);
 // the elasticity preconditioner with F_d and U_d in lane d (members 0..2)
-int launch_el_prec_lanes(const LaunchCtx& c, int sf, int su, double mu, double lam, double scr);
+int launch_el_prec_lanes(const LaunchCtx& c, int sf, int su, double mu, double lam, double scr, bool honor_done);
 int launch_bricks(const LaunchCtx& c, const BrickJob* jobs_host, int njobs, const double* taps_host, int tap_stride,
                   double h0, double h1, double h2, double sigma, float* store_dev, double* max_out_host);
 

@@ -676,35 +676,38 @@ int RDEnsemble::lane_lin(const int* nterms, const int (*lane)[4], const int (*sl
 
 int RDEnsemble::el_prec_lanes(int cf, int cu, double mu, double lam, double scr) {
   if (nm_ < 3) return fail(-2, "the lane preconditioner needs three members");
-  GLIA_TRYC(launch_el_prec_lanes(ctx(), cf, cu, mu, lam, scr));
+  GLIA_TRYC(launch_el_prec_lanes(ctx(), cf, cu, mu, lam, scr, honor_lanes_));
   return 0;
 }
 
 int RDEnsemble::lin_coef_transform(const int* nterms, const int (*lane)[4], const int (*slot)[4], const int (*dir)[4],
                                    int coef_rs, int cs_out) {
   LaunchCtx c = ctx();
-  GLIA_TRYC(launch_col_lin(c, nterms, lane, slot, dir, CS_T7));
-  GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T7, CS_T7, false));
-  GLIA_TRYC(launch_z_point(c, ZPM_COEF, CS_T7, CS_T7, coef_rs, -1, 0.f, false));
-  GLIA_TRYC(launch_col_plain(c, 1, -1, CS_T7, CS_T7, false));
-  GLIA_TRYC(launch_col_plain(c, 0, -1, CS_T7, cs_out, false));
+  const bool hd = honor_lanes_;
+  GLIA_TRYC(launch_col_lin(c, nterms, lane, slot, dir, CS_T7, hd));
+  GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T7, CS_T7, hd));
+  GLIA_TRYC(launch_z_point(c, ZPM_COEF, CS_T7, CS_T7, coef_rs, -1, 0.f, hd));
+  GLIA_TRYC(launch_col_plain(c, 1, -1, CS_T7, CS_T7, hd));
+  GLIA_TRYC(launch_col_plain(c, 0, -1, CS_T7, cs_out, hd));
   return 0;
 }
 
 int RDEnsemble::lin_screen_transform(const int* nterms, const int (*lane)[4], const int (*slot)[4],
                                      const int (*dir)[4], int u_rs, int out_rs) {
   LaunchCtx c = ctx();
-  GLIA_TRYC(launch_col_lin(c, nterms, lane, slot, dir, CS_T7));
-  GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T7, CS_T7, false));
-  GLIA_TRYC(launch_z_point(c, ZPM_SCREEN, CS_T7, u_rs, out_rs, -1, 0.f, false));
+  const bool hd = honor_lanes_;
+  GLIA_TRYC(launch_col_lin(c, nterms, lane, slot, dir, CS_T7, hd));
+  GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T7, CS_T7, hd));
+  GLIA_TRYC(launch_z_point(c, ZPM_SCREEN, CS_T7, u_rs, out_rs, -1, 0.f, hd));
   return 0;
 }
 
 int RDEnsemble::c2r_dots(int cs_in, int out_rs, int r_rs) {
   LaunchCtx c = ctx();
-  GLIA_TRYC(launch_col_plain(c, 0, +1, cs_in, CS_T7, false));
-  GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T7, CS_T7, false));
-  GLIA_TRYC(launch_z_point(c, ZPM_C2R_DOT, CS_T7, r_rs, out_rs, -1, 0.f, false));
+  const bool hd = honor_lanes_;
+  GLIA_TRYC(launch_col_plain(c, 0, +1, cs_in, CS_T7, hd));
+  GLIA_TRYC(launch_col_plain(c, 1, +1, CS_T7, CS_T7, hd));
+  GLIA_TRYC(launch_z_point(c, ZPM_C2R_DOT, CS_T7, r_rs, out_rs, -1, 0.f, hd));
   return 0;
 }
 
@@ -713,21 +716,48 @@ int RDEnsemble::lane_part_sum(int nq, double* out) {
   return 0;
 }
 
-int RDEnsemble::fft_r2c_axpby(int p_rs, int z_rs, double b, int cs_out) {
+int RDEnsemble::fft_r2c_axpby(int p_rs, int z_rs, int cs_out) {
   LaunchCtx c = ctx();
-  GLIA_TRYC(launch_z_upd_r2c(c, ZUM_AXPBY, p_rs, z_rs, -1, -1, nullptr, b, 0.0, 0, cs_out));
-  GLIA_TRYC(launch_col_plain(c, 1, -1, cs_out, cs_out, false));
-  GLIA_TRYC(launch_col_plain(c, 0, -1, cs_out, cs_out, false));
+  const bool hd = honor_lanes_;
+  GLIA_TRYC(launch_z_upd_r2c(c, ZUM_AXPBY, p_rs, z_rs, -1, -1, nullptr, cs_out, hd));
+  GLIA_TRYC(launch_col_plain(c, 1, -1, cs_out, cs_out, hd));
+  GLIA_TRYC(launch_col_plain(c, 0, -1, cs_out, cs_out, hd));
   return 0;
 }
 
-int RDEnsemble::fft_r2c_cgupd(int x_rs, int r_rs, int p_rs, int w_rs, const double* dpi, double beta, double dpiold,
-                              bool first, int cs_out) {
-  if (!dpi) return fail(-2, "fft_r2c_cgupd needs the device scalar <p, A p>");
+int RDEnsemble::fft_r2c_cgupd(int x_rs, int r_rs, int p_rs, int w_rs, const double* sc, int cs_out) {
+  if (!sc) return fail(-2, "fft_r2c_cgupd needs the device scalar <p, A p>");
   LaunchCtx c = ctx();
-  GLIA_TRYC(launch_z_upd_r2c(c, ZUM_CGUPD, x_rs, r_rs, p_rs, w_rs, dpi, beta, dpiold, first ? 1 : 0, cs_out));
-  GLIA_TRYC(launch_col_plain(c, 1, -1, cs_out, cs_out, false));
-  GLIA_TRYC(launch_col_plain(c, 0, -1, cs_out, cs_out, false));
+  const bool hd = honor_lanes_;
+  GLIA_TRYC(launch_z_upd_r2c(c, ZUM_CGUPD, x_rs, r_rs, p_rs, w_rs, sc, cs_out, hd));
+  GLIA_TRYC(launch_col_plain(c, 1, -1, cs_out, cs_out, hd));
+  GLIA_TRYC(launch_col_plain(c, 0, -1, cs_out, cs_out, hd));
+  return 0;
+}
+
+int RDEnsemble::el_cg_begin(double beta0, double dp0, double rnorm0, double ttol, double atol, double dtol, int maxit) {
+  GLIA_TRYC(launch_el_cg_begin(ctx(), beta0, dp0, rnorm0, ttol, atol, dtol, maxit));
+  return 0;
+}
+
+int RDEnsemble::el_cg_fin(const double* sc, int* hflag) {
+  GLIA_TRYC(launch_el_cg_fin(ctx(), sc, hflag));
+  return 0;
+}
+
+int RDEnsemble::el_cg_end() {
+  honor_lanes_ = false;
+  GLIA_TRYC(launch_set_done(ctx(), 0));
+  return 0;
+}
+
+int RDEnsemble::el_cg_state(int* its, int* reason, double* dp) {
+  PcgState s;
+  GLIA_HIP(hipMemcpyAsync(&s, mh_[0].st, sizeof(PcgState), hipMemcpyDeviceToHost, stream_));
+  GLIA_HIP(hipStreamSynchronize(stream_));
+  if (its) *its = s.its;
+  if (reason) *reason = s.reason;
+  if (dp) *dp = s.dp;
   return 0;
 }
 

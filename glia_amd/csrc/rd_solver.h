@@ -110,12 +110,22 @@ class RDEnsemble {
   // out[q] (device), q < nq: the last Z pass's block partials of the lanes in use, summed in a fixed
   // order (lin_screen_transform leaves <u, out>, c2r_dots <out, out> and <out, r>)
   int lane_part_sum(int nq, double* out);
-  // packed spectrum cs_out of the real slot p_rs after p = z + b p in place (p = z when b = 0)
-  int fft_r2c_axpby(int p_rs, int z_rs, double b, int cs_out);
-  // packed spectrum cs_out of the real slot r_rs after x += t p, r -= t w in place, t = beta / *dpi
-  // (skipped when *dpi is 0, not finite, or of the sign opposite to dpiold after the first iteration)
-  int fft_r2c_cgupd(int x_rs, int r_rs, int p_rs, int w_rs, const double* dpi, double beta, double dpiold, bool first,
-                    int cs_out);
+  // The elasticity CG on the lanes, its scalars in lane 0's PcgState (launch_el_cg_begin /
+  // launch_el_cg_fin).  With honor_lanes set, the lane launches below return at once for lanes whose
+  // CG has finished, so iterations enqueued past the stop cost only their launches.
+  void set_honor_lanes(bool v) { honor_lanes_ = v; }
+  // packed spectrum cs_out of the real slot p_rs after p = z + (beta / betaold) p in place (p = z in
+  // the first iteration)
+  int fft_r2c_axpby(int p_rs, int z_rs, int cs_out);
+  // packed spectrum cs_out of the real slot r_rs after x += t p, r -= t w in place, t = beta / sc[0]
+  // (skipped when sc[0] is 0, not finite, or of the sign opposite to the previous one)
+  int fft_r2c_cgupd(int x_rs, int r_rs, int p_rs, int w_rs, const double* sc, int cs_out);
+  int el_cg_begin(double beta0, double dp0, double rnorm0, double ttol, double atol, double dtol, int maxit);
+  // closes an iteration from sc = (<p, A p>, <z, z>, <z, r>); sets *hflag (mapped) when the CG stops
+  int el_cg_fin(const double* sc, int* hflag);
+  int el_cg_state(int* its, int* reason, double* dp);  // synchronizes the stream
+  // after a CG: lane launches no longer honor done, and the lanes' done flags are cleared
+  int el_cg_end();
   int get_spectrum(int m, int cslot, float2* host);
   int set_spectrum(int m, int cslot, const float2* host);
   int solve_state(int linearized, bool keep_history);
@@ -188,6 +198,7 @@ class RDEnsemble {
   unsigned long long uid_ = 0;
   unsigned long long graph_epoch_ = 0;
   int lanes_ = 0;
+  bool honor_lanes_ = false;  // lane launches skip lanes whose elasticity CG has finished
   int eval_mode_ = 2;
   int eval_setup(int mode, const std::vector<std::vector<double>>& x);
   int eval_result(int mode, bool want_grad, std::vector<double>& J, std::vector<std::vector<double>>& dJ);
