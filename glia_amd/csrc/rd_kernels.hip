@@ -804,6 +804,96 @@ k_xf_div_s(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__
   }
 }
 
+// X pass, forward, closing the divergence in three groups: Yx -> i w_x FFT_x(Yx) into LDS; Rr ->
+// FFT_x(Rr) into LDS; the third group loads the operand of the output stage at kernel start and
+// forms D = (first) + (second) and the MODE's output and partial sums as in k_xf_div_s.  A thread
+// holds one field's values, so more waves fit per SIMD than in the two-group kernel; every output
+// value takes the same operations in the same order (the same results bit for bit).
+template <int L, int MODE>
+__global__ void __launch_bounds__(3 * CK<L>::TPB)
+k_xf_div_s3(const MemberDev* __restrict__ mem, Dims d, const float2* __restrict__ tw, int s_yx, int s_rr, int s_p,
+            int s_out) {
+  static_assert(CKS<L>::OK, "thread groups must be whole waves");
+  constexpr int H = GLIA_NT_XF;
+  const MemberDev& M = mem[blockIdx.y];
+  const PcgState* st = M.st;
+  if (st->done) return;
+  using K = CK<L>;
+  constexpr int R1 = K::R1, R2 = K::R2, NT = K::NT;
+  __shared__ float2 ex[2][16 * K::CP];
+  const int grp = (int)threadIdx.x / K::TPB;
+  const int lt = (int)threadIdx.x - grp * K::TPB;
+  const int col = lt & 15, t = lt >> 4;
+  float2* exc = ex[grp == 1 ? 1 : 0] + col * K::CP;
+  const Tile tl = col_tile(0, d);
+  const float tau = (float)st->tau;
+  const float pc = (float)st->pc;
+  const float wy = wnum(tl.o, d.n1), wz = (float)(tl.c0 + col);
+  const double wgt = (tl.c0 + col == 0) ? 1.0 : 2.0;
+  float2 a[K::VPT];
+  if (grp < 2) {
+    GPtrT<H> In = cptr<H>(M, grp == 0 ? s_yx : s_rr) + tl.base;
+#pragma unroll
+    for (int j = 0; j < K::VIN; j++) a[j] = In[(unsigned)in_row<R1, R2, NT>(t, j) * tl.rs + col];
+  } else if (MODE != XF_D) {
+    GPtrT<H> P = cptr<H>(M, s_p) + tl.base;
+#pragma unroll
+    for (int j = 0; j < K::VOUT; j++) if (K::out_ok(t, j)) a[j] = P[(unsigned)out_row<R1, R2, NT>(t, j) * tl.rs + col];
+  }
+  __shared__ float2 twl[L];
+  stage_twiddles<L, 3 * K::TPB>(twl, tw);
+  if (grp < 2)
+    fft2s<R1, R2, NT, -1>(a, exc, t, twl);
+  else
+    __syncthreads();  // the one barrier of fft2s
+  __syncthreads();    // both exchange areas are free again
+  if (grp < 2) {
+#pragma unroll
+    for (int j = 0; j < K::VOUT; j++) {
+      if (!K::out_ok(t, j)) continue;
+      const int row = out_row<R1, R2, NT>(t, j);
+      exc[xpad(row)] = grp == 0 ? mul_i(cscale(a[j], wnum(row, d.n0))) : a[j];
+    }
+  }
+  __syncthreads();
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (grp == 2) {
+    GPtrT<H> O = cptr<H>(M, s_out) + tl.base;
+    const float2* exa = ex[0] + col * K::CP;
+    const float2* exb = ex[1] + col * K::CP;
+#pragma unroll
+    for (int j = 0; j < K::VOUT; j++) {
+      if (!K::out_ok(t, j)) continue;
+      const int row = out_row<R1, R2, NT>(t, j);
+      const unsigned g = (unsigned)row * tl.rs + col;
+      const float2 D = cadd(exa[xpad(row)], exb[xpad(row)]);
+      if (MODE == XF_MATVEC) {
+        const float2 p = a[j];
+        const float2 w = make_float2(p.x - 0.5f * tau * D.x, p.y - 0.5f * tau * D.y);
+        O[g] = w;
+        acc[0] += wgt * ((double)p.x * w.x + (double)p.y * w.y);
+      } else if (MODE == XF_SETUP) {
+        const float2 x = a[j];
+        const float2 bb = make_float2(x.x + 0.5f * tau * D.x, x.y + 0.5f * tau * D.y);
+        const float2 r = cscale(D, tau);
+        O[g] = r;
+        const float f = precf(pc, wnum(row, d.n0), wy, wz);
+        const double r2 = (double)r.x * r.x + (double)r.y * r.y;
+        acc[0] += wgt * (double)f * f * ((double)bb.x * bb.x + (double)bb.y * bb.y);
+        acc[1] += wgt * (double)f * f * r2;
+        acc[2] += wgt * (double)f * r2;
+      } else {
+        O[g] = D;
+      }
+    }
+  }
+  if (MODE == XF_SETUP) block_reduce_store<3>(acc, M.part + (int64_t)blockIdx.x * 3);
+  if (MODE == XF_MATVEC) {
+    double a1[1] = {acc[0]};
+    block_reduce_store<1>(a1, M.part + blockIdx.x);
+  }
+}
+
 // plain column transform: out = FFT_dir(in) along axis DIR (S = -1 forward, +1 inverse)
 template <int L, int DIR, int S>
 __global__ void __launch_bounds__(CK<L>::TPB, GLIA_COL_MINW)
@@ -2106,8 +2196,14 @@ template <int L>
 struct RunXfDiv {
   static int run(const LaunchCtx& c, int mode, int s_yx, int s_rr, int s_p, int s_out) {
     dim3 g(c.d.n1 * (c.d.m2 / 16), c.nm);
-    const int tb = 2 * CK<L>::TPB;
-    if (c.xf_split && mode == XF_MATVEC)
+    const int tb = 2 * CK<L>::TPB, tb3 = 3 * CK<L>::TPB;
+    if (c.xf_split == 2 && mode == XF_MATVEC)
+      k_xf_div_s3<L, XF_MATVEC><<<g, tb3, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
+    else if (c.xf_split == 2 && mode == XF_SETUP)
+      k_xf_div_s3<L, XF_SETUP><<<g, tb3, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
+    else if (c.xf_split == 2)
+      k_xf_div_s3<L, XF_D><<<g, tb3, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
+    else if (c.xf_split && mode == XF_MATVEC)
       k_xf_div_s<L, XF_MATVEC><<<g, tb, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);
     else if (c.xf_split && mode == XF_SETUP)
       k_xf_div_s<L, XF_SETUP><<<g, tb, 0, c.stream>>>(c.mem, c.d, c.tw0, s_yx, s_rr, s_p, s_out);

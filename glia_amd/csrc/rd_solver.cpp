@@ -102,8 +102,8 @@ int RDEnsemble::init() {
   if (const char* v = std::getenv("GLIA_STEP_MARGIN")) margin_ = std::atoi(v);
   // forward divergence variants: the X pass in two thread groups (measured faster at 256 points),
   // the Y pass in one.  GLIA_SPLIT_DIV overrides with a comma-separated list of none, xf (X pass
-  // in two groups), yf (Y pass in three groups), yfb / yfb2 (Y pass split over blocks at 3 / 2
-  // waves per SIMD), both (xf and yf); glia_rd_autotune measures them.
+  // in two groups), xf3 (X pass in three groups), yf (Y pass in three groups), yfb / yfb2 (Y pass
+  // split over blocks at 3 / 2 waves per SIMD), both (xf and yf); glia_rd_autotune measures them.
   xf_split_ = 1;
   yf_split_ = 0;
   if (const char* v = std::getenv("GLIA_SPLIT_DIV")) {
@@ -112,6 +112,7 @@ int RDEnsemble::init() {
     std::string tok;
     while (std::getline(toks, tok, ',')) {
       if (tok == "xf" || tok == "both") xf_split_ = 1;
+      if (tok == "xf3") xf_split_ = 2;
       if (tok == "yf" || tok == "both") yf_split_ = 1;
       if (tok == "yfb") yf_split_ = 2;
       if (tok == "yfb2") yf_split_ = 3;
@@ -1549,7 +1550,7 @@ int RDEnsemble::autotune(const char* cache_path, int* choice) {
       if (line.compare(0, key.str().size() + 1, key.str() + " ") != 0) continue;
       std::istringstream ls(line.substr(key.str().size() + 1));
       int xf = -1, yf = -1;
-      if (ls >> xf >> yf && (xf == 0 || xf == 1) && yf >= 0 && yf < kYfVariants) {
+      if (ls >> xf >> yf && xf >= 0 && xf < kXfVariants && yf >= 0 && yf < kYfVariants) {
         xf_split_ = xf;
         yf_split_ = yf;
         if (choice) { choice[0] = xf; choice[1] = yf; }
@@ -1579,8 +1580,8 @@ int RDEnsemble::autotune(const char* cache_path, int* choice) {
     *ms = t[2];
     return 0;
   };
-  double xf_ms[2], yf_ms[kYfVariants];
-  for (int v = 0; v < 2 && !rc; v++) {
+  double xf_ms[kXfVariants], yf_ms[kYfVariants];
+  for (int v = 0; v < kXfVariants && !rc; v++) {
     xf_split_ = v;
     rc = time_it([&]() { return launch_xf_div(ctx(), XFM_MATVEC, CS_T0, CS_T1, CS_P, CS_W); }, &xf_ms[v]);
   }
@@ -1593,12 +1594,13 @@ int RDEnsemble::autotune(const char* cache_path, int* choice) {
   if (!rc) GLIA_TRYC(launch_set_done(ctx(), 1));  // no solve in progress
   if (!rc) rc = sync();
   if (rc) return rc;
-  xf_split_ = xf_ms[1] < xf_ms[0] ? 1 : 0;
+  xf_split_ = (int)(std::min_element(xf_ms, xf_ms + kXfVariants) - xf_ms);
   yf_split_ = (int)(std::min_element(yf_ms, yf_ms + kYfVariants) - yf_ms);
   if (choice) { choice[0] = xf_split_; choice[1] = yf_split_; }
   if (cache_path) {
     std::ofstream out(cache_path, std::ios::app);
-    out << key.str() << ' ' << xf_split_ << ' ' << yf_split_ << ' ' << xf_ms[0] << ' ' << xf_ms[1];
+    out << key.str() << ' ' << xf_split_ << ' ' << yf_split_;
+    for (int v = 0; v < kXfVariants; v++) out << ' ' << xf_ms[v];
     for (int v = 0; v < kYfVariants; v++) out << ' ' << yf_ms[v];
     out << '\n';
   }

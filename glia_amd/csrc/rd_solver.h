@@ -310,6 +310,7 @@ class RDEnsemble {
   };
   int yf_split_ = 0, xf_split_ = 0;  // variants of the forward divergence passes (autotune)
   static constexpr int kYfVariants = 4;
+  static constexpr int kXfVariants = 3;
   bool quad_on_ = true;     // adjoint steps accumulate the kappa/rho gradient quadratures
   bool use_graphs_ = true;  // replay CG iterations from captured graphs (not while profiling)
   double* smooth_taps_ = nullptr;  // device taps of the last smoothing width (three axes)

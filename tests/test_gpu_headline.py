@@ -171,7 +171,7 @@ def test_headline_grid_gradient_vs_oracle_fixture(rd, name):
         assert abs(g[0][k] - go[k]) / max(abs(go[k]), 1e-2 * np.linalg.norm(go)) < 1e-4
 
 
-@pytest.mark.parametrize("split", ["none", "xf", "yf", "yfb", "yfb2"])
+@pytest.mark.parametrize("split", ["none", "xf", "xf3", "yf", "yfb", "yfb2"])
 @pytest.mark.parametrize("n", [(32, 256, 32), (256, 32, 32), (32, 128, 32)])
 def test_divergence_variants_headline_plans_vs_oracle(rd, oracle_mod, n, split):
     """Every variant of the forward divergence passes the autotuner may choose (GLIA_SPLIT_DIV) on

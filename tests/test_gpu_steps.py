@@ -102,22 +102,28 @@ def test_autotune_choice_cache_and_results(members, tmp_path, oracle_mod):
     s = make(members)
     choice = s.autotune(str(cache))
     print("choice", choice, open(cache).read(), flush=True)
-    assert choice[0] in (0, 1) and choice[1] in (0, 1, 2, 3)
+    assert choice[0] in (0, 1, 2) and choice[1] in (0, 1, 2, 3)
     lines = open(cache).read().splitlines()
-    assert len(lines) == 1 and len(lines[0].split()) == 13
+    assert len(lines) == 1 and len(lines[0].split()) == 14
     Jt, gt = s.objective_and_gradient(x)
     s.close()
     s = make(members)
     assert s.autotune(str(cache)) == choice
     assert len(open(cache).read().splitlines()) == 1
     s.close()
-    for split in ("none", "xf", "yf", "both", "yfb", "xf,yfb2"):
+    by_split = {}
+    for split in ("none", "xf", "xf3", "yf", "both", "yfb", "xf,yfb2"):
         s = make(members, {"GLIA_SPLIT_DIV": split})
         J, g = s.objective_and_gradient(x)
         s.close()
+        by_split[split] = (J, g)
         for m in range(2):
             assert abs(J[m] - Jt[m]) <= 1e-6 * abs(Jt[m]), split
             assert rel(g[m], gt[m]) < 1e-6, split
+    # the X pass in three groups takes the operations of the two-group pass: the same bits
+    for m in range(2):
+        assert by_split["xf3"][0][m] == by_split["xf"][0][m]
+        assert np.array_equal(by_split["xf3"][1][m], by_split["xf"][1][m])
     # member 0 against the oracle
     pr = members[0]
     o = oracle_mod.Oracle(N)
