@@ -58,8 +58,8 @@ namespace glia {
 using namespace dev;
 
 // Non-temporal bits of the passes (bit 0 loads, bit 1 stores; see GRefT), chosen by timing: on the
-// 8 x 256^3 evaluation the CG passes gain about 5% over none (the first X pass streams its loads
-// only: the next pass reads its outputs at once); on the 160^3 elasticity solve the plain column
+// 8 x 256^3 evaluation the CG passes gain about 5% over none (both X passes stream their loads only:
+// the next pass reads their outputs at once); on the 160^3 elasticity solve the plain column
 // transforms and the pointwise Z passes gain about 4% from streamed stores alone.
 #ifndef GLIA_NT_XI
 #define GLIA_NT_XI 1
@@ -74,7 +74,7 @@ using namespace dev;
 #define GLIA_NT_YF 3
 #endif
 #ifndef GLIA_NT_XF
-#define GLIA_NT_XF 3
+#define GLIA_NT_XF 1
 #endif
 #ifndef GLIA_NT_PU
 #define GLIA_NT_PU 3
