@@ -126,6 +126,7 @@ IO_SIGNATURES = {
     "glia_nc_write3_f32": (_i, [_cp, _cp, _vp, _vp, _i]),
     "glia_split_segmentation": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "glia_rd_parse_config": (_i, [_cp, _vp]),
+    "glia_edema_obs_mask": (_i, [_vp, _vp, _i64, _d, _vp]),
 }
 
 
@@ -196,6 +197,8 @@ class FileConfig(ctypes.Structure):
         ("a_seg_path", ctypes.c_char * 512), ("a_wm_path", ctypes.c_char * 512), ("a_gm_path", ctypes.c_char * 512),
         ("a_csf_path", ctypes.c_char * 512), ("a_vt_path", ctypes.c_char * 512), ("d1_path", ctypes.c_char * 512),
         ("obs_mask_path", ctypes.c_char * 512),
+        ("obs_lambda", ctypes.c_double), ("smoothing_factor_data", ctypes.c_double), ("obs_threshold_rel", ctypes.c_int),
+        ("patient_labels", ctypes.c_int * 8), ("p_seg_path", ctypes.c_char * 512),
     ]
 
 

@@ -270,6 +270,13 @@ int glia_nc_write3_f32(const char* path, const char* var, const int* n, const fl
   return f ? 0 : GLIA_RD_ERR_ARG;
 }
 
+int glia_edema_obs_mask(const float* tc, const float* ed, int64_t count, double lambda, float* mask) {
+  if (!tc || !ed || !mask || count <= 0) return GLIA_RD_ERR_ARG;
+  const float lam = (float)lambda;
+  for (int64_t i = 0; i < count; i++) mask[i] = (tc[i] > 0.99f) ? 1.f : ((ed[i] > 0.99f) ? 0.f : lam);
+  return 0;
+}
+
 int glia_split_segmentation(const float* seg, int64_t count, const int* labels, float* wm, float* gm, float* vt,
                             float* csf, float* tc, float* ed) {
   if (!seg || !labels || !wm || !gm || !vt || count <= 0) return GLIA_RD_ERR_ARG;
@@ -317,7 +324,9 @@ int glia_rd_parse_config(const char* path, glia_rd_file_config* o) {
   o->np = 1;
   o->sigma_factor = 1.0;
   o->sigma_spacing = 1.5;
-  for (int i = 0; i < 8; i++) o->atlas_labels[i] = -1;
+  for (int i = 0; i < 8; i++) o->atlas_labels[i] = o->patient_labels[i] = -1;
+  o->obs_lambda = -1.0;
+  o->smoothing_factor_data = 1.0;
   std::ifstream f(path ? path : "");
   if (!f) return GLIA_RD_ERR_ARG;
   std::string line;
@@ -357,7 +366,12 @@ int glia_rd_parse_config(const char* path, glia_rd_file_config* o) {
       else if (k == "a_vt_path") copy_path(o->a_vt_path, v);
       else if (k == "d1_path") copy_path(o->d1_path, v);
       else if (k == "obs_mask_path") copy_path(o->obs_mask_path, v);
-      else if (k == "atlas_labels") {
+      else if (k == "p_seg_path") copy_path(o->p_seg_path, v);
+      else if (k == "obs_lambda") o->obs_lambda = std::stod(v);
+      else if (k == "smoothing_factor_data") o->smoothing_factor_data = std::stod(v);
+      else if (k == "obs_threshold_rel") o->obs_threshold_rel = std::stoi(v) > 0;
+      else if (k == "atlas_labels" || k == "patient_labels") {
+        int* dst = (k == "atlas_labels") ? o->atlas_labels : o->patient_labels;
         // [wm=6,gm=5,vt=7,csf=8,...] (src/utils/IO.cpp:126-170)
         std::string s = v;
         const size_t a = s.find('['), b = s.find(']');
@@ -371,7 +385,7 @@ int glia_rd_parse_config(const char* path, glia_rd_file_config* o) {
           const size_t e = item.find('=');
           if (e != std::string::npos)
             for (int i = 0; i < 8; i++)
-              if (item.substr(0, e) == names[i]) o->atlas_labels[i] = std::stoi(item.substr(e + 1));
+              if (item.substr(0, e) == names[i]) dst[i] = std::stoi(item.substr(e + 1));
           pos = c + 1;
         }
       }

@@ -67,19 +67,39 @@ def parse_config(path):
         v = getattr(c, name)
         if isinstance(v, bytes):
             v = v.decode()
-        elif name == "atlas_labels":
+        elif name in ("atlas_labels", "patient_labels"):
             v = list(v)
         out[name] = v
+    return out
+
+
+def edema_obs_mask(tc, ed, lam):
+    """createEdemaBasedObservationMask (src/utils/Utils.cpp:567-590): OBS = 1[TC] + lambda 1[B/WT],
+    i.e. 1 where tc > 0.99, else 0 where ed > 0.99, else lambda."""
+    t = np.ascontiguousarray(tc, dtype=np.float32)
+    e = np.ascontiguousarray(ed, dtype=np.float32)
+    if t.shape != e.shape:
+        raise ValueError("tumor core and edema maps differ in shape")
+    out = np.empty_like(t)
+    rc = _lib.load().glia_edema_obs_mask(t.ctypes.data_as(ctypes.c_void_p), e.ctypes.data_as(ctypes.c_void_p), t.size,
+                                         float(lam), out.ctypes.data_as(ctypes.c_void_p))
+    if rc:
+        raise ValueError("bad tumor core / edema maps (code %d)" % rc)
     return out
 
 
 def load_atlas(solver, member, seg, labels, smoothing_factor=1.0):
     """SolverInterface::readAtlas with a segmentation: split the label image (an array or a NetCDF
     path), upload the maps, smooth them on the GPU with sigma = smoothing_factor * 2 pi / n0, then
-    derive mask, k, rho, the preconditioner and Phi (MatProp::setValuesCustom)."""
+    derive mask, k, rho, the preconditioner and Phi (MatProp::setValuesCustom).  When the labels name
+    a tumor core (tc, or nec and en) the split also returns it ("tc", counted as white matter, as the
+    reference does) and, with an edema label, the edema ("ed"): the target data of load_data."""
     if isinstance(seg, (str, bytes)):
         seg = read_netcdf(seg)
-    maps = split_segmentation(seg, labels)
+    with_tumor = labels.get("tc", -1) > 0 or (labels.get("nec", -1) > 0 and labels.get("en", -1) > 0)
+    maps = split_segmentation(seg, labels, with_tumor=with_tumor)
+    if with_tumor and labels.get("ed", -1) <= 0:
+        del maps["ed"]
     for k in ("wm", "gm", "csf", "vt"):
         solver.set_field(member, k, maps[k])
     sigma = smoothing_factor * 2 * np.pi / solver.n[0]
@@ -88,3 +108,34 @@ def load_atlas(solver, member, seg, labels, smoothing_factor=1.0):
             solver.smooth_field(member, k, sigma)
     solver.refresh_tissue(member)
     return maps
+
+
+def load_data(solver, member, cfg, maps=None):
+    """SolverInterface::readData (src/SolverInterface.cpp:577-640) for one member: the data d1 from
+    cfg["d1_path"], else the tumor core of the segmentation load_atlas split (maps["tc"]) and, when
+    that segmentation has edema and cfg["obs_lambda"] >= 0, the observation mask
+    OBS = 1[TC] + lambda 1[B/WT] (createEdemaBasedObservationMask); d1 smoothed on the GPU with sigma =
+    smoothing_factor_data * 2 pi / n0; obs_threshold_1 taken relative to max d1 when
+    obs_threshold_rel; a mask file (obs_mask_path) replaces the mask.  cfg is parse_config's dict (or
+    any dict with those keys).  Sets the member's data; returns (d1, mask or None, threshold)."""
+    mask = None
+    if cfg.get("d1_path"):
+        d1 = read_netcdf(cfg["d1_path"]).astype(np.float32)
+    elif maps is not None and "tc" in maps:
+        d1 = np.asarray(maps["tc"], dtype=np.float32)
+        if "ed" in maps and cfg.get("obs_lambda", -1.0) >= 0:
+            mask = edema_obs_mask(maps["tc"], maps["ed"], cfg["obs_lambda"])
+    else:
+        raise ValueError("no target data: neither d1_path nor a tumor label in the segmentation")
+    sfd = float(cfg.get("smoothing_factor_data", 1.0))
+    if sfd > 0:
+        solver.set_field(member, "d1", d1)
+        solver.smooth_field(member, "d1", sfd * 2 * np.pi / solver.n[0])
+        d1 = solver.get_field(member, "d1")
+    thr = float(cfg.get("obs_threshold_1", 0.0))
+    if cfg.get("obs_threshold_rel"):
+        thr *= float(np.max(d1))
+    if cfg.get("obs_mask_path"):
+        mask = read_netcdf(cfg["obs_mask_path"]).astype(np.float32)
+    solver.set_data(member, d1, mask, thr)
+    return d1, mask, thr

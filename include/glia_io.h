@@ -55,11 +55,21 @@ typedef struct glia_rd_file_config {
   int atlas_labels[8];      /* atlas_labels [wm, gm, vt, csf, tc, nec, en, ed]; -1 absent */
   char a_seg_path[512], a_wm_path[512], a_gm_path[512], a_csf_path[512], a_vt_path[512];
   char d1_path[512], obs_mask_path[512];
+  double obs_lambda;            /* obs_lambda (< 0: no observation mask from the segmentation) */
+  double smoothing_factor_data; /* smoothing_factor_data (d1 smoothed with factor * 2 pi / n) */
+  int obs_threshold_rel;        /* obs_threshold_rel (obs_threshold_1 relative to max d1) */
+  int patient_labels[8];        /* patient_labels, as atlas_labels; -1 absent */
+  char p_seg_path[512];         /* p_seg_path */
 } glia_rd_file_config;
 
 /* Defaults of the reference parameter structs for the fields above, then the file's values.
    Unknown keys are ignored; a missing file is GLIA_RD_ERR_ARG. */
 int glia_rd_parse_config(const char* path, glia_rd_file_config* out);
+
+/* createEdemaBasedObservationMask (src/utils/Utils.cpp:567-590): the observation operator
+   OBS = 1[TC] + lambda 1[B/WT] from the tumor-core and edema maps of a segmentation:
+   mask = 1 where tc > 0.99, else 0 where ed > 0.99, else lambda. */
+int glia_edema_obs_mask(const float* tc, const float* ed, int64_t count, double lambda, float* mask);
 
 #ifdef __cplusplus
 }

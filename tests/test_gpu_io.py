@@ -50,3 +50,59 @@ def test_atlas_tissue_and_forward_vs_oracle(oracle_mod, tmp_path):
     print("atlas forward rel err", err, flush=True)
     assert err < 1e-5
     s.close()
+
+
+def test_patient_data_from_segmentation_vs_oracle(oracle_mod):
+    """SolverInterface::readData with the target taken from the segmentation (src/SolverInterface.cpp:
+    577-640): d1 = tumor core (necrotic and enhancing labels), the edema-based observation mask
+    (obs_lambda >= 0), d1 smoothed with sigma = smoothing_factor_data 2 pi / n, the threshold
+    relative to max d1; the member's objective and gradient then match the oracle fed the same d1,
+    mask and threshold."""
+    from glia_amd import io, rd, synth
+    print("case patient data", flush=True)
+    n, nt, dt = 32, 2, 0.02
+    lab = dict(wm=6, gm=5, vt=7, csf=8, ed=2, nec=1, en=4)
+    x = np.arange(n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    r = np.sqrt((X - 16.0) ** 2 + (Y - 14.0) ** 2 + (Z - 15.0) ** 2)
+    seg = np.zeros((n, n, n), np.float32)
+    seg[r < 13] = 8
+    seg[r < 12] = 5
+    seg[r < 9] = 6
+    seg[(np.abs(X - 16) < 2) & (np.abs(Y - 14) < 2) & (r < 11)] = 7
+    r2 = np.sqrt((X - 12.0) ** 2 + (Y - 18.0) ** 2 + (Z - 15.0) ** 2)
+    seg[r2 < 4.5] = 2
+    seg[r2 < 3.0] = 4
+    seg[r2 < 1.5] = 1
+    cfg = dict(obs_lambda=0.3, smoothing_factor_data=1.0, obs_threshold_1=0.2, obs_threshold_rel=1)
+    s = rd.RDSolver(n, dt=dt, nt=nt, k_gm_wm=0.2, r_gm_wm=0.2)
+    maps = io.load_atlas(s, 0, seg, lab, smoothing_factor=1.0)
+    assert "tc" in maps and "ed" in maps
+    d1, mask, thr = io.load_data(s, 0, cfg, maps)
+    o = oracle_mod.Oracle(n)
+    sigma = 2 * np.pi / n
+    d1o = o.smooth(maps["tc"].astype(np.float64), sigma)
+    assert rel(d1, d1o) < 1e-6
+    assert abs(thr - 0.2 * d1o.max()) < 1e-6 * thr
+    tc, ed = maps["tc"], maps["ed"]
+    assert np.array_equal(mask, np.where(tc > 0.99, 1.0, np.where(ed > 0.99, 0.0, 0.3)).astype(np.float32))
+    assert np.array_equal(s.get_field(0, "obs"), mask)
+    sm = {k: o.smooth(maps[k].astype(np.float64), sigma) for k in ("wm", "gm", "csf", "vt")}
+    centers = synth.basis_centers(sm["wm"], 6, 3)
+    p = np.random.default_rng(2).uniform(0.2, 1.0, 6)
+    s.set_basis(0, centers, sigma)
+    xv = np.concatenate([p, [0.05, 8.0]])
+    J, g = s.objective_and_gradient([xv])
+    s.close()
+    o.set_tissue(sm["wm"], sm["gm"], sm["csf"], sm["vt"])
+    o.set_time(dt, nt)
+    o.set_k(0.05, 0.2, 0.0)
+    o.set_rho(8.0, 0.2, 0.0)
+    o.prec_factor()
+    o.set_phi(centers, sigma, 1.0, False)
+    o.set_opt(beta=0.0)
+    o.set_data(d1.astype(np.float64), mask.astype(np.float64), thr)
+    Jo, go = o.objective_and_gradient(xv, mode=2)
+    print("J", J[0], Jo, "dJ rel", rel(g[0], go), flush=True)
+    assert abs(J[0] - Jo) / abs(Jo) < 1e-5
+    assert rel(g[0], go) < 1e-4

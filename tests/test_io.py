@@ -114,8 +114,32 @@ def test_parse_config(tmp_path):
     assert c["atlas_labels"] == [6, 5, 7, 8, -1, -1, -1, -1]
     assert c["a_seg_path"] == "./testdata/atlas.nc"
     assert (c["r_gm_wm"], c["smoothing_factor"], c["sigma_spacing"]) == (0.0, 1.0, 1.5)  # defaults
+    assert (c["obs_lambda"], c["smoothing_factor_data"], c["obs_threshold_rel"]) == (-1.0, 1.0, 0)  # defaults
+    assert c["patient_labels"] == [-1] * 8 and c["p_seg_path"] == ""
+    p.write_text("obs_lambda=0.3\nsmoothing_factor_data=0\nobs_threshold_rel=1\np_seg_path=p.nc\n"
+                 "patient_labels=[wm=6,gm=5,vt=7,csf=8,ed=2,nec=1,en=4]\n")
+    c = io.parse_config(str(p))
+    assert (c["obs_lambda"], c["smoothing_factor_data"], c["obs_threshold_rel"], c["p_seg_path"]) == (0.3, 0.0, 1, "p.nc")
+    assert c["patient_labels"] == [6, 5, 7, 8, -1, 1, 4, 2] and c["atlas_labels"] == [-1] * 8
     with pytest.raises(OSError):
         io.parse_config(str(tmp_path / "missing.txt"))
+
+
+def test_edema_observation_mask():
+    """createEdemaBasedObservationMask (src/utils/Utils.cpp:567-590) on the maps of a BraTS-style
+    label image: 1 on the tumor core (necrotic and enhancing labels), 0 on edema, lambda elsewhere
+    (healthy tissue and background); the 0.99 cut holds for values just below it."""
+    from glia_amd import io
+    lab = dict(wm=6, gm=5, vt=7, csf=8, ed=2, nec=1, en=4)
+    seg = np.array([0, 6, 5, 7, 8, 1, 4, 2], dtype=np.float32)
+    m = io.split_segmentation(seg, lab, with_tumor=True)
+    mask = io.edema_obs_mask(m["tc"], m["ed"], 0.25)
+    assert mask.tolist() == [0.25, 0.25, 0.25, 0.25, 0.25, 1.0, 1.0, 0.0]
+    tc = np.array([0.995, 0.98, 0.0, 0.0], np.float32)
+    ed = np.array([1.0, 1.0, 0.995, 0.98], np.float32)
+    assert io.edema_obs_mask(tc, ed, 0.5).tolist() == [1.0, 0.0, 0.0, 0.5]
+    with pytest.raises(ValueError):
+        io.edema_obs_mask(tc, ed[:3], 0.5)
 
 
 def test_smoother_is_separable_truncated_convolution(oracle_mod):
