@@ -1761,22 +1761,50 @@ __global__ void k_brick_src(const BrickJob* __restrict__ jobs, Dims d, const Mem
 // weierstrassSmoother, src/grad/SpectralOperators.cpp:264-381, whose 8-image Gaussian kernel of
 // src/cuda/SpectralOperators.cu:12-51 is a product of per-axis kernels): out(x) = sum_s taps[s]
 // in(x - s e_axis), indices periodic.  Taps beyond 12 sigma (< e^-72) are dropped.
-__global__ void k_smooth_axis(const float* __restrict__ in, float* __restrict__ out, Dims d, int axis,
-                              const double* __restrict__ taps, int kr) {
-  const int n[3] = {d.n0, d.n1, d.n2};
-  const int64_t st[3] = {(int64_t)d.n1 * d.n2, (int64_t)d.n2, 1};
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.ng; i += (int64_t)gridDim.x * blockDim.x) {
-    const int c[3] = {(int)(i / st[0]), (int)((i / d.n2) % d.n1), (int)(i % d.n2)};
-    const int64_t base = i - (int64_t)c[axis] * st[axis];
-    double acc = 0.0;
-    for (int s = -kr; s <= kr; s++) {
-      int q = c[axis] - s;
-      q += (q < 0) ? n[axis] : 0;
-      q -= (q >= n[axis]) ? n[axis] : 0;
-      acc += taps[s + kr] * (double)in[base + (int64_t)q * st[axis]];
-    }
-    out[i] = (float)acc;
+__global__ void __launch_bounds__(256) k_smooth_axis(const float* __restrict__ in, float* __restrict__ out, Dims d,
+                                                     int axis, const double* __restrict__ taps, int kr) {
+  // block: 64 points along z times 4 rows along y of one x-plane (blockIdx.z); 32-bit offsets
+  const int z = (int)(blockIdx.x * 64 + threadIdx.x), y = (int)(blockIdx.y * 4 + threadIdx.y), x = (int)blockIdx.z;
+  if (z >= d.n2 || y >= d.n1) return;
+  const int n = axis == 0 ? d.n0 : (axis == 1 ? d.n1 : d.n2);
+  const unsigned st = axis == 0 ? (unsigned)(d.n1 * d.n2) : (axis == 1 ? (unsigned)d.n2 : 1u);
+  const int ca = axis == 0 ? x : (axis == 1 ? y : z);
+  const unsigned i = ((unsigned)x * (unsigned)d.n1 + (unsigned)y) * (unsigned)d.n2 + (unsigned)z;
+  const unsigned base = i - (unsigned)ca * st;
+  double acc = 0.0;
+  for (int s = -kr; s <= kr; s++) {
+    int q = ca - s;
+    q += (q < 0) ? n : 0;
+    q -= (q >= n) ? n : 0;
+    acc += taps[s + kr] * (double)in[base + (unsigned)q * st];
   }
+  out[i] = (float)acc;
+}
+
+// the same with the tap radius known at compile time: every load of a point is issued before the
+// sum, which runs in the same order as above (the same results bit for bit)
+template <int KR>
+__global__ void __launch_bounds__(256) k_smooth_axis_r(const float* __restrict__ in, float* __restrict__ out, Dims d,
+                                                       int axis, const double* __restrict__ taps) {
+  const int z = (int)(blockIdx.x * 64 + threadIdx.x), y = (int)(blockIdx.y * 4 + threadIdx.y), x = (int)blockIdx.z;
+  if (z >= d.n2 || y >= d.n1) return;
+  const int n = axis == 0 ? d.n0 : (axis == 1 ? d.n1 : d.n2);
+  const unsigned st = axis == 0 ? (unsigned)(d.n1 * d.n2) : (axis == 1 ? (unsigned)d.n2 : 1u);
+  const int ca = axis == 0 ? x : (axis == 1 ? y : z);
+  const unsigned i = ((unsigned)x * (unsigned)d.n1 + (unsigned)y) * (unsigned)d.n2 + (unsigned)z;
+  const unsigned base = i - (unsigned)ca * st;
+  float v[2 * KR + 1];
+#pragma unroll
+  for (int s = -KR; s <= KR; s++) {
+    int q = ca - s;
+    q += (q < 0) ? n : 0;
+    q -= (q >= n) ? n : 0;
+    v[s + KR] = in[base + (unsigned)q * st];
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int s = -KR; s <= KR; s++) acc += taps[s + KR] * (double)v[s + KR];
+  out[i] = (float)acc;
 }
 
 // one axis of the separable smoothing: the kept extent shrinks by 2*kr[axis] along 'axis'; the
@@ -2377,7 +2405,15 @@ struct RunZGradq {
 int launch_z_gradq(const LaunchCtx& c, int4 sc, int4 sa) { return dispatch_M<RunZGradq>(c.d.m2, c, sc, sa); }
 
 int launch_smooth_axis(const LaunchCtx& c, const float* in, float* out, int axis, const double* taps, int kr) {
-  k_smooth_axis<<<1024, 256, 0, c.stream>>>(in, out, c.d, axis, taps, kr);
+  const dim3 g((unsigned)((c.d.n2 + 63) / 64), (unsigned)((c.d.n1 + 3) / 4), (unsigned)c.d.n0), b(64, 4);
+#define GLIA_SMK(K) \
+  case K: k_smooth_axis_r<K><<<g, b, 0, c.stream>>>(in, out, c.d, axis, taps); break;
+  switch (kr) {
+    GLIA_SMK(1) GLIA_SMK(2) GLIA_SMK(3) GLIA_SMK(4) GLIA_SMK(5) GLIA_SMK(6) GLIA_SMK(7) GLIA_SMK(8)
+    GLIA_SMK(9) GLIA_SMK(10) GLIA_SMK(11) GLIA_SMK(12) GLIA_SMK(13) GLIA_SMK(14) GLIA_SMK(15) GLIA_SMK(16)
+    default: k_smooth_axis<<<g, b, 0, c.stream>>>(in, out, c.d, axis, taps, kr); break;
+  }
+#undef GLIA_SMK
   GLIA_CHECK_LAUNCH();
   return 0;
 }
