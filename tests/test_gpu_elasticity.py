@@ -78,3 +78,23 @@ def test_tumor_force_matches_the_oracle(case):
         fo = o.el_force(c, 1e5, use_tanh)
         print("force tanh", use_tanh, "rel", rel(f, fo), flush=True)
         assert rel(f, fo) < 1e-5
+
+
+def test_cg_state_resets_between_solves(case):
+    """The CG's stopping tests run on the device with iterations enqueued ahead; a solve that stops
+    before its first iteration (zero force: the preconditioned residual is 0, KSP_CONVERGED_ATOL with
+    0 iterations and u = 0, as KSPSolve_CG leaves it) and the solves around it leave no state behind:
+    the same solve before and after gives the same iterations, reason and solution bit for bit, and an
+    operator application after a solve is unaffected."""
+    n, o, s, fields, (mu, lam, scr, avg) = case
+    f = tumor_force(fields[0]).astype(np.float32).astype(np.float64)
+    u1, its1, why1, rn1 = s.solve(f)
+    u0, its0, why0, rn0 = s.solve(np.zeros_like(f))
+    print("zero force:", its0, why0, rn0, flush=True)
+    assert its0 == 0 and why0 == "atol" and rn0 == 0.0 and not np.any(u0)
+    u2, its2, why2, rn2 = s.solve(f)
+    assert (its2, why2) == (its1, why1) and rn2 == rn1
+    assert np.array_equal(u1, u2)
+    w = s.apply(u1)
+    wo = o.el_matvec(mu, lam, scr, u1.astype(np.float64))
+    assert rel(w, wo) < 1e-5
