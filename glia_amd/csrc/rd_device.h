@@ -76,7 +76,6 @@ struct PcgState {
   double m1;       // ||O c(T) - d1||^2
   double c0sq;     // ||c0||^2
   double gk, gr;   // trapezoid sums of <m_k, grad c . grad alpha> and <m_rho, (c^2 - c) alpha>
-  double red[4];   // scratch results of generic reductions
   // accumulators and counters at the start of a captured time step, restored when the step runs
   // again (one of its solves needed more iterations than the capture allowed)
   double gk_snap, gr_snap;

@@ -1602,16 +1602,6 @@ __global__ void __launch_bounds__(256) k_copy(const MemberDev* __restrict__ mem,
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) O[i] = I[i];
 }
 
-__global__ void __launch_bounds__(256) k_copy_real(const MemberDev* __restrict__ mem, Dims d, int r_in, int r_out, int ch_in,
-                                                    int ch_out) {
-  const MemberDev& M = mem[blockIdx.y];
-  const float4* I = reinterpret_cast<const float4*>(ch_in >= 0 ? M.chalf[ch_in] : M.r[r_in]);
-  float4* O = reinterpret_cast<float4*>(ch_out >= 0 ? M.chalf[ch_out] : M.r[r_out]);
-  const int64_t n4 = d.ng / 4;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) O[i] = I[i];
-}
-
 // MatProp::setValuesCustom (src/mat/MatProp.cpp:134-203): clip, mask rule; partial sum(filter)
 __global__ void __launch_bounds__(256) k_tissue(const MemberDev* __restrict__ mem, Dims d) {
   const MemberDev& M = mem[blockIdx.y];
@@ -1668,15 +1658,6 @@ __global__ void __launch_bounds__(256) k_obs_default(const MemberDev* __restrict
   float* ob = M.r[RS_OBS];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < d.ng; i += stride) ob[i] = (d1[i] > thr) ? 1.f : 0.f;
-}
-
-// generic finalize: red[q] = sum of partial column q (one block per member)
-__global__ void k_fin_sum(const MemberDev* __restrict__ mem, int nb, int nq) {
-  const MemberDev& M = mem[blockIdx.x];
-  for (int q = 0; q < nq; q++) {
-    const double v = sum_partials(M.part, nb, nq, q);
-    if (threadIdx.x == 0) M.st->red[q] = v;
-  }
 }
 
 // finalize kinds that fold a reduction into the state
@@ -2455,11 +2436,6 @@ int launch_copy(const LaunchCtx& c, int s_in, int s_out) {
   GLIA_CHECK_LAUNCH();
   return 0;
 }
-int launch_copy_real(const LaunchCtx& c, int r_in, int r_out, int ch_in, int ch_out) {
-  k_copy_real<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d, r_in, r_out, ch_in, ch_out);
-  GLIA_CHECK_LAUNCH();
-  return 0;
-}
 int launch_tissue(const LaunchCtx& c) {
   k_tissue<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d);
   GLIA_CHECK_LAUNCH();
@@ -2472,11 +2448,6 @@ int launch_coef(const LaunchCtx& c, int r_out, const double* coefs_dev) {
 }
 int launch_obs_default(const LaunchCtx& c, float thr) {
   k_obs_default<<<dim3(EW_BLOCKS, c.nm), 256, 0, c.stream>>>(c.mem, c.d, thr);
-  GLIA_CHECK_LAUNCH();
-  return 0;
-}
-int launch_fin_sum(const LaunchCtx& c, int nb, int nq) {
-  k_fin_sum<<<c.nm, 256, 0, c.stream>>>(c.mem, nb, nq);
   GLIA_CHECK_LAUNCH();
   return 0;
 }

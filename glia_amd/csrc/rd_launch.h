@@ -61,11 +61,9 @@ int launch_pcg_fin_dpi(const LaunchCtx& c, int nb);
 int launch_pcg_fin_update(const LaunchCtx& c, int nb, int* hflag, int hslot);
 
 int launch_copy(const LaunchCtx& c, int s_in, int s_out);  // slots >= CS_NUM address the state pool
-int launch_copy_real(const LaunchCtx& c, int r_in, int r_out, int ch_in, int ch_out);
 int launch_tissue(const LaunchCtx& c);
 int launch_coef(const LaunchCtx& c, int r_out, const double* coefs_dev);
 int launch_obs_default(const LaunchCtx& c, float thr);
-int launch_fin_sum(const LaunchCtx& c, int nb, int nq);
 int launch_fin_state(const LaunchCtx& c, int nb, int kind, double w);
 int launch_prec_factor(const LaunchCtx& c, double prec_dt);
 int launch_state_reset(const LaunchCtx& c, int what);
