@@ -304,8 +304,8 @@ def batch_config(args):
     tissue, kappa, rho and 8-sparse p over 64 bases, nt = 50 at dt = 0.02; the three grids are three
     handles evaluated in lockstep from one host thread (time step i of the three handles is one
     graph launch).  One step = one evaluation of J and dJ for all 32 members.  The same handles are
-    then timed evaluated concurrently by three host threads (one stream each), and the lockstep
-    results are checked bit for bit against the threaded ones."""
+    then timed evaluated concurrently by three host threads (one stream each) and one handle after
+    another from one thread, and the lockstep results are checked bit for bit against both."""
     import torch
     from glia_amd import rd
     groups = [(128, 16, 300), (192, 8, 400), (256, 8, 500)]
@@ -334,10 +334,23 @@ def batch_config(args):
         log("%s: %.3f s per evaluation of the 32 members" % (label, dt_ / args.steps))
         return dt_, res
 
+    def one_after_another():
+        Js, gs = [], []
+        for s, x in zip(solvers, xs):
+            J, g = s.objective_and_gradient(x)
+            Js.append(J)
+            gs.append(g)
+        return Js, gs, 0
+
     t_lock, (Jl, gl, spanned) = timed(lambda: rd.objective_and_gradient_lockstep(solvers, xs), "lockstep")
     t_thr, (Jt, gt, _) = timed(lambda: rd.objective_and_gradient_group(solvers, xs), "threads")
-    same = all(np.array_equal(a, b) for a, b in zip(Jl, Jt)) and all(
-        np.array_equal(u, v) for ga, gb in zip(gl, gt) for u, v in zip(ga, gb))
+    t_seq, (Jq, gq, _) = timed(one_after_another, "one handle after another")
+
+    def same_as(Ja, ga):
+        return all(np.array_equal(a, b) for a, b in zip(Jl, Ja)) and all(
+            np.array_equal(u, v) for gx, gy in zip(gl, ga) for u, v in zip(gx, gy))
+    same = same_as(Jt, gt)
+    same_seq = same_as(Jq, gq)
     value = total * args.steps / t_lock
     out = {
         "metric": METRIC,
@@ -358,7 +371,9 @@ def batch_config(args):
                    "groups": names, "nt": args.nt, "dt": args.dt},
         "steps_spanning_handles_last_eval": spanned,
         "threaded_value": round(total * args.steps / t_thr, 4),
+        "sequential_value": round(total * args.steps / t_seq, 4),
         "lockstep_equals_threaded_bitwise": bool(same),
+        "lockstep_equals_sequential_bitwise": bool(same_seq),
     }
     for s in solvers:
         s.close()
