@@ -15,6 +15,18 @@
  * Layout (reference src/grad/SpectralOperators.cpp:3-17): a real field is
  * n0 x n1 x n2, row-major with index x*n1*n2 + y*n2 + z; a spectrum is
  * n0 x n1 x (n2/2+1) interleaved complex, unnormalized forward transform.
+ *
+ * How it is pinned (tests/test_oracle.py, test_adv.py, test_elasticity.py,
+ * test_mass_effect.py, test_multispecies.py, test_me_features.py,
+ * test_sinusoid_simulator.py): the reference's own known answers
+ * (src/test/pdesolver.cpp:7-108, src/test/grad.cpp:7-80 and the sinusoid
+ * simulator src/test/simulator.cpp:5-58, which pins the model-4 loop), closed
+ * forms, exact discrete adjoints against finite differences, invariants and
+ * library cross-checks.  Parity unpinned: the model-5 loop (orc_ms_solve_state)
+ * as a whole, for which the reference holds no known answer -- its parts and
+ * its conservation and symmetry properties are checked; exact PETSc iteration
+ * counts on brain-like k beyond the known-answer test; AccFFT's Nyquist
+ * handling on the reference CPU path (the CUDA path's kernels are followed).
  */
 #include <complex.h>
 #include <math.h>
