@@ -106,3 +106,25 @@ def test_patient_data_from_segmentation_vs_oracle(oracle_mod):
     print("J", J[0], Jo, "dJ rel", rel(g[0], go), flush=True)
     assert abs(J[0] - Jo) / abs(Jo) < 1e-5
     assert rel(g[0], go) < 1e-4
+
+
+@pytest.mark.parametrize("factor", [0.5, 1.0, 2.0])
+def test_smoother_radii_vs_oracle(oracle_mod, factor):
+    """glia_rd_smooth_field at sigma = factor * 2 pi / n: tap radius 6 and 12 (compile-time radius)
+    and 24 (the general loop) against the oracle's FFT convolution of the reference's kernel."""
+    from glia_amd import rd
+    print("case smoother factor", factor, flush=True)
+    n = 64
+    rng = np.random.default_rng(11)
+    f = rng.uniform(0.0, 1.0, (n, n, n))
+    s = rd.RDSolver(n)
+    try:
+        s.set_field(0, "work", f)
+        s.smooth_field(0, "work", factor * 2 * np.pi / n)
+        got = s.get_field(0, "work")
+    finally:
+        s.close()
+    ref = oracle_mod.Oracle(n).smooth(f.astype(np.float32).astype(np.float64), factor * 2 * np.pi / n)
+    err = rel(got, ref)
+    print("rel", err, flush=True)
+    assert err < 1e-6
