@@ -100,11 +100,12 @@ int RDEnsemble::init() {
   // captured steps unroll each solve's CG loop to the last count of its kind plus this margin;
   // GLIA_STEP_MARGIN < 0 makes every captured step run again eagerly (a check of that path)
   if (const char* v = std::getenv("GLIA_STEP_MARGIN")) margin_ = std::atoi(v);
-  // forward divergence variants: the X pass in two thread groups (measured faster at 256 points),
-  // the Y pass in one.  GLIA_SPLIT_DIV overrides with a comma-separated list of none, xf (X pass
-  // in two groups), xf3 (X pass in three groups), yf (Y pass in three groups), yfb / yfb2 (Y pass
-  // split over blocks at 3 / 2 waves per SIMD), both (xf and yf); glia_rd_autotune measures them.
-  xf_split_ = 1;
+  // forward divergence variants: the X pass in two thread groups on power-of-two lengths (measured
+  // faster at 128 and 256 points) and in one on the others (faster at 160 and 192), the Y pass in
+  // one.  GLIA_SPLIT_DIV overrides with a comma-separated list of none, xf (X pass in two groups),
+  // xf3 (X pass in three groups), yf (Y pass in three groups), yfb / yfb2 (Y pass split over blocks
+  // at 3 / 2 waves per SIMD), both (xf and yf); glia_rd_autotune measures them.
+  xf_split_ = (d_.n0 & (d_.n0 - 1)) == 0 ? 1 : 0;
   yf_split_ = 0;
   if (const char* v = std::getenv("GLIA_SPLIT_DIV")) {
     xf_split_ = 0;

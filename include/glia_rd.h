@@ -288,8 +288,10 @@ int glia_rd_use_graphs(glia_rd_t h, int enable);
 /* Autotuning of the kernel variants (no reference counterpart: the reference leaves plans to
    cuFFT): the variants of each forward divergence pass are timed on this handle's grid and
    members and the fastest is kept.  X pass: 0 = one thread group holding both fields, 1 = one
-   group per field, 2 = one group per field and one for the output.  Y pass: 0 = one group holding three fields, 1 = one group per field, 2 and 3 =
-   the fields split over blocks at 3 and 2 waves per SIMD.  cache_path (may be NULL) names a text
+   group per field, 2 = one group per field and one for the output.  Y pass: 0 = one group
+   holding three fields, 1 = one group per field, 2 and 3 = the fields split over blocks at 3 and
+   2 waves per SIMD.  Untuned handles take X pass 1 on power-of-two n0, else 0, and Y pass 0.
+   cache_path (may be NULL) names a text
    file of earlier choices, read first and appended to.  choice (2 ints, may be NULL) receives
    (X-pass variant, Y-pass variant).  Call before the first solve; the results do not change
    beyond FP32 rounding. */
