@@ -67,7 +67,6 @@ struct glia_el_s {
   std::unique_ptr<glia::RDEnsemble> e;
   int64_t ng = 0;
   hipStream_t st = nullptr;
-  float *mu = nullptr, *lam = nullptr, *scr = nullptr;
   float* v[kVecs][3] = {};   // lane d's real slot of each CG vector (the engine's memory)
   double* part = nullptr;    // block partials, then (at part + 3 np) the summed scalars
   std::vector<double> hpart;
@@ -78,8 +77,6 @@ struct glia_el_s {
   bool have_material = false;
   std::string err;
   ~glia_el_s() {
-    for (float* p : {mu, lam, scr})
-      if (p) (void)hipFree(p);
     if (part) (void)hipFree(part);
     if (hflag_host) (void)hipHostFree(hflag_host);
     for (hipEvent_t e : ev)
@@ -277,14 +274,6 @@ int glia_el_create(const int n[3], void* stream, glia_el_t* out) {
   h->st = h->e->stream();
   for (int k = 0; k < kVecs; k++)
     for (int d = 0; d < 3; d++) h->v[k][d] = lane_slot(h, d, kVecSlot[k]);
-  const size_t b = sizeof(float) * h->ng;
-  for (float** p : {&h->mu, &h->lam, &h->scr}) {
-    const hipError_t e = hipMalloc((void**)p, b);
-    if (e != hipSuccess) {
-      delete h;
-      return (int)e;
-    }
-  }
   const int np = glia::el::partial_count();
   if (hipMalloc((void**)&h->part, sizeof(double) * (3 * np + 8)) != hipSuccess) {
     delete h;
@@ -335,14 +324,23 @@ int glia_el_set_material(glia_el_t h, const float* c, const float* wm, const flo
   m.lam_c = lam(q.E_csf, q.nu_csf);
   m.screen_low = (float)q.screen_low;
   m.screen_high = (float)q.screen_high;
-  // the six inputs staged in the CG vectors (free outside a solve)
+  // device inputs are read where they are; host inputs are staged in the CG vectors (free outside
+  // a solve).  The material goes straight to the lanes' slots where the Z passes read it: mu and
+  // the screening in every lane, lam in lane 0.
   const float* src[6] = {c, wm, gm, csf, vt, bg};
-  float* dev[6] = {h->v[VX][0], h->v[VX][1], h->v[VX][2], h->v[VB][0], h->v[VB][1], h->v[VB][2]};
-  for (int k = 0; k < 6; k++)
-    EL_HIP(h, hipMemcpyAsync(dev[k], src[k], sizeof(float) * h->ng,
-                             mem_kind == GLIA_RD_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->st));
-  EL_HIP(h, glia::el::launch_material(h->st, h->ng, dev[0], dev[1], dev[2], dev[3], dev[4], dev[5], m, h->mu, h->lam,
-                                      h->scr, h->part));
+  const float* dev[6];
+  float* stage[6] = {h->v[VX][0], h->v[VX][1], h->v[VX][2], h->v[VB][0], h->v[VB][1], h->v[VB][2]};
+  for (int k = 0; k < 6; k++) {
+    if (mem_kind == GLIA_RD_DEVICE) {
+      dev[k] = src[k];
+    } else {
+      EL_HIP(h, hipMemcpyAsync(stage[k], src[k], sizeof(float) * h->ng, hipMemcpyHostToDevice, h->st));
+      dev[k] = stage[k];
+    }
+  }
+  EL_HIP(h, glia::el::launch_material(h->st, h->ng, dev[0], dev[1], dev[2], dev[3], dev[4], dev[5], m,
+                                      lane_field(h, RS_MU), lane_slot(h, 0, RS_LAM), lane_field(h, glia::RS_RHO),
+                                      h->part));
   const int np = glia::el::partial_count();
   EL_HIP(h, hipMemcpyAsync(h->hpart.data(), h->part, sizeof(double) * 3 * np, hipMemcpyDeviceToHost, h->st));
   EL_HIP(h, hipStreamSynchronize(h->st));
@@ -350,12 +348,6 @@ int glia_el_set_material(glia_el_t h, const float* c, const float* wm, const flo
   for (int i = 0; i < np; i++)
     for (int k = 0; k < 3; k++) s[k] += h->hpart[3 * i + k];
   for (int k = 0; k < 3; k++) h->avg[k] = s[k] / (double)h->ng;
-  const size_t fb = sizeof(float) * h->ng;
-  for (int d = 0; d < 3; d++) {
-    EL_HIP(h, hipMemcpyAsync(lane_slot(h, d, RS_MU), h->mu, fb, hipMemcpyDeviceToDevice, h->st));
-    EL_HIP(h, hipMemcpyAsync(lane_slot(h, d, glia::RS_RHO), h->scr, fb, hipMemcpyDeviceToDevice, h->st));
-  }
-  EL_HIP(h, hipMemcpyAsync(lane_slot(h, 0, RS_LAM), h->lam, fb, hipMemcpyDeviceToDevice, h->st));
   h->have_material = true;
   return 0;
 }
@@ -364,7 +356,7 @@ int glia_el_get_material(glia_el_t h, float* mu, float* lam, float* screen, doub
   if (!h) return GLIA_RD_ERR_ARG;
   if (!valid_kind(mem_kind)) return fail(h, GLIA_RD_ERR_ARG, "bad mem_kind");
   if (!h->have_material) return fail(h, GLIA_RD_ERR_STATE, "material not set");
-  const float* src[3] = {h->mu, h->lam, h->scr};
+  const float* src[3] = {lane_slot(h, 0, RS_MU), lane_slot(h, 0, RS_LAM), lane_slot(h, 0, glia::RS_RHO)};
   float* dst[3] = {mu, lam, screen};
   for (int k = 0; k < 3; k++)
     if (dst[k])

@@ -16,8 +16,8 @@ constexpr int EB = 1024;  // blocks of the grid-stride kernels (the partial-sum 
 __global__ void __launch_bounds__(256) k_el_material(int64_t ng, const float* __restrict__ c, const float* __restrict__ wm,
                                                      const float* __restrict__ gm, const float* __restrict__ csf,
                                                      const float* __restrict__ vt, const float* __restrict__ bg,
-                                                     ElMaterial m, float* __restrict__ mu, float* __restrict__ lam,
-                                                     float* __restrict__ scr, double* __restrict__ part) {
+                                                     ElMaterial m, Vec3 mu, float* __restrict__ lam, Vec3 scr,
+                                                     double* __restrict__ part) {
   double s[3] = {0.0, 0.0, 0.0};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += stride) {
@@ -32,9 +32,11 @@ __global__ void __launch_bounds__(256) k_el_material(int64_t ng, const float* __
     }
     float sc = (ci >= 0.005f) ? m.screen_low : m.screen_high;
     if (bg[i] > 0.95f) sc = 1e6f;
-    mu[i] = a;
+    for (int d = 0; d < 3; d++) {
+      mu.p[d][i] = a;
+      scr.p[d][i] = sc;
+    }
     lam[i] = b;
-    scr[i] = sc;
     s[0] += a;
     s[1] += b;
     s[2] += sc;
@@ -71,7 +73,7 @@ __global__ void __launch_bounds__(256) k_force_scale(int64_t ng, const float* __
 int partial_count() { return EB; }
 
 int launch_material(hipStream_t st, int64_t ng, const float* c, const float* wm, const float* gm, const float* csf,
-                    const float* vt, const float* bg, const ElMaterial& m, float* mu, float* lam, float* scr,
+                    const float* vt, const float* bg, const ElMaterial& m, const Vec3& mu, float* lam, const Vec3& scr,
                     double* part) {
   k_el_material<<<EB, 256, 0, st>>>(ng, c, wm, gm, csf, vt, bg, m, mu, lam, scr, part);
   return (int)hipGetLastError();

@@ -20,7 +20,7 @@ struct Vec3C {
 
 int partial_count();
 int launch_material(hipStream_t st, int64_t ng, const float* c, const float* wm, const float* gm, const float* csf,
-                    const float* vt, const float* bg, const ElMaterial& m, float* mu, float* lam, float* scr,
+                    const float* vt, const float* bg, const ElMaterial& m, const Vec3& mu, float* lam, const Vec3& scr,
                     double* part);
 int launch_force_scale(hipStream_t st, int64_t ng, const float* g, const float* c, float ff, int use_tanh, float* out);
 int launch_axpby3(hipStream_t st, int64_t ng, const Vec3& p, const Vec3C& z, double b, int sub);
