@@ -32,6 +32,8 @@ class DerivativeOperators {
   virtual PetscErrorCode evaluateGradient(Vec dJ, Vec x, std::shared_ptr<Data> data) = 0;
   virtual PetscErrorCode evaluateObjectiveAndGradient(PetscReal* J, Vec dJ, Vec x, std::shared_ptr<Data> data) = 0;
   virtual PetscErrorCode evaluateHessian(Vec y, Vec x) = 0;
+  /* evaluateConstantHessianApproximation (include/grad/DerivativeOperators.h:32): none by default */
+  virtual PetscErrorCode evaluateConstantHessianApproximation(Vec y, Vec x) { return (y && x) ? 0 : GLIA_RD_ERR_ARG; }
   virtual PetscErrorCode updateReactionAndDiffusion(Vec x) = 0;
   virtual PetscErrorCode gradDiffusion(Vec dJ) = 0;
   virtual PetscErrorCode gradReaction(Vec dJ) = 0;
@@ -100,6 +102,16 @@ class DerivativeOperatorsRD : public DerivativeOperators {
     ay.assign(ax.size(), 0.0);
     if ((rc = glia_rd_hessian(h_, params_->opt_->diffusivity_inversion_ ? 1 : 0, ax.data(), ay.data()))) return rc;
     return from_abi(ay, y);
+  }
+
+  /* evaluateConstantHessianApproximation (src/grad/DerivativeOperatorsRD.cpp:440-448): y = beta
+     Phi^T Phi x over the p block (x and y hold np entries; Phi x passes through c_0_) */
+  PetscErrorCode evaluateConstantHessianApproximation(Vec y, Vec x) override {
+    if (rc_) return rc_;
+    if (!y || !x) return GLIA_RD_ERR_ARG;
+    PetscErrorCode rc = tumor_->phi_->apply(tumor_->c_0_, x);
+    if (rc || (rc = tumor_->phi_->applyTranspose(y, tumor_->c_0_))) return rc;
+    return VecScale(y, params_->opt_->beta_);
   }
 
   /* gradDiffusion (src/grad/DerivativeOperators.cpp:189-256): dJ[np] = g_kappa over the history of

@@ -564,6 +564,32 @@ static void test_phi_and_apply_d() {
   VecNorm(d1, NORM_2, &n1);
   std::printf("  ||D(2c) - 2D(c)|| / ||D(c)|| = %.2e\n", e / n1);
   REQUIRE(n1 > 0 && e <= 1e-5 * n1);
+  // the constant Hessian approximation H = beta Phi^T Phi is symmetric and
+  // <u, H u> = beta ||Phi u||^2 (u = p from above, v a second coefficient vector)
+  Vec q, hp, hq;
+  VecDuplicate(p, &q);
+  VecDuplicate(p, &hp);
+  VecDuplicate(p, &hq);
+  VecGetArray(q, &pp);
+  pp[0] = -0.3, pp[1] = 1.5, pp[2] = 0.7, pp[3] = 0.1;
+  VecRestoreArray(q, &pp);
+  REQUIRE(s.tumor->phi_->apply(c0, p) == 0);
+  double phin = 0;
+  VecNorm(c0, NORM_2, &phin);
+  REQUIRE(s.der->evaluateConstantHessianApproximation(hp, p) == 0);
+  REQUIRE(s.der->evaluateConstantHessianApproximation(hq, q) == 0);
+  double qhp = 0, phq = 0, php = 0;
+  VecDot(q, hp, &qhp);
+  VecDot(p, hq, &phq);
+  VecDot(p, hp, &php);
+  const double beta = s.params->opt_->beta_;
+  std::printf("  <q, H p> = %.8e, <H q, p> = %.8e, <p, H p> = %.8e, beta ||Phi p||^2 = %.8e\n", qhp, phq, php,
+              beta * phin * phin);
+  REQUIRE(std::fabs(qhp - phq) <= 1e-5 * std::fabs(qhp));
+  REQUIRE(std::fabs(php - beta * phin * phin) <= 1e-5 * php);
+  VecDestroy(&q);
+  VecDestroy(&hp);
+  VecDestroy(&hq);
   VecDestroy(&p);
   VecDestroy(&pt);
   VecDestroy(&c0);
