@@ -79,6 +79,7 @@ SIGNATURES = {
     "glia_rd_get_beta": (_i, [_vp, _pd]),
     "glia_rd_set_ratios": (_i, [_vp, _d, _d, _d, _d]),
     "glia_rd_set_basis": (_i, [_vp, _i, _i, _vp, _d]),
+    "glia_rd_data_support": (_i, [_vp, _i, _i, _d, _d, _d, _i, _vp, ctypes.POINTER(_i)]),
     "glia_rd_basis_size": (_i, [_vp, _i]),
     "glia_rd_set_data": (_i, [_vp, _i, _vp, _vp, _d, _i]),
     "glia_rd_set_field": (_i, [_vp, _i, _i, _vp, _i]),
@@ -127,6 +128,8 @@ IO_SIGNATURES = {
     "glia_split_segmentation": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "glia_rd_parse_config": (_i, [_cp, _vp]),
     "glia_edema_obs_mask": (_i, [_vp, _vp, _i64, _d, _vp]),
+    "glia_phi_mesh": (_i, [_i, _vp, _d, _d, _vp]),
+    "glia_read_phi_mesh": (_i, [_cp, _i, _vp, _vp, ctypes.POINTER(_i), _pd, _pd]),
 }
 
 
@@ -199,6 +202,9 @@ class FileConfig(ctypes.Structure):
         ("obs_mask_path", ctypes.c_char * 512),
         ("obs_lambda", ctypes.c_double), ("smoothing_factor_data", ctypes.c_double), ("obs_threshold_rel", ctypes.c_int),
         ("patient_labels", ctypes.c_int * 8), ("p_seg_path", ctypes.c_char * 512),
+        ("threshold_data_driven", ctypes.c_double), ("gaussian_volume_fraction", ctypes.c_double),
+        ("sigma_data_driven", ctypes.c_double), ("support_data_path", ctypes.c_char * 512),
+        ("gaussian_cm_path", ctypes.c_char * 512),
     ]
 
 

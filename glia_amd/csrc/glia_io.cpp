@@ -10,6 +10,7 @@
 #include <cctype>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <string>
@@ -277,6 +278,84 @@ int glia_edema_obs_mask(const float* tc, const float* ed, int64_t count, double 
   return 0;
 }
 
+int glia_phi_mesh(int np, const double cm[3], double sigma, double spacing_factor, double* centers) {
+  if (np < 1 || !cm || !centers) return 0;
+  const double sp = spacing_factor * sigma;
+  if (np == 1) {
+    centers[0] = cm[0], centers[1] = cm[1], centers[2] = cm[2];
+    return 1;
+  }
+  const int h = (int)std::lround(std::pow((double)np, 1.0 / 3.0));
+  int k = 0;
+  auto put = [&](double a, double b, double c) {
+    if (k >= np) return;
+    centers[3 * k + 0] = a * sp + cm[0];
+    centers[3 * k + 1] = b * sp + cm[1];
+    centers[3 * k + 2] = c * sp + cm[2];
+    k++;
+  };
+  if (np % 2) {
+    const int e = (h - 1) / 2;
+    for (int z = -e; z <= e; z++)
+      for (int y = -e; y <= e; y++)
+        for (int x = -e; x <= e; x++) put(x, y, z);
+  } else {
+    // the octant cubes: index i != 0 sits at i - sign(i) / 2 spacings
+    const int e = h / 2;
+    auto off = [](int i) { return i - (i > 0 ? 0.5 : -0.5); };
+    for (int z = -e; z <= e; z++)
+      for (int y = -e; y <= e; y++)
+        for (int x = -e; x <= e; x++)
+          if (x && y && z) put(off(x), off(y), off(z));
+  }
+  return k;
+}
+
+int glia_read_phi_mesh(const char* path, int max_np, double* centers, int* comps, int* np, double* sigma,
+                       double* spacing) {
+  if (!path || !np || max_np < 0 || (max_np > 0 && !centers)) return -2;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return -1;
+  std::string line;
+  auto getline = [&](std::string& out) {
+    out.clear();
+    int ch;
+    while ((ch = std::fgetc(f)) != EOF && ch != '\n') out.push_back((char)ch);
+    return !(ch == EOF && out.empty());
+  };
+  double sg = 0, spc = 0;
+  if (getline(line)) {
+    // "sigma = s, spacing = d": the value after the first '=' up to the comma, then after the next '='
+    const size_t e1 = line.find('='), c1 = line.find(',');
+    if (e1 != std::string::npos && c1 != std::string::npos) sg = std::atof(line.c_str() + e1 + 1);
+    const size_t e2 = c1 == std::string::npos ? std::string::npos : line.find('=', c1 + 1);
+    if (e2 != std::string::npos) spc = std::atof(line.c_str() + e2 + 1);
+  }
+  getline(line);  // " centers = ["
+  int k = 0;
+  while (getline(line)) {
+    if (line.find(']') != std::string::npos) break;
+    size_t pos = 0;
+    if (k < max_np) centers[3 * k] = centers[3 * k + 1] = centers[3 * k + 2] = 0;
+    for (int t = 0;; t++) {
+      const size_t comma = line.find(',', pos);
+      const double v = std::atof(line.substr(pos, comma == std::string::npos ? std::string::npos : comma - pos).c_str());
+      if (k < max_np) {
+        if (t < 3) centers[3 * k + t] = v;
+        else if (t == 3 && comps) comps[k] = (int)v;
+      }
+      if (comma == std::string::npos) break;
+      pos = comma + 1;
+    }
+    k++;
+  }
+  std::fclose(f);
+  *np = k;
+  if (sigma) *sigma = sg;
+  if (spacing) *spacing = spc;
+  return k > max_np ? -2 : 0;
+}
+
 int glia_split_segmentation(const float* seg, int64_t count, const int* labels, float* wm, float* gm, float* vt,
                             float* csf, float* tc, float* ed) {
   if (!seg || !labels || !wm || !gm || !vt || count <= 0) return GLIA_RD_ERR_ARG;
@@ -327,9 +406,13 @@ int glia_rd_parse_config(const char* path, glia_rd_file_config* o) {
   for (int i = 0; i < 8; i++) o->atlas_labels[i] = o->patient_labels[i] = -1;
   o->obs_lambda = -1.0;
   o->smoothing_factor_data = 1.0;
+  o->threshold_data_driven = 0.05;
+  o->gaussian_volume_fraction = 0.0;
+  o->sigma_data_driven = 2.0 * M_PI / 256;
   std::ifstream f(path ? path : "");
   if (!f) return GLIA_RD_ERR_ARG;
   std::string line;
+  bool sigma_set = false;
   auto copy_path = [](char* dst, const std::string& v) {
     std::strncpy(dst, v.c_str(), 511);
     dst[511] = 0;
@@ -355,7 +438,7 @@ int glia_rd_parse_config(const char* path, glia_rd_file_config* o) {
       else if (k == "smoothing_factor") o->smoothing_factor = std::stod(v);
       else if (k == "obs_threshold_1") o->obs_threshold_1 = std::stod(v);
       else if (k == "number_gaussians") o->np = std::stoi(v);
-      else if (k == "sigma_factor") o->sigma_factor = std::stod(v);
+      else if (k == "sigma_factor") o->sigma_factor = std::stod(v), sigma_set = true;
       else if (k == "sigma_spacing") o->sigma_spacing = std::stod(v);
       else if (k == "store_phi") o->phi_store = std::stoi(v) > 0;
       else if (k == "store_adjoint") o->adjoint_store = std::stoi(v) > 0;
@@ -370,6 +453,10 @@ int glia_rd_parse_config(const char* path, glia_rd_file_config* o) {
       else if (k == "obs_lambda") o->obs_lambda = std::stod(v);
       else if (k == "smoothing_factor_data") o->smoothing_factor_data = std::stod(v);
       else if (k == "obs_threshold_rel") o->obs_threshold_rel = std::stoi(v) > 0;
+      else if (k == "threshold_data_driven") o->threshold_data_driven = std::stod(v);
+      else if (k == "gaussian_volume_fraction") o->gaussian_volume_fraction = std::stod(v);
+      else if (k == "support_data_path") copy_path(o->support_data_path, v);
+      else if (k == "gaussian_cm_path") copy_path(o->gaussian_cm_path, v);
       else if (k == "atlas_labels" || k == "patient_labels") {
         int* dst = (k == "atlas_labels") ? o->atlas_labels : o->patient_labels;
         // [wm=6,gm=5,vt=7,csf=8,...] (src/utils/IO.cpp:126-170)
@@ -393,6 +480,8 @@ int glia_rd_parse_config(const char* path, glia_rd_file_config* o) {
       return GLIA_RD_ERR_ARG;
     }
   }
+  // sigma_factor sets the data-driven sigma too (src/utils/IO.cpp:175-177), in units of 2 pi / n
+  if (sigma_set && o->n > 0) o->sigma_data_driven = o->sigma_factor * 2.0 * M_PI / o->n;
   return 0;
 }
 

@@ -766,6 +766,21 @@ int glia_rd_smooth_field(glia_rd_t h, int member, int which, double sigma) {
   return ret(h, h->e->smooth_field(member, slot, sigma));
 }
 
+int glia_rd_data_support(glia_rd_t h, int member, int which, double threshold, double sigma, double gvf, int max_np,
+                         double* centers, int* np) {
+  GLIA_H(h);
+  const int slot = field_slot(which);
+  if (slot < 0) return arg_err(h, "unknown field id");
+  if (!np || max_np < 0 || (max_np > 0 && !centers)) return arg_err(h, "null output pointer or negative max_np");
+  std::vector<double> c;
+  const int rc = h->e->data_support(member, slot, threshold, sigma, gvf, &c);
+  if (rc) return ret(h, rc);
+  *np = (int)(c.size() / 3);
+  if (*np > max_np) return arg_err(h, "more support centers than max_np (np holds the count)");
+  if (*np) std::memcpy(centers, c.data(), sizeof(double) * c.size());
+  return 0;
+}
+
 int glia_rd_refresh_tissue(glia_rd_t h, int member) {
   GLIA_H(h);
   return ret(h, h->e->refresh_tissue(member));

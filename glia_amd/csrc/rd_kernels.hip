@@ -1762,6 +1762,34 @@ __global__ void __launch_bounds__(256) k_smooth_axis(const float* __restrict__ i
   out[i] = (float)acc;
 }
 
+// data-driven basis support (Phi::setGaussians(Vec data), src/mat/Phi.cpp:457-522): one thread per
+// lattice candidate (step apart on every axis, l0 x l1 x l2 of them in index order) counts the
+// voxels with f > thr at integer offsets |d|^2 <= d2max, |d_a| <= R, clipped to the grid (no wrap)
+__global__ void __launch_bounds__(256) k_support_count(const float* __restrict__ f, Dims d, int step, int l0, int l1,
+                                                       int l2, int R, int d2max, float thr, int* __restrict__ cnt) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)l0 * l1 * l2) return;
+  const int c2 = (int)(t % l2);
+  const int64_t t1 = t / l2;
+  const int c1 = (int)(t1 % l1), c0 = (int)(t1 / l1);
+  const int x = c0 * step, y = c1 * step, z = c2 * step;
+  int num = 0;
+  for (int i = -R; i <= R; i++) {
+    const int xi = x + i;
+    if (xi < 0 || xi >= d.n0) continue;
+    for (int j = -R; j <= R; j++) {
+      const int yj = y + j, rem = d2max - i * i - j * j;
+      if (yj < 0 || yj >= d.n1 || rem < 0) continue;
+      const float* row = f + ((int64_t)xi * d.n1 + yj) * d.n2;
+      for (int k = -R; k <= R; k++) {
+        const int zk = z + k;
+        if (k * k <= rem && zk >= 0 && zk < d.n2) num += row[zk] > thr ? 1 : 0;
+      }
+    }
+  }
+  cnt[t] = num;
+}
+
 // the same with the tap radius known at compile time: every load of a point is issued before the
 // sum, which runs in the same order as above (the same results bit for bit)
 template <int KR>
@@ -2384,6 +2412,15 @@ struct RunZGradq {
   }
 };
 int launch_z_gradq(const LaunchCtx& c, int4 sc, int4 sa) { return dispatch_M<RunZGradq>(c.d.m2, c, sc, sa); }
+
+int launch_support_count(const LaunchCtx& c, const float* f, int step, int l0, int l1, int l2, int R, int d2max,
+                         float thr, int* cnt) {
+  const int64_t nl = (int64_t)l0 * l1 * l2;
+  if (nl <= 0 || nl > c.d.ng || step < 1 || R < 0) return -1;
+  k_support_count<<<(unsigned)((nl + 255) / 256), 256, 0, c.stream>>>(f, c.d, step, l0, l1, l2, R, d2max, thr, cnt);
+  GLIA_CHECK_LAUNCH();
+  return 0;
+}
 
 int launch_smooth_axis(const LaunchCtx& c, const float* in, float* out, int axis, const double* taps, int kr) {
   const dim3 g((unsigned)((c.d.n2 + 63) / 64), (unsigned)((c.d.n1 + 3) / 4), (unsigned)c.d.n0), b(64, 4);

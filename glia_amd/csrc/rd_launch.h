@@ -32,6 +32,9 @@ int xf_div_blocks(const Dims& d);
 int z_blocks(const Dims& d);
 int z_lines_per_block(int m2);
 int launch_smooth_axis(const LaunchCtx& c, const float* in, float* out, int axis, const double* taps, int kr);
+// per lattice candidate of the data-driven basis support: voxels above thr in its ball (k_support_count)
+int launch_support_count(const LaunchCtx& c, const float* f, int step, int l0, int l1, int l2, int R, int d2max,
+                         float thr, int* cnt);
 int pcg_update_blocks();
 int ew_blocks();
 

@@ -1870,6 +1870,60 @@ int RDEnsemble::smooth_ptr(float* f, float* tmp, double sigma) {
   return rc;
 }
 
+// Phi::setGaussians(Vec data) (src/mat/Phi.cpp:615-940) on one device.  The candidates are the
+// lattice the reference's loops walk (:686-720): per axis from 0 to the largest index that is an
+// exact multiple of space = 2 sigma / h0, in steps x = (int)(x + space).  Radius r = sigma / h0 on
+// every axis; the ball holds the integer offsets with sqrt(|d|^2) <= r.
+int RDEnsemble::data_support(int m, int rslot, double thr, double sigma, double gvf, std::vector<double>* centers) {
+  if (m < 0 || m >= nm_ || rslot < 0 || rslot >= RS_NUM || !centers) return fail(-2, "bad member, field or output");
+  const int n[3] = {d_.n0, d_.n1, d_.n2};
+  const double h[3] = {2.0 * M_PI / n[0], 2.0 * M_PI / n[1], 2.0 * M_PI / n[2]};
+  const double r = sigma / h[0], space = 2.0 * sigma / h[0];
+  if (!(space >= 1.0) || !(r < 64.0)) return fail(-2, "data support: sigma gives a lattice step below one cell or a radius above 64 cells");
+  const int R = (int)std::floor(r);
+  int d2max = -1, interior = 0;
+  for (int q = 0; q <= 3 * R * R; q++)
+    if (std::sqrt((double)q) <= r) d2max = q;
+  for (int i = -R; i <= R; i++)
+    for (int j = -R; j <= R; j++)
+      for (int k = -R; k <= R; k++)
+        if (i * i + j * j + k * k <= d2max) interior++;
+  // the step sequence x -> (int)(x + space) from 0 is the same on every axis; each axis ends at its
+  // largest exact multiple of space
+  int end[3], len[3];
+  for (int a = 0; a < 3; a++) {
+    end[a] = 0;
+    for (int x = n[a] - 1; x >= 0; x--)
+      if (std::fmod((double)x, space) == 0) { end[a] = x; break; }
+  }
+  const int step = (int)(0 + space);
+  for (int a = 0; a < 3; a++) {
+    len[a] = 0;
+    for (int x = 0; x <= end[a]; x = (int)(x + space), len[a]++)
+      if (x != len[a] * step) return fail(-2, "data support: uneven lattice");
+  }
+  const int64_t nl = (int64_t)len[0] * len[1] * len[2];
+  int* cnt = reinterpret_cast<int*>(mh_[0].c[CS_T7]);  // n0*n1*n2 ints; nl <= that
+  LaunchCtx c = ctx();
+  int rc = launch_support_count(c, mh_[m].r[rslot], step, len[0], len[1], len[2], R, d2max, (float)thr, cnt);
+  if (rc) return rc < 0 ? fail(-2, "data support: lattice does not fit") : rc;
+  std::vector<int> host((size_t)nl);
+  if ((rc = check(hipMemcpyAsync(host.data(), cnt, sizeof(int) * nl, hipMemcpyDeviceToHost, stream_), "support counts")))
+    return rc;
+  if ((rc = check(hipStreamSynchronize(stream_), "support sync"))) return rc;
+  centers->clear();
+  int64_t t = 0;
+  for (int a = 0; a < len[0]; a++)
+    for (int b = 0; b < len[1]; b++)
+      for (int cc = 0; cc < len[2]; cc++, t++)
+        if (host[(size_t)t] > gvf * interior) {
+          centers->push_back(a * step * h[0]);
+          centers->push_back(b * step * h[1]);
+          centers->push_back(cc * step * h[2]);
+        }
+  return 0;
+}
+
 // the tail of set_tissue: mask, k, rho, preconditioner and Phi from the stored tissue fields
 int RDEnsemble::refresh_tissue(int m) {
   if (m < 0 || m >= nm_) return fail(-2, "bad member");

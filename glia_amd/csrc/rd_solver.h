@@ -164,6 +164,9 @@ class RDEnsemble {
   // the same smoother on any device field f of the grid, in place, through tmp (a distinct field
   // of the grid) and the spectral work slot CS_T7 of member 0; no copies, no host synchronization
   int smooth_ptr(float* f, float* tmp, double sigma);
+  // Phi::setGaussians(Vec data): the lattice centers whose ball holds more than gvf of its voxels
+  // above thr in real field rslot of member m (counted on the device, selected in index order)
+  int data_support(int m, int rslot, double thr, double sigma, double gvf, std::vector<double>* centers);
   int refresh_tissue(int m);                         // derived quantities from stored tissue fields
   int profile_get(int kind, double* ms, long long* count);
 

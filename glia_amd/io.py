@@ -73,6 +73,35 @@ def parse_config(path):
     return out
 
 
+def phi_mesh(np_, cm, sigma, spacing_factor=1.5):
+    """Phi::phiMesh (src/mat/Phi.cpp:148-234): the bounding-box basis of np_ Gaussians around cm
+    (Phi::setGaussians(cm, sigma, spacing_factor, np), :33-73) as a (k, 3) array."""
+    c = np.ascontiguousarray(cm, dtype=np.float64)
+    if c.size != 3 or int(np_) < 1:
+        raise ValueError("phi_mesh needs a 3-vector center and np >= 1")
+    out = np.zeros(3 * int(np_), dtype=np.float64)
+    k = _lib.load().glia_phi_mesh(int(np_), c.ctypes.data_as(ctypes.c_void_p), float(sigma), float(spacing_factor),
+                                  out.ctypes.data_as(ctypes.c_void_p))
+    return out[: 3 * k].reshape(k, 3)
+
+
+def read_phi_mesh(path, max_np=100000):
+    """readPhiMesh (src/utils/IO.cpp:717-790): (centers (k, 3), component labels (k,), sigma,
+    spacing) from a phi-mesh text file."""
+    L = _lib.load()
+    c = np.zeros(3 * max_np, dtype=np.float64)
+    comps = np.zeros(max_np, dtype=np.int32)
+    k, sg, sp = ctypes.c_int(0), ctypes.c_double(0), ctypes.c_double(0)
+    rc = L.glia_read_phi_mesh(str(path).encode(), int(max_np), c.ctypes.data_as(ctypes.c_void_p),
+                              comps.ctypes.data_as(ctypes.c_void_p), ctypes.byref(k), ctypes.byref(sg),
+                              ctypes.byref(sp))
+    if rc == -1:
+        raise FileNotFoundError(path)
+    if rc:
+        raise ValueError("%s holds %d centers, more than %d" % (path, k.value, max_np))
+    return c[: 3 * k.value].reshape(k.value, 3), comps[: k.value], sg.value, sp.value
+
+
 def edema_obs_mask(tc, ed, lam):
     """createEdemaBasedObservationMask (src/utils/Utils.cpp:567-590): OBS = 1[TC] + lambda 1[B/WT],
     i.e. 1 where tc > 0.99, else 0 where ed > 0.99, else lambda."""
@@ -139,3 +168,25 @@ def load_data(solver, member, cfg, maps=None):
         mask = read_netcdf(cfg["obs_mask_path"]).astype(np.float32)
     solver.set_data(member, d1, mask, thr)
     return d1, mask, thr
+
+
+def init_gaussians(solver, member, cfg):
+    """SolverInterface::initializeGaussians (src/SolverInterface.cpp:860-905) for one member, after
+    load_data: the centers from the phi-mesh file cfg["gaussian_cm_path"] when given, else the
+    data-driven support (Phi::setGaussians(Vec data), counted on the GPU) of the map
+    cfg["support_data_path"] or, without one, of the member's data d1; sigma is
+    cfg["sigma_data_driven"].  Sets the member's basis and returns the centers."""
+    sigma = float(cfg.get("sigma_data_driven", 2 * np.pi / 256))
+    if cfg.get("gaussian_cm_path"):
+        centers = read_phi_mesh(cfg["gaussian_cm_path"])[0]
+    else:
+        which = "d1"
+        if cfg.get("support_data_path"):
+            solver.set_field(member, "work", read_netcdf(cfg["support_data_path"]).astype(np.float32))
+            which = "work"
+        centers = solver.data_support(member, which, float(cfg.get("threshold_data_driven", 0.05)), sigma,
+                                      float(cfg.get("gaussian_volume_fraction", 0.0)))
+    if len(centers) == 0:
+        raise ValueError("no Gaussian centers: the support holds no voxel above the threshold")
+    solver.set_basis(member, centers, sigma)
+    return centers

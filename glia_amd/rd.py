@@ -300,6 +300,20 @@ class RDSolver:
         """weierstrassSmoother of a stored real field, in place."""
         self._chk(self.L.glia_rd_smooth_field(self.h, int(member), _lib.FIELD[which], float(sigma)))
 
+    def data_support(self, member, which="d1", threshold=0.05, sigma=2 * np.pi / 256, gvf=0.0):
+        """Phi::setGaussians(Vec data) (src/mat/Phi.cpp:615-940): the lattice centers whose ball
+        holds more than gvf of its voxels above threshold in a stored field, as a (k, 3) array in
+        index order (counted on the device).  Defaults are the reference's phi_sigma_data_driven,
+        data_threshold and gaussian_vol_frac."""
+        k = ctypes.c_int(0)
+        args = (self.h, int(member), _lib.FIELD[which], float(threshold), float(sigma), float(gvf))
+        rc = self.L.glia_rd_data_support(*args, 0, None, ctypes.byref(k))
+        if rc != 0 and k.value == 0:
+            self._chk(rc)
+        out = np.zeros((max(k.value, 1), 3), dtype=np.float64)
+        self._chk(self.L.glia_rd_data_support(*args, k.value, out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(k)))
+        return out[: k.value]
+
     def refresh_tissue(self, member):
         """Mask, k, rho, preconditioner and Phi from the stored tissue fields."""
         self._chk(self.L.glia_rd_refresh_tissue(self.h, int(member)))

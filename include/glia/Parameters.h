@@ -10,6 +10,7 @@
 #define GLIA_PARAMETERS_H_
 
 #include <cmath>
+#include <array>
 #include <memory>
 
 #include "../glia_opt.h"
@@ -40,6 +41,13 @@ struct TumorParameters {
   ScalarType k_gm_wm_ratio_ = 0.0, k_glm_wm_ratio_ = 0.0;
   ScalarType r_gm_wm_ratio_ = 0.0, r_glm_wm_ratio_ = 0.0;
   ScalarType phi_sigma_ = 2 * M_PI / 64;
+  // basis placement (include/Parameters.h:203-207, 244-246): data-driven sigma, bounding-box
+  // spacing, data threshold, Gaussian volume fraction and the bounding box's center
+  ScalarType phi_sigma_data_driven_ = 2 * M_PI / 256;
+  ScalarType phi_spacing_factor_ = 1.5;
+  ScalarType data_threshold_ = 0.05;
+  ScalarType gaussian_vol_frac_ = 0.0;
+  std::array<ScalarType, 3> user_cm_ = {2 * M_PI / 128 * 64, 2 * M_PI / 128 * 38, 2 * M_PI / 128 * 56};
   ScalarType obs_threshold_1_ = 0.0;
   ScalarType smoothing_factor_ = 1.0;
   int order_ = 2;

@@ -13,8 +13,10 @@
 
 #include <limits>
 #include <memory>
+#include <string>
 #include <vector>
 
+#include "../glia_io.h"
 #include "Parameters.h"
 
 namespace glia {
@@ -208,6 +210,62 @@ class Phi {
     params_->tu_->phi_sigma_ = sigma;
     return glia_rd_set_basis(h_, 0, np_, centers.data(), sigma);
   }
+  /* setGaussians(cm, sigma, spacing_factor, np) (src/mat/Phi.cpp:33-73): the bounding-box basis
+     of phiMesh (:148-234) around cm */
+  PetscErrorCode setGaussians(std::array<ScalarType, 3>& user_cm, ScalarType sigma, ScalarType spacing_factor, int np) {
+    if (rc_) return rc_;
+    if (np < 1) return GLIA_RD_ERR_ARG;
+    std::vector<ScalarType> c(3 * (size_t)np);
+    const int k = glia_phi_mesh(np, user_cm.data(), sigma, spacing_factor, c.data());
+    c.resize(3 * (size_t)k);
+    spacing_factor_ = spacing_factor;
+    params_->tu_->phi_spacing_factor_ = spacing_factor;
+    return setGaussians(c, sigma);
+  }
+  /* setGaussians(Vec data) (:615-940): the data-driven support on the lattice of spacing
+     2 phi_sigma_data_driven, candidates kept when more than gaussian_vol_frac of their ball is
+     above data_threshold (counted on the device); every Gaussian gets component label 1 */
+  PetscErrorCode setGaussians(Vec data) {
+    if (rc_) return rc_;
+    if (!data) return GLIA_RD_ERR_ARG;
+    Vec w = nullptr;
+    PetscErrorCode rc = 0;
+    int which = data->which, member = data->member;
+    if (!(data->h == h_ && which >= 0)) {  // stage a caller vector in the work field
+      which = GLIA_RD_FIELD_WORK, member = 0;
+      VecCreateField(h_, 0, which, data->n, &w);
+      rc = VecCopy(data, w);
+      VecDestroy(&w);
+      if (rc) return rc;
+    }
+    const auto& tu = *params_->tu_;
+    int k = 0;
+    rc = glia_rd_data_support(h_, member, which, tu.data_threshold_, tu.phi_sigma_data_driven_, tu.gaussian_vol_frac_,
+                              0, nullptr, &k);
+    if (rc && k == 0) return rc;
+    std::vector<ScalarType> c(3 * (size_t)k);
+    if ((rc = glia_rd_data_support(h_, member, which, tu.data_threshold_, tu.phi_sigma_data_driven_,
+                                   tu.gaussian_vol_frac_, k, c.data(), &k)))
+      return rc;
+    spacing_factor_ = 2.0;
+    params_->tu_->phi_spacing_factor_ = 2.0;
+    gaussian_labels_.assign((size_t)k, 1);
+    return k ? setGaussians(c, tu.phi_sigma_data_driven_) : GLIA_RD_ERR_ARG;
+  }
+  /* setGaussians(file, read_comp_data) (:525-612): centers (and component labels) from a
+     phi-mesh file (readPhiMesh, src/utils/IO.cpp:717-790), sigma = phi_sigma_data_driven */
+  PetscErrorCode setGaussians(std::string file, bool read_comp_data = false) {
+    if (rc_) return rc_;
+    int k = 0;
+    if (glia_read_phi_mesh(file.c_str(), 0, nullptr, nullptr, &k, nullptr, nullptr) == -1 || k < 1) return 1;
+    std::vector<ScalarType> c(3 * (size_t)k);
+    std::vector<int> comps((size_t)k, 1);
+    if (glia_read_phi_mesh(file.c_str(), k, c.data(), comps.data(), &k, nullptr, nullptr)) return 1;
+    spacing_factor_ = 2.0;
+    params_->tu_->phi_spacing_factor_ = 2.0;
+    if (read_comp_data) gaussian_labels_ = comps;
+    return setGaussians(c, params_->tu_->phi_sigma_data_driven_);
+  }
   /* the basis is built when the centers are set and rebuilt when the tissue changes */
   PetscErrorCode setValues(std::shared_ptr<MatProp>) { return rc_; }
   /* apply (:313-377): out = Phi p (the handle's c_0_ receives it too) */
@@ -238,6 +296,8 @@ class Phi {
 
   int np_ = 0;
   std::vector<ScalarType> centers_;
+  ScalarType spacing_factor_ = 0;
+  std::vector<int> gaussian_labels_;
   ScalarType sigma_ = 0;
   std::shared_ptr<Parameters> params_;
   glia_rd_t h_ = nullptr;

@@ -60,6 +60,11 @@ typedef struct glia_rd_file_config {
   int obs_threshold_rel;        /* obs_threshold_rel (obs_threshold_1 relative to max d1) */
   int patient_labels[8];        /* patient_labels, as atlas_labels; -1 absent */
   char p_seg_path[512];         /* p_seg_path */
+  double threshold_data_driven; /* threshold_data_driven (data-driven basis support threshold) */
+  double gaussian_volume_fraction; /* gaussian_volume_fraction */
+  double sigma_data_driven;     /* phi_sigma_data_driven: 2 pi / 256, or sigma_factor * 2 pi / n */
+  char support_data_path[512];  /* support_data_path (NetCDF map the support is taken from) */
+  char gaussian_cm_path[512];   /* gaussian_cm_path (phi-mesh file of centers) */
 } glia_rd_file_config;
 
 /* Defaults of the reference parameter structs for the fields above, then the file's values.
@@ -70,6 +75,20 @@ int glia_rd_parse_config(const char* path, glia_rd_file_config* out);
    OBS = 1[TC] + lambda 1[B/WT] from the tumor-core and edema maps of a segmentation:
    mask = 1 where tc > 0.99, else 0 where ed > 0.99, else lambda. */
 int glia_edema_obs_mask(const float* tc, const float* ed, int64_t count, double lambda, float* mask);
+
+/* Phi::phiMesh (src/mat/Phi.cpp:148-234): the bounding-box basis of Phi::setGaussians(cm, sigma,
+   spacing_factor, np) (:33-73).  np = 1 gives cm itself; odd np a cube of h = round(np^(1/3))
+   centers per axis spaced spacing_factor * sigma around cm; even np the 8 octants' cubes offset by
+   half a spacing (h/2 per side).  Writes the centers (x, y, z, first axis fastest) into centers
+   (room for 3 np values) and returns their number. */
+int glia_phi_mesh(int np, const double cm[3], double sigma, double spacing_factor, double* centers);
+/* readPhiMesh (src/utils/IO.cpp:717-790): a phi-mesh text file ("sigma = s, spacing = d", one line
+   skipped, then "x, y, z[, component]" per center up to a line holding "]").  Writes at most
+   max_np centers (3 values each) and, when comps is not NULL, their component labels; *np receives
+   the number of centers in the file.  sigma and spacing (may be NULL) receive the header values.
+   Returns 0, -1 (file not readable) or -2 (more than max_np centers). */
+int glia_read_phi_mesh(const char* path, int max_np, double* centers, int* comps, int* np, double* sigma,
+                       double* spacing);
 
 #ifdef __cplusplus
 }

@@ -150,6 +150,20 @@ int glia_rd_smooth_field(glia_rd_t h, int member, int which, double sigma);
    limits) or a hipError_t; err (may be NULL) receives the message. */
 int glia_rd_smooth_noalloc(const int n[3], const float* in, float* out, double sigma, void* stream, char* err,
                            int err_len);
+/* Phi::setGaussians(Vec data) (src/mat/Phi.cpp:615-940, one process): the data-driven basis
+   support from real field `which` of a member (the reference passes Data::dt1() after smoothing, or
+   a support map read with dataIn, src/SolverInterface.cpp:622-633).  Candidates lie on the lattice
+   of the reference's loops: per axis from 0 to the largest index that is an exact multiple of
+   space = 2 sigma / h0, in steps x = (int)(x + space).  A candidate counts the voxels with
+   field > threshold (single precision) at integer offsets d with sqrt(|d|^2) <= sigma / h0, clipped
+   to the grid without wrapping; it is kept when the count exceeds gvf times the full ball's count.
+   The kept centers (x, y, z in [0, 2 pi)) are written in index order with *np their number;
+   centers may be NULL with max_np = 0 to query the count.  The counting runs on the device.
+   Returns GLIA_RD_ERR_ARG (with *np set) when more than max_np are kept, or when space < 1 (the
+   reference's loop would not advance).  The reference's defaults: sigma = 2 pi / 256
+   (phi_sigma_data_driven), threshold 0.05 (data_threshold), gvf 0 (gaussian_vol_frac). */
+int glia_rd_data_support(glia_rd_t h, int member, int which, double threshold, double sigma, double gvf, int max_np,
+                         double* centers, int* np);
 /* MatProp::setValuesCustom on the tissue fields already stored (e.g. after glia_rd_set_field and
    glia_rd_smooth_field): mask, k, rho, preconditioner and Phi of the member */
 int glia_rd_refresh_tissue(glia_rd_t h, int member);

@@ -873,6 +873,58 @@ int orc_phi_mesh(int np, const double *cm, double sigma, double spacing_factor, 
   return ptr / 3;
 }
 
+/* Phi::setGaussians(Vec data) on one process (src/mat/Phi.cpp:615-940).  Candidates sit on the
+   lattice that the loops of :686-720 walk: per axis from 0 to the largest index that is an exact
+   multiple of space = 2 sigma / h0 (fmod test), in steps x = (int)(x + space).  A candidate counts
+   the voxels with data > threshold (compared in single precision, as the GPU build) at integer
+   offsets d with sqrt(|d|^2) <= r = sigma / h0, clipped to the grid without wrapping
+   (checkTumorExistence / checkTumorExistenceOutOfProc, :457-522); it is kept when its count
+   exceeds gvf times the full ball's count (:870-888).  Kept centers (X hx, Y hy, Z hz) in index
+   order, at most max_np written; returns the number kept, or -1 when space < 1 (the reference's
+   loop would not advance). */
+int orc_data_support(const float *data, const int *n, double threshold, double sigma, double gvf, int max_np,
+                     double *centers, int *counts_out) {
+  const double twopi = 2.0 * M_PI;
+  const double hx = twopi / n[0], hy = twopi / n[1], hz = twopi / n[2];
+  const double r = sigma / hx, space = 2.0 * sigma / hx;
+  if (!(space >= 1.0)) return -1;
+  const float thr = (float)threshold;
+  int interior = 0;
+  for (int i = (int)(-r); i <= r; i++)
+    for (int j = (int)(-r); j <= r; j++)
+      for (int k = (int)(-r); k <= r; k++)
+        if (sqrt((double)(i * i + j * j + k * k)) <= r) interior++;
+  int end[3];
+  for (int a = 0; a < 3; a++) {
+    end[a] = 0;
+    for (int x = n[a] - 1; x >= 0; x--)
+      if (fmod((double)x, space) == 0) { end[a] = x; break; }
+  }
+  int np = 0, idx = 0;
+  for (int x = 0; x <= end[0]; x = (int)(x + space))
+    for (int y = 0; y <= end[1]; y = (int)(y + space))
+      for (int z = 0; z <= end[2]; z = (int)(z + space), idx++) {
+        int num = 0;
+        for (int i = (int)(x - r); i <= x + r; i++)
+          for (int j = (int)(y - r); j <= y + r; j++)
+            for (int k = (int)(z - r); k <= z + r; k++) {
+              if (i < 0 || j < 0 || k < 0 || i >= n[0] || j >= n[1] || k >= n[2]) continue;
+              const double dist = sqrt((double)((i - x) * (i - x) + (j - y) * (j - y) + (k - z) * (k - z)));
+              if (dist <= r && data[((int64_t)i * n[1] + j) * n[2] + k] > thr) num++;
+            }
+        if (counts_out) counts_out[idx] = num;
+        if (num > gvf * interior) {
+          if (np < max_np) {
+            centers[3 * np + 0] = x * hx;
+            centers[3 * np + 1] = y * hy;
+            centers[3 * np + 2] = z * hz;
+          }
+          np++;
+        }
+      }
+  return np;
+}
+
 /* Phi::initialize (src/mat/Phi.cpp:264-311) + mask + smoother + Phi::truncate (:238-260) */
 static void phi_build_one(orc_t *o, const double *ctr, double *out) {
   const int n1 = o->n[1], n2 = o->n[2];

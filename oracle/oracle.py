@@ -92,6 +92,7 @@ def _load():
         "orc_solve_state": (i, [vp, i]),
         "orc_solve_adjoint": (i, [vp, i]),
         "orc_phi_mesh": (i, [i, _dp, d, d, _dp]),
+        "orc_data_support": (i, [vp, vp, d, d, d, i, vp, vp]),
         "orc_set_phi": (None, [vp, i, _dp, d, d, i]),
         "orc_get_phi": (None, [vp, i, _dp]),
         "orc_phi_apply": (None, [vp, _dp, _dp]),
@@ -138,6 +139,26 @@ def phi_mesh(np_, cm, sigma, spacing_factor):
     out = np.zeros(3 * max(np_, 1) * 2, dtype=np.float64)
     k = _load().orc_phi_mesh(int(np_), _c(cm), float(sigma), float(spacing_factor), out)
     return out[: 3 * k].reshape(k, 3)
+
+
+def data_support(data, threshold, sigma, gvf=0.0, counts=False):
+    """Phi::setGaussians(Vec data) (src/mat/Phi.cpp:615-940) on one process (orc_data_support):
+    the kept lattice centers as a (k, 3) array, and with counts=True also the per-candidate voxel
+    counts in lattice order."""
+    f = np.ascontiguousarray(data, dtype=np.float32)
+    n = np.array(f.shape, dtype=np.int32)
+    lib = _load()
+    cnt = np.zeros(f.size, dtype=np.int32)
+    k = lib.orc_data_support(f.ctypes.data_as(ctypes.c_void_p), n.ctypes.data_as(ctypes.c_void_p),
+                             float(threshold), float(sigma), float(gvf), 0, None,
+                             cnt.ctypes.data_as(ctypes.c_void_p))
+    if k < 0:
+        raise ValueError("data support lattice spacing below one cell")
+    out = np.zeros(3 * max(k, 1), dtype=np.float64)
+    lib.orc_data_support(f.ctypes.data_as(ctypes.c_void_p), n.ctypes.data_as(ctypes.c_void_p), float(threshold),
+                         float(sigma), float(gvf), k, out.ctypes.data_as(ctypes.c_void_p), None)
+    c = out[: 3 * k].reshape(k, 3)
+    return (c, cnt) if counts else c
 
 
 def stress_point(g, mu, lam):

@@ -10,6 +10,7 @@
 //   TILOptimizer (quasi- and Gauss-Newton), RDOptimizer, SparseTILOptimizer on exact data (src/opt/*.cpp through include/glia/Optimizer.h)
 //
 // Exit status: the number of failed checks.  Needs a GPU.
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -599,6 +600,58 @@ static void test_phi_and_apply_d() {
   VecDestroy(&s.d);
 }
 
+// Phi::setGaussians in its three placements (src/mat/Phi.cpp:33-73, 525-612, 615-940)
+static void test_basis_placement() {
+  std::printf("TEST Phi::setGaussians: bounding box, data-driven support, phi-mesh file\n");
+  TilSetup s;
+  make_til(s);
+  auto& tu = *s.params->tu_;
+  // bounding box: 27 Gaussians around cm, spaced 1.5 sigma
+  std::array<ScalarType, 3> cm = {M_PI, M_PI, M_PI};
+  REQUIRE(s.tumor->phi_->setGaussians(cm, 2 * M_PI / 64, 1.5, 27) == 0);
+  REQUIRE(s.tumor->phi_->np_ == 27 && tu.np_ == 27);
+  REQUIRE(std::fabs(s.tumor->phi_->centers_[3 * 13] - M_PI) < 1e-15);
+  // data-driven: radius 2 cells, candidates every 4 cells; brute force over the host copy of d
+  tu.phi_sigma_data_driven_ = 2 * (2 * M_PI / 64);
+  tu.data_threshold_ = 0.05;
+  tu.gaussian_vol_frac_ = 0.0;
+  REQUIRE(s.tumor->phi_->setGaussians(s.d) == 0);
+  double* dp = nullptr;
+  VecGetArray(s.d, &dp);
+  std::vector<double> want;
+  const double h = 2 * M_PI / 64;
+  for (int x = 0; x <= 60; x += 4)
+    for (int y = 0; y <= 60; y += 4)
+      for (int z = 0; z <= 60; z += 4) {
+        int cnt = 0;
+        for (int i = -2; i <= 2; i++)
+          for (int j = -2; j <= 2; j++)
+            for (int k = -2; k <= 2; k++) {
+              const int a = x + i, b = y + j, c = z + k;
+              if (i * i + j * j + k * k > 4 || a < 0 || b < 0 || c < 0 || a >= 64 || b >= 64 || c >= 64) continue;
+              cnt += (float)dp[((size_t)a * 64 + b) * 64 + c] > 0.05f;
+            }
+        if (cnt > 0) want.insert(want.end(), {x * h, y * h, z * h});
+      }
+  VecRestoreArray(s.d, &dp);
+  std::printf("  data-driven support: %d centers (brute force %d)\n", s.tumor->phi_->np_, (int)(want.size() / 3));
+  REQUIRE(!want.empty() && s.tumor->phi_->centers_ == want && tu.np_ == (int)(want.size() / 3));
+  REQUIRE(s.tumor->phi_->gaussian_labels_.size() == want.size() / 3);
+  // phi-mesh file with component labels
+  const char* path = "/tmp/glia_test_phi_mesh.dat";
+  FILE* f = std::fopen(path, "w");
+  REQUIRE(f != nullptr);
+  std::fprintf(f, " sigma = %.9g, spacing = %.9g\n centers = [\n", tu.phi_sigma_data_driven_, 2 * tu.phi_sigma_data_driven_);
+  for (int i = 0; i < 3; i++) std::fprintf(f, " %.17g, %.17g, %.17g, %d\n", M_PI + 0.1 * i, M_PI, M_PI - 0.1 * i, i + 1);
+  std::fprintf(f, "];\n");
+  std::fclose(f);
+  REQUIRE(s.tumor->phi_->setGaussians(std::string(path), true) == 0);
+  REQUIRE(s.tumor->phi_->np_ == 3 && s.tumor->phi_->gaussian_labels_ == std::vector<int>({1, 2, 3}));
+  REQUIRE(std::fabs(s.tumor->phi_->centers_[3] - (M_PI + 0.1)) < 1e-15);
+  std::remove(path);
+  VecDestroy(&s.d);
+}
+
 // ---- the inversion classes on exact data ------------------------------------------------------
 // the TIL problem of make_til with exact data d = c(T) of p* and the truth kappa = 0.05, rho = 8
 static void make_exact(TilSetup& s) {
@@ -748,6 +801,7 @@ int main() {
   test_interface_matches_abi_and_fd();
   test_boundary_methods();
   test_phi_and_apply_d();
+  test_basis_placement();
   test_til_optimizer(QUASINEWTON);
   test_til_optimizer(GAUSSNEWTON);
   test_rd_optimizer();

@@ -128,3 +128,40 @@ def test_smoother_radii_vs_oracle(oracle_mod, factor):
     err = rel(got, ref)
     print("rel", err, flush=True)
     assert err < 1e-6
+
+
+@pytest.mark.gpu
+def test_init_gaussians_from_config(oracle_mod, tmp_path):
+    """SolverInterface::initializeGaussians (src/SolverInterface.cpp:860-905) after readData: the
+    data-driven support of the smoothed d1 (Phi::setGaussians(Vec data)) equals the oracle's on the
+    same field, and is the basis the member then applies; a phi-mesh file (gaussian_cm_path)
+    replaces it."""
+    from glia_amd import io, rd
+    print("case init gaussians", flush=True)
+    n = 64
+    x = np.arange(n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    seg = np.full((n, n, n), 6, np.float32)
+    seg[np.sqrt((X - 30.0) ** 2 + (Y - 35.0) ** 2 + (Z - 28.0) ** 2) < 6] = 4
+    lab = dict(wm=6, gm=5, vt=7, csf=8, nec=1, en=4)
+    cfg = dict(smoothing_factor_data=1.0, sigma_data_driven=2 * np.pi / 32, threshold_data_driven=0.05,
+               gaussian_volume_fraction=0.0)
+    s = rd.RDSolver(n)
+    try:
+        maps = io.load_atlas(s, 0, seg, lab, smoothing_factor=1.0)
+        d1, _, _ = io.load_data(s, 0, cfg, maps)
+        centers = io.init_gaussians(s, 0, cfg)
+        want = oracle_mod.data_support(s.get_field(0, "d1"), 0.05, 2 * np.pi / 32)
+        print("support centers", len(centers), flush=True)
+        assert len(want) > 8 and np.array_equal(centers, want)
+        p = np.zeros(len(centers))
+        p[len(centers) // 2] = 1.0  # one active basis: Phi p is that basis scaled to maximum 1
+        s.phi_apply([p])
+        assert abs(float(s.get_field(0, "c0").max()) - 1.0) < 1e-5
+        path = tmp_path / "phi-mesh.dat"
+        path.write_text(" sigma = 0.1, spacing = 0.2\n centers = [\n" +
+                        "".join(" %.17g, %.17g, %.17g, 1\n" % tuple(c) for c in centers[:3]) + "];\n")
+        got = io.init_gaussians(s, 0, dict(cfg, gaussian_cm_path=str(path)))
+        assert np.array_equal(got, centers[:3])
+    finally:
+        s.close()
