@@ -130,6 +130,7 @@ IO_SIGNATURES = {
     "glia_edema_obs_mask": (_i, [_vp, _vp, _i64, _d, _vp]),
     "glia_phi_mesh": (_i, [_i, _vp, _d, _d, _vp]),
     "glia_read_phi_mesh": (_i, [_cp, _i, _vp, _vp, ctypes.POINTER(_i), _pd, _pd]),
+    "glia_read_concomp": (_i, [_cp, _i, _vp, _vp, ctypes.POINTER(_i)]),
 }
 
 
@@ -204,7 +205,7 @@ class FileConfig(ctypes.Structure):
         ("patient_labels", ctypes.c_int * 8), ("p_seg_path", ctypes.c_char * 512),
         ("threshold_data_driven", ctypes.c_double), ("gaussian_volume_fraction", ctypes.c_double),
         ("sigma_data_driven", ctypes.c_double), ("support_data_path", ctypes.c_char * 512),
-        ("gaussian_cm_path", ctypes.c_char * 512),
+        ("gaussian_cm_path", ctypes.c_char * 512), ("data_comp_data_path", ctypes.c_char * 512),
     ]
 
 

@@ -356,6 +356,39 @@ int glia_read_phi_mesh(const char* path, int max_np, double* centers, int* comps
   return k > max_np ? -2 : 0;
 }
 
+int glia_read_concomp(const char* path, int max_comp, double* weights, double* centers, int* ncomp) {
+  if (!path || !ncomp || max_comp < 0 || (max_comp > 0 && !weights)) return -2;
+  std::ifstream f(path);
+  if (!f) return -1;
+  std::string line;
+  std::getline(f, line);  // "#components"
+  std::getline(f, line);
+  const int nc = std::max(0, std::atoi(line.c_str()));
+  std::getline(f, line);  // "center of mass:"
+  for (int i = 0; i < nc; i++) {
+    std::getline(f, line);
+    size_t pos = 0;
+    for (int t = 0; t < 3; t++) {
+      const size_t comma = line.find(',', pos);
+      const double v = std::atof(line.substr(pos, comma == std::string::npos ? std::string::npos : comma - pos).c_str());
+      if (centers && i < max_comp) centers[3 * i + t] = v;
+      if (comma == std::string::npos) {
+        for (int r = t + 1; r < 3; r++)
+          if (centers && i < max_comp) centers[3 * i + r] = 0;
+        break;
+      }
+      pos = comma + 1;
+    }
+  }
+  std::getline(f, line);  // "relative mass:"
+  for (int i = 0; i < nc; i++) {
+    std::getline(f, line);
+    if (i < max_comp) weights[i] = std::atof(line.c_str());
+  }
+  *ncomp = nc;
+  return nc > max_comp ? -2 : 0;
+}
+
 int glia_split_segmentation(const float* seg, int64_t count, const int* labels, float* wm, float* gm, float* vt,
                             float* csf, float* tc, float* ed) {
   if (!seg || !labels || !wm || !gm || !vt || count <= 0) return GLIA_RD_ERR_ARG;
@@ -457,6 +490,7 @@ int glia_rd_parse_config(const char* path, glia_rd_file_config* o) {
       else if (k == "gaussian_volume_fraction") o->gaussian_volume_fraction = std::stod(v);
       else if (k == "support_data_path") copy_path(o->support_data_path, v);
       else if (k == "gaussian_cm_path") copy_path(o->gaussian_cm_path, v);
+      else if (k == "data_comp_data_path") copy_path(o->data_comp_data_path, v);
       else if (k == "atlas_labels" || k == "patient_labels") {
         int* dst = (k == "atlas_labels") ? o->atlas_labels : o->patient_labels;
         // [wm=6,gm=5,vt=7,csf=8,...] (src/utils/IO.cpp:126-170)

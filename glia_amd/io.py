@@ -102,6 +102,31 @@ def read_phi_mesh(path, max_np=100000):
     return c[: 3 * k.value].reshape(k.value, 3), comps[: k.value], sg.value, sp.value
 
 
+def read_concomp(path, max_comp=4096):
+    """readConCompDat (src/utils/IO.cpp:929-990): (relative masses (k,), centers of mass (k, 3)) of
+    the connected components in a concomp file."""
+    w = np.zeros(max_comp, dtype=np.float64)
+    c = np.zeros(3 * max_comp, dtype=np.float64)
+    k = ctypes.c_int(0)
+    rc = _lib.load().glia_read_concomp(str(path).encode(), int(max_comp), w.ctypes.data_as(ctypes.c_void_p),
+                                       c.ctypes.data_as(ctypes.c_void_p), ctypes.byref(k))
+    if rc == -1:
+        raise FileNotFoundError(path)
+    if rc:
+        raise ValueError("%s holds %d components, more than %d" % (path, k.value, max_comp))
+    return w[: k.value], c[: 3 * k.value].reshape(k.value, 3)
+
+
+def component_sparsity(sparsity_level, weights, thresh_component_weight=1e-3):
+    """The sparsity level the reference sets from the component weights (src/Solver.cpp:231-248):
+    s * (components of weight >= thresh) + (components of weight in [thresh / 10, thresh))."""
+    w = np.asarray(weights, dtype=np.float64)
+    t = float(thresh_component_weight)
+    nnc = int(np.sum(w >= t))
+    nnc_t = int(np.sum((w < t) & (w >= t / 10.0)))
+    return int(sparsity_level) * nnc + nnc_t
+
+
 def edema_obs_mask(tc, ed, lam):
     """createEdemaBasedObservationMask (src/utils/Utils.cpp:567-590): OBS = 1[TC] + lambda 1[B/WT],
     i.e. 1 where tc > 0.99, else 0 where ed > 0.99, else lambda."""

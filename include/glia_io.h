@@ -65,6 +65,7 @@ typedef struct glia_rd_file_config {
   double sigma_data_driven;     /* phi_sigma_data_driven: 2 pi / 256, or sigma_factor * 2 pi / n */
   char support_data_path[512];  /* support_data_path (NetCDF map the support is taken from) */
   char gaussian_cm_path[512];   /* gaussian_cm_path (phi-mesh file of centers) */
+  char data_comp_data_path[512]; /* data_comp_data_path (connected-component file, readConCompDat) */
 } glia_rd_file_config;
 
 /* Defaults of the reference parameter structs for the fields above, then the file's values.
@@ -89,6 +90,12 @@ int glia_phi_mesh(int np, const double cm[3], double sigma, double spacing_facto
    Returns 0, -1 (file not readable) or -2 (more than max_np centers). */
 int glia_read_phi_mesh(const char* path, int max_np, double* centers, int* comps, int* np, double* sigma,
                        double* spacing);
+/* readConCompDat (src/utils/IO.cpp:929-990): a connected-component file ("#components" line, the
+   count, a header line, one "x,y,z" center of mass per component, a header line, one relative mass
+   per component).  Writes at most max_comp weights and centers (3 values each; centers may be
+   NULL); *ncomp receives the count in the file.  Returns 0, -1 (file not readable) or -2 (more
+   than max_comp components). */
+int glia_read_concomp(const char* path, int max_comp, double* weights, double* centers, int* ncomp);
 
 #ifdef __cplusplus
 }

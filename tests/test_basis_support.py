@@ -129,3 +129,22 @@ def test_gpu_data_support_vs_oracle(oracle_mod, n, sigma, gvf):
     print("centers", len(got), len(want), flush=True)
     assert len(want) > 8
     assert np.array_equal(got, want)
+
+
+def test_read_concomp_and_sparsity(tmp_path):
+    """readConCompDat (src/utils/IO.cpp:929-990) on a file in its layout, and the sparsity level of
+    src/Solver.cpp:231-248: s per component of weight >= thresh, one per component of weight in
+    [thresh / 10, thresh), none below."""
+    from glia_amd import io
+    p = tmp_path / "concomp.dat"
+    p.write_text("#components\n3\ncenter of mass:\n1.5,2.25,3\n0.5,0.75,1\n4,5,6\nrelative mass:\n"
+                 "0.9\n0.0005\n0.00005\n")
+    w, c = io.read_concomp(p)
+    assert w.tolist() == [0.9, 0.0005, 0.00005]
+    assert c.tolist() == [[1.5, 2.25, 3.0], [0.5, 0.75, 1.0], [4.0, 5.0, 6.0]]
+    assert io.component_sparsity(5, w, 1e-3) == 5 * 1 + 1
+    assert io.component_sparsity(5, [0.5, 0.3, 0.2], 1e-3) == 15
+    with pytest.raises(ValueError):
+        io.read_concomp(p, max_comp=2)
+    with pytest.raises(FileNotFoundError):
+        io.read_concomp(tmp_path / "missing.dat")
