@@ -206,6 +206,8 @@ class FileConfig(ctypes.Structure):
         ("threshold_data_driven", ctypes.c_double), ("gaussian_volume_fraction", ctypes.c_double),
         ("sigma_data_driven", ctypes.c_double), ("support_data_path", ctypes.c_char * 512),
         ("gaussian_cm_path", ctypes.c_char * 512), ("data_comp_data_path", ctypes.c_char * 512),
+        ("d0_path", ctypes.c_char * 512), ("smoothing_factor_data_t0", ctypes.c_double),
+        ("obs_threshold_0", ctypes.c_double),
     ]
 
 

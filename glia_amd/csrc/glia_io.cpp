@@ -442,6 +442,8 @@ int glia_rd_parse_config(const char* path, glia_rd_file_config* o) {
   o->threshold_data_driven = 0.05;
   o->gaussian_volume_fraction = 0.0;
   o->sigma_data_driven = 2.0 * M_PI / 256;
+  o->smoothing_factor_data_t0 = 1.0;
+  o->obs_threshold_0 = 0.0;
   std::ifstream f(path ? path : "");
   if (!f) return GLIA_RD_ERR_ARG;
   std::string line;
@@ -491,6 +493,9 @@ int glia_rd_parse_config(const char* path, glia_rd_file_config* o) {
       else if (k == "support_data_path") copy_path(o->support_data_path, v);
       else if (k == "gaussian_cm_path") copy_path(o->gaussian_cm_path, v);
       else if (k == "data_comp_data_path") copy_path(o->data_comp_data_path, v);
+      else if (k == "d0_path") copy_path(o->d0_path, v);
+      else if (k == "smoothing_factor_data_t0") o->smoothing_factor_data_t0 = std::stod(v);
+      else if (k == "obs_threshold_0") o->obs_threshold_0 = std::stod(v);
       else if (k == "atlas_labels" || k == "patient_labels") {
         int* dst = (k == "atlas_labels") ? o->atlas_labels : o->patient_labels;
         // [wm=6,gm=5,vt=7,csf=8,...] (src/utils/IO.cpp:126-170)

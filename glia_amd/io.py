@@ -195,6 +195,36 @@ def load_data(solver, member, cfg, maps=None):
     return d1, mask, thr
 
 
+def load_c0(solver, member, cfg, c0=None):
+    """The c(0) half of SolverInterface::readData (src/SolverInterface.cpp:654-690) for one member:
+    c(0) from cfg["d0_path"] (or the array c0), clipped at zero when it has negative values,
+    smoothed on the GPU with sigma = smoothing_factor_data_t0 * 2 pi / n0, then scaled to maximum 1;
+    stored as the member's initial condition, which the kappa/rho evaluation mode uses as set
+    (use_c0_).  Returns (c(0), threshold), the threshold obs_threshold_0 taken relative to the
+    maximum before scaling under obs_threshold_rel."""
+    if c0 is None:
+        if not cfg.get("d0_path"):
+            raise ValueError("no c(0): neither d0_path nor an array")
+        c0 = read_netcdf(cfg["d0_path"])
+    c = np.asarray(c0, dtype=np.float32)
+    if c.min() < 0:
+        c = np.maximum(c, 0.0).astype(np.float32)
+    solver.set_field(member, "c0", c)
+    sf = float(cfg.get("smoothing_factor_data_t0", 1.0))
+    if sf > 0:
+        solver.smooth_field(member, "c0", sf * 2 * np.pi / solver.n[0])
+    c = solver.get_field(member, "c0")
+    mx = float(c.max())
+    if not mx > 0:
+        raise ValueError("c(0) has no positive value")
+    c = (c * np.float32(1.0 / mx)).astype(np.float32)
+    solver.set_field(member, "c0", c)
+    thr = float(cfg.get("obs_threshold_0", 0.0))
+    if cfg.get("obs_threshold_rel"):
+        thr *= mx
+    return c, thr
+
+
 def init_gaussians(solver, member, cfg):
     """SolverInterface::initializeGaussians (src/SolverInterface.cpp:860-905) for one member, after
     load_data: the centers from the phi-mesh file cfg["gaussian_cm_path"] when given, else the

@@ -66,6 +66,9 @@ typedef struct glia_rd_file_config {
   char support_data_path[512];  /* support_data_path (NetCDF map the support is taken from) */
   char gaussian_cm_path[512];   /* gaussian_cm_path (phi-mesh file of centers) */
   char data_comp_data_path[512]; /* data_comp_data_path (connected-component file, readConCompDat) */
+  char d0_path[512];            /* d0_path (tumor concentration c(0) of the two-time-point data) */
+  double smoothing_factor_data_t0; /* smoothing_factor_data_t0 (c(0) smoothed with factor * 2 pi / n) */
+  double obs_threshold_0;       /* obs_threshold_0 (relative to max c(0) under obs_threshold_rel) */
 } glia_rd_file_config;
 
 /* Defaults of the reference parameter structs for the fields above, then the file's values.

@@ -165,3 +165,55 @@ def test_init_gaussians_from_config(oracle_mod, tmp_path):
         assert np.array_equal(got, centers[:3])
     finally:
         s.close()
+
+
+@pytest.mark.gpu
+def test_c0_input_vs_oracle(oracle_mod, tmp_path):
+    """The c(0) half of readData (src/SolverInterface.cpp:654-690): c(0) from the reference's
+    testdata/patient_tumor_IC.nc, and from a map with negative values (clipped at zero), smoothed
+    with sigma = smoothing_factor_data_t0 * 2 pi / n and scaled to maximum 1, equals the oracle's
+    smoothing of the same map (1e-6); the threshold is taken relative to the maximum before the
+    scaling.  A kappa/rho evaluation from that c(0) as set (use_c0_) then matches the oracle's: J
+    within 1e-5, (g_kappa, g_rho) within 1e-4."""
+    from glia_amd import io, rd
+    print("case c0 input", flush=True)
+    n, nt, dt = 64, 4, 0.02
+    path = tmp_path / "patient_tumor_IC.nc"
+    path.write_bytes(gzip.decompress(open(os.path.join(HERE, "golden", "testdata", "patient_tumor_IC.nc.gz"), "rb").read()))
+    raw = io.read_netcdf(str(path)).astype(np.float32)
+    o = oracle_mod.Oracle(n)
+    sigma = 2 * np.pi / n
+    cfg = dict(d0_path=str(path), smoothing_factor_data_t0=1.0, obs_threshold_0=0.1, obs_threshold_rel=1)
+    s = rd.RDSolver(n, dt=dt, nt=nt)
+    try:
+        c0, thr = io.load_c0(s, 0, cfg)
+        sm = o.smooth(raw.astype(np.float64), sigma)
+        assert rel(c0, sm / sm.max()) < 1e-6 and abs(c0.max() - 1.0) < 1e-6
+        assert abs(thr - 0.1 * sm.max()) < 1e-6 * thr
+        neg = raw - np.float32(0.05)
+        c0n, _ = io.load_c0(s, 0, dict(cfg, d0_path=""), c0=neg)
+        smn = o.smooth(np.maximum(neg, 0.0).astype(np.float64), sigma)
+        assert rel(c0n, smn / smn.max()) < 1e-6
+        c0, _ = io.load_c0(s, 0, cfg)
+        s.set_tissue_sinusoidal(0)
+        s.set_basis(0, np.array([[np.pi, np.pi, np.pi]]), sigma)
+        s.set_field(0, "c0", c0)  # the initial condition the kappa/rho mode uses as set
+        d1 = np.roll(c0, 2, axis=0)
+        s.set_data(0, d1, None, -0.99)
+        x = np.array([0.0, 0.02, 6.0])
+        J, g = s.objective_and_gradient([x], mode=1)
+    finally:
+        s.close()
+    o.set_tissue_sinusoidal()
+    o.set_time(dt, nt)
+    o.set_k(0.02, 0.0, 0.0)
+    o.set_rho(6.0, 0.0, 0.0)
+    o.prec_factor()
+    o.set_phi(np.array([[np.pi, np.pi, np.pi]]), sigma, 1.0, False)
+    o.set_c0(c0.astype(np.float64))
+    o.set_opt(beta=0.0)
+    o.set_data(d1.astype(np.float64), None, -0.99)
+    Jo, go = o.objective_and_gradient(x, mode=1)
+    print("J", J[0], Jo, "g", g[0][-2:], go[-2:], flush=True)
+    assert abs(J[0] - Jo) < 1e-5 * abs(Jo)
+    assert np.abs(g[0][-2:] - go[-2:]).max() < 1e-4 * np.abs(go[-2:]).max()

@@ -125,9 +125,11 @@ def test_parse_config(tmp_path):
     assert c["sigma_data_driven"] == 2 * np.pi / 256 and c["support_data_path"] == c["gaussian_cm_path"] == ""
     # sigma_factor sets the data-driven sigma in units of 2 pi / n (src/utils/IO.cpp:175-177)
     p.write_text("n=128\nsigma_factor=2\nthreshold_data_driven=0.1\ngaussian_volume_fraction=0.25\n"
-                 "support_data_path=s.nc\ngaussian_cm_path=phi-mesh.dat\ndata_comp_data_path=concomp.dat\n")
+                 "support_data_path=s.nc\ngaussian_cm_path=phi-mesh.dat\ndata_comp_data_path=concomp.dat\n"
+                 "d0_path=c0.nc\nsmoothing_factor_data_t0=0.5\nobs_threshold_0=0.2\n")
     c = io.parse_config(str(p))
     assert c["sigma_data_driven"] == 2 * 2.0 * np.pi / 128
+    assert (c["d0_path"], c["smoothing_factor_data_t0"], c["obs_threshold_0"]) == ("c0.nc", 0.5, 0.2)
     assert (c["threshold_data_driven"], c["gaussian_volume_fraction"]) == (0.1, 0.25)
     assert (c["support_data_path"], c["gaussian_cm_path"], c["data_comp_data_path"]) == ("s.nc", "phi-mesh.dat",
                                                                                  "concomp.dat")
