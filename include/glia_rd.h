@@ -160,7 +160,7 @@ int glia_rd_smooth_noalloc(const int n[3], const float* in, float* out, double s
    The kept centers (x, y, z in [0, 2 pi)) are written in index order with *np their number;
    centers may be NULL with max_np = 0 to query the count.  The counting runs on the device.
    Returns GLIA_RD_ERR_ARG (with *np set) when more than max_np are kept, or when space < 1 (the
-   reference's loop would not advance).  The reference's defaults: sigma = 2 pi / 256
+   reference's loop would not advance) or the radius sigma / h0 reaches 64 cells.  The reference's defaults: sigma = 2 pi / 256
    (phi_sigma_data_driven), threshold 0.05 (data_threshold), gvf 0 (gaussian_vol_frac). */
 int glia_rd_data_support(glia_rd_t h, int member, int which, double threshold, double sigma, double gvf, int max_np,
                          double* centers, int* np);
