@@ -121,11 +121,15 @@ __global__ void __launch_bounds__(256) k_sl_trajectories(SLDims d, const float* 
   const float ux = vx[p], uy = vy[p], uz = vz[p];
   Stencil s;
   make_stencil(s, d, i, j, k, -sx * ux, -sy * uy, -sz * uz);
-  const float wx = apply_stencil(s, field_rsrc(vx, d)), wy = apply_stencil(s, field_rsrc(vy, d)),
-              wz = apply_stencil(s, field_rsrc(vz, d));
-  qx[p] = -0.5f * sx * (ux + wx);
-  qy[p] = -0.5f * sy * (uy + wy);
-  qz[p] = -0.5f * sz * (uz + wz);
+  // one component at a time (a loop the compiler keeps), so the 64 taps of one field are in
+  // registers at once: 3 waves per SIMD instead of 2
+  const float* vs[3] = {vx, vy, vz};
+  float w[3];
+#pragma unroll 1
+  for (int a = 0; a < 3; a++) w[a] = apply_stencil(s, field_rsrc(vs[a], d));
+  qx[p] = -0.5f * sx * (ux + w[0]);
+  qy[p] = -0.5f * sy * (uy + w[1]);
+  qz[p] = -0.5f * sz * (uz + w[2]);
 }
 
 // SemiLagrangianSolver::solve (src/pde/AdvectionSolver.cpp:592-660) for up to SL_MAX_FIELDS fields
