@@ -387,7 +387,14 @@ class Tumor {
       VecDestroy(&p_);
       if ((rc = VecDuplicate(p, &p_)) || (rc = VecCopy(p, p_))) return rc;
     }
-    phi_ = phi ? phi : std::make_shared<Phi>(params, spec_ops);
+    if (phi) {
+      phi_ = phi;
+      return phi_->setValues(mat_prop_);
+    }
+    // no basis given: the bounding box of np_ Gaussians around user_cm_ (Tumor.cpp:72-75)
+    phi_ = std::make_shared<Phi>(params, spec_ops);
+    std::array<ScalarType, 3> cm = tu.user_cm_;
+    if ((rc = phi_->setGaussians(cm, tu.phi_sigma_, tu.phi_spacing_factor_, tu.np_))) return rc;
     return phi_->setValues(mat_prop_);
   }
   /* setSinusoidalCoefficients (:102-112) */
